@@ -1,0 +1,140 @@
+// ORACLE — test infrastructure only (see oracle_interval.hpp).
+// extern "C" entry points that the Python tests load with ctypes.
+#include <cstring>
+
+#include "oracle_ccd.hpp"
+
+using namespace oracle;
+
+namespace {
+void unpack_ee(const double* q, EEQuery& e) {
+    std::memcpy(e.a0, q + 0, 3 * sizeof(double));
+    std::memcpy(e.a1, q + 3, 3 * sizeof(double));
+    std::memcpy(e.b0, q + 6, 3 * sizeof(double));
+    std::memcpy(e.b1, q + 9, 3 * sizeof(double));
+    std::memcpy(e.poseA_t0, q + 12, 6 * sizeof(double));
+    std::memcpy(e.poseA_t1, q + 18, 6 * sizeof(double));
+    std::memcpy(e.poseB_t0, q + 24, 6 * sizeof(double));
+    std::memcpy(e.poseB_t1, q + 30, 6 * sizeof(double));
+}
+void unpack_fv(const double* q, FVQuery& f) {
+    std::memcpy(f.v, q + 0, 3 * sizeof(double));
+    std::memcpy(f.f0, q + 3, 3 * sizeof(double));
+    std::memcpy(f.f1, q + 6, 3 * sizeof(double));
+    std::memcpy(f.f2, q + 9, 3 * sizeof(double));
+    std::memcpy(f.poseA_t0, q + 12, 6 * sizeof(double));
+    std::memcpy(f.poseA_t1, q + 18, 6 * sizeof(double));
+    std::memcpy(f.poseB_t0, q + 24, 6 * sizeof(double));
+    std::memcpy(f.poseB_t1, q + 30, 6 * sizeof(double));
+}
+}  // namespace
+
+extern "C" {
+
+// ---- interval primitives
+void oracle_interval_cos(double lo, double hi, double* out) {
+    Interval r = cos(Interval(lo, hi));
+    out[0] = r.lo; out[1] = r.hi;
+}
+void oracle_interval_sin(double lo, double hi, double* out) {
+    Interval r = sin(Interval(lo, hi));
+    out[0] = r.lo; out[1] = r.hi;
+}
+void oracle_interval_sinc(double lo, double hi, double* out) {
+    Interval r = sinc(Interval(lo, hi));
+    out[0] = r.lo; out[1] = r.hi;
+}
+double oracle_sinc(double x) { return sinc(x); }
+double oracle_cos_rn(double x) { return cos_rn(x); }
+double oracle_sin_rn(double x) { return sin_rn_full(x); }
+
+// op: 0 add, 1 sub, 2 mul, 3 div
+void oracle_interval_binop(int op, double xl, double xu, double yl, double yu, double* out) {
+    Interval x(xl, xu), y(yl, yu), r;
+    switch (op) {
+    case 0: r = x + y; break;
+    case 1: r = x - y; break;
+    case 2: r = x * y; break;
+    default: r = x / y; break;
+    }
+    out[0] = r.lo; out[1] = r.hi;
+}
+
+// Interval rotation matrix from an interval rotation vector r (3x2 doubles),
+// out: 3x3x2 doubles row-major.
+void oracle_rotation_interval(const double* r, double* out) {
+    Vec3I rv;
+    for (int i = 0; i < 3; i++) rv[i] = Interval(r[2 * i], r[2 * i + 1]);
+    Mat3I R = rotation_matrix(rv);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            out[2 * (3 * i + j)] = R.m[i][j].lo;
+            out[2 * (3 * i + j) + 1] = R.m[i][j].hi;
+        }
+}
+void oracle_rotation_double(const double* r, double* out) {
+    Mat3d R = rotation_matrix(r);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[3 * i + j] = R.m[i][j];
+}
+
+// Root of a*x^2 + b*x + c on [0,1] (tests/interval/test_interval_root_finder.cpp:9-33),
+// optional constraint x.lower() > lower_gt.
+int oracle_root_quadratic(double a, double b, double c, double tol, int use_constraint,
+                          double lower_gt, double* out) {
+    auto f = [&](const Interval& x) { return a * x * x + b * x + c; };
+    Interval sol;
+    bool found;
+    if (use_constraint)
+        found = interval_root_finder_1d(f, [&](const Interval& x) { return x.lo > lower_gt; },
+                                        Interval(0, 1), tol, sol);
+    else
+        found = interval_root_finder_1d(f, [](const Interval&) { return true; }, Interval(0, 1),
+                                        tol, sol);
+    out[0] = sol.lo; out[1] = sol.hi;
+    return found ? 1 : 0;
+}
+// Root of cos(2*pi*x) + yshift on [0,1] (test_interval_root_finder.cpp:35-52).
+int oracle_root_trig(double two_pi, double yshift, double tol, double* out) {
+    auto f = [&](const Interval& x) { return cos(two_pi * x) + yshift; };
+    Interval sol;
+    bool found = interval_root_finder_1d(f, [](const Interval&) { return true; }, Interval(0, 1),
+                                         tol, sol);
+    out[0] = sol.lo; out[1] = sol.hi;
+    return found ? 1 : 0;
+}
+
+// ---- rigid CCD per query. q layout: 4 body-local vertices (12 doubles),
+// then poseA_t0, poseA_t1, poseB_t0, poseB_t1 (4 x 6 doubles).
+// Returns 1 (impact), 0 (no impact) or -1 (evaluation budget exceeded).
+void oracle_set_eval_budget(long budget) { eval_budget() = budget; }
+int oracle_ee_toi(const double* q, double earliest_toi, double toi_tol, double* toi, long* n_evals) {
+    EEQuery e;
+    unpack_ee(q, e);
+    bool hit = edge_edge_toi(e, *toi, earliest_toi, toi_tol, n_evals);
+    if (budget_exceeded()) return -1;
+    return hit ? 1 : 0;
+}
+int oracle_fv_toi(const double* q, double earliest_toi, double toi_tol, double* toi, long* n_evals) {
+    FVQuery f;
+    unpack_fv(q, f);
+    bool hit = face_vertex_toi(f, *toi, earliest_toi, toi_tol, n_evals);
+    if (budget_exceeded()) return -1;
+    return hit ? 1 : 0;
+}
+void oracle_ee_aabb(const double* q, const double* box, double* out) {
+    EEQuery e;
+    unpack_ee(q, e);
+    Vec3I d = edge_edge_aabb(e, Interval(box[0], box[1]), Interval(box[2], box[3]),
+                             Interval(box[4], box[5]));
+    for (int i = 0; i < 3; i++) { out[2 * i] = d[i].lo; out[2 * i + 1] = d[i].hi; }
+}
+void oracle_fv_aabb(const double* q, const double* box, double* out) {
+    FVQuery f;
+    unpack_fv(q, f);
+    Vec3I d = face_vertex_aabb(f, Interval(box[0], box[1]), Interval(box[2], box[3]),
+                               Interval(box[4], box[5]));
+    for (int i = 0; i < 3; i++) { out[2 * i] = d[i].lo; out[2 * i + 1] = d[i].hi; }
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+"""ctypes loader for the CPU oracle (oracle/_build/liboracle.so).
+
+The oracle is test infrastructure: it is only ever used as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+_lib = None
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_long_p = ctypes.POINTER(ctypes.c_long)
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+
+def dptr(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+def i32ptr(a):
+    return a.ctypes.data_as(c_int32_p)
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(c_int64_p)
+
+
+def lib():
+    """Load the oracle, building it first if it is not there."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.check_call(["make", "-C", ORACLE_DIR, "-j8"])
+        _lib = ctypes.CDLL(ORACLE_SO)
+        L = _lib
+        for name in ("oracle_interval_cos", "oracle_interval_sin", "oracle_interval_sinc"):
+            getattr(L, name).argtypes = [ctypes.c_double, ctypes.c_double, c_double_p]
+        for name in ("oracle_sinc", "oracle_cos_rn", "oracle_sin_rn"):
+            getattr(L, name).argtypes = [ctypes.c_double]
+            getattr(L, name).restype = ctypes.c_double
+        L.oracle_interval_binop.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4 + [c_double_p]
+        L.oracle_rotation_interval.argtypes = [c_double_p, c_double_p]
+        L.oracle_rotation_double.argtypes = [c_double_p, c_double_p]
+        L.oracle_root_quadratic.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_double, c_double_p]
+        L.oracle_root_quadratic.restype = ctypes.c_int
+        L.oracle_root_trig.argtypes = [ctypes.c_double] * 3 + [c_double_p]
+        L.oracle_root_trig.restype = ctypes.c_int
+        for name in ("oracle_ee_toi", "oracle_fv_toi"):
+            getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p]
+            getattr(L, name).restype = ctypes.c_int
+        L.oracle_set_eval_budget.argtypes = [ctypes.c_long]
+        for name in ("oracle_ee_aabb", "oracle_fv_aabb"):
+            getattr(L, name).argtypes = [c_double_p, c_double_p, c_double_p]
+    return _lib
+
+
+def interval_fn(name, lo, hi):
+    out = np.zeros(2)
+    getattr(lib(), name)(lo, hi, dptr(out))
+    return out
+
+
+def binop(op, x, y):
+    out = np.zeros(2)
+    lib().oracle_interval_binop({"+": 0, "-": 1, "*": 2, "/": 3}[op], x[0], x[1], y[0], y[1], dptr(out))
+    return out
+
+
+def ee_toi(q, earliest_toi=1.0, toi_tol=1e-4):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    toi = ctypes.c_double()
+    n = ctypes.c_long(0)
+    hit = lib().oracle_ee_toi(dptr(q), earliest_toi, toi_tol, ctypes.byref(toi), ctypes.byref(n))
+    return (hit if hit < 0 else bool(hit)), toi.value, n.value
+
+
+def fv_toi(q, earliest_toi=1.0, toi_tol=1e-4):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    toi = ctypes.c_double()
+    n = ctypes.c_long(0)
+    hit = lib().oracle_fv_toi(dptr(q), earliest_toi, toi_tol, ctypes.byref(toi), ctypes.byref(n))
+    return (hit if hit < 0 else bool(hit)), toi.value, n.value
+
+
+def set_eval_budget(n):
+    """Cap inclusion-function evaluations per query (0 = unbounded)."""
+    lib().oracle_set_eval_budget(int(n))
