@@ -1,0 +1,99 @@
+/*
+ * rigc — MI355X (gfx950) contact hot path for rigid-body IPC.
+ *
+ * A C ABI over device kernels for the broad phase, the rigid-trajectory
+ * interval CCD and the barrier / friction assembly of Rigid IPC. Every
+ * entry point takes plain host pointers and sizes, returns a rigc_status
+ * and never throws. One context owns one device, one HIP stream and the
+ * device copies of one scene; calls on a context are serialised by the
+ * caller (the reference calls these from its single Newton thread).
+ *
+ * Layout conventions (those of the reference, src/physics/pose.tpp:32-43,
+ * src/physics/rigid_body_assembler.cpp:19-60):
+ *   poses      [num_bodies][6] doubles: position x,y,z then rotation vector
+ *   vertices   body-local (principal-axis frame) coordinates, concatenated
+ *              body after body; edge / face ids are global (concatenated)
+ *   candidates edge-edge pairs (e0 < e1); face-vertex pairs (face, vertex)
+ */
+#ifndef RIGC_H
+#define RIGC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rigc_status;
+#define RIGC_OK 0
+#define RIGC_ERR_INVALID_ARGUMENT 1
+#define RIGC_ERR_DEVICE 2
+#define RIGC_ERR_NO_SCENE 3
+#define RIGC_ERR_CAPACITY 4   /* output too small; the count written is the size needed */
+#define RIGC_ERR_UNRESOLVED 5 /* a CCD query exceeded its evaluation budget */
+
+typedef struct rigc_ctx rigc_ctx;
+
+/* Scene geometry: replaces the RigidBodyAssembler state the reference's
+ * broad phase, CCD and barrier problem read
+ * (src/physics/rigid_body_assembler.hpp, src/physics/rigid_body.hpp). */
+typedef struct rigc_scene {
+    int32_t num_bodies;
+    int32_t num_vertices;
+    int32_t num_edges;
+    int32_t num_faces;
+    const double* vertices;            /* [num_vertices][3] body-local */
+    const int32_t* edges;              /* [num_edges][2] global vertex ids */
+    const int32_t* faces;              /* [num_faces][3] global vertex ids */
+    const int32_t* face_edges;         /* [num_faces][3] global edge of (f0,f1),(f1,f2),(f2,f0) */
+    const int32_t* body_vertex_start;  /* [num_bodies+1] */
+    const int32_t* body_edge_start;    /* [num_bodies+1] */
+    const int32_t* body_face_start;    /* [num_bodies+1] */
+    const int32_t* group_ids;          /* [num_bodies] */
+    const uint8_t* is_static;          /* [num_bodies] RigidBodyType::STATIC */
+} rigc_scene;
+
+/* Device-side timings and counters of the most recent call. */
+typedef struct rigc_stats {
+    double broad_phase_ms;     /* all broad-phase kernels of the last call */
+    double narrow_phase_ms;    /* the CCD kernel of the last call */
+    double assembly_ms;        /* the assembly kernels of the last call */
+    int64_t box_evaluations;   /* inclusion-function evaluations (CCD) */
+    int64_t num_candidates;    /* candidates passed to the narrow phase */
+    int64_t num_items;         /* (element, cell) items sorted by the broad phase */
+} rigc_stats;
+
+/* Create a context on HIP device `device` (ordinal). */
+rigc_status rigc_create(rigc_ctx** ctx, int32_t device);
+rigc_status rigc_destroy(rigc_ctx* ctx);
+/* Message of the last failing call on this context ("" if none). */
+const char* rigc_last_error(const rigc_ctx* ctx);
+rigc_status rigc_get_stats(const rigc_ctx* ctx, rigc_stats* stats);
+
+/* Upload a scene. Replaces RigidBodyAssembler::init
+ * (src/physics/rigid_body_assembler.cpp:19-118): derives per-edge rest
+ * lengths, per-body r_max and the mean edge length on the host. */
+rigc_status rigc_set_scene(rigc_ctx* ctx, const rigc_scene* scene);
+
+/* Narrow phase over given candidates. Replaces
+ * DistanceBarrierConstraint::compute_earliest_toi_narrow_phase
+ * (src/opt/distance_barrier_constraint.cpp:151-265) with edge_edge_ccd /
+ * face_vertex_ccd (src/ccd/ccd.cpp:221-357) for TrajectoryType::RIGID,
+ * i.e. compute_{edge_edge,face_vertex}_time_of_impact
+ * (src/ccd/rigid/time_of_impact.cpp:158-310). Every query searches
+ * t in [0,1]. earliest_toi receives the minimum TOI (+inf if none).
+ * toi_out / hit_out (optional, length n_ee + n_fv, EE first) receive
+ * per-query results: hit 1 = impact, 0 = none, -1 = budget exceeded.
+ * eval_budget > 0 caps box visits per query (0 = unbounded, as the
+ * reference). per_query != 0 disables cross-query pruning so that every
+ * query's own TOI is computed. */
+rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                              const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                              double toi_tolerance, int64_t eval_budget, int32_t per_query,
+                              double* earliest_toi, double* toi_out, int8_t* hit_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIGC_H */

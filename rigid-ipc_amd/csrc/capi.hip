@@ -1,0 +1,304 @@
+// C ABI of librigc: context, scene upload and the entry points declared in
+// include/rigc.h. Host orchestration only; the kernels live in the other
+// translation units.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rigc_internal.hpp"
+
+using namespace rigc;
+
+namespace rigc {
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = std::max<size_t>(n, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace rigc
+
+struct rigc_ctx {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    rigc_stats stats{};
+
+    bool has_scene = false;
+    DevScene scene;
+    std::vector<DevBuf> scene_bufs;
+    double average_edge_length = 0;
+
+    // per-call buffers
+    DevBuf poses0, poses1, ee, fv, toi, hit, counters;
+
+    rigc_status fail(rigc_status s, const std::string& msg) {
+        err = msg;
+        return s;
+    }
+    rigc_status hip_fail(hipError_t e, const char* where) {
+        err = std::string(where) + ": " + hipGetErrorString(e);
+        return RIGC_ERR_DEVICE;
+    }
+};
+
+#define RIGC_HIP(call)                                   \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return ctx->hip_fail(e_, #call); \
+    } while (0)
+
+namespace {
+
+template <typename T>
+hipError_t upload(rigc_ctx* ctx, T** dst, const T* src, size_t n) {
+    ctx->scene_bufs.emplace_back();
+    DevBuf& b = ctx->scene_bufs.back();
+    hipError_t e = b.ensure(std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    *dst = b.as<T>();
+    if (n) e = hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+    return e;
+}
+
+void free_scene(rigc_ctx* ctx) {
+    for (auto& b : ctx->scene_bufs) b.release();
+    ctx->scene_bufs.clear();
+    ctx->scene = DevScene{};
+    ctx->has_scene = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+rigc_status rigc_create(rigc_ctx** out, int32_t device) {
+    if (!out) return RIGC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return RIGC_ERR_DEVICE;
+    rigc_ctx* ctx = new rigc_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return RIGC_ERR_DEVICE;
+    }
+    ctx->num_cus = prop.multiProcessorCount;
+    *out = ctx;
+    return RIGC_OK;
+}
+
+rigc_status rigc_destroy(rigc_ctx* ctx) {
+    if (!ctx) return RIGC_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_scene(ctx);
+    for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->ee, &ctx->fv, &ctx->toi, &ctx->hit, &ctx->counters })
+        b->release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RIGC_OK;
+}
+
+const char* rigc_last_error(const rigc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+rigc_status rigc_get_stats(const rigc_ctx* ctx, rigc_stats* stats) {
+    if (!ctx || !stats) return RIGC_ERR_INVALID_ARGUMENT;
+    *stats = ctx->stats;
+    return RIGC_OK;
+}
+
+rigc_status rigc_set_scene(rigc_ctx* ctx, const rigc_scene* s) {
+    if (!ctx || !s) return RIGC_ERR_INVALID_ARGUMENT;
+    if (s->num_bodies < 1 || s->num_vertices < 0 || s->num_edges < 0 || s->num_faces < 0)
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: negative sizes");
+    if (!s->vertices || !s->body_vertex_start || !s->body_edge_start || !s->body_face_start || !s->group_ids
+        || !s->is_static || (s->num_edges && !s->edges) || (s->num_faces && (!s->faces || !s->face_edges)))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: missing arrays");
+    const int nb = s->num_bodies, nv = s->num_vertices, ne = s->num_edges, nf = s->num_faces;
+    if (s->body_vertex_start[0] != 0 || s->body_vertex_start[nb] != nv || s->body_edge_start[0] != 0
+        || s->body_edge_start[nb] != ne || s->body_face_start[0] != 0 || s->body_face_start[nb] != nf)
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: body ranges do not cover the arrays");
+    for (int i = 0; i < 2 * ne; i++)
+        if (s->edges[i] < 0 || s->edges[i] >= nv)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: edge vertex out of range");
+    for (int i = 0; i < 3 * nf; i++)
+        if (s->faces[i] < 0 || s->faces[i] >= nv || s->face_edges[i] < 0 || s->face_edges[i] >= ne)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: face index out of range");
+
+    // Host-side derived data, following RigidBody's constructor
+    // (src/physics/rigid_body.cpp:124-142) and RigidBody::edge_length
+    // (src/physics/rigid_body.hpp:151-156).
+    std::vector<int> v_body(nv), e_body(ne), f_body(nf);
+    std::vector<double> r_max(nb, 0.0), elen(ne);
+    double avg_sum = 0;
+    for (int b = 0; b < nb; b++) {
+        for (int i = s->body_vertex_start[b]; i < s->body_vertex_start[b + 1]; i++) {
+            v_body[i] = b;
+            const double* v = s->vertices + 3 * i;
+            r_max[b] = std::max(r_max[b], std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]));
+        }
+        double body_avg = 0;
+        const int e0 = s->body_edge_start[b], e1 = s->body_edge_start[b + 1];
+        for (int i = e0; i < e1; i++) {
+            e_body[i] = b;
+            const double* a = s->vertices + 3 * s->edges[2 * i];
+            const double* c = s->vertices + 3 * s->edges[2 * i + 1];
+            const double d0 = c[0] - a[0], d1 = c[1] - a[1], d2 = c[2] - a[2];
+            elen[i] = std::sqrt((d0 * d0 + d1 * d1) + d2 * d2);
+            body_avg += elen[i];
+        }
+        if (e1 > e0) body_avg /= (e1 - e0);
+        avg_sum += (e1 - e0) * body_avg;
+        for (int i = s->body_face_start[b]; i < s->body_face_start[b + 1]; i++) f_body[i] = b;
+    }
+    ctx->average_edge_length = ne ? avg_sum / ne : 0.0;
+
+    RIGC_HIP(hipSetDevice(ctx->device));
+    free_scene(ctx);
+    DevScene& d = ctx->scene;
+    d.nb = nb; d.nv = nv; d.ne = ne; d.nf = nf;
+    RIGC_HIP(upload(ctx, &d.V, s->vertices, 3 * (size_t)nv));
+    RIGC_HIP(upload(ctx, &d.E, s->edges, 2 * (size_t)ne));
+    RIGC_HIP(upload(ctx, &d.F, s->faces, 3 * (size_t)nf));
+    RIGC_HIP(upload(ctx, &d.FE, s->face_edges, 3 * (size_t)nf));
+    RIGC_HIP(upload(ctx, &d.v_body, v_body.data(), (size_t)nv));
+    RIGC_HIP(upload(ctx, &d.e_body, e_body.data(), (size_t)ne));
+    RIGC_HIP(upload(ctx, &d.f_body, f_body.data(), (size_t)nf));
+    RIGC_HIP(upload(ctx, &d.body_vstart, s->body_vertex_start, (size_t)nb + 1));
+    RIGC_HIP(upload(ctx, &d.group, s->group_ids, (size_t)nb));
+    RIGC_HIP(upload(ctx, &d.is_static, s->is_static, (size_t)nb));
+    RIGC_HIP(upload(ctx, &d.edge_len, elen.data(), (size_t)ne));
+    RIGC_HIP(upload(ctx, &d.r_max, r_max.data(), (size_t)nb));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->has_scene = true;
+    return RIGC_OK;
+}
+
+rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                              const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                              double toi_tolerance, int64_t eval_budget, int32_t per_query,
+                              double* earliest_toi, double* toi_out, int8_t* hit_out) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_narrow_phase: no scene");
+    if (!poses_t0 || !poses_t1 || !earliest_toi || n_ee < 0 || n_fv < 0 || (n_ee && !ee) || (n_fv && !fv)
+        || !(toi_tolerance > 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase: bad arguments");
+    const DevScene& d = ctx->scene;
+    for (int64_t i = 0; i < n_ee; i++)
+        if (ee[2 * i] < 0 || ee[2 * i] >= d.ne || ee[2 * i + 1] < 0 || ee[2 * i + 1] >= d.ne)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase: edge id out of range");
+    for (int64_t i = 0; i < n_fv; i++)
+        if (fv[2 * i] < 0 || fv[2 * i] >= d.nf || fv[2 * i + 1] < 0 || fv[2 * i + 1] >= d.nv)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase: face/vertex id out of range");
+
+    RIGC_HIP(hipSetDevice(ctx->device));
+    const size_t nq = (size_t)(n_ee + n_fv);
+    const size_t pose_bytes = 6 * sizeof(double) * (size_t)d.nb;
+    RIGC_HIP(ctx->poses0.ensure(pose_bytes));
+    RIGC_HIP(ctx->poses1.ensure(pose_bytes));
+    RIGC_HIP(ctx->ee.ensure(2 * sizeof(int32_t) * std::max<size_t>(n_ee, 1)));
+    RIGC_HIP(ctx->fv.ensure(2 * sizeof(int32_t) * std::max<size_t>(n_fv, 1)));
+    RIGC_HIP(ctx->toi.ensure(sizeof(double) * std::max<size_t>(nq, 1)));
+    RIGC_HIP(ctx->hit.ensure(std::max<size_t>(nq, 1)));
+    RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(hipMemcpyAsync(ctx->poses0.p, poses_t0, pose_bytes, hipMemcpyHostToDevice, st));
+    RIGC_HIP(hipMemcpyAsync(ctx->poses1.p, poses_t1, pose_bytes, hipMemcpyHostToDevice, st));
+    if (n_ee) RIGC_HIP(hipMemcpyAsync(ctx->ee.p, ee, 2 * sizeof(int32_t) * n_ee, hipMemcpyHostToDevice, st));
+    if (n_fv) RIGC_HIP(hipMemcpyAsync(ctx->fv.p, fv, 2 * sizeof(int32_t) * n_fv, hipMemcpyHostToDevice, st));
+    unsigned long long init[4] = { 0x7ff0000000000000ULL, 0, 0, 0 };
+    RIGC_HIP(hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+
+    CcdArgs a;
+    a.V = d.V; a.E = d.E; a.F = d.F; a.FE = d.FE;
+    a.v_body = d.v_body; a.e_body = d.e_body; a.f_body = d.f_body; a.edge_len = d.edge_len;
+    a.poses_t0 = ctx->poses0.as<double>();
+    a.poses_t1 = ctx->poses1.as<double>();
+    a.ee = ctx->ee.as<int>();
+    a.fv = ctx->fv.as<int>();
+    a.n_ee = n_ee;
+    a.n_fv = n_fv;
+    a.toi_tol = toi_tolerance;
+    a.eval_budget = eval_budget;
+    a.use_global_best = per_query ? 0 : 1;
+    a.global_best = ctx->counters.as<unsigned long long>();
+    a.counter = ctx->counters.as<unsigned long long>() + 1;
+    a.total_evals = ctx->counters.as<unsigned long long>() + 2;
+    a.toi_out = ctx->toi.as<double>();
+    a.hit_out = ctx->hit.as<signed char>();
+
+    const int block = 256;
+    const size_t blocks_needed = (nq + block - 1) / block;
+    const int grid = (int)std::max<size_t>(1, std::min<size_t>(blocks_needed, (size_t)ctx->num_cus * 8));
+    RIGC_HIP(hipEventRecord(ctx->ev0, st));
+    if (nq) launch_ccd(a, grid, block, st);
+    RIGC_HIP(hipGetLastError());
+    RIGC_HIP(hipEventRecord(ctx->ev1, st));
+
+    std::vector<double> tois(nq);
+    std::vector<signed char> hits(nq);
+    if (nq) {
+        RIGC_HIP(hipMemcpyAsync(tois.data(), ctx->toi.p, sizeof(double) * nq, hipMemcpyDeviceToHost, st));
+        RIGC_HIP(hipMemcpyAsync(hits.data(), ctx->hit.p, nq, hipMemcpyDeviceToHost, st));
+    }
+    unsigned long long res[4];
+    RIGC_HIP(hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, st));
+    RIGC_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.narrow_phase_ms = ms;
+    ctx->stats.box_evaluations = (int64_t)res[2];
+    ctx->stats.num_candidates = (int64_t)nq;
+
+    bool unresolved = false;
+    double best = INFINITY;
+    for (size_t i = 0; i < nq; i++) {
+        if (hits[i] < 0) unresolved = true;
+        if (hits[i] == 1) best = std::min(best, tois[i]);
+    }
+    if (per_query) {
+        *earliest_toi = best;
+    } else {
+        double g;
+        std::memcpy(&g, &res[0], sizeof(double));
+        *earliest_toi = g;
+    }
+    if (toi_out) std::memcpy(toi_out, tois.data(), sizeof(double) * nq);
+    if (hit_out) std::memcpy(hit_out, hits.data(), nq);
+    if (unresolved)
+        return ctx->fail(RIGC_ERR_UNRESOLVED, "rigc_narrow_phase: a query exceeded its evaluation budget");
+    return RIGC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Internal declarations shared by the HIP translation units of librigc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rigc.h"
+
+namespace rigc {
+
+// Device-resident scene (body-local geometry, global element ids).
+struct DevScene {
+    int nb = 0, nv = 0, ne = 0, nf = 0;
+    double* V = nullptr;        // [nv][3] body-local vertices
+    int* E = nullptr;           // [ne][2]
+    int* F = nullptr;           // [nf][3]
+    int* FE = nullptr;          // [nf][3]
+    int* v_body = nullptr;      // [nv]
+    int* e_body = nullptr;      // [ne]
+    int* f_body = nullptr;      // [nf]
+    int* body_vstart = nullptr; // [nb+1]
+    int* group = nullptr;       // [nb]
+    uint8_t* is_static = nullptr;
+    double* edge_len = nullptr; // [ne] body-local edge length
+    double* r_max = nullptr;    // [nb]
+};
+
+// Narrow-phase candidate: kind 0 = edge-edge (a = edge0, b = edge1),
+// kind 1 = face-vertex (a = face, b = vertex).
+struct CcdArgs {
+    const double* V;
+    const int* E;
+    const int* F;
+    const int* FE;
+    const int* v_body;
+    const int* e_body;
+    const int* f_body;
+    const double* edge_len;
+    const double* poses_t0;  // [nb][6]
+    const double* poses_t1;
+    const int* ee;           // [n_ee][2]
+    const int* fv;           // [n_fv][2]
+    long long n_ee, n_fv;
+    double toi_tol;
+    long long eval_budget;   // 0 = unbounded
+    int use_global_best;     // prune with the running minimum over all queries
+    unsigned long long* global_best;  // bits of a non-negative double
+    unsigned long long* counter;      // work counter (query index)
+    unsigned long long* total_evals;
+    double* toi_out;         // per query (optional)
+    signed char* hit_out;    // per query (optional): 1 hit, 0 none, -1 budget, -2 depth
+};
+
+void launch_ccd(const CcdArgs& a, int grid, int block, hipStream_t stream);
+
+}  // namespace rigc

@@ -1,0 +1,159 @@
+"""Python bindings for librigc, the MI355X contact hot path of Rigid IPC.
+
+The module loads ``rigid-ipc_amd/librigc.so`` (built by
+``rigid-ipc_amd/build.py``) through its C ABI (``include/rigc.h``). There is
+no CPU fallback: if the library or a GPU is missing, construction raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "librigc.so")
+
+RIGC_OK = 0
+RIGC_ERR_INVALID_ARGUMENT = 1
+RIGC_ERR_DEVICE = 2
+RIGC_ERR_NO_SCENE = 3
+RIGC_ERR_CAPACITY = 4
+RIGC_ERR_UNRESOLVED = 5
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int32_p = ctypes.POINTER(ctypes.c_int32)
+_c_int8_p = ctypes.POINTER(ctypes.c_int8)
+_c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+
+class RigcError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"rigc status {status}: {message}")
+        self.status = status
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [
+        ("num_bodies", ctypes.c_int32), ("num_vertices", ctypes.c_int32),
+        ("num_edges", ctypes.c_int32), ("num_faces", ctypes.c_int32),
+        ("vertices", _c_double_p), ("edges", _c_int32_p), ("faces", _c_int32_p),
+        ("face_edges", _c_int32_p), ("body_vertex_start", _c_int32_p),
+        ("body_edge_start", _c_int32_p), ("body_face_start", _c_int32_p),
+        ("group_ids", _c_int32_p), ("is_static", _c_uint8_p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("broad_phase_ms", ctypes.c_double), ("narrow_phase_ms", ctypes.c_double),
+        ("assembly_ms", ctypes.c_double), ("box_evaluations", ctypes.c_int64),
+        ("num_candidates", ctypes.c_int64), ("num_items", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load librigc.so and declare its entry points (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run rigid-ipc_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.rigc_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32]
+    L.rigc_destroy.argtypes = [vp]
+    L.rigc_last_error.argtypes = [vp]
+    L.rigc_last_error.restype = ctypes.c_char_p
+    L.rigc_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.rigc_set_scene.argtypes = [vp, ctypes.POINTER(Scene)]
+    L.rigc_narrow_phase.argtypes = [vp, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_int64,
+                                    _c_int32_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
+                                    ctypes.c_int32, _c_double_p, _c_double_p, _c_int8_p]
+    for name in ("rigc_create", "rigc_destroy", "rigc_get_stats", "rigc_set_scene", "rigc_narrow_phase"):
+        getattr(L, name).restype = ctypes.c_int32
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Context:
+    """One device context (rigc_ctx) bound to one scene at a time."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self._ctx = ctypes.c_void_p()
+        st = self._L.rigc_create(ctypes.byref(self._ctx), int(device))
+        if st != RIGC_OK:
+            raise RigcError(st, "rigc_create failed (no usable HIP device?)")
+        self._keep = []
+
+    def close(self):
+        if self._ctx:
+            self._L.rigc_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != RIGC_OK:
+            raise RigcError(st, self._L.rigc_last_error(self._ctx).decode())
+
+    def stats(self):
+        s = Stats()
+        self._check(self._L.rigc_get_stats(self._ctx, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def set_scene(self, scene):
+        """scene: an object with numpy fields vertices (nv,3) f64, edges (ne,2),
+        faces (nf,3), face_edges (nf,3), body_vertex_start, body_edge_start,
+        body_face_start (nb+1), group_ids (nb), is_static (nb)."""
+        arrs = dict(
+            vertices=np.ascontiguousarray(scene.vertices, np.float64),
+            edges=np.ascontiguousarray(scene.edges, np.int32),
+            faces=np.ascontiguousarray(scene.faces, np.int32),
+            face_edges=np.ascontiguousarray(scene.face_edges, np.int32),
+            body_vertex_start=np.ascontiguousarray(scene.body_vertex_start, np.int32),
+            body_edge_start=np.ascontiguousarray(scene.body_edge_start, np.int32),
+            body_face_start=np.ascontiguousarray(scene.body_face_start, np.int32),
+            group_ids=np.ascontiguousarray(scene.group_ids, np.int32),
+            is_static=np.ascontiguousarray(scene.is_static, np.uint8),
+        )
+        s = Scene()
+        s.num_bodies = len(arrs["group_ids"])
+        s.num_vertices = len(arrs["vertices"])
+        s.num_edges = len(arrs["edges"])
+        s.num_faces = len(arrs["faces"])
+        for k, a in arrs.items():
+            t = {np.float64: _c_double_p, np.int32: _c_int32_p, np.uint8: _c_uint8_p}[a.dtype.type]
+            setattr(s, k, _p(a, t))
+        self._check(self._L.rigc_set_scene(self._ctx, ctypes.byref(s)))
+        self.num_bodies = s.num_bodies
+
+    def narrow_phase(self, poses_t0, poses_t1, ee, fv, toi_tolerance=1e-4, eval_budget=0,
+                     per_query=False, allow_unresolved=False):
+        """Rigid CCD over candidates; returns (earliest_toi, toi[n], hit[n])."""
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        p1 = np.ascontiguousarray(poses_t1, np.float64).reshape(-1, 6)
+        ee = np.ascontiguousarray(np.asarray(ee, np.int32).reshape(-1, 2))
+        fv = np.ascontiguousarray(np.asarray(fv, np.int32).reshape(-1, 2))
+        n = len(ee) + len(fv)
+        toi = np.empty(max(n, 1))
+        hit = np.empty(max(n, 1), np.int8)
+        best = ctypes.c_double()
+        st = self._L.rigc_narrow_phase(self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p),
+                                       _p(ee, _c_int32_p), len(ee), _p(fv, _c_int32_p), len(fv),
+                                       float(toi_tolerance), int(eval_budget), int(bool(per_query)),
+                                       ctypes.byref(best), _p(toi, _c_double_p), _p(hit, _c_int8_p))
+        if not (allow_unresolved and st == RIGC_ERR_UNRESOLVED):
+            self._check(st)
+        return best.value, toi[:n], hit[:n]
