@@ -92,6 +92,27 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
                               double toi_tolerance, int64_t eval_budget, int32_t per_query,
                               double* earliest_toi, double* toi_out, int8_t* hit_out);
 
+/* Broad phase. Replaces detect_collision_candidates_rigid with
+ * DetectionMethod::HASH_GRID, continuous (poses_t1 != NULL;
+ * src/ccd/rigid/broad_phase.cpp:127-203, RigidBodyHashGrid::{resize,addBodies}
+ * src/ccd/rigid/rigid_body_hash_grid.cpp:90-344) or discrete at poses_t0
+ * (poses_t1 == NULL; broad_phase.cpp:19-90, :36-62, :122-162).
+ * Writes sorted, duplicate-free edge-edge (e0 < e1) and face-vertex
+ * (face, vertex) pairs. If a capacity is too small, nothing is copied,
+ * *n_ee / *n_fv hold the sizes needed and RIGC_ERR_CAPACITY is returned. */
+rigc_status rigc_broad_phase(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                             double inflation_radius, int32_t* ee, int64_t ee_capacity, int64_t* n_ee,
+                             int32_t* fv, int64_t fv_capacity, int64_t* n_fv);
+
+/* Earliest time of impact of the rigid trajectories poses_t0 -> poses_t1:
+ * continuous broad phase with inflation minimum_separation_distance / 2,
+ * then the narrow phase with cross-query pruning, all on the device.
+ * Replaces DistanceBarrierConstraint::compute_earliest_toi
+ * (src/opt/distance_barrier_constraint.cpp:130-149) for RIGID trajectories.
+ * *earliest_toi = +inf when nothing collides. */
+rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                      double minimum_separation_distance, double* earliest_toi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "oracle_ccd.hpp"
+#include "oracle_scene.hpp"
 
 using namespace oracle;
 
@@ -135,6 +136,55 @@ void oracle_fv_aabb(const double* q, const double* box, double* out) {
     Vec3I d = face_vertex_aabb(f, Interval(box[0], box[1]), Interval(box[2], box[3]),
                                Interval(box[4], box[5]));
     for (int i = 0; i < 3; i++) { out[2 * i] = d[i].lo; out[2 * i + 1] = d[i].hi; }
+}
+
+// ---- scene + broad phase
+void oracle_set_scene(int nb, int nv, int ne, int nf, const double* V, const int* E, const int* F,
+                      const int* FE, const int* bv, const int* be, const int* bf, const int* group,
+                      const unsigned char* is_static) {
+    Scene& s = scene();
+    s.nb = nb; s.nv = nv; s.ne = ne; s.nf = nf;
+    s.V.assign(V, V + 3 * (size_t)nv);
+    s.E.assign(E, E + 2 * (size_t)ne);
+    s.F.assign(F, F + 3 * (size_t)nf);
+    s.FE.assign(FE, FE + 3 * (size_t)nf);
+    s.bv.assign(bv, bv + nb + 1);
+    s.be.assign(be, be + nb + 1);
+    s.bf.assign(bf, bf + nb + 1);
+    s.group.assign(group, group + nb);
+    s.is_static.assign(is_static, is_static + nb);
+    scene_derive(s);
+}
+
+static Candidates g_candidates;
+
+// continuous != 0: detect_collision_candidates_rigid(poses_t0, poses_t1);
+// otherwise the discrete version at poses_t0.
+void oracle_broadphase(int continuous, const double* p0, const double* p1, double r, long* n_ee, long* n_fv) {
+    if (continuous) broadphase_ccd(scene(), p0, p1, r, g_candidates);
+    else broadphase_static(scene(), p0, r, g_candidates);
+    *n_ee = (long)g_candidates.ee.size();
+    *n_fv = (long)g_candidates.fv.size();
+}
+void oracle_get_candidates(int* ee, int* fv) {
+    for (size_t i = 0; i < g_candidates.ee.size(); i++) { ee[2 * i] = g_candidates.ee[i][0]; ee[2 * i + 1] = g_candidates.ee[i][1]; }
+    for (size_t i = 0; i < g_candidates.fv.size(); i++) { fv[2 * i] = g_candidates.fv[i][0]; fv[2 * i + 1] = g_candidates.fv[i][1]; }
+}
+// [nv][7]: included flag, AABB min xyz, AABB max xyz of the last broad phase.
+void oracle_last_vertex_aabbs(double* out) {
+    const std::vector<double>& v = last_vertex_aabbs();
+    std::copy(v.begin(), v.end(), out);
+}
+double oracle_average_edge_length() { return scene().average_edge_length; }
+
+// Double-precision world vertices of every body at the given poses.
+void oracle_world_vertices(const double* poses, double* out) {
+    const Scene& s = scene();
+    std::vector<double> w;
+    for (int b = 0; b < s.nb; b++) {
+        world_vertices(s, b, poses + 6 * b, w);
+        std::copy(w.begin(), w.end(), out + 3 * (size_t)s.bv[b]);
+    }
 }
 
 }  // extern "C"

@@ -1,0 +1,624 @@
+// Rigid broad phase for gfx950: the candidate set of Rigid IPC's hash-grid
+// detection (src/ccd/rigid/broad_phase.cpp:19-203,
+// src/ccd/rigid/rigid_body_hash_grid.cpp:17-344) computed as
+//
+//   1. body boxes (RigidBody::compute_bounding_box, rigid_body.cpp:246-272),
+//      one workgroup per body;
+//   2. included bodies: those in any overlapping body-box pair of different
+//      groups (close_bodies + body_pairs_to_body_ids);
+//   3. the conservative domain over included bodies (resize, :90-119);
+//   4. per-vertex AABBs: for continuous detection the interval enclosure of
+//      every vertex trajectory, with the forced split of t at 1/2 and the
+//      adaptive bisection of any t-piece whose enclosure leaves the domain
+//      (compute_vertices_intervals, :164-246), one workgroup per body with
+//      an LDS stack of t-pieces; for discrete detection the world vertex
+//      inflated by r;
+//   5. element AABBs (vertex / edge / face) in the centre / half-extent form
+//      used by the overlap test;
+//   6. body pairs whose element hulls overlap (a conservative prefilter);
+//   7. one workgroup per body pair: each body's elements that reach the
+//      other body's hull are staged in LDS, every edge-edge and face-vertex
+//      pair among them is tested, and hits are appended with one atomic per
+//      wavefront;
+//   8. a stable radix sort of the pair keys, giving the sorted, duplicate-
+//      free candidate lists of the reference.
+// Every element pair of two included bodies with overlapping AABBs and
+// different groups is found exactly once (each pair belongs to exactly one
+// body pair), so the result equals the reference's hash-grid set, which is
+// independent of its cell size.
+#include <algorithm>
+#include <vector>
+
+#include "rigc_geom.hpp"
+#include "rigc_internal.hpp"
+#include "rigc_rigid_iv.hpp"
+
+namespace rigc {
+namespace {
+
+constexpr int BP_THREADS = 256;
+constexpr int LIST_CAP = 1024;
+
+__device__ __forceinline__ void block_minmax6(double* v, double (*red)[BP_THREADS]) {
+    // v[0..2] are minima, v[3..5] maxima
+    const int tid = threadIdx.x;
+    for (int k = 0; k < 6; k++) red[k][tid] = v[k];
+    __syncthreads();
+    for (int s = BP_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            for (int k = 0; k < 3; k++) red[k][tid] = fmin(red[k][tid], red[k][tid + s]);
+            for (int k = 3; k < 6; k++) red[k][tid] = fmax(red[k][tid], red[k][tid + s]);
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < 6; k++) v[k] = red[k][0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(BP_THREADS) void body_box_kernel(const double* V, const int* bvs, const uint8_t* is_static,
+                                                             const double* r_max, const double* p0, const double* p1,
+                                                             double infl, double* box_raw, double* box_inf) {
+    __shared__ double R0[9], R1[9];
+    __shared__ double red[6][BP_THREADS];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const double* q0 = p0 + 6 * b;
+    const double* q1 = p1 + 6 * b;
+    const bool same = q0[3] == q1[3] && q0[4] == q1[4] && q0[5] == q1[5];
+    double v[6];
+    if (is_static[b] || same) {
+        if (tid == 0) rotation_d(q0 + 3, R0);
+        if (tid == 1) rotation_d(q1 + 3, R1);
+        __syncthreads();
+        for (int k = 0; k < 3; k++) { v[k] = INFINITY; v[3 + k] = -INFINITY; }
+        for (int i = bvs[b] + tid; i < bvs[b + 1]; i += BP_THREADS) {
+            double w0[3], w1[3];
+            world_vertex_d(R0, q0, V + 3 * i, w0);
+            world_vertex_d(R1, q1, V + 3 * i, w1);
+            for (int k = 0; k < 3; k++) {
+                v[k] = fmin(v[k], fmin(w0[k], w1[k]));
+                v[3 + k] = fmax(v[3 + k], fmax(w0[k], w1[k]));
+            }
+        }
+        block_minmax6(v, red);
+    } else {
+        for (int k = 0; k < 3; k++) {
+            v[k] = fmin(q0[k], q1[k]) - r_max[b];
+            v[3 + k] = fmax(q0[k], q1[k]) + r_max[b];
+        }
+    }
+    if (tid == 0)
+        for (int k = 0; k < 3; k++) {
+            box_raw[6 * b + k] = v[k];
+            box_raw[6 * b + 3 + k] = v[3 + k];
+            box_inf[6 * b + k] = v[k] - infl;
+            box_inf[6 * b + 3 + k] = v[3 + k] + infl;
+        }
+}
+
+__global__ __launch_bounds__(BP_THREADS) void included_kernel(int nb, const int* group, const double* box_inf, int* inc) {
+    __shared__ double tb[BP_THREADS][6];
+    __shared__ int tg[BP_THREADS];
+    const int i = blockIdx.x * BP_THREADS + threadIdx.x;
+    double mi[6];
+    int gi = 0;
+    if (i < nb) {
+        for (int k = 0; k < 6; k++) mi[k] = box_inf[6 * i + k];
+        gi = group[i];
+    }
+    bool any = false;
+    for (int j0 = 0; j0 < nb; j0 += BP_THREADS) {
+        const int j = j0 + threadIdx.x;
+        if (j < nb) {
+            for (int k = 0; k < 6; k++) tb[threadIdx.x][k] = box_inf[6 * j + k];
+            tg[threadIdx.x] = group[j];
+        }
+        __syncthreads();
+        const int m = min(BP_THREADS, nb - j0);
+        if (i < nb && !any)
+            for (int jj = 0; jj < m; jj++) {
+                if (j0 + jj == i || tg[jj] == gi) continue;
+                const double* o = tb[jj];
+                if (mi[0] <= o[3] && o[0] <= mi[3] && mi[1] <= o[4] && o[1] <= mi[4] && mi[2] <= o[5] && o[2] <= mi[5]) {
+                    any = true;
+                    break;
+                }
+            }
+        __syncthreads();
+    }
+    if (i < nb) inc[i] = any ? 1 : 0;
+}
+
+__global__ __launch_bounds__(BP_THREADS) void domain_kernel(int nb, const int* inc, const double* box_raw, double r,
+                                                           double* dom, int* any_out) {
+    __shared__ double red[6][BP_THREADS];
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    double v[6];
+    for (int k = 0; k < 3; k++) { v[k] = INFINITY; v[3 + k] = -INFINITY; }
+    int c = 0;
+    for (int b = threadIdx.x; b < nb; b += BP_THREADS)
+        if (inc[b]) {
+            c++;
+            for (int k = 0; k < 3; k++) {
+                v[k] = fmin(v[k], box_raw[6 * b + k]);
+                v[3 + k] = fmax(v[3 + k], box_raw[6 * b + 3 + k]);
+            }
+        }
+    if (c) atomicAdd(&cnt, c);
+    block_minmax6(v, red);
+    if (threadIdx.x == 0) {
+        const double dx = v[3] - v[0], dy = v[4] - v[1], dz = v[5] - v[2];
+        const double diag = sqrt((dx * dx + dy * dy) + dz * dz);
+        for (int k = 0; k < 3; k++) {
+            double mn = v[k] - 1e-8 * diag, mx = v[3 + k] + 1e-8 * diag;
+            mn -= r;
+            mn += r;
+            dom[k] = mn;
+            dom[3 + k] = mx;
+        }
+        *any_out = cnt;
+    }
+}
+
+// Interval enclosure of every vertex trajectory of an included body.
+__global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const double* V, const int* bvs, const int* inc,
+                                                                    const double* p0, const double* p1,
+                                                                    const double* dom, double r, double* vmin,
+                                                                    double* vmax, double* bhull) {
+    __shared__ unsigned long long s_idx[64];
+    __shared__ int s_lev[64], s_force[64];
+    __shared__ int sp;
+    __shared__ unsigned long long c_idx;
+    __shared__ int c_lev, c_force;
+    __shared__ Iv Rs[3][3], ps[3];
+    __shared__ double red[6][BP_THREADS];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    if (!inc[b]) return;
+    const int v0 = bvs[b], v1 = bvs[b + 1];
+    const double* q0 = p0 + 6 * b;
+    const double* q1 = p1 + 6 * b;
+    double dmn[3], dmx[3];
+    for (int k = 0; k < 3; k++) { dmn[k] = dom[k]; dmx[k] = dom[3 + k]; }
+    for (int i = v0 + tid; i < v1; i += BP_THREADS)
+        for (int k = 0; k < 3; k++) {
+            vmin[3 * i + k] = INFINITY;
+            vmax[3 * i + k] = -INFINITY;
+        }
+    if (tid == 0) {
+        s_idx[0] = 0; s_lev[0] = 0; s_force[0] = 1;
+        sp = 1;
+    }
+    __syncthreads();
+    for (;;) {
+        if (sp == 0) break;
+        __syncthreads();
+        if (tid == 0) {
+            sp--;
+            c_idx = s_idx[sp]; c_lev = s_lev[sp]; c_force = s_force[sp];
+        }
+        __syncthreads();
+        const unsigned long long idx = c_idx;
+        const int lev = c_lev, force = c_force;
+        const bool evaluate = force <= 0 || lev > 40;
+        bool keep = false;
+        if (evaluate) {
+            if (tid == 0) {
+                const Iv t = Iv{ ldexp((double)idx, -lev), ldexp((double)(idx + 1), -lev) };
+                Iv p[3], R[3][3];
+                pose_iv(q0, q1, t, p, R);
+                for (int i = 0; i < 3; i++) {
+                    ps[i] = p[i];
+                    for (int j = 0; j < 3; j++) Rs[i][j] = R[i][j];
+                }
+            }
+            __syncthreads();
+            Iv R[3][3], p[3];
+            for (int i = 0; i < 3; i++) {
+                p[i] = ps[i];
+                for (int j = 0; j < 3; j++) R[i][j] = Rs[i][j];
+            }
+            bool bad = false;
+            for (int i = v0 + tid; i < v1; i += BP_THREADS) {
+                Iv w[3];
+                world_vertex_iv(R, p, V + 3 * i, w);
+                for (int k = 0; k < 3; k++) bad = bad || (w[k].lo - r < dmn[k]) || (w[k].hi + r > dmx[k]);
+            }
+            const bool fits = !__syncthreads_or(bad);
+            keep = fits || lev > 40;
+            if (keep)
+                for (int i = v0 + tid; i < v1; i += BP_THREADS) {
+                    Iv w[3];
+                    world_vertex_iv(R, p, V + 3 * i, w);
+                    for (int k = 0; k < 3; k++) {
+                        vmin[3 * i + k] = fmin(vmin[3 * i + k], w[k].lo);
+                        vmax[3 * i + k] = fmax(vmax[3 * i + k], w[k].hi);
+                    }
+                }
+        }
+        if (!keep && tid == 0 && sp + 2 <= 64) {
+            s_idx[sp] = 2 * idx + 1; s_lev[sp] = lev + 1; s_force[sp] = force - 1;
+            s_idx[sp + 1] = 2 * idx; s_lev[sp + 1] = lev + 1; s_force[sp + 1] = force - 1;
+            sp += 2;
+        }
+        __syncthreads();
+    }
+    double v[6];
+    for (int k = 0; k < 3; k++) { v[k] = INFINITY; v[3 + k] = -INFINITY; }
+    for (int i = v0 + tid; i < v1; i += BP_THREADS)
+        for (int k = 0; k < 3; k++) {
+            const double mn = vmin[3 * i + k] - r, mx = vmax[3 * i + k] + r;
+            vmin[3 * i + k] = mn;
+            vmax[3 * i + k] = mx;
+            v[k] = fmin(v[k], mn);
+            v[3 + k] = fmax(v[3 + k], mx);
+        }
+    block_minmax6(v, red);
+    if (tid == 0)
+        for (int k = 0; k < 6; k++) bhull[6 * b + k] = v[k];
+}
+
+// Discrete detection: world vertex +- r.
+__global__ __launch_bounds__(BP_THREADS) void vertex_static_kernel(const double* V, const int* bvs, const int* inc,
+                                                                  const double* poses, double r, double* vmin,
+                                                                  double* vmax, double* bhull) {
+    __shared__ double R[9];
+    __shared__ double red[6][BP_THREADS];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    if (!inc[b]) return;
+    const double* q = poses + 6 * b;
+    if (tid == 0) rotation_d(q + 3, R);
+    __syncthreads();
+    double v[6];
+    for (int k = 0; k < 3; k++) { v[k] = INFINITY; v[3 + k] = -INFINITY; }
+    for (int i = bvs[b] + tid; i < bvs[b + 1]; i += BP_THREADS) {
+        double w[3];
+        world_vertex_d(R, q, V + 3 * i, w);
+        for (int k = 0; k < 3; k++) {
+            const double mn = w[k] - r, mx = w[k] + r;
+            vmin[3 * i + k] = mn;
+            vmax[3 * i + k] = mx;
+            v[k] = fmin(v[k], mn);
+            v[3 + k] = fmax(v[3 + k], mx);
+        }
+    }
+    block_minmax6(v, red);
+    if (tid == 0)
+        for (int k = 0; k < 6; k++) bhull[6 * b + k] = v[k];
+}
+
+__device__ __forceinline__ void center_half(const double* mn, const double* mx, double* c, double* h) {
+    for (int k = 0; k < 3; k++) {
+        const double hh = (mx[k] - mn[k]) / 2;
+        h[k] = hh;
+        c[k] = mn[k] + hh;
+    }
+}
+
+__global__ void element_aabb_kernel(int nv, int ne, int nf, const int* E, const int* F, const int* v_body,
+                                    const int* inc, const double* vmin, const double* vmax, double* vc, double* vh,
+                                    double* ec, double* eh, double* fc, double* fh) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nv) {
+        if (inc[v_body[i]]) center_half(vmin + 3 * i, vmax + 3 * i, vc + 3 * i, vh + 3 * i);
+    } else if (i < nv + ne) {
+        const long long e = i - nv;
+        const int a = E[2 * e], c = E[2 * e + 1];
+        if (inc[v_body[a]]) {
+            double mn[3], mx[3];
+            for (int k = 0; k < 3; k++) {
+                mn[k] = fmin(vmin[3 * a + k], vmin[3 * c + k]);
+                mx[k] = fmax(vmax[3 * a + k], vmax[3 * c + k]);
+            }
+            center_half(mn, mx, ec + 3 * e, eh + 3 * e);
+        }
+    } else if (i < nv + ne + nf) {
+        const long long f = i - nv - ne;
+        const int a = F[3 * f], c = F[3 * f + 1], d = F[3 * f + 2];
+        if (inc[v_body[a]]) {
+            double mn[3], mx[3];
+            for (int k = 0; k < 3; k++) {
+                mn[k] = fmin(fmin(vmin[3 * a + k], vmin[3 * c + k]), vmin[3 * d + k]);
+                mx[k] = fmax(fmax(vmax[3 * a + k], vmax[3 * c + k]), vmax[3 * d + k]);
+            }
+            center_half(mn, mx, fc + 3 * f, fh + 3 * f);
+        }
+    }
+}
+
+__device__ __forceinline__ double hull_eps(const double* h) {
+    double m = 1.0;
+    for (int k = 0; k < 6; k++) m = fmax(m, fabs(h[k]));
+    return 1e-12 * m;
+}
+
+__global__ __launch_bounds__(BP_THREADS) void body_pair_kernel(int nb, const int* inc, const int* group,
+                                                              const double* bhull, int* pairs, long long cap,
+                                                              unsigned long long* count) {
+    __shared__ double th[BP_THREADS][6];
+    __shared__ int tg[BP_THREADS], ti[BP_THREADS];
+    const int a = blockIdx.x * BP_THREADS + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    double ha[6];
+    int ga = 0;
+    bool ia = false;
+    double ea = 0;
+    if (a < nb) {
+        ia = inc[a] != 0;
+        ga = group[a];
+        for (int k = 0; k < 6; k++) ha[k] = bhull[6 * a + k];
+        if (ia) ea = hull_eps(ha);
+    }
+    for (int b0 = 0; b0 < nb; b0 += BP_THREADS) {
+        const int bb = b0 + threadIdx.x;
+        if (bb < nb) {
+            ti[threadIdx.x] = inc[bb];
+            tg[threadIdx.x] = group[bb];
+            for (int k = 0; k < 6; k++) th[threadIdx.x][k] = bhull[6 * bb + k];
+        }
+        __syncthreads();
+        const int m = min(BP_THREADS, nb - b0);
+        for (int j = 0; j < m; j++) {
+            const int b = b0 + j;
+            bool hit = false;
+            if (ia && b > a && ti[j] && tg[j] != ga) {
+                const double* hb = th[j];
+                const double eps = fmax(ea, hull_eps(hb));
+                hit = ha[0] <= hb[3] + eps && hb[0] <= ha[3] + eps && ha[1] <= hb[4] + eps && hb[1] <= ha[4] + eps
+                    && ha[2] <= hb[5] + eps && hb[2] <= ha[5] + eps;
+            }
+            const unsigned long long mask = __ballot(hit);
+            if (mask) {
+                const int leader = __ffsll((long long)mask) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+                base = __shfl(base, leader);
+                if (hit) {
+                    const unsigned long long k = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+                    if ((long long)k < cap) {
+                        pairs[2 * k] = a;
+                        pairs[2 * k + 1] = b;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ bool aabb_overlap(const double* ca, const double* ha, const double* cb, const double* hb) {
+    return fabs(ca[0] - cb[0]) <= ha[0] + hb[0] && fabs(ca[1] - cb[1]) <= ha[1] + hb[1]
+        && fabs(ca[2] - cb[2]) <= ha[2] + hb[2];
+}
+
+__device__ __forceinline__ bool reaches(const double* c, const double* h, const double* hull, double eps) {
+    return c[0] + h[0] >= hull[0] - eps && c[0] - h[0] <= hull[3] + eps && c[1] + h[1] >= hull[1] - eps
+        && c[1] - h[1] <= hull[4] + eps && c[2] + h[2] >= hull[2] - eps && c[2] - h[2] <= hull[5] + eps;
+}
+
+// Filtered element list of one body held in LDS, or the body's whole range
+// when the filtered list does not fit.
+struct ElemList {
+    int* ids;
+    int n;
+    int start, count;
+    bool full;
+    __device__ int get(int k) const { return full ? start + k : ids[k]; }
+};
+
+__device__ void build_list(int* lds, int* cnt, int start, int end, const double* c, const double* h,
+                           const double* hull, double eps) {
+    for (int i = start + threadIdx.x; i < end; i += BP_THREADS)
+        if (reaches(c + 3 * i, h + 3 * i, hull, eps)) {
+            const int k = atomicAdd(cnt, 1);
+            if (k < LIST_CAP) lds[k] = i;
+        }
+}
+
+__device__ __forceinline__ void emit(bool hit, unsigned long long key, unsigned long long* out, long long cap,
+                                     unsigned long long* count) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long mask = __ballot(hit);
+    if (!mask) return;
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    if (hit) {
+        const unsigned long long k = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+        if ((long long)k < cap) out[k] = key;
+    }
+}
+
+__global__ __launch_bounds__(BP_THREADS) void element_pair_kernel(
+    const int* pairs, const int* bvs, const int* bes, const int* bfs, const int* group, const double* bhull,
+    const double* vc, const double* vh, const double* ec, const double* eh, const double* fc, const double* fh,
+    long long nv, long long ne, unsigned long long* out_ee, long long cap_ee, unsigned long long* out_fv,
+    long long cap_fv, unsigned long long* counters) {
+    __shared__ int l_ae[LIST_CAP], l_af[LIST_CAP], l_av[LIST_CAP], l_be[LIST_CAP], l_bf[LIST_CAP], l_bv[LIST_CAP];
+    __shared__ int n_ae, n_af, n_av, n_be, n_bf, n_bv;
+    const int A = pairs[2 * blockIdx.x], B = pairs[2 * blockIdx.x + 1];
+    const double* hA = bhull + 6 * A;
+    const double* hB = bhull + 6 * B;
+    const double eps = fmax(hull_eps(hA), hull_eps(hB));
+    if (threadIdx.x == 0) n_ae = n_af = n_av = n_be = n_bf = n_bv = 0;
+    __syncthreads();
+    build_list(l_ae, &n_ae, bes[A], bes[A + 1], ec, eh, hB, eps);
+    build_list(l_af, &n_af, bfs[A], bfs[A + 1], fc, fh, hB, eps);
+    build_list(l_av, &n_av, bvs[A], bvs[A + 1], vc, vh, hB, eps);
+    build_list(l_be, &n_be, bes[B], bes[B + 1], ec, eh, hA, eps);
+    build_list(l_bf, &n_bf, bfs[B], bfs[B + 1], fc, fh, hA, eps);
+    build_list(l_bv, &n_bv, bvs[B], bvs[B + 1], vc, vh, hA, eps);
+    __syncthreads();
+    const ElemList ae{ l_ae, n_ae, bes[A], bes[A + 1] - bes[A], n_ae > LIST_CAP };
+    const ElemList af{ l_af, n_af, bfs[A], bfs[A + 1] - bfs[A], n_af > LIST_CAP };
+    const ElemList av{ l_av, n_av, bvs[A], bvs[A + 1] - bvs[A], n_av > LIST_CAP };
+    const ElemList be{ l_be, n_be, bes[B], bes[B + 1] - bes[B], n_be > LIST_CAP };
+    const ElemList bf{ l_bf, n_bf, bfs[B], bfs[B + 1] - bfs[B], n_bf > LIST_CAP };
+    const ElemList bv{ l_bv, n_bv, bvs[B], bvs[B + 1] - bvs[B], n_bv > LIST_CAP };
+    auto len = [](const ElemList& l) { return l.full ? l.count : l.n; };
+
+    // edge-edge: edges of A (lower ids) against edges of B
+    {
+        const long long na = len(ae), nbb = len(be), tot = na * nbb;
+        for (long long k0 = 0; k0 < tot; k0 += BP_THREADS) {
+            const long long k = k0 + threadIdx.x;
+            bool hit = false;
+            unsigned long long key = 0;
+            if (k < tot) {
+                const int ea = ae.get((int)(k / nbb)), eb = be.get((int)(k % nbb));
+                hit = aabb_overlap(ec + 3 * ea, eh + 3 * ea, ec + 3 * eb, eh + 3 * eb);
+                key = (unsigned long long)ea * (unsigned long long)ne + (unsigned long long)eb;
+            }
+            emit(hit, key, out_ee, cap_ee, counters + 0);
+        }
+    }
+    // face-vertex: faces of A with vertices of B, and faces of B with vertices of A
+    for (int dir = 0; dir < 2; dir++) {
+        const ElemList& lf = dir == 0 ? af : bf;
+        const ElemList& lv = dir == 0 ? bv : av;
+        const long long nfl = len(lf), nvl = len(lv), tot = nfl * nvl;
+        for (long long k0 = 0; k0 < tot; k0 += BP_THREADS) {
+            const long long k = k0 + threadIdx.x;
+            bool hit = false;
+            unsigned long long key = 0;
+            if (k < tot) {
+                const int f = lf.get((int)(k / nvl)), v = lv.get((int)(k % nvl));
+                hit = aabb_overlap(fc + 3 * f, fh + 3 * f, vc + 3 * v, vh + 3 * v);
+                key = (unsigned long long)f * (unsigned long long)nv + (unsigned long long)v;
+            }
+            emit(hit, key, out_fv, cap_fv, counters + 1);
+        }
+    }
+}
+
+__global__ void unpack_kernel(const unsigned long long* keys, long long n, unsigned long long m, int* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const unsigned long long k = keys[i];
+        out[2 * i] = (int)(k / m);
+        out[2 * i + 1] = (int)(k % m);
+    }
+}
+
+int bit_length(unsigned long long x) {
+    int b = 0;
+    while (x) { b++; x >>= 1; }
+    return b;
+}
+
+}  // namespace
+
+#define BP_CHECK(x)                      \
+    do {                                 \
+        hipError_t e_ = (x);             \
+        if (e_ != hipSuccess) return e_; \
+    } while (0)
+
+hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const double* d_p0, const double* d_p1,
+                           double r, bool continuous, hipStream_t st) {
+    bp.n_ee = bp.n_fv = bp.n_body_pairs = 0;
+    const int nb = sc.nb;
+    if (nb <= 1) return hipSuccess;
+    BP_CHECK(bp.box_raw.ensure(6 * sizeof(double) * nb));
+    BP_CHECK(bp.box_inf.ensure(6 * sizeof(double) * nb));
+    BP_CHECK(bp.inc.ensure(sizeof(int) * nb));
+    BP_CHECK(bp.dom.ensure(8 * sizeof(double)));
+    BP_CHECK(bp.bhull.ensure(6 * sizeof(double) * nb));
+    BP_CHECK(bp.vmin.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
+    BP_CHECK(bp.vmax.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
+    BP_CHECK(bp.vc.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
+    BP_CHECK(bp.vh.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
+    BP_CHECK(bp.ec.ensure(3 * sizeof(double) * std::max(sc.ne, 1)));
+    BP_CHECK(bp.eh.ensure(3 * sizeof(double) * std::max(sc.ne, 1)));
+    BP_CHECK(bp.fc.ensure(3 * sizeof(double) * std::max(sc.nf, 1)));
+    BP_CHECK(bp.fh.ensure(3 * sizeof(double) * std::max(sc.nf, 1)));
+    BP_CHECK(bp.counters.ensure(4 * sizeof(unsigned long long)));
+    unsigned long long* cnt = bp.counters.as<unsigned long long>();
+
+    const double* p1 = continuous ? d_p1 : d_p0;
+    hipLaunchKernelGGL(body_box_kernel, dim3(nb), dim3(BP_THREADS), 0, st, sc.V, sc.body_vstart, sc.is_static,
+                       sc.r_max, d_p0, p1, r, bp.box_raw.as<double>(), bp.box_inf.as<double>());
+    hipLaunchKernelGGL(included_kernel, dim3((nb + BP_THREADS - 1) / BP_THREADS), dim3(BP_THREADS), 0, st, nb,
+                       sc.group, bp.box_inf.as<double>(), bp.inc.as<int>());
+    int* any_dev = reinterpret_cast<int*>(bp.dom.as<double>() + 6);
+    hipLaunchKernelGGL(domain_kernel, dim3(1), dim3(BP_THREADS), 0, st, nb, bp.inc.as<int>(),
+                       bp.box_raw.as<double>(), r, bp.dom.as<double>(), any_dev);
+    if (continuous)
+        hipLaunchKernelGGL(vertex_interval_kernel, dim3(nb), dim3(BP_THREADS), 0, st, sc.V, sc.body_vstart,
+                           bp.inc.as<int>(), d_p0, d_p1, bp.dom.as<double>(), r, bp.vmin.as<double>(),
+                           bp.vmax.as<double>(), bp.bhull.as<double>());
+    else
+        hipLaunchKernelGGL(vertex_static_kernel, dim3(nb), dim3(BP_THREADS), 0, st, sc.V, sc.body_vstart,
+                           bp.inc.as<int>(), d_p0, r, bp.vmin.as<double>(), bp.vmax.as<double>(),
+                           bp.bhull.as<double>());
+    const long long nel = (long long)sc.nv + sc.ne + sc.nf;
+    hipLaunchKernelGGL(element_aabb_kernel, dim3((unsigned)((nel + 255) / 256)), dim3(256), 0, st, sc.nv, sc.ne,
+                       sc.nf, sc.E, sc.F, sc.v_body, bp.inc.as<int>(), bp.vmin.as<double>(), bp.vmax.as<double>(),
+                       bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(), bp.eh.as<double>(),
+                       bp.fc.as<double>(), bp.fh.as<double>());
+    BP_CHECK(hipGetLastError());
+
+    // body pairs (retry once with the exact size if the first guess is short)
+    long long cap_pairs = std::max<long long>(1024, 32LL * nb);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * cap_pairs));
+        BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(body_pair_kernel, dim3((nb + BP_THREADS - 1) / BP_THREADS), dim3(BP_THREADS), 0, st, nb,
+                           bp.inc.as<int>(), sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt);
+        unsigned long long n = 0;
+        BP_CHECK(hipMemcpyAsync(&n, cnt, sizeof(n), hipMemcpyDeviceToHost, st));
+        BP_CHECK(hipStreamSynchronize(st));
+        bp.n_body_pairs = (long long)n;
+        if ((long long)n <= cap_pairs) break;
+        cap_pairs = (long long)n;
+    }
+    if (bp.n_body_pairs == 0) return hipSuccess;
+
+    // element pairs
+    long long cap_ee = std::max<long long>(bp.keys_ee.bytes / 8, 1 << 16);
+    long long cap_fv = std::max<long long>(bp.keys_fv.bytes / 8, 1 << 16);
+    unsigned long long got[2] = { 0, 0 };
+    for (int attempt = 0; attempt < 2; attempt++) {
+        BP_CHECK(bp.keys_ee.ensure(8 * cap_ee));
+        BP_CHECK(bp.keys_fv.ensure(8 * cap_fv));
+        BP_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(element_pair_kernel, dim3((unsigned)bp.n_body_pairs), dim3(BP_THREADS), 0, st,
+                           bp.pairs.as<int>(), sc.body_vstart, sc.body_estart, sc.body_fstart, sc.group,
+                           bp.bhull.as<double>(), bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(),
+                           bp.eh.as<double>(), bp.fc.as<double>(), bp.fh.as<double>(), (long long)sc.nv,
+                           (long long)sc.ne, bp.keys_ee.as<unsigned long long>(), cap_ee,
+                           bp.keys_fv.as<unsigned long long>(), cap_fv, cnt);
+        BP_CHECK(hipMemcpyAsync(got, cnt, sizeof(got), hipMemcpyDeviceToHost, st));
+        BP_CHECK(hipStreamSynchronize(st));
+        if ((long long)got[0] <= cap_ee && (long long)got[1] <= cap_fv) break;
+        cap_ee = std::max<long long>(cap_ee, (long long)got[0]);
+        cap_fv = std::max<long long>(cap_fv, (long long)got[1]);
+    }
+    bp.n_ee = (long long)got[0];
+    bp.n_fv = (long long)got[1];
+
+    // sort and unpack
+    const long long nmax = std::max(bp.n_ee, bp.n_fv);
+    BP_CHECK(bp.tmp.ensure(8 * std::max<long long>(nmax, 1)));
+    BP_CHECK(bp.hist.ensure(sizeof(unsigned int) * radix_sort_hist_entries(std::max<long long>(nmax, 1))));
+    BP_CHECK(bp.out_ee.ensure(2 * sizeof(int) * std::max<long long>(bp.n_ee, 1)));
+    BP_CHECK(bp.out_fv.ensure(2 * sizeof(int) * std::max<long long>(bp.n_fv, 1)));
+    const unsigned long long me = (unsigned long long)sc.ne, mv = (unsigned long long)sc.nv;
+    if (bp.n_ee) {
+        BP_CHECK(radix_sort_u64(bp.keys_ee.as<unsigned long long>(), bp.tmp.as<unsigned long long>(),
+                                bp.hist.as<unsigned int>(), bp.n_ee, bit_length(me * me), st));
+        hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)((bp.n_ee + 255) / 256)), dim3(256), 0, st,
+                           bp.keys_ee.as<unsigned long long>(), bp.n_ee, me, bp.out_ee.as<int>());
+    }
+    if (bp.n_fv) {
+        BP_CHECK(radix_sort_u64(bp.keys_fv.as<unsigned long long>(), bp.tmp.as<unsigned long long>(),
+                                bp.hist.as<unsigned int>(), bp.n_fv, bit_length((unsigned long long)sc.nf * mv),
+                                st));
+        hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)((bp.n_fv + 255) / 256)), dim3(256), 0, st,
+                           bp.keys_fv.as<unsigned long long>(), bp.n_fv, mv, bp.out_fv.as<int>());
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rigc

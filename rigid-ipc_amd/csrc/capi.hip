@@ -12,31 +12,6 @@
 
 using namespace rigc;
 
-namespace rigc {
-
-// Grow-only device allocation.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n) {
-        if (n <= bytes) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t want = std::max<size_t>(n, 256);
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) bytes = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <typename T> T* as() const { return static_cast<T*>(p); }
-};
-
-}  // namespace rigc
 
 struct rigc_ctx {
     int device = 0;
@@ -53,6 +28,8 @@ struct rigc_ctx {
 
     // per-call buffers
     DevBuf poses0, poses1, ee, fv, toi, hit, counters;
+    BroadPhase bp;
+    hipEvent_t ev2 = nullptr;
 
     rigc_status fail(rigc_status s, const std::string& msg) {
         err = msg;
@@ -105,6 +82,7 @@ rigc_status rigc_create(rigc_ctx** out, int32_t device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev2);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
@@ -125,6 +103,12 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
         b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+    for (DevBuf* b : { &ctx->bp.box_raw, &ctx->bp.box_inf, &ctx->bp.inc, &ctx->bp.dom, &ctx->bp.vmin, &ctx->bp.vmax,
+                       &ctx->bp.bhull, &ctx->bp.vc, &ctx->bp.vh, &ctx->bp.ec, &ctx->bp.eh, &ctx->bp.fc, &ctx->bp.fh,
+                       &ctx->bp.pairs, &ctx->bp.keys_ee, &ctx->bp.keys_fv, &ctx->bp.tmp, &ctx->bp.hist,
+                       &ctx->bp.counters, &ctx->bp.out_ee, &ctx->bp.out_fv })
+        b->release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RIGC_OK;
@@ -196,6 +180,8 @@ rigc_status rigc_set_scene(rigc_ctx* ctx, const rigc_scene* s) {
     RIGC_HIP(upload(ctx, &d.e_body, e_body.data(), (size_t)ne));
     RIGC_HIP(upload(ctx, &d.f_body, f_body.data(), (size_t)nf));
     RIGC_HIP(upload(ctx, &d.body_vstart, s->body_vertex_start, (size_t)nb + 1));
+    RIGC_HIP(upload(ctx, &d.body_estart, s->body_edge_start, (size_t)nb + 1));
+    RIGC_HIP(upload(ctx, &d.body_fstart, s->body_face_start, (size_t)nb + 1));
     RIGC_HIP(upload(ctx, &d.group, s->group_ids, (size_t)nb));
     RIGC_HIP(upload(ctx, &d.is_static, s->is_static, (size_t)nb));
     RIGC_HIP(upload(ctx, &d.edge_len, elen.data(), (size_t)ne));
@@ -298,6 +284,115 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     if (hit_out) std::memcpy(hit_out, hits.data(), nq);
     if (unresolved)
         return ctx->fail(RIGC_ERR_UNRESOLVED, "rigc_narrow_phase: a query exceeded its evaluation budget");
+    return RIGC_OK;
+}
+
+namespace {
+
+rigc_status upload_poses(rigc_ctx* ctx, const double* p0, const double* p1) {
+    const size_t pose_bytes = 6 * sizeof(double) * (size_t)ctx->scene.nb;
+    RIGC_HIP(ctx->poses0.ensure(pose_bytes));
+    RIGC_HIP(ctx->poses1.ensure(pose_bytes));
+    RIGC_HIP(hipMemcpyAsync(ctx->poses0.p, p0, pose_bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (p1) RIGC_HIP(hipMemcpyAsync(ctx->poses1.p, p1, pose_bytes, hipMemcpyHostToDevice, ctx->stream));
+    return RIGC_OK;
+}
+
+CcdArgs ccd_args(rigc_ctx* ctx) {
+    const DevScene& d = ctx->scene;
+    CcdArgs a{};
+    a.V = d.V; a.E = d.E; a.F = d.F; a.FE = d.FE;
+    a.v_body = d.v_body; a.e_body = d.e_body; a.f_body = d.f_body; a.edge_len = d.edge_len;
+    a.poses_t0 = ctx->poses0.as<double>();
+    a.poses_t1 = ctx->poses1.as<double>();
+    a.global_best = ctx->counters.as<unsigned long long>();
+    a.counter = ctx->counters.as<unsigned long long>() + 1;
+    a.total_evals = ctx->counters.as<unsigned long long>() + 2;
+    return a;
+}
+
+}  // namespace
+
+rigc_status rigc_broad_phase(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                             double inflation_radius, int32_t* ee, int64_t ee_capacity, int64_t* n_ee,
+                             int32_t* fv, int64_t fv_capacity, int64_t* n_fv) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_broad_phase: no scene");
+    if (!poses_t0 || !n_ee || !n_fv || ee_capacity < 0 || fv_capacity < 0 || !(inflation_radius >= 0)
+        || (ee_capacity && !ee) || (fv_capacity && !fv))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_broad_phase: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status st = upload_poses(ctx, poses_t0, poses_t1);
+    if (st != RIGC_OK) return st;
+    RIGC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
+                             inflation_radius, poses_t1 != nullptr, ctx->stream));
+    RIGC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    *n_ee = ctx->bp.n_ee;
+    *n_fv = ctx->bp.n_fv;
+    const bool fits = ctx->bp.n_ee <= ee_capacity && ctx->bp.n_fv <= fv_capacity;
+    if (fits) {
+        if (ctx->bp.n_ee)
+            RIGC_HIP(hipMemcpyAsync(ee, ctx->bp.out_ee.p, 2 * sizeof(int32_t) * ctx->bp.n_ee, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+        if (ctx->bp.n_fv)
+            RIGC_HIP(hipMemcpyAsync(fv, ctx->bp.out_fv.p, 2 * sizeof(int32_t) * ctx->bp.n_fv, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    }
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.broad_phase_ms = ms;
+    ctx->stats.num_candidates = ctx->bp.n_ee + ctx->bp.n_fv;
+    if (!fits) return ctx->fail(RIGC_ERR_CAPACITY, "rigc_broad_phase: output capacity too small");
+    return RIGC_OK;
+}
+
+rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                      double minimum_separation_distance, double* earliest_toi) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_earliest_toi: no scene");
+    if (!poses_t0 || !poses_t1 || !earliest_toi || !(minimum_separation_distance >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_compute_earliest_toi: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status st = upload_poses(ctx, poses_t0, poses_t1);
+    if (st != RIGC_OK) return st;
+    hipStream_t s = ctx->stream;
+    RIGC_HIP(hipEventRecord(ctx->ev0, s));
+    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
+                             minimum_separation_distance / 2.0, true, s));
+    RIGC_HIP(hipEventRecord(ctx->ev1, s));
+    const long long nq = ctx->bp.n_ee + ctx->bp.n_fv;
+    RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
+    unsigned long long init[4] = { 0x7ff0000000000000ULL, 0, 0, 0 };
+    RIGC_HIP(hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    CcdArgs a = ccd_args(ctx);
+    a.ee = ctx->bp.out_ee.as<int>();
+    a.fv = ctx->bp.out_fv.as<int>();
+    a.n_ee = ctx->bp.n_ee;
+    a.n_fv = ctx->bp.n_fv;
+    a.toi_tol = 1e-4;
+    a.eval_budget = 0;
+    a.use_global_best = 1;
+    a.toi_out = nullptr;
+    a.hit_out = nullptr;
+    const int grid = (int)std::max<long long>(1, std::min<long long>((nq + 255) / 256, (long long)ctx->num_cus * 8));
+    if (nq) launch_ccd(a, grid, 256, s);
+    RIGC_HIP(hipGetLastError());
+    RIGC_HIP(hipEventRecord(ctx->ev2, s));
+    unsigned long long res[4];
+    RIGC_HIP(hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, s));
+    RIGC_HIP(hipStreamSynchronize(s));
+    float bms = 0, nms = 0;
+    RIGC_HIP(hipEventElapsedTime(&bms, ctx->ev0, ctx->ev1));
+    RIGC_HIP(hipEventElapsedTime(&nms, ctx->ev1, ctx->ev2));
+    ctx->stats.broad_phase_ms = bms;
+    ctx->stats.narrow_phase_ms = nms;
+    ctx->stats.box_evaluations = (int64_t)res[2];
+    ctx->stats.num_candidates = nq;
+    double g;
+    std::memcpy(&g, &res[0], sizeof(double));
+    *earliest_toi = g;  // +inf when no candidate collides
     return RIGC_OK;
 }
 

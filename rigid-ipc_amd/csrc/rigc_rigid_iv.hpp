@@ -1,0 +1,49 @@
+// Interval rigid-body kinematics on the device: pose interpolation over an
+// interval of time, interval Rodrigues rotation and interval world vertices.
+#pragma once
+
+#include "rigc_interval.hpp"
+
+namespace rigc {
+
+// construct_rotation_matrix<Interval> (src/physics/pose.tpp:202-218)
+RIGC_DI void rotation_iv(const Iv r[3], Iv R[3][3]) {
+    const Iv a0 = iabs(r[0]), a1 = iabs(r[1]), a2 = iabs(r[2]);
+    const Iv sinc_angle = sinc_iv(isqrt((a0 * a0 + a1 * a1) + a2 * a2));
+    Iv h[3];
+    for (int i = 0; i < 3; i++) h[i] = r[i] / 2.0;
+    const Iv h0 = iabs(h[0]), h1 = iabs(h[1]), h2 = iabs(h[2]);
+    const Iv sinc_half = sinc_iv(isqrt((h0 * h0 + h1 * h1) + h2 * h2));
+    const Iv z = iv(0.0);
+    const Iv K[3][3] = { { z, -r[2], r[1] }, { r[2], z, -r[0] }, { -r[1], r[0], z } };
+    const Iv c2 = (sinc_half * 0.5) * sinc_half;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const Iv k2 = K[i][0] * K[0][j] + (K[i][1] * K[1][j] + K[i][2] * K[2][j]);
+            R[i][j] = sinc_angle * K[i][j] + c2 * k2;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; i++) R[i][i] = R[i][i] + iv(1.0);
+}
+
+// Pose<Interval>::interpolate (pose.tpp:141-148) from double endpoints.
+RIGC_DI void pose_iv(const double* p0, const double* p1, Iv t, Iv p[3], Iv R[3][3]) {
+    Iv r[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        p[i] = (iv(p1[i]) - iv(p0[i])) * t + iv(p0[i]);
+        r[i] = (iv(p1[3 + i]) - iv(p0[3 + i])) * t + iv(p0[3 + i]);
+    }
+    rotation_iv(r, R);
+}
+
+// RigidBody::world_vertex (src/physics/rigid_body.tpp:18-24)
+RIGC_DI void world_vertex_iv(const Iv R[3][3], const Iv p[3], const double* v, Iv w[3]) {
+    const double v0 = v[0], v1 = v[1], v2 = v[2];
+#pragma unroll
+    for (int j = 0; j < 3; j++) w[j] = ((R[j][0] * v0 + R[j][1] * v1) + R[j][2] * v2) + p[j];
+}
+
+}  // namespace rigc

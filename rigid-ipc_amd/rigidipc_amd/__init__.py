@@ -72,7 +72,11 @@ def lib():
     L.rigc_narrow_phase.argtypes = [vp, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_int64,
                                     _c_int32_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
                                     ctypes.c_int32, _c_double_p, _c_double_p, _c_int8_p]
-    for name in ("rigc_create", "rigc_destroy", "rigc_get_stats", "rigc_set_scene", "rigc_narrow_phase"):
+    L.rigc_broad_phase.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_int32_p, ctypes.c_int64,
+                                   _c_int64_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
+    L.rigc_compute_earliest_toi.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p]
+    for name in ("rigc_create", "rigc_destroy", "rigc_get_stats", "rigc_set_scene", "rigc_narrow_phase",
+                 "rigc_broad_phase", "rigc_compute_earliest_toi"):
         getattr(L, name).restype = ctypes.c_int32
     _lib = L
     return L
@@ -157,3 +161,33 @@ class Context:
         if not (allow_unresolved and st == RIGC_ERR_UNRESOLVED):
             self._check(st)
         return best.value, toi[:n], hit[:n]
+
+    def broad_phase(self, poses_t0, poses_t1=None, inflation_radius=0.0):
+        """Hash-grid candidates, continuous (poses_t1 given) or discrete;
+        returns sorted (ee (n,2), fv (n,2)) int32 arrays."""
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        p1 = None if poses_t1 is None else np.ascontiguousarray(poses_t1, np.float64).reshape(-1, 6)
+        cap_ee, cap_fv = getattr(self, "_cap_ee", 1 << 16), getattr(self, "_cap_fv", 1 << 16)
+        for _ in range(2):
+            ee = np.empty((cap_ee, 2), np.int32)
+            fv = np.empty((cap_fv, 2), np.int32)
+            ne, nf = ctypes.c_int64(), ctypes.c_int64()
+            st = self._L.rigc_broad_phase(self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p),
+                                          float(inflation_radius), _p(ee, _c_int32_p), cap_ee, ctypes.byref(ne),
+                                          _p(fv, _c_int32_p), cap_fv, ctypes.byref(nf))
+            if st == RIGC_ERR_CAPACITY:
+                cap_ee, cap_fv = max(cap_ee, ne.value), max(cap_fv, nf.value)
+                self._cap_ee, self._cap_fv = cap_ee, cap_fv
+                continue
+            self._check(st)
+            return ee[:ne.value].copy(), fv[:nf.value].copy()
+        raise RigcError(RIGC_ERR_CAPACITY, "broad phase capacity retry failed")
+
+    def compute_earliest_toi(self, poses_t0, poses_t1, minimum_separation_distance=0.0):
+        """Broad + narrow phase on the device; +inf when nothing collides."""
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        p1 = np.ascontiguousarray(poses_t1, np.float64).reshape(-1, 6)
+        toi = ctypes.c_double()
+        self._check(self._L.rigc_compute_earliest_toi(self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p),
+                                                      float(minimum_separation_distance), ctypes.byref(toi)))
+        return toi.value

@@ -56,6 +56,12 @@ def lib():
             getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p]
             getattr(L, name).restype = ctypes.c_int
         L.oracle_set_eval_budget.argtypes = [ctypes.c_long]
+        L.oracle_set_scene.argtypes = [ctypes.c_int] * 4 + [c_double_p] + [c_int32_p] * 7 + [ctypes.c_void_p]
+        L.oracle_broadphase.argtypes = [ctypes.c_int, c_double_p, c_double_p, ctypes.c_double, c_long_p, c_long_p]
+        L.oracle_get_candidates.argtypes = [c_int32_p, c_int32_p]
+        L.oracle_average_edge_length.restype = ctypes.c_double
+        L.oracle_world_vertices.argtypes = [c_double_p, c_double_p]
+        L.oracle_last_vertex_aabbs.argtypes = [c_double_p]
         for name in ("oracle_ee_aabb", "oracle_fv_aabb"):
             getattr(L, name).argtypes = [c_double_p, c_double_p, c_double_p]
     return _lib
@@ -92,3 +98,45 @@ def fv_toi(q, earliest_toi=1.0, toi_tol=1e-4):
 def set_eval_budget(n):
     """Cap inclusion-function evaluations per query (0 = unbounded)."""
     lib().oracle_set_eval_budget(int(n))
+
+
+def set_scene(scene):
+    """Load a scene (same fields as the product's Context.set_scene) into the oracle."""
+    a = dict(
+        V=np.ascontiguousarray(scene.vertices, np.float64), E=np.ascontiguousarray(scene.edges, np.int32),
+        F=np.ascontiguousarray(scene.faces, np.int32), FE=np.ascontiguousarray(scene.face_edges, np.int32),
+        bv=np.ascontiguousarray(scene.body_vertex_start, np.int32),
+        be=np.ascontiguousarray(scene.body_edge_start, np.int32),
+        bf=np.ascontiguousarray(scene.body_face_start, np.int32),
+        g=np.ascontiguousarray(scene.group_ids, np.int32), st=np.ascontiguousarray(scene.is_static, np.uint8))
+    global _scene_keep
+    _scene_keep = a
+    lib().oracle_set_scene(len(a["g"]), len(a["V"]), len(a["E"]), len(a["F"]), dptr(a["V"]), i32ptr(a["E"]),
+                           i32ptr(a["F"]), i32ptr(a["FE"]), i32ptr(a["bv"]), i32ptr(a["be"]), i32ptr(a["bf"]),
+                           i32ptr(a["g"]), a["st"].ctypes.data_as(ctypes.c_void_p))
+
+
+def broadphase(poses_t0, poses_t1=None, inflation_radius=0.0):
+    """Continuous (poses_t1 given) or discrete broad phase; returns (ee, fv) int32 arrays."""
+    p0 = np.ascontiguousarray(poses_t0, np.float64)
+    cont = poses_t1 is not None
+    p1 = np.ascontiguousarray(poses_t1 if cont else poses_t0, np.float64)
+    ne, nf = ctypes.c_long(), ctypes.c_long()
+    lib().oracle_broadphase(int(cont), dptr(p0), dptr(p1), float(inflation_radius), ctypes.byref(ne), ctypes.byref(nf))
+    ee = np.zeros((ne.value, 2), np.int32)
+    fv = np.zeros((nf.value, 2), np.int32)
+    lib().oracle_get_candidates(i32ptr(ee), i32ptr(fv))
+    return ee, fv
+
+
+def world_vertices(poses, nv):
+    out = np.zeros((nv, 3))
+    lib().oracle_world_vertices(dptr(np.ascontiguousarray(poses, np.float64)), dptr(out))
+    return out
+
+
+def last_vertex_aabbs(nv):
+    """(included bool (nv,), min (nv,3), max (nv,3)) of the last broad phase."""
+    out = np.zeros((nv, 7))
+    lib().oracle_last_vertex_aabbs(dptr(out))
+    return out[:, 0] > 0, out[:, 1:4], out[:, 4:7]
