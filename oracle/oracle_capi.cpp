@@ -138,6 +138,20 @@ void oracle_fv_aabb(const double* q, const double* box, double* out) {
     for (int i = 0; i < 3; i++) { out[2 * i] = d[i].lo; out[2 * i + 1] = d[i].hi; }
 }
 
+// ---- FP64 op counts (meaningful in the -DORACLE_COUNT_FLOPS build only)
+long oracle_flops_get() {
+#ifdef ORACLE_COUNT_FLOPS
+    return flop_counter();
+#else
+    return -1;
+#endif
+}
+void oracle_flops_reset() {
+#ifdef ORACLE_COUNT_FLOPS
+    flop_counter() = 0;
+#endif
+}
+
 // ---- scene + broad phase
 void oracle_set_scene(int nb, int nv, int ne, int nf, const double* V, const int* E, const int* F,
                       const int* FE, const int* bv, const int* be, const int* bf, const int* group,
@@ -175,6 +189,53 @@ void oracle_last_vertex_aabbs(double* out) {
     const std::vector<double>& v = last_vertex_aabbs();
     std::copy(v.begin(), v.end(), out);
 }
+// DistanceBarrierConstraint::compute_earliest_toi for RIGID trajectories
+// (src/opt/distance_barrier_constraint.cpp:130-265) with every query on
+// t in [0,1] and the minimum over queries, parallel over candidates.
+double oracle_earliest_toi(const double* p0, const double* p1, double msd, long* n_candidates) {
+    const Scene& s = scene();
+    Candidates c;
+    broadphase_ccd(s, p0, p1, msd / 2.0, c);
+    const long ne = (long)c.ee.size(), nf = (long)c.fv.size();
+    *n_candidates = ne + nf;
+    double best = INFINITY;
+#pragma omp parallel for schedule(dynamic, 16) reduction(min : best)
+    for (long i = 0; i < ne + nf; i++) {
+        double toi = INFINITY;
+        bool hit;
+        if (i < ne) {
+            const int ea = c.ee[i][0], eb = c.ee[i][1];
+            const int A = s.e_body[ea], B = s.e_body[eb];
+            EEQuery q;
+            for (int k = 0; k < 3; k++) {
+                q.a0[k] = s.V[3 * s.E[2 * ea] + k]; q.a1[k] = s.V[3 * s.E[2 * ea + 1] + k];
+                q.b0[k] = s.V[3 * s.E[2 * eb] + k]; q.b1[k] = s.V[3 * s.E[2 * eb + 1] + k];
+            }
+            for (int k = 0; k < 6; k++) {
+                q.poseA_t0[k] = p0[6 * A + k]; q.poseA_t1[k] = p1[6 * A + k];
+                q.poseB_t0[k] = p0[6 * B + k]; q.poseB_t1[k] = p1[6 * B + k];
+            }
+            hit = edge_edge_toi(q, toi);
+        } else {
+            const int f = c.fv[i - ne][0], v = c.fv[i - ne][1];
+            const int A = s.v_body[v], B = s.f_body[f];
+            FVQuery q;
+            for (int k = 0; k < 3; k++) {
+                q.v[k] = s.V[3 * v + k];
+                q.f0[k] = s.V[3 * s.F[3 * f] + k]; q.f1[k] = s.V[3 * s.F[3 * f + 1] + k];
+                q.f2[k] = s.V[3 * s.F[3 * f + 2] + k];
+            }
+            for (int k = 0; k < 6; k++) {
+                q.poseA_t0[k] = p0[6 * A + k]; q.poseA_t1[k] = p1[6 * A + k];
+                q.poseB_t0[k] = p0[6 * B + k]; q.poseB_t1[k] = p1[6 * B + k];
+            }
+            hit = face_vertex_toi(q, toi);
+        }
+        if (hit && toi < best) best = toi;
+    }
+    return best;
+}
+
 double oracle_average_edge_length() { return scene().average_edge_length; }
 
 // Double-precision world vertices of every body at the given poses.

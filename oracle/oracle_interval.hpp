@@ -45,18 +45,28 @@ namespace oracle {
 inline double pred(double x) { return std::nextafter(x, -INFINITY); }
 inline double succ(double x) { return std::nextafter(x, INFINITY); }
 
+// Op-counting build (-DORACLE_COUNT_FLOPS, oracle/_build/liboracle_flops.so):
+// counts every FP64 add, sub, mul, div and sqrt of the interval arithmetic
+// and of the cosine kernel, to price the CCD inclusion function.
+#ifdef ORACLE_COUNT_FLOPS
+inline long& flop_counter() { static thread_local long c = 0; return c; }
+#define ORACLE_FLOPS(n) (oracle::flop_counter() += (n))
+#else
+#define ORACLE_FLOPS(n) ((void)0)
+#endif
+
 // --- FILibRounding (filib_rounding.cpp:12-60); arithmetic follows the
 // current hardware rounding mode, exactly as in the reference.
-inline double add_down(double x, double y) { return x == -y ? 0.0 : pred(x + y); }
-inline double add_up(double x, double y) { return x == -y ? 0.0 : succ(x + y); }
-inline double sub_down(double x, double y) { return x == y ? 0.0 : pred(x - y); }
-inline double sub_up(double x, double y) { return x == y ? 0.0 : succ(x - y); }
-inline double mul_down(double x, double y) { return (x == 0 || y == 0) ? 0.0 : pred(x * y); }
-inline double mul_up(double x, double y) { return (x == 0 || y == 0) ? 0.0 : succ(x * y); }
-inline double div_down(double x, double y) { return x == 0 ? 0.0 : pred(x / y); }
-inline double div_up(double x, double y) { return x == 0 ? 0.0 : succ(x / y); }
-inline double sqrt_down(double x) { return x == 0 ? 0.0 : pred(std::sqrt(x)); }
-inline double sqrt_up(double x) { return x == 0 ? 0.0 : succ(std::sqrt(x)); }
+inline double add_down(double x, double y) { ORACLE_FLOPS(1); return x == -y ? 0.0 : pred(x + y); }
+inline double add_up(double x, double y) { ORACLE_FLOPS(1); return x == -y ? 0.0 : succ(x + y); }
+inline double sub_down(double x, double y) { ORACLE_FLOPS(1); return x == y ? 0.0 : pred(x - y); }
+inline double sub_up(double x, double y) { ORACLE_FLOPS(1); return x == y ? 0.0 : succ(x - y); }
+inline double mul_down(double x, double y) { ORACLE_FLOPS(1); return (x == 0 || y == 0) ? 0.0 : pred(x * y); }
+inline double mul_up(double x, double y) { ORACLE_FLOPS(1); return (x == 0 || y == 0) ? 0.0 : succ(x * y); }
+inline double div_down(double x, double y) { ORACLE_FLOPS(1); return x == 0 ? 0.0 : pred(x / y); }
+inline double div_up(double x, double y) { ORACLE_FLOPS(1); return x == 0 ? 0.0 : succ(x / y); }
+inline double sqrt_down(double x) { ORACLE_FLOPS(1); return x == 0 ? 0.0 : pred(std::sqrt(x)); }
+inline double sqrt_up(double x) { ORACLE_FLOPS(1); return x == 0 ? 0.0 : succ(std::sqrt(x)); }
 
 // Boost.Interval pi constants (boost/numeric/interval/constants.hpp): the two
 // doubles bracketing pi, and their exact halves / doubles.

@@ -27,7 +27,7 @@ struct rigc_ctx {
     double average_edge_length = 0;
 
     // per-call buffers
-    DevBuf poses0, poses1, ee, fv, toi, hit, counters;
+    DevBuf poses0, poses1, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
     BroadPhase bp;
     hipEvent_t ev2 = nullptr;
 
@@ -99,7 +99,8 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx);
-    for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->ee, &ctx->fv, &ctx->toi, &ctx->hit, &ctx->counters })
+    for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->ee, &ctx->fv, &ctx->toi, &ctx->hit, &ctx->counters,
+                       &ctx->qbest, &ctx->qvisits, &ctx->qflag })
         b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -218,6 +219,9 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     RIGC_HIP(ctx->toi.ensure(sizeof(double) * std::max<size_t>(nq, 1)));
     RIGC_HIP(ctx->hit.ensure(std::max<size_t>(nq, 1)));
     RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
+    RIGC_HIP(ctx->qbest.ensure(sizeof(unsigned long long) * std::max<size_t>(nq, 1)));
+    RIGC_HIP(ctx->qvisits.ensure(sizeof(unsigned int) * std::max<size_t>(nq, 1)));
+    RIGC_HIP(ctx->qflag.ensure(sizeof(int) * std::max<size_t>(nq, 1)));
     hipStream_t st = ctx->stream;
     RIGC_HIP(hipMemcpyAsync(ctx->poses0.p, poses_t0, pose_bytes, hipMemcpyHostToDevice, st));
     RIGC_HIP(hipMemcpyAsync(ctx->poses1.p, poses_t1, pose_bytes, hipMemcpyHostToDevice, st));
@@ -243,10 +247,14 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     a.total_evals = ctx->counters.as<unsigned long long>() + 2;
     a.toi_out = ctx->toi.as<double>();
     a.hit_out = ctx->hit.as<signed char>();
+    a.q_best = ctx->qbest.as<unsigned long long>();
+    a.q_visits = ctx->qvisits.as<unsigned int>();
+    a.q_flag = ctx->qflag.as<int>();
 
     const int block = 256;
     const size_t blocks_needed = (nq + block - 1) / block;
-    const int grid = (int)std::max<size_t>(1, std::min<size_t>(blocks_needed, (size_t)ctx->num_cus * 8));
+    (void)blocks_needed;
+    const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
     if (nq) launch_ccd(a, grid, block, st);
     RIGC_HIP(hipGetLastError());
@@ -364,6 +372,9 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     RIGC_HIP(hipEventRecord(ctx->ev1, s));
     const long long nq = ctx->bp.n_ee + ctx->bp.n_fv;
     RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
+    RIGC_HIP(ctx->qbest.ensure(sizeof(unsigned long long) * std::max<long long>(nq, 1)));
+    RIGC_HIP(ctx->qvisits.ensure(sizeof(unsigned int) * std::max<long long>(nq, 1)));
+    RIGC_HIP(ctx->qflag.ensure(sizeof(int) * std::max<long long>(nq, 1)));
     unsigned long long init[4] = { 0x7ff0000000000000ULL, 0, 0, 0 };
     RIGC_HIP(hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, s));
     CcdArgs a = ccd_args(ctx);
@@ -376,7 +387,10 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     a.use_global_best = 1;
     a.toi_out = nullptr;
     a.hit_out = nullptr;
-    const int grid = (int)std::max<long long>(1, std::min<long long>((nq + 255) / 256, (long long)ctx->num_cus * 8));
+    a.q_best = ctx->qbest.as<unsigned long long>();
+    a.q_visits = ctx->qvisits.as<unsigned int>();
+    a.q_flag = ctx->qflag.as<int>();
+    const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
     if (nq) launch_ccd(a, grid, 256, s);
     RIGC_HIP(hipGetLastError());
     RIGC_HIP(hipEventRecord(ctx->ev2, s));

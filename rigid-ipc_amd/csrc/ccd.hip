@@ -5,23 +5,28 @@
 // driving interval_root_finder (src/interval/interval_root_finder.cpp:97-203)
 // over the inclusion functions of src/ccd/rigid/rigid_trajectory_aabb.cpp:70-102.
 //
-// Execution model: a persistent grid; every lane owns one query at a time
-// and pulls the next query from a global counter when it finishes, so a
-// wavefront keeps all 64 lanes busy while any queries remain. The
-// bisection tree of a query is walked depth-first in the reference's order
-// (first half before second half) without a box stack: every box is a
-// dyadic sub-box of [0,1]^3 held as an integer (index, level) per
-// dimension, and the path from the root is held as three 192-bit masks
-// (split dimension t / alpha per level, and whether the second half of
-// that level is already taken). Popping the next box is a find-last-zero
-// plus popcounts. The t-only part of the inclusion function (interval
-// poses, interval Rodrigues matrices, world vertices) is cached per lane
-// and reused while only alpha/beta are being split.
-//
 // The root finder's answer is the least t.lower over all accepted boxes of
-// the bisection tree, which is independent of traversal order, so pruning
-// with the running minimum over all queries (earliest-TOI mode) returns
-// exactly the minimum of the per-query answers.
+// the bisection tree (boxes reached through ancestors whose enclosure holds
+// zero, and not skipped because their t.lower is already no better than an
+// accepted box). That minimum does not depend on the order in which the
+// tree is walked, so the tree of one query may be walked by many lanes at
+// once, and pruning with the running minimum over all queries
+// (earliest-TOI mode) returns exactly the minimum of the per-query answers.
+//
+// Execution model: a persistent grid of wavefronts. Every lane walks one
+// subtree depth-first without a box stack: a box is a dyadic sub-box of
+// [0,1]^3 held as an integer (index, level) per dimension, and the path
+// from the subtree root is three 192-bit masks (split dimension t / alpha
+// per level, and whether the second half of that level is taken); popping
+// the next box is a find-last-zero plus popcounts. The t-only part of the
+// inclusion function (interval poses, interval Rodrigues matrices, world
+// vertices) is cached per lane and reused while only alpha/beta split.
+// Idle lanes first take new queries from a global counter; once that pool
+// is empty, each busy lane with an untaken second half on its path hands
+// the shallowest one (its largest pending subtree) to an idle lane of the
+// same wavefront through cross-lane shuffles, so a deep query ends up
+// bisected by up to 64 lanes. Per-query minima live in global memory and
+// are combined with 64-bit atomicMin on the bit pattern (t >= 0).
 #include "rigc_interval.hpp"
 #include "rigc_internal.hpp"
 #include "rigc_rigid_iv.hpp"
@@ -117,16 +122,81 @@ RIGC_DI bool contains_zero(int kind, const Iv A0[3], const Iv DA[3], const Iv B0
     return z;
 }
 
-__global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
-    const unsigned long long nq = (unsigned long long)(a.n_ee + a.n_fv);
 
+// i-th set bit (0-based) of a 64-bit mask, i < popcount(mask).
+RIGC_DI int nth_set_bit(unsigned long long m, int i) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long low = m & ((w == 64 ? 0ULL : (1ULL << w)) - 1ULL);
+        const int c = __popcll(low);
+        if (i >= c) {
+            i -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
+RIGC_DI void load_query(const CcdArgs& a, long long qi, Query& q, double tol[3]) {
+    if (qi < a.n_ee) {
+        const int ea = a.ee[2 * qi], eb = a.ee[2 * qi + 1];
+        q.kind = 0;
+        q.bA = a.e_body[ea];
+        q.bB = a.e_body[eb];
+        q.g[0] = a.E[2 * ea];
+        q.g[1] = a.E[2 * ea + 1];
+        q.g[2] = a.E[2 * eb];
+        q.g[3] = a.E[2 * eb + 1];
+        tol[1] = 1e-4 / a.edge_len[ea];
+        tol[2] = 1e-4 / a.edge_len[eb];
+    } else {
+        const long long k = qi - a.n_ee;
+        const int f = a.fv[2 * k], v = a.fv[2 * k + 1];
+        q.kind = 1;
+        q.bA = a.v_body[v];
+        q.bB = a.f_body[f];
+        q.g[0] = v;
+        q.g[1] = a.F[3 * f];
+        q.g[2] = a.F[3 * f + 1];
+        q.g[3] = a.F[3 * f + 2];
+        tol[1] = 1e-4 / a.edge_len[a.FE[3 * f]];
+        tol[2] = 1e-4 / a.edge_len[a.FE[3 * f + 1]];
+    }
+}
+
+__global__ void ccd_init_kernel(CcdArgs a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_ee + a.n_fv) {
+        a.q_best[i] = 0x7ff0000000000000ULL;
+        a.q_visits[i] = 0;
+        a.q_flag[i] = 0;
+    }
+}
+
+__global__ void ccd_finish_kernel(CcdArgs a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n_ee + a.n_fv) {
+        const int f = a.q_flag[i];
+        const unsigned long long b = a.q_best[i];
+        if (a.toi_out) a.toi_out[i] = __longlong_as_double((long long)b);
+        if (a.hit_out) a.hit_out[i] = f == 1 ? -1 : (f == 2 ? -2 : (b < 0x7ff0000000000000ULL ? 1 : 0));
+    }
+}
+
+__global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
+    const long long nq = a.n_ee + a.n_fv;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
+
+    long long qi = -1;  // query of the subtree this lane walks, -1 = idle
+    int phase = 0;      // 1 = start-box check, 2 = bisection
     Query q;
-    unsigned long long qi = 0;
-    int phase = 0;  // 0 = fetch, 1 = start-box check, 2 = bisection
     double tol[3];
-    double best = 0;
-    int hit = 0;
-    long long visits = 0, evals = 0;
+    long long evals = 0;
 
     int depth = 0;
     Bits192 s, mt, ma;
@@ -138,63 +208,125 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
     int c_lev = -1;
     Iv A0[3], DA[3], B0[3], DB[3];
 
+    // Queries are spread over all wavefronts of the grid: a wavefront takes
+    // at most `quota` new queries at a time, and its remaining idle lanes
+    // help with the subtrees of the queries it holds.
+    const long long nwaves = (long long)gridDim.x * (blockDim.x / 64);
+    const int quota = (int)max(1LL, min(64LL, (nq + nwaves - 1) / nwaves));
+    bool pool_open = true;
     for (;;) {
-        if (phase == 0) {
-            qi = atomicAdd(a.counter, 1ULL);
-            if (qi >= nq) break;
-            if (qi < (unsigned long long)a.n_ee) {
-                const int ea = a.ee[2 * qi], eb = a.ee[2 * qi + 1];
-                q.kind = 0;
-                q.bA = a.e_body[ea];
-                q.bB = a.e_body[eb];
-                q.g[0] = a.E[2 * ea];
-                q.g[1] = a.E[2 * ea + 1];
-                q.g[2] = a.E[2 * eb];
-                q.g[3] = a.E[2 * eb + 1];
-                tol[1] = 1e-4 / a.edge_len[ea];
-                tol[2] = 1e-4 / a.edge_len[eb];
-            } else {
-                const unsigned long long k = qi - a.n_ee;
-                const int f = a.fv[2 * k], v = a.fv[2 * k + 1];
-                q.kind = 1;
-                q.bA = a.v_body[v];
-                q.bB = a.f_body[f];
-                q.g[0] = v;
-                q.g[1] = a.F[3 * f];
-                q.g[2] = a.F[3 * f + 1];
-                q.g[3] = a.F[3 * f + 2];
-                tol[1] = 1e-4 / a.edge_len[a.FE[3 * f]];
-                tol[2] = 1e-4 / a.edge_len[a.FE[3 * f + 1]];
+        // A. idle lanes take new queries while the pool lasts
+        if (pool_open) {
+            unsigned long long im = __ballot(qi < 0);
+            if (__popcll(im) > quota) {
+                // keep the `quota` lowest idle lanes
+                const int cut = nth_set_bit(im, quota);
+                im &= (1ULL << cut) - 1ULL;
             }
-            tol[0] = a.toi_tol;
-            best = __longlong_as_double(0x7ff0000000000000LL);
-            hit = 0;
-            visits = 0;
-            evals = 0;
-            cache_ok = false;
-            phase = 1;
+            if (im) {
+                const int leader = __ffsll((long long)im) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)__popcll(im));
+                base = __shfl(base, leader);
+                if (base + __popcll(im) >= (unsigned long long)nq) pool_open = false;
+                if ((im >> lane) & 1ULL) {
+                    const long long mine = (long long)(base + __popcll(im & lt_mask));
+                    if (mine < nq) {
+                        qi = mine;
+                        load_query(a, qi, q, tol);
+                        tol[0] = a.toi_tol;
+                        phase = 1;
+                        cache_ok = false;
+                    }
+                }
+            }
         }
+        // B. idle lanes receive pending subtrees of busy lanes
+        {
+            const unsigned long long im = __ballot(qi < 0);
+            if (im == ~0ULL && !pool_open) break;  // no lane of this wavefront has work left
+            if (im) {
+                const int first_open = (qi >= 0 && phase == 2) ? [&] {
+                    for (int wi = 0; wi < 3; wi++) {
+                        const unsigned long long m = ~s.w[wi] & range_mask(wi, 0, depth);
+                        if (m) return 64 * wi + __ffsll((long long)m) - 1;
+                    }
+                    return -1;
+                }()
+                                                                : -1;
+                const unsigned long long dm = __ballot(first_open >= 0);
+                const int k = min(__popcll(im), __popcll(dm));
+                // donor side: the second half of the shallowest open level
+                long long d_q = -1;
+                double d_tol0 = 0;
+                unsigned long long d_idx[3] = { 0, 0, 0 };
+                int d_lev = 0;
+                const int drank = __popcll(dm & lt_mask);
+                if (first_open >= 0 && drank < k) {
+                    const int L = first_open;
+                    const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
+                    const int cb = depth - 1 - L - ct - ca;
+                    d_idx[0] = idx[0] >> ct;
+                    d_idx[1] = idx[1] >> ca;
+                    d_idx[2] = idx[2] >> cb;
+                    const int l0 = lev[0] - ct, l1 = lev[1] - ca, l2 = lev[2] - cb;
+                    const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
+                    d_idx[d] += 1;
+                    bset(s, L, true);
+                    d_q = qi;
+                    d_tol0 = tol[0];
+                    d_lev = l0 | (l1 << 8) | (l2 << 16);
+                }
+                // receiver side
+                const int irank = __popcll(im & lt_mask);
+                const int src = (qi < 0 && irank < k) ? nth_set_bit(dm, irank) : lane;
+                const long long r_q = __shfl(d_q, src);
+                const double r_tol0 = __shfl(d_tol0, src);
+                const unsigned long long r0 = __shfl(d_idx[0], src);
+                const unsigned long long r1 = __shfl(d_idx[1], src);
+                const unsigned long long r2 = __shfl(d_idx[2], src);
+                const int r_lev = __shfl(d_lev, src);
+                if (qi < 0 && irank < k) {
+                    qi = r_q;
+                    load_query(a, qi, q, tol);
+                    tol[0] = r_tol0;
+                    idx[0] = r0; idx[1] = r1; idx[2] = r2;
+                    lev[0] = r_lev & 255; lev[1] = (r_lev >> 8) & 255; lev[2] = (r_lev >> 16) & 255;
+                    depth = 0;
+                    s.w[0] = s.w[1] = s.w[2] = 0;
+                    mt = s;
+                    ma = s;
+                    phase = 2;
+                    cache_ok = false;
+                }
+            }
+        }
+        if (qi < 0) continue;
 
-        bool finished = false;
+        // C. one box per lane
+        bool done = false;
         Iv t, u, v;
         if (phase == 1) {
             t = Iv{ 0.0, tol[0] };
             u = Iv{ 0.0, tol[1] };
             v = Iv{ 0.0, tol[2] };
-            cache_ok = false;
         } else {
-            double thr = best;
+            double thr = __longlong_as_double((long long)__hip_atomic_load(a.q_best + qi, __ATOMIC_RELAXED,
+                                                                            __HIP_MEMORY_SCOPE_AGENT));
             if (a.use_global_best) {
                 const unsigned long long gb =
                     __hip_atomic_load(a.global_best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 thr = fmin(thr, __longlong_as_double((long long)gb));
             }
-            // Pop boxes until one needs an evaluation.
-            for (;;) {
-                if (a.eval_budget > 0 && ++visits > a.eval_budget) {
-                    hit = -1;
-                    finished = true;
-                    break;
+            if (a.eval_budget > 0 && a.q_flag[qi] != 0) done = true;
+            while (!done) {
+                if (a.eval_budget > 0) {
+                    const unsigned int nv = atomicAdd(a.q_visits + qi, 1u) + 1u;
+                    if ((long long)nv > a.eval_budget) {
+                        atomicMax(a.q_flag + qi, 1);
+                        done = true;
+                        break;
+                    }
                 }
                 bool skip = dyad(idx[0], lev[0]) >= thr;
                 if (!skip && q.kind == 1) {
@@ -203,10 +335,9 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     skip = !overlap(uu + vv, Iv{ 0.0, 1.0 });
                 }
                 if (!skip) break;
-                // advance to the next box in depth-first order
                 const int L = last_zero_below(s, depth);
                 if (L < 0) {
-                    finished = true;
+                    done = true;
                     break;
                 }
                 const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
@@ -219,15 +350,15 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 bset(s, L, true);
                 depth = L + 1;
             }
-            if (!finished) {
+            if (!done) {
                 t = Iv{ dyad(idx[0], lev[0]), dyad(idx[0] + 1, lev[0]) };
                 u = Iv{ dyad(idx[1], lev[1]), dyad(idx[1] + 1, lev[1]) };
                 v = Iv{ dyad(idx[2], lev[2]), dyad(idx[2] + 1, lev[2]) };
             }
         }
 
-        if (!finished) {
-            if (!cache_ok || c_idx != idx[0] || c_lev != lev[0] || phase == 1) {
+        if (!done) {
+            if (phase == 1 || !cache_ok || c_idx != idx[0] || c_lev != lev[0]) {
                 t_part(a, q, t, A0, DA, B0, DB);
                 cache_ok = phase == 2;
                 c_idx = idx[0];
@@ -235,7 +366,6 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
             }
             const bool z = contains_zero(q.kind, A0, DA, B0, DB, u, v);
             evals++;
-
             if (phase == 1) {
                 if (z) tol[0] /= 1e2;
                 phase = 2;
@@ -248,7 +378,6 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 cache_ok = false;
                 continue;
             }
-
             bool leaf = !z;
             if (z) {
                 const double w0 = next_up(ldexp(1.0, -lev[0]));
@@ -258,10 +387,9 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 const bool all_sat = w0 <= tol[0] && w1 <= tol[1] && w2 <= tol[2];
                 const bool all_zero = w0 <= 1e-10 && w1 <= 1e-10 && w2 <= 1e-10;
                 if ((idx[0] > 0 || all_zero) && all_sat) {
-                    best = t.lo;
-                    hit = 1;
-                    if (a.use_global_best)
-                        atomicMin(a.global_best, (unsigned long long)__double_as_longlong(best));
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(t.lo);
+                    atomicMin(a.q_best + qi, bits);
+                    if (a.use_global_best) atomicMin(a.global_best, bits);
                     leaf = true;
                 } else {
                     int sp = -1;
@@ -269,8 +397,8 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     for (int i = 0; i < 3; i++)
                         if ((all_sat || w[i] > tol[i]) && (sp == -1 || w[i] * tol[sp] > w[sp] * tol[i])) sp = i;
                     if (depth >= MAX_DEPTH) {
-                        hit = -2;
-                        finished = true;
+                        atomicMax(a.q_flag + qi, 2);
+                        done = true;
                     } else {
                         bset(s, depth, false);
                         bset(mt, depth, sp == 0);
@@ -281,10 +409,10 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     }
                 }
             }
-            if (leaf && !finished) {
+            if (leaf && !done) {
                 const int L = last_zero_below(s, depth);
                 if (L < 0) {
-                    finished = true;
+                    done = true;
                 } else {
                     const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
                     const int cb = depth - 1 - L - ct - ca;
@@ -298,20 +426,20 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 }
             }
         }
-
-        if (finished) {
-            if (a.toi_out) a.toi_out[qi] = (hit == 1) ? best : __longlong_as_double(0x7ff0000000000000LL);
-            if (a.hit_out) a.hit_out[qi] = (signed char)hit;
-            atomicAdd(a.total_evals, (unsigned long long)evals);
-            phase = 0;
-        }
+        if (done) qi = -1;
     }
+    if (evals) atomicAdd(a.total_evals, (unsigned long long)evals);
 }
 
 }  // namespace
 
 void launch_ccd(const CcdArgs& a, int grid, int block, hipStream_t stream) {
+    const long long nq = a.n_ee + a.n_fv;
+    if (nq <= 0) return;
+    const unsigned eb = (unsigned)((nq + 255) / 256);
+    hipLaunchKernelGGL(ccd_init_kernel, dim3(eb), dim3(256), 0, stream, a);
     hipLaunchKernelGGL(ccd_kernel, dim3(grid), dim3(block), 0, stream, a);
+    hipLaunchKernelGGL(ccd_finish_kernel, dim3(eb), dim3(256), 0, stream, a);
 }
 
 }  // namespace rigc

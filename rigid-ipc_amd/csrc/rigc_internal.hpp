@@ -94,6 +94,10 @@ struct CcdArgs {
     unsigned long long* total_evals;
     double* toi_out;         // per query (optional)
     signed char* hit_out;    // per query (optional): 1 hit, 0 none, -1 budget, -2 depth
+    // per-query workspace, n_ee + n_fv entries each
+    unsigned long long* q_best;  // least accepted t.lower (bits), +inf if none
+    unsigned int* q_visits;      // box visits (only counted when eval_budget > 0)
+    int* q_flag;                 // 0 ok, 1 budget exceeded, 2 depth exceeded
 };
 
 void launch_ccd(const CcdArgs& a, int grid, int block, hipStream_t stream);

@@ -62,6 +62,8 @@ def lib():
         L.oracle_average_edge_length.restype = ctypes.c_double
         L.oracle_world_vertices.argtypes = [c_double_p, c_double_p]
         L.oracle_last_vertex_aabbs.argtypes = [c_double_p]
+        L.oracle_earliest_toi.argtypes = [c_double_p, c_double_p, ctypes.c_double, c_long_p]
+        L.oracle_earliest_toi.restype = ctypes.c_double
         for name in ("oracle_ee_aabb", "oracle_fv_aabb"):
             getattr(L, name).argtypes = [c_double_p, c_double_p, c_double_p]
     return _lib
@@ -140,3 +142,11 @@ def last_vertex_aabbs(nv):
     out = np.zeros((nv, 7))
     lib().oracle_last_vertex_aabbs(dptr(out))
     return out[:, 0] > 0, out[:, 1:4], out[:, 4:7]
+
+
+def earliest_toi(poses_t0, poses_t1, msd=0.0):
+    """(earliest toi, number of candidates) on the current scene."""
+    n = ctypes.c_long()
+    t = lib().oracle_earliest_toi(dptr(np.ascontiguousarray(poses_t0, np.float64)),
+                                  dptr(np.ascontiguousarray(poses_t1, np.float64)), float(msd), ctypes.byref(n))
+    return t, n.value

@@ -78,16 +78,20 @@ def _oracle_all(kinds, qs):
 @pytest.mark.parametrize("seed,rot", [(0, 0.2), (1, 1.0), (2, 3.0)])
 def test_random_rotating_queries_bit_exact(ctx, seed, rot):
     kinds, qs = scenes.random_queries(300, seed, rot_scale=rot)
+    # The oracle walks each tree depth-first with a visit budget; the GPU
+    # walks the same tree in parallel (a different visit order), so it gets
+    # a larger budget and must resolve, bit for bit, every query the oracle
+    # resolved.
     budget = 50000
     O.set_eval_budget(budget)
     try:
         h_ref, t_ref = _oracle_all(kinds, qs)
     finally:
         O.set_eval_budget(0)
-    _, toi, hit = run_gpu(ctx, kinds, qs, budget=budget, allow_unresolved=True)
+    _, toi, hit = run_gpu(ctx, kinds, qs, budget=20 * budget, allow_unresolved=True)
     print(f"seed {seed}: hits {np.sum(h_ref == 1)} unresolved {np.sum(h_ref < 0)}", flush=True)
-    np.testing.assert_array_equal(hit, h_ref)
     ok = h_ref >= 0
+    np.testing.assert_array_equal(hit[ok], h_ref[ok])
     np.testing.assert_array_equal(toi[ok].view(np.int64), t_ref[ok].view(np.int64))
 
 
