@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "oracle_ccd.hpp"
+#include "oracle_barrier.hpp"
 #include "oracle_scene.hpp"
 
 using namespace oracle;
@@ -246,6 +247,68 @@ void oracle_world_vertices(const double* poses, double* out) {
         world_vertices(s, b, poses + 6 * b, w);
         std::copy(w.begin(), w.end(), out + 3 * (size_t)s.bv[b]);
     }
+}
+
+// ---- constraint set, barrier, friction
+static std::vector<Constraint> g_constraints;
+static std::vector<FrictionConstraint> g_friction;
+
+void oracle_set_psd_projection(int on) { psd_projection_enabled() = on != 0; }
+
+// counts: vv, ev, ee, fv
+void oracle_constraint_set(const double* poses, double dhat, double dmin, long* counts) {
+    construct_constraint_set(scene(), poses, dhat, dmin, g_constraints);
+    for (int k = 0; k < 4; k++) counts[k] = 0;
+    for (const auto& c : g_constraints) counts[c.type]++;
+}
+// per constraint: type, a, b, multiplicity, body0, body1 (int64) and eps_x
+void oracle_get_constraints(long* rec, double* eps_x) {
+    for (size_t i = 0; i < g_constraints.size(); i++) {
+        const Constraint& c = g_constraints[i];
+        long* r = rec + 6 * i;
+        r[0] = c.type; r[1] = c.a; r[2] = c.b; r[3] = c.multiplicity; r[4] = c.body[0]; r[5] = c.body[1];
+        eps_x[i] = c.eps_x;
+    }
+}
+// Barrier term on the current constraint set: energy, grad (6 nb, zeroed
+// here) and per-constraint projected 12x12 (optional, n x 144).
+double oracle_barrier(const double* poses, double dhat, double dmin, double* grad, double* local_hess) {
+    const Scene& s = scene();
+    for (int i = 0; i < 6 * s.nb; i++) grad[i] = 0;
+    std::vector<std::array<double, 144>> lh;
+    const double e = barrier_term(s, poses, g_constraints, dhat, dmin, grad, local_hess ? &lh : nullptr);
+    if (local_hess)
+        for (size_t i = 0; i < lh.size(); i++) std::copy(lh[i].begin(), lh[i].end(), local_hess + 144 * i);
+    return e;
+}
+double oracle_min_distance(const double* poses) {
+    return minimum_distance(scene(), poses, g_constraints);
+}
+long oracle_friction_update(const double* poses, double dhat, double dmin, double kappa, double mu) {
+    construct_friction_constraint_set(scene(), poses, g_constraints, dhat, dmin, kappa, mu, g_friction);
+    return (long)g_friction.size();
+}
+// per friction constraint: normal force, coords (2), basis (3x2 row-major)
+void oracle_get_friction(double* rec) {
+    for (size_t i = 0; i < g_friction.size(); i++) {
+        const FrictionConstraint& f = g_friction[i];
+        double* r = rec + 9 * i;
+        r[0] = f.normal_force; r[1] = f.coords[0]; r[2] = f.coords[1];
+        for (int k = 0; k < 3; k++) { r[3 + 2 * k] = f.basis[k][0]; r[4 + 2 * k] = f.basis[k][1]; }
+    }
+}
+double oracle_friction(const double* poses, const double* poses_t0, double epsv_times_h, double* grad,
+                       double* local_hess) {
+    const Scene& s = scene();
+    for (int i = 0; i < 6 * s.nb; i++) grad[i] = 0;
+    std::vector<std::array<double, 144>> lh;
+    const double e = friction_term(s, poses, poses_t0, g_friction, epsv_times_h, grad, local_hess ? &lh : nullptr);
+    if (local_hess)
+        for (size_t i = 0; i < lh.size(); i++) std::copy(lh[i].begin(), lh[i].end(), local_hess + 144 * i);
+    return e;
+}
+void oracle_vertex_derivatives(const double* pose, int vertex, double* w, double* jac, double* hess) {
+    vertex_derivatives(scene(), pose, vertex, w, jac, hess);
 }
 
 }  // extern "C"

@@ -150,3 +150,93 @@ def earliest_toi(poses_t0, poses_t1, msd=0.0):
     t = lib().oracle_earliest_toi(dptr(np.ascontiguousarray(poses_t0, np.float64)),
                                   dptr(np.ascontiguousarray(poses_t1, np.float64)), float(msd), ctypes.byref(n))
     return t, n.value
+
+
+def _declare_barrier():
+    L = lib()
+    if getattr(L, "_barrier_declared", False):
+        return L
+    L.oracle_set_psd_projection.argtypes = [ctypes.c_int]
+    L.oracle_constraint_set.argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_long_p]
+    L.oracle_get_constraints.argtypes = [c_long_p, c_double_p]
+    L.oracle_barrier.argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_double_p]
+    L.oracle_barrier.restype = ctypes.c_double
+    L.oracle_min_distance.argtypes = [c_double_p]
+    L.oracle_min_distance.restype = ctypes.c_double
+    L.oracle_friction_update.argtypes = [c_double_p] + [ctypes.c_double] * 4
+    L.oracle_friction_update.restype = ctypes.c_long
+    L.oracle_get_friction.argtypes = [c_double_p]
+    L.oracle_friction.argtypes = [c_double_p, c_double_p, ctypes.c_double, c_double_p, c_double_p]
+    L.oracle_friction.restype = ctypes.c_double
+    L.oracle_vertex_derivatives.argtypes = [c_double_p, ctypes.c_int, c_double_p, c_double_p, c_double_p]
+    L._barrier_declared = True
+    return L
+
+
+def set_psd_projection(on):
+    _declare_barrier().oracle_set_psd_projection(int(bool(on)))
+
+
+def constraint_set(poses, dhat, dmin=0.0):
+    """Build the constraint set at `poses`; returns (records (n,6) int64:
+    type, a, b, multiplicity, body0, body1; eps_x (n,))."""
+    L = _declare_barrier()
+    counts = np.zeros(4, np.int64)
+    L.oracle_constraint_set(dptr(np.ascontiguousarray(poses, np.float64)), float(dhat), float(dmin),
+                            counts.ctypes.data_as(c_long_p))
+    n = int(counts.sum())
+    rec = np.zeros((n, 6), np.int64)
+    eps = np.zeros(n)
+    L.oracle_get_constraints(rec.ctypes.data_as(c_long_p), dptr(eps))
+    return rec, eps
+
+
+def barrier(poses, dhat, dmin=0.0, nb=None, want_hess=False, n_constraints=0):
+    L = _declare_barrier()
+    p = np.ascontiguousarray(poses, np.float64)
+    g = np.zeros(p.size)
+    h = np.zeros((max(n_constraints, 1), 144)) if want_hess else None
+    e = L.oracle_barrier(dptr(p), float(dhat), float(dmin), dptr(g), dptr(h) if want_hess else None)
+    return e, g, (h[:n_constraints] if want_hess else None)
+
+
+def min_distance(poses):
+    return _declare_barrier().oracle_min_distance(dptr(np.ascontiguousarray(poses, np.float64)))
+
+
+def friction_update(poses, dhat, dmin, kappa, mu):
+    L = _declare_barrier()
+    n = L.oracle_friction_update(dptr(np.ascontiguousarray(poses, np.float64)), float(dhat), float(dmin),
+                                 float(kappa), float(mu))
+    rec = np.zeros((max(n, 1), 9))
+    L.oracle_get_friction(dptr(rec))
+    return rec[:n]
+
+
+def friction(poses, poses_t0, epsv_times_h, want_hess=False, n_constraints=0):
+    L = _declare_barrier()
+    p = np.ascontiguousarray(poses, np.float64)
+    g = np.zeros(p.size)
+    h = np.zeros((max(n_constraints, 1), 144)) if want_hess else None
+    e = L.oracle_friction(dptr(p), dptr(np.ascontiguousarray(poses_t0, np.float64)), float(epsv_times_h), dptr(g),
+                          dptr(h) if want_hess else None)
+    return e, g, (h[:n_constraints] if want_hess else None)
+
+
+def vertex_derivatives(pose, vertex):
+    L = _declare_barrier()
+    w, J, H = np.zeros(3), np.zeros(18), np.zeros(27)
+    L.oracle_vertex_derivatives(dptr(np.ascontiguousarray(pose, np.float64)), int(vertex), dptr(w), dptr(J), dptr(H))
+    return w, J.reshape(3, 6), H.reshape(3, 3, 3)
+
+
+def assemble_hessian(rec, local_hess, nb):
+    """Sum per-constraint 12x12 blocks into a dense (6nb x 6nb) matrix."""
+    Hd = np.zeros((6 * nb, 6 * nb))
+    for r, h in zip(rec, local_hess):
+        h = h.reshape(12, 12)
+        bodies = (r[4], r[5])
+        for i in range(2):
+            for j in range(2):
+                Hd[6 * bodies[i]:6 * bodies[i] + 6, 6 * bodies[j]:6 * bodies[j] + 6] += h[6 * i:6 * i + 6, 6 * j:6 * j + 6]
+    return Hd
