@@ -113,6 +113,80 @@ rigc_status rigc_broad_phase(rigc_ctx* ctx, const double* poses_t0, const double
 rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
                                       double minimum_separation_distance, double* earliest_toi);
 
+/* ---------------------------------------------------------------- assembly
+ * Derivative flags of the potential entry points. */
+#define RIGC_GRADIENT 1
+#define RIGC_HESSIAN 2            /* implies RIGC_GRADIENT */
+#define RIGC_BUILD_CONSTRAINTS 4  /* rebuild the constraint set at `poses` first */
+
+/* Block-sparse symmetric Hessian in the 6-dof body layout. diag holds
+ * every body's (i,i) block; pairs / off hold the coupled body pairs
+ * (i < j) with block (i,j); block (j,i) is its transpose. Row-major 6x6.
+ * count is written with the number of off-diagonal blocks; when it
+ * exceeds capacity, pairs / off are left untouched and the call returns
+ * RIGC_ERR_CAPACITY (energy, grad and diag are still written). */
+typedef struct rigc_block_hessian {
+    double* diag;     /* [num_bodies][36] */
+    int32_t* pairs;   /* [capacity][2] */
+    double* off;      /* [capacity][36] */
+    int64_t capacity;
+    int64_t count;
+} rigc_block_hessian;
+
+/* Collision constraint set at `poses`: static hash-grid candidates with
+ * inflation (dhat + dmin) / 2, kept if the squared distance is below
+ * (dhat + dmin)^2, classified by closest-feature pair, vertex-vertex and
+ * edge-vertex duplicates merged into a multiplicity, nearly parallel
+ * edge-edge pairs mollified (eps_x from the current positions).
+ * Replaces DistanceBarrierConstraint::construct_constraint_set
+ * (src/opt/distance_barrier_constraint.cpp:267-306). The set stays on the
+ * device for the calls below; counts[4] receives VV, EV, EE, FV sizes. */
+rigc_status rigc_construct_constraint_set(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
+                                          int64_t* counts);
+
+/* Copy of the current constraint set, VV then EV then EE then FV, each
+ * sorted by its key: records [n][6] = type (0 VV, 1 EV, 2 EE, 3 FV), a, b
+ * (VV: v0 < v1; EV: edge, vertex; EE: edge0, edge1; FV: face, vertex),
+ * multiplicity, body0, body1; eps_x [n] (-1 unless EE). */
+rigc_status rigc_get_constraint_set(rigc_ctx* ctx, int64_t capacity, int32_t* records, double* eps_x, int64_t* n);
+
+/* Barrier potential sum_c m_c [mollifier] b(d_c - dmin^2, 2 dmin dhat + dhat^2)
+ * over the current constraint set, its gradient (grad [num_bodies][6]) and
+ * its Hessian with each constraint's 12x12 block projected to PSD.
+ * Replaces DistanceBarrierRBProblem::compute_barrier_term
+ * (src/problems/distance_barrier_rb_problem.cpp:1299-1399, with
+ * apply_chain_rule :1182-1248). grad / hess may be NULL when the matching
+ * flag is off. */
+rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double dhat, double dmin, int32_t flags,
+                                      double* energy, double* grad, rigc_block_hessian* hess);
+
+/* sqrt of the minimum squared distance over the constraint set built at
+ * `poses` (+inf when the set is empty). Replaces
+ * DistanceBarrierConstraint::compute_minimum_distance
+ * (src/opt/distance_barrier_constraint.cpp:308-323). */
+rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
+                                          double* min_distance);
+
+/* Lagged friction set: the constraint set is rebuilt at `poses`, then each
+ * constraint's closest point, tangent basis and normal force
+ * -kappa b'(d) 2 sqrt(d) are frozen (ipc::construct_friction_constraint_set
+ * as called by DistanceBarrierRBProblem::update_friction_constraints,
+ * src/problems/distance_barrier_rb_problem.cpp:168-188). */
+rigc_status rigc_update_friction_constraints(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
+                                             double barrier_stiffness, double mu, int64_t* count);
+
+/* Copy of the friction set: records [n][9] = normal force, closest-point
+ * coordinates (2), tangent basis (3x2 row-major), in constraint-set order. */
+rigc_status rigc_get_friction_constraints(rigc_ctx* ctx, int64_t capacity, double* records, int64_t* n);
+
+/* Friction potential sum_c m_c mu lambda_c f0(|T_c^T (V(poses) - V(poses_t0))|;
+ * epsv_times_h) with derivatives as rigc_compute_barrier_term. Replaces
+ * DistanceBarrierRBProblem::compute_friction_term
+ * (src/problems/distance_barrier_rb_problem.cpp:1410-1572). */
+rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const double* poses_t0,
+                                       double epsv_times_h, int32_t flags, double* energy, double* grad,
+                                       rigc_block_hessian* hess);
+
 #ifdef __cplusplus
 }
 #endif

@@ -344,6 +344,10 @@ D3 sinc_normx(const D3 x[3]) {
 }  // namespace
 
 void vertex_derivatives(const Scene& s, const double* pose, int vertex, double w[3], double jac[18], double hess[27]) {
+    point_derivatives(pose, &s.V[3 * vertex], w, jac, hess);
+}
+
+void point_derivatives(const double* pose, const double* v, double w[3], double jac[18], double hess[27]) {
     D3 r[3], rh[3];
     for (int k = 0; k < 3; k++) {
         r[k] = D3::var(pose[3 + k], k);
@@ -360,7 +364,6 @@ void vertex_derivatives(const Scene& s, const double* pose, int vertex, double w
             R[i][j] = sa * K[i][j] + c2 * k2;
         }
     for (int i = 0; i < 3; i++) R[i][i] = R[i][i] + D3(1.0);
-    const double* v = &s.V[3 * vertex];
     for (int j = 0; j < 3; j++) {
         const D3 wj = (v[0] * R[j][0] + v[1] * R[j][1]) + v[2] * R[j][2];
         w[j] = wj.v + pose[j];
@@ -616,6 +619,26 @@ double barrier_term(const Scene& s, const double* poses, const std::vector<Const
             for (int k = 0; k < 12; k++) grad[6 * C[ci].body[k / 6] + k % 6] += lgrad[ci][k];
     }
     return e;
+}
+
+// One constraint's potential in its vertices' coordinates (X: 4 slots x 3),
+// unprojected: used to check closed-form derivatives.
+double constraint_local(int type, const double* X, double eps_x, double dhat, double dmin, long multiplicity,
+                        double* g, double* H) {
+    Constraint c;
+    c.type = type;
+    c.eps_x = eps_x;
+    c.multiplicity = multiplicity;
+    const D12 d = constraint_distance(c, X);
+    const double dh = 2 * dmin * dhat + dhat * dhat, dm2 = dmin * dmin;
+    D12 B = chain(d, barrier(d.v - dm2, dh), barrier_gradient(d.v - dm2, dh), barrier_hessian(d.v - dm2, dh));
+    if (c.type == EE && c.eps_x >= 0) B = mollifier(X, c.eps_x) * B;
+    B = (double)c.multiplicity * B;
+    for (int i = 0; i < 12; i++) {
+        g[i] = B.g[i];
+        for (int j = 0; j < 12; j++) H[12 * i + j] = B.h[i][j];
+    }
+    return B.v;
 }
 
 double minimum_distance(const Scene& s, const double* poses, const std::vector<Constraint>& C) {

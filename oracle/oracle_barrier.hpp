@@ -63,6 +63,11 @@ double barrier_term(const Scene& s, const double* poses, const std::vector<Const
 
 double minimum_distance(const Scene& s, const double* poses, const std::vector<Constraint>& C);
 
+// One constraint's potential and its unprojected vertex-space derivatives
+// (X: 4 slots x 3; g: 12; H: 12x12 row-major).
+double constraint_local(int type, const double* X, double eps_x, double dhat, double dmin, long multiplicity,
+                        double* g, double* H);
+
 void construct_friction_constraint_set(const Scene& s, const double* poses, const std::vector<Constraint>& C,
                                        double dhat, double dmin, double barrier_stiffness, double mu,
                                        std::vector<FrictionConstraint>& out);
@@ -75,6 +80,8 @@ double friction_term(const Scene& s, const double* poses, const double* poses_t0
 // jac (3x6 row-major), hess (3 x 3x3 rotational block, [j][k][l]).
 void vertex_derivatives(const Scene& s, const double* pose, int vertex, double w[3], double jac[18],
                         double hess[27]);
+// The same for a body-local point v.
+void point_derivatives(const double* pose, const double* v, double w[3], double jac[18], double hess[27]);
 
 // project_to_psd on a symmetric n x n matrix (row-major, in place).
 void project_to_psd(double* A, int n);

@@ -307,6 +307,13 @@ double oracle_friction(const double* poses, const double* poses_t0, double epsv_
         for (size_t i = 0; i < lh.size(); i++) std::copy(lh[i].begin(), lh[i].end(), local_hess + 144 * i);
     return e;
 }
+double oracle_constraint_local(int type, const double* X, double eps_x, double dhat, double dmin, long multiplicity,
+                               double* g, double* H) {
+    return constraint_local(type, X, eps_x, dhat, dmin, multiplicity, g, H);
+}
+void oracle_point_derivatives(const double* pose, const double* v, double* w, double* jac, double* hess) {
+    point_derivatives(pose, v, w, jac, hess);
+}
 void oracle_vertex_derivatives(const double* pose, int vertex, double* w, double* jac, double* hess) {
     vertex_derivatives(scene(), pose, vertex, w, jac, hess);
 }

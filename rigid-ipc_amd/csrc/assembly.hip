@@ -1,0 +1,949 @@
+// Constraint set, barrier and lagged-friction assembly on the device.
+//
+//   construct_constraint_set   src/opt/distance_barrier_constraint.cpp:267-306
+//                              (-> ipc::construct_constraint_set)
+//   compute_barrier_term       src/problems/distance_barrier_rb_problem.cpp:1299-1399
+//   apply_chain_rule           :1182-1248 (world_vertices_diff, project_to_psd)
+//   local_*_to_global          :1143-1179, merge_derivative_storage :1260-1297
+//   update_friction_constraints :168-188, compute_friction_term :1410-1572
+//   compute_minimum_distance   src/opt/distance_barrier_constraint.cpp:308-323
+//
+// Pipeline (one HIP stream):
+//   body_frame_kernel      per body: R(r), dR/dr, d2R/dr2 (closed form)
+//   world_vertex_kernel    per vertex: R x + p (round-to-nearest, as the host)
+//   static broad phase     (broadphase.hip) with inflation (dhat + dmin) / 2
+//   classify_kernel        per candidate: closest-feature type, activity
+//                          d < (dhat + dmin)^2, edge-edge mollifier test;
+//                          appends a sort key to the VV / EV / EE / FV list
+//   radix sort + run-length encoding: VV / EV duplicates -> multiplicity
+//   record_kernel          ConRec list, order VV, EV, EE, FV
+//   local_eg_kernel        thread per constraint: energy (+ 12-gradient)
+//   local_hess_kernel      one wavefront per constraint: vertex-space
+//                          Hessian in LDS, chain rule to the two bodies'
+//                          12 dofs, parallel-order Jacobi PSD projection
+//   body / pair reduction  sorted (body, constraint) keys -> per-body
+//                          gradient and diagonal 6x6 blocks, sorted body
+//                          pairs -> off-diagonal 6x6 blocks; fixed order,
+//                          so results are deterministic.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "rigc_barrier.hpp"
+#include "rigc_internal.hpp"
+
+namespace rigc {
+namespace {
+
+#define AS_CHECK(x)                           \
+    do {                                      \
+        hipError_t e_ = (x);                  \
+        if (e_ != hipSuccess) return e_;      \
+    } while (0)
+
+constexpr int THREADS = 256;
+
+int bits_for(unsigned long long maxval) {
+    int b = 0;
+    while (b < 64 && (maxval >> b) != 0ULL) b++;
+    return b;
+}
+
+// --------------------------------------------------------------- frames
+__global__ __launch_bounds__(THREADS) void body_frame_kernel(int nb, const double* poses, double* R, BodyFrame* F) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    rotation_d(poses + 6 * b + 3, R + 9 * b);
+    if (F) body_frame(poses + 6 * b + 3, F[b]);
+}
+
+__global__ __launch_bounds__(THREADS) void world_vertex_kernel(int nv, const double* V, const int* v_body,
+                                                              const double* poses, const double* R, double* W) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const int b = v_body[i];
+    world_vertex_d(R + 9 * b, poses + 6 * b, V + 3 * i, W + 3 * i);
+}
+
+// --------------------------------------------------------------- constraint set
+struct ClassifyArgs {
+    const int* ee;
+    const int* fv;
+    long long n_ee, n_fv;
+    const int* E;
+    const int* F;
+    const int* FE;
+    const double* W;
+    int nv;
+    double offset_sqr;
+    unsigned long long* keys[4];
+    unsigned int* count;
+};
+
+__device__ inline void load3(const double* W, int v, double* x) {
+    x[0] = W[3 * v];
+    x[1] = W[3 * v + 1];
+    x[2] = W[3 * v + 2];
+}
+
+// ipc::construct_constraint_set per candidate: the closest-feature pair
+// decides the constraint kind; a nearly parallel edge pair stays an
+// edge-edge constraint (mollified).
+__global__ __launch_bounds__(THREADS) void classify_kernel(ClassifyArgs a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_ee + a.n_fv) return;
+    double X[12];
+    int cls;
+    unsigned long long key;
+    const unsigned long long NV = (unsigned long long)a.nv;
+    if (i < a.n_ee) {
+        const int ei = a.ee[2 * i], ej = a.ee[2 * i + 1];
+        const int i0 = a.E[2 * ei], i1 = a.E[2 * ei + 1], j0 = a.E[2 * ej], j1 = a.E[2 * ej + 1];
+        load3(a.W, i0, X);
+        load3(a.W, i1, X + 3);
+        load3(a.W, j0, X + 6);
+        load3(a.W, j1, X + 9);
+        int t = edge_edge_type(X, X + 3, X + 6, X + 9);
+        DistForm f;
+        constraint_form(CON_EE, X, f);
+        if (!(f.d < a.offset_sqr)) return;
+        double ua[3], ub[3], n[3];
+        v3_sub(X + 3, X, ua);
+        v3_sub(X + 9, X + 6, ub);
+        v3_cross(ua, ub, n);
+        const double eps_x = 1e-3 * v3_dot(ua, ua) * v3_dot(ub, ub);
+        if (v3_dot(n, n) < eps_x) t = EE_AB;
+        auto vv = [&](int p, int q) {
+            cls = CON_VV;
+            key = (unsigned long long)min(p, q) * NV + (unsigned long long)max(p, q);
+        };
+        auto ev = [&](int e, int v) {
+            cls = CON_EV;
+            key = (unsigned long long)e * NV + (unsigned long long)v;
+        };
+        switch (t) {
+        case EE_A0B0: vv(i0, j0); break;
+        case EE_A0B1: vv(i0, j1); break;
+        case EE_A1B0: vv(i1, j0); break;
+        case EE_A1B1: vv(i1, j1); break;
+        case EE_AB0: ev(ei, j0); break;
+        case EE_AB1: ev(ei, j1); break;
+        case EE_A0B: ev(ej, i0); break;
+        case EE_A1B: ev(ej, i1); break;
+        default: cls = CON_EE; key = (unsigned long long)i; break;
+        }
+    } else {
+        const long long j = i - a.n_ee;
+        const int fc = a.fv[2 * j], v = a.fv[2 * j + 1];
+        const int f0 = a.F[3 * fc], f1 = a.F[3 * fc + 1], f2 = a.F[3 * fc + 2];
+        load3(a.W, v, X);
+        load3(a.W, f0, X + 3);
+        load3(a.W, f1, X + 6);
+        load3(a.W, f2, X + 9);
+        DistForm f;
+        constraint_form(CON_FV, X, f);
+        if (!(f.d < a.offset_sqr)) return;
+        auto vv = [&](int p, int q) {
+            cls = CON_VV;
+            key = (unsigned long long)min(p, q) * NV + (unsigned long long)max(p, q);
+        };
+        auto ev = [&](int e, int w) {
+            cls = CON_EV;
+            key = (unsigned long long)e * NV + (unsigned long long)w;
+        };
+        switch (point_triangle_type(X, X + 3, X + 6, X + 9)) {
+        case PT_T0: vv(v, f0); break;
+        case PT_T1: vv(v, f1); break;
+        case PT_T2: vv(v, f2); break;
+        case PT_E0: ev(a.FE[3 * fc], v); break;
+        case PT_E1: ev(a.FE[3 * fc + 1], v); break;
+        case PT_E2: ev(a.FE[3 * fc + 2], v); break;
+        default: cls = CON_FV; key = (unsigned long long)j; break;
+        }
+    }
+    const unsigned int slot = atomicAdd(&a.count[cls], 1u);
+    a.keys[cls][slot] = key;
+}
+
+__global__ __launch_bounds__(THREADS) void head_flags_kernel(const unsigned long long* keys, long long n,
+                                                            unsigned int* flags) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// pos = exclusive scan of the head flags: start[pos[i]] = i at heads.
+__global__ __launch_bounds__(THREADS) void head_starts_kernel(const unsigned int* flags, const unsigned int* pos,
+                                                             long long n, unsigned int* start) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flags[i]) start[pos[i]] = (unsigned int)i;
+}
+
+struct RecordArgs {
+    int cls;
+    const unsigned long long* keys;  // sorted keys of this class
+    const unsigned int* start;       // VV / EV: run starts (nullptr: one record per key)
+    long long n_keys;                // keys of this class
+    long long n_out;                 // records to write
+    ConRec* out;
+    const int* ee;
+    const int* fv;
+    const int* E;
+    const int* F;
+    const int* v_body;
+    const double* W;
+    int nv;
+};
+
+__global__ __launch_bounds__(THREADS) void record_kernel(RecordArgs a) {
+    const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= a.n_out) return;
+    ConRec r;
+    r.eps_x = -1.0;
+    r.v[0] = r.v[1] = r.v[2] = r.v[3] = -1;
+    long long k = u;
+    int mult = 1;
+    if (a.start) {
+        k = a.start[u];
+        const long long end = (u + 1 < a.n_out) ? (long long)a.start[u + 1] : a.n_keys;
+        mult = (int)(end - k);
+    }
+    const unsigned long long key = a.keys[k];
+    const unsigned long long NV = (unsigned long long)a.nv;
+    int nvx = 4;
+    switch (a.cls) {
+    case CON_VV:
+        r.a = (int)(key / NV);
+        r.b = (int)(key % NV);
+        r.v[0] = r.a;
+        r.v[1] = r.b;
+        nvx = 2;
+        break;
+    case CON_EV:
+        r.a = (int)(key / NV);
+        r.b = (int)(key % NV);
+        r.v[0] = r.b;
+        r.v[1] = a.E[2 * r.a];
+        r.v[2] = a.E[2 * r.a + 1];
+        nvx = 3;
+        break;
+    case CON_EE: {
+        r.a = a.ee[2 * key];
+        r.b = a.ee[2 * key + 1];
+        r.v[0] = a.E[2 * r.a];
+        r.v[1] = a.E[2 * r.a + 1];
+        r.v[2] = a.E[2 * r.b];
+        r.v[3] = a.E[2 * r.b + 1];
+        // eps_x from V_rest = V (distance_barrier_constraint.cpp:297-299)
+        double X[12], ua[3], ub[3];
+        for (int s = 0; s < 4; s++) load3(a.W, r.v[s], X + 3 * s);
+        v3_sub(X + 3, X, ua);
+        v3_sub(X + 9, X + 6, ub);
+        r.eps_x = 1e-3 * v3_dot(ua, ua) * v3_dot(ub, ub);
+        break;
+    }
+    default:
+        r.a = a.fv[2 * key];
+        r.b = a.fv[2 * key + 1];
+        r.v[0] = r.b;
+        r.v[1] = a.F[3 * r.a];
+        r.v[2] = a.F[3 * r.a + 1];
+        r.v[3] = a.F[3 * r.a + 2];
+        break;
+    }
+    r.type = a.cls;
+    r.mult = mult;
+    r.body[0] = a.v_body[r.v[0]];
+    r.body[1] = a.v_body[r.v[nvx - 1]];
+    a.out[u] = r;
+}
+
+// --------------------------------------------------------------- local terms
+__device__ inline int con_nv(int type) { return type == CON_VV ? 2 : (type == CON_EV ? 3 : 4); }
+// RigidBody*Constraint::vertex_local_body_ids
+// (src/problems/rigid_body_collision_constraint.hpp:12-240)
+__device__ inline int slot_side(int type, int s) { return type == CON_EE ? (s >> 1) : (s == 0 ? 0 : 1); }
+
+struct LocalArgs {
+    const ConRec* cons;
+    const FricRec* fric;
+    long long n;
+    const double* V;
+    const int* v_body;
+    const double* W;   // world vertices at the evaluation poses
+    const double* W0;  // friction: at the start of the step
+    const BodyFrame* frames;
+    double dhat, dmin, epsv_h;
+    int want_grad;
+    double* energy;    // [n]
+    double* lgrad;     // [n][12]
+    double* lhess;     // [n][144]
+};
+
+template <bool FRIC>
+__device__ inline void local_setup(const LocalArgs& a, long long c, ConRec& rec, BarrierLocal& B,
+                                   FrictionLocal& Fl) {
+    double X[12];
+    if (FRIC) {
+        const FricRec& f = a.fric[c];
+        rec = f.c;
+        const int nv = con_nv(rec.type);
+        for (int s = 0; s < 4; s++)
+            for (int k = 0; k < 3; k++)
+                X[3 * s + k] = s < nv ? a.W[3 * rec.v[s] + k] - a.W0[3 * rec.v[s] + k] : 0.0;
+        double basis[3][2];
+        for (int k = 0; k < 3; k++) {
+            basis[k][0] = f.basis[2 * k];
+            basis[k][1] = f.basis[2 * k + 1];
+        }
+        friction_local_setup(Fl, rec.type, nv, f.coords, basis, f.normal_force, f.mu, rec.mult, X, a.epsv_h);
+    } else {
+        rec = a.cons[c];
+        const int nv = con_nv(rec.type);
+        for (int s = 0; s < 4; s++)
+            for (int k = 0; k < 3; k++) X[3 * s + k] = s < nv ? a.W[3 * rec.v[s] + k] : 0.0;
+        barrier_local_setup(B, rec.type, X, rec.eps_x, rec.mult, a.dhat, a.dmin);
+    }
+}
+
+// Vertex Jacobian column c of slot vertex: (dR/dr_c) x, component k.
+__device__ inline double jac_rot(const BodyFrame& F, const double* x, int c, int k) {
+    return (F.dR[c][3 * k] * x[0] + F.dR[c][3 * k + 1] * x[1]) + F.dR[c][3 * k + 2] * x[2];
+}
+
+// Thread per constraint: energy and (optionally) the 12-gradient in the
+// two bodies' dofs.
+template <bool FRIC>
+__global__ __launch_bounds__(THREADS) void local_eg_kernel(LocalArgs a) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    ConRec rec;
+    BarrierLocal B;
+    FrictionLocal Fl;
+    local_setup<FRIC>(a, c, rec, B, Fl);
+    a.energy[c] = FRIC ? Fl.energy : barrier_local_energy(B);
+    if (!a.want_grad) return;
+    const int nv = con_nv(rec.type);
+    double lg[12];
+    for (int r = 0; r < 12; r++) lg[r] = 0.0;
+    for (int s = 0; s < nv; s++) {
+        double g[3];
+        for (int k = 0; k < 3; k++) g[k] = FRIC ? friction_local_grad(Fl, 3 * s + k) : barrier_local_grad(B, s, k);
+        const int side = slot_side(rec.type, s);
+        const int v = rec.v[s];
+        const BodyFrame& F = a.frames[a.v_body[v]];
+        const double* x = a.V + 3 * v;
+        double t[6];
+        for (int d = 0; d < 3; d++) t[d] = g[d];
+        for (int d = 0; d < 3; d++) {
+            double sum = 0.0;
+            for (int k = 0; k < 3; k++) sum += jac_rot(F, x, d, k) * g[k];
+            t[3 + d] = sum;
+        }
+        for (int d = 0; d < 6; d++) lg[6 * side + d] += t[d];
+    }
+    for (int r = 0; r < 12; r++) a.lgrad[12 * c + r] = lg[r];
+}
+
+// Round-robin pairing of 12 indices: 11 rounds of 6 disjoint pairs that
+// together cover each of the 66 pairs once.
+__constant__ signed char JACOBI_PAIRS[11][6][2] = {
+    { { 0, 11 }, { 1, 10 }, { 2, 9 }, { 3, 8 }, { 4, 7 }, { 5, 6 } },
+    { { 0, 10 }, { 11, 9 }, { 1, 8 }, { 2, 7 }, { 3, 6 }, { 4, 5 } },
+    { { 0, 9 }, { 10, 8 }, { 11, 7 }, { 1, 6 }, { 2, 5 }, { 3, 4 } },
+    { { 0, 8 }, { 9, 7 }, { 10, 6 }, { 11, 5 }, { 1, 4 }, { 2, 3 } },
+    { { 0, 7 }, { 8, 6 }, { 9, 5 }, { 10, 4 }, { 11, 3 }, { 1, 2 } },
+    { { 0, 6 }, { 7, 5 }, { 8, 4 }, { 9, 3 }, { 10, 2 }, { 11, 1 } },
+    { { 0, 5 }, { 6, 4 }, { 7, 3 }, { 8, 2 }, { 9, 1 }, { 10, 11 } },
+    { { 0, 4 }, { 5, 3 }, { 6, 2 }, { 7, 1 }, { 8, 11 }, { 9, 10 } },
+    { { 0, 3 }, { 4, 2 }, { 5, 1 }, { 6, 11 }, { 7, 10 }, { 8, 9 } },
+    { { 0, 2 }, { 3, 1 }, { 4, 11 }, { 5, 10 }, { 6, 9 }, { 7, 8 } },
+    { { 0, 1 }, { 2, 11 }, { 3, 10 }, { 4, 9 }, { 5, 8 }, { 6, 7 } },
+};
+
+__device__ inline double wave_sum_uniform(double x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return __shfl(x, 0);  // lane 0's value, so every lane takes the same branch
+}
+
+// One wavefront (the whole block) per constraint.
+template <bool FRIC>
+__global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
+    __shared__ double sHv[144];  // vertex-space Hessian
+    __shared__ double sHJ[144];  // Hv * J
+    __shared__ double sB[144];   // body-space Hessian before projection
+    __shared__ double sA[144];   // Jacobi iterate
+    __shared__ double sV[144];   // eigenvectors
+    __shared__ double sg[12];
+    __shared__ double sG[4][9];  // slot Jacobian (rotation columns): [k][c]
+    __shared__ double sT[4][6];  // slot rotational Hessian term, (c,d) packed
+    __shared__ double scs[6][2];
+    const int lane = threadIdx.x;
+    for (long long c = blockIdx.x; c < a.n; c += gridDim.x) {
+        ConRec rec;
+        BarrierLocal B;
+        FrictionLocal Fl;
+        local_setup<FRIC>(a, c, rec, B, Fl);
+        const int nv = con_nv(rec.type);
+        if (lane == 0) a.energy[c] = FRIC ? Fl.energy : barrier_local_energy(B);
+        if (lane < 12) sg[lane] = FRIC ? friction_local_grad(Fl, lane) : barrier_local_grad(B, lane / 3, lane % 3);
+        for (int e = lane; e < 144; e += 64) {
+            const int r = e / 12, q = e % 12;
+            sHv[e] = FRIC ? friction_local_hess(Fl, r, q) : barrier_local_hess(B, r / 3, r % 3, q / 3, q % 3);
+        }
+        if (lane < 36) {
+            const int s = lane / 9, kc = lane % 9;
+            double val = 0.0;
+            if (s < nv) {
+                const int v = rec.v[s];
+                val = jac_rot(a.frames[a.v_body[v]], a.V + 3 * v, kc % 3, kc / 3);
+            }
+            sG[s][kc] = val;  // [k][c] with k = kc / 3, c = kc % 3
+        }
+        __syncthreads();
+        if (lane < 24) {
+            const int s = lane / 6, cd = lane % 6;
+            double val = 0.0;
+            if (s < nv) {
+                const int v = rec.v[s];
+                const BodyFrame& F = a.frames[a.v_body[v]];
+                const double* x = a.V + 3 * v;
+                for (int k = 0; k < 3; k++) {
+                    const double h = (F.d2R[cd][3 * k] * x[0] + F.d2R[cd][3 * k + 1] * x[1]) + F.d2R[cd][3 * k + 2] * x[2];
+                    val += h * sg[3 * s + k];
+                }
+            }
+            sT[s][cd] = val;
+        }
+        // HJ = Hv J (vertex rows x body columns)
+        for (int e = lane; e < 144; e += 64) {
+            const int r = e / 12, col = e % 12, side = col / 6, q = col % 6;
+            double sum = 0.0;
+            for (int s = 0; s < nv; s++) {
+                if (slot_side(rec.type, s) != side) continue;
+                if (q < 3) {
+                    sum += sHv[12 * r + 3 * s + q];
+                } else {
+                    for (int j = 0; j < 3; j++) sum += sHv[12 * r + 3 * s + j] * sG[s][3 * j + (q - 3)];
+                }
+            }
+            sHJ[e] = sum;
+        }
+        __syncthreads();
+        // B = J^T HJ + rotational terms; body gradient J^T g
+        for (int e = lane; e < 144; e += 64) {
+            const int rr = e / 12, col = e % 12, side = rr / 6, p = rr % 6;
+            double sum = 0.0;
+            for (int s = 0; s < nv; s++) {
+                if (slot_side(rec.type, s) != side) continue;
+                if (p < 3) {
+                    sum += sHJ[12 * (3 * s + p) + col];
+                } else {
+                    for (int j = 0; j < 3; j++) sum += sG[s][3 * j + (p - 3)] * sHJ[12 * (3 * s + j) + col];
+                }
+            }
+            if (col / 6 == side && p >= 3 && col % 6 >= 3)
+                for (int s = 0; s < nv; s++)
+                    if (slot_side(rec.type, s) == side) sum += sT[s][sym_index(p - 3, col % 6 - 3)];
+            sB[e] = sum;
+            sA[e] = sum;
+            sV[e] = (rr == col) ? 1.0 : 0.0;
+        }
+        if (lane < 12) {
+            const int side = lane / 6, p = lane % 6;
+            double sum = 0.0;
+            for (int s = 0; s < nv; s++) {
+                if (slot_side(rec.type, s) != side) continue;
+                if (p < 3) {
+                    sum += sg[3 * s + p];
+                } else {
+                    for (int j = 0; j < 3; j++) sum += sG[s][3 * j + (p - 3)] * sg[3 * s + j];
+                }
+            }
+            a.lgrad[12 * c + lane] = sum;
+        }
+        __syncthreads();
+        // project_to_psd: Jacobi in parallel order; unchanged when PSD
+        for (int sweep = 0; sweep < 100; sweep++) {
+            double off = 0.0, tot = 0.0;
+            for (int e = lane; e < 144; e += 64) {
+                const double v = sA[e];
+                tot += v * v;
+                if (e / 12 != e % 12) off += v * v;
+            }
+            off = wave_sum_uniform(off);
+            tot = wave_sum_uniform(tot);
+            if (off <= 1e-30 * tot || off == 0) break;
+            for (int round = 0; round < 11; round++) {
+                if (lane < 6) {
+                    const int p = JACOBI_PAIRS[round][lane][0], q = JACOBI_PAIRS[round][lane][1];
+                    const double apq = sA[12 * p + q];
+                    double cc = 1.0, ss = 0.0;
+                    if (apq != 0.0) {
+                        const double theta = (sA[13 * q] - sA[13 * p]) / (2 * apq);
+                        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+                        cc = 1 / sqrt(t * t + 1);
+                        ss = t * cc;
+                    }
+                    scs[lane][0] = cc;
+                    scs[lane][1] = ss;
+                }
+                __syncthreads();
+                for (int t = lane; t < 144; t += 64) {
+                    double* M = t < 72 ? sA : sV;
+                    const int rem = t % 72, i = rem / 6, k = rem % 6;
+                    const int p = JACOBI_PAIRS[round][k][0], q = JACOBI_PAIRS[round][k][1];
+                    const double cc = scs[k][0], ss = scs[k][1];
+                    const double x = M[12 * i + p], y = M[12 * i + q];
+                    M[12 * i + p] = cc * x - ss * y;
+                    M[12 * i + q] = ss * x + cc * y;
+                }
+                __syncthreads();
+                for (int t = lane; t < 72; t += 64) {
+                    const int j = t / 6, k = t % 6;
+                    const int p = JACOBI_PAIRS[round][k][0], q = JACOBI_PAIRS[round][k][1];
+                    const double cc = scs[k][0], ss = scs[k][1];
+                    const double x = sA[12 * p + j], y = sA[12 * q + j];
+                    sA[12 * p + j] = cc * x - ss * y;
+                    sA[12 * q + j] = ss * x + cc * y;
+                }
+                __syncthreads();
+            }
+        }
+        bool psd = true;
+        for (int i = 0; i < 12; i++) psd = psd && sA[13 * i] >= 0.0;
+        for (int e = lane; e < 144; e += 64) {
+            double out;
+            if (psd) {
+                out = sB[e];
+            } else {
+                const int i = e / 12, j = e % 12;
+                out = 0.0;
+                for (int k = 0; k < 12; k++) {
+                    const double lam = sA[13 * k] > 0.0 ? sA[13 * k] : 0.0;
+                    out += sV[12 * i + k] * lam * sV[12 * j + k];
+                }
+            }
+            a.lhess[144 * c + e] = out;
+        }
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------- reduction
+// Deterministic sum of n doubles with one block.
+__global__ __launch_bounds__(1024) void sum_kernel(const double* x, long long n, double* out) {
+    __shared__ double part[1024];
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 1024) s += x[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
+// key = body << 32 | (2 c + side)
+__global__ __launch_bounds__(THREADS) void diag_keys_kernel(const ConRec* cons, const FricRec* fric, long long n,
+                                                           unsigned long long* keys) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const ConRec& r = fric ? fric[c].c : cons[c];
+    keys[2 * c] = ((unsigned long long)r.body[0] << 32) | (unsigned long long)(2 * c);
+    keys[2 * c + 1] = ((unsigned long long)r.body[1] << 32) | (unsigned long long)(2 * c + 1);
+}
+
+// key = (min body * nb + max body) << idx_bits | c
+__global__ __launch_bounds__(THREADS) void off_keys_kernel(const ConRec* cons, const FricRec* fric, long long n,
+                                                          int nb, int idx_bits, unsigned long long* keys) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const ConRec& r = fric ? fric[c].c : cons[c];
+    const unsigned long long lo = (unsigned long long)min(r.body[0], r.body[1]);
+    const unsigned long long hi = (unsigned long long)max(r.body[0], r.body[1]);
+    keys[c] = ((lo * (unsigned long long)nb + hi) << idx_bits) | (unsigned long long)c;
+}
+
+__device__ long long lower_bound_u64(const unsigned long long* k, long long n, unsigned long long x) {
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (k[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Block per body: lanes 0..35 the diagonal 6x6 block, lanes 36..41 the
+// gradient, summed in sorted (constraint) order.
+__global__ __launch_bounds__(64) void body_reduce_kernel(const unsigned long long* keys, long long nk,
+                                                        const double* lgrad, const double* lhess, int want_hess,
+                                                        double* grad, double* diag) {
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const long long lo = lower_bound_u64(keys, nk, (unsigned long long)b << 32);
+    const long long hi = lower_bound_u64(keys, nk, (unsigned long long)(b + 1) << 32);
+    if (lane < 36) {
+        if (!want_hess) return;
+        const int p = lane / 6, q = lane % 6;
+        double s = 0.0;
+        for (long long i = lo; i < hi; i++) {
+            const unsigned long long t = keys[i] & 0xffffffffULL;
+            const long long c = (long long)(t >> 1);
+            const int side = (int)(t & 1ULL);
+            s += lhess[144 * c + 12 * (6 * side + p) + 6 * side + q];
+        }
+        diag[36 * b + lane] = s;
+    } else if (lane < 42) {
+        const int p = lane - 36;
+        double s = 0.0;
+        for (long long i = lo; i < hi; i++) {
+            const unsigned long long t = keys[i] & 0xffffffffULL;
+            const long long c = (long long)(t >> 1);
+            const int side = (int)(t & 1ULL);
+            s += lgrad[12 * c + 6 * side + p];
+        }
+        grad[6 * b + p] = s;
+    }
+}
+
+__global__ __launch_bounds__(THREADS) void off_heads_kernel(const unsigned long long* keys, long long n, int idx_bits,
+                                                           unsigned int* flags) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flags[i] = (i == 0 || (keys[i] >> idx_bits) != (keys[i - 1] >> idx_bits)) ? 1u : 0u;
+}
+
+// Block per sorted key; heads sum their run into block pos[i].
+__global__ __launch_bounds__(64) void pair_reduce_kernel(const unsigned long long* keys, long long n, int idx_bits,
+                                                        int nb, const unsigned int* flags, const unsigned int* pos,
+                                                        const ConRec* cons, const FricRec* fric, const double* lhess,
+                                                        int* pairs, double* blocks) {
+    const long long i = blockIdx.x;
+    if (!flags[i]) return;
+    const int lane = threadIdx.x;
+    const unsigned long long pk = keys[i] >> idx_bits;
+    const int bi = (int)(pk / (unsigned long long)nb), bj = (int)(pk % (unsigned long long)nb);
+    const unsigned long long mask = (idx_bits >= 64) ? ~0ULL : ((1ULL << idx_bits) - 1ULL);
+    const unsigned int u = pos[i];
+    if (lane < 36) {
+        const int p = lane / 6, q = lane % 6;
+        double s = 0.0;
+        for (long long j = i; j < n && (keys[j] >> idx_bits) == pk; j++) {
+            const long long c = (long long)(keys[j] & mask);
+            const ConRec& r = fric ? fric[c].c : cons[c];
+            // block (bi, bj) = local (side of bi, side of bj)
+            if (r.body[0] == bi) s += lhess[144 * c + 12 * p + 6 + q];
+            else s += lhess[144 * c + 12 * (6 + p) + q];
+        }
+        blocks[36 * (long long)u + lane] = s;
+    } else if (lane == 36) {
+        pairs[2 * (long long)u] = bi;
+        pairs[2 * (long long)u + 1] = bj;
+    }
+}
+
+// --------------------------------------------------------------- friction set
+__global__ __launch_bounds__(THREADS) void friction_record_kernel(const ConRec* cons, long long n, const double* W,
+                                                                 double dhat, double dmin, double kappa, double mu,
+                                                                 FricRec* out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const ConRec r = cons[c];
+    const int nv = con_nv(r.type);
+    double X[12];
+    for (int s = 0; s < 4; s++)
+        for (int k = 0; k < 3; k++) X[3 * s + k] = s < nv ? W[3 * r.v[s] + k] : 0.0;
+    DistForm f;
+    constraint_form(r.type, X, f);
+    FricRec o;
+    o.c = r;
+    double basis[3][2];
+    friction_frame(r.type, X, o.coords, basis);
+    for (int k = 0; k < 3; k++) {
+        o.basis[2 * k] = basis[k][0];
+        o.basis[2 * k + 1] = basis[k][1];
+    }
+    const double dh = 2 * dmin * dhat + dhat * dhat, dm2 = dmin * dmin;
+    // compute_normal_force_magnitude: -kappa b'(d) 2 sqrt(d)
+    o.normal_force = -kappa * (ipc_barrier_gradient(f.d - dm2, dh) * (2 * sqrt(f.d)));
+    o.mu = mu;
+    out[c] = o;
+}
+
+// Minimum squared distance over the set (non-negative doubles order as
+// their bit patterns).
+__global__ __launch_bounds__(THREADS) void min_distance_kernel(const ConRec* cons, long long n, const double* W,
+                                                              unsigned long long* out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const ConRec r = cons[c];
+    const int nv = con_nv(r.type);
+    double X[12];
+    for (int s = 0; s < 4; s++)
+        for (int k = 0; k < 3; k++) X[3 * s + k] = s < nv ? W[3 * r.v[s] + k] : 0.0;
+    DistForm f;
+    constraint_form(r.type, X, f);
+    unsigned long long bits;
+    memcpy(&bits, &f.d, sizeof(bits));
+    atomicMin(out, bits);
+}
+
+unsigned grid_for(long long n, int threads = THREADS) { return (unsigned)((n + threads - 1) / threads); }
+
+hipError_t world_vertices(Assembly& as, const DevScene& sc, const double* d_poses, DevBuf& W, bool frames,
+                          hipStream_t st) {
+    AS_CHECK(as.R.ensure(9 * sizeof(double) * sc.nb));
+    AS_CHECK(as.frames.ensure(sizeof(BodyFrame) * sc.nb));
+    AS_CHECK(W.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
+    hipLaunchKernelGGL(body_frame_kernel, dim3(grid_for(sc.nb)), dim3(THREADS), 0, st, sc.nb, d_poses,
+                       as.R.as<double>(), frames ? as.frames.as<BodyFrame>() : nullptr);
+    if (sc.nv)
+        hipLaunchKernelGGL(world_vertex_kernel, dim3(grid_for(sc.nv)), dim3(THREADS), 0, st, sc.nv, sc.V, sc.v_body,
+                           d_poses, as.R.as<double>(), W.as<double>());
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc, const double* d_poses,
+                                double dhat, double dmin, int num_cus, hipStream_t st) {
+    (void)num_cus;
+    as.have_cons = false;  // the lagged friction set is independent and survives
+    as.n_con = 0;
+    for (int k = 0; k < 4; k++) as.counts[k] = 0;
+    if (sc.nb <= 1) {
+        as.have_cons = true;
+        return hipSuccess;
+    }
+    AS_CHECK(run_broad_phase(bp, sc, d_poses, d_poses, (dhat + dmin) / 2.0, false, st));
+    AS_CHECK(world_vertices(as, sc, d_poses, as.W, false, st));
+    const long long nc = bp.n_ee + bp.n_fv;
+    if (nc == 0) {
+        as.have_cons = true;
+        return hipSuccess;
+    }
+    for (int k = 0; k < 4; k++) AS_CHECK(as.ckeys[k].ensure(sizeof(unsigned long long) * nc));
+    AS_CHECK(as.ccount.ensure(8 * sizeof(unsigned int)));
+    AS_CHECK(hipMemsetAsync(as.ccount.p, 0, 8 * sizeof(unsigned int), st));
+    ClassifyArgs ca;
+    ca.ee = bp.out_ee.as<int>();
+    ca.fv = bp.out_fv.as<int>();
+    ca.n_ee = bp.n_ee;
+    ca.n_fv = bp.n_fv;
+    ca.E = sc.E;
+    ca.F = sc.F;
+    ca.FE = sc.FE;
+    ca.W = as.W.as<double>();
+    ca.nv = sc.nv;
+    ca.offset_sqr = (dmin + dhat) * (dmin + dhat);
+    for (int k = 0; k < 4; k++) ca.keys[k] = as.ckeys[k].as<unsigned long long>();
+    ca.count = as.ccount.as<unsigned int>();
+    hipLaunchKernelGGL(classify_kernel, dim3(grid_for(nc)), dim3(THREADS), 0, st, ca);
+    AS_CHECK(hipGetLastError());
+    unsigned int cnt[4];
+    AS_CHECK(hipMemcpyAsync(cnt, as.ccount.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    AS_CHECK(hipStreamSynchronize(st));
+
+    const unsigned long long key_max[4] = {
+        (unsigned long long)sc.nv * (unsigned long long)sc.nv, (unsigned long long)sc.ne * (unsigned long long)sc.nv,
+        (unsigned long long)bp.n_ee, (unsigned long long)bp.n_fv };
+    long long nmax = 1;
+    for (int k = 0; k < 4; k++) nmax = std::max<long long>(nmax, cnt[k]);
+    AS_CHECK(as.ctmp.ensure(sizeof(unsigned long long) * nmax));
+    AS_CHECK(as.chist.ensure(sizeof(unsigned int) * radix_sort_hist_entries(nmax)));
+    AS_CHECK(as.cflags.ensure(sizeof(unsigned int) * nmax));
+    AS_CHECK(as.cpos.ensure(sizeof(unsigned int) * (2 * nmax + 2)));
+    AS_CHECK(as.cwork.ensure(sizeof(unsigned int) * (scan_workspace_entries(nmax) + 4)));
+    for (int k = 0; k < 4; k++)
+        if (cnt[k] > 1)
+            AS_CHECK(radix_sort_u64(as.ckeys[k].as<unsigned long long>(), as.ctmp.as<unsigned long long>(),
+                                    as.chist.as<unsigned int>(), cnt[k], bits_for(key_max[k]), st));
+
+    // run-length encode VV and EV (multiplicities); run starts kept per class
+    long long nout[4] = { cnt[0], cnt[1], cnt[2], cnt[3] };
+    DevBuf* starts[2] = { &as.cstart[0], &as.cstart[1] };
+    for (int k = 0; k < 2; k++) {
+        if (cnt[k] == 0) continue;
+        unsigned int* flags = as.cflags.as<unsigned int>();
+        unsigned int* pos = as.cpos.as<unsigned int>();
+        unsigned int* total = as.cwork.as<unsigned int>() + scan_workspace_entries(nmax);
+        AS_CHECK(starts[k]->ensure(sizeof(unsigned int) * cnt[k]));
+        hipLaunchKernelGGL(head_flags_kernel, dim3(grid_for(cnt[k])), dim3(THREADS), 0, st,
+                           as.ckeys[k].as<unsigned long long>(), (long long)cnt[k], flags);
+        exclusive_scan_u32(flags, pos, cnt[k], as.cwork.as<unsigned int>(), total, st);
+        hipLaunchKernelGGL(head_starts_kernel, dim3(grid_for(cnt[k])), dim3(THREADS), 0, st, flags, pos,
+                           (long long)cnt[k], starts[k]->as<unsigned int>());
+        unsigned int nu = 0;
+        AS_CHECK(hipMemcpyAsync(&nu, total, sizeof(nu), hipMemcpyDeviceToHost, st));
+        AS_CHECK(hipStreamSynchronize(st));
+        nout[k] = nu;
+    }
+    const long long n = nout[0] + nout[1] + nout[2] + nout[3];
+    AS_CHECK(as.cons.ensure(sizeof(ConRec) * std::max<long long>(n, 1)));
+    long long off = 0;
+    for (int k = 0; k < 4; k++) {
+        if (nout[k] == 0) continue;
+        RecordArgs ra;
+        ra.cls = k;
+        ra.keys = as.ckeys[k].as<unsigned long long>();
+        ra.start = k < 2 ? starts[k]->as<unsigned int>() : nullptr;
+        ra.n_keys = cnt[k];
+        ra.n_out = nout[k];
+        ra.out = as.cons.as<ConRec>() + off;
+        ra.ee = bp.out_ee.as<int>();
+        ra.fv = bp.out_fv.as<int>();
+        ra.E = sc.E;
+        ra.F = sc.F;
+        ra.v_body = sc.v_body;
+        ra.W = as.W.as<double>();
+        ra.nv = sc.nv;
+        hipLaunchKernelGGL(record_kernel, dim3(grid_for(nout[k])), dim3(THREADS), 0, st, ra);
+        off += nout[k];
+    }
+    AS_CHECK(hipGetLastError());
+    for (int k = 0; k < 4; k++) as.counts[k] = nout[k];
+    as.n_con = n;
+    as.have_cons = true;
+    return hipSuccess;
+}
+
+hipError_t build_friction_set(Assembly& as, const DevScene& sc, const double* d_poses, double dhat, double dmin,
+                              double kappa, double mu, hipStream_t st) {
+    as.have_fric = false;
+    as.n_fric = as.n_con;
+    AS_CHECK(as.fric.ensure(sizeof(FricRec) * std::max<long long>(as.n_con, 1)));
+    if (as.n_con) {
+        AS_CHECK(world_vertices(as, sc, d_poses, as.W, false, st));
+        hipLaunchKernelGGL(friction_record_kernel, dim3(grid_for(as.n_con)), dim3(THREADS), 0, st,
+                           as.cons.as<ConRec>(), as.n_con, as.W.as<double>(), dhat, dmin, kappa, mu,
+                           as.fric.as<FricRec>());
+        AS_CHECK(hipGetLastError());
+    }
+    as.have_fric = true;
+    return hipSuccess;
+}
+
+hipError_t minimum_distance(Assembly& as, const DevScene& sc, const double* d_poses, double& min_sq,
+                            hipStream_t st) {
+    min_sq = HUGE_VAL;
+    if (as.n_con == 0) return hipSuccess;
+    AS_CHECK(world_vertices(as, sc, d_poses, as.W, false, st));
+    AS_CHECK(as.mind.ensure(sizeof(unsigned long long)));
+    const unsigned long long inf_bits = 0x7ff0000000000000ULL;
+    AS_CHECK(hipMemcpyAsync(as.mind.p, &inf_bits, sizeof(inf_bits), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(min_distance_kernel, dim3(grid_for(as.n_con)), dim3(THREADS), 0, st, as.cons.as<ConRec>(),
+                       as.n_con, as.W.as<double>(), as.mind.as<unsigned long long>());
+    AS_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    AS_CHECK(hipMemcpyAsync(&bits, as.mind.p, sizeof(bits), hipMemcpyDeviceToHost, st));
+    AS_CHECK(hipStreamSynchronize(st));
+    std::memcpy(&min_sq, &bits, sizeof(min_sq));
+    return hipSuccess;
+}
+
+hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, const double* d_poses,
+                              const double* d_poses_t0, double dhat, double dmin, double epsv_h, bool want_grad,
+                              bool want_hess, int num_cus, AssemblyOut& out, hipStream_t st) {
+    out = AssemblyOut{};
+    as.n_off = 0;
+    const long long n = friction ? as.n_fric : as.n_con;
+    const int nb = sc.nb;
+    want_grad = want_grad || want_hess;
+    AS_CHECK(as.grad.ensure(6 * sizeof(double) * nb));
+    AS_CHECK(as.diag.ensure(36 * sizeof(double) * nb));
+    if (n == 0) {
+        if (want_grad) AS_CHECK(hipMemsetAsync(as.grad.p, 0, 6 * sizeof(double) * nb, st));
+        if (want_hess) AS_CHECK(hipMemsetAsync(as.diag.p, 0, 36 * sizeof(double) * nb, st));
+        return hipStreamSynchronize(st);
+    }
+    AS_CHECK(world_vertices(as, sc, d_poses, as.W, want_grad, st));
+    if (friction) AS_CHECK(world_vertices(as, sc, d_poses_t0, as.W0, false, st));  // frames stay at d_poses
+    AS_CHECK(as.energy.ensure(sizeof(double) * n));
+    AS_CHECK(as.lgrad.ensure(12 * sizeof(double) * n));
+    if (want_hess) AS_CHECK(as.lhess.ensure(144 * sizeof(double) * n));
+    AS_CHECK(as.esum.ensure(sizeof(double)));
+    LocalArgs la;
+    la.cons = as.cons.as<ConRec>();
+    la.fric = friction ? as.fric.as<FricRec>() : nullptr;
+    la.n = n;
+    la.V = sc.V;
+    la.v_body = sc.v_body;
+    la.W = as.W.as<double>();
+    la.W0 = friction ? as.W0.as<double>() : nullptr;
+    la.frames = as.frames.as<BodyFrame>();
+    la.dhat = dhat;
+    la.dmin = dmin;
+    la.epsv_h = epsv_h;
+    la.want_grad = want_grad ? 1 : 0;
+    la.energy = as.energy.as<double>();
+    la.lgrad = as.lgrad.as<double>();
+    la.lhess = want_hess ? as.lhess.as<double>() : nullptr;
+    if (want_hess) {
+        const unsigned grid = (unsigned)std::min<long long>(n, (long long)num_cus * 32);
+        if (friction) hipLaunchKernelGGL(local_hess_kernel<true>, dim3(grid), dim3(64), 0, st, la);
+        else hipLaunchKernelGGL(local_hess_kernel<false>, dim3(grid), dim3(64), 0, st, la);
+    } else {
+        if (friction) hipLaunchKernelGGL(local_eg_kernel<true>, dim3(grid_for(n)), dim3(THREADS), 0, st, la);
+        else hipLaunchKernelGGL(local_eg_kernel<false>, dim3(grid_for(n)), dim3(THREADS), 0, st, la);
+    }
+    AS_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, st, as.energy.as<double>(), n, as.esum.as<double>());
+
+    const ConRec* cons = friction ? nullptr : as.cons.as<ConRec>();
+    const FricRec* fric = friction ? as.fric.as<FricRec>() : nullptr;
+    if (want_grad) {
+        const long long nk = 2 * n;
+        AS_CHECK(as.dkeys.ensure(sizeof(unsigned long long) * nk));
+        AS_CHECK(as.ktmp.ensure(sizeof(unsigned long long) * nk));
+        AS_CHECK(as.khist.ensure(sizeof(unsigned int) * radix_sort_hist_entries(nk)));
+        hipLaunchKernelGGL(diag_keys_kernel, dim3(grid_for(n)), dim3(THREADS), 0, st, cons, fric, n,
+                           as.dkeys.as<unsigned long long>());
+        const int bits = 32 + bits_for((unsigned long long)nb);
+        AS_CHECK(radix_sort_u64(as.dkeys.as<unsigned long long>(), as.ktmp.as<unsigned long long>(),
+                                as.khist.as<unsigned int>(), nk, bits, st));
+        hipLaunchKernelGGL(body_reduce_kernel, dim3(nb), dim3(64), 0, st, as.dkeys.as<unsigned long long>(), nk,
+                           as.lgrad.as<double>(), want_hess ? as.lhess.as<double>() : nullptr, want_hess ? 1 : 0,
+                           as.grad.as<double>(), as.diag.as<double>());
+        AS_CHECK(hipGetLastError());
+    }
+    if (want_hess) {
+        const int idx_bits = std::max(1, bits_for((unsigned long long)n));
+        const int pair_bits = bits_for((unsigned long long)nb * (unsigned long long)nb);
+        if (idx_bits + pair_bits > 64) return hipErrorInvalidValue;
+        AS_CHECK(as.okeys.ensure(sizeof(unsigned long long) * n));
+        AS_CHECK(as.ktmp.ensure(sizeof(unsigned long long) * n));
+        AS_CHECK(as.khist.ensure(sizeof(unsigned int) * radix_sort_hist_entries(n)));
+        AS_CHECK(as.oflags.ensure(sizeof(unsigned int) * n));
+        AS_CHECK(as.opos.ensure(sizeof(unsigned int) * n));
+        AS_CHECK(as.owork.ensure(sizeof(unsigned int) * (scan_workspace_entries(n) + 4)));
+        AS_CHECK(as.opairs.ensure(2 * sizeof(int) * n));
+        AS_CHECK(as.oblocks.ensure(36 * sizeof(double) * n));
+        hipLaunchKernelGGL(off_keys_kernel, dim3(grid_for(n)), dim3(THREADS), 0, st, cons, fric, n, nb, idx_bits,
+                           as.okeys.as<unsigned long long>());
+        AS_CHECK(radix_sort_u64(as.okeys.as<unsigned long long>(), as.ktmp.as<unsigned long long>(),
+                                as.khist.as<unsigned int>(), n, idx_bits + pair_bits, st));
+        hipLaunchKernelGGL(off_heads_kernel, dim3(grid_for(n)), dim3(THREADS), 0, st,
+                           as.okeys.as<unsigned long long>(), n, idx_bits, as.oflags.as<unsigned int>());
+        unsigned int* total = as.owork.as<unsigned int>() + scan_workspace_entries(n);
+        exclusive_scan_u32(as.oflags.as<unsigned int>(), as.opos.as<unsigned int>(), n, as.owork.as<unsigned int>(),
+                           total, st);
+        hipLaunchKernelGGL(pair_reduce_kernel, dim3((unsigned)n), dim3(64), 0, st, as.okeys.as<unsigned long long>(),
+                           n, idx_bits, nb, as.oflags.as<unsigned int>(), as.opos.as<unsigned int>(), cons, fric,
+                           as.lhess.as<double>(), as.opairs.as<int>(), as.oblocks.as<double>());
+        AS_CHECK(hipGetLastError());
+        unsigned int np = 0;
+        AS_CHECK(hipMemcpyAsync(&np, total, sizeof(np), hipMemcpyDeviceToHost, st));
+        AS_CHECK(hipMemcpyAsync(&out.energy, as.esum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        AS_CHECK(hipStreamSynchronize(st));
+        as.n_off = np;
+        out.n_off = np;
+        return hipSuccess;
+    }
+    AS_CHECK(hipMemcpyAsync(&out.energy, as.esum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+}  // namespace rigc

@@ -29,6 +29,7 @@ struct rigc_ctx {
     // per-call buffers
     DevBuf poses0, poses1, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
     BroadPhase bp;
+    Assembly as;
     hipEvent_t ev2 = nullptr;
 
     rigc_status fail(rigc_status s, const std::string& msg) {
@@ -110,6 +111,7 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
                        &ctx->bp.pairs, &ctx->bp.keys_ee, &ctx->bp.keys_fv, &ctx->bp.tmp, &ctx->bp.hist,
                        &ctx->bp.counters, &ctx->bp.out_ee, &ctx->bp.out_fv })
         b->release();
+    ctx->as.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RIGC_OK;
@@ -188,6 +190,8 @@ rigc_status rigc_set_scene(rigc_ctx* ctx, const rigc_scene* s) {
     RIGC_HIP(upload(ctx, &d.edge_len, elen.data(), (size_t)ne));
     RIGC_HIP(upload(ctx, &d.r_max, r_max.data(), (size_t)nb));
     RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->as.have_cons = ctx->as.have_fric = false;
+    ctx->as.n_con = ctx->as.n_fric = 0;
     ctx->has_scene = true;
     return RIGC_OK;
 }
@@ -408,6 +412,195 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     std::memcpy(&g, &res[0], sizeof(double));
     *earliest_toi = g;  // +inf when no candidate collides
     return RIGC_OK;
+}
+
+namespace {
+
+rigc_status copy_potential(rigc_ctx* ctx, const AssemblyOut& out, int32_t flags, double* energy, double* grad,
+                           rigc_block_hessian* hess) {
+    const DevScene& d = ctx->scene;
+    hipStream_t st = ctx->stream;
+    *energy = out.energy;
+    const bool want_hess = (flags & RIGC_HESSIAN) != 0;
+    const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
+    if (want_grad && grad)
+        RIGC_HIP(hipMemcpyAsync(grad, ctx->as.grad.p, 6 * sizeof(double) * d.nb, hipMemcpyDeviceToHost, st));
+    bool fits = true;
+    if (want_hess && hess) {
+        RIGC_HIP(hipMemcpyAsync(hess->diag, ctx->as.diag.p, 36 * sizeof(double) * d.nb, hipMemcpyDeviceToHost, st));
+        hess->count = out.n_off;
+        fits = out.n_off <= hess->capacity;
+        if (fits && out.n_off) {
+            RIGC_HIP(hipMemcpyAsync(hess->pairs, ctx->as.opairs.p, 2 * sizeof(int32_t) * out.n_off,
+                                    hipMemcpyDeviceToHost, st));
+            RIGC_HIP(hipMemcpyAsync(hess->off, ctx->as.oblocks.p, 36 * sizeof(double) * out.n_off,
+                                    hipMemcpyDeviceToHost, st));
+        }
+    }
+    RIGC_HIP(hipStreamSynchronize(st));
+    if (!fits) return ctx->fail(RIGC_ERR_CAPACITY, "off-diagonal Hessian capacity too small");
+    return RIGC_OK;
+}
+
+bool bad_hess(int32_t flags, double* grad, rigc_block_hessian* hess) {
+    const bool want_hess = (flags & RIGC_HESSIAN) != 0;
+    const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
+    if (want_grad && !grad) return true;
+    if (want_hess && (!hess || !hess->diag || hess->capacity < 0 || (hess->capacity && (!hess->pairs || !hess->off))))
+        return true;
+    return false;
+}
+
+}  // namespace
+
+rigc_status rigc_construct_constraint_set(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
+                                          int64_t* counts) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_construct_constraint_set: no scene");
+    if (!poses || !(dhat > 0) || !(dmin >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_construct_constraint_set: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses, nullptr);
+    if (s != RIGC_OK) return s;
+    RIGC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, ctx->num_cus,
+                                  ctx->stream));
+    RIGC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.assembly_ms = ms;
+    ctx->stats.num_candidates = ctx->bp.n_ee + ctx->bp.n_fv;
+    if (counts)
+        for (int k = 0; k < 4; k++) counts[k] = ctx->as.counts[k];
+    return RIGC_OK;
+}
+
+rigc_status rigc_get_constraint_set(rigc_ctx* ctx, int64_t capacity, int32_t* records, double* eps_x, int64_t* n) {
+    if (!ctx || !n) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->as.have_cons) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_get_constraint_set: no constraint set");
+    *n = ctx->as.n_con;
+    if (ctx->as.n_con > capacity) return ctx->fail(RIGC_ERR_CAPACITY, "rigc_get_constraint_set: capacity too small");
+    if (ctx->as.n_con == 0) return RIGC_OK;
+    if (!records) return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_get_constraint_set: records is NULL");
+    std::vector<ConRec> h(ctx->as.n_con);
+    RIGC_HIP(hipMemcpyAsync(h.data(), ctx->as.cons.p, sizeof(ConRec) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < h.size(); i++) {
+        int32_t* r = records + 6 * i;
+        r[0] = h[i].type; r[1] = h[i].a; r[2] = h[i].b; r[3] = h[i].mult; r[4] = h[i].body[0]; r[5] = h[i].body[1];
+        if (eps_x) eps_x[i] = h[i].eps_x;
+    }
+    return RIGC_OK;
+}
+
+rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double dhat, double dmin, int32_t flags,
+                                      double* energy, double* grad, rigc_block_hessian* hess) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_barrier_term: no scene");
+    if (!poses || !energy || !(dhat > 0) || !(dmin >= 0) || bad_hess(flags, grad, hess))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_compute_barrier_term: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses, nullptr);
+    if (s != RIGC_OK) return s;
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(hipEventRecord(ctx->ev0, st));
+    if (flags & RIGC_BUILD_CONSTRAINTS)
+        RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin,
+                                      ctx->num_cus, st));
+    if (!ctx->as.have_cons)
+        return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_barrier_term: no constraint set (build one first)");
+    AssemblyOut out;
+    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, false, ctx->poses0.as<double>(), nullptr, dhat, dmin, 0.0,
+                                (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
+    RIGC_HIP(hipEventRecord(ctx->ev1, st));
+    s = copy_potential(ctx, out, flags, energy, grad, hess);
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.assembly_ms = ms;
+    return s;
+}
+
+rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
+                                          double* min_distance) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_minimum_distance: no scene");
+    if (!poses || !min_distance || !(dhat > 0) || !(dmin >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_compute_minimum_distance: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses, nullptr);
+    if (s != RIGC_OK) return s;
+    RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, ctx->num_cus,
+                                  ctx->stream));
+    double m2 = HUGE_VAL;
+    RIGC_HIP(rigc::minimum_distance(ctx->as, ctx->scene, ctx->poses0.as<double>(), m2, ctx->stream));
+    *min_distance = std::sqrt(m2);
+    return RIGC_OK;
+}
+
+rigc_status rigc_update_friction_constraints(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
+                                             double barrier_stiffness, double mu, int64_t* count) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_update_friction_constraints: no scene");
+    if (!poses || !(dhat > 0) || !(dmin >= 0) || !(barrier_stiffness >= 0) || !(mu >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_update_friction_constraints: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses, nullptr);
+    if (s != RIGC_OK) return s;
+    RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, ctx->num_cus,
+                                  ctx->stream));
+    RIGC_HIP(build_friction_set(ctx->as, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, barrier_stiffness, mu,
+                                ctx->stream));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    if (count) *count = ctx->as.n_fric;
+    return RIGC_OK;
+}
+
+rigc_status rigc_get_friction_constraints(rigc_ctx* ctx, int64_t capacity, double* records, int64_t* n) {
+    if (!ctx || !n) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->as.have_fric) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_get_friction_constraints: no friction set");
+    *n = ctx->as.n_fric;
+    if (ctx->as.n_fric > capacity)
+        return ctx->fail(RIGC_ERR_CAPACITY, "rigc_get_friction_constraints: capacity too small");
+    if (ctx->as.n_fric == 0) return RIGC_OK;
+    if (!records) return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_get_friction_constraints: records is NULL");
+    std::vector<FricRec> h(ctx->as.n_fric);
+    RIGC_HIP(hipMemcpyAsync(h.data(), ctx->as.fric.p, sizeof(FricRec) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < h.size(); i++) {
+        double* r = records + 9 * i;
+        r[0] = h[i].normal_force;
+        r[1] = h[i].coords[0];
+        r[2] = h[i].coords[1];
+        for (int k = 0; k < 6; k++) r[3 + k] = h[i].basis[k];
+    }
+    return RIGC_OK;
+}
+
+rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const double* poses_t0,
+                                       double epsv_times_h, int32_t flags, double* energy, double* grad,
+                                       rigc_block_hessian* hess) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_friction_term: no scene");
+    if (!poses || !poses_t0 || !energy || !(epsv_times_h > 0) || bad_hess(flags, grad, hess))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_compute_friction_term: bad arguments");
+    if (!ctx->as.have_fric)
+        return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_friction_term: no friction set (update it first)");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses, poses_t0);
+    if (s != RIGC_OK) return s;
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(hipEventRecord(ctx->ev0, st));
+    AssemblyOut out;
+    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, true, ctx->poses0.as<double>(), ctx->poses1.as<double>(), 0.0,
+                                0.0, epsv_times_h, (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0,
+                                ctx->num_cus, out, st));
+    RIGC_HIP(hipEventRecord(ctx->ev1, st));
+    s = copy_potential(ctx, out, flags, energy, grad, hess);
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.assembly_ms = ms;
+    return s;
 }
 
 }  // extern "C"

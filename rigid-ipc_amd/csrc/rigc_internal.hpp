@@ -102,4 +102,75 @@ struct CcdArgs {
 
 void launch_ccd(const CcdArgs& a, int grid, int block, hipStream_t stream);
 
+// Exclusive prefix sum of n counts; work holds scan_workspace_entries(n)
+// entries and *total (device) receives the sum.
+size_t scan_workspace_entries(long long n);
+void exclusive_scan_u32(const unsigned int* in, unsigned int* out, long long n, unsigned int* work,
+                        unsigned int* total, hipStream_t stream);
+
+// One collision constraint (ipc-toolkit vertex order):
+//   VV (v0, v1), EV (v, e0, e1), EE (ea0, ea1, eb0, eb1), FV (v, f0, f1, f2)
+// a, b: VV (v0, v1); EV (edge, vertex); EE (edge0, edge1); FV (face, vertex).
+struct ConRec {
+    int v[4];
+    int type;
+    int mult;
+    int body[2];
+    int a, b;
+    double eps_x;  // EE mollifier threshold (-1 otherwise)
+};
+
+// Lagged friction constraint: the collision constraint plus its closest
+// point, tangent basis (3x2 row-major), normal force and coefficient.
+struct FricRec {
+    ConRec c;
+    double coords[2];
+    double basis[6];
+    double normal_force;
+    double mu;
+};
+
+// Assembly working set (device buffers reused across calls).
+struct Assembly {
+    DevBuf R, frames, W, W0;            // body rotations / frames, world vertices
+    DevBuf ckeys[4], cstart[2], ccount, ctmp, chist, cflags, cpos, cwork;
+    DevBuf cons;                        // ConRec [n_con]
+    long long n_con = 0, counts[4] = { 0, 0, 0, 0 };
+    bool have_cons = false;
+    DevBuf fric;                        // FricRec [n_fric]
+    long long n_fric = 0;
+    bool have_fric = false;
+    DevBuf energy, lgrad, lhess, esum;  // per-constraint outputs and the energy sum
+    DevBuf dkeys, okeys, ktmp, khist, oflags, opos, owork;
+    DevBuf diag, grad, opairs, oblocks, mind;
+    long long n_off = 0;
+
+    void release() {
+        for (DevBuf* b : { &R, &frames, &W, &W0, &ckeys[0], &ckeys[1], &ckeys[2], &ckeys[3], &cstart[0], &cstart[1],
+                           &ccount, &ctmp, &chist, &cflags, &cpos, &cwork, &cons, &fric, &energy, &lgrad, &lhess,
+                           &esum, &dkeys, &okeys, &ktmp, &khist, &oflags, &opos, &owork, &diag, &grad, &opairs,
+                           &oblocks, &mind })
+            b->release();
+    }
+};
+
+struct AssemblyOut {
+    double energy = 0;
+    long long n_off = 0;  // off-diagonal blocks produced
+};
+
+// Constraint set at poses (device), kept in as.cons.
+hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc, const double* d_poses,
+                                double dhat, double dmin, int num_cus, hipStream_t stream);
+// Barrier (friction == false) or friction potential over the current set.
+// Outputs stay on the device: as.grad [nb][6], as.diag [nb][36],
+// as.opairs / as.oblocks [n_off].
+hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, const double* d_poses,
+                              const double* d_poses_t0, double dhat, double dmin, double epsv_h, bool want_grad,
+                              bool want_hess, int num_cus, AssemblyOut& out, hipStream_t stream);
+hipError_t build_friction_set(Assembly& as, const DevScene& sc, const double* d_poses, double dhat, double dmin,
+                              double kappa, double mu, hipStream_t stream);
+hipError_t minimum_distance(Assembly& as, const DevScene& sc, const double* d_poses, double& min_sq,
+                            hipStream_t stream);
+
 }  // namespace rigc

@@ -43,6 +43,16 @@ class Scene(ctypes.Structure):
     ]
 
 
+RIGC_GRADIENT = 1
+RIGC_HESSIAN = 2
+RIGC_BUILD_CONSTRAINTS = 4
+
+
+class BlockHessian(ctypes.Structure):
+    _fields_ = [("diag", _c_double_p), ("pairs", _c_int32_p), ("off", _c_double_p),
+                ("capacity", ctypes.c_int64), ("count", ctypes.c_int64)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("broad_phase_ms", ctypes.c_double), ("narrow_phase_ms", ctypes.c_double),
@@ -75,8 +85,20 @@ def lib():
     L.rigc_broad_phase.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_int32_p, ctypes.c_int64,
                                    _c_int64_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
     L.rigc_compute_earliest_toi.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p]
+    d, i32, i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
+    bh = ctypes.POINTER(BlockHessian)
+    L.rigc_construct_constraint_set.argtypes = [vp, _c_double_p, d, d, _c_int64_p]
+    L.rigc_get_constraint_set.argtypes = [vp, i64, _c_int32_p, _c_double_p, _c_int64_p]
+    L.rigc_compute_barrier_term.argtypes = [vp, _c_double_p, d, d, i32, _c_double_p, _c_double_p, bh]
+    L.rigc_compute_minimum_distance.argtypes = [vp, _c_double_p, d, d, _c_double_p]
+    L.rigc_update_friction_constraints.argtypes = [vp, _c_double_p, d, d, d, d, _c_int64_p]
+    L.rigc_get_friction_constraints.argtypes = [vp, i64, _c_double_p, _c_int64_p]
+    L.rigc_compute_friction_term.argtypes = [vp, _c_double_p, _c_double_p, d, i32, _c_double_p, _c_double_p, bh]
     for name in ("rigc_create", "rigc_destroy", "rigc_get_stats", "rigc_set_scene", "rigc_narrow_phase",
-                 "rigc_broad_phase", "rigc_compute_earliest_toi"):
+                 "rigc_broad_phase", "rigc_compute_earliest_toi", "rigc_construct_constraint_set",
+                 "rigc_get_constraint_set", "rigc_compute_barrier_term", "rigc_compute_minimum_distance",
+                 "rigc_update_friction_constraints", "rigc_get_friction_constraints",
+                 "rigc_compute_friction_term"):
         getattr(L, name).restype = ctypes.c_int32
     _lib = L
     return L
@@ -191,3 +213,103 @@ class Context:
         self._check(self._L.rigc_compute_earliest_toi(self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p),
                                                       float(minimum_separation_distance), ctypes.byref(toi)))
         return toi.value
+
+    # ------------------------------------------------------------ assembly
+    def construct_constraint_set(self, poses, dhat, dmin=0.0):
+        """Build the constraint set at poses on the device; returns the
+        (VV, EV, EE, FV) counts."""
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        counts = np.zeros(4, np.int64)
+        self._check(self._L.rigc_construct_constraint_set(self._ctx, _p(p, _c_double_p), float(dhat), float(dmin),
+                                                          _p(counts, _c_int64_p)))
+        return counts
+
+    def constraint_set(self):
+        """Current constraint set: (records (n,6) int32: type, a, b,
+        multiplicity, body0, body1; eps_x (n,))."""
+        n = ctypes.c_int64()
+        st = self._L.rigc_get_constraint_set(self._ctx, 0, None, None, ctypes.byref(n))
+        if st not in (RIGC_OK, RIGC_ERR_CAPACITY):
+            self._check(st)
+        rec = np.zeros((max(n.value, 1), 6), np.int32)
+        eps = np.zeros(max(n.value, 1))
+        self._check(self._L.rigc_get_constraint_set(self._ctx, n.value, _p(rec, _c_int32_p), _p(eps, _c_double_p),
+                                                    ctypes.byref(n)))
+        return rec[:n.value], eps[:n.value]
+
+    def _potential(self, fn, args, flags):
+        nb = self.num_bodies
+        energy = ctypes.c_double()
+        grad = np.zeros(6 * nb) if flags & (RIGC_GRADIENT | RIGC_HESSIAN) else None
+        hess = None
+        if flags & RIGC_HESSIAN:
+            cap = getattr(self, "_cap_off", 1024)
+            for _ in range(2):
+                diag = np.zeros((nb, 36))
+                pairs = np.zeros((max(cap, 1), 2), np.int32)
+                off = np.zeros((max(cap, 1), 36))
+                h = BlockHessian(_p(diag, _c_double_p), _p(pairs, _c_int32_p), _p(off, _c_double_p), cap, 0)
+                st = fn(self._ctx, *args, int(flags), ctypes.byref(energy), _p(grad, _c_double_p), ctypes.byref(h))
+                if st == RIGC_ERR_CAPACITY:
+                    cap = self._cap_off = int(h.count)
+                    continue
+                self._check(st)
+                hess = (diag.reshape(nb, 6, 6), pairs[:h.count].copy(), off[:h.count].reshape(-1, 6, 6))
+                break
+            else:
+                raise RigcError(RIGC_ERR_CAPACITY, "Hessian capacity retry failed")
+        else:
+            self._check(fn(self._ctx, *args, int(flags), ctypes.byref(energy), _p(grad, _c_double_p), None))
+        return energy.value, grad, hess
+
+    def barrier_term(self, poses, dhat, dmin=0.0, grad=False, hess=False, rebuild=False):
+        """Barrier energy over the current constraint set (rebuilt at poses
+        if rebuild); grad (6 nb) and hess (diag (nb,6,6), pairs (m,2),
+        off (m,6,6)) when asked."""
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0) | (RIGC_BUILD_CONSTRAINTS if rebuild else 0)
+        self._keep = [p]
+        return self._potential(self._L.rigc_compute_barrier_term, (_p(p, _c_double_p), float(dhat), float(dmin)), flags)
+
+    def minimum_distance(self, poses, dhat, dmin=0.0):
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        out = ctypes.c_double()
+        self._check(self._L.rigc_compute_minimum_distance(self._ctx, _p(p, _c_double_p), float(dhat), float(dmin),
+                                                          ctypes.byref(out)))
+        return out.value
+
+    def update_friction_constraints(self, poses, dhat, dmin, barrier_stiffness, mu):
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        n = ctypes.c_int64()
+        self._check(self._L.rigc_update_friction_constraints(self._ctx, _p(p, _c_double_p), float(dhat), float(dmin),
+                                                             float(barrier_stiffness), float(mu), ctypes.byref(n)))
+        return n.value
+
+    def friction_constraints(self):
+        """(n, 9): normal force, coords (2), basis (3x2 row-major)."""
+        n = ctypes.c_int64()
+        st = self._L.rigc_get_friction_constraints(self._ctx, 0, None, ctypes.byref(n))
+        if st not in (RIGC_OK, RIGC_ERR_CAPACITY):
+            self._check(st)
+        rec = np.zeros((max(n.value, 1), 9))
+        self._check(self._L.rigc_get_friction_constraints(self._ctx, n.value, _p(rec, _c_double_p), ctypes.byref(n)))
+        return rec[:n.value]
+
+    def friction_term(self, poses, poses_t0, epsv_times_h, grad=False, hess=False):
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0)
+        return self._potential(self._L.rigc_compute_friction_term,
+                               (_p(p, _c_double_p), _p(p0, _c_double_p), float(epsv_times_h)), flags)
+
+
+def dense_hessian(hess, nb):
+    """Dense (6 nb x 6 nb) matrix from the block form (test helper)."""
+    diag, pairs, off = hess
+    H = np.zeros((6 * nb, 6 * nb))
+    for b in range(nb):
+        H[6 * b:6 * b + 6, 6 * b:6 * b + 6] = diag[b]
+    for (i, j), B in zip(pairs, off):
+        H[6 * i:6 * i + 6, 6 * j:6 * j + 6] += B
+        H[6 * j:6 * j + 6, 6 * i:6 * i + 6] += B.T
+    return H
