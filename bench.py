@@ -1,15 +1,21 @@
 """Benchmark of the MI355X contact hot path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ccd]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sim|ccd] [--no-cpu]
 
-Workload "ccd": the 1,000-body mixed scene (config 2: ico2 / torus / cube,
-seeded, plus the ground plane). Each step is one call of the device CCD path
-(continuous broad phase + rigid interval narrow phase with cross-query
-pruning, rigc_compute_earliest_toi) from the poses at step s to the inertial
-prediction of step s+1 (x + h v + h^2 g, r + h w), along a ballistic
-trajectory. Prints one JSON line (see the keys below). The CPU baseline
-is the oracle (CPU restatement of the same path, OpenMP over all host
-cores) on the first steps of the same sequence.
+Workload "sim" (default): the 1,000-body mixed scene (ico2 / torus / cube,
+seeded, plus the ground plane; dt 0.01, dhat 1e-3, mu 0.2) stepped with
+the reference's implicit-Euler IPC Newton solve (host, librigc_sim.so)
+over the device contact path (librigc.so: CCD in every line search,
+constraint set, barrier and friction assembly, minimum distance). W
+warm-up steps run on a separate simulation of the same scene; the K timed
+steps start from the initial state. value = K / wall time of the K steps.
+The CPU baseline runs the same stepper over the oracle (the CPU
+restatement, OpenMP over the host cores) from the same initial state for a
+bounded sample of steps.
+
+Workload "ccd": one device CCD call (continuous broad phase + rigid
+interval narrow phase, rigc_compute_earliest_toi) per step along a
+ballistic trajectory of the same scene; value = CCD candidate pairs/s.
 """
 import argparse
 import ctypes
@@ -22,11 +28,14 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rigid-ipc_amd"))
-sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "time steps/sec + CCD candidate pairs/sec at 1k bodies; batched vs plain GPU"
-HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# FP64 add/sub/mul/div/sqrt of one interval inclusion-function evaluation
+# (rigid edge-edge / face-vertex trajectory box), counted with the
+# op-counting build of the CPU restatement over the recorded CCD queries;
+# tests/test_bench_constants.py recomputes it.
+FLOPS_PER_BOX = 1015.7
 
 
 def log(msg):
@@ -49,42 +58,101 @@ def ballistic_steps(sc, n, dt=0.01):
     return out
 
 
-def flops_per_box():
-    """FP64 add/sub/mul/div/sqrt of one inclusion-function evaluation,
-    counted by the op-counting oracle build over the reference's recorded
-    queries' start boxes (EE and FV averaged by their share)."""
-    so = os.path.join(ROOT, "oracle", "_build", "liboracle_flops.so")
-    if not os.path.exists(so):
+def ccd_roofline(evals, narrow_ms, calls):
+    if not narrow_ms or not evals:
         return None
-    L = ctypes.CDLL(so)
-    dp = ctypes.POINTER(ctypes.c_double)
-    L.oracle_flops_get.restype = ctypes.c_long
-    L.oracle_ee_aabb.argtypes = [dp, dp, dp]
-    L.oracle_fv_aabb.argtypes = [dp, dp, dp]
-    g = np.load(os.path.join(ROOT, "tests", "golden", "ccd_queries.npz"))
-    counts = []
-    out = np.zeros(6)
-    rng = np.random.default_rng(0)
-    for kind, q in zip(g["kind"], g["q"]):
-        q = np.ascontiguousarray(q)
-        for _ in range(4):
-            lev = rng.integers(0, 12)
-            i = rng.integers(0, 2 ** lev)
-            w = 2.0 ** -lev
-            box = np.array([i * w, (i + 1) * w, 0, 0.5, 0.25, 0.5])
-            L.oracle_flops_reset()
-            (L.oracle_ee_aabb if kind == 0 else L.oracle_fv_aabb)(
-                q.ctypes.data_as(dp), box.ctypes.data_as(dp), out.ctypes.data_as(dp))
-            counts.append(L.oracle_flops_get())
-    return float(np.mean(counts))
+    achieved = evals * FLOPS_PER_BOX / (narrow_ms * 1e-3) / 1e12
+    return dict(bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
+                traffic=None, kernel="ccd_kernel", flops_per_box=FLOPS_PER_BOX, box_evaluations=int(evals),
+                kernel_ms_total=narrow_ms, launches=int(calls), kernel_ms_avg=narrow_ms / max(calls, 1))
 
 
-def cpu_baseline(sc, steps, budget_s=20.0):
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+# ------------------------------------------------------------------ sim
+def cpu_baseline_sim(sc, budget_s):
+    """The same stepper over the oracle backend, from the initial state, for
+    as many steps as fit in budget_s (at least one)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from rigidipc_amd import sim as M
+    L = O.lib()
+    L.oracle_backend.argtypes = [ctypes.POINTER(M.Backend)]
+    L.oracle_backend.restype = None
+    be = M.Backend()
+    L.oracle_backend(ctypes.byref(be))
+    sim = M.Simulation(sc, backend=ctypes.pointer(be))
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        sim.step(1)
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    st = sim.stats()
+    sim.close()
+    return n, dt, st
+
+
+def bench_sim(args):
+    from rigidipc_amd import scenes as S
+    from rigidipc_amd import sim as M
+    sc = S.config2(seed=0)
+    if args.warmup:
+        w = M.Simulation(sc)
+        w.step(args.warmup)
+        w.close()
+    sim = M.Simulation(sc)
+    per_step = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        sim.step(1)
+        per_step.append(time.perf_counter() - t0)
+    wall = time.perf_counter() - t_all
+    st = sim.stats()
+    sim.close()
+    ccd_dev_ms = st["ccd_broad_phase_ms"] + st["ccd_narrow_phase_ms"]
+    line = dict(
+        metric=METRIC, value=args.steps / wall, unit="time steps/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f64", data="synthetic",
+        config=dict(
+            workload="config2 time stepping: 1000 mixed bodies (ico2 / torus 24x12 / cube) + static plane, "
+                     "dt 0.01, dhat 1e-3, mu 0.2, implicit-Euler IPC Newton on the host, contact terms on the device",
+            bodies=int(len(sc.group_ids)), vertices=int(len(sc.vertices)), faces=int(len(sc.faces)),
+            newton_iterations=int(st["newton_iterations"]), line_search_iterations=int(st["line_search_iterations"]),
+            ccd_calls=int(st["ccd_calls"]), constraint_builds=int(st["constraint_builds"]),
+            constraints_last=int(st["num_constraints"]), failed_solves=int(st["failed_solves"]),
+            contact_ms_per_step=st["contact_ms"] / args.steps, host_ms_per_step=st["host_ms"] / args.steps,
+            ccd_candidate_pairs_per_s=(st["ccd_candidates"] / (ccd_dev_ms * 1e-3)) if ccd_dev_ms > 0 else None,
+            ccd_candidate_pairs_per_s_wall=(st["ccd_candidates"] / (st["ccd_ms"] * 1e-3)) if st["ccd_ms"] > 0 else None,
+            ccd_candidates=int(st["ccd_candidates"]), step_ms=[round(1e3 * t, 3) for t in per_step]),
+        roofline=ccd_roofline(st["ccd_box_evaluations"], st["ccd_narrow_phase_ms"], st["ccd_calls"]),
+    )
+    if not args.no_cpu:
+        n, dt, cst = cpu_baseline_sim(sc, args.cpu_budget)
+        gpu_same = sum(per_step[:n]) if n <= len(per_step) else None
+        line["cpu_baseline"] = dict(
+            value=n / dt, unit="time steps/s", cores=cpu_threads(), kind="port",
+            sample=f"first {n} time steps of the same scene from the same initial state, same host stepper over "
+                   f"the oracle backend (OpenMP, {cpu_threads()} threads): {dt:.2f} s; device path over the "
+                   f"same {n} steps: " + (f"{gpu_same:.3f} s" if gpu_same is not None else "n/a")
+                   + f"; CPU newton iterations {int(cst['newton_iterations'])}, CCD candidates "
+                   f"{int(cst['ccd_candidates'])}")
+    return line
+
+
+# ------------------------------------------------------------------ ccd
+def cpu_baseline_ccd(sc, steps, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     O.set_scene(sc)
     t0 = time.perf_counter()
-    ncand = 0
-    done = 0
+    ncand = done = 0
     for p0, p1 in steps:
         _, n = O.earliest_toi(p0, p1, 0.0)
         ncand += n
@@ -92,36 +160,21 @@ def cpu_baseline(sc, steps, budget_s=20.0):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return dict(value=ncand / dt, unit="CCD candidate pairs/s", cores=os.cpu_count(), kind="port",
-                sample=f"{done} CCD calls (broad + narrow) of the same sequence, OpenMP "
-                       f"{os.environ.get('OMP_NUM_THREADS', os.cpu_count())} threads; "
+    return dict(value=ncand / dt, unit="CCD candidate pairs/s", cores=cpu_threads(), kind="port",
+                sample=f"{done} CCD calls (broad + narrow) of the same sequence, OpenMP {cpu_threads()} threads; "
                        f"{done / dt:.3f} calls/s")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ccd", choices=["ccd"])
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the scaling design is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
+def bench_ccd(args):
     import rigidipc_amd
     from rigidipc_amd import scenes as S
-
     sc = S.config2(seed=0)
     seq = ballistic_steps(sc, args.warmup + args.steps)
     ctx = rigidipc_amd.Context(0)
     ctx.set_scene(sc)
     for p0, p1 in seq[:args.warmup]:
         ctx.compute_earliest_toi(p0, p1)
-    tot_cand = 0
-    tot_evals = 0
+    tot_cand = tot_evals = 0
     bp_ms = np_ms = 0.0
     t0 = time.perf_counter()
     for p0, p1 in seq[args.warmup:]:
@@ -132,26 +185,40 @@ def main():
         bp_ms += st["broad_phase_ms"]
         np_ms += st["narrow_phase_ms"]
     wall = time.perf_counter() - t0
-    ms = 1e3 * wall / args.steps
-    fbox = flops_per_box()
-    roof = None
-    if fbox and np_ms > 0:
-        achieved = tot_evals * fbox / (np_ms * 1e-3) / 1e12
-        roof = dict(bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP64_PEAK_TFLOPS, traffic=None, kernel="ccd_kernel",
-                    flops_per_box=fbox, box_evaluations=tot_evals, kernel_ms_total=np_ms)
+    ctx.close()
     line = dict(
         metric=METRIC, value=tot_cand / wall, unit="CCD candidate pairs/s", n_gpus=1, steps=args.steps,
-        warmup=args.warmup, ms_per_step=ms, higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f64", data="synthetic",
+        warmup=args.warmup, ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f64", data="synthetic",
         config=dict(workload="config2 ballistic CCD sequence: 1000 mixed bodies + plane, one "
                              "rigc_compute_earliest_toi per step (poses_t -> inertial prediction)",
                     bodies=int(len(sc.group_ids)), candidates_per_step=tot_cand / args.steps,
                     broad_phase_ms_per_step=bp_ms / args.steps, narrow_phase_ms_per_step=np_ms / args.steps),
-        roofline=roof,
+        roofline=ccd_roofline(tot_evals, np_ms, args.steps),
     )
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sc, seq[args.warmup:])
+        line["cpu_baseline"] = cpu_baseline_ccd(sc, seq[args.warmup:], args.cpu_budget)
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="sim", choices=["sim", "ccd"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, not part of the build)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = 50 if args.workload == "sim" else 40
+    if args.warmup is None:
+        args.warmup = 2 if args.workload == "sim" else 3
+    line = bench_sim(args) if args.workload == "sim" else bench_ccd(args)
     print(json.dumps(line), flush=True)
 
 

@@ -160,20 +160,23 @@ rigc_status rigc_get_constraint_set(rigc_ctx* ctx, int64_t capacity, int32_t* re
 rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double dhat, double dmin, int32_t flags,
                                       double* energy, double* grad, rigc_block_hessian* hess);
 
-/* sqrt of the minimum squared distance over the constraint set built at
- * `poses` (+inf when the set is empty). Replaces
+/* sqrt of the minimum squared distance over the constraint set (+inf when
+ * the set is empty), the set being rebuilt at `poses` first when flags
+ * holds RIGC_BUILD_CONSTRAINTS. Replaces
  * DistanceBarrierConstraint::compute_minimum_distance
  * (src/opt/distance_barrier_constraint.cpp:308-323). */
 rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
-                                          double* min_distance);
+                                          int32_t flags, double* min_distance);
 
-/* Lagged friction set: the constraint set is rebuilt at `poses`, then each
- * constraint's closest point, tangent basis and normal force
- * -kappa b'(d) 2 sqrt(d) are frozen (ipc::construct_friction_constraint_set
- * as called by DistanceBarrierRBProblem::update_friction_constraints,
- * src/problems/distance_barrier_rb_problem.cpp:168-188). */
+/* Lagged friction set from the constraint set (rebuilt at `poses` first
+ * when flags holds RIGC_BUILD_CONSTRAINTS): each constraint's closest
+ * point, tangent basis and normal force -kappa b'(d) 2 sqrt(d) at `poses`
+ * are frozen (ipc::construct_friction_constraint_set as called by
+ * DistanceBarrierRBProblem::update_friction_constraints,
+ * src/problems/distance_barrier_rb_problem.cpp:168-188). The friction set
+ * survives later rebuilds of the constraint set. */
 rigc_status rigc_update_friction_constraints(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
-                                             double barrier_stiffness, double mu, int64_t* count);
+                                             double barrier_stiffness, double mu, int32_t flags, int64_t* count);
 
 /* Copy of the friction set: records [n][9] = normal force, closest-point
  * coordinates (2), tangent basis (3x2 row-major), in constraint-set order. */

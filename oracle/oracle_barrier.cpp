@@ -74,20 +74,20 @@ using V12 = V3<12>;
 
 // ------------------------------------------------------------------------
 // Distances (squared) as formulas of the vertices (ipc-toolkit restated).
-D12 point_point(const V12& p, const V12& q) { V12 d = vsub(p, q); return vdot(d, d); }
-D12 point_line(const V12& p, const V12& e0, const V12& e1) {
-    V12 c = vcross(vsub(e0, p), vsub(e1, p));
-    V12 e = vsub(e1, e0);
+template <int N> D2<N> point_point(const V3<N>& p, const V3<N>& q) { V3<N> d = vsub(p, q); return vdot(d, d); }
+template <int N> D2<N> point_line(const V3<N>& p, const V3<N>& e0, const V3<N>& e1) {
+    V3<N> c = vcross(vsub(e0, p), vsub(e1, p));
+    V3<N> e = vsub(e1, e0);
     return vdot(c, c) / vdot(e, e);
 }
-D12 point_plane(const V12& p, const V12& t0, const V12& t1, const V12& t2) {
-    V12 n = vcross(vsub(t1, t0), vsub(t2, t0));
-    D12 s = vdot(vsub(p, t0), n);
+template <int N> D2<N> point_plane(const V3<N>& p, const V3<N>& t0, const V3<N>& t1, const V3<N>& t2) {
+    V3<N> n = vcross(vsub(t1, t0), vsub(t2, t0));
+    D2<N> s = vdot(vsub(p, t0), n);
     return (s * s) / vdot(n, n);
 }
-D12 line_line(const V12& a0, const V12& a1, const V12& b0, const V12& b1) {
-    V12 n = vcross(vsub(a1, a0), vsub(b1, b0));
-    D12 s = vdot(vsub(b0, a0), n);
+template <int N> D2<N> line_line(const V3<N>& a0, const V3<N>& a1, const V3<N>& b0, const V3<N>& b1) {
+    V3<N> n = vcross(vsub(a1, a0), vsub(b1, b0));
+    D2<N> s = vdot(vsub(b0, a0), n);
     return (s * s) / vdot(n, n);
 }
 
@@ -171,6 +171,25 @@ EEType edge_edge_type(const double* ea0, const double* ea1, const double* eb0, c
     return def;
 }
 
+// Symmetric 2x2 solve as Eigen's LDLT (3.3, ldlt().solve) does it: the larger
+// diagonal entry is pivoted first, a zero pivot keeps its column
+// unscaled, and D is pseudo-inverted (components of zero pivots are 0).
+// ipc-toolkit's closest-point coordinates use this solve.
+void solve2_ldlt(double a00, double a01, double a11, double b0, double b1, double& x0, double& x1) {
+    const bool swap = std::fabs(a11) > std::fabs(a00);
+    if (swap) { std::swap(a00, a11); std::swap(b0, b1); }
+    const double d0 = a00;
+    const double l10 = d0 != 0.0 ? a01 / d0 : a01;
+    const double d1 = a11 - l10 * (d0 * l10);
+    const double y0 = b0, y1 = b1 - l10 * y0;
+    const double tiny = 2.2250738585072014e-308;  // numeric_limits<double>::min()
+    const double z0 = std::fabs(d0) > tiny ? y0 / d0 : 0.0;
+    const double z1 = std::fabs(d1) > tiny ? y1 / d1 : 0.0;
+    const double w1 = z1, w0 = z0 - l10 * w1;
+    x0 = swap ? w1 : w0;
+    x1 = swap ? w0 : w1;
+}
+
 // 2x2 solve of the projection onto (r0, r1).
 void solve2(double m00, double m01, double m11, double b0, double b1, double& x0, double& x1) {
     const double det = m00 * m11 - m01 * m01;
@@ -216,11 +235,14 @@ double barrier_hessian(double d, double dhat) {
 }  // namespace
 
 // Distance (squared) of a constraint with its type recomputed at V, as a
-// dual number over the constraint's (up to 4) vertices.
-static D12 constraint_distance(const Constraint& c, const double* X /*4x3*/) {
-    V12 P[4];
+// dual number over the constraint's (up to 4) vertices (N = 12), or its
+// value only (N = 1, no variables seeded).
+template <int N>
+static D2<N> constraint_distance_t(const Constraint& c, const double* X /*4x3*/) {
+    V3<N> P[4];
     for (int i = 0; i < 4; i++)
-        for (int k = 0; k < 3; k++) P[i].c[k] = D12::var(X[3 * i + k], 3 * i + k);
+        for (int k = 0; k < 3; k++)
+            P[i].c[k] = (N == 12) ? D2<N>::var(X[3 * i + k], (3 * i + k) % N) : D2<N>(X[3 * i + k]);
     const double* x0 = X;
     const double* x1 = X + 3;
     const double* x2 = X + 6;
@@ -259,6 +281,21 @@ static D12 constraint_distance(const Constraint& c, const double* X /*4x3*/) {
         }
     }
     }
+}
+
+static D12 constraint_distance(const Constraint& c, const double* X) { return constraint_distance_t<12>(c, X); }
+static double constraint_distance_value(const Constraint& c, const double* X) { return constraint_distance_t<1>(c, X).v; }
+
+// edge_edge_mollifier(ea0, ea1, eb0, eb1, eps_x) value.
+static double mollifier_value(const double* X, double eps_x) {
+    double ea[3], eb[3], n[3];
+    for (int k = 0; k < 3; k++) { ea[k] = X[3 + k] - X[k]; eb[k] = X[9 + k] - X[6 + k]; }
+    n[0] = ea[1] * eb[2] - ea[2] * eb[1];
+    n[1] = ea[2] * eb[0] - ea[0] * eb[2];
+    n[2] = ea[0] * eb[1] - ea[1] * eb[0];
+    const double x = (n[0] * n[0] + n[1] * n[1]) + n[2] * n[2];
+    if (x >= eps_x) return 1.0;
+    return (-x / eps_x + 2.0) * x / eps_x;
 }
 
 // edge_edge_mollifier(ea0, ea1, eb0, eb1, eps_x) as a dual number.
@@ -538,7 +575,7 @@ void construct_constraint_set(const Scene& s, const double* poses, double dhat, 
         constraint_vertices(s, probe);
         double X[12];
         gather(W, probe, X);
-        const double d = constraint_distance(probe, X).v;
+        const double d = constraint_distance_value(probe, X);
         if (!(d < offset_sqr)) continue;
         // eps_x from V_rest = V (distance_barrier_constraint.cpp:297-299)
         double ua[3], ub[3], n[3];
@@ -564,7 +601,7 @@ void construct_constraint_set(const Scene& s, const double* poses, double dhat, 
         constraint_vertices(s, probe);
         double X[12];
         gather(W, probe, X);
-        const double d = constraint_distance(probe, X).v;
+        const double d = constraint_distance_value(probe, X);
         if (!(d < offset_sqr)) continue;
         switch (point_triangle_type(&W[3 * v], &W[3 * f0], &W[3 * f1], &W[3 * f2])) {
         case P_T0: add_vv(v, f0); break;
@@ -590,6 +627,22 @@ double barrier_term(const Scene& s, const double* poses, const std::vector<Const
     const double dh = 2 * dmin * dhat + dhat * dhat, dm2 = dmin * dmin;
     const size_t n = C.size();
     std::vector<double> energy(n);
+    if (!grad && !local_hess) {
+        // value only (compute_potential)
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long ci = 0; ci < (long)n; ci++) {
+            const Constraint& c = C[ci];
+            double X[12];
+            gather(W, c, X);
+            const double d = constraint_distance_value(c, X);
+            double b = barrier(d - dm2, dh);
+            if (c.type == EE && c.eps_x >= 0) b = mollifier_value(X, c.eps_x) * b;
+            energy[ci] = (double)c.multiplicity * b;
+        }
+        double e = 0;
+        for (size_t ci = 0; ci < n; ci++) e += energy[ci];
+        return e;
+    }
     std::vector<std::array<double, 12>> lgrad(n);
     if (local_hess) local_hess->assign(n, std::array<double, 144>{});
 #pragma omp parallel for schedule(dynamic, 8)
@@ -647,7 +700,7 @@ double minimum_distance(const Scene& s, const double* poses, const std::vector<C
     for (const auto& c : C) {
         double X[12];
         gather(W, c, X);
-        m = std::min(m, constraint_distance(c, X).v);
+        m = std::min(m, constraint_distance_value(c, X));
     }
     return m;
 }
@@ -691,7 +744,7 @@ void friction_frame(const Constraint& c, const double* X, double coords[2], doub
         double ea[3], eb[3], ba[3];
         sub3(x1, x0, ea); sub3(x3, x2, eb); sub3(x0, x2, ba);
         double al, be;
-        solve2(dot3(ea, ea), -dot3(ea, eb), dot3(eb, eb), -dot3(ba, ea), dot3(ba, eb), al, be);
+        solve2_ldlt(dot3(ea, ea), -dot3(ea, eb), dot3(eb, eb), -dot3(ba, ea), dot3(ba, eb), al, be);
         coords[0] = al; coords[1] = be;
         cross3(ea, eb, n);
         if (dot3(n, n) == 0) { const double ez[3] = { 0, 0, 1 }; cross3(ea, ez, n); }
@@ -704,7 +757,7 @@ void friction_frame(const Constraint& c, const double* X, double coords[2], doub
         double e0[3], e1[3], d[3];
         sub3(x2, x1, e0); sub3(x3, x1, e1); sub3(x0, x1, d);
         double b1, b2;
-        solve2(dot3(e0, e0), dot3(e0, e1), dot3(e1, e1), dot3(e0, d), dot3(e1, d), b1, b2);
+        solve2_ldlt(dot3(e0, e0), dot3(e0, e1), dot3(e1, e1), dot3(e0, d), dot3(e1, d), b1, b2);
         coords[0] = b1; coords[1] = b2;
         cross3(e0, e1, n);
         for (int k = 0; k < 3; k++) c0[k] = e0[k];
@@ -735,7 +788,7 @@ void construct_friction_constraint_set(const Scene& s, const double* poses, cons
         f.c = c;
         double X[12];
         gather(W, c, X);
-        const double d = constraint_distance(c, X).v;
+        const double d = constraint_distance_value(c, X);
         double w[4];
         friction_frame(c, X, f.coords, f.basis, w);
         // compute_normal_force_magnitude: -kappa * b'(d) * 2 sqrt(d)
@@ -751,6 +804,7 @@ double friction_term(const Scene& s, const double* poses, const double* poses_t0
     const std::vector<double> W1 = all_world_vertices(s, poses), W0 = all_world_vertices(s, poses_t0);
     const size_t n = FC.size();
     std::vector<double> energy(n);
+    const bool value_only = !grad && !local_hess;
     std::vector<std::array<double, 12>> lgrad(n);
     if (local_hess) local_hess->assign(n, std::array<double, 144>{});
 #pragma omp parallel for schedule(dynamic, 8)
@@ -782,6 +836,7 @@ double friction_term(const Scene& s, const double* poses, const double* poses_t0
         const double nu = std::sqrt(u[0] * u[0] + u[1] * u[1]);
         const double scale = (double)c.multiplicity * f.mu * f.normal_force;
         energy[ci] = scale * f0_SF(nu, eh);
+        if (value_only) continue;
         const double f1x = f1_SF_over_x(nu, eh);
         double g[12], H[144];
         for (int r = 0; r < nn; r++) g[r] = scale * f1x * (T[r][0] * u[0] + T[r][1] * u[1]);
@@ -803,7 +858,7 @@ double friction_term(const Scene& s, const double* poses, const double* poses_t0
     double e = 0;
     for (size_t ci = 0; ci < n; ci++) {
         e += energy[ci];
-        if (grad)
+        if (grad && !value_only)
             for (int k = 0; k < 12; k++) grad[6 * FC[ci].c.body[k / 6] + k % 6] += lgrad[ci][k];
     }
     return e;

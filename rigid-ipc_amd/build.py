@@ -38,6 +38,25 @@ def compile_one(src):
     return obj, r.stderr
 
 
+HOST = os.path.join(HERE, "host")
+SIM_OUT = os.path.join(HERE, "librigc_sim.so")
+
+
+def build_host(verbose=False):
+    """The host time stepper (host/simulation.cpp) as librigc_sim.so,
+    linked against librigc.so (found next to it at run time)."""
+    tmp = SIM_OUT + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-shared", os.path.join(HOST, "simulation.cpp"), "-L" + HERE, "-lrigc",
+                             "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for host/simulation.cpp:\n{r.stderr}")
+    if verbose and r.stderr.strip():
+        print(r.stderr)
+    os.replace(tmp, SIM_OUT)
+    return SIM_OUT
+
+
 def build(jobs=8, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -54,6 +73,7 @@ def build(jobs=8, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_host(verbose)
     return OUT
 
 

@@ -522,7 +522,7 @@ rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double
 }
 
 rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
-                                          double* min_distance) {
+                                          int32_t flags, double* min_distance) {
     if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_minimum_distance: no scene");
     if (!poses || !min_distance || !(dhat > 0) || !(dmin >= 0))
@@ -530,8 +530,11 @@ rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, do
     RIGC_HIP(hipSetDevice(ctx->device));
     rigc_status s = upload_poses(ctx, poses, nullptr);
     if (s != RIGC_OK) return s;
-    RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, ctx->num_cus,
-                                  ctx->stream));
+    if (flags & RIGC_BUILD_CONSTRAINTS)
+        RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin,
+                                      ctx->num_cus, ctx->stream));
+    if (!ctx->as.have_cons)
+        return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_minimum_distance: no constraint set (build one first)");
     double m2 = HUGE_VAL;
     RIGC_HIP(rigc::minimum_distance(ctx->as, ctx->scene, ctx->poses0.as<double>(), m2, ctx->stream));
     *min_distance = std::sqrt(m2);
@@ -539,7 +542,7 @@ rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, do
 }
 
 rigc_status rigc_update_friction_constraints(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
-                                             double barrier_stiffness, double mu, int64_t* count) {
+                                             double barrier_stiffness, double mu, int32_t flags, int64_t* count) {
     if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_update_friction_constraints: no scene");
     if (!poses || !(dhat > 0) || !(dmin >= 0) || !(barrier_stiffness >= 0) || !(mu >= 0))
@@ -547,8 +550,11 @@ rigc_status rigc_update_friction_constraints(rigc_ctx* ctx, const double* poses,
     RIGC_HIP(hipSetDevice(ctx->device));
     rigc_status s = upload_poses(ctx, poses, nullptr);
     if (s != RIGC_OK) return s;
-    RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, ctx->num_cus,
-                                  ctx->stream));
+    if (flags & RIGC_BUILD_CONSTRAINTS)
+        RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin,
+                                      ctx->num_cus, ctx->stream));
+    if (!ctx->as.have_cons)
+        return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_update_friction_constraints: no constraint set (build one first)");
     RIGC_HIP(build_friction_set(ctx->as, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, barrier_stiffness, mu,
                                 ctx->stream));
     RIGC_HIP(hipStreamSynchronize(ctx->stream));

@@ -146,6 +146,27 @@ RIGC_HD void solve_2x2(double m00, double m01, double m11, double b0, double b1,
     x1 = (m00 * b1 - m01 * b0) / det;
 }
 
+// Symmetric 2x2 solve in the manner of an LDL^T with diagonal pivoting:
+// the larger diagonal entry first, D pseudo-inverted (a zero pivot gives a
+// zero component), as ipc-toolkit's closest-point coordinates (Eigen LDLT).
+RIGC_HD void solve_2x2_ldlt(double a00, double a01, double a11, double b0, double b1, double& x0, double& x1) {
+    const bool swap = fabs(a11) > fabs(a00);
+    if (swap) {
+        const double t = a00; a00 = a11; a11 = t;
+        const double u = b0; b0 = b1; b1 = u;
+    }
+    const double d0 = a00;
+    const double l10 = d0 != 0.0 ? a01 / d0 : a01;
+    const double d1 = a11 - l10 * (d0 * l10);
+    const double y0 = b0, y1 = b1 - l10 * y0;
+    const double tiny = 2.2250738585072014e-308;
+    const double z0 = fabs(d0) > tiny ? y0 / d0 : 0.0;
+    const double z1 = fabs(d1) > tiny ? y1 / d1 : 0.0;
+    const double w1 = z1, w0 = z0 - l10 * w1;
+    x0 = swap ? w1 : w0;
+    x1 = swap ? w0 : w1;
+}
+
 RIGC_HD int point_triangle_type(const double* p, const double* t0, const double* t1, const double* t2) {
     double e01[3], e02[3], n[3];
     v3_sub(t1, t0, e01);
@@ -515,8 +536,8 @@ RIGC_HD void friction_frame(int type, const double* X, double coords[2], double 
         v3_sub(x1, x0, ea);
         v3_sub(x3, x2, eb);
         v3_sub(x0, x2, ba);
-        solve_2x2(v3_dot(ea, ea), -v3_dot(ea, eb), v3_dot(eb, eb), -v3_dot(ba, ea), v3_dot(ba, eb), coords[0],
-                  coords[1]);
+        solve_2x2_ldlt(v3_dot(ea, ea), -v3_dot(ea, eb), v3_dot(eb, eb), -v3_dot(ba, ea), v3_dot(ba, eb),
+                       coords[0], coords[1]);
         v3_cross(ea, eb, n);
         if (v3_dot(n, n) == 0) {
             const double ez[3] = { 0, 0, 1 };
@@ -531,8 +552,8 @@ RIGC_HD void friction_frame(int type, const double* X, double coords[2], double 
         v3_sub(x2, x1, e0);
         v3_sub(x3, x1, e1);
         v3_sub(x0, x1, d);
-        solve_2x2(v3_dot(e0, e0), v3_dot(e0, e1), v3_dot(e1, e1), v3_dot(e0, d), v3_dot(e1, d), coords[0],
-                  coords[1]);
+        solve_2x2_ldlt(v3_dot(e0, e0), v3_dot(e0, e1), v3_dot(e1, e1), v3_dot(e0, d), v3_dot(e1, d), coords[0],
+                       coords[1]);
         v3_cross(e0, e1, n);
         for (int k = 0; k < 3; k++) c0[k] = e0[k];
         v3_cross(n, e0, c1);

@@ -1,0 +1,126 @@
+/*
+ * rigs — host time stepper of Rigid IPC over a contact backend.
+ *
+ * The Newton solve stays on the host, as in the reference: this layer
+ * restates DistanceBarrierRBProblem's simulation step
+ * (src/problems/distance_barrier_rb_problem.cpp:118-626: update_constraints,
+ * solve_constraints with lagged friction, take_step with the implicit-Euler
+ * velocity update, compute_objective = E / m + kappa / m B + D / m), the
+ * IPC solver's adaptive barrier stiffness (src/solvers/ipc_solver.cpp) and
+ * the Newton solver with its CCD-filtered backtracking line search and
+ * Tikhonov regularisation (src/solvers/newton_solver.cpp:160-520). The
+ * contact terms (CCD, constraint set, barrier, friction, minimum distance)
+ * go through a rigs_backend: by default the device library (include/rigc.h);
+ * any other implementation of the same table can be passed in.
+ *
+ * Layout: poses / velocities [num_bodies][6] (position, rotation vector;
+ * linear velocity in world coordinates, angular velocity in the body's
+ * principal frame), inertia [num_bodies][3] principal moments.
+ */
+#ifndef RIGS_H
+#define RIGS_H
+
+#include <stdint.h>
+
+#include "../../include/rigc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Contact backend: the rigc entry points with the context as `self`. */
+typedef struct rigs_backend {
+    void* self;
+    rigc_status (*set_scene)(void* self, const rigc_scene* scene);
+    rigc_status (*compute_earliest_toi)(void* self, const double* poses_t0, const double* poses_t1,
+                                        double minimum_separation_distance, double* earliest_toi);
+    rigc_status (*construct_constraint_set)(void* self, const double* poses, double dhat, double dmin,
+                                            int64_t* counts);
+    rigc_status (*compute_barrier_term)(void* self, const double* poses, double dhat, double dmin, int32_t flags,
+                                        double* energy, double* grad, rigc_block_hessian* hess);
+    rigc_status (*compute_minimum_distance)(void* self, const double* poses, double dhat, double dmin,
+                                            int32_t flags, double* min_distance);
+    rigc_status (*update_friction_constraints)(void* self, const double* poses, double dhat, double dmin,
+                                               double barrier_stiffness, double mu, int32_t flags, int64_t* count);
+    rigc_status (*compute_friction_term)(void* self, const double* poses, const double* poses_t0,
+                                         double epsv_times_h, int32_t flags, double* energy, double* grad,
+                                         rigc_block_hessian* hess);
+    const char* (*last_error)(void* self);
+    /* Optional (may be NULL): counters of the last call (rigc_get_stats). */
+    rigc_status (*get_stats)(void* self, rigc_stats* stats);
+} rigs_backend;
+
+/* Simulation settings; rigs_default_params fills the reference defaults
+ * (src/SimState.cpp:130-208, src/constants.hpp). */
+typedef struct rigs_params {
+    double timestep;
+    double gravity[3];
+    double dhat;                       /* initial_barrier_activation_distance */
+    double minimum_separation_distance;
+    double coefficient_friction;
+    int32_t friction_iterations;
+    double static_friction_speed_bound;
+    double velocity_conv_tol;          /* relative to the world bbox diagonal */
+    double line_search_lower_bound;
+    int32_t max_iterations;
+    double dhat_epsilon;
+    double min_barrier_stiffness_scale;
+} rigs_params;
+
+typedef struct rigs_bodies {
+    int32_t num_bodies;
+    const double* poses;
+    const double* velocities;
+    const double* mass;
+    const double* inertia;
+} rigs_bodies;
+
+typedef struct rigs_stats {
+    int64_t steps;
+    int64_t newton_iterations;
+    int64_t line_search_iterations;
+    int64_t ccd_calls;
+    int64_t constraint_builds;
+    int64_t objective_evaluations;
+    int64_t regularizations;
+    int64_t failed_solves;
+    double contact_ms;   /* time inside backend calls */
+    double host_ms;      /* body energy, assembly of the Newton system, LDL^T */
+    double total_ms;
+    double kappa;        /* barrier stiffness of the last step */
+    double min_distance; /* after the last step (-1: no constraints) */
+    int64_t num_constraints; /* of the last constraint-set build */
+    /* CCD calls (line-search filter): candidates, host wall time around the
+     * call, and, when the backend reports them, device broad / narrow phase
+     * time and interval box evaluations. */
+    int64_t ccd_candidates;
+    double ccd_ms;
+    double ccd_broad_phase_ms;
+    double ccd_narrow_phase_ms;
+    int64_t ccd_box_evaluations;
+} rigs_stats;
+
+typedef struct rigs_sim rigs_sim;
+
+void rigs_default_params(rigs_params* params);
+
+/* backend NULL: the device library on HIP device `device`. */
+rigc_status rigs_create(rigs_sim** sim, const rigs_backend* backend, int32_t device, const rigc_scene* scene,
+                        const rigs_bodies* bodies, const rigs_params* params);
+rigc_status rigs_destroy(rigs_sim* sim);
+const char* rigs_last_error(const rigs_sim* sim);
+/* Advance num_steps time steps (DistanceBarrierRBProblem::simulation_step). */
+rigc_status rigs_step(rigs_sim* sim, int32_t num_steps);
+rigc_status rigs_get_state(const rigs_sim* sim, double* poses, double* velocities);
+rigc_status rigs_get_stats(const rigs_sim* sim, rigs_stats* stats);
+
+/* Sparse block LDL^T solve used for the Newton system (exposed for tests):
+ * n bodies, diag [n][36], m pairs (i != j) with blocks (i,j), rhs b [6n]. */
+rigc_status rigs_block_ldlt_solve(int32_t n, const double* diag, int64_t m, const int32_t* pairs,
+                                  const double* off, const double* b, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIGS_H */

@@ -1,0 +1,794 @@
+// Host time stepper (rigs.h): DistanceBarrierRBProblem's simulation step,
+// the IPC solver's barrier-stiffness logic and the Newton solver, over a
+// contact backend.
+//
+//   simulation_step / update_constraints / solve_constraints / take_step
+//                            src/problems/distance_barrier_rb_problem.cpp:118-626
+//   compute_objective        :633-703
+//   compute_energy_term / compute_body_energy (IMPLICIT_EULER)  :706-944
+//   IPCSolver::init_solve / post_step_update  src/solvers/ipc_solver.cpp:35-88
+//   NewtonSolver::solve / line_search / compute_regularized_direction /
+//   converged                src/solvers/newton_solver.cpp:95-520
+//   free_dof                 distance_barrier_rb_problem.cpp:100-110
+//   constraint-set cache     src/opt/distance_barrier_constraint.cpp:272-281
+// Adaptive stiffness restates ipc-toolkit @d41d6ca's
+// initial_barrier_stiffness / update_barrier_stiffness (not in the
+// reference tree).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../csrc/rigc_barrier.hpp"
+#include "block_ldlt.hpp"
+#include "rigs.h"
+
+using rigs::Block;
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------ device backend
+rigc_status gb_set_scene(void* s, const rigc_scene* sc) { return rigc_set_scene((rigc_ctx*)s, sc); }
+rigc_status gb_toi(void* s, const double* p0, const double* p1, double msd, double* toi) {
+    return rigc_compute_earliest_toi((rigc_ctx*)s, p0, p1, msd, toi);
+}
+rigc_status gb_build(void* s, const double* p, double dhat, double dmin, int64_t* counts) {
+    return rigc_construct_constraint_set((rigc_ctx*)s, p, dhat, dmin, counts);
+}
+rigc_status gb_barrier(void* s, const double* p, double dhat, double dmin, int32_t flags, double* e, double* g,
+                       rigc_block_hessian* h) {
+    return rigc_compute_barrier_term((rigc_ctx*)s, p, dhat, dmin, flags, e, g, h);
+}
+rigc_status gb_mind(void* s, const double* p, double dhat, double dmin, int32_t flags, double* out) {
+    return rigc_compute_minimum_distance((rigc_ctx*)s, p, dhat, dmin, flags, out);
+}
+rigc_status gb_fupdate(void* s, const double* p, double dhat, double dmin, double kappa, double mu, int32_t flags,
+                       int64_t* n) {
+    return rigc_update_friction_constraints((rigc_ctx*)s, p, dhat, dmin, kappa, mu, flags, n);
+}
+rigc_status gb_friction(void* s, const double* p, const double* p0, double eh, int32_t flags, double* e, double* g,
+                        rigc_block_hessian* h) {
+    return rigc_compute_friction_term((rigc_ctx*)s, p, p0, eh, flags, e, g, h);
+}
+const char* gb_error(void* s) { return rigc_last_error((rigc_ctx*)s); }
+rigc_status gb_stats(void* s, rigc_stats* st) { return rigc_get_stats((rigc_ctx*)s, st); }
+
+struct BodyState {
+    double mass = 0;
+    double inertia[3] = { 0, 0, 0 };
+    double J[3] = { 0, 0, 0 };  // compute_J: 1/2 (-I0 + I1 + I2), ...
+    bool dynamic = false;
+    double pose[6], pose_prev[6], velocity[6];
+    double Qdot[9];
+};
+
+// Symmetric block Hessian in body layout.
+struct BlockHess {
+    std::vector<Block> diag;
+    std::map<std::pair<int, int>, Block> off;  // i < j
+    void reset(int nb) {
+        diag.assign(nb, Block{});
+        off.clear();
+    }
+};
+
+}  // namespace
+
+struct rigs_sim {
+    rigs_backend be{};
+    rigc_ctx* own_ctx = nullptr;
+    rigs_params P{};
+    int nb = 0, nv = 0;
+    std::vector<double> V;             // body-local vertices
+    std::vector<int> vstart;           // body vertex ranges
+    std::vector<BodyState> bodies;
+    std::vector<int> dyn_index;        // body -> free block index (-1 fixed)
+    int n_free = 0;
+    double bbox_diagonal = 0, average_mass = 0;
+    double kappa = 1, max_kappa = 0, prev_min_distance = -1;
+    std::vector<double> poses_t0;      // start of the current step
+    std::vector<double> cached_poses;  // poses of the backend's constraint set
+    bool cache_valid = false;
+    // backend output staging
+    std::vector<double> hdiag, hoff;
+    std::vector<int32_t> hpairs;
+    int64_t hcap = 0;
+    rigs_stats stats{};
+    std::string err;
+    bool verbose = false;  // RIGS_VERBOSE=1: one line per Newton iteration on stderr
+
+    rigc_status fail(rigc_status s, const std::string& m) {
+        err = m;
+        return s;
+    }
+    rigc_status backend_fail(rigc_status s, const char* where) {
+        err = std::string(where) + ": " + (be.last_error ? be.last_error(be.self) : "backend error");
+        return s;
+    }
+
+    // ------------------------------------------------------------ contact calls
+    rigc_status ensure_constraints(const double* poses) {
+        if (cache_valid && std::memcmp(cached_poses.data(), poses, sizeof(double) * 6 * nb) == 0) return RIGC_OK;
+        int64_t counts[4];
+        const double t0 = now_ms();
+        rigc_status s = be.construct_constraint_set(be.self, poses, P.dhat, P.minimum_separation_distance, counts);
+        stats.contact_ms += now_ms() - t0;
+        if (s != RIGC_OK) return backend_fail(s, "construct_constraint_set");
+        stats.constraint_builds++;
+        stats.num_constraints = counts[0] + counts[1] + counts[2] + counts[3];
+        cached_poses.assign(poses, poses + 6 * nb);
+        cache_valid = true;
+        return RIGC_OK;
+    }
+
+    // Calls a potential entry point, growing the off-diagonal capacity once.
+    template <typename F>
+    rigc_status potential(F&& call, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
+                          const char* what) {
+        if (g) g->assign(6 * nb, 0.0);
+        rigc_block_hessian h{};
+        for (int attempt = 0; attempt < 2; attempt++) {
+            if (H) {
+                hdiag.resize(36 * (size_t)nb);
+                hpairs.resize(2 * (size_t)std::max<int64_t>(hcap, 1));
+                hoff.resize(36 * (size_t)std::max<int64_t>(hcap, 1));
+                h.diag = hdiag.data();
+                h.pairs = hpairs.data();
+                h.off = hoff.data();
+                h.capacity = hcap;
+            }
+            const double t0 = now_ms();
+            rigc_status s = call(flags, &e, g ? g->data() : nullptr, H ? &h : nullptr);
+            stats.contact_ms += now_ms() - t0;
+            if (s == RIGC_ERR_CAPACITY && H && h.count > hcap) {
+                hcap = h.count;
+                continue;
+            }
+            if (s != RIGC_OK) return backend_fail(s, what);
+            break;
+        }
+        if (H) {
+            H->reset(nb);
+            for (int b = 0; b < nb; b++) std::copy(&hdiag[36 * b], &hdiag[36 * b] + 36, H->diag[b].begin());
+            for (int64_t k = 0; k < h.count; k++) {
+                Block blk;
+                std::copy(&hoff[36 * k], &hoff[36 * k] + 36, blk.begin());
+                H->off[{ hpairs[2 * k], hpairs[2 * k + 1] }] = blk;
+            }
+        }
+        return RIGC_OK;
+    }
+
+    rigc_status barrier_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H) {
+        rigc_status s = ensure_constraints(x);
+        if (s != RIGC_OK) return s;
+        auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
+            return be.compute_barrier_term(be.self, x, P.dhat, P.minimum_separation_distance, f, pe, pg, ph);
+        };
+        return potential(call, flags, e, g, H, "compute_barrier_term");
+    }
+
+    rigc_status friction_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H) {
+        auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
+            return be.compute_friction_term(be.self, x, poses_t0.data(), P.static_friction_speed_bound * P.timestep,
+                                            f, pe, pg, ph);
+        };
+        return potential(call, flags, e, g, H, "compute_friction_term");
+    }
+
+    rigc_status update_friction(const double* poses) {
+        if (P.coefficient_friction <= 0) return RIGC_OK;
+        rigc_status s = ensure_constraints(poses);
+        if (s != RIGC_OK) return s;
+        int64_t n = 0;
+        const double t0 = now_ms();
+        s = be.update_friction_constraints(be.self, poses, P.dhat, P.minimum_separation_distance, kappa,
+                                           P.coefficient_friction, 0, &n);
+        stats.contact_ms += now_ms() - t0;
+        if (s != RIGC_OK) return backend_fail(s, "update_friction_constraints");
+        return RIGC_OK;
+    }
+
+    // compute_min_distance: sqrt of the minimum, -1 without constraints
+    rigc_status min_distance(const double* x, double& out) {
+        rigc_status s = ensure_constraints(x);
+        if (s != RIGC_OK) return s;
+        const double t0 = now_ms();
+        double m = 0;
+        s = be.compute_minimum_distance(be.self, x, P.dhat, P.minimum_separation_distance, 0, &m);
+        stats.contact_ms += now_ms() - t0;
+        if (s != RIGC_OK) return backend_fail(s, "compute_minimum_distance");
+        out = std::isfinite(m) ? m : -1;
+        return RIGC_OK;
+    }
+
+    rigc_status earliest_toi(const double* x0, const double* x1, double& toi) {
+        const double t0 = now_ms();
+        rigc_status s = be.compute_earliest_toi(be.self, x0, x1, P.minimum_separation_distance, &toi);
+        const double dt = now_ms() - t0;
+        stats.contact_ms += dt;
+        stats.ccd_ms += dt;
+        stats.ccd_calls++;
+        if (s != RIGC_OK) return backend_fail(s, "compute_earliest_toi");
+        if (be.get_stats) {
+            rigc_stats cs{};
+            if (be.get_stats(be.self, &cs) == RIGC_OK) {
+                stats.ccd_candidates += cs.num_candidates;
+                stats.ccd_broad_phase_ms += cs.broad_phase_ms;
+                stats.ccd_narrow_phase_ms += cs.narrow_phase_ms;
+                stats.ccd_box_evaluations += cs.box_evaluations;
+            }
+        }
+        return RIGC_OK;
+    }
+
+    // ------------------------------------------------------------ body energy
+    // compute_body_energy<T> for IMPLICIT_EULER without external forces:
+    //   1/2 m q.q - m q.(q_t0 + h (qdot_t0 + h g))
+    //   + 1/2 tr(Q J Q^T) - tr(Q J (Q_t0 + h Qdot_t0)^T)
+    double energy_term(const double* x, std::vector<double>* g, BlockHess* H) {
+        const double h = P.timestep;
+        double E = 0;
+        if (g) g->assign(6 * nb, 0.0);
+        if (H) H->reset(nb);
+        for (int b = 0; b < nb; b++) {
+            const BodyState& B = bodies[b];
+            if (!B.dynamic) continue;
+            const double* q = x + 6 * b;
+            double target[3];
+            for (int k = 0; k < 3; k++) target[k] = B.pose[k] + h * (B.velocity[k] + h * P.gravity[k]);
+            double qq = 0, qt = 0;
+            for (int k = 0; k < 3; k++) {
+                qq += q[k] * q[k];
+                qt += q[k] * target[k];
+            }
+            E += 0.5 * B.mass * qq - B.mass * qt;
+            double Q[9], Q0[9], Pm[9];
+            rigc::rotation_d(q + 3, Q);
+            rigc::rotation_d(B.pose + 3, Q0);
+            for (int e = 0; e < 9; e++) Pm[e] = Q0[e] + h * B.Qdot[e];
+            double tQJQ = 0, tQJP = 0;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    tQJQ += Q[3 * i + j] * B.J[j] * Q[3 * i + j];
+                    tQJP += Q[3 * i + j] * B.J[j] * Pm[3 * i + j];
+                }
+            E += 0.5 * tQJQ - tQJP;
+            if (!g && !H) continue;
+            rigc::BodyFrame F;
+            rigc::body_frame(q + 3, F);
+            // dE/dQ_ij = J_j (Q_ij - P_ij)
+            double G[9];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) G[3 * i + j] = B.J[j] * (Q[3 * i + j] - Pm[3 * i + j]);
+            if (g) {
+                for (int k = 0; k < 3; k++) (*g)[6 * b + k] = B.mass * q[k] - B.mass * target[k];
+                for (int c = 0; c < 3; c++) {
+                    double s = 0;
+                    for (int e = 0; e < 9; e++) s += F.dR[c][e] * G[e];
+                    (*g)[6 * b + 3 + c] = s;
+                }
+            }
+            if (H) {
+                Block& D = H->diag[b];
+                for (int k = 0; k < 3; k++) D[6 * k + k] = B.mass;
+                for (int c = 0; c < 3; c++)
+                    for (int d = 0; d < 3; d++) {
+                        const double* d2 = F.d2R[rigc::sym_index(c, d)];
+                        double s = 0;
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++)
+                                s += d2[3 * i + j] * G[3 * i + j] + F.dR[c][3 * i + j] * B.J[j] * F.dR[d][3 * i + j];
+                        D[6 * (3 + c) + 3 + d] = s;
+                    }
+            }
+        }
+        return E;
+    }
+
+    // ------------------------------------------------------------ objective
+    // f = E / m + kappa / m B + D / m (compute_objective :633-703)
+    rigc_status objective(const double* x, bool want_grad, bool want_hess, double& f, std::vector<double>& g,
+                          BlockHess& H) {
+        stats.objective_evaluations++;
+        const double t0 = now_ms();
+        std::vector<double> gE, gB, gD;
+        BlockHess HB, HD;
+        const double E = energy_term(x, want_grad ? &g : nullptr, want_hess ? &H : nullptr);
+        stats.host_ms += now_ms() - t0;
+        const int32_t flags = (want_grad ? RIGC_GRADIENT : 0) | (want_hess ? RIGC_HESSIAN : 0);
+        double B = 0, D = 0;
+        rigc_status s = barrier_term(x, flags, B, want_grad ? &gB : nullptr, want_hess ? &HB : nullptr);
+        if (s != RIGC_OK) return s;
+        if (P.coefficient_friction > 0) {
+            s = friction_term(x, flags, D, want_grad ? &gD : nullptr, want_hess ? &HD : nullptr);
+            if (s != RIGC_OK) return s;
+        }
+        const double t1 = now_ms();
+        const double im = 1.0 / average_mass, km = kappa / average_mass;
+        f = E * im + km * B + D * im;
+        if (want_grad)
+            for (int i = 0; i < 6 * nb; i++)
+                g[i] = g[i] * im + km * gB[i] + (P.coefficient_friction > 0 ? gD[i] * im : 0.0);
+        if (want_hess) {
+            for (int b = 0; b < nb; b++)
+                for (int e = 0; e < 36; e++) {
+                    double v = H.diag[b][e] * im + km * HB.diag[b][e];
+                    if (P.coefficient_friction > 0) v += HD.diag[b][e] * im;
+                    H.diag[b][e] = v;
+                }
+            for (auto& kv : HB.off) {
+                Block& o = H.off[kv.first];
+                for (int e = 0; e < 36; e++) o[e] += km * kv.second[e];
+            }
+            if (P.coefficient_friction > 0)
+                for (auto& kv : HD.off) {
+                    Block& o = H.off[kv.first];
+                    for (int e = 0; e < 36; e++) o[e] += im * kv.second[e];
+                }
+        }
+        stats.host_ms += now_ms() - t1;
+        return RIGC_OK;
+    }
+
+    // ------------------------------------------------------------ Newton
+    void world_vertices(const double* x, std::vector<double>& W) const {
+        W.resize(3 * (size_t)nv);
+        for (int b = 0; b < nb; b++) {
+            double R[9];
+            rigc::rotation_d(x + 6 * b + 3, R);
+            for (int i = vstart[b]; i < vstart[b + 1]; i++) rigc::world_vertex_d(R, x + 6 * b, &V[3 * i], &W[3 * i]);
+        }
+    }
+
+    // H_free d = -g_free (compute_direction with SimplicialLDLT)
+    bool solve_direction(const std::vector<double>& g, const BlockHess& H, double coeff, std::vector<double>& dir) {
+        std::vector<Block> diag(n_free);
+        std::vector<std::pair<int, int>> pairs;
+        std::vector<Block> off;
+        for (int b = 0; b < nb; b++)
+            if (dyn_index[b] >= 0) {
+                diag[dyn_index[b]] = H.diag[b];
+                for (int k = 0; k < 6; k++) diag[dyn_index[b]][7 * k] += coeff;
+            }
+        for (const auto& kv : H.off) {
+            const int i = dyn_index[kv.first.first], j = dyn_index[kv.first.second];
+            if (i < 0 || j < 0) continue;
+            pairs.push_back({ i, j });
+            off.push_back(kv.second);
+        }
+        std::vector<double> rhs(6 * (size_t)n_free), d(6 * (size_t)n_free);
+        for (int b = 0; b < nb; b++)
+            if (dyn_index[b] >= 0)
+                for (int k = 0; k < 6; k++) rhs[6 * dyn_index[b] + k] = -g[6 * b + k];
+        dir.assign(6 * nb, 0.0);
+        rigs::BlockLDLT ldlt;
+        bool ok = ldlt.factorize(n_free, diag, pairs, off);
+        if (ok) {
+            ldlt.solve(rhs.data(), d.data());
+            // residual must be finite (compute_direction :540-556)
+            double res = 0;
+            std::vector<double> Hd(6 * (size_t)n_free, 0.0);
+            for (int i = 0; i < n_free; i++)
+                for (int r = 0; r < 6; r++)
+                    for (int c = 0; c < 6; c++) Hd[6 * i + r] += diag[i][6 * r + c] * d[6 * i + c];
+            for (size_t e = 0; e < pairs.size(); e++) {
+                const int i = pairs[e].first, j = pairs[e].second;
+                for (int r = 0; r < 6; r++)
+                    for (int c = 0; c < 6; c++) {
+                        Hd[6 * i + r] += off[e][6 * r + c] * d[6 * j + c];
+                        Hd[6 * j + c] += off[e][6 * r + c] * d[6 * i + r];
+                    }
+            }
+            for (int i = 0; i < 6 * n_free; i++) res += (Hd[i] - rhs[i]) * (Hd[i] - rhs[i]);
+            ok = std::isfinite(res);
+        }
+        if (ok) {
+            for (int b = 0; b < nb; b++)
+                if (dyn_index[b] >= 0)
+                    for (int k = 0; k < 6; k++) dir[6 * b + k] = d[6 * dyn_index[b] + k];
+        } else {
+            for (int b = 0; b < nb; b++)
+                if (dyn_index[b] >= 0)
+                    for (int k = 0; k < 6; k++) dir[6 * b + k] = -g[6 * b + k];
+        }
+        return ok;
+    }
+
+    double dot_free(const std::vector<double>& a, const std::vector<double>& b) const {
+        double s = 0;
+        for (int i = 0; i < nb; i++)
+            if (dyn_index[i] >= 0)
+                for (int k = 0; k < 6; k++) s += a[6 * i + k] * b[6 * i + k];
+        return s;
+    }
+
+    bool converged(const std::vector<double>& x, const std::vector<double>& dir) {
+        std::vector<double> W0, W1, x1(x);
+        for (size_t i = 0; i < x.size(); i++) x1[i] += dir[i];
+        world_vertices(x.data(), W0);
+        world_vertices(x1.data(), W1);
+        double m = 0;
+        for (size_t i = 0; i < W0.size(); i++) m = std::max(m, std::fabs(W1[i] - W0[i]));
+        return m / P.timestep <= P.velocity_conv_tol * bbox_diagonal;
+    }
+
+    // NewtonSolver::line_search (:262-360)
+    rigc_status line_search(const std::vector<double>& x, const std::vector<double>& dir, double fx,
+                            const std::vector<double>& g, double& step, bool& success) {
+        success = false;
+        double lower = P.line_search_lower_bound / std::sqrt(std::fabs(dot_free(g, dir)));
+        lower = std::min(lower, 1e-1);
+        std::vector<double> x1(x);
+        for (size_t i = 0; i < x.size(); i++) x1[i] += dir[i];
+        double toi = 0;
+        rigc_status s = earliest_toi(x.data(), x1.data(), toi);
+        if (s != RIGC_OK) return s;
+        step = std::min(step, std::min(toi, 1.0));
+        std::vector<double> xi(x.size()), gdummy;
+        BlockHess Hdummy;
+        while (std::isfinite(lower) && step >= lower) {
+            stats.line_search_iterations++;
+            for (size_t i = 0; i < x.size(); i++) xi[i] = x[i] + step * dir[i];
+            double fxi = 0;
+            s = objective(xi.data(), false, false, fxi, gdummy, Hdummy);
+            if (s != RIGC_OK) return s;
+            if (fxi < fx) {
+                success = true;
+                break;
+            }
+            step /= 2.0;
+        }
+        return RIGC_OK;
+    }
+
+    // IPCSolver::init_solve: kappa from the energy / barrier gradients
+    // (ipc::initial_barrier_stiffness), previous minimum distance.
+    rigc_status init_solve(const std::vector<double>& x0) {
+        std::vector<double> gE, gB;
+        energy_term(x0.data(), &gE, nullptr);
+        double B = 0;
+        rigc_status s = barrier_term(x0.data(), RIGC_GRADIENT, B, &gB, nullptr);
+        if (s != RIGC_OK) return s;
+        const double dhat = P.dhat;
+        double d0 = 1e-8 * bbox_diagonal;
+        d0 *= d0;
+        if (d0 >= dhat * dhat) d0 = 0.5 * dhat * dhat;
+        double min_kappa = 4 * d0 * rigc::ipc_barrier_hessian(d0, dhat * dhat);
+        min_kappa = P.min_barrier_stiffness_scale * average_mass / min_kappa;
+        max_kappa = 100 * min_kappa;
+        double k = 1.0, gg = 0, ge = 0;
+        for (int i = 0; i < 6 * nb; i++) {
+            gg += gB[i] * gB[i];
+            ge += gB[i] * gE[i];
+        }
+        if (gg > 0) k = -ge / gg;
+        kappa = std::min(max_kappa, std::max(min_kappa, k));
+        return min_distance(x0.data(), prev_min_distance);
+    }
+
+    // IPCSolver::post_step_update (ipc::update_barrier_stiffness)
+    rigc_status post_step_update(const std::vector<double>& x) {
+        double md = 0;
+        rigc_status s = min_distance(x.data(), md);
+        if (s != RIGC_OK) return s;
+        double eps = P.dhat_epsilon * bbox_diagonal;
+        eps *= eps;
+        if (prev_min_distance < eps && md < eps && md < prev_min_distance) kappa = std::min(max_kappa, 2 * kappa);
+        prev_min_distance = md;
+        return RIGC_OK;
+    }
+
+    // IPCSolver::solve = init_solve + NewtonSolver::solve
+    rigc_status newton_solve(std::vector<double>& x, bool& success) {
+        rigc_status s = init_solve(x);
+        if (s != RIGC_OK) return s;
+        success = false;
+        double coeff = 0;
+        std::vector<double> g(6 * nb), dir(6 * nb), gdir;
+        BlockHess H;
+        for (int it = 0; it < P.max_iterations; it++) {
+            double fx = 0;
+            s = objective(x.data(), true, true, fx, g, H);
+            if (s != RIGC_OK) return s;
+            // compute_regularized_direction (:420-470)
+            const double t0 = now_ms();
+            for (;;) {
+                const bool ok = solve_direction(g, H, coeff, dir) && dot_free(g, dir) <= 0;
+                if (coeff > 0) stats.regularizations++;
+                if (ok) {
+                    coeff /= 2;
+                    if (coeff < 1e-8) coeff = 0;
+                    break;
+                }
+                coeff = std::max(2 * coeff, 1e-8);
+                if (verbose && coeff == 1e-8)
+                    std::fprintf(stderr, "rigs: iter=%d solve failed or not descent (g.d=%g); regularizing\n", it,
+                                 dot_free(g, dir));
+                if (!std::isfinite(coeff)) {
+                    stats.host_ms += now_ms() - t0;
+                    return fail(RIGC_ERR_UNRESOLVED, "Newton regularization failed");
+                }
+            }
+            stats.host_ms += now_ms() - t0;
+            if (it > 0 && converged(x, dir)) {
+                success = true;
+                break;
+            }
+            double step = 1;
+            bool found = false;
+            s = line_search(x, dir, fx, g, step, found);
+            if (s != RIGC_OK) return s;
+            if (verbose)
+                std::fprintf(stderr, "rigs: iter=%d f=%.17g g.d=%.6g coeff=%g step=%g found=%d kappa=%g\n", it, fx,
+                             dot_free(g, dir), coeff, step, (int)found, kappa);
+            if (!found) {
+                gdir.assign(6 * nb, 0.0);
+                for (int b = 0; b < nb; b++)
+                    if (dyn_index[b] >= 0)
+                        for (int k = 0; k < 6; k++) gdir[6 * b + k] = -g[6 * b + k];
+                dir = gdir;
+                step = 1;
+                s = line_search(x, dir, fx, g, step, found);
+                if (s != RIGC_OK) return s;
+                if (!found) {
+                    if (it == 0 && converged(x, dir)) success = true;
+                    break;
+                }
+            }
+            for (int i = 0; i < 6 * nb; i++) x[i] += step * dir[i];
+            stats.newton_iterations++;
+            s = post_step_update(x);
+            if (s != RIGC_OK) return s;
+        }
+        return RIGC_OK;
+    }
+
+    // ------------------------------------------------------------ step
+    rigc_status step() {
+        const double t_start = now_ms();
+        for (auto& B : bodies) {
+            std::memcpy(B.pose_prev, B.pose, sizeof(B.pose));
+        }
+        for (int b = 0; b < nb; b++) std::memcpy(&poses_t0[6 * b], bodies[b].pose, sizeof(double) * 6);
+        // update_constraints: barrier gradient at t0 (kept by the reference for
+        // its integrators) and the lagged friction set at t0
+        {
+            double B = 0;
+            std::vector<double> gB;
+            rigc_status s = barrier_term(poses_t0.data(), RIGC_GRADIENT, B, &gB, nullptr);
+            if (s != RIGC_OK) return s;
+            s = update_friction(poses_t0.data());
+            if (s != RIGC_OK) return s;
+        }
+        // solve_constraints with lagged friction
+        std::vector<double> x(poses_t0);
+        const double eps_d = 1e-2 * bbox_diagonal;
+        int i = 0;
+        bool success = false;
+        for (;;) {
+            rigc_status s = newton_solve(x, success);
+            if (s != RIGC_OK) return s;
+            if (!success) {
+                stats.failed_solves++;
+                break;
+            }
+            s = update_friction(x.data());
+            if (s != RIGC_OK) return s;
+            std::vector<double> gE, gB, gD;
+            energy_term(x.data(), &gE, nullptr);
+            double B = 0, D = 0;
+            s = barrier_term(x.data(), RIGC_GRADIENT, B, &gB, nullptr);
+            if (s != RIGC_OK) return s;
+            if (P.coefficient_friction > 0) {
+                s = friction_term(x.data(), RIGC_GRADIENT, D, &gD, nullptr);
+                if (s != RIGC_OK) return s;
+            }
+            double bal = 0;
+            for (int b = 0; b < nb; b++) {
+                if (dyn_index[b] < 0) continue;
+                for (int k = 0; k < 6; k++) {
+                    const int e = 6 * b + k;
+                    const double t = gE[e] + kappa * gB[e] + (P.coefficient_friction > 0 ? gD[e] : 0.0);
+                    bal += t * t;
+                }
+            }
+            i++;
+            if (!((P.friction_iterations < 0 || i < P.friction_iterations) && std::sqrt(bal) > eps_d)) break;
+        }
+        // take_step
+        double md = 0;
+        rigc_status s = min_distance(x.data(), md);
+        if (s != RIGC_OK) return s;
+        stats.min_distance = md;
+        const double h = P.timestep;
+        for (int b = 0; b < nb; b++) {
+            BodyState& B = bodies[b];
+            std::memcpy(B.pose, &x[6 * b], sizeof(double) * 6);
+            if (!B.dynamic) continue;
+            for (int k = 0; k < 3; k++) B.velocity[k] = (B.pose[k] - B.pose_prev[k]) / h;
+            double Q[9], Qp[9];
+            rigc::rotation_d(B.pose + 3, Q);
+            rigc::rotation_d(B.pose_prev + 3, Qp);
+            for (int e = 0; e < 9; e++) B.Qdot[e] = (Q[e] - Qp[e]) / h;
+            // omega in the body frame: Q^T Qdot ~ [omega]_x
+            double W[9] = { 0 };
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++)
+                    for (int k = 0; k < 3; k++) W[3 * r + c] += Q[3 * k + r] * B.Qdot[3 * k + c];
+            B.velocity[3] = 0.5 * (W[7] - W[5]);
+            B.velocity[4] = 0.5 * (W[2] - W[6]);
+            B.velocity[5] = 0.5 * (W[3] - W[1]);
+        }
+        stats.steps++;
+        stats.kappa = kappa;
+        stats.total_ms += now_ms() - t_start;
+        return RIGC_OK;
+    }
+};
+
+extern "C" {
+
+void rigs_default_params(rigs_params* p) {
+    if (!p) return;
+    p->timestep = 0.01;
+    p->gravity[0] = 0;
+    p->gravity[1] = -9.81;
+    p->gravity[2] = 0;
+    p->dhat = 1e-3;
+    p->minimum_separation_distance = 0;
+    p->coefficient_friction = 0;
+    p->friction_iterations = 1;
+    p->static_friction_speed_bound = 1e-3;
+    p->velocity_conv_tol = 1e-2;
+    p->line_search_lower_bound = 1e-12;
+    p->max_iterations = 3000;
+    p->dhat_epsilon = 1e-9;
+    p->min_barrier_stiffness_scale = 1e11;
+}
+
+rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t device, const rigc_scene* scene,
+                        const rigs_bodies* bodies, const rigs_params* params) {
+    if (!out || !scene || !bodies || !params || bodies->num_bodies != scene->num_bodies || !bodies->poses
+        || !bodies->velocities || !bodies->mass || !bodies->inertia || !(params->timestep > 0) || !(params->dhat > 0))
+        return RIGC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    rigs_sim* sim = new rigs_sim();
+    sim->P = *params;
+    const char* vb = std::getenv("RIGS_VERBOSE");
+    sim->verbose = vb && vb[0] == '1';
+    if (backend) {
+        sim->be = *backend;
+    } else {
+        rigc_status s = rigc_create(&sim->own_ctx, device);
+        if (s != RIGC_OK) {
+            delete sim;
+            return s;
+        }
+        sim->be = rigs_backend{ sim->own_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
+                                gb_fupdate, gb_friction, gb_error, gb_stats };
+    }
+    rigc_status s = sim->be.set_scene(sim->be.self, scene);
+    if (s != RIGC_OK) {
+        if (sim->own_ctx) rigc_destroy(sim->own_ctx);
+        delete sim;
+        return s;
+    }
+    const int nb = scene->num_bodies;
+    sim->nb = nb;
+    sim->nv = scene->num_vertices;
+    sim->V.assign(scene->vertices, scene->vertices + 3 * (size_t)sim->nv);
+    sim->vstart.assign(scene->body_vertex_start, scene->body_vertex_start + nb + 1);
+    sim->bodies.resize(nb);
+    sim->dyn_index.assign(nb, -1);
+    double mass_sum = 0;
+    int n_free_dof = 0;
+    for (int b = 0; b < nb; b++) {
+        BodyState& B = sim->bodies[b];
+        B.dynamic = !scene->is_static[b];
+        B.mass = bodies->mass[b];
+        for (int k = 0; k < 3; k++) B.inertia[k] = bodies->inertia[3 * b + k];
+        B.J[0] = 0.5 * (-B.inertia[0] + B.inertia[1] + B.inertia[2]);
+        B.J[1] = 0.5 * (B.inertia[0] - B.inertia[1] + B.inertia[2]);
+        B.J[2] = 0.5 * (B.inertia[0] + B.inertia[1] - B.inertia[2]);
+        std::memcpy(B.pose, bodies->poses + 6 * b, sizeof(B.pose));
+        std::memcpy(B.pose_prev, B.pose, sizeof(B.pose));
+        std::memcpy(B.velocity, bodies->velocities + 6 * b, sizeof(B.velocity));
+        // Qdot = Q [omega]_x (rigid_body.cpp:159-160)
+        double Q[9];
+        rigc::rotation_d(B.pose + 3, Q);
+        const double* w = B.velocity + 3;
+        const double K[9] = { 0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0 };
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++) v += Q[3 * r + k] * K[3 * k + c];
+                B.Qdot[3 * r + c] = B.dynamic ? v : 0.0;
+            }
+        if (B.dynamic) {
+            sim->dyn_index[b] = sim->n_free++;
+            mass_sum += 3 * B.mass + B.inertia[0] + B.inertia[1] + B.inertia[2];
+            n_free_dof += 6;
+        } else {
+            for (int k = 0; k < 6; k++) B.velocity[k] = 0;
+        }
+    }
+    sim->average_mass = n_free_dof ? mass_sum / n_free_dof : 1.0;
+    // init_bbox_diagonal over the initial world vertices
+    std::vector<double> x0(6 * (size_t)nb), W;
+    for (int b = 0; b < nb; b++) std::memcpy(&x0[6 * b], sim->bodies[b].pose, sizeof(double) * 6);
+    sim->world_vertices(x0.data(), W);
+    double lo[3] = { HUGE_VAL, HUGE_VAL, HUGE_VAL }, hi[3] = { -HUGE_VAL, -HUGE_VAL, -HUGE_VAL };
+    for (int i = 0; i < sim->nv; i++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], W[3 * i + k]);
+            hi[k] = std::max(hi[k], W[3 * i + k]);
+        }
+    double d2 = 0;
+    for (int k = 0; k < 3; k++) d2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
+    sim->bbox_diagonal = std::sqrt(d2);
+    sim->poses_t0 = x0;
+    *out = sim;
+    return RIGC_OK;
+}
+
+rigc_status rigs_destroy(rigs_sim* sim) {
+    if (!sim) return RIGC_OK;
+    if (sim->own_ctx) rigc_destroy(sim->own_ctx);
+    delete sim;
+    return RIGC_OK;
+}
+
+const char* rigs_last_error(const rigs_sim* sim) { return sim ? sim->err.c_str() : "null simulation"; }
+
+rigc_status rigs_step(rigs_sim* sim, int32_t num_steps) {
+    if (!sim || num_steps < 0) return RIGC_ERR_INVALID_ARGUMENT;
+    for (int i = 0; i < num_steps; i++) {
+        rigc_status s = sim->step();
+        if (s != RIGC_OK) return s;
+    }
+    return RIGC_OK;
+}
+
+rigc_status rigs_get_state(const rigs_sim* sim, double* poses, double* velocities) {
+    if (!sim) return RIGC_ERR_INVALID_ARGUMENT;
+    for (int b = 0; b < sim->nb; b++) {
+        if (poses) std::memcpy(poses + 6 * b, sim->bodies[b].pose, sizeof(double) * 6);
+        if (velocities) std::memcpy(velocities + 6 * b, sim->bodies[b].velocity, sizeof(double) * 6);
+    }
+    return RIGC_OK;
+}
+
+rigc_status rigs_get_stats(const rigs_sim* sim, rigs_stats* stats) {
+    if (!sim || !stats) return RIGC_ERR_INVALID_ARGUMENT;
+    *stats = sim->stats;
+    return RIGC_OK;
+}
+
+rigc_status rigs_block_ldlt_solve(int32_t n, const double* diag, int64_t m, const int32_t* pairs, const double* off,
+                                  const double* b, double* x) {
+    if (n < 0 || m < 0 || (n && (!diag || !b || !x)) || (m && (!pairs || !off))) return RIGC_ERR_INVALID_ARGUMENT;
+    std::vector<Block> D(n), O(m);
+    std::vector<std::pair<int, int>> P(m);
+    for (int i = 0; i < n; i++) std::copy(diag + 36 * i, diag + 36 * i + 36, D[i].begin());
+    for (int64_t e = 0; e < m; e++) {
+        std::copy(off + 36 * e, off + 36 * e + 36, O[e].begin());
+        P[e] = { pairs[2 * e], pairs[2 * e + 1] };
+        if (P[e].first < 0 || P[e].first >= n || P[e].second < 0 || P[e].second >= n) return RIGC_ERR_INVALID_ARGUMENT;
+    }
+    rigs::BlockLDLT L;
+    if (!L.factorize(n, D, P, O)) return RIGC_ERR_UNRESOLVED;
+    L.solve(b, x);
+    return RIGC_OK;
+}
+
+}  // extern "C"

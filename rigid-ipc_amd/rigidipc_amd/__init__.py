@@ -90,8 +90,8 @@ def lib():
     L.rigc_construct_constraint_set.argtypes = [vp, _c_double_p, d, d, _c_int64_p]
     L.rigc_get_constraint_set.argtypes = [vp, i64, _c_int32_p, _c_double_p, _c_int64_p]
     L.rigc_compute_barrier_term.argtypes = [vp, _c_double_p, d, d, i32, _c_double_p, _c_double_p, bh]
-    L.rigc_compute_minimum_distance.argtypes = [vp, _c_double_p, d, d, _c_double_p]
-    L.rigc_update_friction_constraints.argtypes = [vp, _c_double_p, d, d, d, d, _c_int64_p]
+    L.rigc_compute_minimum_distance.argtypes = [vp, _c_double_p, d, d, i32, _c_double_p]
+    L.rigc_update_friction_constraints.argtypes = [vp, _c_double_p, d, d, d, d, i32, _c_int64_p]
     L.rigc_get_friction_constraints.argtypes = [vp, i64, _c_double_p, _c_int64_p]
     L.rigc_compute_friction_term.argtypes = [vp, _c_double_p, _c_double_p, d, i32, _c_double_p, _c_double_p, bh]
     for name in ("rigc_create", "rigc_destroy", "rigc_get_stats", "rigc_set_scene", "rigc_narrow_phase",
@@ -271,18 +271,21 @@ class Context:
         self._keep = [p]
         return self._potential(self._L.rigc_compute_barrier_term, (_p(p, _c_double_p), float(dhat), float(dmin)), flags)
 
-    def minimum_distance(self, poses, dhat, dmin=0.0):
+    def minimum_distance(self, poses, dhat, dmin=0.0, rebuild=True):
         p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
         out = ctypes.c_double()
+        flags = RIGC_BUILD_CONSTRAINTS if rebuild else 0
         self._check(self._L.rigc_compute_minimum_distance(self._ctx, _p(p, _c_double_p), float(dhat), float(dmin),
-                                                          ctypes.byref(out)))
+                                                          flags, ctypes.byref(out)))
         return out.value
 
-    def update_friction_constraints(self, poses, dhat, dmin, barrier_stiffness, mu):
+    def update_friction_constraints(self, poses, dhat, dmin, barrier_stiffness, mu, rebuild=True):
         p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
         n = ctypes.c_int64()
+        flags = RIGC_BUILD_CONSTRAINTS if rebuild else 0
         self._check(self._L.rigc_update_friction_constraints(self._ctx, _p(p, _c_double_p), float(dhat), float(dmin),
-                                                             float(barrier_stiffness), float(mu), ctypes.byref(n)))
+                                                             float(barrier_stiffness), float(mu), flags,
+                                                             ctypes.byref(n)))
         return n.value
 
     def friction_constraints(self):

@@ -1,0 +1,180 @@
+"""Host time stepper (``librigc_sim.so``, ``rigid-ipc_amd/host/rigs.h``).
+
+``Simulation(scene)`` steps a scene from :mod:`rigidipc_amd.scenes` with the
+reference's implicit-Euler IPC Newton solve on the host and every contact
+term (CCD, constraint set, barrier, friction, minimum distance) on the
+device. Another contact backend can be passed as a ``Backend`` table
+(ctypes pointer), which is how the tests and the CPU baseline run the same
+stepper over the CPU restatement.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import Scene, RigcError, RIGC_OK, _p, _c_double_p, _c_int32_p, _c_uint8_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SIM_PATH = os.path.join(os.path.dirname(_HERE), "librigc_sim.so")
+
+
+class Backend(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in (
+        "self", "set_scene", "compute_earliest_toi", "construct_constraint_set", "compute_barrier_term",
+        "compute_minimum_distance", "update_friction_constraints", "compute_friction_term", "last_error",
+        "get_stats")]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("timestep", ctypes.c_double), ("gravity", ctypes.c_double * 3), ("dhat", ctypes.c_double),
+        ("minimum_separation_distance", ctypes.c_double), ("coefficient_friction", ctypes.c_double),
+        ("friction_iterations", ctypes.c_int32), ("static_friction_speed_bound", ctypes.c_double),
+        ("velocity_conv_tol", ctypes.c_double), ("line_search_lower_bound", ctypes.c_double),
+        ("max_iterations", ctypes.c_int32), ("dhat_epsilon", ctypes.c_double),
+        ("min_barrier_stiffness_scale", ctypes.c_double),
+    ]
+
+
+class Bodies(ctypes.Structure):
+    _fields_ = [("num_bodies", ctypes.c_int32), ("poses", _c_double_p), ("velocities", _c_double_p),
+                ("mass", _c_double_p), ("inertia", _c_double_p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int64), ("newton_iterations", ctypes.c_int64),
+        ("line_search_iterations", ctypes.c_int64), ("ccd_calls", ctypes.c_int64),
+        ("constraint_builds", ctypes.c_int64), ("objective_evaluations", ctypes.c_int64),
+        ("regularizations", ctypes.c_int64), ("failed_solves", ctypes.c_int64),
+        ("contact_ms", ctypes.c_double), ("host_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("kappa", ctypes.c_double), ("min_distance", ctypes.c_double), ("num_constraints", ctypes.c_int64),
+        ("ccd_candidates", ctypes.c_int64), ("ccd_ms", ctypes.c_double), ("ccd_broad_phase_ms", ctypes.c_double),
+        ("ccd_narrow_phase_ms", ctypes.c_double), ("ccd_box_evaluations", ctypes.c_int64),
+    ]
+
+
+_sim = None
+
+
+def sim_lib():
+    global _sim
+    if _sim is not None:
+        return _sim
+    if not os.path.exists(SIM_PATH):
+        raise RuntimeError(f"{SIM_PATH} is not built; run rigid-ipc_amd/build.py")
+    L = ctypes.CDLL(SIM_PATH)
+    vp = ctypes.c_void_p
+    L.rigs_default_params.argtypes = [ctypes.POINTER(Params)]
+    L.rigs_default_params.restype = None
+    L.rigs_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Backend), ctypes.c_int32, ctypes.POINTER(Scene),
+                              ctypes.POINTER(Bodies), ctypes.POINTER(Params)]
+    L.rigs_destroy.argtypes = [vp]
+    L.rigs_last_error.argtypes = [vp]
+    L.rigs_last_error.restype = ctypes.c_char_p
+    L.rigs_step.argtypes = [vp, ctypes.c_int32]
+    L.rigs_get_state.argtypes = [vp, _c_double_p, _c_double_p]
+    L.rigs_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.rigs_block_ldlt_solve.argtypes = [ctypes.c_int32, _c_double_p, ctypes.c_int64, _c_int32_p, _c_double_p,
+                                        _c_double_p, _c_double_p]
+    for name in ("rigs_create", "rigs_destroy", "rigs_step", "rigs_get_state", "rigs_get_stats",
+                 "rigs_block_ldlt_solve"):
+        getattr(L, name).restype = ctypes.c_int32
+    _sim = L
+    return L
+
+
+def default_params(scene=None):
+    p = Params()
+    sim_lib().rigs_default_params(ctypes.byref(p))
+    if scene is not None:
+        p.timestep = float(getattr(scene, "dt", p.timestep))
+        p.dhat = float(getattr(scene, "dhat", p.dhat))
+        p.coefficient_friction = float(getattr(scene, "mu", 0.0))
+        g = getattr(scene, "gravity", None)
+        if g is not None:
+            for k in range(3):
+                p.gravity[k] = float(g[k])
+    return p
+
+
+def block_ldlt_solve(diag, pairs, off, b):
+    """Solve with the stepper's sparse block LDL^T (test hook)."""
+    L = sim_lib()
+    diag = np.ascontiguousarray(diag, np.float64).reshape(-1, 36)
+    pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    off = np.ascontiguousarray(off, np.float64).reshape(-1, 36)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b)
+    st = L.rigs_block_ldlt_solve(len(diag), _p(diag, _c_double_p), len(pairs), _p(pairs, _c_int32_p),
+                                 _p(off, _c_double_p), _p(b, _c_double_p), _p(x, _c_double_p))
+    if st != RIGC_OK:
+        raise RigcError(st, "block LDL^T failed")
+    return x
+
+
+class Simulation:
+    """DistanceBarrierRBProblem time stepping over a contact backend."""
+
+    def __init__(self, scene, params=None, device=0, backend=None):
+        self._L = sim_lib()
+        self._keep = dict(
+            vertices=np.ascontiguousarray(scene.vertices, np.float64),
+            edges=np.ascontiguousarray(scene.edges, np.int32),
+            faces=np.ascontiguousarray(scene.faces, np.int32),
+            face_edges=np.ascontiguousarray(scene.face_edges, np.int32),
+            body_vertex_start=np.ascontiguousarray(scene.body_vertex_start, np.int32),
+            body_edge_start=np.ascontiguousarray(scene.body_edge_start, np.int32),
+            body_face_start=np.ascontiguousarray(scene.body_face_start, np.int32),
+            group_ids=np.ascontiguousarray(scene.group_ids, np.int32),
+            is_static=np.ascontiguousarray(scene.is_static, np.uint8),
+            poses=np.ascontiguousarray(scene.poses, np.float64),
+            velocities=np.ascontiguousarray(scene.velocities, np.float64),
+            mass=np.ascontiguousarray(scene.mass, np.float64),
+            inertia=np.ascontiguousarray(scene.inertia, np.float64),
+        )
+        k = self._keep
+        s = Scene()
+        s.num_bodies, s.num_vertices = len(k["group_ids"]), len(k["vertices"])
+        s.num_edges, s.num_faces = len(k["edges"]), len(k["faces"])
+        for name in ("vertices", "edges", "faces", "face_edges", "body_vertex_start", "body_edge_start",
+                     "body_face_start", "group_ids", "is_static"):
+            a = k[name]
+            t = {np.float64: _c_double_p, np.int32: _c_int32_p, np.uint8: _c_uint8_p}[a.dtype.type]
+            setattr(s, name, _p(a, t))
+        b = Bodies(s.num_bodies, _p(k["poses"], _c_double_p), _p(k["velocities"], _c_double_p),
+                   _p(k["mass"], _c_double_p), _p(k["inertia"], _c_double_p))
+        self.params = params if params is not None else default_params(scene)
+        self.num_bodies = s.num_bodies
+        self._sim = ctypes.c_void_p()
+        st = self._L.rigs_create(ctypes.byref(self._sim), backend, int(device), ctypes.byref(s), ctypes.byref(b),
+                                 ctypes.byref(self.params))
+        if st != RIGC_OK:
+            raise RigcError(st, "rigs_create failed")
+
+    def close(self):
+        if self._sim:
+            self._L.rigs_destroy(self._sim)
+            self._sim = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        st = self._L.rigs_step(self._sim, int(n))
+        if st != RIGC_OK:
+            raise RigcError(st, self._L.rigs_last_error(self._sim).decode())
+
+    def state(self):
+        poses = np.zeros((self.num_bodies, 6))
+        vel = np.zeros((self.num_bodies, 6))
+        self._L.rigs_get_state(self._sim, _p(poses, _c_double_p), _p(vel, _c_double_p))
+        return poses, vel
+
+    def stats(self):
+        s = Stats()
+        self._L.rigs_get_stats(self._sim, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
