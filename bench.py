@@ -72,29 +72,63 @@ def cpu_threads():
 
 
 # ------------------------------------------------------------------ sim
-def cpu_baseline_sim(sc, budget_s):
-    """The same stepper over the oracle backend, from the initial state, for
-    as many steps as fit in budget_s (at least one)."""
+def cpu_baseline_child(budget_s):
+    """Child process: the same stepper over the oracle backend, from the
+    initial state; prints one line per completed step (elapsed seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    from rigidipc_amd import scenes as S
     from rigidipc_amd import sim as M
+    sc = S.config2(seed=0)
     L = O.lib()
     L.oracle_backend.argtypes = [ctypes.POINTER(M.Backend)]
     L.oracle_backend.restype = None
     be = M.Backend()
     L.oracle_backend(ctypes.byref(be))
     sim = M.Simulation(sc, backend=ctypes.pointer(be))
+    print("READY", flush=True)
     t0 = time.perf_counter()
-    n = 0
-    while True:
+    while time.perf_counter() - t0 < budget_s:
         sim.step(1)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
+        st = sim.stats()
+        print(f"STEP {int(st['steps'])} {time.perf_counter() - t0:.6f} {int(st['newton_iterations'])}", flush=True)
+
+
+def cpu_baseline_sim(budget_s):
+    """Runs cpu_baseline_child in a subprocess (it never touches the GPU) and
+    stops it at the deadline: a step still in progress then is not counted,
+    so the rate is an upper bound on the CPU's steps/s over that time."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--cpu-child", "--cpu-budget", str(budget_s)]
+    proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+    steps, elapsed, newton = 0, 0.0, 0
+    import selectors
+    sel = selectors.DefaultSelector()
+    sel.register(proc.stdout, selectors.EVENT_READ)
+    t0 = time.perf_counter()
+    if sel.select(timeout=300.0):  # scene set-up in the child
+        proc.stdout.readline()
+    t0 = time.perf_counter()
+    deadline = t0 + budget_s
+    while True:
+        left = deadline - time.perf_counter()
+        if left <= 0:
             break
-    dt = time.perf_counter() - t0
-    st = sim.stats()
-    sim.close()
-    return n, dt, st
+        if not sel.select(timeout=min(left, 5.0)):
+            log(f"cpu baseline: {steps} steps after {time.perf_counter() - t0:.1f} s")
+            continue
+        line = proc.stdout.readline()
+        if not line:
+            break
+        if line.startswith("STEP"):
+            _, n, t, it = line.split()
+            steps, elapsed, newton = int(n), float(t), int(it)
+            log(f"cpu baseline step {steps}: {elapsed:.2f} s")
+    wall = min(time.perf_counter() - t0, budget_s)
+    if proc.poll() is None:
+        proc.kill()
+    proc.wait()
+    return steps, wall, elapsed, newton
 
 
 def bench_sim(args):
@@ -103,15 +137,18 @@ def bench_sim(args):
     sc = S.config2(seed=0)
     if args.warmup:
         w = M.Simulation(sc)
-        w.step(args.warmup)
+        for k in range(args.warmup):
+            w.step(1)
+            log(f"warmup step {k + 1}/{args.warmup}")
         w.close()
     sim = M.Simulation(sc)
     per_step = []
     t_all = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         t0 = time.perf_counter()
         sim.step(1)
         per_step.append(time.perf_counter() - t0)
+        log(f"step {k + 1}/{args.steps}: {1e3 * per_step[-1]:.1f} ms")
     wall = time.perf_counter() - t_all
     st = sim.stats()
     sim.close()
@@ -134,15 +171,14 @@ def bench_sim(args):
         roofline=ccd_roofline(st["ccd_box_evaluations"], st["ccd_narrow_phase_ms"], st["ccd_calls"]),
     )
     if not args.no_cpu:
-        n, dt, cst = cpu_baseline_sim(sc, args.cpu_budget)
-        gpu_same = sum(per_step[:n]) if n <= len(per_step) else None
+        n, wall_cpu, t_done, newton = cpu_baseline_sim(args.cpu_budget)
+        gpu_same = sum(per_step[:n]) if 0 < n <= len(per_step) else None
         line["cpu_baseline"] = dict(
-            value=n / dt, unit="time steps/s", cores=cpu_threads(), kind="port",
-            sample=f"first {n} time steps of the same scene from the same initial state, same host stepper over "
-                   f"the oracle backend (OpenMP, {cpu_threads()} threads): {dt:.2f} s; device path over the "
-                   f"same {n} steps: " + (f"{gpu_same:.3f} s" if gpu_same is not None else "n/a")
-                   + f"; CPU newton iterations {int(cst['newton_iterations'])}, CCD candidates "
-                   f"{int(cst['ccd_candidates'])}")
+            value=n / wall_cpu, unit="time steps/s", cores=cpu_threads(), kind="port",
+            sample=f"same scene, same initial state, same host stepper over the oracle backend (OpenMP, "
+                   f"{cpu_threads()} threads), stopped after {wall_cpu:.1f} s: {n} steps completed "
+                   f"({t_done:.2f} s, {newton} Newton iterations); the device path took "
+                   + (f"{gpu_same:.3f} s" if gpu_same is not None else "n/a") + " for those steps")
     return line
 
 
@@ -209,15 +245,19 @@ def main():
     ap.add_argument("--workload", default="sim", choices=["sim", "ccd"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_baseline_child(args.cpu_budget)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not part of the build)", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = 50 if args.workload == "sim" else 40
+        args.steps = 8 if args.workload == "sim" else 40
     if args.warmup is None:
-        args.warmup = 2 if args.workload == "sim" else 3
+        args.warmup = 1 if args.workload == "sim" else 3
     line = bench_sim(args) if args.workload == "sim" else bench_ccd(args)
     print(json.dumps(line), flush=True)
 

@@ -95,37 +95,36 @@ __global__ __launch_bounds__(BP_THREADS) void body_box_kernel(const double* V, c
         }
 }
 
-__global__ __launch_bounds__(BP_THREADS) void included_kernel(int nb, const int* group, const double* box_inf, int* inc) {
-    __shared__ double tb[BP_THREADS][6];
-    __shared__ int tg[BP_THREADS];
-    const int i = blockIdx.x * BP_THREADS + threadIdx.x;
+// Body i is included when its box overlaps the box of a body of another
+// group. One 64-lane block per (i tile, j tile) of the nb x nb pair grid;
+// inc[] is cleared beforehand and only ever set to 1.
+constexpr int PAIR_TILE = 64;
+
+__global__ __launch_bounds__(PAIR_TILE) void included_kernel(int nb, const int* group, const double* box_inf,
+                                                            int* inc) {
+    __shared__ double tb[PAIR_TILE][6];
+    __shared__ int tg[PAIR_TILE];
+    const int i = blockIdx.x * PAIR_TILE + threadIdx.x;
+    const int j0 = blockIdx.y * PAIR_TILE;
+    const int jt = j0 + threadIdx.x;
+    if (jt < nb) {
+        for (int k = 0; k < 6; k++) tb[threadIdx.x][k] = box_inf[6 * jt + k];
+        tg[threadIdx.x] = group[jt];
+    }
+    __syncthreads();
+    if (i >= nb) return;
     double mi[6];
-    int gi = 0;
-    if (i < nb) {
-        for (int k = 0; k < 6; k++) mi[k] = box_inf[6 * i + k];
-        gi = group[i];
-    }
-    bool any = false;
-    for (int j0 = 0; j0 < nb; j0 += BP_THREADS) {
-        const int j = j0 + threadIdx.x;
-        if (j < nb) {
-            for (int k = 0; k < 6; k++) tb[threadIdx.x][k] = box_inf[6 * j + k];
-            tg[threadIdx.x] = group[j];
+    for (int k = 0; k < 6; k++) mi[k] = box_inf[6 * i + k];
+    const int gi = group[i];
+    const int m = min(PAIR_TILE, nb - j0);
+    for (int jj = 0; jj < m; jj++) {
+        if (j0 + jj == i || tg[jj] == gi) continue;
+        const double* o = tb[jj];
+        if (mi[0] <= o[3] && o[0] <= mi[3] && mi[1] <= o[4] && o[1] <= mi[4] && mi[2] <= o[5] && o[2] <= mi[5]) {
+            inc[i] = 1;
+            return;
         }
-        __syncthreads();
-        const int m = min(BP_THREADS, nb - j0);
-        if (i < nb && !any)
-            for (int jj = 0; jj < m; jj++) {
-                if (j0 + jj == i || tg[jj] == gi) continue;
-                const double* o = tb[jj];
-                if (mi[0] <= o[3] && o[0] <= mi[3] && mi[1] <= o[4] && o[1] <= mi[4] && mi[2] <= o[5] && o[2] <= mi[5]) {
-                    any = true;
-                    break;
-                }
-            }
-        __syncthreads();
     }
-    if (i < nb) inc[i] = any ? 1 : 0;
 }
 
 __global__ __launch_bounds__(BP_THREADS) void domain_kernel(int nb, const int* inc, const double* box_raw, double r,
@@ -332,13 +331,24 @@ __device__ __forceinline__ double hull_eps(const double* h) {
     return 1e-12 * m;
 }
 
-__global__ __launch_bounds__(BP_THREADS) void body_pair_kernel(int nb, const int* inc, const int* group,
-                                                              const double* bhull, int* pairs, long long cap,
-                                                              unsigned long long* count) {
-    __shared__ double th[BP_THREADS][6];
-    __shared__ int tg[BP_THREADS], ti[BP_THREADS];
-    const int a = blockIdx.x * BP_THREADS + threadIdx.x;
-    const int lane = threadIdx.x & 63;
+// Close body pairs (a < b): one 64-lane block per (a tile, b tile) with
+// b tile >= a tile; hits appended with one atomic per wavefront.
+__global__ __launch_bounds__(PAIR_TILE) void body_pair_kernel(int nb, const int* inc, const int* group,
+                                                             const double* bhull, int* pairs, long long cap,
+                                                             unsigned long long* count) {
+    if (blockIdx.y < blockIdx.x) return;
+    __shared__ double th[PAIR_TILE][6];
+    __shared__ int tg[PAIR_TILE], ti[PAIR_TILE];
+    const int a = blockIdx.x * PAIR_TILE + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int b0 = blockIdx.y * PAIR_TILE;
+    const int bb = b0 + threadIdx.x;
+    if (bb < nb) {
+        ti[threadIdx.x] = inc[bb];
+        tg[threadIdx.x] = group[bb];
+        for (int k = 0; k < 6; k++) th[threadIdx.x][k] = bhull[6 * bb + k];
+    }
+    __syncthreads();
     double ha[6];
     int ga = 0;
     bool ia = false;
@@ -349,40 +359,30 @@ __global__ __launch_bounds__(BP_THREADS) void body_pair_kernel(int nb, const int
         for (int k = 0; k < 6; k++) ha[k] = bhull[6 * a + k];
         if (ia) ea = hull_eps(ha);
     }
-    for (int b0 = 0; b0 < nb; b0 += BP_THREADS) {
-        const int bb = b0 + threadIdx.x;
-        if (bb < nb) {
-            ti[threadIdx.x] = inc[bb];
-            tg[threadIdx.x] = group[bb];
-            for (int k = 0; k < 6; k++) th[threadIdx.x][k] = bhull[6 * bb + k];
+    const int m = min(PAIR_TILE, nb - b0);
+    for (int j = 0; j < m; j++) {
+        const int b = b0 + j;
+        bool hit = false;
+        if (ia && b > a && ti[j] && tg[j] != ga) {
+            const double* hb = th[j];
+            const double eps = fmax(ea, hull_eps(hb));
+            hit = ha[0] <= hb[3] + eps && hb[0] <= ha[3] + eps && ha[1] <= hb[4] + eps && hb[1] <= ha[4] + eps
+                && ha[2] <= hb[5] + eps && hb[2] <= ha[5] + eps;
         }
-        __syncthreads();
-        const int m = min(BP_THREADS, nb - b0);
-        for (int j = 0; j < m; j++) {
-            const int b = b0 + j;
-            bool hit = false;
-            if (ia && b > a && ti[j] && tg[j] != ga) {
-                const double* hb = th[j];
-                const double eps = fmax(ea, hull_eps(hb));
-                hit = ha[0] <= hb[3] + eps && hb[0] <= ha[3] + eps && ha[1] <= hb[4] + eps && hb[1] <= ha[4] + eps
-                    && ha[2] <= hb[5] + eps && hb[2] <= ha[5] + eps;
-            }
-            const unsigned long long mask = __ballot(hit);
-            if (mask) {
-                const int leader = __ffsll((long long)mask) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
-                base = __shfl(base, leader);
-                if (hit) {
-                    const unsigned long long k = base + __popcll(mask & ((1ULL << lane) - 1ULL));
-                    if ((long long)k < cap) {
-                        pairs[2 * k] = a;
-                        pairs[2 * k + 1] = b;
-                    }
+        const unsigned long long mask = __ballot(hit);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+            base = __shfl(base, leader);
+            if (hit) {
+                const unsigned long long k = base + __popcll(mask & ((1ULL << lane) - 1ULL));
+                if ((long long)k < cap) {
+                    pairs[2 * k] = a;
+                    pairs[2 * k + 1] = b;
                 }
             }
         }
-        __syncthreads();
     }
 }
 
@@ -539,8 +539,10 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const double* d_p
     const double* p1 = continuous ? d_p1 : d_p0;
     hipLaunchKernelGGL(body_box_kernel, dim3(nb), dim3(BP_THREADS), 0, st, sc.V, sc.body_vstart, sc.is_static,
                        sc.r_max, d_p0, p1, r, bp.box_raw.as<double>(), bp.box_inf.as<double>());
-    hipLaunchKernelGGL(included_kernel, dim3((nb + BP_THREADS - 1) / BP_THREADS), dim3(BP_THREADS), 0, st, nb,
-                       sc.group, bp.box_inf.as<double>(), bp.inc.as<int>());
+    const unsigned tiles = (unsigned)((nb + PAIR_TILE - 1) / PAIR_TILE);
+    BP_CHECK(hipMemsetAsync(bp.inc.p, 0, sizeof(int) * nb, st));
+    hipLaunchKernelGGL(included_kernel, dim3(tiles, tiles), dim3(PAIR_TILE), 0, st, nb, sc.group,
+                       bp.box_inf.as<double>(), bp.inc.as<int>());
     int* any_dev = reinterpret_cast<int*>(bp.dom.as<double>() + 6);
     hipLaunchKernelGGL(domain_kernel, dim3(1), dim3(BP_THREADS), 0, st, nb, bp.inc.as<int>(),
                        bp.box_raw.as<double>(), r, bp.dom.as<double>(), any_dev);
@@ -564,8 +566,8 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const double* d_p
     for (int attempt = 0; attempt < 2; attempt++) {
         BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * cap_pairs));
         BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(body_pair_kernel, dim3((nb + BP_THREADS - 1) / BP_THREADS), dim3(BP_THREADS), 0, st, nb,
-                           bp.inc.as<int>(), sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt);
+        hipLaunchKernelGGL(body_pair_kernel, dim3(tiles, tiles), dim3(PAIR_TILE), 0, st, nb, bp.inc.as<int>(),
+                           sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt);
         unsigned long long n = 0;
         BP_CHECK(hipMemcpyAsync(&n, cnt, sizeof(n), hipMemcpyDeviceToHost, st));
         BP_CHECK(hipStreamSynchronize(st));

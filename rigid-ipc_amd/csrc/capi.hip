@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -30,6 +31,8 @@ struct rigc_ctx {
     DevBuf poses0, poses1, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
     BroadPhase bp;
     Assembly as;
+    CcdWork ccdw;
+    int walk_budget = 4096;
     hipEvent_t ev2 = nullptr;
 
     rigc_status fail(rigc_status s, const std::string& msg) {
@@ -91,6 +94,9 @@ rigc_status rigc_create(rigc_ctx** out, int32_t device) {
         return RIGC_ERR_DEVICE;
     }
     ctx->num_cus = prop.multiProcessorCount;
+    // RIGC_CCD_WALK_BUDGET: box evaluations after which a lane hands its
+    // pending subtrees to the next narrow-phase pass (0 = never)
+    if (const char* wb = std::getenv("RIGC_CCD_WALK_BUDGET")) ctx->walk_budget = std::atoi(wb);
     *out = ctx;
     return RIGC_OK;
 }
@@ -112,6 +118,7 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
                        &ctx->bp.counters, &ctx->bp.out_ee, &ctx->bp.out_fv })
         b->release();
     ctx->as.release();
+    ctx->ccdw.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RIGC_OK;
@@ -234,7 +241,7 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     unsigned long long init[4] = { 0x7ff0000000000000ULL, 0, 0, 0 };
     RIGC_HIP(hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, st));
 
-    CcdArgs a;
+    CcdArgs a{};
     a.V = d.V; a.E = d.E; a.F = d.F; a.FE = d.FE;
     a.v_body = d.v_body; a.e_body = d.e_body; a.f_body = d.f_body; a.edge_len = d.edge_len;
     a.poses_t0 = ctx->poses0.as<double>();
@@ -260,8 +267,8 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     (void)blocks_needed;
     const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
-    if (nq) launch_ccd(a, grid, block, st);
-    RIGC_HIP(hipGetLastError());
+    a.walk_budget = ctx->walk_budget;
+    if (nq) RIGC_HIP(launch_ccd(a, grid, block, st, ctx->ccdw));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
 
     std::vector<double> tois(nq);
@@ -395,8 +402,8 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     a.q_visits = ctx->qvisits.as<unsigned int>();
     a.q_flag = ctx->qflag.as<int>();
     const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
-    if (nq) launch_ccd(a, grid, 256, s);
-    RIGC_HIP(hipGetLastError());
+    a.walk_budget = ctx->walk_budget;
+    if (nq) RIGC_HIP(launch_ccd(a, grid, 256, s, ctx->ccdw));
     RIGC_HIP(hipEventRecord(ctx->ev2, s));
     unsigned long long res[4];
     RIGC_HIP(hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, s));

@@ -27,6 +27,16 @@
 // same wavefront through cross-lane shuffles, so a deep query ends up
 // bisected by up to 64 lanes. Per-query minima live in global memory and
 // are combined with 64-bit atomicMin on the bit pattern (t >= 0).
+//
+// To put more than one wavefront on a deep query, the walk runs in passes:
+// once the pool of a pass is empty, a lane that has evaluated walk_budget
+// boxes since it took its subtree writes every pending subtree of its path
+// (the second halves still to visit, and its next box) to a task buffer
+// and goes idle; the next pass takes those tasks as its pool and spreads
+// them over the whole grid. Each pass ends when its lanes have finished or
+// exported, so no lane ever waits for another.
+#include <algorithm>
+
 #include "rigc_interval.hpp"
 #include "rigc_internal.hpp"
 #include "rigc_rigid_iv.hpp"
@@ -197,6 +207,8 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
     Query q;
     double tol[3];
     long long evals = 0;
+    int walk = 0;       // evaluations since this lane took its subtree
+    const long long npool = a.tasks_in ? a.n_tasks_in : nq;
 
     int depth = 0;
     Bits192 s, mt, ma;
@@ -212,7 +224,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
     // at most `quota` new queries at a time, and its remaining idle lanes
     // help with the subtrees of the queries it holds.
     const long long nwaves = (long long)gridDim.x * (blockDim.x / 64);
-    const int quota = (int)max(1LL, min(64LL, (nq + nwaves - 1) / nwaves));
+    const int quota = (int)max(1LL, min(64LL, (npool + nwaves - 1) / nwaves));
     bool pool_open = true;
     for (;;) {
         // A. idle lanes take new queries while the pool lasts
@@ -228,15 +240,30 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 unsigned long long base = 0;
                 if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)__popcll(im));
                 base = __shfl(base, leader);
-                if (base + __popcll(im) >= (unsigned long long)nq) pool_open = false;
+                if (base + __popcll(im) >= (unsigned long long)npool) pool_open = false;
                 if ((im >> lane) & 1ULL) {
                     const long long mine = (long long)(base + __popcll(im & lt_mask));
-                    if (mine < nq) {
-                        qi = mine;
-                        load_query(a, qi, q, tol);
-                        tol[0] = a.toi_tol;
-                        phase = 1;
+                    if (mine < npool) {
+                        walk = 0;
                         cache_ok = false;
+                        if (a.tasks_in) {
+                            const CcdTask& tk = a.tasks_in[mine];
+                            qi = tk.qi;
+                            load_query(a, qi, q, tol);
+                            tol[0] = tk.tol0;
+                            idx[0] = tk.idx[0]; idx[1] = tk.idx[1]; idx[2] = tk.idx[2];
+                            lev[0] = tk.lev & 255; lev[1] = (tk.lev >> 8) & 255; lev[2] = (tk.lev >> 16) & 255;
+                            depth = 0;
+                            s.w[0] = s.w[1] = s.w[2] = 0;
+                            mt = s;
+                            ma = s;
+                            phase = 2;
+                        } else {
+                            qi = mine;
+                            load_query(a, qi, q, tol);
+                            tol[0] = a.toi_tol;
+                            phase = 1;
+                        }
                     }
                 }
             }
@@ -298,6 +325,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     ma = s;
                     phase = 2;
                     cache_ok = false;
+                    walk = 0;
                 }
             }
         }
@@ -366,6 +394,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
             }
             const bool z = contains_zero(q.kind, A0, DA, B0, DB, u, v);
             evals++;
+            walk++;
             if (phase == 1) {
                 if (z) tol[0] /= 1e2;
                 phase = 2;
@@ -426,6 +455,46 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 }
             }
         }
+        // D. export the pending subtrees of a long walk to the next pass
+        if (!done && phase == 2 && !pool_open && a.tasks_out && walk >= a.walk_budget) {
+            int n_open = 0;
+            for (int wi = 0; wi < 3; wi++) n_open += __popcll(~s.w[wi] & range_mask(wi, 0, depth));
+            const unsigned long long base = atomicAdd(a.n_tasks_out, (unsigned long long)(n_open + 1));
+            const unsigned long long cap = (unsigned long long)a.tasks_cap;
+            int k = 0;
+            for (int L = 0; L < depth; L++) {
+                if (bget(s, L)) continue;
+                const unsigned long long slot = base + (unsigned long long)(k++);
+                if (slot >= cap) continue;
+                const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
+                const int cb = depth - 1 - L - ct - ca;
+                CcdTask tk;
+                tk.qi = qi;
+                tk.tol0 = tol[0];
+                tk.idx[0] = idx[0] >> ct;
+                tk.idx[1] = idx[1] >> ca;
+                tk.idx[2] = idx[2] >> cb;
+                const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
+                tk.idx[d] += 1;
+                tk.lev = (lev[0] - ct) | ((lev[1] - ca) << 8) | ((lev[2] - cb) << 16);
+                tk.pad = 0;
+                a.tasks_out[slot] = tk;
+                bset(s, L, true);
+            }
+            const unsigned long long slot = base + (unsigned long long)n_open;
+            if (slot < cap) {
+                CcdTask tk;
+                tk.qi = qi;
+                tk.tol0 = tol[0];
+                tk.idx[0] = idx[0]; tk.idx[1] = idx[1]; tk.idx[2] = idx[2];
+                tk.lev = lev[0] | (lev[1] << 8) | (lev[2] << 16);
+                tk.pad = 0;
+                a.tasks_out[slot] = tk;
+                done = true;
+            } else {
+                walk = -(1 << 30);  // the buffer is full: finish this subtree here
+            }
+        }
         if (done) qi = -1;
     }
     if (evals) atomicAdd(a.total_evals, (unsigned long long)evals);
@@ -433,13 +502,37 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
 
 }  // namespace
 
-void launch_ccd(const CcdArgs& a, int grid, int block, hipStream_t stream) {
+hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& w) {
     const long long nq = a.n_ee + a.n_fv;
-    if (nq <= 0) return;
+    w.passes = 0;
+    if (nq <= 0) return hipSuccess;
+    const long long cap = 1LL << 18;
+    hipError_t e;
+    if ((e = w.tasks[0].ensure(sizeof(CcdTask) * cap)) != hipSuccess) return e;
+    if ((e = w.tasks[1].ensure(sizeof(CcdTask) * cap)) != hipSuccess) return e;
+    if ((e = w.count.ensure(sizeof(unsigned long long))) != hipSuccess) return e;
     const unsigned eb = (unsigned)((nq + 255) / 256);
     hipLaunchKernelGGL(ccd_init_kernel, dim3(eb), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(ccd_kernel, dim3(grid), dim3(block), 0, stream, a);
+    a.tasks_in = nullptr;
+    a.n_tasks_in = 0;
+    a.tasks_cap = cap;
+    a.n_tasks_out = w.count.as<unsigned long long>();
+    for (int pass = 0;; pass++) {
+        a.tasks_out = a.walk_budget > 0 ? w.tasks[pass & 1].as<CcdTask>() : nullptr;
+        if ((e = hipMemsetAsync(a.counter, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
+        if ((e = hipMemsetAsync(a.n_tasks_out, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
+        hipLaunchKernelGGL(ccd_kernel, dim3(grid), dim3(block), 0, stream, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        w.passes = pass + 1;
+        unsigned long long n = 0;
+        if ((e = hipMemcpyAsync(&n, a.n_tasks_out, sizeof(n), hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+        if (n == 0) break;
+        a.tasks_in = w.tasks[pass & 1].as<CcdTask>();
+        a.n_tasks_in = (long long)std::min<unsigned long long>(n, (unsigned long long)cap);
+    }
     hipLaunchKernelGGL(ccd_finish_kernel, dim3(eb), dim3(256), 0, stream, a);
+    return hipGetLastError();
 }
 
 }  // namespace rigc

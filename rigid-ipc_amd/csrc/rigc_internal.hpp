@@ -70,6 +70,17 @@ hipError_t radix_sort_u64(unsigned long long* keys, unsigned long long* tmp, uns
                           int bits, hipStream_t stream);
 size_t radix_sort_hist_entries(long long n);
 
+// A pending subtree of one query's bisection tree: the dyadic box
+// (index, level per dimension t, alpha, beta) and the query's t tolerance
+// after its start-box check.
+struct CcdTask {
+    long long qi;
+    double tol0;
+    unsigned long long idx[3];
+    int lev;  // lev_t | lev_a << 8 | lev_b << 16
+    int pad;
+};
+
 // Narrow-phase candidate: kind 0 = edge-edge (a = edge0, b = edge1),
 // kind 1 = face-vertex (a = face, b = vertex).
 struct CcdArgs {
@@ -98,9 +109,31 @@ struct CcdArgs {
     unsigned long long* q_best;  // least accepted t.lower (bits), +inf if none
     unsigned int* q_visits;      // box visits (only counted when eval_budget > 0)
     int* q_flag;                 // 0 ok, 1 budget exceeded, 2 depth exceeded
+    // pass structure (set by launch_ccd): the work pool is the queries
+    // (tasks_in == nullptr) or the subtrees exported by the previous pass
+    const CcdTask* tasks_in;
+    long long n_tasks_in;
+    CcdTask* tasks_out;
+    unsigned long long* n_tasks_out;
+    long long tasks_cap;
+    int walk_budget;             // evaluations after which a lane exports its open subtrees
 };
 
-void launch_ccd(const CcdArgs& a, int grid, int block, hipStream_t stream);
+// Task buffers of the multi-pass narrow phase (grow-only).
+struct CcdWork {
+    DevBuf tasks[2], count;
+    int passes = 0;
+    void release() {
+        tasks[0].release();
+        tasks[1].release();
+        count.release();
+    }
+};
+
+// Narrow phase: a first pass over the queries, then passes over exported
+// subtrees until none is left (a host read of the export count between
+// passes). a.counter is reset before every pass.
+hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& work);
 
 // Exclusive prefix sum of n counts; work holds scan_workspace_entries(n)
 // entries and *total (device) receives the sum.

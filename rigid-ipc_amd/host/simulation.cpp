@@ -104,7 +104,8 @@ struct rigs_sim {
     int64_t hcap = 0;
     rigs_stats stats{};
     std::string err;
-    bool verbose = false;  // RIGS_VERBOSE=1: one line per Newton iteration on stderr
+    int verbose = 0;  // RIGS_VERBOSE=1: one line per Newton iteration on stderr; 2: line-search trials too
+    double last_terms[3] = { 0, 0, 0 };  // E / m, kappa / m B, D / m of the last objective
 
     rigc_status fail(rigc_status s, const std::string& m) {
         err = m;
@@ -316,6 +317,9 @@ struct rigs_sim {
         const double t1 = now_ms();
         const double im = 1.0 / average_mass, km = kappa / average_mass;
         f = E * im + km * B + D * im;
+        last_terms[0] = E * im;
+        last_terms[1] = km * B;
+        last_terms[2] = D * im;
         if (want_grad)
             for (int i = 0; i < 6 * nb; i++)
                 g[i] = g[i] * im + km * gB[i] + (P.coefficient_friction > 0 ? gD[i] * im : 0.0);
@@ -442,6 +446,9 @@ struct rigs_sim {
             double fxi = 0;
             s = objective(xi.data(), false, false, fxi, gdummy, Hdummy);
             if (s != RIGC_OK) return s;
+            if (verbose >= 2)
+                std::fprintf(stderr, "rigs:   alpha=%g f=%.17g (E %.17g, B %.17g, D %.17g)\n", step, fxi,
+                             last_terms[0], last_terms[1], last_terms[2]);
             if (fxi < fx) {
                 success = true;
                 break;
@@ -526,6 +533,9 @@ struct rigs_sim {
             }
             double step = 1;
             bool found = false;
+            if (verbose >= 2)
+                std::fprintf(stderr, "rigs: iter=%d at x: E %.17g, B %.17g, D %.17g\n", it, last_terms[0],
+                             last_terms[1], last_terms[2]);
             s = line_search(x, dir, fx, g, step, found);
             if (s != RIGC_OK) return s;
             if (verbose)
@@ -665,7 +675,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
     rigs_sim* sim = new rigs_sim();
     sim->P = *params;
     const char* vb = std::getenv("RIGS_VERBOSE");
-    sim->verbose = vb && vb[0] == '1';
+    sim->verbose = vb ? std::atoi(vb) : 0;
     if (backend) {
         sim->be = *backend;
     } else {
