@@ -33,6 +33,7 @@ struct rigc_ctx {
     Assembly as;
     CcdWork ccdw;
     int walk_budget = 4096;
+    double last_toi = HUGE_VAL;  // answer of the previous rigc_compute_earliest_toi
     hipEvent_t ev2 = nullptr;
 
     rigc_status fail(rigc_status s, const std::string& msg) {
@@ -199,6 +200,7 @@ rigc_status rigc_set_scene(rigc_ctx* ctx, const rigc_scene* s) {
     RIGC_HIP(hipStreamSynchronize(ctx->stream));
     ctx->as.have_cons = ctx->as.have_fric = false;
     ctx->as.n_con = ctx->as.n_fric = 0;
+    ctx->last_toi = HUGE_VAL;
     ctx->has_scene = true;
     return RIGC_OK;
 }
@@ -386,8 +388,6 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     RIGC_HIP(ctx->qbest.ensure(sizeof(unsigned long long) * std::max<long long>(nq, 1)));
     RIGC_HIP(ctx->qvisits.ensure(sizeof(unsigned int) * std::max<long long>(nq, 1)));
     RIGC_HIP(ctx->qflag.ensure(sizeof(int) * std::max<long long>(nq, 1)));
-    unsigned long long init[4] = { 0x7ff0000000000000ULL, 0, 0, 0 };
-    RIGC_HIP(hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, s));
     CcdArgs a = ccd_args(ctx);
     a.ee = ctx->bp.out_ee.as<int>();
     a.fv = ctx->bp.out_fv.as<int>();
@@ -403,20 +403,44 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     a.q_flag = ctx->qflag.as<int>();
     const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
     a.walk_budget = ctx->walk_budget;
-    if (nq) RIGC_HIP(launch_ccd(a, grid, 256, s, ctx->ccdw));
+    // Windowed search: a first pass starts the running minimum at a bound T
+    // (four times the previous call's answer; successive line-search calls
+    // return similar times). It prunes every box with t.lower >= T, so if
+    // it accepts a box the minimum below T is the answer; otherwise the
+    // full search runs. Either way the result is the exact minimum.
+    const double inf = HUGE_VAL;
+    double window = inf;
+    if (ctx->last_toi < 0.25) window = std::max(4.0 * ctx->last_toi, std::ldexp(1.0, -20));
+    double g = inf;
+    long long evals = 0;
+    auto run = [&](double bound) -> hipError_t {
+        unsigned long long init[4] = { 0, 0, 0, 0 };
+        std::memcpy(&init[0], &bound, sizeof(double));
+        hipError_t e = hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && nq) e = launch_ccd(a, grid, 256, s, ctx->ccdw);
+        unsigned long long res[4] = { 0, 0, 0, 0 };
+        if (e == hipSuccess) e = hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        std::memcpy(&g, &res[0], sizeof(double));
+        evals += (long long)res[2];
+        return e;
+    };
+    if (window < inf) {
+        RIGC_HIP(run(window));
+        if (!(g < window)) RIGC_HIP(run(inf));
+    } else {
+        RIGC_HIP(run(inf));
+    }
     RIGC_HIP(hipEventRecord(ctx->ev2, s));
-    unsigned long long res[4];
-    RIGC_HIP(hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, s));
     RIGC_HIP(hipStreamSynchronize(s));
     float bms = 0, nms = 0;
     RIGC_HIP(hipEventElapsedTime(&bms, ctx->ev0, ctx->ev1));
     RIGC_HIP(hipEventElapsedTime(&nms, ctx->ev1, ctx->ev2));
     ctx->stats.broad_phase_ms = bms;
     ctx->stats.narrow_phase_ms = nms;
-    ctx->stats.box_evaluations = (int64_t)res[2];
+    ctx->stats.box_evaluations = evals;
     ctx->stats.num_candidates = nq;
-    double g;
-    std::memcpy(&g, &res[0], sizeof(double));
+    ctx->last_toi = g;
     *earliest_toi = g;  // +inf when no candidate collides
     return RIGC_OK;
 }

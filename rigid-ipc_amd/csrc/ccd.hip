@@ -517,8 +517,12 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
     a.n_tasks_in = 0;
     a.tasks_cap = cap;
     a.n_tasks_out = w.count.as<unsigned long long>();
+    const int budget0 = a.walk_budget;
     for (int pass = 0;; pass++) {
-        a.tasks_out = a.walk_budget > 0 ? w.tasks[pass & 1].as<CcdTask>() : nullptr;
+        // later passes export only much longer walks, and from the fourth
+        // pass on not at all, so the number of passes stays small
+        a.walk_budget = budget0 << (2 * std::min(pass, 8));
+        a.tasks_out = (budget0 > 0 && pass < 3) ? w.tasks[pass & 1].as<CcdTask>() : nullptr;
         if ((e = hipMemsetAsync(a.counter, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
         if ((e = hipMemsetAsync(a.n_tasks_out, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
         hipLaunchKernelGGL(ccd_kernel, dim3(grid), dim3(block), 0, stream, a);

@@ -101,6 +101,25 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const unsigned lo
     }
 }
 
+// Small inputs: one block ranks every key against all others in LDS
+// (stable: equal keys keep their order) and scatters in place.
+constexpr int RANK_SORT_MAX = 1024;
+
+__global__ __launch_bounds__(RANK_SORT_MAX) void rank_sort_kernel(unsigned long long* keys, int n) {
+    __shared__ unsigned long long k[RANK_SORT_MAX];
+    const int i = threadIdx.x;
+    if (i < n) k[i] = keys[i];
+    __syncthreads();
+    if (i >= n) return;
+    const unsigned long long mine = k[i];
+    int rank = 0;
+    for (int j = 0; j < n; j++) {
+        const unsigned long long o = k[j];
+        rank += (o < mine || (o == mine && j < i)) ? 1 : 0;
+    }
+    keys[rank] = mine;
+}
+
 }  // namespace
 
 // Sorts keys[0..n) ascending; tmp must hold n keys. Only the digits in
@@ -108,6 +127,10 @@ __global__ __launch_bounds__(SORT_THREADS) void scatter_kernel(const unsigned lo
 hipError_t radix_sort_u64(unsigned long long* keys, unsigned long long* tmp, unsigned int* hist, long long n,
                           int bits, hipStream_t stream) {
     if (n <= 1) return hipSuccess;
+    if (n <= RANK_SORT_MAX) {
+        hipLaunchKernelGGL(rank_sort_kernel, dim3(1), dim3(RANK_SORT_MAX), 0, stream, keys, (int)n);
+        return hipGetLastError();
+    }
     const int nblocks = (int)((n + TILE - 1) / TILE);
     unsigned long long* src = keys;
     unsigned long long* dst = tmp;

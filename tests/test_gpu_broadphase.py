@@ -126,6 +126,27 @@ def test_earliest_toi_matches_oracle(ctx, seed):
     assert got == best
 
 
+def test_earliest_toi_sequence_matches_oracle(ctx):
+    """Successive calls on one scene, as a line search makes them: the
+    device first searches below four times the previous answer and falls
+    back to the full search when nothing collides there; every answer must
+    still be the oracle's exact minimum (small after large, large after
+    small, none)."""
+    sc = cluster(3, n=27)
+    p0, p1 = moved(sc, 77, dpos=0.05, drot=0.1)
+    O.set_scene(sc)
+    O.set_eval_budget(0)
+    ctx.set_scene(sc)
+    got, want = [], []
+    for s in (1.0, 0.9, 0.3, 1.0, 0.02, 0.5, 1.0, 0.0):
+        ps = p0 + s * (p1 - p0)
+        want.append(O.earliest_toi(p0, ps, 0.0)[0])
+        got.append(ctx.compute_earliest_toi(p0, ps, 0.0))
+    print(f"earliest tois {want}", flush=True)
+    assert got == want
+    assert any(math.isfinite(w) for w in want) and want[-1] == math.inf
+
+
 def query_record(sc, p0, p1, kind, a, b):
     """The 36-double CCD record of one candidate (body-local vertices and the
     two bodies' poses), as the reference's edge_edge_ccd / face_vertex_ccd
