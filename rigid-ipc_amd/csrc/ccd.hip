@@ -45,6 +45,7 @@ namespace rigc {
 namespace {
 
 constexpr int MAX_DEPTH = 192;
+constexpr int THR_REFRESH = 8;
 
 struct Bits192 {
     unsigned long long w[3];
@@ -208,6 +209,9 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
     double tol[3];
     long long evals = 0;
     int walk = 0;       // evaluations since this lane took its subtree
+    double thr = 0.0;   // cached pruning bound of query thr_q
+    long long thr_q = -1;
+    int thr_age = 0;
     const long long npool = a.tasks_in ? a.n_tasks_in : nq;
 
     int depth = 0;
@@ -339,13 +343,21 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
             u = Iv{ 0.0, tol[1] };
             v = Iv{ 0.0, tol[2] };
         } else {
-            double thr = __longlong_as_double((long long)__hip_atomic_load(a.q_best + qi, __ATOMIC_RELAXED,
-                                                                            __HIP_MEMORY_SCOPE_AGENT));
-            if (a.use_global_best) {
-                const unsigned long long gb =
-                    __hip_atomic_load(a.global_best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                thr = fmin(thr, __longlong_as_double((long long)gb));
+            // The pruning bound only decreases; a stale (larger) bound prunes
+            // less but never changes the minimum, so it is refreshed from
+            // global memory every THR_REFRESH boxes (or on a new subtree).
+            if (thr_age <= 0 || thr_q != qi) {
+                thr = __longlong_as_double((long long)__hip_atomic_load(a.q_best + qi, __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT));
+                if (a.use_global_best) {
+                    const unsigned long long gb =
+                        __hip_atomic_load(a.global_best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    thr = fmin(thr, __longlong_as_double((long long)gb));
+                }
+                thr_q = qi;
+                thr_age = THR_REFRESH;
             }
+            thr_age--;
             if (a.eval_budget > 0 && a.q_flag[qi] != 0) done = true;
             while (!done) {
                 if (a.eval_budget > 0) {
@@ -418,6 +430,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 if ((idx[0] > 0 || all_zero) && all_sat) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong(t.lo);
                     atomicMin(a.q_best + qi, bits);
+                    thr = fmin(thr, t.lo);
                     if (a.use_global_best) atomicMin(a.global_best, bits);
                     leaf = true;
                 } else {

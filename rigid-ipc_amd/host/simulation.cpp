@@ -416,14 +416,27 @@ struct rigs_sim {
         return s;
     }
 
+    // NewtonSolver::converged, VELOCITY criterion (newton_solver.cpp:95-128):
+    // max |V(x + dir) - V(x)|_inf / h <= tol. Evaluated body by body with an
+    // early exit; bodies that do not move contribute zero.
     bool converged(const std::vector<double>& x, const std::vector<double>& dir) {
-        std::vector<double> W0, W1, x1(x);
-        for (size_t i = 0; i < x.size(); i++) x1[i] += dir[i];
-        world_vertices(x.data(), W0);
-        world_vertices(x1.data(), W1);
-        double m = 0;
-        for (size_t i = 0; i < W0.size(); i++) m = std::max(m, std::fabs(W1[i] - W0[i]));
-        return m / P.timestep <= P.velocity_conv_tol * bbox_diagonal;
+        const double limit = P.velocity_conv_tol * bbox_diagonal;
+        for (int b = 0; b < nb; b++) {
+            const double* d = &dir[6 * b];
+            if (d[0] == 0 && d[1] == 0 && d[2] == 0 && d[3] == 0 && d[4] == 0 && d[5] == 0) continue;
+            double x1[6], R0[9], R1[9];
+            for (int k = 0; k < 6; k++) x1[k] = x[6 * b + k] + d[k];
+            rigc::rotation_d(&x[6 * b + 3], R0);
+            rigc::rotation_d(x1 + 3, R1);
+            for (int i = vstart[b]; i < vstart[b + 1]; i++) {
+                double w0[3], w1[3];
+                rigc::world_vertex_d(R0, &x[6 * b], &V[3 * i], w0);
+                rigc::world_vertex_d(R1, x1, &V[3 * i], w1);
+                for (int k = 0; k < 3; k++)
+                    if (!(std::fabs(w1[k] - w0[k]) / P.timestep <= limit)) return false;
+            }
+        }
+        return true;
     }
 
     // NewtonSolver::line_search (:262-360)
