@@ -36,6 +36,12 @@ FP64_PEAK_TFLOPS = 78.6
 # op-counting build of the CPU restatement over the recorded CCD queries;
 # tests/test_bench_constants.py recomputes it.
 FLOPS_PER_BOX = 1015.7
+# HBM bytes per ccd_kernel launch from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs (profiles/ccd_kernel_traffic_pmc.md): FETCH_SIZE scaled
+# by 2.0 and WRITE_SIZE by 1.0, the factors at which the counters read a
+# 256 MiB device copy as 256 MiB read and 256 MiB written.
+CCD_TRAFFIC = (1832.4 * 2.0 + 137.8) * 1024   # ballistic CCD sequence
+SIM_CCD_TRAFFIC = None                         # set from the stepping workload's counters
 
 
 def log(msg):
@@ -58,13 +64,19 @@ def ballistic_steps(sc, n, dt=0.01):
     return out
 
 
-def ccd_roofline(evals, narrow_ms, calls):
-    if not narrow_ms or not evals:
+def ccd_roofline(evals, kernel_ms, launches, traffic=None):
+    """Roofline of the CCD traversal kernel: evaluations x FLOPS_PER_BOX per
+    launch over its average launch duration (HIP events recorded around each
+    launch on the context stream). traffic: HBM bytes per launch from the
+    FETCH_SIZE / WRITE_SIZE counters (None when not measured for the
+    workload)."""
+    if not kernel_ms or not evals or not launches:
         return None
-    achieved = evals * FLOPS_PER_BOX / (narrow_ms * 1e-3) / 1e12
+    avg_s = kernel_ms * 1e-3 / launches
+    achieved = evals / launches * FLOPS_PER_BOX / avg_s / 1e12
     return dict(bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP64_PEAK_TFLOPS,
-                traffic=None, kernel="ccd_kernel", flops_per_box=FLOPS_PER_BOX, box_evaluations=int(evals),
-                kernel_ms_total=narrow_ms, launches=int(calls), kernel_ms_avg=narrow_ms / max(calls, 1))
+                traffic=traffic, kernel="ccd_kernel", flops_per_box=FLOPS_PER_BOX, box_evaluations=int(evals),
+                launches=int(launches), kernel_ms_total=kernel_ms, kernel_us_avg=1e3 * kernel_ms / launches)
 
 
 def cpu_threads():
@@ -174,7 +186,7 @@ def bench_sim(args):
             ccd_candidate_pairs_per_s=(st["ccd_candidates"] / (ccd_dev_ms * 1e-3)) if ccd_dev_ms > 0 else None,
             ccd_candidate_pairs_per_s_wall=(st["ccd_candidates"] / (st["ccd_ms"] * 1e-3)) if st["ccd_ms"] > 0 else None,
             ccd_candidates=int(st["ccd_candidates"]), step_ms=[round(1e3 * t, 3) for t in per_step]),
-        roofline=ccd_roofline(st["ccd_box_evaluations"], st["ccd_narrow_phase_ms"], st["ccd_calls"]),
+        roofline=ccd_roofline(st["ccd_box_evaluations"], st["ccd_kernel_ms"], st["ccd_launches"], SIM_CCD_TRAFFIC),
     )
     if not args.no_cpu:
         iters, steps, t_cpu = cpu_baseline_sim(args.cpu_budget)
@@ -221,8 +233,8 @@ def bench_ccd(args):
     ctx.set_scene(sc)
     for p0, p1 in seq[:args.warmup]:
         ctx.compute_earliest_toi(p0, p1)
-    tot_cand = tot_evals = 0
-    bp_ms = np_ms = 0.0
+    tot_cand = tot_evals = launches = 0
+    bp_ms = np_ms = k_ms = 0.0
     t0 = time.perf_counter()
     for p0, p1 in seq[args.warmup:]:
         ctx.compute_earliest_toi(p0, p1)
@@ -231,6 +243,8 @@ def bench_ccd(args):
         tot_evals += st["box_evaluations"]
         bp_ms += st["broad_phase_ms"]
         np_ms += st["narrow_phase_ms"]
+        k_ms += st["ccd_kernel_ms"]
+        launches += st["ccd_launches"]
     wall = time.perf_counter() - t0
     ctx.close()
     line = dict(
@@ -241,11 +255,33 @@ def bench_ccd(args):
                              "rigc_compute_earliest_toi per step (poses_t -> inertial prediction)",
                     bodies=int(len(sc.group_ids)), candidates_per_step=tot_cand / args.steps,
                     broad_phase_ms_per_step=bp_ms / args.steps, narrow_phase_ms_per_step=np_ms / args.steps),
-        roofline=ccd_roofline(tot_evals, np_ms, args.steps),
+        roofline=ccd_roofline(tot_evals, k_ms, launches, CCD_TRAFFIC),
     )
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_ccd(sc, seq[args.warmup:], args.cpu_budget)
     return line
+
+
+def bench_copy(args):
+    """Calibration for the PMC traffic counters: device-to-device copies of a
+    known size (reads and writes 256 MiB each per launch)."""
+    import torch
+    n = 32 * 1024 * 1024  # doubles: 256 MiB
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nbytes = 2 * 8 * n
+    return dict(metric="device copy bandwidth (PMC calibration)", value=nbytes * args.steps / wall / 1e9,
+                unit="GB/s", n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=1e3 * wall / args.steps,
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload=f"torch copy_ of {8 * n} bytes (read) into {8 * n} bytes (write)"))
 
 
 def main():
@@ -253,7 +289,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sim", choices=["sim", "ccd"])
+    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "copy"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
@@ -269,7 +305,10 @@ def main():
         args.steps = 8 if args.workload == "sim" else 40
     if args.warmup is None:
         args.warmup = 1 if args.workload == "sim" else 3
-    line = bench_sim(args) if args.workload == "sim" else bench_ccd(args)
+    if args.workload == "copy":
+        line = bench_copy(args)
+    else:
+        line = bench_sim(args) if args.workload == "sim" else bench_ccd(args)
     print(json.dumps(line), flush=True)
 
 

@@ -61,6 +61,8 @@ typedef struct rigc_stats {
     int64_t box_evaluations;   /* inclusion-function evaluations (CCD) */
     int64_t num_candidates;    /* candidates passed to the narrow phase */
     int64_t num_items;         /* (element, cell) items sorted by the broad phase */
+    double ccd_kernel_ms;      /* the CCD traversal kernel launches alone (CCD) */
+    int64_t ccd_launches;      /* CCD traversal kernel launches (passes) of the last call */
 } rigc_stats;
 
 /* Create a context on HIP device `device` (ordinal). */

@@ -265,8 +265,6 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     a.q_flag = ctx->qflag.as<int>();
 
     const int block = 256;
-    const size_t blocks_needed = (nq + block - 1) / block;
-    (void)blocks_needed;
     const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
     a.walk_budget = ctx->walk_budget;
@@ -287,6 +285,8 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     ctx->stats.narrow_phase_ms = ms;
     ctx->stats.box_evaluations = (int64_t)res[2];
     ctx->stats.num_candidates = (int64_t)nq;
+    ctx->stats.ccd_kernel_ms = nq ? ctx->ccdw.kernel_ms : 0.0;
+    ctx->stats.ccd_launches = nq ? ctx->ccdw.passes : 0;
 
     bool unresolved = false;
     double best = INFINITY;
@@ -412,7 +412,8 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     double window = inf;
     if (ctx->last_toi < 0.25) window = std::max(4.0 * ctx->last_toi, std::ldexp(1.0, -20));
     double g = inf;
-    long long evals = 0;
+    long long evals = 0, launches = 0;
+    double kernel_ms = 0.0;
     auto run = [&](double bound) -> hipError_t {
         unsigned long long init[4] = { 0, 0, 0, 0 };
         std::memcpy(&init[0], &bound, sizeof(double));
@@ -423,6 +424,8 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         std::memcpy(&g, &res[0], sizeof(double));
         evals += (long long)res[2];
+        launches += ctx->ccdw.passes;
+        kernel_ms += ctx->ccdw.kernel_ms;
         return e;
     };
     if (window < inf) {
@@ -440,6 +443,8 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     ctx->stats.narrow_phase_ms = nms;
     ctx->stats.box_evaluations = evals;
     ctx->stats.num_candidates = nq;
+    ctx->stats.ccd_kernel_ms = kernel_ms;
+    ctx->stats.ccd_launches = launches;
     ctx->last_toi = g;
     *earliest_toi = g;  // +inf when no candidate collides
     return RIGC_OK;

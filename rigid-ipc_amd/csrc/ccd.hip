@@ -518,9 +518,12 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
 hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& w) {
     const long long nq = a.n_ee + a.n_fv;
     w.passes = 0;
+    w.kernel_ms = 0.0f;
     if (nq <= 0) return hipSuccess;
     const long long cap = 1LL << 18;
     hipError_t e;
+    if (!w.k0 && (e = hipEventCreate(&w.k0)) != hipSuccess) return e;
+    if (!w.k1 && (e = hipEventCreate(&w.k1)) != hipSuccess) return e;
     if ((e = w.tasks[0].ensure(sizeof(CcdTask) * cap)) != hipSuccess) return e;
     if ((e = w.tasks[1].ensure(sizeof(CcdTask) * cap)) != hipSuccess) return e;
     if ((e = w.count.ensure(sizeof(unsigned long long))) != hipSuccess) return e;
@@ -538,12 +541,17 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
         a.tasks_out = (budget0 > 0 && pass < 3) ? w.tasks[pass & 1].as<CcdTask>() : nullptr;
         if ((e = hipMemsetAsync(a.counter, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
         if ((e = hipMemsetAsync(a.n_tasks_out, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
+        if ((e = hipEventRecord(w.k0, stream)) != hipSuccess) return e;
         hipLaunchKernelGGL(ccd_kernel, dim3(grid), dim3(block), 0, stream, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(w.k1, stream)) != hipSuccess) return e;
         w.passes = pass + 1;
         unsigned long long n = 0;
         if ((e = hipMemcpyAsync(&n, a.n_tasks_out, sizeof(n), hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+        float ms = 0.0f;
+        if ((e = hipEventElapsedTime(&ms, w.k0, w.k1)) != hipSuccess) return e;
+        w.kernel_ms += ms;
         if (n == 0) break;
         a.tasks_in = w.tasks[pass & 1].as<CcdTask>();
         a.n_tasks_in = (long long)std::min<unsigned long long>(n, (unsigned long long)cap);

@@ -122,11 +122,16 @@ struct CcdArgs {
 // Task buffers of the multi-pass narrow phase (grow-only).
 struct CcdWork {
     DevBuf tasks[2], count;
-    int passes = 0;
+    int passes = 0;          // traversal launches of the last launch_ccd
+    float kernel_ms = 0.0f;  // their summed duration (events around each launch)
+    hipEvent_t k0 = nullptr, k1 = nullptr;
     void release() {
         tasks[0].release();
         tasks[1].release();
         count.release();
+        if (k0) hipEventDestroy(k0);
+        if (k1) hipEventDestroy(k1);
+        k0 = k1 = nullptr;
     }
 };
 

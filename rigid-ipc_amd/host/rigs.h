@@ -98,6 +98,8 @@ typedef struct rigs_stats {
     double ccd_broad_phase_ms;
     double ccd_narrow_phase_ms;
     int64_t ccd_box_evaluations;
+    double ccd_kernel_ms;        /* CCD traversal kernel launches alone */
+    int64_t ccd_launches;
 } rigs_stats;
 
 typedef struct rigs_sim rigs_sim;

@@ -227,6 +227,8 @@ struct rigs_sim {
                 stats.ccd_broad_phase_ms += cs.broad_phase_ms;
                 stats.ccd_narrow_phase_ms += cs.narrow_phase_ms;
                 stats.ccd_box_evaluations += cs.box_evaluations;
+                stats.ccd_kernel_ms += cs.ccd_kernel_ms;
+                stats.ccd_launches += cs.ccd_launches;
             }
         }
         return RIGC_OK;

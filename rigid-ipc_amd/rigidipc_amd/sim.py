@@ -51,6 +51,7 @@ class Stats(ctypes.Structure):
         ("kappa", ctypes.c_double), ("min_distance", ctypes.c_double), ("num_constraints", ctypes.c_int64),
         ("ccd_candidates", ctypes.c_int64), ("ccd_ms", ctypes.c_double), ("ccd_broad_phase_ms", ctypes.c_double),
         ("ccd_narrow_phase_ms", ctypes.c_double), ("ccd_box_evaluations", ctypes.c_int64),
+        ("ccd_kernel_ms", ctypes.c_double), ("ccd_launches", ctypes.c_int64),
     ]
 
 
