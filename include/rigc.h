@@ -77,6 +77,16 @@ rigc_status rigc_get_stats(const rigc_ctx* ctx, rigc_stats* stats);
  * lengths, per-body r_max and the mean edge length on the host. */
 rigc_status rigc_set_scene(rigc_ctx* ctx, const rigc_scene* scene);
 
+/* Upload several independent scenes into one context (batched simulation;
+ * no counterpart in the reference, which holds one scene per
+ * RigidBodyAssembler). Scene k keeps its own body / element numbering in
+ * the rigc_batch_* calls below; no body of one scene is ever paired with a
+ * body of another. The single-scene entry points above and below act on
+ * all scenes at once (energies summed, earliest TOI and minimum distance
+ * taken over all scenes). */
+rigc_status rigc_set_scenes(rigc_ctx* ctx, int32_t num_scenes, const rigc_scene* scenes);
+rigc_status rigc_get_scene_count(const rigc_ctx* ctx, int32_t* num_scenes);
+
 /* Narrow phase over given candidates. Replaces
  * DistanceBarrierConstraint::compute_earliest_toi_narrow_phase
  * (src/opt/distance_barrier_constraint.cpp:151-265) with edge_edge_ccd /
@@ -191,6 +201,39 @@ rigc_status rigc_get_friction_constraints(rigc_ctx* ctx, int64_t capacity, doubl
 rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const double* poses_t0,
                                        double epsv_times_h, int32_t flags, double* energy, double* grad,
                                        rigc_block_hessian* hess);
+
+/* ---- Batched scenes (rigc_set_scenes). One call serves the n scenes
+ * scene_ids[0..n) (strictly ascending) with one launch sequence. Pose
+ * arrays hold those scenes' bodies concatenated in list order ([sum nb][6]),
+ * and so do grad ([sum nb][6]) and hess->diag ([sum nb][36]); hess->pairs
+ * use the same concatenated body numbering. Per-scene results are equal,
+ * bit for bit, to those of a context holding that scene alone. */
+
+/* Per-scene rigc_compute_earliest_toi: toi[i] for scene scene_ids[i]. */
+rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* poses_t0,
+                                    const double* poses_t1, double minimum_separation_distance, double* toi);
+/* Per-scene rigc_compute_barrier_term with the constraint set built at
+ * build_poses (DistanceBarrierConstraint::construct_constraint_set) and the
+ * potential evaluated at poses; energy[i] per scene; counts (optional,
+ * [n][4]) receive each scene's VV / EV / EE / FV constraint counts. */
+rigc_status rigc_batch_barrier_term(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* build_poses,
+                                    const double* poses, double dhat, double dmin, int32_t flags, double* energy,
+                                    double* grad, rigc_block_hessian* hess, int64_t* counts);
+/* Per-scene rigc_compute_minimum_distance over the set built at build_poses. */
+rigc_status rigc_batch_minimum_distance(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids,
+                                        const double* build_poses, const double* poses, double dhat, double dmin,
+                                        double* min_distance);
+/* Per-scene rigc_update_friction_constraints (constraint set built at
+ * poses) with per-scene barrier_stiffness[i] and mu[i]; the other scenes'
+ * friction sets are kept. counts (optional, [n]) receive the set sizes. */
+rigc_status rigc_batch_update_friction_constraints(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids,
+                                                   const double* poses, double dhat, double dmin,
+                                                   const double* barrier_stiffness, const double* mu,
+                                                   int64_t* counts);
+/* Per-scene rigc_compute_friction_term with per-scene epsv_times_h[i]. */
+rigc_status rigc_batch_friction_term(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* poses,
+                                     const double* poses_t0, const double* epsv_times_h, int32_t flags,
+                                     double* energy, double* grad, rigc_block_hessian* hess);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,21 @@ __global__ __launch_bounds__(THREADS) void world_vertex_kernel(int nv, const dou
     world_vertex_d(R + 9 * b, poses + 6 * b, V + 3 * i, W + 3 * i);
 }
 
+// Frames and world vertices of listed bodies only (a block per body).
+__global__ __launch_bounds__(THREADS) void body_list_vertex_kernel(const int* bodies, const int* bvs, const double* V,
+                                                                  const double* poses, double* R, BodyFrame* F,
+                                                                  double* W) {
+    __shared__ double Rs[9];
+    const int b = bodies[blockIdx.x];
+    if (threadIdx.x == 0) {
+        rotation_d(poses + 6 * b + 3, Rs);
+        for (int k = 0; k < 9; k++) R[9 * b + k] = Rs[k];
+    }
+    if (F && threadIdx.x == 64) body_frame(poses + 6 * b + 3, F[b]);
+    __syncthreads();
+    for (int i = bvs[b] + threadIdx.x; i < bvs[b + 1]; i += THREADS) world_vertex_d(Rs, poses + 6 * b, V + 3 * i, W + 3 * i);
+}
+
 // --------------------------------------------------------------- constraint set
 struct ClassifyArgs {
     const int* ee;
@@ -275,6 +290,8 @@ struct LocalArgs {
     const double* W0;  // friction: at the start of the step
     const BodyFrame* frames;
     double dhat, dmin, epsv_h;
+    const double* epsv_h_scene;  // friction: per-scene epsv_h (nullptr: epsv_h)
+    const int* body_scene;
     int want_grad;
     double* energy;    // [n]
     double* lgrad;     // [n][12]
@@ -297,7 +314,8 @@ __device__ inline void local_setup(const LocalArgs& a, long long c, ConRec& rec,
             basis[k][0] = f.basis[2 * k];
             basis[k][1] = f.basis[2 * k + 1];
         }
-        friction_local_setup(Fl, rec.type, nv, f.coords, basis, f.normal_force, f.mu, rec.mult, X, a.epsv_h);
+        const double eh = a.epsv_h_scene ? a.epsv_h_scene[a.body_scene[rec.body[0]]] : a.epsv_h;
+        friction_local_setup(Fl, rec.type, nv, f.coords, basis, f.normal_force, f.mu, rec.mult, X, eh);
     } else {
         rec = a.cons[c];
         const int nv = con_nv(rec.type);
@@ -532,18 +550,87 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
 }
 
 // --------------------------------------------------------------- reduction
-// Deterministic sum of n doubles with one block.
-__global__ __launch_bounds__(1024) void sum_kernel(const double* x, long long n, double* out) {
+// Per-scene sums of per-constraint values, one block per selected scene.
+// The records are grouped by class; within a class the records of one scene
+// are contiguous and scenes appear in ascending order. Each scene's range of
+// each class is summed with the same thread partition relative to the range
+// start, and the class sums are added in class order, so a scene's sum does
+// not depend on which other scenes share the call.
+__device__ inline int rec_scene(const ConRec* cons, const FricRec* fric, const int* body_scene, long long i) {
+    return body_scene[fric ? fric[i].c.body[0] : cons[i].body[0]];
+}
+
+__global__ __launch_bounds__(1024) void scene_sum_kernel(const double* x, const ConRec* cons, const FricRec* fric,
+                                                        ClassRanges cr, const int* body_scene, const int* scenes,
+                                                        int whole, double* out) {
     __shared__ double part[1024];
-    double s = 0.0;
-    for (long long i = threadIdx.x; i < n; i += 1024) s += x[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __shared__ long long rng[2];
+    const int sc = scenes[blockIdx.x];
+    double acc = 0.0;
+    for (int k = 0; k < cr.n_class; k++) {
+        if (threadIdx.x == 0) {
+            long long lo = cr.off[k], hi = cr.off[k + 1];
+            if (!whole) {
+                long long a = lo, b = hi;
+                while (a < b) {
+                    const long long m = (a + b) >> 1;
+                    if (rec_scene(cons, fric, body_scene, m) < sc) a = m + 1;
+                    else b = m;
+                }
+                long long c = a, d = hi;
+                while (c < d) {
+                    const long long m = (c + d) >> 1;
+                    if (rec_scene(cons, fric, body_scene, m) <= sc) c = m + 1;
+                    else d = m;
+                }
+                lo = a;
+                hi = c;
+            }
+            rng[0] = lo;
+            rng[1] = hi;
+        }
+        __syncthreads();
+        double v = 0.0;
+        for (long long i = rng[0] + threadIdx.x; i < rng[1]; i += 1024) v += x[i];
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = 512; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) acc += part[0];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = part[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+
+// Per selected scene, the record range of each class ([lo, hi) pairs).
+__global__ void scene_class_ranges_kernel(const ConRec* cons, ClassRanges cr, const int* body_scene,
+                                          const int* scenes, int n_sel, long long* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_sel) return;
+    const int sc = scenes[i];
+    for (int k = 0; k < 4; k++) {
+        long long lo = 0, hi = 0;
+        if (k < cr.n_class) {
+            long long a = cr.off[k], b = cr.off[k + 1];
+            while (a < b) {
+                const long long m = (a + b) >> 1;
+                if (body_scene[cons[m].body[0]] < sc) a = m + 1;
+                else b = m;
+            }
+            long long c = a, d = cr.off[k + 1];
+            while (c < d) {
+                const long long m = (c + d) >> 1;
+                if (body_scene[cons[m].body[0]] <= sc) c = m + 1;
+                else d = m;
+            }
+            lo = a;
+            hi = c;
+        }
+        out[8 * i + 2 * k] = lo;
+        out[8 * i + 2 * k + 1] = hi;
+    }
 }
 
 // key = body << 32 | (2 c + side)
@@ -579,10 +666,10 @@ __device__ long long lower_bound_u64(const unsigned long long* k, long long n, u
 
 // Block per body: lanes 0..35 the diagonal 6x6 block, lanes 36..41 the
 // gradient, summed in sorted (constraint) order.
-__global__ __launch_bounds__(64) void body_reduce_kernel(const unsigned long long* keys, long long nk,
-                                                        const double* lgrad, const double* lhess, int want_hess,
-                                                        double* grad, double* diag) {
-    const int b = blockIdx.x, lane = threadIdx.x;
+__global__ __launch_bounds__(64) void body_reduce_kernel(const int* bodies, const unsigned long long* keys,
+                                                        long long nk, const double* lgrad, const double* lhess,
+                                                        int want_hess, double* grad, double* diag) {
+    const int b = bodies[blockIdx.x], lane = threadIdx.x;
     const long long lo = lower_bound_u64(keys, nk, (unsigned long long)b << 32);
     const long long hi = lower_bound_u64(keys, nk, (unsigned long long)(b + 1) << 32);
     if (lane < 36) {
@@ -647,11 +734,14 @@ __global__ __launch_bounds__(64) void pair_reduce_kernel(const unsigned long lon
 
 // --------------------------------------------------------------- friction set
 __global__ __launch_bounds__(THREADS) void friction_record_kernel(const ConRec* cons, long long n, const double* W,
-                                                                 double dhat, double dmin, double kappa, double mu,
+                                                                 double dhat, double dmin, const double* kappa_scene,
+                                                                 const double* mu_scene, const int* body_scene,
                                                                  FricRec* out) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const ConRec r = cons[c];
+    const int sc = body_scene[r.body[0]];
+    const double kappa = kappa_scene[sc], mu = mu_scene[sc];
     const int nv = con_nv(r.type);
     double X[12];
     for (int s = 0; s < 4; s++)
@@ -676,7 +766,7 @@ __global__ __launch_bounds__(THREADS) void friction_record_kernel(const ConRec* 
 // Minimum squared distance over the set (non-negative doubles order as
 // their bit patterns).
 __global__ __launch_bounds__(THREADS) void min_distance_kernel(const ConRec* cons, long long n, const double* W,
-                                                              unsigned long long* out) {
+                                                              const int* body_scene, unsigned long long* out_scene) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const ConRec r = cons[c];
@@ -688,38 +778,117 @@ __global__ __launch_bounds__(THREADS) void min_distance_kernel(const ConRec* con
     constraint_form(r.type, X, f);
     unsigned long long bits;
     memcpy(&bits, &f.d, sizeof(bits));
-    atomicMin(out, bits);
+    atomicMin(out_scene + body_scene[r.body[0]], bits);
 }
 
 unsigned grid_for(long long n, int threads = THREADS) { return (unsigned)((n + threads - 1) / threads); }
 
-hipError_t world_vertices(Assembly& as, const DevScene& sc, const double* d_poses, DevBuf& W, bool frames,
-                          hipStream_t st) {
+hipError_t world_vertices(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses, DevBuf& W,
+                          bool frames, hipStream_t st) {
     AS_CHECK(as.R.ensure(9 * sizeof(double) * sc.nb));
     AS_CHECK(as.frames.ensure(sizeof(BodyFrame) * sc.nb));
     AS_CHECK(W.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
-    hipLaunchKernelGGL(body_frame_kernel, dim3(grid_for(sc.nb)), dim3(THREADS), 0, st, sc.nb, d_poses,
-                       as.R.as<double>(), frames ? as.frames.as<BodyFrame>() : nullptr);
-    if (sc.nv)
-        hipLaunchKernelGGL(world_vertex_kernel, dim3(grid_for(sc.nv)), dim3(THREADS), 0, st, sc.nv, sc.V, sc.v_body,
-                           d_poses, as.R.as<double>(), W.as<double>());
+    if (sel.all) {
+        hipLaunchKernelGGL(body_frame_kernel, dim3(grid_for(sc.nb)), dim3(THREADS), 0, st, sc.nb, d_poses,
+                           as.R.as<double>(), frames ? as.frames.as<BodyFrame>() : nullptr);
+        if (sc.nv)
+            hipLaunchKernelGGL(world_vertex_kernel, dim3(grid_for(sc.nv)), dim3(THREADS), 0, st, sc.nv, sc.V,
+                               sc.v_body, d_poses, as.R.as<double>(), W.as<double>());
+    } else if (sel.n_bodies) {
+        hipLaunchKernelGGL(body_list_vertex_kernel, dim3((unsigned)sel.n_bodies), dim3(THREADS), 0, st, sel.bodies,
+                           sc.body_vstart, sc.V, d_poses, as.R.as<double>(),
+                           frames ? as.frames.as<BodyFrame>() : nullptr, W.as<double>());
+    }
+    return hipGetLastError();
+}
+
+__global__ void gather_rows_kernel(const int* rows, int n, int width, const double* src, double* dst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * width) return;
+    const long long r = i / width, k = i % width;
+    dst[i] = src[(long long)rows[r] * width + k];
+}
+
+__global__ void scatter_rows_kernel(const int* rows, int n, int width, const double* src, double* dst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * width) return;
+    const long long r = i / width, k = i % width;
+    dst[(long long)rows[r] * width + k] = src[i];
+}
+
+// Byte-range copies: entry e = (source 0 / 1, source offset, destination
+// offset, bytes), all multiples of 16; one block per entry.
+__global__ __launch_bounds__(THREADS) void copy_ranges_kernel(const long long* ent, const char* src0, const char* src1,
+                                                             char* dst) {
+    const long long* e = ent + 4 * blockIdx.x;
+    const uint4* s = reinterpret_cast<const uint4*>((e[0] ? src1 : src0) + e[1]);
+    uint4* d = reinterpret_cast<uint4*>(dst + e[2]);
+    const long long n = e[3] / 16;
+    for (long long i = threadIdx.x; i < n; i += THREADS) d[i] = s[i];
+}
+
+ClassRanges cons_ranges(const Assembly& as) {
+    ClassRanges cr{};
+    cr.n_class = 4;
+    cr.off[0] = 0;
+    for (int k = 0; k < 4; k++) cr.off[k + 1] = cr.off[k] + as.counts[k];
+    return cr;
+}
+
+// Per selected scene, [lo, hi) of each class of the current constraint set
+// (host copy, [n_sel][8]).
+hipError_t scene_ranges(Assembly& as, const DevScene& sc, const SceneSel& sel, std::vector<long long>& h,
+                        hipStream_t st) {
+    h.assign(8 * (size_t)sel.n_scenes, 0);
+    if (sel.n_scenes == 0 || as.n_con == 0) return hipSuccess;
+    AS_CHECK(as.ranges.ensure(8 * sizeof(long long) * sel.n_scenes));
+    hipLaunchKernelGGL(scene_class_ranges_kernel, dim3(grid_for(sel.n_scenes)), dim3(THREADS), 0, st,
+                       as.cons.as<ConRec>(), cons_ranges(as), sc.body_scene, sel.scenes, sel.n_scenes,
+                       as.ranges.as<long long>());
+    AS_CHECK(hipGetLastError());
+    AS_CHECK(hipMemcpyAsync(h.data(), as.ranges.p, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+hipError_t run_copies(Assembly& as, const std::vector<long long>& ent, const void* src0, const void* src1, void* dst,
+                      hipStream_t st) {
+    const long long n = (long long)ent.size() / 4;
+    if (n == 0) return hipSuccess;
+    as.h_entries = ent;
+    AS_CHECK(as.entries.ensure(sizeof(long long) * ent.size()));
+    AS_CHECK(hipMemcpyAsync(as.entries.p, as.h_entries.data(), sizeof(long long) * ent.size(), hipMemcpyHostToDevice,
+                            st));
+    hipLaunchKernelGGL(copy_ranges_kernel, dim3((unsigned)n), dim3(THREADS), 0, st, as.entries.as<long long>(),
+                       static_cast<const char*>(src0), static_cast<const char*>(src1), static_cast<char*>(dst));
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc, const double* d_poses,
-                                double dhat, double dmin, int num_cus, hipStream_t st) {
+void gather_rows(const int* rows, int n, int width, const double* src, double* dst, hipStream_t st) {
+    if (n > 0)
+        hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for((long long)n * width)), dim3(THREADS), 0, st, rows, n,
+                           width, src, dst);
+}
+
+void scatter_rows(const int* rows, int n, int width, const double* src, double* dst, hipStream_t st) {
+    if (n > 0)
+        hipLaunchKernelGGL(scatter_rows_kernel, dim3(grid_for((long long)n * width)), dim3(THREADS), 0, st, rows, n,
+                           width, src, dst);
+}
+
+hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc, const SceneSel& sel,
+                                const double* d_poses, double dhat, double dmin, int num_cus, hipStream_t st) {
     (void)num_cus;
-    as.have_cons = false;  // the lagged friction set is independent and survives
+    as.have_cons = false;  // the lagged friction sets are independent and survive
     as.n_con = 0;
     for (int k = 0; k < 4; k++) as.counts[k] = 0;
-    if (sc.nb <= 1) {
+    if (sc.nb <= 1 || sel.n_bodies <= 1) {
         as.have_cons = true;
         return hipSuccess;
     }
-    AS_CHECK(run_broad_phase(bp, sc, d_poses, d_poses, (dhat + dmin) / 2.0, false, st));
-    AS_CHECK(world_vertices(as, sc, d_poses, as.W, false, st));
+    AS_CHECK(run_broad_phase(bp, sc, sel, d_poses, d_poses, (dhat + dmin) / 2.0, false, st));
+    AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, false, st));
     const long long nc = bp.n_ee + bp.n_fv;
     if (nc == 0) {
         as.have_cons = true;
@@ -810,64 +979,150 @@ hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc
     return hipSuccess;
 }
 
-hipError_t build_friction_set(Assembly& as, const DevScene& sc, const double* d_poses, double dhat, double dmin,
-                              double kappa, double mu, hipStream_t st) {
-    as.have_fric = false;
-    as.n_fric = as.n_con;
-    AS_CHECK(as.fric.ensure(sizeof(FricRec) * std::max<long long>(as.n_con, 1)));
+hipError_t constraint_counts(Assembly& as, const DevScene& sc, const SceneSel& sel, std::vector<long long>& counts,
+                             hipStream_t st) {
+    counts.assign(4 * (size_t)sel.n_scenes, 0);
+    if (sc.n_scenes == 1) {
+        for (int k = 0; k < 4 && sel.n_scenes == 1; k++) counts[k] = as.counts[k];
+        return hipSuccess;
+    }
+    std::vector<long long> h;
+    AS_CHECK(scene_ranges(as, sc, sel, h, st));
+    for (int i = 0; i < sel.n_scenes; i++)
+        for (int k = 0; k < 4; k++) counts[4 * i + k] = h[8 * i + 2 * k + 1] - h[8 * i + 2 * k];
+    return hipSuccess;
+}
+
+hipError_t build_friction_set(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                              double dhat, double dmin, const double* d_kappa_scene, const double* d_mu_scene,
+                              hipStream_t st) {
+    const int ns = sc.n_scenes;
+    if ((int)as.fric_off.size() != ns + 1) {
+        as.fric_off.assign(ns + 1, 0);
+        as.fric_valid.assign(ns, 0);
+    }
+    AS_CHECK(as.fric_new.ensure(sizeof(FricRec) * std::max<long long>(as.n_con, 1)));
     if (as.n_con) {
-        AS_CHECK(world_vertices(as, sc, d_poses, as.W, false, st));
+        AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, false, st));
         hipLaunchKernelGGL(friction_record_kernel, dim3(grid_for(as.n_con)), dim3(THREADS), 0, st,
-                           as.cons.as<ConRec>(), as.n_con, as.W.as<double>(), dhat, dmin, kappa, mu,
-                           as.fric.as<FricRec>());
+                           as.cons.as<ConRec>(), as.n_con, as.W.as<double>(), dhat, dmin, d_kappa_scene, d_mu_scene,
+                           sc.body_scene, as.fric_new.as<FricRec>());
         AS_CHECK(hipGetLastError());
     }
+    // Merge: the selected scenes' new records (scene-major, class order kept
+    // within a scene) and the other scenes' current records.
+    std::vector<long long> rng;
+    if (ns == 1) {
+        rng.assign(8, 0);
+        for (int k = 0, o = 0; k < 4; k++) {
+            rng[2 * k] = o;
+            rng[2 * k + 1] = o + as.counts[k];
+            o += (int)as.counts[k];
+        }
+    } else {
+        AS_CHECK(scene_ranges(as, sc, sel, rng, st));
+    }
+    std::vector<long long> new_off(ns + 1, 0), ent;
+    std::vector<int> pos(ns, -1);
+    for (int i = 0; i < sel.n_scenes; i++) pos[sel.h_scenes[i]] = i;
+    const long long R = (long long)sizeof(FricRec);
+    for (int sidx = 0; sidx < ns; sidx++) {
+        long long cnt = 0;
+        if (pos[sidx] >= 0) {
+            const long long* r = &rng[8 * pos[sidx]];
+            for (int k = 0; k < 4; k++) {
+                const long long m = r[2 * k + 1] - r[2 * k];
+                if (m > 0) {
+                    ent.insert(ent.end(), { 0, r[2 * k] * R, (new_off[sidx] + cnt) * R, m * R });
+                    cnt += m;
+                }
+            }
+        } else if (as.fric_valid[sidx]) {
+            cnt = as.fric_off[sidx + 1] - as.fric_off[sidx];
+            if (cnt > 0) ent.insert(ent.end(), { 1, as.fric_off[sidx] * R, new_off[sidx] * R, cnt * R });
+        }
+        new_off[sidx + 1] = new_off[sidx] + cnt;
+    }
+    const long long total = new_off[ns];
+    AS_CHECK(as.fric_tmp.ensure(sizeof(FricRec) * std::max<long long>(total, 1)));
+    AS_CHECK(run_copies(as, ent, as.fric_new.p, as.fric.p, as.fric_tmp.p, st));
+    std::swap(as.fric, as.fric_tmp);
+    as.fric_off = new_off;
+    for (int i = 0; i < sel.n_scenes; i++) as.fric_valid[sel.h_scenes[i]] = 1;
+    as.n_fric = total;
     as.have_fric = true;
-    return hipSuccess;
+    for (int sidx = 0; sidx < ns; sidx++) as.have_fric = as.have_fric && as.fric_valid[sidx];
+    return hipStreamSynchronize(st);
 }
 
-hipError_t minimum_distance(Assembly& as, const DevScene& sc, const double* d_poses, double& min_sq,
-                            hipStream_t st) {
-    min_sq = HUGE_VAL;
+hipError_t minimum_distance(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                            std::vector<double>& min_sq, hipStream_t st) {
+    min_sq.assign(sel.n_scenes, HUGE_VAL);
     if (as.n_con == 0) return hipSuccess;
-    AS_CHECK(world_vertices(as, sc, d_poses, as.W, false, st));
-    AS_CHECK(as.mind.ensure(sizeof(unsigned long long)));
-    const unsigned long long inf_bits = 0x7ff0000000000000ULL;
-    AS_CHECK(hipMemcpyAsync(as.mind.p, &inf_bits, sizeof(inf_bits), hipMemcpyHostToDevice, st));
+    AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, false, st));
+    AS_CHECK(as.mind.ensure(sizeof(unsigned long long) * sc.n_scenes));
+    const std::vector<unsigned long long> inf_bits(sc.n_scenes, 0x7ff0000000000000ULL);
+    AS_CHECK(hipMemcpyAsync(as.mind.p, inf_bits.data(), sizeof(unsigned long long) * sc.n_scenes,
+                            hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(min_distance_kernel, dim3(grid_for(as.n_con)), dim3(THREADS), 0, st, as.cons.as<ConRec>(),
-                       as.n_con, as.W.as<double>(), as.mind.as<unsigned long long>());
+                       as.n_con, as.W.as<double>(), sc.body_scene, as.mind.as<unsigned long long>());
     AS_CHECK(hipGetLastError());
-    unsigned long long bits = 0;
-    AS_CHECK(hipMemcpyAsync(&bits, as.mind.p, sizeof(bits), hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> bits(sc.n_scenes);
+    AS_CHECK(hipMemcpyAsync(bits.data(), as.mind.p, sizeof(unsigned long long) * sc.n_scenes, hipMemcpyDeviceToHost,
+                            st));
     AS_CHECK(hipStreamSynchronize(st));
-    std::memcpy(&min_sq, &bits, sizeof(min_sq));
+    for (int i = 0; i < sel.n_scenes; i++) std::memcpy(&min_sq[i], &bits[sel.h_scenes[i]], sizeof(double));
     return hipSuccess;
 }
 
-hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, const double* d_poses,
-                              const double* d_poses_t0, double dhat, double dmin, double epsv_h, bool want_grad,
-                              bool want_hess, int num_cus, AssemblyOut& out, hipStream_t st) {
+hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& sel, bool friction,
+                              const double* d_poses, const double* d_poses_t0, double dhat, double dmin,
+                              double epsv_h, const double* d_epsv_h_scene, bool want_grad, bool want_hess,
+                              int num_cus, AssemblyOut& out, hipStream_t st) {
     out = AssemblyOut{};
+    out.energy.assign(sel.n_scenes, 0.0);
     as.n_off = 0;
-    const long long n = friction ? as.n_fric : as.n_con;
     const int nb = sc.nb;
     want_grad = want_grad || want_hess;
     AS_CHECK(as.grad.ensure(6 * sizeof(double) * nb));
     AS_CHECK(as.diag.ensure(36 * sizeof(double) * nb));
+
+    // the records: the constraint set, or the selected scenes' friction sets
+    const FricRec* fric = nullptr;
+    long long n = as.n_con;
+    if (friction) {
+        if (sel.all) {
+            fric = as.fric.as<FricRec>();
+            n = as.n_fric;
+        } else {
+            std::vector<long long> ent;
+            n = 0;
+            const long long R = (long long)sizeof(FricRec);
+            for (int i = 0; i < sel.n_scenes; i++) {
+                const int sidx = sel.h_scenes[i];
+                const long long m = as.fric_off[sidx + 1] - as.fric_off[sidx];
+                if (m > 0) ent.insert(ent.end(), { 0, as.fric_off[sidx] * R, n * R, m * R });
+                n += m;
+            }
+            AS_CHECK(as.fric_stage.ensure(sizeof(FricRec) * std::max<long long>(n, 1)));
+            AS_CHECK(run_copies(as, ent, as.fric.p, as.fric.p, as.fric_stage.p, st));
+            fric = as.fric_stage.as<FricRec>();
+        }
+    }
     if (n == 0) {
         if (want_grad) AS_CHECK(hipMemsetAsync(as.grad.p, 0, 6 * sizeof(double) * nb, st));
         if (want_hess) AS_CHECK(hipMemsetAsync(as.diag.p, 0, 36 * sizeof(double) * nb, st));
         return hipStreamSynchronize(st);
     }
-    AS_CHECK(world_vertices(as, sc, d_poses, as.W, want_grad, st));
-    if (friction) AS_CHECK(world_vertices(as, sc, d_poses_t0, as.W0, false, st));  // frames stay at d_poses
+    AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, want_grad, st));
+    if (friction) AS_CHECK(world_vertices(as, sc, sel, d_poses_t0, as.W0, false, st));  // frames stay at d_poses
     AS_CHECK(as.energy.ensure(sizeof(double) * n));
     AS_CHECK(as.lgrad.ensure(12 * sizeof(double) * n));
     if (want_hess) AS_CHECK(as.lhess.ensure(144 * sizeof(double) * n));
-    AS_CHECK(as.esum.ensure(sizeof(double)));
+    AS_CHECK(as.esum.ensure(sizeof(double) * std::max(sel.n_scenes, 1)));
     LocalArgs la;
     la.cons = as.cons.as<ConRec>();
-    la.fric = friction ? as.fric.as<FricRec>() : nullptr;
+    la.fric = fric;
     la.n = n;
     la.V = sc.V;
     la.v_body = sc.v_body;
@@ -877,6 +1132,8 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, c
     la.dhat = dhat;
     la.dmin = dmin;
     la.epsv_h = epsv_h;
+    la.epsv_h_scene = d_epsv_h_scene;
+    la.body_scene = sc.body_scene;
     la.want_grad = want_grad ? 1 : 0;
     la.energy = as.energy.as<double>();
     la.lgrad = as.lgrad.as<double>();
@@ -890,10 +1147,19 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, c
         else hipLaunchKernelGGL(local_eg_kernel<false>, dim3(grid_for(n)), dim3(THREADS), 0, st, la);
     }
     AS_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(1024), 0, st, as.energy.as<double>(), n, as.esum.as<double>());
+    ClassRanges cr{};
+    if (friction) {
+        cr.n_class = 1;
+        cr.off[0] = 0;
+        cr.off[1] = n;
+    } else {
+        cr = cons_ranges(as);
+    }
+    hipLaunchKernelGGL(scene_sum_kernel, dim3((unsigned)sel.n_scenes), dim3(1024), 0, st, as.energy.as<double>(),
+                       friction ? nullptr : as.cons.as<ConRec>(), fric, cr, sc.body_scene, sel.scenes,
+                       sc.n_scenes == 1 ? 1 : 0, as.esum.as<double>());
 
     const ConRec* cons = friction ? nullptr : as.cons.as<ConRec>();
-    const FricRec* fric = friction ? as.fric.as<FricRec>() : nullptr;
     if (want_grad) {
         const long long nk = 2 * n;
         AS_CHECK(as.dkeys.ensure(sizeof(unsigned long long) * nk));
@@ -904,9 +1170,10 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, c
         const int bits = 32 + bits_for((unsigned long long)nb);
         AS_CHECK(radix_sort_u64(as.dkeys.as<unsigned long long>(), as.ktmp.as<unsigned long long>(),
                                 as.khist.as<unsigned int>(), nk, bits, st));
-        hipLaunchKernelGGL(body_reduce_kernel, dim3(nb), dim3(64), 0, st, as.dkeys.as<unsigned long long>(), nk,
-                           as.lgrad.as<double>(), want_hess ? as.lhess.as<double>() : nullptr, want_hess ? 1 : 0,
-                           as.grad.as<double>(), as.diag.as<double>());
+        hipLaunchKernelGGL(body_reduce_kernel, dim3((unsigned)sel.n_bodies), dim3(64), 0, st, sel.bodies,
+                           as.dkeys.as<unsigned long long>(), nk, as.lgrad.as<double>(),
+                           want_hess ? as.lhess.as<double>() : nullptr, want_hess ? 1 : 0, as.grad.as<double>(),
+                           as.diag.as<double>());
         AS_CHECK(hipGetLastError());
     }
     if (want_hess) {
@@ -936,13 +1203,14 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, c
         AS_CHECK(hipGetLastError());
         unsigned int np = 0;
         AS_CHECK(hipMemcpyAsync(&np, total, sizeof(np), hipMemcpyDeviceToHost, st));
-        AS_CHECK(hipMemcpyAsync(&out.energy, as.esum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        AS_CHECK(hipMemcpyAsync(out.energy.data(), as.esum.p, sizeof(double) * sel.n_scenes, hipMemcpyDeviceToHost,
+                                st));
         AS_CHECK(hipStreamSynchronize(st));
         as.n_off = np;
         out.n_off = np;
         return hipSuccess;
     }
-    AS_CHECK(hipMemcpyAsync(&out.energy, as.esum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    AS_CHECK(hipMemcpyAsync(out.energy.data(), as.esum.p, sizeof(double) * sel.n_scenes, hipMemcpyDeviceToHost, st));
     return hipStreamSynchronize(st);
 }
 

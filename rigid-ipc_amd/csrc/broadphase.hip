@@ -55,12 +55,13 @@ __device__ __forceinline__ void block_minmax6(double* v, double (*red)[BP_THREAD
     __syncthreads();
 }
 
-__global__ __launch_bounds__(BP_THREADS) void body_box_kernel(const double* V, const int* bvs, const uint8_t* is_static,
-                                                             const double* r_max, const double* p0, const double* p1,
-                                                             double infl, double* box_raw, double* box_inf) {
+__global__ __launch_bounds__(BP_THREADS) void body_box_kernel(const int* bodies, const double* V, const int* bvs,
+                                                             const uint8_t* is_static, const double* r_max,
+                                                             const double* p0, const double* p1, double infl,
+                                                             double* box_raw, double* box_inf) {
     __shared__ double R0[9], R1[9];
     __shared__ double red[6][BP_THREADS];
-    const int b = blockIdx.x, tid = threadIdx.x;
+    const int b = bodies[blockIdx.x], tid = threadIdx.x;
     const double* q0 = p0 + 6 * b;
     const double* q1 = p1 + 6 * b;
     const bool same = q0[3] == q1[3] && q0[4] == q1[4] && q0[5] == q1[5];
@@ -96,55 +97,58 @@ __global__ __launch_bounds__(BP_THREADS) void body_box_kernel(const double* V, c
 }
 
 // Body i is included when its box overlaps the box of a body of another
-// group. One 64-lane block per (i tile, j tile) of the nb x nb pair grid;
+// group in its scene. One 64-lane block per upper-triangular tile of a
+// scene's body-pair grid; both bodies of an overlapping pair are marked.
 // inc[] is cleared beforehand and only ever set to 1.
 constexpr int PAIR_TILE = 64;
 
-__global__ __launch_bounds__(PAIR_TILE) void included_kernel(int nb, const int* group, const double* box_inf,
-                                                            int* inc) {
+__global__ __launch_bounds__(PAIR_TILE) void included_kernel(const int4* tiles, const int* group,
+                                                            const double* box_inf, int* inc) {
     __shared__ double tb[PAIR_TILE][6];
     __shared__ int tg[PAIR_TILE];
-    const int i = blockIdx.x * PAIR_TILE + threadIdx.x;
-    const int j0 = blockIdx.y * PAIR_TILE;
+    const int4 T = tiles[blockIdx.x];
+    const int end = T.z;
+    const int i = T.x + threadIdx.x;
+    const int j0 = T.y;
     const int jt = j0 + threadIdx.x;
-    if (jt < nb) {
+    if (jt < end) {
         for (int k = 0; k < 6; k++) tb[threadIdx.x][k] = box_inf[6 * jt + k];
         tg[threadIdx.x] = group[jt];
     }
     __syncthreads();
-    if (i >= nb) return;
+    if (i >= end) return;
     double mi[6];
     for (int k = 0; k < 6; k++) mi[k] = box_inf[6 * i + k];
     const int gi = group[i];
-    const int m = min(PAIR_TILE, nb - j0);
+    const int m = min(PAIR_TILE, end - j0);
+    bool hit_i = false;
     for (int jj = 0; jj < m; jj++) {
-        if (j0 + jj == i || tg[jj] == gi) continue;
+        const int j = j0 + jj;
+        if (j <= i || tg[jj] == gi) continue;
         const double* o = tb[jj];
         if (mi[0] <= o[3] && o[0] <= mi[3] && mi[1] <= o[4] && o[1] <= mi[4] && mi[2] <= o[5] && o[2] <= mi[5]) {
-            inc[i] = 1;
-            return;
+            hit_i = true;
+            inc[j] = 1;
         }
     }
+    if (hit_i) inc[i] = 1;
 }
 
-__global__ __launch_bounds__(BP_THREADS) void domain_kernel(int nb, const int* inc, const double* box_raw, double r,
-                                                           double* dom, int* any_out) {
+// Domain of each selected scene: one block per scene over its bodies.
+__global__ __launch_bounds__(BP_THREADS) void domain_kernel(const int* scenes, const int* scene_body, const int* inc,
+                                                           const double* box_raw, double r, double* dom_all) {
     __shared__ double red[6][BP_THREADS];
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
+    const int sc = scenes[blockIdx.x];
+    double* dom = dom_all + 8 * sc;
     double v[6];
     for (int k = 0; k < 3; k++) { v[k] = INFINITY; v[3 + k] = -INFINITY; }
-    int c = 0;
-    for (int b = threadIdx.x; b < nb; b += BP_THREADS)
+    for (int b = scene_body[sc] + threadIdx.x; b < scene_body[sc + 1]; b += BP_THREADS)
         if (inc[b]) {
-            c++;
             for (int k = 0; k < 3; k++) {
                 v[k] = fmin(v[k], box_raw[6 * b + k]);
                 v[3 + k] = fmax(v[3 + k], box_raw[6 * b + 3 + k]);
             }
         }
-    if (c) atomicAdd(&cnt, c);
     block_minmax6(v, red);
     if (threadIdx.x == 0) {
         const double dx = v[3] - v[0], dy = v[4] - v[1], dz = v[5] - v[2];
@@ -156,14 +160,14 @@ __global__ __launch_bounds__(BP_THREADS) void domain_kernel(int nb, const int* i
             dom[k] = mn;
             dom[3 + k] = mx;
         }
-        *any_out = cnt;
     }
 }
 
 // Interval enclosure of every vertex trajectory of an included body.
-__global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const double* V, const int* bvs, const int* inc,
+__global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* bodies, const int* body_scene,
+                                                                    const double* V, const int* bvs, const int* inc,
                                                                     const double* p0, const double* p1,
-                                                                    const double* dom, double r, double* vmin,
+                                                                    const double* dom_all, double r, double* vmin,
                                                                     double* vmax, double* bhull) {
     __shared__ unsigned long long s_idx[64];
     __shared__ int s_lev[64], s_force[64];
@@ -172,8 +176,9 @@ __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const doubl
     __shared__ int c_lev, c_force;
     __shared__ Iv Rs[3][3], ps[3];
     __shared__ double red[6][BP_THREADS];
-    const int b = blockIdx.x, tid = threadIdx.x;
+    const int b = bodies[blockIdx.x], tid = threadIdx.x;
     if (!inc[b]) return;
+    const double* dom = dom_all + 8 * body_scene[b];
     const int v0 = bvs[b], v1 = bvs[b + 1];
     const double* q0 = p0 + 6 * b;
     const double* q1 = p1 + 6 * b;
@@ -258,12 +263,12 @@ __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const doubl
 }
 
 // Discrete detection: world vertex +- r.
-__global__ __launch_bounds__(BP_THREADS) void vertex_static_kernel(const double* V, const int* bvs, const int* inc,
-                                                                  const double* poses, double r, double* vmin,
-                                                                  double* vmax, double* bhull) {
+__global__ __launch_bounds__(BP_THREADS) void vertex_static_kernel(const int* bodies, const double* V, const int* bvs,
+                                                                  const int* inc, const double* poses, double r,
+                                                                  double* vmin, double* vmax, double* bhull) {
     __shared__ double R[9];
     __shared__ double red[6][BP_THREADS];
-    const int b = blockIdx.x, tid = threadIdx.x;
+    const int b = bodies[blockIdx.x], tid = threadIdx.x;
     if (!inc[b]) return;
     const double* q = poses + 6 * b;
     if (tid == 0) rotation_d(q + 3, R);
@@ -294,34 +299,33 @@ __device__ __forceinline__ void center_half(const double* mn, const double* mx, 
     }
 }
 
-__global__ void element_aabb_kernel(int nv, int ne, int nf, const int* E, const int* F, const int* v_body,
-                                    const int* inc, const double* vmin, const double* vmax, double* vc, double* vh,
-                                    double* ec, double* eh, double* fc, double* fh) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nv) {
-        if (inc[v_body[i]]) center_half(vmin + 3 * i, vmax + 3 * i, vc + 3 * i, vh + 3 * i);
-    } else if (i < nv + ne) {
-        const long long e = i - nv;
+// Element boxes of one included body per block.
+__global__ __launch_bounds__(BP_THREADS) void element_aabb_kernel(const int* bodies, const int* bvs, const int* bes,
+                                                                 const int* bfs, const int* E, const int* F,
+                                                                 const int* inc, const double* vmin,
+                                                                 const double* vmax, double* vc, double* vh,
+                                                                 double* ec, double* eh, double* fc, double* fh) {
+    const int b = bodies[blockIdx.x];
+    if (!inc[b]) return;
+    for (int i = bvs[b] + threadIdx.x; i < bvs[b + 1]; i += BP_THREADS)
+        center_half(vmin + 3 * i, vmax + 3 * i, vc + 3 * i, vh + 3 * i);
+    for (int e = bes[b] + threadIdx.x; e < bes[b + 1]; e += BP_THREADS) {
         const int a = E[2 * e], c = E[2 * e + 1];
-        if (inc[v_body[a]]) {
-            double mn[3], mx[3];
-            for (int k = 0; k < 3; k++) {
-                mn[k] = fmin(vmin[3 * a + k], vmin[3 * c + k]);
-                mx[k] = fmax(vmax[3 * a + k], vmax[3 * c + k]);
-            }
-            center_half(mn, mx, ec + 3 * e, eh + 3 * e);
+        double mn[3], mx[3];
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fmin(vmin[3 * a + k], vmin[3 * c + k]);
+            mx[k] = fmax(vmax[3 * a + k], vmax[3 * c + k]);
         }
-    } else if (i < nv + ne + nf) {
-        const long long f = i - nv - ne;
+        center_half(mn, mx, ec + 3 * e, eh + 3 * e);
+    }
+    for (int f = bfs[b] + threadIdx.x; f < bfs[b + 1]; f += BP_THREADS) {
         const int a = F[3 * f], c = F[3 * f + 1], d = F[3 * f + 2];
-        if (inc[v_body[a]]) {
-            double mn[3], mx[3];
-            for (int k = 0; k < 3; k++) {
-                mn[k] = fmin(fmin(vmin[3 * a + k], vmin[3 * c + k]), vmin[3 * d + k]);
-                mx[k] = fmax(fmax(vmax[3 * a + k], vmax[3 * c + k]), vmax[3 * d + k]);
-            }
-            center_half(mn, mx, fc + 3 * f, fh + 3 * f);
+        double mn[3], mx[3];
+        for (int k = 0; k < 3; k++) {
+            mn[k] = fmin(fmin(vmin[3 * a + k], vmin[3 * c + k]), vmin[3 * d + k]);
+            mx[k] = fmax(fmax(vmax[3 * a + k], vmax[3 * c + k]), vmax[3 * d + k]);
         }
+        center_half(mn, mx, fc + 3 * f, fh + 3 * f);
     }
 }
 
@@ -331,19 +335,20 @@ __device__ __forceinline__ double hull_eps(const double* h) {
     return 1e-12 * m;
 }
 
-// Close body pairs (a < b): one 64-lane block per (a tile, b tile) with
-// b tile >= a tile; hits appended with one atomic per wavefront.
-__global__ __launch_bounds__(PAIR_TILE) void body_pair_kernel(int nb, const int* inc, const int* group,
+// Close body pairs (a < b) of one scene: one 64-lane block per
+// upper-triangular tile; hits appended with one atomic per wavefront.
+__global__ __launch_bounds__(PAIR_TILE) void body_pair_kernel(const int4* tiles, const int* inc, const int* group,
                                                              const double* bhull, int* pairs, long long cap,
                                                              unsigned long long* count) {
-    if (blockIdx.y < blockIdx.x) return;
     __shared__ double th[PAIR_TILE][6];
     __shared__ int tg[PAIR_TILE], ti[PAIR_TILE];
-    const int a = blockIdx.x * PAIR_TILE + threadIdx.x;
+    const int4 T = tiles[blockIdx.x];
+    const int end = T.z;
+    const int a = T.x + threadIdx.x;
     const int lane = threadIdx.x;
-    const int b0 = blockIdx.y * PAIR_TILE;
+    const int b0 = T.y;
     const int bb = b0 + threadIdx.x;
-    if (bb < nb) {
+    if (bb < end) {
         ti[threadIdx.x] = inc[bb];
         tg[threadIdx.x] = group[bb];
         for (int k = 0; k < 6; k++) th[threadIdx.x][k] = bhull[6 * bb + k];
@@ -353,13 +358,13 @@ __global__ __launch_bounds__(PAIR_TILE) void body_pair_kernel(int nb, const int*
     int ga = 0;
     bool ia = false;
     double ea = 0;
-    if (a < nb) {
+    if (a < end) {
         ia = inc[a] != 0;
         ga = group[a];
         for (int k = 0; k < 6; k++) ha[k] = bhull[6 * a + k];
         if (ia) ea = hull_eps(ha);
     }
-    const int m = min(PAIR_TILE, nb - b0);
+    const int m = min(PAIR_TILE, end - b0);
     for (int j = 0; j < m; j++) {
         const int b = b0 + j;
         bool hit = false;
@@ -515,15 +520,15 @@ int bit_length(unsigned long long x) {
         if (e_ != hipSuccess) return e_; \
     } while (0)
 
-hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const double* d_p0, const double* d_p1,
-                           double r, bool continuous, hipStream_t st) {
+hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                           const double* d_p1, double r, bool continuous, hipStream_t st) {
     bp.n_ee = bp.n_fv = bp.n_body_pairs = 0;
     const int nb = sc.nb;
-    if (nb <= 1) return hipSuccess;
+    if (nb <= 1 || sel.n_bodies <= 1 || sel.n_tiles == 0) return hipSuccess;
     BP_CHECK(bp.box_raw.ensure(6 * sizeof(double) * nb));
     BP_CHECK(bp.box_inf.ensure(6 * sizeof(double) * nb));
     BP_CHECK(bp.inc.ensure(sizeof(int) * nb));
-    BP_CHECK(bp.dom.ensure(8 * sizeof(double)));
+    BP_CHECK(bp.dom.ensure(8 * sizeof(double) * sc.n_scenes));
     BP_CHECK(bp.bhull.ensure(6 * sizeof(double) * nb));
     BP_CHECK(bp.vmin.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
     BP_CHECK(bp.vmax.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
@@ -537,36 +542,35 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const double* d_p
     unsigned long long* cnt = bp.counters.as<unsigned long long>();
 
     const double* p1 = continuous ? d_p1 : d_p0;
-    hipLaunchKernelGGL(body_box_kernel, dim3(nb), dim3(BP_THREADS), 0, st, sc.V, sc.body_vstart, sc.is_static,
-                       sc.r_max, d_p0, p1, r, bp.box_raw.as<double>(), bp.box_inf.as<double>());
-    const unsigned tiles = (unsigned)((nb + PAIR_TILE - 1) / PAIR_TILE);
+    const unsigned nbs = (unsigned)sel.n_bodies;
+    const unsigned ntiles = (unsigned)sel.n_tiles;
+    hipLaunchKernelGGL(body_box_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.V, sc.body_vstart,
+                       sc.is_static, sc.r_max, d_p0, p1, r, bp.box_raw.as<double>(), bp.box_inf.as<double>());
     BP_CHECK(hipMemsetAsync(bp.inc.p, 0, sizeof(int) * nb, st));
-    hipLaunchKernelGGL(included_kernel, dim3(tiles, tiles), dim3(PAIR_TILE), 0, st, nb, sc.group,
+    hipLaunchKernelGGL(included_kernel, dim3(ntiles), dim3(PAIR_TILE), 0, st, sel.tiles, sc.group,
                        bp.box_inf.as<double>(), bp.inc.as<int>());
-    int* any_dev = reinterpret_cast<int*>(bp.dom.as<double>() + 6);
-    hipLaunchKernelGGL(domain_kernel, dim3(1), dim3(BP_THREADS), 0, st, nb, bp.inc.as<int>(),
-                       bp.box_raw.as<double>(), r, bp.dom.as<double>(), any_dev);
+    hipLaunchKernelGGL(domain_kernel, dim3((unsigned)sel.n_scenes), dim3(BP_THREADS), 0, st, sel.scenes,
+                       sc.scene_body, bp.inc.as<int>(), bp.box_raw.as<double>(), r, bp.dom.as<double>());
     if (continuous)
-        hipLaunchKernelGGL(vertex_interval_kernel, dim3(nb), dim3(BP_THREADS), 0, st, sc.V, sc.body_vstart,
-                           bp.inc.as<int>(), d_p0, d_p1, bp.dom.as<double>(), r, bp.vmin.as<double>(),
-                           bp.vmax.as<double>(), bp.bhull.as<double>());
+        hipLaunchKernelGGL(vertex_interval_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.body_scene,
+                           sc.V, sc.body_vstart, bp.inc.as<int>(), d_p0, d_p1, bp.dom.as<double>(), r,
+                           bp.vmin.as<double>(), bp.vmax.as<double>(), bp.bhull.as<double>());
     else
-        hipLaunchKernelGGL(vertex_static_kernel, dim3(nb), dim3(BP_THREADS), 0, st, sc.V, sc.body_vstart,
-                           bp.inc.as<int>(), d_p0, r, bp.vmin.as<double>(), bp.vmax.as<double>(),
+        hipLaunchKernelGGL(vertex_static_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.V,
+                           sc.body_vstart, bp.inc.as<int>(), d_p0, r, bp.vmin.as<double>(), bp.vmax.as<double>(),
                            bp.bhull.as<double>());
-    const long long nel = (long long)sc.nv + sc.ne + sc.nf;
-    hipLaunchKernelGGL(element_aabb_kernel, dim3((unsigned)((nel + 255) / 256)), dim3(256), 0, st, sc.nv, sc.ne,
-                       sc.nf, sc.E, sc.F, sc.v_body, bp.inc.as<int>(), bp.vmin.as<double>(), bp.vmax.as<double>(),
-                       bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(), bp.eh.as<double>(),
-                       bp.fc.as<double>(), bp.fh.as<double>());
+    hipLaunchKernelGGL(element_aabb_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.body_vstart,
+                       sc.body_estart, sc.body_fstart, sc.E, sc.F, bp.inc.as<int>(), bp.vmin.as<double>(),
+                       bp.vmax.as<double>(), bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(),
+                       bp.eh.as<double>(), bp.fc.as<double>(), bp.fh.as<double>());
     BP_CHECK(hipGetLastError());
 
     // body pairs (retry once with the exact size if the first guess is short)
-    long long cap_pairs = std::max<long long>(1024, 32LL * nb);
+    long long cap_pairs = std::max<long long>(1024, 32LL * sel.n_bodies);
     for (int attempt = 0; attempt < 2; attempt++) {
         BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * cap_pairs));
         BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(body_pair_kernel, dim3(tiles, tiles), dim3(PAIR_TILE), 0, st, nb, bp.inc.as<int>(),
+        hipLaunchKernelGGL(body_pair_kernel, dim3(ntiles), dim3(PAIR_TILE), 0, st, sel.tiles, bp.inc.as<int>(),
                            sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt);
         unsigned long long n = 0;
         BP_CHECK(hipMemcpyAsync(&n, cnt, sizeof(n), hipMemcpyDeviceToHost, st));

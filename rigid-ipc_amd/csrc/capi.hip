@@ -14,6 +14,15 @@
 using namespace rigc;
 
 
+// Selected scenes of a call: the device lists of SceneSel in one buffer,
+// plus the host-side layout of the request order.
+struct SelState {
+    DevBuf buf;
+    SceneSel sel;
+    std::vector<int> req_start;  // [n_sel + 1] first body of each selected scene in request order
+    std::vector<char> host;      // staging of the last upload (kept alive for the asynchronous copy)
+};
+
 struct rigc_ctx {
     int device = 0;
     int num_cus = 256;
@@ -25,15 +34,17 @@ struct rigc_ctx {
     bool has_scene = false;
     DevScene scene;
     std::vector<DevBuf> scene_bufs;
-    double average_edge_length = 0;
+    std::vector<int> h_body_scene;  // [nb]
 
     // per-call buffers
-    DevBuf poses0, poses1, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
+    DevBuf poses0, poses1, stage0, stage1, out_rows, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
+    DevBuf scene_best, scene_skip, scene_par;
+    SelState sel_all, sel_req;
     BroadPhase bp;
     Assembly as;
     CcdWork ccdw;
     int walk_budget = 4096;
-    double last_toi = HUGE_VAL;  // answer of the previous rigc_compute_earliest_toi
+    std::vector<double> last_toi;  // per scene: answer of its previous earliest-TOI call
     hipEvent_t ev2 = nullptr;
 
     rigc_status fail(rigc_status s, const std::string& msg) {
@@ -69,7 +80,161 @@ void free_scene(rigc_ctx* ctx) {
     for (auto& b : ctx->scene_bufs) b.release();
     ctx->scene_bufs.clear();
     ctx->scene = DevScene{};
+    ctx->h_body_scene.clear();
     ctx->has_scene = false;
+}
+
+// Builds and uploads the selection of scenes ids[0..n) (strictly ascending).
+hipError_t make_sel(rigc_ctx* ctx, SelState& S, const int* ids, int n) {
+    const DevScene& d = ctx->scene;
+    const std::vector<int>& sb = d.h_scene_body;
+    std::vector<int> scenes(ids, ids + n), bodies;
+    std::vector<int4> tiles;
+    S.req_start.assign(1, 0);
+    for (int i = 0; i < n; i++) {
+        const int b0 = sb[ids[i]], b1 = sb[ids[i] + 1];
+        for (int b = b0; b < b1; b++) bodies.push_back(b);
+        S.req_start.push_back((int)bodies.size());
+        if (b1 - b0 < 2) continue;
+        const int nt = (b1 - b0 + 63) / 64;
+        for (int ti = 0; ti < nt; ti++)
+            for (int tj = ti; tj < nt; tj++) tiles.push_back(make_int4(b0 + 64 * ti, b0 + 64 * tj, b1, 0));
+    }
+    // layout: tiles (16-byte aligned) | scenes | bodies
+    const size_t tb = sizeof(int4) * tiles.size(), sbytes = sizeof(int) * scenes.size();
+    std::vector<char>& host = S.host;
+    host.assign(tb + sbytes + sizeof(int) * bodies.size() + 16, 0);
+    if (!tiles.empty()) std::memcpy(host.data(), tiles.data(), tb);
+    if (!scenes.empty()) std::memcpy(host.data() + tb, scenes.data(), sbytes);
+    if (!bodies.empty()) std::memcpy(host.data() + tb + sbytes, bodies.data(), sizeof(int) * bodies.size());
+    hipError_t e = S.buf.ensure(host.size());
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(S.buf.p, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return e;
+    char* base = S.buf.as<char>();
+    S.sel.n_scenes = n;
+    S.sel.scenes = reinterpret_cast<const int*>(base + tb);
+    S.sel.n_bodies = (int)bodies.size();
+    S.sel.bodies = reinterpret_cast<const int*>(base + tb + sbytes);
+    S.sel.n_tiles = (long long)tiles.size();
+    S.sel.tiles = reinterpret_cast<const int4*>(base);
+    S.sel.h_scenes = scenes;
+    S.sel.all = n == d.n_scenes;
+    return hipSuccess;
+}
+
+// Uploads scenes[0..n) as one concatenated scene set.
+rigc_status load_scenes(rigc_ctx* ctx, int n, const rigc_scene* scenes, const char* who) {
+    const std::string w(who);
+    if (n < 1) return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": no scenes");
+    int NB = 0, NV = 0, NE = 0, NF = 0;
+    for (int k = 0; k < n; k++) {
+        const rigc_scene* s = scenes + k;
+        if (s->num_bodies < 1 || s->num_vertices < 0 || s->num_edges < 0 || s->num_faces < 0)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": negative sizes");
+        if (!s->vertices || !s->body_vertex_start || !s->body_edge_start || !s->body_face_start || !s->group_ids
+            || !s->is_static || (s->num_edges && !s->edges) || (s->num_faces && (!s->faces || !s->face_edges)))
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": missing arrays");
+        const int nb = s->num_bodies, nv = s->num_vertices, ne = s->num_edges, nf = s->num_faces;
+        if (s->body_vertex_start[0] != 0 || s->body_vertex_start[nb] != nv || s->body_edge_start[0] != 0
+            || s->body_edge_start[nb] != ne || s->body_face_start[0] != 0 || s->body_face_start[nb] != nf)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": body ranges do not cover the arrays");
+        for (int i = 0; i < 2 * ne; i++)
+            if (s->edges[i] < 0 || s->edges[i] >= nv)
+                return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": edge vertex out of range");
+        for (int i = 0; i < 3 * nf; i++)
+            if (s->faces[i] < 0 || s->faces[i] >= nv || s->face_edges[i] < 0 || s->face_edges[i] >= ne)
+                return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": face index out of range");
+        if ((long long)NV + nv > INT32_MAX / 2 || (long long)NE + ne > INT32_MAX / 2 || (long long)NF + nf > INT32_MAX / 2)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": scenes too large");
+        NB += nb; NV += nv; NE += ne; NF += nf;
+    }
+
+    // Concatenation with offset element ids, and the host-side derived data
+    // of RigidBody's constructor (src/physics/rigid_body.cpp:124-142) and
+    // RigidBody::edge_length (src/physics/rigid_body.hpp:151-156).
+    std::vector<double> V, r_max, elen;
+    std::vector<int> E, F, FE, v_body, e_body, f_body, bvs{ 0 }, bes{ 0 }, bfs{ 0 }, group, body_scene, scene_body{ 0 };
+    std::vector<uint8_t> is_static;
+    V.reserve(3 * (size_t)NV);
+    int gmax = 0;
+    for (int k = 0; k < n; k++) {
+        const rigc_scene* s = scenes + k;
+        const int B0 = (int)group.size(), V0 = (int)(V.size() / 3), E0 = (int)(E.size() / 2);
+        const int nb = s->num_bodies, nv = s->num_vertices, ne = s->num_edges, nf = s->num_faces;
+        V.insert(V.end(), s->vertices, s->vertices + 3 * (size_t)nv);
+        for (int i = 0; i < 2 * ne; i++) E.push_back(V0 + s->edges[i]);
+        for (int i = 0; i < 3 * nf; i++) {
+            F.push_back(V0 + s->faces[i]);
+            FE.push_back(E0 + s->face_edges[i]);
+        }
+        // group ids of different scenes never compare equal
+        int g0 = 0;
+        for (int b = 0; b < nb; b++) g0 = std::max(g0, s->group_ids[b] + 1);
+        for (int b = 0; b < nb; b++) {
+            group.push_back(gmax + s->group_ids[b]);
+            is_static.push_back(s->is_static[b]);
+            body_scene.push_back(k);
+            r_max.push_back(0.0);
+            for (int i = s->body_vertex_start[b]; i < s->body_vertex_start[b + 1]; i++) {
+                v_body.push_back(B0 + b);
+                const double* v = s->vertices + 3 * i;
+                r_max.back() = std::max(r_max.back(), std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]));
+            }
+            for (int i = s->body_edge_start[b]; i < s->body_edge_start[b + 1]; i++) {
+                e_body.push_back(B0 + b);
+                const double* a = s->vertices + 3 * s->edges[2 * i];
+                const double* c = s->vertices + 3 * s->edges[2 * i + 1];
+                const double d0 = c[0] - a[0], d1 = c[1] - a[1], d2 = c[2] - a[2];
+                elen.push_back(std::sqrt((d0 * d0 + d1 * d1) + d2 * d2));
+            }
+            for (int i = s->body_face_start[b]; i < s->body_face_start[b + 1]; i++) f_body.push_back(B0 + b);
+            bvs.push_back(V0 + s->body_vertex_start[b + 1]);
+            bes.push_back(E0 + s->body_edge_start[b + 1]);
+            bfs.push_back((int)(F.size() / 3) - nf + s->body_face_start[b + 1]);
+        }
+        gmax += std::max(g0, 1);
+        scene_body.push_back((int)group.size());
+    }
+
+    RIGC_HIP(hipSetDevice(ctx->device));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    free_scene(ctx);
+    DevScene& d = ctx->scene;
+    d.nb = NB; d.nv = NV; d.ne = NE; d.nf = NF;
+    d.n_scenes = n;
+    RIGC_HIP(upload(ctx, &d.V, V.data(), V.size()));
+    RIGC_HIP(upload(ctx, &d.E, E.data(), E.size()));
+    RIGC_HIP(upload(ctx, &d.F, F.data(), F.size()));
+    RIGC_HIP(upload(ctx, &d.FE, FE.data(), FE.size()));
+    RIGC_HIP(upload(ctx, &d.v_body, v_body.data(), v_body.size()));
+    RIGC_HIP(upload(ctx, &d.e_body, e_body.data(), e_body.size()));
+    RIGC_HIP(upload(ctx, &d.f_body, f_body.data(), f_body.size()));
+    RIGC_HIP(upload(ctx, &d.body_vstart, bvs.data(), bvs.size()));
+    RIGC_HIP(upload(ctx, &d.body_estart, bes.data(), bes.size()));
+    RIGC_HIP(upload(ctx, &d.body_fstart, bfs.data(), bfs.size()));
+    RIGC_HIP(upload(ctx, &d.group, group.data(), group.size()));
+    RIGC_HIP(upload(ctx, &d.is_static, is_static.data(), is_static.size()));
+    RIGC_HIP(upload(ctx, &d.edge_len, elen.data(), elen.size()));
+    RIGC_HIP(upload(ctx, &d.r_max, r_max.data(), r_max.size()));
+    RIGC_HIP(upload(ctx, &d.body_scene, body_scene.data(), body_scene.size()));
+    RIGC_HIP(upload(ctx, &d.scene_body, scene_body.data(), scene_body.size()));
+    d.h_scene_body = scene_body;
+    ctx->h_body_scene = body_scene;
+    std::vector<int> all(n);
+    for (int k = 0; k < n; k++) all[k] = k;
+    RIGC_HIP(make_sel(ctx, ctx->sel_all, all.data(), n));
+    RIGC_HIP(ctx->scene_best.ensure(sizeof(unsigned long long) * n));
+    RIGC_HIP(ctx->scene_skip.ensure((size_t)n));
+    RIGC_HIP(ctx->scene_par.ensure(3 * sizeof(double) * n));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->as.have_cons = ctx->as.have_fric = false;
+    ctx->as.n_con = ctx->as.n_fric = 0;
+    ctx->as.fric_off.assign(n + 1, 0);
+    ctx->as.fric_valid.assign(n, 0);
+    ctx->last_toi.assign(n, HUGE_VAL);
+    ctx->has_scene = true;
+    return RIGC_OK;
 }
 
 }  // namespace
@@ -107,8 +272,9 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx);
-    for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->ee, &ctx->fv, &ctx->toi, &ctx->hit, &ctx->counters,
-                       &ctx->qbest, &ctx->qvisits, &ctx->qflag })
+    for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->stage0, &ctx->stage1, &ctx->out_rows, &ctx->ee, &ctx->fv,
+                       &ctx->toi, &ctx->hit, &ctx->counters, &ctx->qbest, &ctx->qvisits, &ctx->qflag,
+                       &ctx->scene_best, &ctx->scene_skip, &ctx->scene_par, &ctx->sel_all.buf, &ctx->sel_req.buf })
         b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -135,73 +301,17 @@ rigc_status rigc_get_stats(const rigc_ctx* ctx, rigc_stats* stats) {
 
 rigc_status rigc_set_scene(rigc_ctx* ctx, const rigc_scene* s) {
     if (!ctx || !s) return RIGC_ERR_INVALID_ARGUMENT;
-    if (s->num_bodies < 1 || s->num_vertices < 0 || s->num_edges < 0 || s->num_faces < 0)
-        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: negative sizes");
-    if (!s->vertices || !s->body_vertex_start || !s->body_edge_start || !s->body_face_start || !s->group_ids
-        || !s->is_static || (s->num_edges && !s->edges) || (s->num_faces && (!s->faces || !s->face_edges)))
-        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: missing arrays");
-    const int nb = s->num_bodies, nv = s->num_vertices, ne = s->num_edges, nf = s->num_faces;
-    if (s->body_vertex_start[0] != 0 || s->body_vertex_start[nb] != nv || s->body_edge_start[0] != 0
-        || s->body_edge_start[nb] != ne || s->body_face_start[0] != 0 || s->body_face_start[nb] != nf)
-        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: body ranges do not cover the arrays");
-    for (int i = 0; i < 2 * ne; i++)
-        if (s->edges[i] < 0 || s->edges[i] >= nv)
-            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: edge vertex out of range");
-    for (int i = 0; i < 3 * nf; i++)
-        if (s->faces[i] < 0 || s->faces[i] >= nv || s->face_edges[i] < 0 || s->face_edges[i] >= ne)
-            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_scene: face index out of range");
+    return load_scenes(ctx, 1, s, "rigc_set_scene");
+}
 
-    // Host-side derived data, following RigidBody's constructor
-    // (src/physics/rigid_body.cpp:124-142) and RigidBody::edge_length
-    // (src/physics/rigid_body.hpp:151-156).
-    std::vector<int> v_body(nv), e_body(ne), f_body(nf);
-    std::vector<double> r_max(nb, 0.0), elen(ne);
-    double avg_sum = 0;
-    for (int b = 0; b < nb; b++) {
-        for (int i = s->body_vertex_start[b]; i < s->body_vertex_start[b + 1]; i++) {
-            v_body[i] = b;
-            const double* v = s->vertices + 3 * i;
-            r_max[b] = std::max(r_max[b], std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]));
-        }
-        double body_avg = 0;
-        const int e0 = s->body_edge_start[b], e1 = s->body_edge_start[b + 1];
-        for (int i = e0; i < e1; i++) {
-            e_body[i] = b;
-            const double* a = s->vertices + 3 * s->edges[2 * i];
-            const double* c = s->vertices + 3 * s->edges[2 * i + 1];
-            const double d0 = c[0] - a[0], d1 = c[1] - a[1], d2 = c[2] - a[2];
-            elen[i] = std::sqrt((d0 * d0 + d1 * d1) + d2 * d2);
-            body_avg += elen[i];
-        }
-        if (e1 > e0) body_avg /= (e1 - e0);
-        avg_sum += (e1 - e0) * body_avg;
-        for (int i = s->body_face_start[b]; i < s->body_face_start[b + 1]; i++) f_body[i] = b;
-    }
-    ctx->average_edge_length = ne ? avg_sum / ne : 0.0;
+rigc_status rigc_set_scenes(rigc_ctx* ctx, int32_t num_scenes, const rigc_scene* scenes) {
+    if (!ctx || !scenes) return RIGC_ERR_INVALID_ARGUMENT;
+    return load_scenes(ctx, num_scenes, scenes, "rigc_set_scenes");
+}
 
-    RIGC_HIP(hipSetDevice(ctx->device));
-    free_scene(ctx);
-    DevScene& d = ctx->scene;
-    d.nb = nb; d.nv = nv; d.ne = ne; d.nf = nf;
-    RIGC_HIP(upload(ctx, &d.V, s->vertices, 3 * (size_t)nv));
-    RIGC_HIP(upload(ctx, &d.E, s->edges, 2 * (size_t)ne));
-    RIGC_HIP(upload(ctx, &d.F, s->faces, 3 * (size_t)nf));
-    RIGC_HIP(upload(ctx, &d.FE, s->face_edges, 3 * (size_t)nf));
-    RIGC_HIP(upload(ctx, &d.v_body, v_body.data(), (size_t)nv));
-    RIGC_HIP(upload(ctx, &d.e_body, e_body.data(), (size_t)ne));
-    RIGC_HIP(upload(ctx, &d.f_body, f_body.data(), (size_t)nf));
-    RIGC_HIP(upload(ctx, &d.body_vstart, s->body_vertex_start, (size_t)nb + 1));
-    RIGC_HIP(upload(ctx, &d.body_estart, s->body_edge_start, (size_t)nb + 1));
-    RIGC_HIP(upload(ctx, &d.body_fstart, s->body_face_start, (size_t)nb + 1));
-    RIGC_HIP(upload(ctx, &d.group, s->group_ids, (size_t)nb));
-    RIGC_HIP(upload(ctx, &d.is_static, s->is_static, (size_t)nb));
-    RIGC_HIP(upload(ctx, &d.edge_len, elen.data(), (size_t)ne));
-    RIGC_HIP(upload(ctx, &d.r_max, r_max.data(), (size_t)nb));
-    RIGC_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->as.have_cons = ctx->as.have_fric = false;
-    ctx->as.n_con = ctx->as.n_fric = 0;
-    ctx->last_toi = HUGE_VAL;
-    ctx->has_scene = true;
+rigc_status rigc_get_scene_count(const rigc_ctx* ctx, int32_t* num_scenes) {
+    if (!ctx || !num_scenes) return RIGC_ERR_INVALID_ARGUMENT;
+    *num_scenes = ctx->has_scene ? ctx->scene.n_scenes : 0;
     return RIGC_OK;
 }
 
@@ -240,8 +350,11 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     RIGC_HIP(hipMemcpyAsync(ctx->poses1.p, poses_t1, pose_bytes, hipMemcpyHostToDevice, st));
     if (n_ee) RIGC_HIP(hipMemcpyAsync(ctx->ee.p, ee, 2 * sizeof(int32_t) * n_ee, hipMemcpyHostToDevice, st));
     if (n_fv) RIGC_HIP(hipMemcpyAsync(ctx->fv.p, fv, 2 * sizeof(int32_t) * n_fv, hipMemcpyHostToDevice, st));
-    unsigned long long init[4] = { 0x7ff0000000000000ULL, 0, 0, 0 };
+    const unsigned long long init[4] = { 0, 0, 0, 0 };
     RIGC_HIP(hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+    const std::vector<unsigned long long> inf_bits(d.n_scenes, 0x7ff0000000000000ULL);
+    RIGC_HIP(hipMemcpyAsync(ctx->scene_best.p, inf_bits.data(), sizeof(unsigned long long) * d.n_scenes,
+                            hipMemcpyHostToDevice, st));
 
     CcdArgs a{};
     a.V = d.V; a.E = d.E; a.F = d.F; a.FE = d.FE;
@@ -255,7 +368,9 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     a.toi_tol = toi_tolerance;
     a.eval_budget = eval_budget;
     a.use_global_best = per_query ? 0 : 1;
-    a.global_best = ctx->counters.as<unsigned long long>();
+    a.global_best = ctx->scene_best.as<unsigned long long>();
+    a.body_scene = d.body_scene;
+    a.scene_skip = nullptr;
     a.counter = ctx->counters.as<unsigned long long>() + 1;
     a.total_evals = ctx->counters.as<unsigned long long>() + 2;
     a.toi_out = ctx->toi.as<double>();
@@ -278,7 +393,10 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
         RIGC_HIP(hipMemcpyAsync(hits.data(), ctx->hit.p, nq, hipMemcpyDeviceToHost, st));
     }
     unsigned long long res[4];
+    std::vector<unsigned long long> sbest(d.n_scenes);
     RIGC_HIP(hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, st));
+    RIGC_HIP(hipMemcpyAsync(sbest.data(), ctx->scene_best.p, sizeof(unsigned long long) * d.n_scenes,
+                            hipMemcpyDeviceToHost, st));
     RIGC_HIP(hipStreamSynchronize(st));
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -297,8 +415,12 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     if (per_query) {
         *earliest_toi = best;
     } else {
-        double g;
-        std::memcpy(&g, &res[0], sizeof(double));
+        double g = HUGE_VAL;
+        for (int k = 0; k < d.n_scenes; k++) {
+            double gk;
+            std::memcpy(&gk, &sbest[k], sizeof(double));
+            g = std::min(g, gk);
+        }
         *earliest_toi = g;
     }
     if (toi_out) std::memcpy(toi_out, tois.data(), sizeof(double) * nq);
@@ -310,12 +432,46 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
 
 namespace {
 
+// Poses of every body of the context (scene order).
 rigc_status upload_poses(rigc_ctx* ctx, const double* p0, const double* p1) {
     const size_t pose_bytes = 6 * sizeof(double) * (size_t)ctx->scene.nb;
     RIGC_HIP(ctx->poses0.ensure(pose_bytes));
     RIGC_HIP(ctx->poses1.ensure(pose_bytes));
     RIGC_HIP(hipMemcpyAsync(ctx->poses0.p, p0, pose_bytes, hipMemcpyHostToDevice, ctx->stream));
     if (p1) RIGC_HIP(hipMemcpyAsync(ctx->poses1.p, p1, pose_bytes, hipMemcpyHostToDevice, ctx->stream));
+    return RIGC_OK;
+}
+
+// Poses of the selected scenes' bodies, in request order, into the
+// scene-order pose arrays (the other rows keep their values).
+rigc_status upload_sel_poses(rigc_ctx* ctx, const SelState& S, const double* p0, const double* p1) {
+    const size_t nb = (size_t)ctx->scene.nb, ns = (size_t)S.sel.n_bodies;
+    RIGC_HIP(ctx->poses0.ensure(6 * sizeof(double) * nb));
+    RIGC_HIP(ctx->poses1.ensure(6 * sizeof(double) * nb));
+    if (S.sel.all) return upload_poses(ctx, p0, p1);
+    RIGC_HIP(ctx->stage0.ensure(6 * sizeof(double) * std::max<size_t>(ns, 1)));
+    RIGC_HIP(ctx->stage1.ensure(6 * sizeof(double) * std::max<size_t>(ns, 1)));
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(hipMemcpyAsync(ctx->stage0.p, p0, 6 * sizeof(double) * ns, hipMemcpyHostToDevice, st));
+    scatter_rows(S.sel.bodies, (int)ns, 6, ctx->stage0.as<double>(), ctx->poses0.as<double>(), st);
+    if (p1) {
+        RIGC_HIP(hipMemcpyAsync(ctx->stage1.p, p1, 6 * sizeof(double) * ns, hipMemcpyHostToDevice, st));
+        scatter_rows(S.sel.bodies, (int)ns, 6, ctx->stage1.as<double>(), ctx->poses1.as<double>(), st);
+    }
+    RIGC_HIP(hipGetLastError());
+    return RIGC_OK;
+}
+
+// Checks and uploads a request's scene list.
+rigc_status select_scenes(rigc_ctx* ctx, int32_t n, const int32_t* ids, const char* who) {
+    const std::string w(who);
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, w + ": no scene");
+    if (n < 1 || !ids) return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": empty scene list");
+    for (int i = 0; i < n; i++)
+        if (ids[i] < 0 || ids[i] >= ctx->scene.n_scenes || (i && ids[i] <= ids[i - 1]))
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": scene ids must be ascending and in range");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    RIGC_HIP(make_sel(ctx, ctx->sel_req, ids, n));
     return RIGC_OK;
 }
 
@@ -326,10 +482,218 @@ CcdArgs ccd_args(rigc_ctx* ctx) {
     a.v_body = d.v_body; a.e_body = d.e_body; a.f_body = d.f_body; a.edge_len = d.edge_len;
     a.poses_t0 = ctx->poses0.as<double>();
     a.poses_t1 = ctx->poses1.as<double>();
-    a.global_best = ctx->counters.as<unsigned long long>();
+    a.global_best = ctx->scene_best.as<unsigned long long>();
+    a.body_scene = d.body_scene;
     a.counter = ctx->counters.as<unsigned long long>() + 1;
     a.total_evals = ctx->counters.as<unsigned long long>() + 2;
     return a;
+}
+
+// Earliest TOI of each selected scene (poses already uploaded): broad phase
+// over the selection, then the narrow phase with a running minimum per
+// scene. Windowed search: a first pass starts each scene's minimum at a
+// bound T_s (four times that scene's previous answer; successive
+// line-search calls return similar times). It prunes every box with
+// t.lower >= T_s, so if it accepts a box of scene s the minimum below T_s
+// is s's answer; the scenes where it accepts none are searched again in
+// full, with the queries of every other scene skipped. Either way each
+// result is the exact minimum.
+rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::vector<double>& out) {
+    const SceneSel& sel = S.sel;
+    const int ns = ctx->scene.n_scenes;
+    hipStream_t s = ctx->stream;
+    RIGC_HIP(hipEventRecord(ctx->ev0, s));
+    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, sel, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
+                             msd / 2.0, true, s));
+    RIGC_HIP(hipEventRecord(ctx->ev1, s));
+    const long long nq = ctx->bp.n_ee + ctx->bp.n_fv;
+    RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
+    RIGC_HIP(ctx->qbest.ensure(sizeof(unsigned long long) * std::max<long long>(nq, 1)));
+    RIGC_HIP(ctx->qvisits.ensure(sizeof(unsigned int) * std::max<long long>(nq, 1)));
+    RIGC_HIP(ctx->qflag.ensure(sizeof(int) * std::max<long long>(nq, 1)));
+    CcdArgs a = ccd_args(ctx);
+    a.ee = ctx->bp.out_ee.as<int>();
+    a.fv = ctx->bp.out_fv.as<int>();
+    a.n_ee = ctx->bp.n_ee;
+    a.n_fv = ctx->bp.n_fv;
+    a.toi_tol = 1e-4;
+    a.eval_budget = 0;
+    a.use_global_best = 1;
+    a.toi_out = nullptr;
+    a.hit_out = nullptr;
+    a.q_best = ctx->qbest.as<unsigned long long>();
+    a.q_visits = ctx->qvisits.as<unsigned int>();
+    a.q_flag = ctx->qflag.as<int>();
+    const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
+    a.walk_budget = ctx->walk_budget;
+
+    const double inf = HUGE_VAL;
+    std::vector<double> window(sel.n_scenes, inf);
+    bool any_window = false;
+    for (int i = 0; i < sel.n_scenes; i++) {
+        const double lt = ctx->last_toi[sel.h_scenes[i]];
+        if (lt < 0.25) {
+            window[i] = std::max(4.0 * lt, std::ldexp(1.0, -20));
+            any_window = true;
+        }
+    }
+    out.assign(sel.n_scenes, inf);
+    long long evals = 0, launches = 0;
+    double kernel_ms = 0.0;
+    std::vector<unsigned long long> best_bits(ns, 0x7ff0000000000000ULL), res_bits(ns);
+    std::vector<unsigned char> skip(ns, 0);
+    // bound[i]: the starting minimum of selected scene i (NaN: skip it)
+    auto run = [&](const std::vector<double>& bound) -> hipError_t {
+        bool any_skip = false;
+        for (int i = 0; i < sel.n_scenes; i++) {
+            const int k = sel.h_scenes[i];
+            skip[k] = std::isnan(bound[i]) ? 1 : 0;
+            any_skip = any_skip || skip[k];
+            const double b = skip[k] ? 0.0 : bound[i];
+            std::memcpy(&best_bits[k], &b, sizeof(double));
+        }
+        const unsigned long long init[4] = { 0, 0, 0, 0 };
+        hipError_t e = hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ctx->scene_best.p, best_bits.data(), sizeof(unsigned long long) * ns,
+                               hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && any_skip)
+            e = hipMemcpyAsync(ctx->scene_skip.p, skip.data(), (size_t)ns, hipMemcpyHostToDevice, s);
+        a.scene_skip = any_skip ? ctx->scene_skip.as<unsigned char>() : nullptr;
+        if (e == hipSuccess && nq) e = launch_ccd(a, grid, 256, s, ctx->ccdw);
+        unsigned long long res[4] = { 0, 0, 0, 0 };
+        if (e == hipSuccess) e = hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(res_bits.data(), ctx->scene_best.p, sizeof(unsigned long long) * ns,
+                               hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        evals += (long long)res[2];
+        launches += ctx->ccdw.passes;
+        kernel_ms += ctx->ccdw.kernel_ms;
+        return e;
+    };
+    auto result = [&](int i) {
+        double g;
+        std::memcpy(&g, &res_bits[sel.h_scenes[i]], sizeof(double));
+        return g;
+    };
+    if (nq) {
+        RIGC_HIP(run(window));
+        std::vector<double> again(sel.n_scenes, std::nan(""));
+        bool second = false;
+        for (int i = 0; i < sel.n_scenes; i++) {
+            const double g = result(i);
+            if (window[i] < inf && !(g < window[i])) {
+                again[i] = inf;
+                second = true;
+            } else {
+                out[i] = g;
+            }
+        }
+        if (second) {
+            RIGC_HIP(run(again));
+            for (int i = 0; i < sel.n_scenes; i++)
+                if (!std::isnan(again[i])) out[i] = result(i);
+        }
+        (void)any_window;
+    }
+    RIGC_HIP(hipEventRecord(ctx->ev2, s));
+    RIGC_HIP(hipStreamSynchronize(s));
+    float bms = 0, nms = 0;
+    RIGC_HIP(hipEventElapsedTime(&bms, ctx->ev0, ctx->ev1));
+    RIGC_HIP(hipEventElapsedTime(&nms, ctx->ev1, ctx->ev2));
+    ctx->stats.broad_phase_ms = bms;
+    ctx->stats.narrow_phase_ms = nms;
+    ctx->stats.box_evaluations = evals;
+    ctx->stats.num_candidates = nq;
+    ctx->stats.ccd_kernel_ms = kernel_ms;
+    ctx->stats.ccd_launches = launches;
+    for (int i = 0; i < sel.n_scenes; i++) ctx->last_toi[sel.h_scenes[i]] = out[i];
+    return RIGC_OK;
+}
+
+rigc_status copy_potential(rigc_ctx* ctx, const SelState& S, const AssemblyOut& out, int32_t flags, double* grad,
+                           rigc_block_hessian* hess) {
+    const DevScene& d = ctx->scene;
+    hipStream_t st = ctx->stream;
+    const bool want_hess = (flags & RIGC_HESSIAN) != 0;
+    const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
+    const int nsb = S.sel.n_bodies;
+    if (!S.sel.all && (want_grad || want_hess)) {
+        RIGC_HIP(ctx->out_rows.ensure(36 * sizeof(double) * std::max(nsb, 1)));
+    }
+    if (want_grad && grad) {
+        if (S.sel.all) {
+            RIGC_HIP(hipMemcpyAsync(grad, ctx->as.grad.p, 6 * sizeof(double) * d.nb, hipMemcpyDeviceToHost, st));
+        } else {
+            gather_rows(S.sel.bodies, nsb, 6, ctx->as.grad.as<double>(), ctx->out_rows.as<double>(), st);
+            RIGC_HIP(hipMemcpyAsync(grad, ctx->out_rows.p, 6 * sizeof(double) * nsb, hipMemcpyDeviceToHost, st));
+            RIGC_HIP(hipStreamSynchronize(st));
+        }
+    }
+    bool fits = true;
+    if (want_hess && hess) {
+        if (S.sel.all) {
+            RIGC_HIP(hipMemcpyAsync(hess->diag, ctx->as.diag.p, 36 * sizeof(double) * d.nb, hipMemcpyDeviceToHost,
+                                    st));
+        } else {
+            gather_rows(S.sel.bodies, nsb, 36, ctx->as.diag.as<double>(), ctx->out_rows.as<double>(), st);
+            RIGC_HIP(hipMemcpyAsync(hess->diag, ctx->out_rows.p, 36 * sizeof(double) * nsb, hipMemcpyDeviceToHost,
+                                    st));
+        }
+        hess->count = out.n_off;
+        fits = out.n_off <= hess->capacity;
+        if (fits && out.n_off) {
+            RIGC_HIP(hipMemcpyAsync(hess->pairs, ctx->as.opairs.p, 2 * sizeof(int32_t) * out.n_off,
+                                    hipMemcpyDeviceToHost, st));
+            RIGC_HIP(hipMemcpyAsync(hess->off, ctx->as.oblocks.p, 36 * sizeof(double) * out.n_off,
+                                    hipMemcpyDeviceToHost, st));
+        }
+    }
+    RIGC_HIP(hipStreamSynchronize(st));
+    if (fits && want_hess && hess && !S.sel.all && out.n_off) {
+        // scene-order body ids -> request order
+        std::vector<int> first(d.n_scenes, -1);
+        for (int i = 0; i < S.sel.n_scenes; i++) first[S.sel.h_scenes[i]] = S.req_start[i];
+        for (long long k = 0; k < 2 * out.n_off; k++) {
+            const int b = hess->pairs[k];
+            const int sidx = ctx->h_body_scene[b];
+            hess->pairs[k] = first[sidx] + (b - d.h_scene_body[sidx]);
+        }
+    }
+    if (!fits) return ctx->fail(RIGC_ERR_CAPACITY, "off-diagonal Hessian capacity too small");
+    return RIGC_OK;
+}
+
+bool bad_hess(int32_t flags, double* grad, rigc_block_hessian* hess) {
+    const bool want_hess = (flags & RIGC_HESSIAN) != 0;
+    const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
+    if (want_grad && !grad) return true;
+    if (want_hess && (!hess || !hess->diag || hess->capacity < 0 || (hess->capacity && (!hess->pairs || !hess->off))))
+        return true;
+    return false;
+}
+
+double total(const std::vector<double>& v) {
+    double t = 0.0;
+    for (double x : v) t += x;
+    return t;
+}
+
+rigc_status build_sel(rigc_ctx* ctx, const SelState& S, double dhat, double dmin) {
+    RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, S.sel, ctx->poses0.as<double>(), dhat, dmin,
+                                  ctx->num_cus, ctx->stream));
+    return RIGC_OK;
+}
+
+// Uploads per-scene parameters (rows 0..2 of scene_par: kappa, mu, epsv_h).
+rigc_status upload_scene_par(rigc_ctx* ctx, int row, const SelState& S, const double* vals) {
+    std::vector<double> h(ctx->scene.n_scenes, 0.0);
+    for (int i = 0; i < S.sel.n_scenes; i++) h[S.sel.h_scenes[i]] = vals[i];
+    RIGC_HIP(hipMemcpyAsync(ctx->scene_par.as<double>() + (size_t)row * ctx->scene.n_scenes, h.data(),
+                            sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    return RIGC_OK;
 }
 
 }  // namespace
@@ -346,8 +710,8 @@ rigc_status rigc_broad_phase(rigc_ctx* ctx, const double* poses_t0, const double
     rigc_status st = upload_poses(ctx, poses_t0, poses_t1);
     if (st != RIGC_OK) return st;
     RIGC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
-                             inflation_radius, poses_t1 != nullptr, ctx->stream));
+    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
+                             ctx->poses1.as<double>(), inflation_radius, poses_t1 != nullptr, ctx->stream));
     RIGC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     *n_ee = ctx->bp.n_ee;
     *n_fv = ctx->bp.n_fv;
@@ -378,116 +742,14 @@ rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, con
     RIGC_HIP(hipSetDevice(ctx->device));
     rigc_status st = upload_poses(ctx, poses_t0, poses_t1);
     if (st != RIGC_OK) return st;
-    hipStream_t s = ctx->stream;
-    RIGC_HIP(hipEventRecord(ctx->ev0, s));
-    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
-                             minimum_separation_distance / 2.0, true, s));
-    RIGC_HIP(hipEventRecord(ctx->ev1, s));
-    const long long nq = ctx->bp.n_ee + ctx->bp.n_fv;
-    RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
-    RIGC_HIP(ctx->qbest.ensure(sizeof(unsigned long long) * std::max<long long>(nq, 1)));
-    RIGC_HIP(ctx->qvisits.ensure(sizeof(unsigned int) * std::max<long long>(nq, 1)));
-    RIGC_HIP(ctx->qflag.ensure(sizeof(int) * std::max<long long>(nq, 1)));
-    CcdArgs a = ccd_args(ctx);
-    a.ee = ctx->bp.out_ee.as<int>();
-    a.fv = ctx->bp.out_fv.as<int>();
-    a.n_ee = ctx->bp.n_ee;
-    a.n_fv = ctx->bp.n_fv;
-    a.toi_tol = 1e-4;
-    a.eval_budget = 0;
-    a.use_global_best = 1;
-    a.toi_out = nullptr;
-    a.hit_out = nullptr;
-    a.q_best = ctx->qbest.as<unsigned long long>();
-    a.q_visits = ctx->qvisits.as<unsigned int>();
-    a.q_flag = ctx->qflag.as<int>();
-    const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
-    a.walk_budget = ctx->walk_budget;
-    // Windowed search: a first pass starts the running minimum at a bound T
-    // (four times the previous call's answer; successive line-search calls
-    // return similar times). It prunes every box with t.lower >= T, so if
-    // it accepts a box the minimum below T is the answer; otherwise the
-    // full search runs. Either way the result is the exact minimum.
-    const double inf = HUGE_VAL;
-    double window = inf;
-    if (ctx->last_toi < 0.25) window = std::max(4.0 * ctx->last_toi, std::ldexp(1.0, -20));
-    double g = inf;
-    long long evals = 0, launches = 0;
-    double kernel_ms = 0.0;
-    auto run = [&](double bound) -> hipError_t {
-        unsigned long long init[4] = { 0, 0, 0, 0 };
-        std::memcpy(&init[0], &bound, sizeof(double));
-        hipError_t e = hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess && nq) e = launch_ccd(a, grid, 256, s, ctx->ccdw);
-        unsigned long long res[4] = { 0, 0, 0, 0 };
-        if (e == hipSuccess) e = hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        std::memcpy(&g, &res[0], sizeof(double));
-        evals += (long long)res[2];
-        launches += ctx->ccdw.passes;
-        kernel_ms += ctx->ccdw.kernel_ms;
-        return e;
-    };
-    if (window < inf) {
-        RIGC_HIP(run(window));
-        if (!(g < window)) RIGC_HIP(run(inf));
-    } else {
-        RIGC_HIP(run(inf));
-    }
-    RIGC_HIP(hipEventRecord(ctx->ev2, s));
-    RIGC_HIP(hipStreamSynchronize(s));
-    float bms = 0, nms = 0;
-    RIGC_HIP(hipEventElapsedTime(&bms, ctx->ev0, ctx->ev1));
-    RIGC_HIP(hipEventElapsedTime(&nms, ctx->ev1, ctx->ev2));
-    ctx->stats.broad_phase_ms = bms;
-    ctx->stats.narrow_phase_ms = nms;
-    ctx->stats.box_evaluations = evals;
-    ctx->stats.num_candidates = nq;
-    ctx->stats.ccd_kernel_ms = kernel_ms;
-    ctx->stats.ccd_launches = launches;
-    ctx->last_toi = g;
+    std::vector<double> out;
+    st = earliest_toi_sel(ctx, ctx->sel_all, minimum_separation_distance, out);
+    if (st != RIGC_OK) return st;
+    double g = HUGE_VAL;
+    for (double x : out) g = std::min(g, x);
     *earliest_toi = g;  // +inf when no candidate collides
     return RIGC_OK;
 }
-
-namespace {
-
-rigc_status copy_potential(rigc_ctx* ctx, const AssemblyOut& out, int32_t flags, double* energy, double* grad,
-                           rigc_block_hessian* hess) {
-    const DevScene& d = ctx->scene;
-    hipStream_t st = ctx->stream;
-    *energy = out.energy;
-    const bool want_hess = (flags & RIGC_HESSIAN) != 0;
-    const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
-    if (want_grad && grad)
-        RIGC_HIP(hipMemcpyAsync(grad, ctx->as.grad.p, 6 * sizeof(double) * d.nb, hipMemcpyDeviceToHost, st));
-    bool fits = true;
-    if (want_hess && hess) {
-        RIGC_HIP(hipMemcpyAsync(hess->diag, ctx->as.diag.p, 36 * sizeof(double) * d.nb, hipMemcpyDeviceToHost, st));
-        hess->count = out.n_off;
-        fits = out.n_off <= hess->capacity;
-        if (fits && out.n_off) {
-            RIGC_HIP(hipMemcpyAsync(hess->pairs, ctx->as.opairs.p, 2 * sizeof(int32_t) * out.n_off,
-                                    hipMemcpyDeviceToHost, st));
-            RIGC_HIP(hipMemcpyAsync(hess->off, ctx->as.oblocks.p, 36 * sizeof(double) * out.n_off,
-                                    hipMemcpyDeviceToHost, st));
-        }
-    }
-    RIGC_HIP(hipStreamSynchronize(st));
-    if (!fits) return ctx->fail(RIGC_ERR_CAPACITY, "off-diagonal Hessian capacity too small");
-    return RIGC_OK;
-}
-
-bool bad_hess(int32_t flags, double* grad, rigc_block_hessian* hess) {
-    const bool want_hess = (flags & RIGC_HESSIAN) != 0;
-    const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
-    if (want_grad && !grad) return true;
-    if (want_hess && (!hess || !hess->diag || hess->capacity < 0 || (hess->capacity && (!hess->pairs || !hess->off))))
-        return true;
-    return false;
-}
-
-}  // namespace
 
 rigc_status rigc_construct_constraint_set(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
                                           int64_t* counts) {
@@ -499,8 +761,8 @@ rigc_status rigc_construct_constraint_set(rigc_ctx* ctx, const double* poses, do
     rigc_status s = upload_poses(ctx, poses, nullptr);
     if (s != RIGC_OK) return s;
     RIGC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, ctx->num_cus,
-                                  ctx->stream));
+    s = build_sel(ctx, ctx->sel_all, dhat, dmin);
+    if (s != RIGC_OK) return s;
     RIGC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     RIGC_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0;
@@ -541,16 +803,19 @@ rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double
     if (s != RIGC_OK) return s;
     hipStream_t st = ctx->stream;
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
-    if (flags & RIGC_BUILD_CONSTRAINTS)
-        RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin,
-                                      ctx->num_cus, st));
+    if (flags & RIGC_BUILD_CONSTRAINTS) {
+        s = build_sel(ctx, ctx->sel_all, dhat, dmin);
+        if (s != RIGC_OK) return s;
+    }
     if (!ctx->as.have_cons)
         return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_barrier_term: no constraint set (build one first)");
     AssemblyOut out;
-    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, false, ctx->poses0.as<double>(), nullptr, dhat, dmin, 0.0,
-                                (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
+    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, ctx->sel_all.sel, false, ctx->poses0.as<double>(), nullptr,
+                                dhat, dmin, 0.0, nullptr, (flags & RIGC_GRADIENT) != 0,
+                                (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
-    s = copy_potential(ctx, out, flags, energy, grad, hess);
+    *energy = total(out.energy);
+    s = copy_potential(ctx, ctx->sel_all, out, flags, grad, hess);
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;
@@ -566,14 +831,18 @@ rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, do
     RIGC_HIP(hipSetDevice(ctx->device));
     rigc_status s = upload_poses(ctx, poses, nullptr);
     if (s != RIGC_OK) return s;
-    if (flags & RIGC_BUILD_CONSTRAINTS)
-        RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin,
-                                      ctx->num_cus, ctx->stream));
+    if (flags & RIGC_BUILD_CONSTRAINTS) {
+        s = build_sel(ctx, ctx->sel_all, dhat, dmin);
+        if (s != RIGC_OK) return s;
+    }
     if (!ctx->as.have_cons)
         return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_minimum_distance: no constraint set (build one first)");
-    double m2 = HUGE_VAL;
-    RIGC_HIP(rigc::minimum_distance(ctx->as, ctx->scene, ctx->poses0.as<double>(), m2, ctx->stream));
-    *min_distance = std::sqrt(m2);
+    std::vector<double> m2;
+    RIGC_HIP(rigc::minimum_distance(ctx->as, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(), m2,
+                                    ctx->stream));
+    double m = HUGE_VAL;
+    for (double x : m2) m = std::min(m, x);
+    *min_distance = std::sqrt(m);
     return RIGC_OK;
 }
 
@@ -586,14 +855,19 @@ rigc_status rigc_update_friction_constraints(rigc_ctx* ctx, const double* poses,
     RIGC_HIP(hipSetDevice(ctx->device));
     rigc_status s = upload_poses(ctx, poses, nullptr);
     if (s != RIGC_OK) return s;
-    if (flags & RIGC_BUILD_CONSTRAINTS)
-        RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, ctx->poses0.as<double>(), dhat, dmin,
-                                      ctx->num_cus, ctx->stream));
+    if (flags & RIGC_BUILD_CONSTRAINTS) {
+        s = build_sel(ctx, ctx->sel_all, dhat, dmin);
+        if (s != RIGC_OK) return s;
+    }
     if (!ctx->as.have_cons)
         return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_update_friction_constraints: no constraint set (build one first)");
-    RIGC_HIP(build_friction_set(ctx->as, ctx->scene, ctx->poses0.as<double>(), dhat, dmin, barrier_stiffness, mu,
-                                ctx->stream));
-    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    const int ns = ctx->scene.n_scenes;
+    const std::vector<double> kap(ns, barrier_stiffness), mus(ns, mu);
+    s = upload_scene_par(ctx, 0, ctx->sel_all, kap.data());
+    if (s == RIGC_OK) s = upload_scene_par(ctx, 1, ctx->sel_all, mus.data());
+    if (s != RIGC_OK) return s;
+    RIGC_HIP(build_friction_set(ctx->as, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(), dhat, dmin,
+                                ctx->scene_par.as<double>(), ctx->scene_par.as<double>() + ns, ctx->stream));
     if (count) *count = ctx->as.n_fric;
     return RIGC_OK;
 }
@@ -634,11 +908,144 @@ rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const
     hipStream_t st = ctx->stream;
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
     AssemblyOut out;
-    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, true, ctx->poses0.as<double>(), ctx->poses1.as<double>(), 0.0,
-                                0.0, epsv_times_h, (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0,
+    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, ctx->sel_all.sel, true, ctx->poses0.as<double>(),
+                                ctx->poses1.as<double>(), 0.0, 0.0, epsv_times_h, nullptr,
+                                (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
+    RIGC_HIP(hipEventRecord(ctx->ev1, st));
+    *energy = total(out.energy);
+    s = copy_potential(ctx, ctx->sel_all, out, flags, grad, hess);
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.assembly_ms = ms;
+    return s;
+}
+
+// ------------------------------------------------------------ batched scenes
+rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* poses_t0,
+                                    const double* poses_t1, double minimum_separation_distance, double* toi) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!poses_t0 || !poses_t1 || !toi || !(minimum_separation_distance >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_earliest_toi: bad arguments");
+    rigc_status s = select_scenes(ctx, n, scene_ids, "rigc_batch_earliest_toi");
+    if (s != RIGC_OK) return s;
+    s = upload_sel_poses(ctx, ctx->sel_req, poses_t0, poses_t1);
+    if (s != RIGC_OK) return s;
+    std::vector<double> out;
+    s = earliest_toi_sel(ctx, ctx->sel_req, minimum_separation_distance, out);
+    if (s != RIGC_OK) return s;
+    std::memcpy(toi, out.data(), sizeof(double) * n);
+    return RIGC_OK;
+}
+
+rigc_status rigc_batch_barrier_term(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* build_poses,
+                                    const double* poses, double dhat, double dmin, int32_t flags, double* energy,
+                                    double* grad, rigc_block_hessian* hess, int64_t* counts) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!build_poses || !poses || !energy || !(dhat > 0) || !(dmin >= 0) || bad_hess(flags, grad, hess))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_barrier_term: bad arguments");
+    rigc_status s = select_scenes(ctx, n, scene_ids, "rigc_batch_barrier_term");
+    if (s != RIGC_OK) return s;
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(hipEventRecord(ctx->ev0, st));
+    s = upload_sel_poses(ctx, ctx->sel_req, build_poses, nullptr);
+    if (s == RIGC_OK) s = build_sel(ctx, ctx->sel_req, dhat, dmin);
+    if (s != RIGC_OK) return s;
+    if (counts) {
+        std::vector<long long> c;
+        RIGC_HIP(constraint_counts(ctx->as, ctx->scene, ctx->sel_req.sel, c, st));
+        for (size_t k = 0; k < c.size(); k++) counts[k] = c[k];
+    }
+    if (poses != build_poses) {
+        s = upload_sel_poses(ctx, ctx->sel_req, poses, nullptr);
+        if (s != RIGC_OK) return s;
+    }
+    AssemblyOut out;
+    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, ctx->sel_req.sel, false, ctx->poses0.as<double>(), nullptr, dhat,
+                                dmin, 0.0, nullptr, (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0,
                                 ctx->num_cus, out, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
-    s = copy_potential(ctx, out, flags, energy, grad, hess);
+    std::memcpy(energy, out.energy.data(), sizeof(double) * n);
+    s = copy_potential(ctx, ctx->sel_req, out, flags, grad, hess);
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.assembly_ms = ms;
+    return s;
+}
+
+rigc_status rigc_batch_minimum_distance(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids,
+                                        const double* build_poses, const double* poses, double dhat, double dmin,
+                                        double* min_distance) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!build_poses || !poses || !min_distance || !(dhat > 0) || !(dmin >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_minimum_distance: bad arguments");
+    rigc_status s = select_scenes(ctx, n, scene_ids, "rigc_batch_minimum_distance");
+    if (s != RIGC_OK) return s;
+    s = upload_sel_poses(ctx, ctx->sel_req, build_poses, nullptr);
+    if (s == RIGC_OK) s = build_sel(ctx, ctx->sel_req, dhat, dmin);
+    if (s == RIGC_OK && poses != build_poses) s = upload_sel_poses(ctx, ctx->sel_req, poses, nullptr);
+    if (s != RIGC_OK) return s;
+    std::vector<double> m2;
+    RIGC_HIP(rigc::minimum_distance(ctx->as, ctx->scene, ctx->sel_req.sel, ctx->poses0.as<double>(), m2,
+                                    ctx->stream));
+    for (int i = 0; i < n; i++) min_distance[i] = std::sqrt(m2[i]);
+    return RIGC_OK;
+}
+
+rigc_status rigc_batch_update_friction_constraints(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids,
+                                                   const double* poses, double dhat, double dmin,
+                                                   const double* barrier_stiffness, const double* mu,
+                                                   int64_t* counts) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!poses || !barrier_stiffness || !mu || !(dhat > 0) || !(dmin >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_update_friction_constraints: bad arguments");
+    for (int i = 0; i < n; i++)
+        if (!(barrier_stiffness[i] >= 0) || !(mu[i] >= 0))
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_update_friction_constraints: bad kappa or mu");
+    rigc_status s = select_scenes(ctx, n, scene_ids, "rigc_batch_update_friction_constraints");
+    if (s != RIGC_OK) return s;
+    s = upload_sel_poses(ctx, ctx->sel_req, poses, nullptr);
+    if (s == RIGC_OK) s = build_sel(ctx, ctx->sel_req, dhat, dmin);
+    if (s == RIGC_OK) s = upload_scene_par(ctx, 0, ctx->sel_req, barrier_stiffness);
+    if (s == RIGC_OK) s = upload_scene_par(ctx, 1, ctx->sel_req, mu);
+    if (s != RIGC_OK) return s;
+    const int ns = ctx->scene.n_scenes;
+    RIGC_HIP(build_friction_set(ctx->as, ctx->scene, ctx->sel_req.sel, ctx->poses0.as<double>(), dhat, dmin,
+                                ctx->scene_par.as<double>(), ctx->scene_par.as<double>() + ns, ctx->stream));
+    if (counts)
+        for (int i = 0; i < n; i++) {
+            const int k = scene_ids[i];
+            counts[i] = ctx->as.fric_off[k + 1] - ctx->as.fric_off[k];
+        }
+    return RIGC_OK;
+}
+
+rigc_status rigc_batch_friction_term(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* poses,
+                                     const double* poses_t0, const double* epsv_times_h, int32_t flags,
+                                     double* energy, double* grad, rigc_block_hessian* hess) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!poses || !poses_t0 || !epsv_times_h || !energy || bad_hess(flags, grad, hess))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_friction_term: bad arguments");
+    for (int i = 0; i < n; i++)
+        if (!(epsv_times_h[i] > 0))
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_friction_term: bad epsv_times_h");
+    rigc_status s = select_scenes(ctx, n, scene_ids, "rigc_batch_friction_term");
+    if (s != RIGC_OK) return s;
+    for (int i = 0; i < n; i++)
+        if (!ctx->as.fric_valid[scene_ids[i]])
+            return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_batch_friction_term: a scene has no friction set");
+    s = upload_sel_poses(ctx, ctx->sel_req, poses, poses_t0);
+    if (s == RIGC_OK) s = upload_scene_par(ctx, 2, ctx->sel_req, epsv_times_h);
+    if (s != RIGC_OK) return s;
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(hipEventRecord(ctx->ev0, st));
+    AssemblyOut out;
+    RIGC_HIP(assemble_potential(ctx->as, ctx->scene, ctx->sel_req.sel, true, ctx->poses0.as<double>(),
+                                ctx->poses1.as<double>(), 0.0, 0.0, 0.0,
+                                ctx->scene_par.as<double>() + 2 * (size_t)ctx->scene.n_scenes,
+                                (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
+    RIGC_HIP(hipEventRecord(ctx->ev1, st));
+    std::memcpy(energy, out.energy.data(), sizeof(double) * n);
+    s = copy_potential(ctx, ctx->sel_req, out, flags, grad, hess);
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;

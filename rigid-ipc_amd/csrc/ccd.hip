@@ -84,7 +84,7 @@ RIGC_DI int last_zero_below(const Bits192& b, int depth) {
 RIGC_DI double dyad(unsigned long long i, int l) { return ldexp((double)i, -l); }
 
 struct Query {
-    int kind, bA, bB;
+    int kind, bA, bB, sc;
     int g[4];  // EE: ea0 ea1 eb0 eb1; FV: v f0 f1 f2 (global vertex ids)
 };
 
@@ -177,6 +177,7 @@ RIGC_DI void load_query(const CcdArgs& a, long long qi, Query& q, double tol[3])
         tol[1] = 1e-4 / a.edge_len[a.FE[3 * f]];
         tol[2] = 1e-4 / a.edge_len[a.FE[3 * f + 1]];
     }
+    q.sc = a.body_scene[q.bA];
 }
 
 __global__ void ccd_init_kernel(CcdArgs a) {
@@ -267,6 +268,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                             load_query(a, qi, q, tol);
                             tol[0] = a.toi_tol;
                             phase = 1;
+                            if (a.scene_skip && a.scene_skip[q.sc]) qi = -1;
                         }
                     }
                 }
@@ -351,7 +353,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                                                                         __HIP_MEMORY_SCOPE_AGENT));
                 if (a.use_global_best) {
                     const unsigned long long gb =
-                        __hip_atomic_load(a.global_best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_load(a.global_best + q.sc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     thr = fmin(thr, __longlong_as_double((long long)gb));
                 }
                 thr_q = qi;
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     const unsigned long long bits = (unsigned long long)__double_as_longlong(t.lo);
                     atomicMin(a.q_best + qi, bits);
                     thr = fmin(thr, t.lo);
-                    if (a.use_global_best) atomicMin(a.global_best, bits);
+                    if (a.use_global_best) atomicMin(a.global_best + q.sc, bits);
                     leaf = true;
                 } else {
                     int sp = -1;

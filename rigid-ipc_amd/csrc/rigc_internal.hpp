@@ -28,6 +28,30 @@ struct DevScene {
     uint8_t* is_static = nullptr;
     double* edge_len = nullptr; // [ne] body-local edge length
     double* r_max = nullptr;    // [nb]
+    // A context may hold several independent scenes concatenated (bodies,
+    // vertices, edges and faces of scene s follow those of scene s - 1):
+    // no pair of bodies of different scenes is ever a candidate, and every
+    // reduction (domain, earliest TOI, energies, minimum distance) is per
+    // scene.
+    int n_scenes = 1;
+    int* body_scene = nullptr;      // [nb]
+    int* scene_body = nullptr;      // [n_scenes + 1] body ranges
+    std::vector<int> h_scene_body;  // host copy of scene_body
+};
+
+// Scenes taking part in one call: their ids (ascending), their bodies, and
+// the 64 x 64 body tiles of each scene's upper-triangular body-pair grid
+// (x = first body of the row tile, y = first body of the column tile,
+// z = end of the scene's body range; y >= x).
+struct SceneSel {
+    int n_scenes = 0;
+    const int* scenes = nullptr;
+    int n_bodies = 0;
+    const int* bodies = nullptr;
+    long long n_tiles = 0;
+    const int4* tiles = nullptr;
+    std::vector<int> h_scenes;
+    bool all = false;  // every scene of the context is selected
 };
 
 // Grow-only device allocation.
@@ -61,10 +85,11 @@ struct BroadPhase {
     long long n_ee = 0, n_fv = 0, n_body_pairs = 0;
 };
 
-// Candidate detection (continuous when p1 != p0 is meant: `continuous`),
-// leaving sorted int32 pairs in bp.out_ee / bp.out_fv.
-hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const double* d_p0, const double* d_p1,
-                           double inflation_radius, bool continuous, hipStream_t stream);
+// Candidate detection over the selected scenes (continuous when p1 != p0
+// is meant: `continuous`), leaving sorted int32 pairs in bp.out_ee /
+// bp.out_fv.
+hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                           const double* d_p1, double inflation_radius, bool continuous, hipStream_t stream);
 
 hipError_t radix_sort_u64(unsigned long long* keys, unsigned long long* tmp, unsigned int* hist, long long n,
                           int bits, hipStream_t stream);
@@ -99,8 +124,10 @@ struct CcdArgs {
     long long n_ee, n_fv;
     double toi_tol;
     long long eval_budget;   // 0 = unbounded
-    int use_global_best;     // prune with the running minimum over all queries
-    unsigned long long* global_best;  // bits of a non-negative double
+    int use_global_best;     // prune with the running minimum over the query's scene
+    unsigned long long* global_best;  // [n_scenes] bits of non-negative doubles
+    const int* body_scene;   // [nb]
+    const unsigned char* scene_skip;  // optional [n_scenes]: queries of flagged scenes are skipped
     unsigned long long* counter;      // work counter (query index)
     unsigned long long* total_evals;
     double* toi_out;         // per query (optional)
@@ -168,16 +195,27 @@ struct FricRec {
     double mu;
 };
 
+// Record ranges of up to four classes: class k is [off[k], off[k + 1]).
+struct ClassRanges {
+    long long off[5];
+    int n_class;
+};
+
 // Assembly working set (device buffers reused across calls).
 struct Assembly {
     DevBuf R, frames, W, W0;            // body rotations / frames, world vertices
     DevBuf ckeys[4], cstart[2], ccount, ctmp, chist, cflags, cpos, cwork;
-    DevBuf cons;                        // ConRec [n_con]
+    DevBuf cons;                        // ConRec [n_con], classes VV, EV, EE, FV in turn
     long long n_con = 0, counts[4] = { 0, 0, 0, 0 };
     bool have_cons = false;
-    DevBuf fric;                        // FricRec [n_fric]
+    // Lagged friction sets of all scenes, scene after scene: scene s owns
+    // [fric_off[s], fric_off[s + 1]) when fric_valid[s].
+    DevBuf fric, fric_tmp, fric_new, fric_stage, ranges, entries;
     long long n_fric = 0;
-    bool have_fric = false;
+    bool have_fric = false;             // every scene has a friction set
+    std::vector<long long> fric_off;
+    std::vector<char> fric_valid;
+    std::vector<long long> h_entries;   // staging of the last range-copy table
     DevBuf energy, lgrad, lhess, esum;  // per-constraint outputs and the energy sum
     DevBuf dkeys, okeys, ktmp, khist, oflags, opos, owork;
     DevBuf diag, grad, opairs, oblocks, mind;
@@ -185,30 +223,45 @@ struct Assembly {
 
     void release() {
         for (DevBuf* b : { &R, &frames, &W, &W0, &ckeys[0], &ckeys[1], &ckeys[2], &ckeys[3], &cstart[0], &cstart[1],
-                           &ccount, &ctmp, &chist, &cflags, &cpos, &cwork, &cons, &fric, &energy, &lgrad, &lhess,
-                           &esum, &dkeys, &okeys, &ktmp, &khist, &oflags, &opos, &owork, &diag, &grad, &opairs,
-                           &oblocks, &mind })
+                           &ccount, &ctmp, &chist, &cflags, &cpos, &cwork, &cons, &fric, &fric_tmp, &fric_new,
+                           &fric_stage, &ranges, &entries, &energy, &lgrad, &lhess, &esum, &dkeys, &okeys, &ktmp,
+                           &khist, &oflags, &opos, &owork, &diag, &grad, &opairs, &oblocks, &mind })
             b->release();
     }
 };
 
 struct AssemblyOut {
-    double energy = 0;
-    long long n_off = 0;  // off-diagonal blocks produced
+    std::vector<double> energy;  // per selected scene
+    long long n_off = 0;         // off-diagonal blocks produced
 };
 
-// Constraint set at poses (device), kept in as.cons.
-hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc, const double* d_poses,
-                                double dhat, double dmin, int num_cus, hipStream_t stream);
-// Barrier (friction == false) or friction potential over the current set.
-// Outputs stay on the device: as.grad [nb][6], as.diag [nb][36],
-// as.opairs / as.oblocks [n_off].
-hipError_t assemble_potential(Assembly& as, const DevScene& sc, bool friction, const double* d_poses,
-                              const double* d_poses_t0, double dhat, double dmin, double epsv_h, bool want_grad,
-                              bool want_hess, int num_cus, AssemblyOut& out, hipStream_t stream);
-hipError_t build_friction_set(Assembly& as, const DevScene& sc, const double* d_poses, double dhat, double dmin,
-                              double kappa, double mu, hipStream_t stream);
-hipError_t minimum_distance(Assembly& as, const DevScene& sc, const double* d_poses, double& min_sq,
-                            hipStream_t stream);
+// Constraint set of the selected scenes at poses, kept in as.cons.
+hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc, const SceneSel& sel,
+                                const double* d_poses, double dhat, double dmin, int num_cus, hipStream_t stream);
+// Constraints per selected scene and class ([n_sel][4]) of the current set.
+hipError_t constraint_counts(Assembly& as, const DevScene& sc, const SceneSel& sel, std::vector<long long>& counts,
+                             hipStream_t stream);
+// Barrier (friction == false, over the current constraint set) or friction
+// potential (over the selected scenes' friction sets). Outputs stay on the
+// device: as.grad [nb][6], as.diag [nb][36] (rows of the selected bodies),
+// as.opairs / as.oblocks [n_off]; energies per selected scene.
+hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& sel, bool friction,
+                              const double* d_poses, const double* d_poses_t0, double dhat, double dmin,
+                              double epsv_h, const double* d_epsv_h_scene, bool want_grad, bool want_hess,
+                              int num_cus, AssemblyOut& out, hipStream_t stream);
+// Friction sets of the selected scenes from the current constraint set
+// (which must be the selected scenes' set at d_poses); kappa / mu per scene
+// (device arrays indexed by scene id).
+hipError_t build_friction_set(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                              double dhat, double dmin, const double* d_kappa_scene, const double* d_mu_scene,
+                              hipStream_t stream);
+// Minimum squared distance over the current set, per selected scene.
+hipError_t minimum_distance(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                            std::vector<double>& min_sq, hipStream_t stream);
+
+// Row gathers / scatters between request order and scene order:
+// dst[i][:] = src[rows[i]][:] (gather) or dst[rows[i]][:] = src[i][:].
+void gather_rows(const int* rows, int n, int width, const double* src, double* dst, hipStream_t stream);
+void scatter_rows(const int* rows, int n, int width, const double* src, double* dst, hipStream_t stream);
 
 }  // namespace rigc

@@ -95,6 +95,20 @@ def lib():
     L.rigc_update_friction_constraints.argtypes = [vp, _c_double_p, d, d, d, d, i32, _c_int64_p]
     L.rigc_get_friction_constraints.argtypes = [vp, i64, _c_double_p, _c_int64_p]
     L.rigc_compute_friction_term.argtypes = [vp, _c_double_p, _c_double_p, d, i32, _c_double_p, _c_double_p, bh]
+    L.rigc_set_scenes.argtypes = [vp, i32, ctypes.POINTER(Scene)]
+    L.rigc_get_scene_count.argtypes = [vp, _c_int32_p]
+    L.rigc_batch_earliest_toi.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, _c_double_p]
+    L.rigc_batch_barrier_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, i32, _c_double_p,
+                                          _c_double_p, bh, _c_int64_p]
+    L.rigc_batch_minimum_distance.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, _c_double_p]
+    L.rigc_batch_update_friction_constraints.argtypes = [vp, i32, _c_int32_p, _c_double_p, d, d, _c_double_p,
+                                                         _c_double_p, _c_int64_p]
+    L.rigc_batch_friction_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, _c_double_p, i32,
+                                           _c_double_p, _c_double_p, bh]
+    for name in ("rigc_set_scenes", "rigc_get_scene_count", "rigc_batch_earliest_toi", "rigc_batch_barrier_term",
+                 "rigc_batch_minimum_distance", "rigc_batch_update_friction_constraints",
+                 "rigc_batch_friction_term"):
+        getattr(L, name).restype = ctypes.c_int32
     for name in ("rigc_create", "rigc_destroy", "rigc_get_stats", "rigc_set_scene", "rigc_narrow_phase",
                  "rigc_broad_phase", "rigc_compute_earliest_toi", "rigc_construct_constraint_set",
                  "rigc_get_constraint_set", "rigc_compute_barrier_term", "rigc_compute_minimum_distance",
@@ -140,10 +154,8 @@ class Context:
         self._check(self._L.rigc_get_stats(self._ctx, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in Stats._fields_}
 
-    def set_scene(self, scene):
-        """scene: an object with numpy fields vertices (nv,3) f64, edges (ne,2),
-        faces (nf,3), face_edges (nf,3), body_vertex_start, body_edge_start,
-        body_face_start (nb+1), group_ids (nb), is_static (nb)."""
+    @staticmethod
+    def _scene_struct(scene, keep):
         arrs = dict(
             vertices=np.ascontiguousarray(scene.vertices, np.float64),
             edges=np.ascontiguousarray(scene.edges, np.int32),
@@ -163,8 +175,129 @@ class Context:
         for k, a in arrs.items():
             t = {np.float64: _c_double_p, np.int32: _c_int32_p, np.uint8: _c_uint8_p}[a.dtype.type]
             setattr(s, k, _p(a, t))
+        keep.append(arrs)
+        return s
+
+    def set_scene(self, scene):
+        """scene: an object with numpy fields vertices (nv,3) f64, edges (ne,2),
+        faces (nf,3), face_edges (nf,3), body_vertex_start, body_edge_start,
+        body_face_start (nb+1), group_ids (nb), is_static (nb)."""
+        keep = []
+        s = self._scene_struct(scene, keep)
         self._check(self._L.rigc_set_scene(self._ctx, ctypes.byref(s)))
         self.num_bodies = s.num_bodies
+        self.scene_bodies = [s.num_bodies]
+
+    def set_scenes(self, scenes):
+        """Several independent scenes in one context (rigc_set_scenes)."""
+        keep = []
+        arr = (Scene * len(scenes))(*[self._scene_struct(sc, keep) for sc in scenes])
+        self._check(self._L.rigc_set_scenes(self._ctx, len(scenes), arr))
+        self.scene_bodies = [int(len(sc.group_ids)) for sc in scenes]
+        self.num_bodies = sum(self.scene_bodies)
+
+    # ---------------------------------------------------------- batched scenes
+    def _ids_poses(self, ids, *pose_lists):
+        ids = np.ascontiguousarray(ids, np.int32)
+        out = [ids]
+        for pl in pose_lists:
+            out.append(np.ascontiguousarray(np.concatenate([np.asarray(p, np.float64).reshape(-1, 6) for p in pl])))
+        return out
+
+    def batch_earliest_toi(self, ids, poses_t0, poses_t1, minimum_separation_distance=0.0):
+        ids, p0, p1 = self._ids_poses(ids, poses_t0, poses_t1)
+        toi = np.zeros(len(ids))
+        self._check(self._L.rigc_batch_earliest_toi(self._ctx, len(ids), _p(ids, _c_int32_p), _p(p0, _c_double_p),
+                                                    _p(p1, _c_double_p), float(minimum_separation_distance),
+                                                    _p(toi, _c_double_p)))
+        return toi
+
+    def _batch_potential(self, fn, ids, args, flags):
+        n = len(ids)
+        nbs = [self.scene_bodies[i] for i in ids]
+        starts = np.concatenate([[0], np.cumsum(nbs)]).astype(int)
+        nb = int(starts[-1])
+        energy = np.zeros(n)
+        grad = np.zeros(6 * nb) if flags & (RIGC_GRADIENT | RIGC_HESSIAN) else None
+        h = None
+        diag = pairs = off = None
+        if flags & RIGC_HESSIAN:
+            cap = getattr(self, "_cap_off", 1024)
+            for _ in range(2):
+                diag = np.zeros((nb, 36))
+                pairs = np.zeros((max(cap, 1), 2), np.int32)
+                off = np.zeros((max(cap, 1), 36))
+                h = BlockHessian(_p(diag, _c_double_p), _p(pairs, _c_int32_p), _p(off, _c_double_p), cap, 0)
+                st = fn(self._ctx, n, _p(ids, _c_int32_p), *args, int(flags), _p(energy, _c_double_p),
+                        _p(grad, _c_double_p), ctypes.byref(h))
+                if st == RIGC_ERR_CAPACITY:
+                    cap = self._cap_off = int(h.count)
+                    continue
+                self._check(st)
+                break
+            else:
+                raise RigcError(RIGC_ERR_CAPACITY, "Hessian capacity retry failed")
+        else:
+            self._check(fn(self._ctx, n, _p(ids, _c_int32_p), *args, int(flags), _p(energy, _c_double_p),
+                           _p(grad, _c_double_p), None))
+        out = []
+        for i in range(n):
+            a, b = starts[i], starts[i + 1]
+            g = grad[6 * a:6 * b].copy() if grad is not None else None
+            hh = None
+            if h is not None:
+                P = pairs[:h.count]
+                m = (P[:, 0] >= a) & (P[:, 0] < b)
+                hh = (diag[a:b].reshape(-1, 6, 6).copy(), (P[m] - a).astype(np.int32),
+                      off[:h.count][m].reshape(-1, 6, 6).copy())
+            out.append((float(energy[i]), g, hh))
+        return out
+
+    def batch_barrier_term(self, ids, build_poses, poses, dhat, dmin=0.0, grad=False, hess=False):
+        """Per-scene (energy, grad, hess) with each scene's constraint set
+        built at build_poses[i] and evaluated at poses[i]."""
+        ids, pb, p = self._ids_poses(ids, build_poses, poses)
+        flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0)
+        self._keep = [ids, pb, p]
+        fn = lambda ctx, n, i, a0, a1, a2, a3, f, e, g, hh: self._L.rigc_batch_barrier_term(  # noqa: E731
+            ctx, n, i, a0, a1, a2, a3, f, e, g, hh, None)
+        return self._batch_potential(fn, ids, (_p(pb, _c_double_p), _p(p, _c_double_p), float(dhat), float(dmin)),
+                                     flags)
+
+    def batch_constraint_counts(self, ids, poses, dhat, dmin=0.0):
+        ids, p = self._ids_poses(ids, poses)
+        e = np.zeros(len(ids))
+        counts = np.zeros((len(ids), 4), np.int64)
+        self._check(self._L.rigc_batch_barrier_term(self._ctx, len(ids), _p(ids, _c_int32_p), _p(p, _c_double_p),
+                                                    _p(p, _c_double_p), float(dhat), float(dmin), 0,
+                                                    _p(e, _c_double_p), None, None, _p(counts, _c_int64_p)))
+        return counts
+
+    def batch_minimum_distance(self, ids, build_poses, poses, dhat, dmin=0.0):
+        ids, pb, p = self._ids_poses(ids, build_poses, poses)
+        out = np.zeros(len(ids))
+        self._check(self._L.rigc_batch_minimum_distance(self._ctx, len(ids), _p(ids, _c_int32_p), _p(pb, _c_double_p),
+                                                        _p(p, _c_double_p), float(dhat), float(dmin),
+                                                        _p(out, _c_double_p)))
+        return out
+
+    def batch_update_friction_constraints(self, ids, poses, dhat, dmin, barrier_stiffness, mu):
+        ids, p = self._ids_poses(ids, poses)
+        kap = np.ascontiguousarray(barrier_stiffness, np.float64)
+        mus = np.ascontiguousarray(mu, np.float64)
+        counts = np.zeros(len(ids), np.int64)
+        self._check(self._L.rigc_batch_update_friction_constraints(
+            self._ctx, len(ids), _p(ids, _c_int32_p), _p(p, _c_double_p), float(dhat), float(dmin),
+            _p(kap, _c_double_p), _p(mus, _c_double_p), _p(counts, _c_int64_p)))
+        return counts
+
+    def batch_friction_term(self, ids, poses, poses_t0, epsv_times_h, grad=False, hess=False):
+        ids, p, p0 = self._ids_poses(ids, poses, poses_t0)
+        eh = np.ascontiguousarray(epsv_times_h, np.float64)
+        flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0)
+        self._keep = [ids, p, p0, eh]
+        return self._batch_potential(self._L.rigc_batch_friction_term, ids,
+                                     (_p(p, _c_double_p), _p(p0, _c_double_p), _p(eh, _c_double_p)), flags)
 
     def narrow_phase(self, poses_t0, poses_t1, ee, fv, toi_tolerance=1e-4, eval_budget=0,
                      per_query=False, allow_unresolved=False):
