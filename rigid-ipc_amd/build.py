@@ -43,14 +43,15 @@ SIM_OUT = os.path.join(HERE, "librigc_sim.so")
 
 
 def build_host(verbose=False):
-    """The host time stepper (host/simulation.cpp) as librigc_sim.so,
+    """The host time stepper (host/simulation.cpp, host/batch.cpp) as librigc_sim.so,
     linked against librigc.so (found next to it at run time)."""
     tmp = SIM_OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-shared", os.path.join(HOST, "simulation.cpp"), "-L" + HERE, "-lrigc",
-                             "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    cmd = [HIPCC] + FLAGS + ["-shared", "-pthread", os.path.join(HOST, "simulation.cpp"),
+                             os.path.join(HOST, "batch.cpp"), "-L" + HERE, "-lrigc", "-Wl,-rpath,$ORIGIN",
+                             "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for host/simulation.cpp:\n{r.stderr}")
+        raise RuntimeError(f"hipcc failed for the host stepper:\n{r.stderr}")
     if verbose and r.stderr.strip():
         print(r.stderr)
     os.replace(tmp, SIM_OUT)

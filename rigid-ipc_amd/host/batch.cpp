@@ -1,0 +1,482 @@
+// Batched time stepping of many independent scenes (rigs.h, rigs_batch_*).
+//
+// Fused mode: every scene runs the host stepper (simulation.cpp, unchanged)
+// on a thread of its own, over a proxy contact backend. A proxy call parks
+// its thread with a request; once every scene still stepping has parked, the
+// last one to arrive executes all pending requests as fused device calls on
+// one multi-scene context (one rigc_batch_* call per kind of request and
+// argument set: earliest TOI, barrier, minimum distance, friction update,
+// friction potential), hands each scene its part of the results and wakes
+// the others. Each scene's Newton loop therefore proceeds independently
+// (different iteration counts, line-search lengths and step ends), while
+// the device sees one launch sequence per round for all scenes that are
+// waiting on it. Per-scene results are bit-identical to a single-scene
+// context, so every scene follows exactly the trajectory it follows alone.
+//
+// Sequential mode: the plain path, one scene after another, each with its
+// own single-scene context (the comparison point for the fused mode).
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "rigs.h"
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+enum ReqKind { REQ_TOI = 0, REQ_BARRIER, REQ_MIND, REQ_FUPDATE, REQ_FRICTION };
+
+struct Request {
+    int kind = 0;
+    int scene = 0;
+    // inputs (owned by the parked caller)
+    const double* p0 = nullptr;
+    const double* p1 = nullptr;
+    double a = 0, b = 0, c = 0;  // TOI: msd; BARRIER / MIND: dhat, dmin; FRICTION: epsv_h; FUPDATE: dhat, dmin, kappa
+    double d = 0;                // FUPDATE: mu
+    int32_t flags = 0;
+    // outputs
+    double* value = nullptr;     // toi / energy / min distance
+    double* grad = nullptr;
+    rigc_block_hessian* hess = nullptr;
+    int64_t* count = nullptr;
+    rigc_status status = RIGC_OK;
+    std::string error;
+    bool done = false;
+};
+
+struct Batch;
+
+struct Proxy {
+    Batch* B = nullptr;
+    int scene = 0;
+    int nb = 0;
+    std::vector<double> build_poses;  // poses of the scene's last construct_constraint_set
+    int64_t counts[4] = { 0, 0, 0, 0 };
+    std::string err;
+};
+
+rigc_status submit(Proxy* P, Request& r);
+
+// ------------------------------------------------------------ proxy backend
+rigc_status px_set_scene(void*, const rigc_scene*) { return RIGC_OK; }
+
+rigc_status px_toi(void* s, const double* p0, const double* p1, double msd, double* toi) {
+    Proxy* P = static_cast<Proxy*>(s);
+    Request r;
+    r.kind = REQ_TOI;
+    r.p0 = p0;
+    r.p1 = p1;
+    r.a = msd;
+    r.value = toi;
+    return submit(P, r);
+}
+
+// The set is built lazily, by the call that uses it: remember the poses.
+rigc_status px_build(void* s, const double* poses, double, double, int64_t* counts) {
+    Proxy* P = static_cast<Proxy*>(s);
+    P->build_poses.assign(poses, poses + 6 * (size_t)P->nb);
+    if (counts)
+        for (int k = 0; k < 4; k++) counts[k] = P->counts[k];  // of the previous barrier call
+    return RIGC_OK;
+}
+
+rigc_status px_barrier(void* s, const double* poses, double dhat, double dmin, int32_t flags, double* e, double* g,
+                       rigc_block_hessian* h) {
+    Proxy* P = static_cast<Proxy*>(s);
+    if (flags & RIGC_BUILD_CONSTRAINTS) P->build_poses.assign(poses, poses + 6 * (size_t)P->nb);
+    if (P->build_poses.empty()) {
+        P->err = "no constraint set (build one first)";
+        return RIGC_ERR_NO_SCENE;
+    }
+    Request r;
+    r.kind = REQ_BARRIER;
+    r.p0 = P->build_poses.data();
+    r.p1 = poses;
+    r.a = dhat;
+    r.b = dmin;
+    r.flags = flags & (RIGC_GRADIENT | RIGC_HESSIAN);
+    r.value = e;
+    r.grad = g;
+    r.hess = h;
+    return submit(P, r);
+}
+
+rigc_status px_mind(void* s, const double* poses, double dhat, double dmin, int32_t flags, double* out) {
+    Proxy* P = static_cast<Proxy*>(s);
+    if (flags & RIGC_BUILD_CONSTRAINTS) P->build_poses.assign(poses, poses + 6 * (size_t)P->nb);
+    if (P->build_poses.empty()) {
+        P->err = "no constraint set (build one first)";
+        return RIGC_ERR_NO_SCENE;
+    }
+    Request r;
+    r.kind = REQ_MIND;
+    r.p0 = P->build_poses.data();
+    r.p1 = poses;
+    r.a = dhat;
+    r.b = dmin;
+    r.value = out;
+    return submit(P, r);
+}
+
+rigc_status px_fupdate(void* s, const double* poses, double dhat, double dmin, double kappa, double mu, int32_t flags,
+                       int64_t* n) {
+    Proxy* P = static_cast<Proxy*>(s);
+    if ((flags & RIGC_BUILD_CONSTRAINTS) || P->build_poses.empty())
+        P->build_poses.assign(poses, poses + 6 * (size_t)P->nb);
+    Request r;
+    r.kind = REQ_FUPDATE;
+    r.p0 = P->build_poses.data();
+    r.a = dhat;
+    r.b = dmin;
+    r.c = kappa;
+    r.d = mu;
+    r.count = n;
+    return submit(P, r);
+}
+
+rigc_status px_friction(void* s, const double* poses, const double* poses_t0, double eh, int32_t flags, double* e,
+                        double* g, rigc_block_hessian* h) {
+    Proxy* P = static_cast<Proxy*>(s);
+    Request r;
+    r.kind = REQ_FRICTION;
+    r.p0 = poses;
+    r.p1 = poses_t0;
+    r.c = eh;
+    r.flags = flags & (RIGC_GRADIENT | RIGC_HESSIAN);
+    r.value = e;
+    r.grad = g;
+    r.hess = h;
+    return submit(P, r);
+}
+
+const char* px_error(void* s) { return static_cast<Proxy*>(s)->err.c_str(); }
+
+// ------------------------------------------------------------ combiner
+struct Batch {
+    rigc_ctx* ctx = nullptr;
+    int n = 0;
+    std::vector<int> nb;
+    std::vector<Proxy> proxies;
+    std::mutex m;
+    std::condition_variable cv;
+    std::vector<Request*> pending;
+    int active = 0;
+    rigs_batch_stats st{};
+    // fused-call staging
+    std::vector<int32_t> ids;
+    std::vector<double> c0, c1, vals, kap, mus, grad, diag, off;
+    std::vector<int32_t> pairs;
+    std::vector<int64_t> cnt;
+    int64_t cap = 0;
+
+    void execute();
+    void run_group(int kind, std::vector<Request*>& g);
+};
+
+rigc_status submit(Proxy* P, Request& r) {
+    Batch& B = *P->B;
+    r.scene = P->scene;
+    std::unique_lock<std::mutex> lk(B.m);
+    B.pending.push_back(&r);
+    if ((int)B.pending.size() == B.active) {
+        B.execute();
+        B.cv.notify_all();
+    } else {
+        B.cv.wait(lk, [&] { return r.done; });
+    }
+    if (r.status != RIGC_OK) P->err = r.error;
+    return r.status;
+}
+
+// A scene finished stepping (or failed): it no longer takes part in rounds.
+void leave(Batch& B) {
+    std::unique_lock<std::mutex> lk(B.m);
+    B.active--;
+    if (B.active > 0 && (int)B.pending.size() == B.active) {
+        B.execute();
+        B.cv.notify_all();
+    }
+}
+
+void Batch::execute() {
+    const double t0 = now_ms();
+    std::vector<Request*> reqs;
+    reqs.swap(pending);
+    // group by kind and by the scalar arguments every scene of a fused call shares
+    std::map<std::tuple<int, int32_t, double, double>, std::vector<Request*>> groups;
+    for (Request* r : reqs) {
+        const double k1 = r->kind == REQ_TOI ? r->a : (r->kind == REQ_FRICTION ? 0.0 : r->a);
+        const double k2 = (r->kind == REQ_BARRIER || r->kind == REQ_MIND || r->kind == REQ_FUPDATE) ? r->b : 0.0;
+        groups[std::make_tuple(r->kind, r->flags, k1, k2)].push_back(r);
+    }
+    for (auto& kv : groups) {
+        std::vector<Request*>& g = kv.second;
+        std::sort(g.begin(), g.end(), [](const Request* x, const Request* y) { return x->scene < y->scene; });
+        run_group(std::get<0>(kv.first), g);
+        st.fused_calls++;
+        st.requests += (int64_t)g.size();
+    }
+    for (Request* r : reqs) r->done = true;
+    st.rounds++;
+    st.device_ms += now_ms() - t0;
+}
+
+void fail_all(std::vector<Request*>& g, rigc_status s, const char* msg) {
+    for (Request* r : g) {
+        r->status = s;
+        r->error = msg;
+    }
+}
+
+void Batch::run_group(int kind, std::vector<Request*>& g) {
+    const int k = (int)g.size();
+    ids.resize(k);
+    std::vector<int> start(k + 1, 0);
+    for (int i = 0; i < k; i++) {
+        ids[i] = g[i]->scene;
+        start[i + 1] = start[i] + nb[g[i]->scene];
+    }
+    const int tot = start[k];
+    auto concat = [&](std::vector<double>& dst, bool second) {
+        dst.resize(6 * (size_t)tot);
+        for (int i = 0; i < k; i++) {
+            const double* src = second ? g[i]->p1 : g[i]->p0;
+            std::memcpy(&dst[6 * (size_t)start[i]], src, sizeof(double) * 6 * nb[g[i]->scene]);
+        }
+    };
+    vals.assign(k, 0.0);
+    rigc_status s = RIGC_OK;
+    if (kind == REQ_TOI) {
+        concat(c0, false);
+        concat(c1, true);
+        s = rigc_batch_earliest_toi(ctx, k, ids.data(), c0.data(), c1.data(), g[0]->a, vals.data());
+        if (s == RIGC_OK) {
+            rigc_stats cs{};
+            rigc_get_stats(ctx, &cs);
+            st.ccd_candidates += cs.num_candidates;
+            st.ccd_kernel_ms += cs.ccd_kernel_ms;
+            st.ccd_launches += cs.ccd_launches;
+            st.ccd_box_evaluations += cs.box_evaluations;
+            st.ccd_calls++;
+            for (int i = 0; i < k; i++) *g[i]->value = vals[i];
+        }
+    } else if (kind == REQ_MIND) {
+        concat(c0, false);
+        concat(c1, true);
+        s = rigc_batch_minimum_distance(ctx, k, ids.data(), c0.data(), c1.data(), g[0]->a, g[0]->b, vals.data());
+        if (s == RIGC_OK)
+            for (int i = 0; i < k; i++) *g[i]->value = vals[i];
+    } else if (kind == REQ_FUPDATE) {
+        concat(c0, false);
+        kap.resize(k);
+        mus.resize(k);
+        for (int i = 0; i < k; i++) {
+            kap[i] = g[i]->c;
+            mus[i] = g[i]->d;
+        }
+        cnt.assign(k, 0);
+        s = rigc_batch_update_friction_constraints(ctx, k, ids.data(), c0.data(), g[0]->a, g[0]->b, kap.data(),
+                                                   mus.data(), cnt.data());
+        if (s == RIGC_OK)
+            for (int i = 0; i < k; i++)
+                if (g[i]->count) *g[i]->count = cnt[i];
+    } else {
+        // barrier or friction potential
+        const int32_t flags = g[0]->flags;
+        const bool want_hess = (flags & RIGC_HESSIAN) != 0;
+        const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
+        concat(c0, false);
+        concat(c1, true);
+        if (want_grad) grad.assign(6 * (size_t)tot, 0.0);
+        std::vector<double> eh;
+        if (kind == REQ_FRICTION) {
+            eh.resize(k);
+            for (int i = 0; i < k; i++) eh[i] = g[i]->c;
+        }
+        cnt.assign(4 * (size_t)k, 0);
+        rigc_block_hessian h{};
+        for (int attempt = 0; attempt < 2; attempt++) {
+            if (want_hess) {
+                diag.resize(36 * (size_t)tot);
+                pairs.resize(2 * (size_t)std::max<int64_t>(cap, 1));
+                off.resize(36 * (size_t)std::max<int64_t>(cap, 1));
+                h = rigc_block_hessian{ diag.data(), pairs.data(), off.data(), cap, 0 };
+            }
+            if (kind == REQ_BARRIER)
+                s = rigc_batch_barrier_term(ctx, k, ids.data(), c0.data(), c1.data(), g[0]->a, g[0]->b, flags,
+                                            vals.data(), want_grad ? grad.data() : nullptr,
+                                            want_hess ? &h : nullptr, cnt.data());
+            else
+                s = rigc_batch_friction_term(ctx, k, ids.data(), c0.data(), c1.data(), eh.data(), flags, vals.data(),
+                                             want_grad ? grad.data() : nullptr, want_hess ? &h : nullptr);
+            if (s == RIGC_ERR_CAPACITY && want_hess && h.count > cap) {
+                cap = h.count;
+                continue;
+            }
+            break;
+        }
+        if (s == RIGC_OK) {
+            int64_t q = 0;  // pairs are sorted by first body: contiguous per scene
+            for (int i = 0; i < k; i++) {
+                Request* r = g[i];
+                *r->value = vals[i];
+                if (kind == REQ_BARRIER) {
+                    Proxy& P = proxies[r->scene];
+                    for (int c = 0; c < 4; c++) P.counts[c] = cnt[4 * i + c];
+                }
+                if (want_grad && r->grad)
+                    std::memcpy(r->grad, &grad[6 * (size_t)start[i]], sizeof(double) * 6 * nb[r->scene]);
+                if (want_hess && r->hess) {
+                    std::memcpy(r->hess->diag, &diag[36 * (size_t)start[i]], sizeof(double) * 36 * nb[r->scene]);
+                    int64_t q1 = q;
+                    while (q1 < h.count && pairs[2 * q1] < start[i + 1]) q1++;
+                    const int64_t m = q1 - q;
+                    r->hess->count = m;
+                    if (m > r->hess->capacity) {
+                        r->status = RIGC_ERR_CAPACITY;
+                        r->error = "off-diagonal Hessian capacity too small";
+                    } else {
+                        for (int64_t e = 0; e < m; e++) {
+                            r->hess->pairs[2 * e] = pairs[2 * (q + e)] - start[i];
+                            r->hess->pairs[2 * e + 1] = pairs[2 * (q + e) + 1] - start[i];
+                        }
+                        if (m) std::memcpy(r->hess->off, &off[36 * (size_t)q], sizeof(double) * 36 * m);
+                    }
+                    q = q1;
+                }
+            }
+            return;
+        }
+    }
+    if (s != RIGC_OK) fail_all(g, s, rigc_last_error(ctx));
+}
+
+}  // namespace
+
+struct rigs_batch {
+    int mode = RIGS_BATCH_FUSED;
+    int n = 0;
+    Batch B;
+    std::vector<rigs_sim*> sims;
+    std::vector<rigs_backend> tables;
+    std::vector<std::string> errors;
+    std::string err;
+};
+
+extern "C" {
+
+rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scenes, const rigc_scene* scenes,
+                              const rigs_bodies* bodies, const rigs_params* params, int32_t mode) {
+    if (!out || num_scenes < 1 || !scenes || !bodies || !params
+        || (mode != RIGS_BATCH_FUSED && mode != RIGS_BATCH_SEQUENTIAL))
+        return RIGC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    rigs_batch* b = new rigs_batch();
+    b->mode = mode;
+    b->n = num_scenes;
+    b->sims.assign(num_scenes, nullptr);
+    b->errors.assign(num_scenes, "");
+    rigc_status s = RIGC_OK;
+    if (mode == RIGS_BATCH_FUSED) {
+        Batch& B = b->B;
+        B.n = num_scenes;
+        s = rigc_create(&B.ctx, device);
+        if (s == RIGC_OK) s = rigc_set_scenes(B.ctx, num_scenes, scenes);
+        if (s != RIGC_OK) {
+            b->err = B.ctx ? rigc_last_error(B.ctx) : "rigc_create failed";
+        } else {
+            B.proxies.resize(num_scenes);
+            B.nb.resize(num_scenes);
+            b->tables.resize(num_scenes);
+            for (int i = 0; i < num_scenes; i++) {
+                B.nb[i] = scenes[i].num_bodies;
+                Proxy& P = B.proxies[i];
+                P.B = &B;
+                P.scene = i;
+                P.nb = scenes[i].num_bodies;
+                b->tables[i] = rigs_backend{ &P, px_set_scene, px_toi, px_build, px_barrier, px_mind, px_fupdate,
+                                             px_friction, px_error, nullptr };
+            }
+        }
+    }
+    for (int i = 0; i < num_scenes && s == RIGC_OK; i++) {
+        s = rigs_create(&b->sims[i], mode == RIGS_BATCH_FUSED ? &b->tables[i] : nullptr, device, &scenes[i],
+                        &bodies[i], &params[i]);
+        if (s != RIGC_OK) b->err = "rigs_create failed for scene " + std::to_string(i);
+    }
+    if (s != RIGC_OK) {
+        rigs_batch_destroy(b);
+        return s;
+    }
+    *out = b;
+    return RIGC_OK;
+}
+
+rigc_status rigs_batch_destroy(rigs_batch* b) {
+    if (!b) return RIGC_OK;
+    for (rigs_sim* s : b->sims)
+        if (s) rigs_destroy(s);
+    if (b->B.ctx) rigc_destroy(b->B.ctx);
+    delete b;
+    return RIGC_OK;
+}
+
+const char* rigs_batch_last_error(const rigs_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+rigc_status rigs_batch_step(rigs_batch* b, int32_t num_steps) {
+    if (!b || num_steps < 0) return RIGC_ERR_INVALID_ARGUMENT;
+    std::vector<rigc_status> res(b->n, RIGC_OK);
+    if (b->mode == RIGS_BATCH_SEQUENTIAL) {
+        for (int i = 0; i < b->n; i++) {
+            res[i] = rigs_step(b->sims[i], num_steps);
+            if (res[i] != RIGC_OK) b->errors[i] = rigs_last_error(b->sims[i]);
+        }
+    } else {
+        Batch& B = b->B;
+        B.active = b->n;
+        std::vector<std::thread> threads;
+        threads.reserve(b->n);
+        for (int i = 0; i < b->n; i++)
+            threads.emplace_back([b, i, num_steps, &res] {
+                res[i] = rigs_step(b->sims[i], num_steps);
+                if (res[i] != RIGC_OK) b->errors[i] = rigs_last_error(b->sims[i]);
+                leave(b->B);
+            });
+        for (auto& t : threads) t.join();
+    }
+    for (int i = 0; i < b->n; i++)
+        if (res[i] != RIGC_OK) {
+            b->err = "scene " + std::to_string(i) + ": " + b->errors[i];
+            return res[i];
+        }
+    return RIGC_OK;
+}
+
+rigc_status rigs_batch_get_state(const rigs_batch* b, int32_t scene, double* poses, double* velocities) {
+    if (!b || scene < 0 || scene >= b->n) return RIGC_ERR_INVALID_ARGUMENT;
+    return rigs_get_state(b->sims[scene], poses, velocities);
+}
+
+rigc_status rigs_batch_get_stats(const rigs_batch* b, int32_t scene, rigs_stats* stats) {
+    if (!b || scene < 0 || scene >= b->n) return RIGC_ERR_INVALID_ARGUMENT;
+    return rigs_get_stats(b->sims[scene], stats);
+}
+
+rigc_status rigs_batch_get_batch_stats(const rigs_batch* b, rigs_batch_stats* stats) {
+    if (!b || !stats) return RIGC_ERR_INVALID_ARGUMENT;
+    *stats = b->B.st;
+    return RIGC_OK;
+}
+
+}  // extern "C"

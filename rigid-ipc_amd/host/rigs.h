@@ -116,6 +116,41 @@ rigc_status rigs_step(rigs_sim* sim, int32_t num_steps);
 rigc_status rigs_get_state(const rigs_sim* sim, double* poses, double* velocities);
 rigc_status rigs_get_stats(const rigs_sim* sim, rigs_stats* stats);
 
+/* ---- Batches of independent scenes (no counterpart in the reference,
+ * which steps one scene per process).
+ * RIGS_BATCH_FUSED: each scene's stepper runs on a thread of its own; their
+ *   contact calls are gathered into rounds and served by fused calls on one
+ *   multi-scene device context (rigc_set_scenes / rigc_batch_*). Results
+ *   are bit-identical to stepping each scene alone.
+ * RIGS_BATCH_SEQUENTIAL: one scene after another, each with its own
+ *   single-scene context (the plain path). */
+enum { RIGS_BATCH_SEQUENTIAL = 0, RIGS_BATCH_FUSED = 1 };
+
+typedef struct rigs_batch_stats {
+    int64_t rounds;          /* combiner rounds (every stepping scene parked) */
+    int64_t fused_calls;     /* device calls made for those rounds */
+    int64_t requests;        /* scene requests they served */
+    double device_ms;        /* wall time of the rounds */
+    int64_t ccd_calls;       /* fused earliest-TOI calls */
+    int64_t ccd_candidates;
+    double ccd_kernel_ms;
+    int64_t ccd_launches;
+    int64_t ccd_box_evaluations;
+} rigs_batch_stats;
+
+typedef struct rigs_batch rigs_batch;
+
+/* scenes, bodies, params: arrays of num_scenes. */
+rigc_status rigs_batch_create(rigs_batch** batch, int32_t device, int32_t num_scenes, const rigc_scene* scenes,
+                              const rigs_bodies* bodies, const rigs_params* params, int32_t mode);
+rigc_status rigs_batch_destroy(rigs_batch* batch);
+const char* rigs_batch_last_error(const rigs_batch* batch);
+/* Every scene advances num_steps time steps. */
+rigc_status rigs_batch_step(rigs_batch* batch, int32_t num_steps);
+rigc_status rigs_batch_get_state(const rigs_batch* batch, int32_t scene, double* poses, double* velocities);
+rigc_status rigs_batch_get_stats(const rigs_batch* batch, int32_t scene, rigs_stats* stats);
+rigc_status rigs_batch_get_batch_stats(const rigs_batch* batch, rigs_batch_stats* stats);
+
 /* Sparse block LDL^T solve used for the Newton system (exposed for tests):
  * n bodies, diag [n][36], m pairs (i != j) with blocks (i,j), rhs b [6n]. */
 rigc_status rigs_block_ldlt_solve(int32_t n, const double* diag, int64_t m, const int32_t* pairs,

@@ -55,6 +55,18 @@ class Stats(ctypes.Structure):
     ]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("rounds", ctypes.c_int64), ("fused_calls", ctypes.c_int64), ("requests", ctypes.c_int64),
+        ("device_ms", ctypes.c_double), ("ccd_calls", ctypes.c_int64), ("ccd_candidates", ctypes.c_int64),
+        ("ccd_kernel_ms", ctypes.c_double), ("ccd_launches", ctypes.c_int64),
+        ("ccd_box_evaluations", ctypes.c_int64),
+    ]
+
+
+BATCH_SEQUENTIAL = 0
+BATCH_FUSED = 1
+
 _sim = None
 
 
@@ -78,8 +90,18 @@ def sim_lib():
     L.rigs_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.rigs_block_ldlt_solve.argtypes = [ctypes.c_int32, _c_double_p, ctypes.c_int64, _c_int32_p, _c_double_p,
                                         _c_double_p, _c_double_p]
+    L.rigs_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Scene),
+                                    ctypes.POINTER(Bodies), ctypes.POINTER(Params), ctypes.c_int32]
+    L.rigs_batch_destroy.argtypes = [vp]
+    L.rigs_batch_last_error.argtypes = [vp]
+    L.rigs_batch_last_error.restype = ctypes.c_char_p
+    L.rigs_batch_step.argtypes = [vp, ctypes.c_int32]
+    L.rigs_batch_get_state.argtypes = [vp, ctypes.c_int32, _c_double_p, _c_double_p]
+    L.rigs_batch_get_stats.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Stats)]
+    L.rigs_batch_get_batch_stats.argtypes = [vp, ctypes.POINTER(BatchStats)]
     for name in ("rigs_create", "rigs_destroy", "rigs_step", "rigs_get_state", "rigs_get_stats",
-                 "rigs_block_ldlt_solve"):
+                 "rigs_block_ldlt_solve", "rigs_batch_create", "rigs_batch_destroy", "rigs_batch_step",
+                 "rigs_batch_get_state", "rigs_batch_get_stats", "rigs_batch_get_batch_stats"):
         getattr(L, name).restype = ctypes.c_int32
     _sim = L
     return L
@@ -112,6 +134,90 @@ def block_ldlt_solve(diag, pairs, off, b):
     if st != RIGC_OK:
         raise RigcError(st, "block LDL^T failed")
     return x
+
+
+def _scene_inputs(scene):
+    """(keep-alive arrays, Scene, Bodies) of a scene from rigidipc_amd.scenes."""
+    k = dict(
+        vertices=np.ascontiguousarray(scene.vertices, np.float64),
+        edges=np.ascontiguousarray(scene.edges, np.int32),
+        faces=np.ascontiguousarray(scene.faces, np.int32),
+        face_edges=np.ascontiguousarray(scene.face_edges, np.int32),
+        body_vertex_start=np.ascontiguousarray(scene.body_vertex_start, np.int32),
+        body_edge_start=np.ascontiguousarray(scene.body_edge_start, np.int32),
+        body_face_start=np.ascontiguousarray(scene.body_face_start, np.int32),
+        group_ids=np.ascontiguousarray(scene.group_ids, np.int32),
+        is_static=np.ascontiguousarray(scene.is_static, np.uint8),
+        poses=np.ascontiguousarray(scene.poses, np.float64),
+        velocities=np.ascontiguousarray(scene.velocities, np.float64),
+        mass=np.ascontiguousarray(scene.mass, np.float64),
+        inertia=np.ascontiguousarray(scene.inertia, np.float64),
+    )
+    s = Scene()
+    s.num_bodies, s.num_vertices = len(k["group_ids"]), len(k["vertices"])
+    s.num_edges, s.num_faces = len(k["edges"]), len(k["faces"])
+    for name in ("vertices", "edges", "faces", "face_edges", "body_vertex_start", "body_edge_start",
+                 "body_face_start", "group_ids", "is_static"):
+        a = k[name]
+        t = {np.float64: _c_double_p, np.int32: _c_int32_p, np.uint8: _c_uint8_p}[a.dtype.type]
+        setattr(s, name, _p(a, t))
+    b = Bodies(s.num_bodies, _p(k["poses"], _c_double_p), _p(k["velocities"], _c_double_p),
+               _p(k["mass"], _c_double_p), _p(k["inertia"], _c_double_p))
+    return k, s, b
+
+
+class Batch:
+    """Many independent scenes stepped together (rigs_batch_*): mode
+    BATCH_FUSED serves all scenes' contact calls with fused multi-scene device
+    calls; BATCH_SEQUENTIAL steps them one after another on the plain path."""
+
+    def __init__(self, scenes, mode=BATCH_FUSED, params=None, device=0):
+        self._L = sim_lib()
+        ins = [_scene_inputs(sc) for sc in scenes]
+        self._keep = [k for k, _, _ in ins]
+        n = len(scenes)
+        S = (Scene * n)(*[s for _, s, _ in ins])
+        B = (Bodies * n)(*[b for _, _, b in ins])
+        P = (Params * n)(*[(params[i] if params is not None else default_params(sc)) for i, sc in enumerate(scenes)])
+        self._arrays = (S, B, P)
+        self.n = n
+        self.num_bodies = [int(len(sc.group_ids)) for sc in scenes]
+        self._b = ctypes.c_void_p()
+        st = self._L.rigs_batch_create(ctypes.byref(self._b), int(device), n, S, B, P, int(mode))
+        if st != RIGC_OK:
+            raise RigcError(st, "rigs_batch_create failed")
+
+    def close(self):
+        if self._b:
+            self._L.rigs_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        st = self._L.rigs_batch_step(self._b, int(n))
+        if st != RIGC_OK:
+            raise RigcError(st, self._L.rigs_batch_last_error(self._b).decode())
+
+    def state(self, i):
+        poses = np.zeros((self.num_bodies[i], 6))
+        vel = np.zeros((self.num_bodies[i], 6))
+        self._L.rigs_batch_get_state(self._b, int(i), _p(poses, _c_double_p), _p(vel, _c_double_p))
+        return poses, vel
+
+    def stats(self, i):
+        s = Stats()
+        self._L.rigs_batch_get_stats(self._b, int(i), ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def batch_stats(self):
+        s = BatchStats()
+        self._L.rigs_batch_get_batch_stats(self._b, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in BatchStats._fields_}
 
 
 class Simulation:
