@@ -1,6 +1,6 @@
 """Benchmark of the MI355X contact hot path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sim|ccd] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sim|ccd|batch] [--no-cpu]
 
 Workload "sim" (default): the 1,000-body mixed scene (ico2 / torus / cube,
 seeded, plus the ground plane; dt 0.01, dhat 1e-3, mu 0.2) stepped with
@@ -12,6 +12,11 @@ steps start from the initial state. value = K / wall time of the K steps.
 The CPU baseline runs the same stepper over the oracle (the CPU
 restatement, OpenMP over the host cores) from the same initial state for a
 bounded sample of steps.
+
+The sim line also carries "batched": the batching workload (256
+independent scenes of 8-512 bodies) advanced --batch-steps steps with the
+fused batch and with the plain path scene after scene (same trajectories).
+Workload "batch" runs that comparison alone for K steps (default all 10).
 
 Workload "ccd": one device CCD call (continuous broad phase + rigid
 interval narrow phase, rigc_compute_earliest_toi) per step along a
@@ -262,6 +267,53 @@ def bench_ccd(args):
     return line
 
 
+# ------------------------------------------------------------------ batch
+def batch_compare(n_scenes, steps, warm=True):
+    """The batching workload: n_scenes independent scenes (body counts
+    log-uniform in [8, 512], mix as the 1,000-body scene, dt 0.005 / 0.01,
+    dhat 1e-3), each advanced `steps` steps from its initial state, once with
+    the fused batch (one multi-scene context, every scene's Newton loop on a
+    host thread, contact calls served by fused device calls per round) and
+    once scene after scene on the plain single-scene path. Both produce the
+    same trajectories bit for bit; returns aggregate scene-steps/s of each."""
+    from rigidipc_amd import scenes as S
+    from rigidipc_amd import sim as M
+    scenes = [S.config3_scene(i) for i in range(n_scenes)]
+    if warm:
+        # device code and allocator warm-up on a small batch
+        w = M.Batch(scenes[:4], mode=M.BATCH_FUSED)
+        w.step(1)
+        w.close()
+    out = dict(scenes=n_scenes, steps=steps, bodies=int(sum(len(sc.group_ids) for sc in scenes)))
+    for name, mode in (("fused", M.BATCH_FUSED), ("plain", M.BATCH_SEQUENTIAL)):
+        b = M.Batch(scenes, mode=mode)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            t = time.perf_counter()
+            b.step(1)
+            log(f"batch {name} step {k + 1}/{steps}: {time.perf_counter() - t:.2f} s")
+        wall = time.perf_counter() - t0
+        newton = sum(b.stats(i)["newton_iterations"] for i in range(n_scenes))
+        out[name] = dict(scene_steps_per_s=n_scenes * steps / wall, wall_s=wall, newton_iterations=int(newton))
+        if mode == M.BATCH_FUSED:
+            bs = b.batch_stats()
+            out[name].update(rounds=int(bs["rounds"]), fused_calls=int(bs["fused_calls"]),
+                             requests=int(bs["requests"]))
+        b.close()
+    out["fused_over_plain"] = out["fused"]["scene_steps_per_s"] / out["plain"]["scene_steps_per_s"]
+    return out
+
+
+def bench_batch(args):
+    r = batch_compare(256, args.steps)
+    return dict(metric=METRIC, value=r["fused"]["scene_steps_per_s"], unit="scene time steps/s (batched)", n_gpus=1,
+                steps=args.steps, warmup=args.warmup, ms_per_step=1e3 * r["fused"]["wall_s"] / args.steps,
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload="256 independent scenes (8-512 bodies each, log-uniform; mixed ico2 / torus / "
+                                     "cube + plane; dt 0.005 or 0.01; dhat 1e-3; mu 0.2), all advanced together",
+                            **r))
+
+
 def bench_copy(args):
     """Calibration for the PMC traffic counters: device-to-device copies of a
     known size (reads and writes 256 MiB each per launch)."""
@@ -289,7 +341,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "copy"])
+    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "batch", "copy"])
+    ap.add_argument("--batch-steps", type=int, default=3,
+                    help="steps of the batched-vs-plain comparison added to the sim line (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
@@ -302,13 +356,21 @@ def main():
               "(the multi-GPU decomposition is a design, not part of the build)", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = 8 if args.workload == "sim" else 40
+        args.steps = {"sim": 8, "batch": 10}.get(args.workload, 40)
     if args.warmup is None:
-        args.warmup = 1 if args.workload == "sim" else 3
+        args.warmup = 1 if args.workload in ("sim", "batch") else 3
     if args.workload == "copy":
         line = bench_copy(args)
+    elif args.workload == "batch":
+        line = bench_batch(args)
+    elif args.workload == "ccd":
+        line = bench_ccd(args)
     else:
-        line = bench_sim(args) if args.workload == "sim" else bench_ccd(args)
+        line = bench_sim(args)
+        if args.batch_steps > 0:
+            line["batched"] = batch_compare(256, args.batch_steps)
+            line["batched"]["note"] = (f"first {args.batch_steps} of the batching workload's 10 steps; "
+                                       "bench.py --workload batch runs all 10")
     print(json.dumps(line), flush=True)
 
 

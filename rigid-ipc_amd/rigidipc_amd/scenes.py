@@ -73,7 +73,22 @@ def plane(size=20.0):
     return V, F
 
 
+_MESH_CACHE = {}
+
+
+def _cached(kind, arrays, fn):
+    """fn(*arrays), memoised on the arrays' bytes (bodies share meshes)."""
+    key = (kind,) + tuple((a.shape, a.dtype.str, a.tobytes()) for a in arrays)
+    if key not in _MESH_CACHE:
+        _MESH_CACHE[key] = fn(*arrays)
+    return _MESH_CACHE[key]
+
+
 def edges_of(F):
+    return _cached("edges", (np.asarray(F),), _edges_of).copy()
+
+
+def _edges_of(F):
     E = set()
     for f in F:
         for j in range(3):
@@ -142,7 +157,7 @@ class Body:
         self.group = group
         E = edges_of(F)
         if not static:
-            vol, com, I = mass_properties(V, F)
+            vol, com, I = _cached("mass", (np.asarray(V, np.float64), np.asarray(F)), mass_properties)
             V = V - com
             p = p + R @ com
             evals, evecs = np.linalg.eigh(I)
@@ -165,15 +180,19 @@ class Body:
         self.velocity = np.concatenate([v, Rw.T @ w]) if not static else np.zeros(6)
 
 
+def _face_edges(E, F):
+    emap = {(int(a), int(c)): i for i, (a, c) in enumerate(E)}
+    return np.array([[emap[(min(f[j], f[(j + 1) % 3]), max(f[j], f[(j + 1) % 3]))] for j in range(3)]
+                     for f in F], np.int64).reshape(-1, 3)
+
+
 def assemble(bodies):
     """Concatenate bodies into the arrays of include/rigc.h's rigc_scene."""
     V, E, F, FE = [], [], [], []
     bv, be, bf = [0], [0], [0]
+    nv0 = ne0 = 0
     for b in bodies:
-        nv0, ne0 = sum(len(x) for x in V), sum(len(x) for x in E)
-        emap = {(int(a), int(c)): i for i, (a, c) in enumerate(b.E)}
-        fe = np.array([[emap[(min(f[j], f[(j + 1) % 3]), max(f[j], f[(j + 1) % 3]))] for j in range(3)]
-                       for f in b.F], np.int64).reshape(-1, 3)
+        fe = _cached("face_edges", (np.asarray(b.E), np.asarray(b.F)), _face_edges)
         V.append(b.V)
         E.append(b.E + nv0)
         F.append(b.F + nv0)
@@ -181,7 +200,8 @@ def assemble(bodies):
         bv.append(bv[-1] + len(b.V))
         be.append(be[-1] + len(b.E))
         bf.append(bf[-1] + len(b.F))
-    nb = len(bodies)
+        nv0 += len(b.V)
+        ne0 += len(b.E)
     groups = [b.group if b.group is not None else i for i, b in enumerate(bodies)]
     return types.SimpleNamespace(
         vertices=np.concatenate(V).astype(np.float64),
