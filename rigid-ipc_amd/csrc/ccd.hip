@@ -46,6 +46,7 @@ namespace {
 
 constexpr int MAX_DEPTH = 192;
 constexpr int THR_REFRESH = 8;
+constexpr int SPEC_LEVELS = 5;  // speculative levels below an owner's box (62 boxes)
 
 struct Bits192 {
     unsigned long long w[3];
@@ -178,6 +179,49 @@ RIGC_DI void load_query(const CcdArgs& a, long long qi, Query& q, double tol[3])
         tol[2] = 1e-4 / a.edge_len[a.FE[3 * f + 1]];
     }
     q.sc = a.body_scene[q.bA];
+}
+
+// Widths of a dyadic box at levels lev (rounded up, as the root finder's
+// width()) and the bisection rule of interval_root_finder: split the
+// dimension of largest width / tolerance among those still wider than their
+// tolerance, or among all dimensions once every width is within tolerance.
+RIGC_DI void box_widths(const int lev[3], double w[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) w[i] = next_up(ldexp(1.0, -lev[i]));
+}
+RIGC_DI int split_dim(const double w[3], const double tol[3], bool all_sat) {
+    int sp = -1;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if ((all_sat || w[i] > tol[i]) && (sp == -1 || w[i] * tol[sp] > w[sp] * tol[i])) sp = i;
+    return sp;
+}
+RIGC_DI int split_dim_at(const int lev[3], const double tol[3]) {
+    double w[3];
+    box_widths(lev, w);
+    return split_dim(w, tol, w[0] <= tol[0] && w[1] <= tol[1] && w[2] <= tol[2]);
+}
+
+// Heap index (root 0, children of j at 2j + 1 and 2j + 2) of box (idx, lev)
+// in the bisection subtree of root (ridx, rlev) down to `levels` levels, or
+// -1 if the box is not a descendant within those levels.
+RIGC_DI int subtree_node(const unsigned long long ridx[3], const int rlev[3], const unsigned long long idx[3],
+                         const int lev[3], const double tol[3], int levels) {
+    const int dl = (lev[0] + lev[1] + lev[2]) - (rlev[0] + rlev[1] + rlev[2]);
+    if (dl < 1 || dl > levels) return -1;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if (lev[i] < rlev[i] || (idx[i] >> (lev[i] - rlev[i])) != ridx[i]) return -1;
+    int cl[3] = { rlev[0], rlev[1], rlev[2] };
+    unsigned m = 1;
+    for (int k = 0; k < dl; k++) {
+        const int sp = split_dim_at(cl, tol);
+        if (cl[sp] >= lev[sp]) return -1;
+        m = 2 * m + (unsigned)((idx[sp] >> (lev[sp] - cl[sp] - 1)) & 1ULL);
+        cl[sp]++;
+    }
+    if (cl[0] != lev[0] || cl[1] != lev[1] || cl[2] != lev[2]) return -1;
+    return (int)m - 1;
 }
 
 __global__ void ccd_init_kernel(CcdArgs a) {
@@ -335,32 +379,35 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 }
             }
         }
-        if (qi < 0) continue;
-
-        // C. one box per lane
+        // C. one box per lane of its own query. Lanes left without work
+        // evaluate, for lanes evaluating a bisection box, the descendants of
+        // that box down to SPEC_LEVELS levels (each box of the subtree is a
+        // fixed function of its root: the split dimension depends only on
+        // the box's widths); the owner then walks those levels from the
+        // recorded results without waiting for further evaluations. Which
+        // boxes are reached and accepted is unchanged, so is the minimum.
         bool done = false;
+        bool own_eval = false;
         Iv t, u, v;
-        if (phase == 1) {
-            t = Iv{ 0.0, tol[0] };
-            u = Iv{ 0.0, tol[1] };
-            v = Iv{ 0.0, tol[2] };
-        } else {
-            // The pruning bound only decreases; a stale (larger) bound prunes
-            // less but never changes the minimum, so it is refreshed from
-            // global memory every THR_REFRESH boxes (or on a new subtree).
-            if (thr_age <= 0 || thr_q != qi) {
-                thr = __longlong_as_double((long long)__hip_atomic_load(a.q_best + qi, __ATOMIC_RELAXED,
-                                                                        __HIP_MEMORY_SCOPE_AGENT));
-                if (a.use_global_best) {
-                    const unsigned long long gb =
-                        __hip_atomic_load(a.global_best + q.sc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    thr = fmin(thr, __longlong_as_double((long long)gb));
-                }
-                thr_q = qi;
-                thr_age = THR_REFRESH;
+        // Pops to the next pending second half; done when none is left.
+        auto pop = [&]() {
+            const int L = last_zero_below(s, depth);
+            if (L < 0) {
+                done = true;
+                return;
             }
-            thr_age--;
-            if (a.eval_budget > 0 && a.q_flag[qi] != 0) done = true;
+            const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
+            const int cb = depth - 1 - L - ct - ca;
+            idx[0] >>= ct; lev[0] -= ct;
+            idx[1] >>= ca; lev[1] -= ca;
+            idx[2] >>= cb; lev[2] -= cb;
+            const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
+            idx[d] += 1;
+            bset(s, L, true);
+            depth = L + 1;
+        };
+        // Skips boxes pruned by the running minimum or outside the FV domain.
+        auto skip_pruned = [&]() {
             while (!done) {
                 if (a.eval_budget > 0) {
                     const unsigned int nv = atomicAdd(a.q_visits + qi, 1u) + 1u;
@@ -377,69 +424,26 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     skip = !overlap(uu + vv, Iv{ 0.0, 1.0 });
                 }
                 if (!skip) break;
-                const int L = last_zero_below(s, depth);
-                if (L < 0) {
-                    done = true;
-                    break;
-                }
-                const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
-                const int cb = depth - 1 - L - ct - ca;
-                idx[0] >>= ct; lev[0] -= ct;
-                idx[1] >>= ca; lev[1] -= ca;
-                idx[2] >>= cb; lev[2] -= cb;
-                const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
-                idx[d] += 1;
-                bset(s, L, true);
-                depth = L + 1;
+                pop();
             }
-            if (!done) {
-                t = Iv{ dyad(idx[0], lev[0]), dyad(idx[0] + 1, lev[0]) };
-                u = Iv{ dyad(idx[1], lev[1]), dyad(idx[1] + 1, lev[1]) };
-                v = Iv{ dyad(idx[2], lev[2]), dyad(idx[2] + 1, lev[2]) };
-            }
-        }
-
-        if (!done) {
-            if (phase == 1 || !cache_ok || c_idx != idx[0] || c_lev != lev[0]) {
-                t_part(a, q, t, A0, DA, B0, DB);
-                cache_ok = phase == 2;
-                c_idx = idx[0];
-                c_lev = lev[0];
-            }
-            const bool z = contains_zero(q.kind, A0, DA, B0, DB, u, v);
-            evals++;
-            walk++;
-            if (phase == 1) {
-                if (z) tol[0] /= 1e2;
-                phase = 2;
-                depth = 0;
-                s.w[0] = s.w[1] = s.w[2] = 0;
-                mt = s;
-                ma = s;
-                idx[0] = idx[1] = idx[2] = 0;
-                lev[0] = lev[1] = lev[2] = 0;
-                cache_ok = false;
-                continue;
-            }
+        };
+        // Consumes the zero-inclusion result z of the current box.
+        auto process = [&](bool z) {
             bool leaf = !z;
             if (z) {
-                const double w0 = next_up(ldexp(1.0, -lev[0]));
-                const double w1 = next_up(ldexp(1.0, -lev[1]));
-                const double w2 = next_up(ldexp(1.0, -lev[2]));
-                const double w[3] = { w0, w1, w2 };
-                const bool all_sat = w0 <= tol[0] && w1 <= tol[1] && w2 <= tol[2];
-                const bool all_zero = w0 <= 1e-10 && w1 <= 1e-10 && w2 <= 1e-10;
+                double w[3];
+                box_widths(lev, w);
+                const bool all_sat = w[0] <= tol[0] && w[1] <= tol[1] && w[2] <= tol[2];
+                const bool all_zero = w[0] <= 1e-10 && w[1] <= 1e-10 && w[2] <= 1e-10;
                 if ((idx[0] > 0 || all_zero) && all_sat) {
-                    const unsigned long long bits = (unsigned long long)__double_as_longlong(t.lo);
+                    const double tlo = dyad(idx[0], lev[0]);
+                    const unsigned long long bits = (unsigned long long)__double_as_longlong(tlo);
                     atomicMin(a.q_best + qi, bits);
-                    thr = fmin(thr, t.lo);
+                    thr = fmin(thr, tlo);
                     if (a.use_global_best) atomicMin(a.global_best + q.sc, bits);
                     leaf = true;
                 } else {
-                    int sp = -1;
-#pragma unroll
-                    for (int i = 0; i < 3; i++)
-                        if ((all_sat || w[i] > tol[i]) && (sp == -1 || w[i] * tol[sp] > w[sp] * tol[i])) sp = i;
+                    const int sp = split_dim(w, tol, all_sat);
                     if (depth >= MAX_DEPTH) {
                         atomicMax(a.q_flag + qi, 2);
                         done = true;
@@ -453,25 +457,167 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     }
                 }
             }
-            if (leaf && !done) {
-                const int L = last_zero_below(s, depth);
-                if (L < 0) {
-                    done = true;
-                } else {
+            if (leaf && !done) pop();
+        };
+
+        if (qi >= 0) {
+            if (phase == 1) {
+                t = Iv{ 0.0, tol[0] };
+                u = Iv{ 0.0, tol[1] };
+                v = Iv{ 0.0, tol[2] };
+                own_eval = true;
+            } else {
+                // The pruning bound only decreases; a stale (larger) bound
+                // prunes less but never changes the minimum, so it is
+                // refreshed from global memory every THR_REFRESH boxes (or
+                // on a new subtree).
+                if (thr_age <= 0 || thr_q != qi) {
+                    thr = __longlong_as_double((long long)__hip_atomic_load(a.q_best + qi, __ATOMIC_RELAXED,
+                                                                            __HIP_MEMORY_SCOPE_AGENT));
+                    if (a.use_global_best) {
+                        const unsigned long long gb =
+                            __hip_atomic_load(a.global_best + q.sc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        thr = fmin(thr, __longlong_as_double((long long)gb));
+                    }
+                    thr_q = qi;
+                    thr_age = THR_REFRESH;
+                }
+                thr_age--;
+                if (a.eval_budget > 0 && a.q_flag[qi] != 0) done = true;
+                skip_pruned();
+                if (!done) {
+                    t = Iv{ dyad(idx[0], lev[0]), dyad(idx[0] + 1, lev[0]) };
+                    u = Iv{ dyad(idx[1], lev[1]), dyad(idx[1] + 1, lev[1]) };
+                    v = Iv{ dyad(idx[2], lev[2]), dyad(idx[2] + 1, lev[2]) };
+                    own_eval = true;
+                }
+            }
+        }
+
+        // speculation assignment: idle lane of rank h serves owner h / per,
+        // subtree node h % per + 1
+        const unsigned long long owners = __ballot(own_eval && phase == 2 && a.eval_budget == 0);
+        const unsigned long long idle = __ballot(qi < 0);
+        const int n_own = __popcll(owners), n_idle = __popcll(idle);
+        const int g = n_own ? n_idle / n_own : 0;
+        int levels = 0;
+        while (levels < SPEC_LEVELS && (2 << (levels + 1)) - 2 <= g) levels++;
+        const int per = levels ? (2 << levels) - 2 : 0;
+        int src = lane, node = 0;
+        if (per && qi < 0) {
+            const int h = __popcll(idle & lt_mask);
+            const int o = h / per;
+            if (o < n_own) {
+                src = nth_set_bit(owners, o);
+                node = h % per + 1;
+            }
+        }
+        const long long s_q = __shfl(qi, src);
+        const double s_tol0 = __shfl(tol[0], src);
+        const unsigned long long s_i0 = __shfl(idx[0], src), s_i1 = __shfl(idx[1], src), s_i2 = __shfl(idx[2], src);
+        const int s_lev = __shfl(lev[0] | (lev[1] << 8) | (lev[2] << 16), src);
+        bool spec = false;
+        unsigned long long e_idx0 = idx[0];
+        int e_lev0 = lev[0];
+        if (node > 0) {
+            // this lane is idle: its query registers are free until it takes work
+            load_query(a, s_q, q, tol);
+            tol[0] = s_tol0;
+            unsigned long long bi[3] = { s_i0, s_i1, s_i2 };
+            int bl[3] = { s_lev & 255, (s_lev >> 8) & 255, (s_lev >> 16) & 255 };
+            const unsigned m = (unsigned)node + 1u;
+            for (int j = 30 - __clz(m); j >= 0; j--) {
+                const int sp = split_dim_at(bl, tol);
+                bi[sp] = (bi[sp] << 1) + (unsigned long long)((m >> j) & 1u);
+                bl[sp] += 1;
+            }
+            t = Iv{ dyad(bi[0], bl[0]), dyad(bi[0] + 1, bl[0]) };
+            u = Iv{ dyad(bi[1], bl[1]), dyad(bi[1] + 1, bl[1]) };
+            v = Iv{ dyad(bi[2], bl[2]), dyad(bi[2] + 1, bl[2]) };
+            e_idx0 = bi[0];
+            e_lev0 = bl[0];
+            spec = true;
+            cache_ok = false;
+        }
+
+        bool z = false;
+        if (own_eval || spec) {
+            if (spec || phase == 1 || !cache_ok || c_idx != e_idx0 || c_lev != e_lev0) {
+                t_part(a, q, t, A0, DA, B0, DB);
+                cache_ok = !spec && phase == 2;
+                c_idx = e_idx0;
+                c_lev = e_lev0;
+            }
+            z = contains_zero(q.kind, A0, DA, B0, DB, u, v);
+            evals++;
+        }
+        if (spec) cache_ok = false;
+        const unsigned long long zmask = __ballot(spec && z);
+
+        if (own_eval) {
+            walk++;
+            if (phase == 1) {
+                if (z) tol[0] /= 1e2;
+                phase = 2;
+                depth = 0;
+                s.w[0] = s.w[1] = s.w[2] = 0;
+                mt = s;
+                ma = s;
+                idx[0] = idx[1] = idx[2] = 0;
+                lev[0] = lev[1] = lev[2] = 0;
+                cache_ok = false;
+                continue;
+            }
+            // results of this owner's helpers (idle lanes of ranks
+            // [o * per, o * per + per)), bit k = subtree node k
+            unsigned long long known = 0;
+            int k_levels = 0;
+            unsigned long long ridx[3] = { idx[0], idx[1], idx[2] };
+            int rlev[3] = { lev[0], lev[1], lev[2] };
+            if (per && ((owners >> lane) & 1ULL)) {
+                const int o = __popcll(owners & lt_mask);
+                unsigned long long hm = idle;
+                int r = 0;
+                while (hm) {
+                    const int l = __ffsll((long long)hm) - 1;
+                    hm &= hm - 1ULL;
+                    if (r >= o * per && r < o * per + per) known |= ((zmask >> l) & 1ULL) << (r - o * per + 1);
+                    r++;
+                }
+                k_levels = levels;
+            }
+            process(z);
+            while (!done && k_levels > 0) {
+                skip_pruned();
+                if (done) break;
+                const int kn = subtree_node(ridx, rlev, idx, lev, tol, k_levels);
+                if (kn < 0) break;
+                walk++;
+                process(((known >> kn) & 1ULL) != 0ULL);
+            }
+            // The walk left the subtree below its last level: every pending
+            // second half inside the subtree whose box is known not to hold
+            // zero is a leaf of the search, so it is marked visited (what
+            // visiting it would conclude).
+            if (!done && k_levels > 0) {
+                for (int L = depth - 1; L >= 0; L--) {
+                    if (bget(s, L)) continue;
                     const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
                     const int cb = depth - 1 - L - ct - ca;
-                    idx[0] >>= ct; lev[0] -= ct;
-                    idx[1] >>= ca; lev[1] -= ca;
-                    idx[2] >>= cb; lev[2] -= cb;
+                    unsigned long long si[3] = { idx[0] >> ct, idx[1] >> ca, idx[2] >> cb };
+                    const int sl[3] = { lev[0] - ct, lev[1] - ca, lev[2] - cb };
                     const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
-                    idx[d] += 1;
-                    bset(s, L, true);
-                    depth = L + 1;
+                    si[d] += 1;
+                    const int rd = (sl[0] + sl[1] + sl[2]) - (rlev[0] + rlev[1] + rlev[2]);
+                    if (rd > k_levels) continue;  // below the subtree's last level
+                    if (rd < 1) break;            // at or above its root: shallower levels are outside it
+                    const int kn = subtree_node(ridx, rlev, si, sl, tol, k_levels);
+                    if (kn >= 0 && ((known >> kn) & 1ULL) == 0ULL) bset(s, L, true);
                 }
             }
         }
         // D. export the pending subtrees of a long walk to the next pass
-        if (!done && phase == 2 && !pool_open && a.tasks_out && walk >= a.walk_budget) {
+        if (qi >= 0 && !done && phase == 2 && !pool_open && a.tasks_out && walk >= a.walk_budget) {
             int n_open = 0;
             for (int wi = 0; wi < 3; wi++) n_open += __popcll(~s.w[wi] & range_mask(wi, 0, depth));
             const unsigned long long base = atomicAdd(a.n_tasks_out, (unsigned long long)(n_open + 1));
