@@ -275,6 +275,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
     const long long nwaves = (long long)gridDim.x * (blockDim.x / 64);
     const int quota = (int)max(1LL, min(64LL, (npool + nwaves - 1) / nwaves));
     bool pool_open = true;
+    long long spec_q = -1;  // query whose data this idle lane holds from helping (-1: none)
     for (;;) {
         // A. idle lanes take new queries while the pool lasts
         if (pool_open) {
@@ -295,6 +296,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                     if (mine < npool) {
                         walk = 0;
                         cache_ok = false;
+                        spec_q = -1;
                         if (a.tasks_in) {
                             const CcdTask& tk = a.tasks_in[mine];
                             qi = tk.qi;
@@ -365,6 +367,7 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 const int r_lev = __shfl(d_lev, src);
                 if (qi < 0 && irank < k) {
                     qi = r_q;
+                    spec_q = -1;
                     load_query(a, qi, q, tol);
                     tol[0] = r_tol0;
                     idx[0] = r0; idx[1] = r1; idx[2] = r2;
@@ -521,7 +524,10 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
         int e_lev0 = lev[0];
         if (node > 0) {
             // this lane is idle: its query registers are free until it takes work
-            load_query(a, s_q, q, tol);
+            if (s_q != spec_q) {
+                load_query(a, s_q, q, tol);
+                spec_q = s_q;
+            }
             tol[0] = s_tol0;
             unsigned long long bi[3] = { s_i0, s_i1, s_i2 };
             int bl[3] = { s_lev & 255, (s_lev >> 8) & 255, (s_lev >> 16) & 255 };

@@ -156,8 +156,8 @@ struct CcdWork {
         tasks[0].release();
         tasks[1].release();
         count.release();
-        if (k0) hipEventDestroy(k0);
-        if (k1) hipEventDestroy(k1);
+        if (k0) (void)hipEventDestroy(k0);
+        if (k1) (void)hipEventDestroy(k1);
         k0 = k1 = nullptr;
     }
 };

@@ -17,12 +17,18 @@ RIGC_DI void rotation_iv(const Iv r[3], Iv R[3][3]) {
     const Iv z = iv(0.0);
     const Iv K[3][3] = { { z, -r[2], r[1] }, { r[2], z, -r[0] }, { -r[1], r[0], z } };
     const Iv c2 = (sinc_half * 0.5) * sinc_half;
+    // K2 = K * K and sinc * K term by term in Eigen's order; a product with
+    // a diagonal (exactly zero) entry of K is the exact zero interval, so it
+    // is not computed, while its (outward-rounded) addition is kept.
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const Iv k2 = K[i][0] * K[0][j] + (K[i][1] * K[1][j] + K[i][2] * K[2][j]);
-            R[i][j] = sinc_angle * K[i][j] + c2 * k2;
+            const Iv t0 = (i == 0 || j == 0) ? z : K[i][0] * K[0][j];
+            const Iv t1 = (i == 1 || j == 1) ? z : K[i][1] * K[1][j];
+            const Iv t2 = (i == 2 || j == 2) ? z : K[i][2] * K[2][j];
+            const Iv k2 = t0 + (t1 + t2);
+            R[i][j] = (i == j ? z : sinc_angle * K[i][j]) + c2 * k2;
         }
 #pragma unroll
     for (int i = 0; i < 3; i++) R[i][i] = R[i][i] + iv(1.0);
