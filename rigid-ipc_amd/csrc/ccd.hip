@@ -92,30 +92,91 @@ struct Query {
 // t-dependent part of the inclusion function.
 // EE: A0 = ea0(t), DA = ea1(t) - ea0(t), B0 = eb0(t), DB = eb1(t) - eb0(t)
 // FV: A0 = v(t),   DA = f1(t) - f0(t),   B0 = f0(t),  DB = f2(t) - f0(t)
+// Written as loops over the body, the sinc argument (the norm of r and of
+// r / 2), the sinc piece (the monotone piece's end points and the piece
+// beyond its maximum) and the vertices, so that each interval routine is
+// inlined once: the kernel stays within the instruction cache. The
+// operations and their order per value are those of pose_iv,
+// rotation_iv (sinc_iv) and world_vertex_iv.
 RIGC_DI void t_part(const CcdArgs& a, const Query& q, Iv t, Iv A0[3], Iv DA[3], Iv B0[3], Iv DB[3]) {
-    Iv p[3], R[3][3];
-    pose_iv(a.poses_t0 + 6 * q.bA, a.poses_t1 + 6 * q.bA, t, p, R);
-    world_vertex_iv(R, p, a.V + 3 * q.g[0], A0);
-    if (q.kind == 0) {
-        Iv w1[3];
-        world_vertex_iv(R, p, a.V + 3 * q.g[1], w1);
+#pragma unroll 1
+    for (int side = 0; side < 2; side++) {
+        const int body = side ? q.bB : q.bA;
+        const double* p0 = a.poses_t0 + 6 * body;
+        const double* p1 = a.poses_t1 + 6 * body;
+        Iv r[3], xa, xh;
+        pose_rot_iv(p0, p1, t, r);
+        rotation_norms(r, xa, xh);
+        double lo_a = 0.0, hi_a = 0.0, lo_h = 0.0, hi_h = 0.0;
+        Iv g_a = iv(0.0), g_h = iv(0.0);
+#pragma unroll 1
+        for (int k = 0; k < 6; k++) {
+            const bool half = k >= 3;
+            const int piece = k % 3;
+            const Iv x = half ? xh : xa;
+            const bool active = piece < 2 ? x.lo <= RIGC_SINC_MB : sinc_has_gt(x);
+            if (!active) continue;
+            Iv arg, val;
+            bool taylor = false;
+            if (piece < 2) {
+                const double d = piece == 0 ? x.lo : (x.hi > RIGC_SINC_MB ? RIGC_SINC_MB : x.hi);
+                arg = iv(d);
+                taylor = !(fabs(d) >= 1.220703125e-04);
+            } else {
+                arg = (x.lo <= RIGC_SINC_MB) ? Iv{ RIGC_SINC_MB, x.hi } : x;
+            }
+            // sinc_point(d) for a point argument (Taylor below 2^-13),
+            // sin(x) / x for the piece beyond mb
+            if (taylor) {
+                const Iv x2 = arg * arg;
+                val = 1.0 + (x2 / 6.0) * ((x2 / 20.0) - 1.0);
+            } else {
+                val = div_nonzero(sin_iv(arg), arg);
+            }
+            if (k == 0) lo_a = val.hi;
+            else if (k == 1) hi_a = val.lo;
+            else if (k == 2) g_a = val;
+            else if (k == 3) lo_h = val.hi;
+            else if (k == 4) hi_h = val.lo;
+            else g_h = val;
+        }
+        Iv R[3][3], p[3];
+        rotation_from_sincs(r, sinc_combine(xa, lo_a, hi_a, g_a), sinc_combine(xh, lo_h, hi_h, g_h), R);
+        pose_pos_iv(p0, p1, t, p);
+        // EE: this body's edge (2 vertices); FV: the vertex (side 0) or the
+        // face (side 1, 3 vertices)
+        const int nvx = q.kind == 0 ? 2 : (side ? 3 : 1);
+        const int v0 = q.kind == 0 ? (side ? q.g[2] : q.g[0]) : (side ? q.g[1] : q.g[0]);
+        const int v1 = q.kind == 0 ? (side ? q.g[3] : q.g[1]) : q.g[2];
+        Iv o1[3], o2[3], o3[3];
 #pragma unroll
-        for (int j = 0; j < 3; j++) DA[j] = w1[j] - A0[j];
-        pose_iv(a.poses_t0 + 6 * q.bB, a.poses_t1 + 6 * q.bB, t, p, R);
-        world_vertex_iv(R, p, a.V + 3 * q.g[2], B0);
-        world_vertex_iv(R, p, a.V + 3 * q.g[3], w1);
+        for (int j = 0; j < 3; j++) o1[j] = o2[j] = o3[j] = iv(0.0);
+#pragma unroll 1
+        for (int m = 0; m < nvx; m++) {
+            const int vid = m == 0 ? v0 : (m == 1 ? v1 : q.g[3]);
+            Iv w[3];
+            world_vertex_iv(R, p, a.V + 3 * vid, w);
 #pragma unroll
-        for (int j = 0; j < 3; j++) DB[j] = w1[j] - B0[j];
-    } else {
-        Iv f1[3], f2[3];
-        pose_iv(a.poses_t0 + 6 * q.bB, a.poses_t1 + 6 * q.bB, t, p, R);
-        world_vertex_iv(R, p, a.V + 3 * q.g[1], B0);
-        world_vertex_iv(R, p, a.V + 3 * q.g[2], f1);
-        world_vertex_iv(R, p, a.V + 3 * q.g[3], f2);
+            for (int j = 0; j < 3; j++) {
+                if (m == 0) o1[j] = w[j];
+                else if (m == 1) o2[j] = w[j];
+                else o3[j] = w[j];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            DA[j] = f1[j] - B0[j];
-            DB[j] = f2[j] - B0[j];
+            if (side == 0) {
+                A0[j] = o1[j];
+                if (q.kind == 0) DA[j] = o2[j] - o1[j];
+            } else {
+                B0[j] = o1[j];
+                if (q.kind == 0) {
+                    DB[j] = o2[j] - o1[j];
+                } else {
+                    DA[j] = o2[j] - o1[j];
+                    DB[j] = o3[j] - o1[j];
+                }
+            }
         }
     }
 }
@@ -202,28 +263,6 @@ RIGC_DI int split_dim_at(const int lev[3], const double tol[3]) {
     return split_dim(w, tol, w[0] <= tol[0] && w[1] <= tol[1] && w[2] <= tol[2]);
 }
 
-// Heap index (root 0, children of j at 2j + 1 and 2j + 2) of box (idx, lev)
-// in the bisection subtree of root (ridx, rlev) down to `levels` levels, or
-// -1 if the box is not a descendant within those levels.
-RIGC_DI int subtree_node(const unsigned long long ridx[3], const int rlev[3], const unsigned long long idx[3],
-                         const int lev[3], const double tol[3], int levels) {
-    const int dl = (lev[0] + lev[1] + lev[2]) - (rlev[0] + rlev[1] + rlev[2]);
-    if (dl < 1 || dl > levels) return -1;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-        if (lev[i] < rlev[i] || (idx[i] >> (lev[i] - rlev[i])) != ridx[i]) return -1;
-    int cl[3] = { rlev[0], rlev[1], rlev[2] };
-    unsigned m = 1;
-    for (int k = 0; k < dl; k++) {
-        const int sp = split_dim_at(cl, tol);
-        if (cl[sp] >= lev[sp]) return -1;
-        m = 2 * m + (unsigned)((idx[sp] >> (lev[sp] - cl[sp] - 1)) & 1ULL);
-        cl[sp]++;
-    }
-    if (cl[0] != lev[0] || cl[1] != lev[1] || cl[2] != lev[2]) return -1;
-    return (int)m - 1;
-}
-
 __global__ void ccd_init_kernel(CcdArgs a) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.n_ee + a.n_fv) {
@@ -243,7 +282,7 @@ __global__ void ccd_finish_kernel(CcdArgs a) {
     }
 }
 
-__global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void ccd_kernel(CcdArgs a) {
     const long long nq = a.n_ee + a.n_fv;
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
@@ -393,12 +432,14 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
         bool own_eval = false;
         Iv t, u, v;
         // Pops to the next pending second half; done when none is left.
+        int pop_level = -1;  // level of the last pop (-1: none since reset)
         auto pop = [&]() {
             const int L = last_zero_below(s, depth);
             if (L < 0) {
                 done = true;
                 return;
             }
+            pop_level = L;
             const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
             const int cb = depth - 1 - L - ct - ca;
             idx[0] >>= ct; lev[0] -= ct;
@@ -498,7 +539,8 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
         }
 
         // speculation assignment: idle lane of rank h serves owner h / per,
-        // subtree node h % per + 1
+        // subtree node h % per + 1 (heap order: root 0, children of node j
+        // at 2j + 1 and 2j + 2)
         const unsigned long long owners = __ballot(own_eval && phase == 2 && a.eval_budget == 0);
         const unsigned long long idle = __ballot(qi < 0);
         const int n_own = __popcll(owners), n_idle = __popcll(idle);
@@ -546,8 +588,9 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
             cache_ok = false;
         }
 
+        const bool evaluating = own_eval || spec;
         bool z = false;
-        if (own_eval || spec) {
+        if (evaluating) {
             if (spec || phase == 1 || !cache_ok || c_idx != e_idx0 || c_lev != e_lev0) {
                 t_part(a, q, t, A0, DA, B0, DB);
                 cache_ok = !spec && phase == 2;
@@ -578,8 +621,6 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
             // [o * per, o * per + per)), bit k = subtree node k
             unsigned long long known = 0;
             int k_levels = 0;
-            unsigned long long ridx[3] = { idx[0], idx[1], idx[2] };
-            int rlev[3] = { lev[0], lev[1], lev[2] };
             if (per && ((owners >> lane) & 1ULL)) {
                 const int o = __popcll(owners & lt_mask);
                 unsigned long long hm = idle;
@@ -592,33 +633,53 @@ __global__ __launch_bounds__(256) void ccd_kernel(CcdArgs a) {
                 }
                 k_levels = levels;
             }
-            process(z);
-            while (!done && k_levels > 0) {
+            // The current box's heap index kc and depth dc below the subtree
+            // root (the box just evaluated), tracked through descents (child
+            // 2 kc + 1) and pops (the second half at the popped level: the
+            // path's node there, plus one).
+            const int root_d = depth;
+            int kc = 0, dc = 0;
+            auto track = [&](int d_before) {
+                if (pop_level >= 0) {
+                    const int dn = pop_level + 1 - root_d;
+                    if (dn < 1) {
+                        dc = -1;
+                    } else {
+                        kc = ((kc + 1) >> (dc - dn)) - 1 + 1;
+                        dc = dn;
+                    }
+                } else if (depth == d_before + 1) {
+                    kc = 2 * kc + 1;
+                    dc++;
+                }
+            };
+            bool zc = z;
+            for (;;) {
+                int d0 = depth;
+                pop_level = -1;
+                process(zc);
+                track(d0);
+                if (done || !(dc >= 1 && dc <= k_levels)) break;
+                d0 = depth;
+                pop_level = -1;
                 skip_pruned();
                 if (done) break;
-                const int kn = subtree_node(ridx, rlev, idx, lev, tol, k_levels);
-                if (kn < 0) break;
+                track(d0);
+                if (!(dc >= 1 && dc <= k_levels)) break;
                 walk++;
-                process(((known >> kn) & 1ULL) != 0ULL);
+                zc = ((known >> kc) & 1ULL) != 0ULL;
             }
             // The walk left the subtree below its last level: every pending
             // second half inside the subtree whose box is known not to hold
             // zero is a leaf of the search, so it is marked visited (what
             // visiting it would conclude).
-            if (!done && k_levels > 0) {
-                for (int L = depth - 1; L >= 0; L--) {
+            if (!done && k_levels > 0 && dc > k_levels) {
+                for (int L = depth - 1; L >= root_d; L--) {
                     if (bget(s, L)) continue;
-                    const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
-                    const int cb = depth - 1 - L - ct - ca;
-                    unsigned long long si[3] = { idx[0] >> ct, idx[1] >> ca, idx[2] >> cb };
-                    const int sl[3] = { lev[0] - ct, lev[1] - ca, lev[2] - cb };
-                    const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
-                    si[d] += 1;
-                    const int rd = (sl[0] + sl[1] + sl[2]) - (rlev[0] + rlev[1] + rlev[2]);
-                    if (rd > k_levels) continue;  // below the subtree's last level
-                    if (rd < 1) break;            // at or above its root: shallower levels are outside it
-                    const int kn = subtree_node(ridx, rlev, si, sl, tol, k_levels);
-                    if (kn >= 0 && ((known >> kn) & 1ULL) == 0ULL) bset(s, L, true);
+                    const int dn = L + 1 - root_d;
+                    if (dn > k_levels) continue;
+                    const int ks = ((kc + 1) >> (dc - dn)) - 1 + 1;
+                    if (((known >> ks) & 1ULL) == 0ULL) bset(s, L, true);
                 }
             }
         }

@@ -205,6 +205,7 @@ RIGC_DI Iv cos_iv(Iv x) {
     bool negate = false;
     bool downward = false;  // FPU state at the current fmod
     Iv r;
+#pragma unroll 1
     for (int level = 0; level < 4; level++) {
         const double yb = (x.lo < 0.0) ? pi2.lo : pi2.hi;
         const double q = downward ? step_dn(div_rd(x.lo, yb)) : step_dn(x.lo / yb);
@@ -246,24 +247,34 @@ RIGC_DI Iv sinc_point(double xd) {
 }
 
 // sinc(Interval) (src/utils/sinc.cpp:49-101) for x.lo >= 0 (x is always a
-// norm here).
-RIGC_DI Iv sinc_iv(Iv x) {
-    const double mb = 4.4934094579;
-    Iv y = iv_emptyval();
-    Iv x_gt = x;
-    bool has_gt = true;
-    if (x.lo <= mb) {
-        double mono_hi = x.hi;
-        if (x.hi > mb) {
-            mono_hi = mb;
-            x_gt = Iv{ mb, x.hi };
-        } else {
-            has_gt = false;
-        }
-        y = Iv{ fmax(sinc_point(mono_hi).lo, -0.271724), fmin(sinc_point(x.lo).hi, 1.0) };
+// norm here), in three parts that may run on different lanes: the upper
+// end of the monotone piece (sinc at x.lo), its lower end (sinc at
+// min(x.hi, mb)) together with the enclosure beyond mb, and their
+// combination.
+#define RIGC_SINC_MB 4.4934094579
+RIGC_DI bool sinc_has_gt(Iv x) { return !(x.lo <= RIGC_SINC_MB) || x.hi > RIGC_SINC_MB; }
+RIGC_DI double sinc_part_lo(Iv x) { return x.lo <= RIGC_SINC_MB ? sinc_point(x.lo).hi : 0.0; }
+RIGC_DI double sinc_part_hi(Iv x, Iv& g) {
+    double v = 0.0;
+    if (x.lo <= RIGC_SINC_MB) v = sinc_point(x.hi > RIGC_SINC_MB ? RIGC_SINC_MB : x.hi).lo;
+    g = iv(0.0);
+    if (sinc_has_gt(x)) {
+        const Iv xg = (x.lo <= RIGC_SINC_MB) ? Iv{ RIGC_SINC_MB, x.hi } : x;
+        g = div_nonzero(sin_iv(xg), xg);
     }
-    if (has_gt) y = hull(y, div_nonzero(sin_iv(x_gt), x_gt));
+    return v;
+}
+RIGC_DI Iv sinc_combine(Iv x, double lo_part, double hi_part, Iv g) {
+    Iv y = iv_emptyval();
+    if (x.lo <= RIGC_SINC_MB) y = Iv{ fmax(hi_part, -0.271724), fmin(lo_part, 1.0) };
+    if (sinc_has_gt(x)) y = hull(y, g);
     return y;
+}
+RIGC_DI Iv sinc_iv(Iv x) {
+    Iv g;
+    const double lo = sinc_part_lo(x);
+    const double hi = sinc_part_hi(x, g);
+    return sinc_combine(x, lo, hi, g);
 }
 
 }  // namespace rigc
