@@ -560,13 +560,10 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
         if (e == hipSuccess && any_skip)
             e = hipMemcpyAsync(ctx->scene_skip.p, skip.data(), (size_t)ns, hipMemcpyHostToDevice, s);
         a.scene_skip = any_skip ? ctx->scene_skip.as<unsigned char>() : nullptr;
-        if (e == hipSuccess && nq) e = launch_ccd(a, grid, 256, s, ctx->ccdw);
         unsigned long long res[4] = { 0, 0, 0, 0 };
-        if (e == hipSuccess) e = hipMemcpyAsync(res, ctx->counters.p, sizeof(res), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(res_bits.data(), ctx->scene_best.p, sizeof(unsigned long long) * ns,
-                               hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        const CcdReadback rb[2] = { { res, ctx->counters.p, sizeof(res) },
+                                    { res_bits.data(), ctx->scene_best.p, sizeof(unsigned long long) * ns } };
+        if (e == hipSuccess) e = launch_ccd(a, grid, 256, s, ctx->ccdw, rb, 2);
         evals += (long long)res[2];
         launches += ctx->ccdw.passes;
         kernel_ms += ctx->ccdw.kernel_ms;

@@ -730,7 +730,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 }  // namespace
 
-hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& w) {
+hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& w, const CcdReadback* rb,
+                      int n_rb) {
     const long long nq = a.n_ee + a.n_fv;
     w.passes = 0;
     w.kernel_ms = 0.0f;
@@ -763,6 +764,11 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
         w.passes = pass + 1;
         unsigned long long n = 0;
         if ((e = hipMemcpyAsync(&n, a.n_tasks_out, sizeof(n), hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
+        // the caller's results ride on the same synchronisation; a further
+        // pass overwrites them
+        for (int i = 0; i < n_rb; i++)
+            if ((e = hipMemcpyAsync(rb[i].dst, rb[i].src, rb[i].bytes, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+                return e;
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
         float ms = 0.0f;
         if ((e = hipEventElapsedTime(&ms, w.k0, w.k1)) != hipSuccess) return e;
@@ -771,7 +777,7 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
         a.tasks_in = w.tasks[pass & 1].as<CcdTask>();
         a.n_tasks_in = (long long)std::min<unsigned long long>(n, (unsigned long long)cap);
     }
-    hipLaunchKernelGGL(ccd_finish_kernel, dim3(eb), dim3(256), 0, stream, a);
+    if (a.toi_out || a.hit_out) hipLaunchKernelGGL(ccd_finish_kernel, dim3(eb), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

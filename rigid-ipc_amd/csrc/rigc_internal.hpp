@@ -165,7 +165,15 @@ struct CcdWork {
 // Narrow phase: a first pass over the queries, then passes over exported
 // subtrees until none is left (a host read of the export count between
 // passes). a.counter is reset before every pass.
-hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& work);
+// Device-to-host copies a caller wants made when the narrow phase ends; they
+// are queued after every pass, ahead of that pass's synchronisation.
+struct CcdReadback {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& work,
+                      const CcdReadback* readback = nullptr, int n_readback = 0);
 
 // Exclusive prefix sum of n counts; work holds scan_workspace_entries(n)
 // entries and *total (device) receives the sum.
