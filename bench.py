@@ -188,6 +188,8 @@ def bench_sim(args):
             ccd_calls=int(st["ccd_calls"]), constraint_builds=int(st["constraint_builds"]),
             constraints_last=int(st["num_constraints"]), failed_solves=int(st["failed_solves"]),
             contact_ms_per_step=st["contact_ms"] / args.steps, host_ms_per_step=st["host_ms"] / args.steps,
+            contact_ms_by_call={k: round(st[k + "_ms"] / args.steps, 3)
+                                for k in ("ccd", "constraint", "barrier", "friction", "distance")},
             ccd_candidate_pairs_per_s=(st["ccd_candidates"] / (ccd_dev_ms * 1e-3)) if ccd_dev_ms > 0 else None,
             ccd_candidate_pairs_per_s_wall=(st["ccd_candidates"] / (st["ccd_ms"] * 1e-3)) if st["ccd_ms"] > 0 else None,
             ccd_candidates=int(st["ccd_candidates"]), step_ms=[round(1e3 * t, 3) for t in per_step]),

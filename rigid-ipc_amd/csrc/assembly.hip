@@ -846,17 +846,23 @@ hipError_t scene_ranges(Assembly& as, const DevScene& sc, const SceneSel& sel, s
                        as.cons.as<ConRec>(), cons_ranges(as), sc.body_scene, sel.scenes, sel.n_scenes,
                        as.ranges.as<long long>());
     AS_CHECK(hipGetLastError());
-    AS_CHECK(hipMemcpyAsync(h.data(), as.ranges.p, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, st));
-    return hipStreamSynchronize(st);
+    AS_CHECK(as.h_ranges.ensure(sizeof(long long) * h.size()));
+    AS_CHECK(hipMemcpyAsync(as.h_ranges.p, as.ranges.p, sizeof(long long) * h.size(), hipMemcpyDeviceToHost, st));
+    AS_CHECK(hipStreamSynchronize(st));
+    std::memcpy(h.data(), as.h_ranges.p, sizeof(long long) * h.size());
+    return hipSuccess;
 }
 
 hipError_t run_copies(Assembly& as, const std::vector<long long>& ent, const void* src0, const void* src1, void* dst,
                       hipStream_t st) {
     const long long n = (long long)ent.size() / 4;
     if (n == 0) return hipSuccess;
-    as.h_entries = ent;
+    // the table is staged in page-locked memory: the caller synchronises the
+    // stream before the next table is staged
+    AS_CHECK(as.h_entries.ensure(sizeof(long long) * ent.size()));
+    std::memcpy(as.h_entries.p, ent.data(), sizeof(long long) * ent.size());
     AS_CHECK(as.entries.ensure(sizeof(long long) * ent.size()));
-    AS_CHECK(hipMemcpyAsync(as.entries.p, as.h_entries.data(), sizeof(long long) * ent.size(), hipMemcpyHostToDevice,
+    AS_CHECK(hipMemcpyAsync(as.entries.p, as.h_entries.p, sizeof(long long) * ent.size(), hipMemcpyHostToDevice,
                             st));
     hipLaunchKernelGGL(copy_ranges_kernel, dim3((unsigned)n), dim3(THREADS), 0, st, as.entries.as<long long>(),
                        static_cast<const char*>(src0), static_cast<const char*>(src1), static_cast<char*>(dst));
@@ -912,9 +918,12 @@ hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc
     ca.count = as.ccount.as<unsigned int>();
     hipLaunchKernelGGL(classify_kernel, dim3(grid_for(nc)), dim3(THREADS), 0, st, ca);
     AS_CHECK(hipGetLastError());
-    unsigned int cnt[4];
-    AS_CHECK(hipMemcpyAsync(cnt, as.ccount.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    AS_CHECK(as.h_mail.ensure(sizeof(unsigned int) * MAIL_SLOTS));
+    unsigned int* mail = as.h_mail.as<unsigned int>();
+    AS_CHECK(hipMemcpyAsync(mail + MAIL_CLASS, as.ccount.p, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
     AS_CHECK(hipStreamSynchronize(st));
+    unsigned int cnt[4];
+    for (int k = 0; k < 4; k++) cnt[k] = mail[MAIL_CLASS + k];
 
     const unsigned long long key_max[4] = {
         (unsigned long long)sc.nv * (unsigned long long)sc.nv, (unsigned long long)sc.ne * (unsigned long long)sc.nv,
@@ -931,24 +940,29 @@ hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc
             AS_CHECK(radix_sort_u64(as.ckeys[k].as<unsigned long long>(), as.ctmp.as<unsigned long long>(),
                                     as.chist.as<unsigned int>(), cnt[k], bits_for(key_max[k]), st));
 
-    // run-length encode VV and EV (multiplicities); run starts kept per class
+    // run-length encode VV and EV (multiplicities); run starts kept per
+    // class; both run counts come back with one synchronisation
     long long nout[4] = { cnt[0], cnt[1], cnt[2], cnt[3] };
     DevBuf* starts[2] = { &as.cstart[0], &as.cstart[1] };
+    bool runs = false;
     for (int k = 0; k < 2; k++) {
         if (cnt[k] == 0) continue;
         unsigned int* flags = as.cflags.as<unsigned int>();
         unsigned int* pos = as.cpos.as<unsigned int>();
-        unsigned int* total = as.cwork.as<unsigned int>() + scan_workspace_entries(nmax);
+        unsigned int* total = as.cwork.as<unsigned int>() + scan_workspace_entries(nmax) + k;
         AS_CHECK(starts[k]->ensure(sizeof(unsigned int) * cnt[k]));
         hipLaunchKernelGGL(head_flags_kernel, dim3(grid_for(cnt[k])), dim3(THREADS), 0, st,
                            as.ckeys[k].as<unsigned long long>(), (long long)cnt[k], flags);
         exclusive_scan_u32(flags, pos, cnt[k], as.cwork.as<unsigned int>(), total, st);
         hipLaunchKernelGGL(head_starts_kernel, dim3(grid_for(cnt[k])), dim3(THREADS), 0, st, flags, pos,
                            (long long)cnt[k], starts[k]->as<unsigned int>());
-        unsigned int nu = 0;
-        AS_CHECK(hipMemcpyAsync(&nu, total, sizeof(nu), hipMemcpyDeviceToHost, st));
+        AS_CHECK(hipMemcpyAsync(mail + MAIL_RUNS + k, total, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        runs = true;
+    }
+    if (runs) {
         AS_CHECK(hipStreamSynchronize(st));
-        nout[k] = nu;
+        for (int k = 0; k < 2; k++)
+            if (cnt[k]) nout[k] = mail[MAIL_RUNS + k];
     }
     const long long n = nout[0] + nout[1] + nout[2] + nout[3];
     AS_CHECK(as.cons.ensure(sizeof(ConRec) * std::max<long long>(n, 1)));
@@ -1061,17 +1075,19 @@ hipError_t minimum_distance(Assembly& as, const DevScene& sc, const SceneSel& se
     if (as.n_con == 0) return hipSuccess;
     AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, false, st));
     AS_CHECK(as.mind.ensure(sizeof(unsigned long long) * sc.n_scenes));
-    const std::vector<unsigned long long> inf_bits(sc.n_scenes, 0x7ff0000000000000ULL);
-    AS_CHECK(hipMemcpyAsync(as.mind.p, inf_bits.data(), sizeof(unsigned long long) * sc.n_scenes,
-                            hipMemcpyHostToDevice, st));
+    // staging: [0, n_scenes) the +inf start values, [n_scenes, 2 n_scenes) the result
+    AS_CHECK(as.h_mind.ensure(2 * sizeof(unsigned long long) * sc.n_scenes));
+    unsigned long long* bits = as.h_mind.as<unsigned long long>();
+    for (int i = 0; i < sc.n_scenes; i++) bits[i] = 0x7ff0000000000000ULL;
+    AS_CHECK(hipMemcpyAsync(as.mind.p, bits, sizeof(unsigned long long) * sc.n_scenes, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(min_distance_kernel, dim3(grid_for(as.n_con)), dim3(THREADS), 0, st, as.cons.as<ConRec>(),
                        as.n_con, as.W.as<double>(), sc.body_scene, as.mind.as<unsigned long long>());
     AS_CHECK(hipGetLastError());
-    std::vector<unsigned long long> bits(sc.n_scenes);
-    AS_CHECK(hipMemcpyAsync(bits.data(), as.mind.p, sizeof(unsigned long long) * sc.n_scenes, hipMemcpyDeviceToHost,
-                            st));
+    AS_CHECK(hipMemcpyAsync(bits + sc.n_scenes, as.mind.p, sizeof(unsigned long long) * sc.n_scenes,
+                            hipMemcpyDeviceToHost, st));
     AS_CHECK(hipStreamSynchronize(st));
-    for (int i = 0; i < sel.n_scenes; i++) std::memcpy(&min_sq[i], &bits[sel.h_scenes[i]], sizeof(double));
+    for (int i = 0; i < sel.n_scenes; i++)
+        std::memcpy(&min_sq[i], &bits[sc.n_scenes + sel.h_scenes[i]], sizeof(double));
     return hipSuccess;
 }
 
@@ -1081,6 +1097,7 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& 
                               int num_cus, AssemblyOut& out, hipStream_t st) {
     out = AssemblyOut{};
     out.energy.assign(sel.n_scenes, 0.0);
+    out.hess = want_hess;
     as.n_off = 0;
     const int nb = sc.nb;
     want_grad = want_grad || want_hess;
@@ -1112,8 +1129,11 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& 
     if (n == 0) {
         if (want_grad) AS_CHECK(hipMemsetAsync(as.grad.p, 0, 6 * sizeof(double) * nb, st));
         if (want_hess) AS_CHECK(hipMemsetAsync(as.diag.p, 0, 36 * sizeof(double) * nb, st));
-        return hipStreamSynchronize(st);
+        return hipGetLastError();
     }
+    out.n_rec = n;
+    AS_CHECK(as.h_mail.ensure(sizeof(unsigned int) * MAIL_SLOTS));
+    AS_CHECK(as.h_esum.ensure(sizeof(double) * std::max(sel.n_scenes, 1)));
     AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, want_grad, st));
     if (friction) AS_CHECK(world_vertices(as, sc, sel, d_poses_t0, as.W0, false, st));  // frames stay at d_poses
     AS_CHECK(as.energy.ensure(sizeof(double) * n));
@@ -1201,17 +1221,22 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& 
                            n, idx_bits, nb, as.oflags.as<unsigned int>(), as.opos.as<unsigned int>(), cons, fric,
                            as.lhess.as<double>(), as.opairs.as<int>(), as.oblocks.as<double>());
         AS_CHECK(hipGetLastError());
-        unsigned int np = 0;
-        AS_CHECK(hipMemcpyAsync(&np, total, sizeof(np), hipMemcpyDeviceToHost, st));
-        AS_CHECK(hipMemcpyAsync(out.energy.data(), as.esum.p, sizeof(double) * sel.n_scenes, hipMemcpyDeviceToHost,
-                                st));
-        AS_CHECK(hipStreamSynchronize(st));
-        as.n_off = np;
-        out.n_off = np;
-        return hipSuccess;
+        AS_CHECK(hipMemcpyAsync(as.h_mail.as<unsigned int>() + MAIL_OFF, total, sizeof(unsigned int),
+                                hipMemcpyDeviceToHost, st));
     }
-    AS_CHECK(hipMemcpyAsync(out.energy.data(), as.esum.p, sizeof(double) * sel.n_scenes, hipMemcpyDeviceToHost, st));
-    return hipStreamSynchronize(st);
+    AS_CHECK(hipMemcpyAsync(as.h_esum.p, as.esum.p, sizeof(double) * sel.n_scenes, hipMemcpyDeviceToHost, st));
+    out.pending = true;
+    return hipGetLastError();
+}
+
+void finish_potential(Assembly& as, AssemblyOut& out) {
+    if (!out.pending) return;
+    std::memcpy(out.energy.data(), as.h_esum.p, sizeof(double) * out.energy.size());
+    if (out.hess) {
+        out.n_off = as.h_mail.as<unsigned int>()[MAIL_OFF];
+        as.n_off = out.n_off;
+    }
+    out.pending = false;
 }
 
 }  // namespace rigc

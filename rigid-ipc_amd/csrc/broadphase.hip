@@ -439,60 +439,66 @@ __global__ __launch_bounds__(BP_THREADS) void element_pair_kernel(
     const int* pairs, const int* bvs, const int* bes, const int* bfs, const int* group, const double* bhull,
     const double* vc, const double* vh, const double* ec, const double* eh, const double* fc, const double* fh,
     long long nv, long long ne, unsigned long long* out_ee, long long cap_ee, unsigned long long* out_fv,
-    long long cap_fv, unsigned long long* counters) {
+    long long cap_fv, unsigned long long* counters, const unsigned long long* n_pairs, long long cap_pairs) {
     __shared__ int l_ae[LIST_CAP], l_af[LIST_CAP], l_av[LIST_CAP], l_be[LIST_CAP], l_bf[LIST_CAP], l_bv[LIST_CAP];
     __shared__ int n_ae, n_af, n_av, n_be, n_bf, n_bv;
-    const int A = pairs[2 * blockIdx.x], B = pairs[2 * blockIdx.x + 1];
-    const double* hA = bhull + 6 * A;
-    const double* hB = bhull + 6 * B;
-    const double eps = fmax(hull_eps(hA), hull_eps(hB));
-    if (threadIdx.x == 0) n_ae = n_af = n_av = n_be = n_bf = n_bv = 0;
-    __syncthreads();
-    build_list(l_ae, &n_ae, bes[A], bes[A + 1], ec, eh, hB, eps);
-    build_list(l_af, &n_af, bfs[A], bfs[A + 1], fc, fh, hB, eps);
-    build_list(l_av, &n_av, bvs[A], bvs[A + 1], vc, vh, hB, eps);
-    build_list(l_be, &n_be, bes[B], bes[B + 1], ec, eh, hA, eps);
-    build_list(l_bf, &n_bf, bfs[B], bfs[B + 1], fc, fh, hA, eps);
-    build_list(l_bv, &n_bv, bvs[B], bvs[B + 1], vc, vh, hA, eps);
-    __syncthreads();
-    const ElemList ae{ l_ae, n_ae, bes[A], bes[A + 1] - bes[A], n_ae > LIST_CAP };
-    const ElemList af{ l_af, n_af, bfs[A], bfs[A + 1] - bfs[A], n_af > LIST_CAP };
-    const ElemList av{ l_av, n_av, bvs[A], bvs[A + 1] - bvs[A], n_av > LIST_CAP };
-    const ElemList be{ l_be, n_be, bes[B], bes[B + 1] - bes[B], n_be > LIST_CAP };
-    const ElemList bf{ l_bf, n_bf, bfs[B], bfs[B + 1] - bfs[B], n_bf > LIST_CAP };
-    const ElemList bv{ l_bv, n_bv, bvs[B], bvs[B + 1] - bvs[B], n_bv > LIST_CAP };
-    auto len = [](const ElemList& l) { return l.full ? l.count : l.n; };
+    // body pairs in grid stride; their number is read on the device, so no
+    // host round trip separates the two pair kernels
+    const long long np = (long long)min(*n_pairs, (unsigned long long)cap_pairs);
+    for (long long pi = blockIdx.x; pi < np; pi += gridDim.x) {
+        __syncthreads();  // the previous pair's lists are no longer read
+        const int A = pairs[2 * pi], B = pairs[2 * pi + 1];
+        const double* hA = bhull + 6 * A;
+        const double* hB = bhull + 6 * B;
+        const double eps = fmax(hull_eps(hA), hull_eps(hB));
+        if (threadIdx.x == 0) n_ae = n_af = n_av = n_be = n_bf = n_bv = 0;
+        __syncthreads();
+        build_list(l_ae, &n_ae, bes[A], bes[A + 1], ec, eh, hB, eps);
+        build_list(l_af, &n_af, bfs[A], bfs[A + 1], fc, fh, hB, eps);
+        build_list(l_av, &n_av, bvs[A], bvs[A + 1], vc, vh, hB, eps);
+        build_list(l_be, &n_be, bes[B], bes[B + 1], ec, eh, hA, eps);
+        build_list(l_bf, &n_bf, bfs[B], bfs[B + 1], fc, fh, hA, eps);
+        build_list(l_bv, &n_bv, bvs[B], bvs[B + 1], vc, vh, hA, eps);
+        __syncthreads();
+        const ElemList ae{ l_ae, n_ae, bes[A], bes[A + 1] - bes[A], n_ae > LIST_CAP };
+        const ElemList af{ l_af, n_af, bfs[A], bfs[A + 1] - bfs[A], n_af > LIST_CAP };
+        const ElemList av{ l_av, n_av, bvs[A], bvs[A + 1] - bvs[A], n_av > LIST_CAP };
+        const ElemList be{ l_be, n_be, bes[B], bes[B + 1] - bes[B], n_be > LIST_CAP };
+        const ElemList bf{ l_bf, n_bf, bfs[B], bfs[B + 1] - bfs[B], n_bf > LIST_CAP };
+        const ElemList bv{ l_bv, n_bv, bvs[B], bvs[B + 1] - bvs[B], n_bv > LIST_CAP };
+        auto len = [](const ElemList& l) { return l.full ? l.count : l.n; };
 
-    // edge-edge: edges of A (lower ids) against edges of B
-    {
-        const long long na = len(ae), nbb = len(be), tot = na * nbb;
-        for (long long k0 = 0; k0 < tot; k0 += BP_THREADS) {
-            const long long k = k0 + threadIdx.x;
-            bool hit = false;
-            unsigned long long key = 0;
-            if (k < tot) {
-                const int ea = ae.get((int)(k / nbb)), eb = be.get((int)(k % nbb));
-                hit = aabb_overlap(ec + 3 * ea, eh + 3 * ea, ec + 3 * eb, eh + 3 * eb);
-                key = (unsigned long long)ea * (unsigned long long)ne + (unsigned long long)eb;
+        // edge-edge: edges of A (lower ids) against edges of B
+        {
+            const long long na = len(ae), nbb = len(be), tot = na * nbb;
+            for (long long k0 = 0; k0 < tot; k0 += BP_THREADS) {
+                const long long k = k0 + threadIdx.x;
+                bool hit = false;
+                unsigned long long key = 0;
+                if (k < tot) {
+                    const int ea = ae.get((int)(k / nbb)), eb = be.get((int)(k % nbb));
+                    hit = aabb_overlap(ec + 3 * ea, eh + 3 * ea, ec + 3 * eb, eh + 3 * eb);
+                    key = (unsigned long long)ea * (unsigned long long)ne + (unsigned long long)eb;
+                }
+                emit(hit, key, out_ee, cap_ee, counters + 0);
             }
-            emit(hit, key, out_ee, cap_ee, counters + 0);
         }
-    }
-    // face-vertex: faces of A with vertices of B, and faces of B with vertices of A
-    for (int dir = 0; dir < 2; dir++) {
-        const ElemList& lf = dir == 0 ? af : bf;
-        const ElemList& lv = dir == 0 ? bv : av;
-        const long long nfl = len(lf), nvl = len(lv), tot = nfl * nvl;
-        for (long long k0 = 0; k0 < tot; k0 += BP_THREADS) {
-            const long long k = k0 + threadIdx.x;
-            bool hit = false;
-            unsigned long long key = 0;
-            if (k < tot) {
-                const int f = lf.get((int)(k / nvl)), v = lv.get((int)(k % nvl));
-                hit = aabb_overlap(fc + 3 * f, fh + 3 * f, vc + 3 * v, vh + 3 * v);
-                key = (unsigned long long)f * (unsigned long long)nv + (unsigned long long)v;
+        // face-vertex: faces of A with vertices of B, and faces of B with vertices of A
+        for (int dir = 0; dir < 2; dir++) {
+            const ElemList& lf = dir == 0 ? af : bf;
+            const ElemList& lv = dir == 0 ? bv : av;
+            const long long nfl = len(lf), nvl = len(lv), tot = nfl * nvl;
+            for (long long k0 = 0; k0 < tot; k0 += BP_THREADS) {
+                const long long k = k0 + threadIdx.x;
+                bool hit = false;
+                unsigned long long key = 0;
+                if (k < tot) {
+                    const int f = lf.get((int)(k / nvl)), v = lv.get((int)(k % nvl));
+                    hit = aabb_overlap(fc + 3 * f, fh + 3 * f, vc + 3 * v, vh + 3 * v);
+                    key = (unsigned long long)f * (unsigned long long)nv + (unsigned long long)v;
+                }
+                emit(hit, key, out_fv, cap_fv, counters + 1);
             }
-            emit(hit, key, out_fv, cap_fv, counters + 1);
         }
     }
 }
@@ -565,42 +571,41 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& s
                        bp.eh.as<double>(), bp.fc.as<double>(), bp.fh.as<double>());
     BP_CHECK(hipGetLastError());
 
-    // body pairs (retry once with the exact size if the first guess is short)
-    long long cap_pairs = std::max<long long>(1024, 32LL * sel.n_bodies);
-    for (int attempt = 0; attempt < 2; attempt++) {
-        BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * cap_pairs));
-        BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(body_pair_kernel, dim3(ntiles), dim3(PAIR_TILE), 0, st, sel.tiles, bp.inc.as<int>(),
-                           sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt);
-        unsigned long long n = 0;
-        BP_CHECK(hipMemcpyAsync(&n, cnt, sizeof(n), hipMemcpyDeviceToHost, st));
-        BP_CHECK(hipStreamSynchronize(st));
-        bp.n_body_pairs = (long long)n;
-        if ((long long)n <= cap_pairs) break;
-        cap_pairs = (long long)n;
-    }
-    if (bp.n_body_pairs == 0) return hipSuccess;
-
-    // element pairs
+    // Body pairs, then element pairs of every body pair, with one read-back
+    // of the three counts ([0] edge-edge, [1] face-vertex, [2] body pairs).
+    // A pass whose body pairs overflow sees only part of them, so up to two
+    // more passes, each with capacities of at least the counts seen.
+    BP_CHECK(bp.h_counters.ensure(4 * sizeof(unsigned long long)));
+    const unsigned long long* got = bp.h_counters.as<unsigned long long>();
+    long long cap_pairs = std::max<long long>(std::max<long long>(1024, 32LL * sel.n_bodies),
+                                              (long long)(bp.pairs.bytes / (2 * sizeof(int))));
     long long cap_ee = std::max<long long>(bp.keys_ee.bytes / 8, 1 << 16);
     long long cap_fv = std::max<long long>(bp.keys_fv.bytes / 8, 1 << 16);
-    unsigned long long got[2] = { 0, 0 };
-    for (int attempt = 0; attempt < 2; attempt++) {
+    bool fits = false;
+    for (int attempt = 0; attempt < 3 && !fits; attempt++) {
+        BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * cap_pairs));
         BP_CHECK(bp.keys_ee.ensure(8 * cap_ee));
         BP_CHECK(bp.keys_fv.ensure(8 * cap_fv));
-        BP_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(element_pair_kernel, dim3((unsigned)bp.n_body_pairs), dim3(BP_THREADS), 0, st,
-                           bp.pairs.as<int>(), sc.body_vstart, sc.body_estart, sc.body_fstart, sc.group,
-                           bp.bhull.as<double>(), bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(),
-                           bp.eh.as<double>(), bp.fc.as<double>(), bp.fh.as<double>(), (long long)sc.nv,
-                           (long long)sc.ne, bp.keys_ee.as<unsigned long long>(), cap_ee,
-                           bp.keys_fv.as<unsigned long long>(), cap_fv, cnt);
-        BP_CHECK(hipMemcpyAsync(got, cnt, sizeof(got), hipMemcpyDeviceToHost, st));
+        BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(body_pair_kernel, dim3(ntiles), dim3(PAIR_TILE), 0, st, sel.tiles, bp.inc.as<int>(),
+                           sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt + 2);
+        const unsigned grid = (unsigned)std::min<long long>(cap_pairs, 2048);
+        hipLaunchKernelGGL(element_pair_kernel, dim3(grid), dim3(BP_THREADS), 0, st, bp.pairs.as<int>(),
+                           sc.body_vstart, sc.body_estart, sc.body_fstart, sc.group, bp.bhull.as<double>(),
+                           bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(), bp.eh.as<double>(),
+                           bp.fc.as<double>(), bp.fh.as<double>(), (long long)sc.nv, (long long)sc.ne,
+                           bp.keys_ee.as<unsigned long long>(), cap_ee, bp.keys_fv.as<unsigned long long>(), cap_fv,
+                           cnt, cnt + 2, cap_pairs);
+        BP_CHECK(hipGetLastError());
+        BP_CHECK(hipMemcpyAsync(bp.h_counters.p, cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         BP_CHECK(hipStreamSynchronize(st));
-        if ((long long)got[0] <= cap_ee && (long long)got[1] <= cap_fv) break;
+        fits = (long long)got[2] <= cap_pairs && (long long)got[0] <= cap_ee && (long long)got[1] <= cap_fv;
+        cap_pairs = std::max<long long>(cap_pairs, (long long)got[2]);
         cap_ee = std::max<long long>(cap_ee, (long long)got[0]);
         cap_fv = std::max<long long>(cap_fv, (long long)got[1]);
     }
+    if (!fits) return hipErrorUnknown;
+    bp.n_body_pairs = (long long)got[2];
     bp.n_ee = (long long)got[0];
     bp.n_fv = (long long)got[1];
 

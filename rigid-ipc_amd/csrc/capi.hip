@@ -20,7 +20,7 @@ struct SelState {
     DevBuf buf;
     SceneSel sel;
     std::vector<int> req_start;  // [n_sel + 1] first body of each selected scene in request order
-    std::vector<char> host;      // staging of the last upload (kept alive for the asynchronous copy)
+    HostBuf host;                // page-locked staging of the last upload
 };
 
 struct rigc_ctx {
@@ -39,6 +39,12 @@ struct rigc_ctx {
     // per-call buffers
     DevBuf poses0, poses1, stage0, stage1, out_rows, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
     DevBuf scene_best, scene_skip, scene_par;
+    // page-locked staging: poses (uploads), per-scene parameters, the
+    // earliest-TOI bounds and results, and potential outputs; a staging
+    // buffer is rewritten only after ev_stage, recorded behind the last
+    // upload from it, has completed
+    HostBuf h_pose0, h_pose1, h_par, h_toi, h_out;
+    hipEvent_t ev_stage = nullptr;
     SelState sel_all, sel_req;
     BroadPhase bp;
     Assembly as;
@@ -84,6 +90,10 @@ void free_scene(rigc_ctx* ctx) {
     ctx->has_scene = false;
 }
 
+// Waits until the uploads queued from the page-locked staging buffers have
+// been made (normally long done: every entry point synchronises its stream).
+hipError_t stage_wait(rigc_ctx* ctx) { return hipEventSynchronize(ctx->ev_stage); }
+
 // Builds and uploads the selection of scenes ids[0..n) (strictly ascending).
 hipError_t make_sel(rigc_ctx* ctx, SelState& S, const int* ids, int n) {
     const DevScene& d = ctx->scene;
@@ -102,14 +112,18 @@ hipError_t make_sel(rigc_ctx* ctx, SelState& S, const int* ids, int n) {
     }
     // layout: tiles (16-byte aligned) | scenes | bodies
     const size_t tb = sizeof(int4) * tiles.size(), sbytes = sizeof(int) * scenes.size();
-    std::vector<char>& host = S.host;
-    host.assign(tb + sbytes + sizeof(int) * bodies.size() + 16, 0);
-    if (!tiles.empty()) std::memcpy(host.data(), tiles.data(), tb);
-    if (!scenes.empty()) std::memcpy(host.data() + tb, scenes.data(), sbytes);
-    if (!bodies.empty()) std::memcpy(host.data() + tb + sbytes, bodies.data(), sizeof(int) * bodies.size());
-    hipError_t e = S.buf.ensure(host.size());
+    const size_t bytes = tb + sbytes + sizeof(int) * bodies.size() + 16;
+    hipError_t e = stage_wait(ctx);
+    if (e == hipSuccess) e = S.host.ensure(bytes);
+    if (e == hipSuccess) e = S.buf.ensure(bytes);
     if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(S.buf.p, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream);
+    char* host = S.host.as<char>();
+    std::memset(host, 0, bytes);
+    if (!tiles.empty()) std::memcpy(host, tiles.data(), tb);
+    if (!scenes.empty()) std::memcpy(host + tb, scenes.data(), sbytes);
+    if (!bodies.empty()) std::memcpy(host + tb + sbytes, bodies.data(), sizeof(int) * bodies.size());
+    e = hipMemcpyAsync(S.buf.p, host, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stage, ctx->stream);
     if (e != hipSuccess) return e;
     char* base = S.buf.as<char>();
     S.sel.n_scenes = n;
@@ -253,6 +267,8 @@ rigc_status rigc_create(rigc_ctx** out, int32_t device) {
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev2);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_stage);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev_stage, ctx->stream);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
@@ -279,6 +295,10 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+    if (ctx->ev_stage) (void)hipEventDestroy(ctx->ev_stage);
+    for (HostBuf* b : { &ctx->h_pose0, &ctx->h_pose1, &ctx->h_par, &ctx->h_toi, &ctx->h_out, &ctx->sel_all.host,
+                        &ctx->sel_req.host, &ctx->bp.h_counters })
+        b->release();
     for (DevBuf* b : { &ctx->bp.box_raw, &ctx->bp.box_inf, &ctx->bp.inc, &ctx->bp.dom, &ctx->bp.vmin, &ctx->bp.vmax,
                        &ctx->bp.bhull, &ctx->bp.vc, &ctx->bp.vh, &ctx->bp.ec, &ctx->bp.eh, &ctx->bp.fc, &ctx->bp.fh,
                        &ctx->bp.pairs, &ctx->bp.keys_ee, &ctx->bp.keys_fv, &ctx->bp.tmp, &ctx->bp.hist,
@@ -433,13 +453,28 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
 namespace {
 
 // Poses of every body of the context (scene order).
+// Copies n_rows pose rows of p0 (and p1) through the page-locked staging
+// into dst0 (dst1).
+rigc_status stage_poses(rigc_ctx* ctx, const double* p0, const double* p1, size_t n_rows, void* dst0, void* dst1) {
+    const size_t bytes = 6 * sizeof(double) * n_rows;
+    RIGC_HIP(stage_wait(ctx));
+    RIGC_HIP(ctx->h_pose0.ensure(bytes));
+    std::memcpy(ctx->h_pose0.p, p0, bytes);
+    RIGC_HIP(hipMemcpyAsync(dst0, ctx->h_pose0.p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (p1) {
+        RIGC_HIP(ctx->h_pose1.ensure(bytes));
+        std::memcpy(ctx->h_pose1.p, p1, bytes);
+        RIGC_HIP(hipMemcpyAsync(dst1, ctx->h_pose1.p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    RIGC_HIP(hipEventRecord(ctx->ev_stage, ctx->stream));
+    return RIGC_OK;
+}
+
 rigc_status upload_poses(rigc_ctx* ctx, const double* p0, const double* p1) {
     const size_t pose_bytes = 6 * sizeof(double) * (size_t)ctx->scene.nb;
     RIGC_HIP(ctx->poses0.ensure(pose_bytes));
     RIGC_HIP(ctx->poses1.ensure(pose_bytes));
-    RIGC_HIP(hipMemcpyAsync(ctx->poses0.p, p0, pose_bytes, hipMemcpyHostToDevice, ctx->stream));
-    if (p1) RIGC_HIP(hipMemcpyAsync(ctx->poses1.p, p1, pose_bytes, hipMemcpyHostToDevice, ctx->stream));
-    return RIGC_OK;
+    return stage_poses(ctx, p0, p1, (size_t)ctx->scene.nb, ctx->poses0.p, ctx->poses1.p);
 }
 
 // Poses of the selected scenes' bodies, in request order, into the
@@ -452,12 +487,10 @@ rigc_status upload_sel_poses(rigc_ctx* ctx, const SelState& S, const double* p0,
     RIGC_HIP(ctx->stage0.ensure(6 * sizeof(double) * std::max<size_t>(ns, 1)));
     RIGC_HIP(ctx->stage1.ensure(6 * sizeof(double) * std::max<size_t>(ns, 1)));
     hipStream_t st = ctx->stream;
-    RIGC_HIP(hipMemcpyAsync(ctx->stage0.p, p0, 6 * sizeof(double) * ns, hipMemcpyHostToDevice, st));
+    const rigc_status s = stage_poses(ctx, p0, p1, ns, ctx->stage0.p, ctx->stage1.p);
+    if (s != RIGC_OK) return s;
     scatter_rows(S.sel.bodies, (int)ns, 6, ctx->stage0.as<double>(), ctx->poses0.as<double>(), st);
-    if (p1) {
-        RIGC_HIP(hipMemcpyAsync(ctx->stage1.p, p1, 6 * sizeof(double) * ns, hipMemcpyHostToDevice, st));
-        scatter_rows(S.sel.bodies, (int)ns, 6, ctx->stage1.as<double>(), ctx->poses1.as<double>(), st);
-    }
+    if (p1) scatter_rows(S.sel.bodies, (int)ns, 6, ctx->stage1.as<double>(), ctx->poses1.as<double>(), st);
     RIGC_HIP(hipGetLastError());
     return RIGC_OK;
 }
@@ -540,8 +573,16 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
     out.assign(sel.n_scenes, inf);
     long long evals = 0, launches = 0;
     double kernel_ms = 0.0;
-    std::vector<unsigned long long> best_bits(ns, 0x7ff0000000000000ULL), res_bits(ns);
-    std::vector<unsigned char> skip(ns, 0);
+    // page-locked staging: per-scene start bounds and results (bits),
+    // the four counters read back, the skip flags
+    RIGC_HIP(stage_wait(ctx));
+    RIGC_HIP(ctx->h_toi.ensure(sizeof(unsigned long long) * (2 * (size_t)ns + 4) + (size_t)ns));
+    unsigned long long* best_bits = ctx->h_toi.as<unsigned long long>();
+    unsigned long long* res_bits = best_bits + ns;
+    unsigned long long* res = res_bits + ns;
+    unsigned char* skip = reinterpret_cast<unsigned char*>(res + 4);
+    for (int k = 0; k < ns; k++) best_bits[k] = 0x7ff0000000000000ULL;
+    std::memset(skip, 0, (size_t)ns);
     // bound[i]: the starting minimum of selected scene i (NaN: skip it)
     auto run = [&](const std::vector<double>& bound) -> hipError_t {
         bool any_skip = false;
@@ -552,17 +593,18 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
             const double b = skip[k] ? 0.0 : bound[i];
             std::memcpy(&best_bits[k], &b, sizeof(double));
         }
-        const unsigned long long init[4] = { 0, 0, 0, 0 };
-        hipError_t e = hipMemcpyAsync(ctx->counters.p, init, sizeof(init), hipMemcpyHostToDevice, s);
+        // the staging is rewritten only after the previous run's
+        // synchronisation (launch_ccd synchronises after every pass)
+        hipError_t e = hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), s);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(ctx->scene_best.p, best_bits.data(), sizeof(unsigned long long) * ns,
-                               hipMemcpyHostToDevice, s);
+            e = hipMemcpyAsync(ctx->scene_best.p, best_bits, sizeof(unsigned long long) * ns, hipMemcpyHostToDevice,
+                               s);
         if (e == hipSuccess && any_skip)
-            e = hipMemcpyAsync(ctx->scene_skip.p, skip.data(), (size_t)ns, hipMemcpyHostToDevice, s);
+            e = hipMemcpyAsync(ctx->scene_skip.p, skip, (size_t)ns, hipMemcpyHostToDevice, s);
         a.scene_skip = any_skip ? ctx->scene_skip.as<unsigned char>() : nullptr;
-        unsigned long long res[4] = { 0, 0, 0, 0 };
-        const CcdReadback rb[2] = { { res, ctx->counters.p, sizeof(res) },
-                                    { res_bits.data(), ctx->scene_best.p, sizeof(unsigned long long) * ns } };
+        for (int k = 0; k < 4; k++) res[k] = 0;
+        const CcdReadback rb[2] = { { res, ctx->counters.p, 4 * sizeof(unsigned long long) },
+                                    { res_bits, ctx->scene_best.p, sizeof(unsigned long long) * ns } };
         if (e == hipSuccess) e = launch_ccd(a, grid, 256, s, ctx->ccdw, rb, 2);
         evals += (long long)res[2];
         launches += ctx->ccdw.passes;
@@ -609,53 +651,64 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
     return RIGC_OK;
 }
 
-rigc_status copy_potential(rigc_ctx* ctx, const SelState& S, const AssemblyOut& out, int32_t flags, double* grad,
+// Queues the potential's outputs (rows of the selected bodies, in request
+// order) into the page-locked staging, synchronises once, finishes `out`
+// and copies to the caller's arrays. The off-diagonal blocks are staged up
+// to min(records, capacity): their number is known only after the
+// synchronisation and never exceeds the number of records.
+rigc_status copy_potential(rigc_ctx* ctx, const SelState& S, AssemblyOut& out, int32_t flags, double* grad,
                            rigc_block_hessian* hess) {
     const DevScene& d = ctx->scene;
     hipStream_t st = ctx->stream;
-    const bool want_hess = (flags & RIGC_HESSIAN) != 0;
-    const bool want_grad = want_hess || (flags & RIGC_GRADIENT) != 0;
-    const int nsb = S.sel.n_bodies;
-    if (!S.sel.all && (want_grad || want_hess)) {
-        RIGC_HIP(ctx->out_rows.ensure(36 * sizeof(double) * std::max(nsb, 1)));
-    }
-    if (want_grad && grad) {
-        if (S.sel.all) {
-            RIGC_HIP(hipMemcpyAsync(grad, ctx->as.grad.p, 6 * sizeof(double) * d.nb, hipMemcpyDeviceToHost, st));
-        } else {
-            gather_rows(S.sel.bodies, nsb, 6, ctx->as.grad.as<double>(), ctx->out_rows.as<double>(), st);
-            RIGC_HIP(hipMemcpyAsync(grad, ctx->out_rows.p, 6 * sizeof(double) * nsb, hipMemcpyDeviceToHost, st));
-            RIGC_HIP(hipStreamSynchronize(st));
+    const bool want_hess = (flags & RIGC_HESSIAN) != 0 && hess;
+    const bool want_grad = (flags & (RIGC_HESSIAN | RIGC_GRADIENT)) != 0 && grad;
+    const size_t rows = S.sel.all ? (size_t)d.nb : (size_t)S.sel.n_bodies;
+    const long long m = want_hess ? std::min<long long>(out.n_rec, std::max<long long>(hess->capacity, 0)) : 0;
+    const size_t o_diag = 6 * sizeof(double) * rows, o_off = o_diag + 36 * sizeof(double) * rows;
+    const size_t o_pairs = o_off + 36 * sizeof(double) * (size_t)m;
+    RIGC_HIP(ctx->h_out.ensure(o_pairs + 2 * sizeof(int32_t) * (size_t)m));
+    char* h = ctx->h_out.as<char>();
+    if (!S.sel.all && (want_grad || want_hess))
+        RIGC_HIP(ctx->out_rows.ensure(36 * sizeof(double) * std::max<size_t>(rows, 1)));
+    // the two row gathers share out_rows; the stream orders each copy
+    // before the next gather
+    auto rows_out = [&](const DevBuf& src, int width, size_t off) -> hipError_t {
+        const void* from = src.p;
+        if (!S.sel.all) {
+            gather_rows(S.sel.bodies, (int)rows, width, src.as<double>(), ctx->out_rows.as<double>(), st);
+            from = ctx->out_rows.p;
         }
+        return hipMemcpyAsync(h + off, from, width * sizeof(double) * rows, hipMemcpyDeviceToHost, st);
+    };
+    if (want_grad && rows) RIGC_HIP(rows_out(ctx->as.grad, 6, 0));
+    if (want_hess && rows) RIGC_HIP(rows_out(ctx->as.diag, 36, o_diag));
+    if (m > 0) {
+        RIGC_HIP(hipMemcpyAsync(h + o_off, ctx->as.oblocks.p, 36 * sizeof(double) * (size_t)m, hipMemcpyDeviceToHost,
+                                st));
+        RIGC_HIP(hipMemcpyAsync(h + o_pairs, ctx->as.opairs.p, 2 * sizeof(int32_t) * (size_t)m,
+                                hipMemcpyDeviceToHost, st));
     }
+    RIGC_HIP(hipStreamSynchronize(st));
+    finish_potential(ctx->as, out);
+    if (want_grad) std::memcpy(grad, h, 6 * sizeof(double) * rows);
     bool fits = true;
-    if (want_hess && hess) {
-        if (S.sel.all) {
-            RIGC_HIP(hipMemcpyAsync(hess->diag, ctx->as.diag.p, 36 * sizeof(double) * d.nb, hipMemcpyDeviceToHost,
-                                    st));
-        } else {
-            gather_rows(S.sel.bodies, nsb, 36, ctx->as.diag.as<double>(), ctx->out_rows.as<double>(), st);
-            RIGC_HIP(hipMemcpyAsync(hess->diag, ctx->out_rows.p, 36 * sizeof(double) * nsb, hipMemcpyDeviceToHost,
-                                    st));
-        }
+    if (want_hess) {
+        std::memcpy(hess->diag, h + o_diag, 36 * sizeof(double) * rows);
         hess->count = out.n_off;
         fits = out.n_off <= hess->capacity;
         if (fits && out.n_off) {
-            RIGC_HIP(hipMemcpyAsync(hess->pairs, ctx->as.opairs.p, 2 * sizeof(int32_t) * out.n_off,
-                                    hipMemcpyDeviceToHost, st));
-            RIGC_HIP(hipMemcpyAsync(hess->off, ctx->as.oblocks.p, 36 * sizeof(double) * out.n_off,
-                                    hipMemcpyDeviceToHost, st));
-        }
-    }
-    RIGC_HIP(hipStreamSynchronize(st));
-    if (fits && want_hess && hess && !S.sel.all && out.n_off) {
-        // scene-order body ids -> request order
-        std::vector<int> first(d.n_scenes, -1);
-        for (int i = 0; i < S.sel.n_scenes; i++) first[S.sel.h_scenes[i]] = S.req_start[i];
-        for (long long k = 0; k < 2 * out.n_off; k++) {
-            const int b = hess->pairs[k];
-            const int sidx = ctx->h_body_scene[b];
-            hess->pairs[k] = first[sidx] + (b - d.h_scene_body[sidx]);
+            std::memcpy(hess->off, h + o_off, 36 * sizeof(double) * (size_t)out.n_off);
+            std::memcpy(hess->pairs, h + o_pairs, 2 * sizeof(int32_t) * (size_t)out.n_off);
+            if (!S.sel.all) {
+                // scene-order body ids -> request order
+                std::vector<int> first(d.n_scenes, -1);
+                for (int i = 0; i < S.sel.n_scenes; i++) first[S.sel.h_scenes[i]] = S.req_start[i];
+                for (long long k = 0; k < 2 * out.n_off; k++) {
+                    const int b = hess->pairs[k];
+                    const int sidx = ctx->h_body_scene[b];
+                    hess->pairs[k] = first[sidx] + (b - d.h_scene_body[sidx]);
+                }
+            }
         }
     }
     if (!fits) return ctx->fail(RIGC_ERR_CAPACITY, "off-diagonal Hessian capacity too small");
@@ -685,11 +738,15 @@ rigc_status build_sel(rigc_ctx* ctx, const SelState& S, double dhat, double dmin
 
 // Uploads per-scene parameters (rows 0..2 of scene_par: kappa, mu, epsv_h).
 rigc_status upload_scene_par(rigc_ctx* ctx, int row, const SelState& S, const double* vals) {
-    std::vector<double> h(ctx->scene.n_scenes, 0.0);
+    const size_t ns = (size_t)ctx->scene.n_scenes;
+    RIGC_HIP(stage_wait(ctx));
+    RIGC_HIP(ctx->h_par.ensure(3 * sizeof(double) * ns));
+    double* h = ctx->h_par.as<double>() + (size_t)row * ns;
+    for (size_t k = 0; k < ns; k++) h[k] = 0.0;
     for (int i = 0; i < S.sel.n_scenes; i++) h[S.sel.h_scenes[i]] = vals[i];
-    RIGC_HIP(hipMemcpyAsync(ctx->scene_par.as<double>() + (size_t)row * ctx->scene.n_scenes, h.data(),
-                            sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
-    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    RIGC_HIP(hipMemcpyAsync(ctx->scene_par.as<double>() + (size_t)row * ns, h, sizeof(double) * ns,
+                            hipMemcpyHostToDevice, ctx->stream));
+    RIGC_HIP(hipEventRecord(ctx->ev_stage, ctx->stream));
     return RIGC_OK;
 }
 
@@ -811,8 +868,8 @@ rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double
                                 dhat, dmin, 0.0, nullptr, (flags & RIGC_GRADIENT) != 0,
                                 (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
-    *energy = total(out.energy);
     s = copy_potential(ctx, ctx->sel_all, out, flags, grad, hess);
+    if (!out.pending) *energy = total(out.energy);
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;
@@ -909,8 +966,8 @@ rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const
                                 ctx->poses1.as<double>(), 0.0, 0.0, epsv_times_h, nullptr,
                                 (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
-    *energy = total(out.energy);
     s = copy_potential(ctx, ctx->sel_all, out, flags, grad, hess);
+    if (!out.pending) *energy = total(out.energy);
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;
@@ -961,8 +1018,8 @@ rigc_status rigc_batch_barrier_term(rigc_ctx* ctx, int32_t n, const int32_t* sce
                                 dmin, 0.0, nullptr, (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0,
                                 ctx->num_cus, out, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
-    std::memcpy(energy, out.energy.data(), sizeof(double) * n);
     s = copy_potential(ctx, ctx->sel_req, out, flags, grad, hess);
+    if (!out.pending) std::memcpy(energy, out.energy.data(), sizeof(double) * n);
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;
@@ -1041,8 +1098,8 @@ rigc_status rigc_batch_friction_term(rigc_ctx* ctx, int32_t n, const int32_t* sc
                                 ctx->scene_par.as<double>() + 2 * (size_t)ctx->scene.n_scenes,
                                 (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
-    std::memcpy(energy, out.energy.data(), sizeof(double) * n);
     s = copy_potential(ctx, ctx->sel_req, out, flags, grad, hess);
+    if (!out.pending) std::memcpy(energy, out.energy.data(), sizeof(double) * n);
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;

@@ -743,6 +743,8 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
     if ((e = w.tasks[0].ensure(sizeof(CcdTask) * cap)) != hipSuccess) return e;
     if ((e = w.tasks[1].ensure(sizeof(CcdTask) * cap)) != hipSuccess) return e;
     if ((e = w.count.ensure(sizeof(unsigned long long))) != hipSuccess) return e;
+    if ((e = w.h_count.ensure(sizeof(unsigned long long))) != hipSuccess) return e;
+    unsigned long long* h_n = w.h_count.as<unsigned long long>();
     const unsigned eb = (unsigned)((nq + 255) / 256);
     hipLaunchKernelGGL(ccd_init_kernel, dim3(eb), dim3(256), 0, stream, a);
     a.tasks_in = nullptr;
@@ -762,8 +764,8 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipEventRecord(w.k1, stream)) != hipSuccess) return e;
         w.passes = pass + 1;
-        unsigned long long n = 0;
-        if ((e = hipMemcpyAsync(&n, a.n_tasks_out, sizeof(n), hipMemcpyDeviceToHost, stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(h_n, a.n_tasks_out, sizeof(*h_n), hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return e;
         // the caller's results ride on the same synchronisation; a further
         // pass overwrites them
         for (int i = 0; i < n_rb; i++)
@@ -773,6 +775,7 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
         float ms = 0.0f;
         if ((e = hipEventElapsedTime(&ms, w.k0, w.k1)) != hipSuccess) return e;
         w.kernel_ms += ms;
+        const unsigned long long n = *h_n;
         if (n == 0) break;
         a.tasks_in = w.tasks[pass & 1].as<CcdTask>();
         a.n_tasks_in = (long long)std::min<unsigned long long>(n, (unsigned long long)cap);

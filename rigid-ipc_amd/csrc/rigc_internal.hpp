@@ -76,12 +76,40 @@ struct DevBuf {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Grow-only page-locked host allocation. Copies between it and the device
+// are queued without blocking the host (a copy from pageable memory makes
+// the runtime wait for it). Its contents may change or be read only once the
+// stream has been synchronised after the last copy queued on it, and it is
+// grown only then.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = 256;
+        while (want < n) want *= 2;
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
 // Broad-phase working set (device buffers reused across calls).
 struct BroadPhase {
     DevBuf box_raw, box_inf, inc, dom, vmin, vmax, bhull;
     DevBuf vc, vh, ec, eh, fc, fh;
     DevBuf pairs, keys_ee, keys_fv, tmp, hist, counters;
     DevBuf out_ee, out_fv;  // sorted (a, b) int32 pairs
+    HostBuf h_counters;     // element-pair counts read back
     long long n_ee = 0, n_fv = 0, n_body_pairs = 0;
 };
 
@@ -149,6 +177,7 @@ struct CcdArgs {
 // Task buffers of the multi-pass narrow phase (grow-only).
 struct CcdWork {
     DevBuf tasks[2], count;
+    HostBuf h_count;         // export count read back after each pass
     int passes = 0;          // traversal launches of the last launch_ccd
     float kernel_ms = 0.0f;  // their summed duration (events around each launch)
     hipEvent_t k0 = nullptr, k1 = nullptr;
@@ -156,6 +185,7 @@ struct CcdWork {
         tasks[0].release();
         tasks[1].release();
         count.release();
+        h_count.release();
         if (k0) (void)hipEventDestroy(k0);
         if (k1) (void)hipEventDestroy(k1);
         k0 = k1 = nullptr;
@@ -223,11 +253,13 @@ struct Assembly {
     bool have_fric = false;             // every scene has a friction set
     std::vector<long long> fric_off;
     std::vector<char> fric_valid;
-    std::vector<long long> h_entries;   // staging of the last range-copy table
     DevBuf energy, lgrad, lhess, esum;  // per-constraint outputs and the energy sum
     DevBuf dkeys, okeys, ktmp, khist, oflags, opos, owork;
     DevBuf diag, grad, opairs, oblocks, mind;
     long long n_off = 0;
+    // page-locked staging: counts read back (h_mail), the range-copy table,
+    // per-scene class ranges, energy sums and minimum distances
+    HostBuf h_mail, h_entries, h_ranges, h_esum, h_mind;
 
     void release() {
         for (DevBuf* b : { &R, &frames, &W, &W0, &ckeys[0], &ckeys[1], &ckeys[2], &ckeys[3], &cstart[0], &cstart[1],
@@ -235,12 +267,22 @@ struct Assembly {
                            &fric_stage, &ranges, &entries, &energy, &lgrad, &lhess, &esum, &dkeys, &okeys, &ktmp,
                            &khist, &oflags, &opos, &owork, &diag, &grad, &opairs, &oblocks, &mind })
             b->release();
+        for (HostBuf* b : { &h_mail, &h_entries, &h_ranges, &h_esum, &h_mind }) b->release();
     }
 };
 
+// Slots of Assembly::h_mail (unsigned int).
+enum { MAIL_CLASS = 0, MAIL_RUNS = 4, MAIL_OFF = 6, MAIL_SLOTS = 8 };
+
+// Result of assemble_potential. Its read-backs are queued, not waited for:
+// once the stream has been synchronised, finish_potential fills energy and
+// n_off.
 struct AssemblyOut {
     std::vector<double> energy;  // per selected scene
     long long n_off = 0;         // off-diagonal blocks produced
+    long long n_rec = 0;         // records assembled (n_off <= n_rec)
+    bool pending = false;
+    bool hess = false;
 };
 
 // Constraint set of the selected scenes at poses, kept in as.cons.
@@ -252,11 +294,13 @@ hipError_t constraint_counts(Assembly& as, const DevScene& sc, const SceneSel& s
 // Barrier (friction == false, over the current constraint set) or friction
 // potential (over the selected scenes' friction sets). Outputs stay on the
 // device: as.grad [nb][6], as.diag [nb][36] (rows of the selected bodies),
-// as.opairs / as.oblocks [n_off]; energies per selected scene.
+// as.opairs / as.oblocks [n_off]; energies per selected scene. Nothing is
+// waited for: synchronise the stream, then call finish_potential.
 hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& sel, bool friction,
                               const double* d_poses, const double* d_poses_t0, double dhat, double dmin,
                               double epsv_h, const double* d_epsv_h_scene, bool want_grad, bool want_hess,
                               int num_cus, AssemblyOut& out, hipStream_t stream);
+void finish_potential(Assembly& as, AssemblyOut& out);
 // Friction sets of the selected scenes from the current constraint set
 // (which must be the selected scenes' set at d_poses); kappa / mu per scene
 // (device arrays indexed by scene id).

@@ -100,6 +100,11 @@ typedef struct rigs_stats {
     int64_t ccd_box_evaluations;
     double ccd_kernel_ms;        /* CCD traversal kernel launches alone */
     int64_t ccd_launches;
+    /* host wall time of the other backend calls, by kind */
+    double constraint_ms;        /* construct_constraint_set */
+    double barrier_ms;           /* compute_barrier_term */
+    double friction_ms;          /* compute_friction_term, update_friction_constraints */
+    double distance_ms;          /* compute_minimum_distance */
 } rigs_stats;
 
 typedef struct rigs_sim rigs_sim;

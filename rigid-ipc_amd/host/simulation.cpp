@@ -122,7 +122,9 @@ struct rigs_sim {
         int64_t counts[4];
         const double t0 = now_ms();
         rigc_status s = be.construct_constraint_set(be.self, poses, P.dhat, P.minimum_separation_distance, counts);
-        stats.contact_ms += now_ms() - t0;
+        const double dt = now_ms() - t0;
+        stats.contact_ms += dt;
+        stats.constraint_ms += dt;
         if (s != RIGC_OK) return backend_fail(s, "construct_constraint_set");
         stats.constraint_builds++;
         stats.num_constraints = counts[0] + counts[1] + counts[2] + counts[3];
@@ -134,7 +136,7 @@ struct rigs_sim {
     // Calls a potential entry point, growing the off-diagonal capacity once.
     template <typename F>
     rigc_status potential(F&& call, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
-                          const char* what) {
+                          const char* what, double& kind_ms) {
         if (g) g->assign(6 * nb, 0.0);
         rigc_block_hessian h{};
         for (int attempt = 0; attempt < 2; attempt++) {
@@ -149,7 +151,9 @@ struct rigs_sim {
             }
             const double t0 = now_ms();
             rigc_status s = call(flags, &e, g ? g->data() : nullptr, H ? &h : nullptr);
-            stats.contact_ms += now_ms() - t0;
+            const double dt = now_ms() - t0;
+            stats.contact_ms += dt;
+            kind_ms += dt;
             if (s == RIGC_ERR_CAPACITY && H && h.count > hcap) {
                 hcap = h.count;
                 continue;
@@ -175,7 +179,7 @@ struct rigs_sim {
         auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
             return be.compute_barrier_term(be.self, x, P.dhat, P.minimum_separation_distance, f, pe, pg, ph);
         };
-        return potential(call, flags, e, g, H, "compute_barrier_term");
+        return potential(call, flags, e, g, H, "compute_barrier_term", stats.barrier_ms);
     }
 
     rigc_status friction_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H) {
@@ -183,7 +187,7 @@ struct rigs_sim {
             return be.compute_friction_term(be.self, x, poses_t0.data(), P.static_friction_speed_bound * P.timestep,
                                             f, pe, pg, ph);
         };
-        return potential(call, flags, e, g, H, "compute_friction_term");
+        return potential(call, flags, e, g, H, "compute_friction_term", stats.friction_ms);
     }
 
     rigc_status update_friction(const double* poses) {
@@ -194,7 +198,9 @@ struct rigs_sim {
         const double t0 = now_ms();
         s = be.update_friction_constraints(be.self, poses, P.dhat, P.minimum_separation_distance, kappa,
                                            P.coefficient_friction, 0, &n);
-        stats.contact_ms += now_ms() - t0;
+        const double dt = now_ms() - t0;
+        stats.contact_ms += dt;
+        stats.friction_ms += dt;
         if (s != RIGC_OK) return backend_fail(s, "update_friction_constraints");
         return RIGC_OK;
     }
@@ -206,7 +212,9 @@ struct rigs_sim {
         const double t0 = now_ms();
         double m = 0;
         s = be.compute_minimum_distance(be.self, x, P.dhat, P.minimum_separation_distance, 0, &m);
-        stats.contact_ms += now_ms() - t0;
+        const double dt = now_ms() - t0;
+        stats.contact_ms += dt;
+        stats.distance_ms += dt;
         if (s != RIGC_OK) return backend_fail(s, "compute_minimum_distance");
         out = std::isfinite(m) ? m : -1;
         return RIGC_OK;

@@ -52,6 +52,8 @@ class Stats(ctypes.Structure):
         ("ccd_candidates", ctypes.c_int64), ("ccd_ms", ctypes.c_double), ("ccd_broad_phase_ms", ctypes.c_double),
         ("ccd_narrow_phase_ms", ctypes.c_double), ("ccd_box_evaluations", ctypes.c_int64),
         ("ccd_kernel_ms", ctypes.c_double), ("ccd_launches", ctypes.c_int64),
+        ("constraint_ms", ctypes.c_double), ("barrier_ms", ctypes.c_double), ("friction_ms", ctypes.c_double),
+        ("distance_ms", ctypes.c_double),
     ]
 
 
