@@ -385,6 +385,19 @@ __device__ inline double wave_sum_uniform(double x) {
     return __shfl(x, 0);  // lane 0's value, so every lane takes the same branch
 }
 
+// Jacobi rotation (c, s) that zeroes A[p][q] of the symmetric 12 x 12 A.
+__device__ inline void jacobi_rotation(const double* A, int p, int q, double& c, double& s) {
+    const double apq = A[12 * p + q];
+    c = 1.0;
+    s = 0.0;
+    if (apq != 0.0) {
+        const double theta = (A[13 * q] - A[13 * p]) / (2 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        c = 1 / sqrt(t * t + 1);
+        s = t * c;
+    }
+}
+
 // One wavefront (the whole block) per constraint.
 template <bool FRIC>
 __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
@@ -396,7 +409,6 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
     __shared__ double sg[12];
     __shared__ double sG[4][9];  // slot Jacobian (rotation columns): [k][c]
     __shared__ double sT[4][6];  // slot rotational Hessian term, (c,d) packed
-    __shared__ double scs[6][2];
     const int lane = threadIdx.x;
     for (long long c = blockIdx.x; c < a.n; c += gridDim.x) {
         ConRec rec;
@@ -493,38 +505,51 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
             off = wave_sum_uniform(off);
             tot = wave_sum_uniform(tot);
             if (off <= 1e-30 * tot || off == 0) break;
+            // Round r rotates the six pairs of JACOBI_PAIRS[r] at once.
+            // Lanes 0..5 compute the rotations, which the other lanes take
+            // by shuffles; lane l < 36 owns the 2 x 2 block of A at
+            // (pair l / 6, pair l % 6), and the 72 (row, pair) entries of V
+            // go to lanes l and l + 64. One barrier separates a round's
+            // reads from its writes, one follows the writes.
             for (int round = 0; round < 11; round++) {
-                if (lane < 6) {
-                    const int p = JACOBI_PAIRS[round][lane][0], q = JACOBI_PAIRS[round][lane][1];
-                    const double apq = sA[12 * p + q];
-                    double cc = 1.0, ss = 0.0;
-                    if (apq != 0.0) {
-                        const double theta = (sA[13 * q] - sA[13 * p]) / (2 * apq);
-                        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
-                        cc = 1 / sqrt(t * t + 1);
-                        ss = t * cc;
-                    }
-                    scs[lane][0] = cc;
-                    scs[lane][1] = ss;
+                const signed char(*P)[2] = JACOBI_PAIRS[round];
+                double rc = 1.0, rs = 0.0;
+                if (lane < 6) jacobi_rotation(sA, P[lane][0], P[lane][1], rc, rs);
+                const int ia = lane < 36 ? lane / 6 : 0, ib = lane % 6;
+                const int pa = P[ia][0], qa = P[ia][1], pb = P[ib][0], qb = P[ib][1];
+                double x00 = 0.0, x01 = 0.0, x10 = 0.0, x11 = 0.0;
+                if (lane < 36) {
+                    x00 = sA[12 * pa + pb]; x01 = sA[12 * pa + qb];
+                    x10 = sA[12 * qa + pb]; x11 = sA[12 * qa + qb];
                 }
-                __syncthreads();
-                for (int t = lane; t < 144; t += 64) {
-                    double* M = t < 72 ? sA : sV;
-                    const int rem = t % 72, i = rem / 6, k = rem % 6;
-                    const int p = JACOBI_PAIRS[round][k][0], q = JACOBI_PAIRS[round][k][1];
-                    const double cc = scs[k][0], ss = scs[k][1];
-                    const double x = M[12 * i + p], y = M[12 * i + q];
-                    M[12 * i + p] = cc * x - ss * y;
-                    M[12 * i + q] = ss * x + cc * y;
+                // V entries: row t / 6 with pair t % 6, t = lane, lane + 64
+                const int k1 = (lane + 64) % 6;
+                const int vr0 = lane / 6, vr1 = (lane + 64) / 6;
+                const int vp0 = P[ib][0], vq0 = P[ib][1], vp1 = P[k1][0], vq1 = P[k1][1];
+                const double u0 = sV[12 * vr0 + vp0], w0 = sV[12 * vr0 + vq0];
+                double u1 = 0.0, w1 = 0.0;
+                if (lane + 64 < 72) {
+                    u1 = sV[12 * vr1 + vp1];
+                    w1 = sV[12 * vr1 + vq1];
                 }
+                const double ca = __shfl(rc, ia), sa = __shfl(rs, ia);
+                const double cb = __shfl(rc, ib), sb = __shfl(rs, ib);
+                const double c1 = __shfl(rc, k1), s1 = __shfl(rs, k1);
                 __syncthreads();
-                for (int t = lane; t < 72; t += 64) {
-                    const int j = t / 6, k = t % 6;
-                    const int p = JACOBI_PAIRS[round][k][0], q = JACOBI_PAIRS[round][k][1];
-                    const double cc = scs[k][0], ss = scs[k][1];
-                    const double x = sA[12 * p + j], y = sA[12 * q + j];
-                    sA[12 * p + j] = cc * x - ss * y;
-                    sA[12 * q + j] = ss * x + cc * y;
+                if (lane < 36) {
+                    // columns (pair b), then rows (pair a)
+                    const double y00 = cb * x00 - sb * x01, y01 = sb * x00 + cb * x01;
+                    const double y10 = cb * x10 - sb * x11, y11 = sb * x10 + cb * x11;
+                    sA[12 * pa + pb] = ca * y00 - sa * y10;
+                    sA[12 * qa + pb] = sa * y00 + ca * y10;
+                    sA[12 * pa + qb] = ca * y01 - sa * y11;
+                    sA[12 * qa + qb] = sa * y01 + ca * y11;
+                }
+                sV[12 * vr0 + vp0] = cb * u0 - sb * w0;
+                sV[12 * vr0 + vq0] = sb * u0 + cb * w0;
+                if (lane + 64 < 72) {
+                    sV[12 * vr1 + vp1] = c1 * u1 - s1 * w1;
+                    sV[12 * vr1 + vq1] = s1 * u1 + c1 * w1;
                 }
                 __syncthreads();
             }
