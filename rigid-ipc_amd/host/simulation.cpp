@@ -173,25 +173,75 @@ struct rigs_sim {
         return RIGC_OK;
     }
 
-    rigc_status barrier_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H) {
+    // A device term at the first trial point of a line search, evaluated
+    // with its gradient and Hessian (prefetch): when the trial is accepted
+    // the next Newton iteration starts there and takes them from here
+    // instead of making a second call. Entries are keyed by the poses; the
+    // barrier term is a function of the poses alone (its constraint set is
+    // built at them), the friction term also of the lagged friction set,
+    // so the friction entry is dropped when that set changes.
+    struct TermCache {
+        bool valid = false;
+        std::vector<double> x;
+        double e = 0;
+        std::vector<double> g;
+        BlockHess H;
+    };
+    TermCache bcache, fcache;
+
+    template <typename F>
+    rigc_status cached_term(TermCache& c, F&& eval, const double* x, int32_t flags, bool prefetch, double& e,
+                            std::vector<double>* g, BlockHess* H) {
+        const int32_t full = RIGC_GRADIENT | RIGC_HESSIAN;
+        // an energy-only request is served by a full entry: the energy is
+        // the same sum of the same per-constraint values
+        if (c.valid && (flags == 0 || flags == full) &&
+            std::memcmp(c.x.data(), x, sizeof(double) * 6 * (size_t)nb) == 0) {
+            e = c.e;
+            if (g) *g = c.g;
+            if (H) *H = c.H;
+            return RIGC_OK;
+        }
+        if (prefetch && flags == 0) {
+            c.valid = false;
+            rigc_status s = eval(full, c.e, &c.g, &c.H);
+            if (s != RIGC_OK) return s;
+            c.x.assign(x, x + 6 * (size_t)nb);
+            c.valid = true;
+            e = c.e;
+            return RIGC_OK;
+        }
+        return eval(flags, e, g, H);
+    }
+
+    rigc_status barrier_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
+                             bool prefetch = false) {
         rigc_status s = ensure_constraints(x);
         if (s != RIGC_OK) return s;
         auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
             return be.compute_barrier_term(be.self, x, P.dhat, P.minimum_separation_distance, f, pe, pg, ph);
         };
-        return potential(call, flags, e, g, H, "compute_barrier_term", stats.barrier_ms);
+        auto eval = [&](int32_t f, double& pe, std::vector<double>* pg, BlockHess* pH) {
+            return potential(call, f, pe, pg, pH, "compute_barrier_term", stats.barrier_ms);
+        };
+        return cached_term(bcache, eval, x, flags, prefetch, e, g, H);
     }
 
-    rigc_status friction_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H) {
+    rigc_status friction_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
+                              bool prefetch = false) {
         auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
             return be.compute_friction_term(be.self, x, poses_t0.data(), P.static_friction_speed_bound * P.timestep,
                                             f, pe, pg, ph);
         };
-        return potential(call, flags, e, g, H, "compute_friction_term", stats.friction_ms);
+        auto eval = [&](int32_t f, double& pe, std::vector<double>* pg, BlockHess* pH) {
+            return potential(call, f, pe, pg, pH, "compute_friction_term", stats.friction_ms);
+        };
+        return cached_term(fcache, eval, x, flags, prefetch, e, g, H);
     }
 
     rigc_status update_friction(const double* poses) {
         if (P.coefficient_friction <= 0) return RIGC_OK;
+        fcache.valid = false;
         rigc_status s = ensure_constraints(poses);
         if (s != RIGC_OK) return s;
         int64_t n = 0;
@@ -309,7 +359,7 @@ struct rigs_sim {
     // ------------------------------------------------------------ objective
     // f = E / m + kappa / m B + D / m (compute_objective :633-703)
     rigc_status objective(const double* x, bool want_grad, bool want_hess, double& f, std::vector<double>& g,
-                          BlockHess& H) {
+                          BlockHess& H, bool prefetch = false) {
         stats.objective_evaluations++;
         const double t0 = now_ms();
         std::vector<double> gE, gB, gD;
@@ -318,10 +368,10 @@ struct rigs_sim {
         stats.host_ms += now_ms() - t0;
         const int32_t flags = (want_grad ? RIGC_GRADIENT : 0) | (want_hess ? RIGC_HESSIAN : 0);
         double B = 0, D = 0;
-        rigc_status s = barrier_term(x, flags, B, want_grad ? &gB : nullptr, want_hess ? &HB : nullptr);
+        rigc_status s = barrier_term(x, flags, B, want_grad ? &gB : nullptr, want_hess ? &HB : nullptr, prefetch);
         if (s != RIGC_OK) return s;
         if (P.coefficient_friction > 0) {
-            s = friction_term(x, flags, D, want_grad ? &gD : nullptr, want_hess ? &HD : nullptr);
+            s = friction_term(x, flags, D, want_grad ? &gD : nullptr, want_hess ? &HD : nullptr, prefetch);
             if (s != RIGC_OK) return s;
         }
         const double t1 = now_ms();
@@ -463,11 +513,15 @@ struct rigs_sim {
         step = std::min(step, std::min(toi, 1.0));
         std::vector<double> xi(x.size()), gdummy;
         BlockHess Hdummy;
+        bool first = true;
         while (std::isfinite(lower) && step >= lower) {
             stats.line_search_iterations++;
             for (size_t i = 0; i < x.size(); i++) xi[i] = x[i] + step * dir[i];
             double fxi = 0;
-            s = objective(xi.data(), false, false, fxi, gdummy, Hdummy);
+            // the first trial is usually accepted: its device terms come
+            // with the gradient and Hessian the next iteration needs
+            s = objective(xi.data(), false, false, fxi, gdummy, Hdummy, first);
+            first = false;
             if (s != RIGC_OK) return s;
             if (verbose >= 2)
                 std::fprintf(stderr, "rigs:   alpha=%g f=%.17g (E %.17g, B %.17g, D %.17g)\n", step, fxi,
@@ -593,6 +647,7 @@ struct rigs_sim {
             std::memcpy(B.pose_prev, B.pose, sizeof(B.pose));
         }
         for (int b = 0; b < nb; b++) std::memcpy(&poses_t0[6 * b], bodies[b].pose, sizeof(double) * 6);
+        bcache.valid = fcache.valid = false;  // the friction term depends on poses_t0
         // update_constraints: barrier gradient at t0 (kept by the reference for
         // its integrators) and the lagged friction set at t0
         {
