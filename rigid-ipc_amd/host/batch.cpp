@@ -28,6 +28,12 @@
 
 #include "rigs.h"
 
+namespace rigs {
+// host/simulation.cpp: whether a stepper computes the body energy on a
+// helper thread beside its device calls
+void set_host_overlap(rigs_sim* sim, bool on);
+}  // namespace rigs
+
 namespace {
 
 double now_ms() {
@@ -414,6 +420,7 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
         s = rigs_create(&b->sims[i], mode == RIGS_BATCH_FUSED ? &b->tables[i] : nullptr, device, &scenes[i],
                         &bodies[i], &params[i]);
         if (s != RIGC_OK) b->err = "rigs_create failed for scene " + std::to_string(i);
+        else rigs::set_host_overlap(b->sims[i], false);
     }
     if (s != RIGC_OK) {
         rigs_batch_destroy(b);

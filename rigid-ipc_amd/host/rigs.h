@@ -85,7 +85,9 @@ typedef struct rigs_stats {
     int64_t regularizations;
     int64_t failed_solves;
     double contact_ms;   /* time inside backend calls */
-    double host_ms;      /* body energy, assembly of the Newton system, LDL^T */
+    double host_ms;      /* body energy, assembly of the Newton system, LDL^T; the body
+                          * energy computed beside backend calls counts only for its
+                          * wait */
     double total_ms;
     double kappa;        /* barrier stiffness of the last step */
     double min_distance; /* after the last step (-1: no constraints) */

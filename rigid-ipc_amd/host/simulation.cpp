@@ -17,11 +17,15 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../csrc/rigc_barrier.hpp"
@@ -69,6 +73,57 @@ struct BodyState {
     bool dynamic = false;
     double pose[6], pose_prev[6], velocity[6];
     double Qdot[9];
+    // inertial targets of the current step: position and rotation matrix
+    double target[3], Pm[9];
+};
+
+// One helper thread that runs a task while the caller makes backend calls.
+class Helper {
+  public:
+    ~Helper() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            quit_ = true;
+        }
+        cv_.notify_all();
+        if (t_.joinable()) t_.join();
+    }
+    void run(std::function<void()> task) {
+        if (!t_.joinable()) t_ = std::thread([this] { loop(); });
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            task_ = std::move(task);
+            busy_ = true;
+        }
+        cv_.notify_all();
+    }
+    void wait() {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [this] { return !busy_; });
+    }
+
+  private:
+    void loop() {
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [this] { return busy_ || quit_; });
+            if (busy_) {
+                std::function<void()> task = std::move(task_);
+                lk.unlock();
+                task();
+                lk.lock();
+                busy_ = false;
+                cv_.notify_all();
+            } else if (quit_) {
+                return;
+            }
+        }
+    }
+    std::thread t_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::function<void()> task_;
+    bool busy_ = false, quit_ = false;
 };
 
 // Symmetric block Hessian in body layout.
@@ -296,8 +351,18 @@ struct rigs_sim {
     // compute_body_energy<T> for IMPLICIT_EULER without external forces:
     //   1/2 m q.q - m q.(q_t0 + h (qdot_t0 + h g))
     //   + 1/2 tr(Q J Q^T) - tr(Q J (Q_t0 + h Qdot_t0)^T)
-    double energy_term(const double* x, std::vector<double>* g, BlockHess* H) {
+    // Inertial targets of the step starting at the bodies' current state.
+    void step_targets() {
         const double h = P.timestep;
+        for (auto& B : bodies) {
+            for (int k = 0; k < 3; k++) B.target[k] = B.pose[k] + h * (B.velocity[k] + h * P.gravity[k]);
+            double Q0[9];
+            rigc::rotation_d(B.pose + 3, Q0);
+            for (int e = 0; e < 9; e++) B.Pm[e] = Q0[e] + h * B.Qdot[e];
+        }
+    }
+
+    double energy_term(const double* x, std::vector<double>* g, BlockHess* H) const {
         double E = 0;
         if (g) g->assign(6 * nb, 0.0);
         if (H) H->reset(nb);
@@ -305,18 +370,16 @@ struct rigs_sim {
             const BodyState& B = bodies[b];
             if (!B.dynamic) continue;
             const double* q = x + 6 * b;
-            double target[3];
-            for (int k = 0; k < 3; k++) target[k] = B.pose[k] + h * (B.velocity[k] + h * P.gravity[k]);
+            const double* target = B.target;
             double qq = 0, qt = 0;
             for (int k = 0; k < 3; k++) {
                 qq += q[k] * q[k];
                 qt += q[k] * target[k];
             }
             E += 0.5 * B.mass * qq - B.mass * qt;
-            double Q[9], Q0[9], Pm[9];
+            double Q[9];
+            const double* Pm = B.Pm;
             rigc::rotation_d(q + 3, Q);
-            rigc::rotation_d(B.pose + 3, Q0);
-            for (int e = 0; e < 9; e++) Pm[e] = Q0[e] + h * B.Qdot[e];
             double tQJQ = 0, tQJP = 0;
             for (int i = 0; i < 3; i++)
                 for (int j = 0; j < 3; j++) {
@@ -358,22 +421,64 @@ struct rigs_sim {
 
     // ------------------------------------------------------------ objective
     // f = E / m + kappa / m B + D / m (compute_objective :633-703)
+    // The body energy at a line search's first trial point, with gradient
+    // and Hessian: computed on the helper thread (when `overlap`) while the
+    // device terms are evaluated, and taken by the next Newton iteration
+    // when that trial is accepted. Its energy is the same value an
+    // energy-only call gives.
+    struct EnergyCache {
+        bool valid = false;
+        std::vector<double> x;
+        double E = 0;
+        std::vector<double> g;
+        BlockHess H;
+    };
+    EnergyCache ecache;
+    Helper helper;
+    bool overlap = true;
+
     rigc_status objective(const double* x, bool want_grad, bool want_hess, double& f, std::vector<double>& g,
                           BlockHess& H, bool prefetch = false) {
         stats.objective_evaluations++;
         const double t0 = now_ms();
-        std::vector<double> gE, gB, gD;
+        std::vector<double> gB, gD;
         BlockHess HB, HD;
-        const double E = energy_term(x, want_grad ? &g : nullptr, want_hess ? &H : nullptr);
+        double E = 0;
+        bool pending = false;
+        if (ecache.valid && want_grad == want_hess &&
+            std::memcmp(ecache.x.data(), x, sizeof(double) * 6 * (size_t)nb) == 0) {
+            E = ecache.E;
+            if (want_grad) g = ecache.g;
+            if (want_hess) H = ecache.H;
+        } else if (prefetch && !want_grad && !want_hess) {
+            ecache.valid = false;
+            ecache.x.assign(x, x + 6 * (size_t)nb);
+            auto task = [this] { ecache.E = energy_term(ecache.x.data(), &ecache.g, &ecache.H); };
+            if (overlap) {
+                helper.run(task);
+                pending = true;
+            } else {
+                task();
+                ecache.valid = true;
+                E = ecache.E;
+            }
+        } else {
+            E = energy_term(x, want_grad ? &g : nullptr, want_hess ? &H : nullptr);
+        }
         stats.host_ms += now_ms() - t0;
         const int32_t flags = (want_grad ? RIGC_GRADIENT : 0) | (want_hess ? RIGC_HESSIAN : 0);
         double B = 0, D = 0;
         rigc_status s = barrier_term(x, flags, B, want_grad ? &gB : nullptr, want_hess ? &HB : nullptr, prefetch);
-        if (s != RIGC_OK) return s;
-        if (P.coefficient_friction > 0) {
+        if (s == RIGC_OK && P.coefficient_friction > 0)
             s = friction_term(x, flags, D, want_grad ? &gD : nullptr, want_hess ? &HD : nullptr, prefetch);
-            if (s != RIGC_OK) return s;
+        if (pending) {
+            const double tw = now_ms();
+            helper.wait();
+            stats.host_ms += now_ms() - tw;
+            ecache.valid = true;
+            E = ecache.E;
         }
+        if (s != RIGC_OK) return s;
         const double t1 = now_ms();
         const double im = 1.0 / average_mass, km = kappa / average_mass;
         f = E * im + km * B + D * im;
@@ -648,6 +753,8 @@ struct rigs_sim {
         }
         for (int b = 0; b < nb; b++) std::memcpy(&poses_t0[6 * b], bodies[b].pose, sizeof(double) * 6);
         bcache.valid = fcache.valid = false;  // the friction term depends on poses_t0
+        ecache.valid = false;                 // and the body energy on the step's targets
+        step_targets();
         // update_constraints: barrier gradient at t0 (kept by the reference for
         // its integrators) and the lagged friction set at t0
         {
@@ -723,6 +830,13 @@ struct rigs_sim {
         return RIGC_OK;
     }
 };
+
+namespace rigs {
+// The batch stepper runs every scene's stepper on its own thread already.
+void set_host_overlap(rigs_sim* sim, bool on) {
+    if (sim) sim->overlap = on;
+}
+}  // namespace rigs
 
 extern "C" {
 
