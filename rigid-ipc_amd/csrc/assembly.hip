@@ -808,8 +808,18 @@ __global__ __launch_bounds__(THREADS) void min_distance_kernel(const ConRec* con
 
 unsigned grid_for(long long n, int threads = THREADS) { return (unsigned)((n + threads - 1) / threads); }
 
+// World vertices into W (as.W or as.W0) and, with `frames`, the body frames
+// at d_poses; nothing is recomputed when the pose array's content tag shows
+// that W (and the frames) already hold its values for every body.
 hipError_t world_vertices(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses, DevBuf& W,
                           bool frames, hipStream_t st) {
+    unsigned long long t = 0;
+    for (int k = 0; k < 3; k++)
+        if (as.tag_ptr[k] == d_poses) t = as.tag[k];
+    unsigned long long& w_tag = &W == &as.W0 ? as.W0_tag : as.W_tag;
+    if (sel.all && t && w_tag == t && (!frames || as.F_tag == t)) return hipSuccess;
+    w_tag = 0;
+    if (frames) as.F_tag = 0;
     AS_CHECK(as.R.ensure(9 * sizeof(double) * sc.nb));
     AS_CHECK(as.frames.ensure(sizeof(BodyFrame) * sc.nb));
     AS_CHECK(W.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
@@ -824,7 +834,12 @@ hipError_t world_vertices(Assembly& as, const DevScene& sc, const SceneSel& sel,
                            sc.body_vstart, sc.V, d_poses, as.R.as<double>(),
                            frames ? as.frames.as<BodyFrame>() : nullptr, W.as<double>());
     }
-    return hipGetLastError();
+    AS_CHECK(hipGetLastError());
+    if (sel.all) {
+        w_tag = t;
+        if (frames) as.F_tag = t;
+    }
+    return hipSuccess;
 }
 
 __global__ void gather_rows_kernel(const int* rows, int n, int width, const double* src, double* dst) {

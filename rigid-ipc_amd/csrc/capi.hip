@@ -38,12 +38,19 @@ struct rigc_ctx {
 
     // per-call buffers
     DevBuf poses0, poses1, stage0, stage1, out_rows, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
+    DevBuf poses2;  // the friction term's start-of-step poses
+    // Content tags of poses0 / poses1 / poses2 (0: unknown; equal non-zero
+    // tags, equal contents) and the host copies of their last full uploads:
+    // an upload of the same values is skipped, and world vertices computed
+    // from them are reused.
+    unsigned long long pose_tag[3] = { 0, 0, 0 }, tag_counter = 0;
+    std::vector<double> h_last[3];
     DevBuf scene_best, scene_skip, scene_par;
     // page-locked staging: poses (uploads), per-scene parameters, the
     // earliest-TOI bounds and results, and potential outputs; a staging
     // buffer is rewritten only after ev_stage, recorded behind the last
     // upload from it, has completed
-    HostBuf h_pose0, h_pose1, h_par, h_toi, h_out;
+    HostBuf h_pose0, h_pose1, h_pose2, h_par, h_toi, h_out;
     hipEvent_t ev_stage = nullptr;
     SelState sel_all, sel_req;
     BroadPhase bp;
@@ -93,6 +100,22 @@ void free_scene(rigc_ctx* ctx) {
 // Waits until the uploads queued from the page-locked staging buffers have
 // been made (normally long done: every entry point synchronises its stream).
 hipError_t stage_wait(rigc_ctx* ctx) { return hipEventSynchronize(ctx->ev_stage); }
+
+DevBuf& pose_buf(rigc_ctx* ctx, int k) { return k == 0 ? ctx->poses0 : (k == 1 ? ctx->poses1 : ctx->poses2); }
+
+void share_tags(rigc_ctx* ctx) {
+    for (int k = 0; k < 3; k++) {
+        ctx->as.tag_ptr[k] = pose_buf(ctx, k).as<double>();
+        ctx->as.tag[k] = ctx->pose_tag[k];
+    }
+}
+
+// Pose array k was written other than by put_poses (or lost its contents).
+void forget_poses(rigc_ctx* ctx, int k) {
+    ctx->pose_tag[k] = 0;
+    ctx->h_last[k].clear();
+    share_tags(ctx);
+}
 
 // Builds and uploads the selection of scenes ids[0..n) (strictly ascending).
 hipError_t make_sel(rigc_ctx* ctx, SelState& S, const int* ids, int n) {
@@ -214,6 +237,13 @@ rigc_status load_scenes(rigc_ctx* ctx, int n, const rigc_scene* scenes, const ch
     RIGC_HIP(hipSetDevice(ctx->device));
     RIGC_HIP(hipStreamSynchronize(ctx->stream));
     free_scene(ctx);
+    // new geometry: no pose array or world-vertex array is reusable
+    for (int k = 0; k < 3; k++) {
+        ctx->pose_tag[k] = 0;
+        ctx->h_last[k].clear();
+        ctx->as.tag[k] = 0;
+    }
+    ctx->as.W_tag = ctx->as.W0_tag = ctx->as.F_tag = 0;
     DevScene& d = ctx->scene;
     d.nb = NB; d.nv = NV; d.ne = NE; d.nf = NF;
     d.n_scenes = n;
@@ -288,7 +318,7 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx);
-    for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->stage0, &ctx->stage1, &ctx->out_rows, &ctx->ee, &ctx->fv,
+    for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->poses2, &ctx->stage0, &ctx->stage1, &ctx->out_rows, &ctx->ee, &ctx->fv,
                        &ctx->toi, &ctx->hit, &ctx->counters, &ctx->qbest, &ctx->qvisits, &ctx->qflag,
                        &ctx->scene_best, &ctx->scene_skip, &ctx->scene_par, &ctx->sel_all.buf, &ctx->sel_req.buf })
         b->release();
@@ -296,7 +326,7 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
     if (ctx->ev_stage) (void)hipEventDestroy(ctx->ev_stage);
-    for (HostBuf* b : { &ctx->h_pose0, &ctx->h_pose1, &ctx->h_par, &ctx->h_toi, &ctx->h_out, &ctx->sel_all.host,
+    for (HostBuf* b : { &ctx->h_pose0, &ctx->h_pose1, &ctx->h_pose2, &ctx->h_par, &ctx->h_toi, &ctx->h_out, &ctx->sel_all.host,
                         &ctx->sel_req.host, &ctx->bp.h_counters })
         b->release();
     for (DevBuf* b : { &ctx->bp.box_raw, &ctx->bp.box_inf, &ctx->bp.inc, &ctx->bp.dom, &ctx->bp.vmin, &ctx->bp.vmax,
@@ -366,6 +396,8 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     RIGC_HIP(ctx->qvisits.ensure(sizeof(unsigned int) * std::max<size_t>(nq, 1)));
     RIGC_HIP(ctx->qflag.ensure(sizeof(int) * std::max<size_t>(nq, 1)));
     hipStream_t st = ctx->stream;
+    forget_poses(ctx, 0);
+    forget_poses(ctx, 1);
     RIGC_HIP(hipMemcpyAsync(ctx->poses0.p, poses_t0, pose_bytes, hipMemcpyHostToDevice, st));
     RIGC_HIP(hipMemcpyAsync(ctx->poses1.p, poses_t1, pose_bytes, hipMemcpyHostToDevice, st));
     if (n_ee) RIGC_HIP(hipMemcpyAsync(ctx->ee.p, ee, 2 * sizeof(int32_t) * n_ee, hipMemcpyHostToDevice, st));
@@ -470,11 +502,36 @@ rigc_status stage_poses(rigc_ctx* ctx, const double* p0, const double* p1, size_
     return RIGC_OK;
 }
 
+// Full upload of pose array k, skipped when the array holds these values
+// already (the same bytes as its last full upload).
+rigc_status put_poses(rigc_ctx* ctx, int k, const double* p) {
+    const size_t n = 6 * (size_t)ctx->scene.nb, bytes = sizeof(double) * n;
+    DevBuf& d = pose_buf(ctx, k);
+    const void* before = d.p;
+    RIGC_HIP(d.ensure(bytes));
+    std::vector<double>& last = ctx->h_last[k];
+    if (d.p == before && ctx->pose_tag[k] && last.size() == n && std::memcmp(last.data(), p, bytes) == 0)
+        return RIGC_OK;
+    forget_poses(ctx, k);
+    HostBuf& h = k == 0 ? ctx->h_pose0 : (k == 1 ? ctx->h_pose1 : ctx->h_pose2);
+    RIGC_HIP(stage_wait(ctx));
+    RIGC_HIP(h.ensure(bytes));
+    std::memcpy(h.p, p, bytes);
+    RIGC_HIP(hipMemcpyAsync(d.p, h.p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RIGC_HIP(hipEventRecord(ctx->ev_stage, ctx->stream));
+    last.assign(p, p + n);
+    ctx->pose_tag[k] = ++ctx->tag_counter;
+    share_tags(ctx);
+    return RIGC_OK;
+}
+
 rigc_status upload_poses(rigc_ctx* ctx, const double* p0, const double* p1) {
     const size_t pose_bytes = 6 * sizeof(double) * (size_t)ctx->scene.nb;
     RIGC_HIP(ctx->poses0.ensure(pose_bytes));
     RIGC_HIP(ctx->poses1.ensure(pose_bytes));
-    return stage_poses(ctx, p0, p1, (size_t)ctx->scene.nb, ctx->poses0.p, ctx->poses1.p);
+    rigc_status s = put_poses(ctx, 0, p0);
+    if (s == RIGC_OK && p1) s = put_poses(ctx, 1, p1);
+    return s;
 }
 
 // Poses of the selected scenes' bodies, in request order, into the
@@ -487,6 +544,8 @@ rigc_status upload_sel_poses(rigc_ctx* ctx, const SelState& S, const double* p0,
     RIGC_HIP(ctx->stage0.ensure(6 * sizeof(double) * std::max<size_t>(ns, 1)));
     RIGC_HIP(ctx->stage1.ensure(6 * sizeof(double) * std::max<size_t>(ns, 1)));
     hipStream_t st = ctx->stream;
+    forget_poses(ctx, 0);
+    if (p1) forget_poses(ctx, 1);
     const rigc_status s = stage_poses(ctx, p0, p1, ns, ctx->stage0.p, ctx->stage1.p);
     if (s != RIGC_OK) return s;
     scatter_rows(S.sel.bodies, (int)ns, 6, ctx->stage0.as<double>(), ctx->poses0.as<double>(), st);
@@ -957,13 +1016,14 @@ rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const
     if (!ctx->as.have_fric)
         return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_friction_term: no friction set (update it first)");
     RIGC_HIP(hipSetDevice(ctx->device));
-    rigc_status s = upload_poses(ctx, poses, poses_t0);
+    rigc_status s = upload_poses(ctx, poses, nullptr);
+    if (s == RIGC_OK) s = put_poses(ctx, 2, poses_t0);
     if (s != RIGC_OK) return s;
     hipStream_t st = ctx->stream;
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
     AssemblyOut out;
     RIGC_HIP(assemble_potential(ctx->as, ctx->scene, ctx->sel_all.sel, true, ctx->poses0.as<double>(),
-                                ctx->poses1.as<double>(), 0.0, 0.0, epsv_times_h, nullptr,
+                                ctx->poses2.as<double>(), 0.0, 0.0, epsv_times_h, nullptr,
                                 (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
     s = copy_potential(ctx, ctx->sel_all, out, flags, grad, hess);

@@ -257,6 +257,13 @@ struct Assembly {
     DevBuf dkeys, okeys, ktmp, khist, oflags, opos, owork;
     DevBuf diag, grad, opairs, oblocks, mind;
     long long n_off = 0;
+    // Content tags of the pose arrays world vertices are computed from
+    // (set by the caller: equal non-zero tags mean equal contents, 0 means
+    // unknown), and the tags W, W0 and frames were last computed at over
+    // every body (0: not reusable).
+    const double* tag_ptr[3] = { nullptr, nullptr, nullptr };
+    unsigned long long tag[3] = { 0, 0, 0 };
+    unsigned long long W_tag = 0, W0_tag = 0, F_tag = 0;
     // page-locked staging: counts read back (h_mail), the range-copy table,
     // per-scene class ranges, energy sums and minimum distances
     HostBuf h_mail, h_entries, h_ranges, h_esum, h_mind;
