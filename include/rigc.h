@@ -215,10 +215,13 @@ rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* sce
 /* Per-scene rigc_compute_barrier_term with the constraint set built at
  * build_poses (DistanceBarrierConstraint::construct_constraint_set) and the
  * potential evaluated at poses; energy[i] per scene; counts (optional,
- * [n][4]) receive each scene's VV / EV / EE / FV constraint counts. */
+ * [n][4]) receive each scene's VV / EV / EE / FV constraint counts;
+ * min_distance (optional, [n]) receives what rigc_batch_minimum_distance
+ * returns for the same arguments, from the same constraint set (a stepper
+ * that asks for both at one point saves a call). */
 rigc_status rigc_batch_barrier_term(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* build_poses,
                                     const double* poses, double dhat, double dmin, int32_t flags, double* energy,
-                                    double* grad, rigc_block_hessian* hess, int64_t* counts);
+                                    double* grad, rigc_block_hessian* hess, int64_t* counts, double* min_distance);
 /* Per-scene rigc_compute_minimum_distance over the set built at build_poses. */
 rigc_status rigc_batch_minimum_distance(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids,
                                         const double* build_poses, const double* poses, double dhat, double dmin,

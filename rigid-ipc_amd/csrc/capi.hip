@@ -1053,7 +1053,8 @@ rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* sce
 
 rigc_status rigc_batch_barrier_term(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* build_poses,
                                     const double* poses, double dhat, double dmin, int32_t flags, double* energy,
-                                    double* grad, rigc_block_hessian* hess, int64_t* counts) {
+                                    double* grad, rigc_block_hessian* hess, int64_t* counts,
+                                    double* min_distance) {
     if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
     if (!build_poses || !poses || !energy || !(dhat > 0) || !(dmin >= 0) || bad_hess(flags, grad, hess))
         return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_barrier_term: bad arguments");
@@ -1080,6 +1081,11 @@ rigc_status rigc_batch_barrier_term(rigc_ctx* ctx, int32_t n, const int32_t* sce
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
     s = copy_potential(ctx, ctx->sel_req, out, flags, grad, hess);
     if (!out.pending) std::memcpy(energy, out.energy.data(), sizeof(double) * n);
+    if (s == RIGC_OK && min_distance) {
+        std::vector<double> m2;
+        RIGC_HIP(rigc::minimum_distance(ctx->as, ctx->scene, ctx->sel_req.sel, ctx->poses0.as<double>(), m2, st));
+        for (int i = 0; i < n; i++) min_distance[i] = std::sqrt(m2[i]);
+    }
     float ms = 0;
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;

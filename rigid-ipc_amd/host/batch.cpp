@@ -53,6 +53,7 @@ struct Request {
     int32_t flags = 0;
     // outputs
     double* value = nullptr;     // toi / energy / min distance
+    double* mind = nullptr;      // BARRIER: also the minimum distance (optional)
     double* grad = nullptr;
     rigc_block_hessian* hess = nullptr;
     int64_t* count = nullptr;
@@ -69,6 +70,11 @@ struct Proxy {
     int nb = 0;
     std::vector<double> build_poses;  // poses of the scene's last construct_constraint_set
     int64_t counts[4] = { 0, 0, 0, 0 };
+    // minimum distance returned with the last barrier evaluated at its
+    // build poses (the stepper asks for it there after accepting a step)
+    std::vector<double> mind_poses;
+    double mind = 0, mind_new = 0, mind_dhat = 0, mind_dmin = 0;
+    bool mind_valid = false;
     std::string err;
 };
 
@@ -115,12 +121,31 @@ rigc_status px_barrier(void* s, const double* poses, double dhat, double dmin, i
     r.value = e;
     r.grad = g;
     r.hess = h;
-    return submit(P, r);
+    const size_t bytes = sizeof(double) * 6 * (size_t)P->nb;
+    const bool at_build = std::memcmp(P->build_poses.data(), poses, bytes) == 0;
+    P->mind_valid = false;
+    if (at_build) r.mind = &P->mind_new;
+    rigc_status st = submit(P, r);
+    if (st == RIGC_OK && at_build) {
+        P->mind = P->mind_new;
+        P->mind_poses.assign(poses, poses + 6 * (size_t)P->nb);
+        P->mind_dhat = dhat;
+        P->mind_dmin = dmin;
+        P->mind_valid = true;
+    }
+    return st;
 }
 
 rigc_status px_mind(void* s, const double* poses, double dhat, double dmin, int32_t flags, double* out) {
     Proxy* P = static_cast<Proxy*>(s);
     if (flags & RIGC_BUILD_CONSTRAINTS) P->build_poses.assign(poses, poses + 6 * (size_t)P->nb);
+    const size_t bytes = sizeof(double) * 6 * (size_t)P->nb;
+    if (P->mind_valid && dhat == P->mind_dhat && dmin == P->mind_dmin && !P->build_poses.empty() &&
+        std::memcmp(P->mind_poses.data(), poses, bytes) == 0 &&
+        std::memcmp(P->mind_poses.data(), P->build_poses.data(), bytes) == 0) {
+        *out = P->mind;  // same set, same poses: the barrier call's answer
+        return RIGC_OK;
+    }
     if (P->build_poses.empty()) {
         P->err = "no constraint set (build one first)";
         return RIGC_ERR_NO_SCENE;
@@ -311,6 +336,9 @@ void Batch::run_group(int kind, std::vector<Request*>& g) {
             for (int i = 0; i < k; i++) eh[i] = g[i]->c;
         }
         cnt.assign(4 * (size_t)k, 0);
+        bool any_mind = false;
+        for (Request* r : g) any_mind = any_mind || r->mind;
+        std::vector<double> md(any_mind ? k : 0);
         rigc_block_hessian h{};
         for (int attempt = 0; attempt < 2; attempt++) {
             if (want_hess) {
@@ -322,7 +350,7 @@ void Batch::run_group(int kind, std::vector<Request*>& g) {
             if (kind == REQ_BARRIER)
                 s = rigc_batch_barrier_term(ctx, k, ids.data(), c0.data(), c1.data(), g[0]->a, g[0]->b, flags,
                                             vals.data(), want_grad ? grad.data() : nullptr,
-                                            want_hess ? &h : nullptr, cnt.data());
+                                            want_hess ? &h : nullptr, cnt.data(), any_mind ? md.data() : nullptr);
             else
                 s = rigc_batch_friction_term(ctx, k, ids.data(), c0.data(), c1.data(), eh.data(), flags, vals.data(),
                                              want_grad ? grad.data() : nullptr, want_hess ? &h : nullptr);
@@ -340,6 +368,7 @@ void Batch::run_group(int kind, std::vector<Request*>& g) {
                 if (kind == REQ_BARRIER) {
                     Proxy& P = proxies[r->scene];
                     for (int c = 0; c < 4; c++) P.counts[c] = cnt[4 * i + c];
+                    if (r->mind) *r->mind = md[i];
                 }
                 if (want_grad && r->grad)
                     std::memcpy(r->grad, &grad[6 * (size_t)start[i]], sizeof(double) * 6 * nb[r->scene]);

@@ -99,7 +99,7 @@ def lib():
     L.rigc_get_scene_count.argtypes = [vp, _c_int32_p]
     L.rigc_batch_earliest_toi.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, _c_double_p]
     L.rigc_batch_barrier_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, i32, _c_double_p,
-                                          _c_double_p, bh, _c_int64_p]
+                                          _c_double_p, bh, _c_int64_p, _c_double_p]
     L.rigc_batch_minimum_distance.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, _c_double_p]
     L.rigc_batch_update_friction_constraints.argtypes = [vp, i32, _c_int32_p, _c_double_p, d, d, _c_double_p,
                                                          _c_double_p, _c_int64_p]
@@ -260,7 +260,7 @@ class Context:
         flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0)
         self._keep = [ids, pb, p]
         fn = lambda ctx, n, i, a0, a1, a2, a3, f, e, g, hh: self._L.rigc_batch_barrier_term(  # noqa: E731
-            ctx, n, i, a0, a1, a2, a3, f, e, g, hh, None)
+            ctx, n, i, a0, a1, a2, a3, f, e, g, hh, None, None)
         return self._batch_potential(fn, ids, (_p(pb, _c_double_p), _p(p, _c_double_p), float(dhat), float(dmin)),
                                      flags)
 
@@ -270,8 +270,19 @@ class Context:
         counts = np.zeros((len(ids), 4), np.int64)
         self._check(self._L.rigc_batch_barrier_term(self._ctx, len(ids), _p(ids, _c_int32_p), _p(p, _c_double_p),
                                                     _p(p, _c_double_p), float(dhat), float(dmin), 0,
-                                                    _p(e, _c_double_p), None, None, _p(counts, _c_int64_p)))
+                                                    _p(e, _c_double_p), None, None, _p(counts, _c_int64_p), None))
         return counts
+
+    def batch_barrier_energy_and_distance(self, ids, build_poses, poses, dhat, dmin=0.0):
+        """Per-scene barrier energy and minimum distance from one
+        rigc_batch_barrier_term call."""
+        ids, pb, p = self._ids_poses(ids, build_poses, poses)
+        e = np.zeros(len(ids))
+        md = np.zeros(len(ids))
+        self._check(self._L.rigc_batch_barrier_term(self._ctx, len(ids), _p(ids, _c_int32_p), _p(pb, _c_double_p),
+                                                    _p(p, _c_double_p), float(dhat), float(dmin), 0,
+                                                    _p(e, _c_double_p), None, None, None, _p(md, _c_double_p)))
+        return e, md
 
     def batch_minimum_distance(self, ids, build_poses, poses, dhat, dmin=0.0):
         ids, pb, p = self._ids_poses(ids, build_poses, poses)

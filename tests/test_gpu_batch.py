@@ -133,6 +133,23 @@ def test_minimum_distance_matches_single_scene_contexts(setup):
     assert any(np.isfinite(want))
 
 
+def test_barrier_call_returns_the_minimum_distance(setup):
+    # the optional min_distance output of rigc_batch_barrier_term equals
+    # rigc_batch_minimum_distance for the same arguments, bit for bit
+    scenes, batch, plain = setup
+    subset = (0, 2, 3, 4)
+    build = [lowered(scenes[k], 90 + k) for k in subset]
+    evalp = [b.copy() for b in build]
+    for e in evalp:
+        e[:-1, 1] += 2e-4
+    for pe in (build, evalp):
+        e, md = batch.batch_barrier_energy_and_distance(subset, build, pe, DHAT)
+        want = batch.batch_minimum_distance(subset, build, pe, DHAT)
+        assert list(md) == list(want)
+        assert [x[0] for x in batch.batch_barrier_term(subset, build, pe, DHAT)] == list(e)
+        assert any(np.isfinite(want))
+
+
 def test_friction_matches_single_scene_contexts(setup):
     scenes, batch, plain = setup
     kap = {k: 10.0 ** (3 + k) for k in range(len(scenes))}
