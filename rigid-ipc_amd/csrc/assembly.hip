@@ -813,6 +813,9 @@ unsigned grid_for(long long n, int threads = THREADS) { return (unsigned)((n + t
 // that W (and the frames) already hold its values for every body.
 hipError_t world_vertices(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses, DevBuf& W,
                           bool frames, hipStream_t st) {
+    // W over every body brings the frames along: a gradient at the same
+    // poses (the usual next call) then needs no second launch
+    frames = frames || (sel.all && &W == &as.W);
     unsigned long long t = 0;
     for (int k = 0; k < 3; k++)
         if (as.tag_ptr[k] == d_poses) t = as.tag[k];

@@ -97,23 +97,30 @@ __global__ __launch_bounds__(BP_THREADS) void body_box_kernel(const int* bodies,
 }
 
 // Body i is included when its box overlaps the box of a body of another
-// group in its scene. One 64-lane block per upper-triangular tile of a
-// scene's body-pair grid; both bodies of an overlapping pair are marked.
+// group in its scene. One block per upper-triangular tile of a scene's
+// body-pair grid; both bodies of an overlapping pair are marked.
 // inc[] is cleared beforehand and only ever set to 1.
 constexpr int PAIR_TILE = 64;
+// Threads per tile: each of the four wavefronts takes a quarter of the
+// tile's columns for all 64 rows (lane = row).
+constexpr int TILE_THREADS = 256;
+constexpr int TILE_COLS = PAIR_TILE / (TILE_THREADS / 64);
 
-__global__ __launch_bounds__(PAIR_TILE) void included_kernel(const int4* tiles, const int* group,
-                                                            const double* box_inf, int* inc) {
+__global__ __launch_bounds__(TILE_THREADS) void included_kernel(const int4* tiles, const int* group,
+                                                               const double* box_inf, int* inc) {
     __shared__ double tb[PAIR_TILE][6];
     __shared__ int tg[PAIR_TILE];
     const int4 T = tiles[blockIdx.x];
     const int end = T.z;
-    const int i = T.x + threadIdx.x;
+    const int row = threadIdx.x & 63, quarter = threadIdx.x >> 6;
+    const int i = T.x + row;
     const int j0 = T.y;
-    const int jt = j0 + threadIdx.x;
-    if (jt < end) {
-        for (int k = 0; k < 6; k++) tb[threadIdx.x][k] = box_inf[6 * jt + k];
-        tg[threadIdx.x] = group[jt];
+    if (threadIdx.x < PAIR_TILE) {
+        const int jt = j0 + threadIdx.x;
+        if (jt < end) {
+            for (int k = 0; k < 6; k++) tb[threadIdx.x][k] = box_inf[6 * jt + k];
+            tg[threadIdx.x] = group[jt];
+        }
     }
     __syncthreads();
     if (i >= end) return;
@@ -122,11 +129,15 @@ __global__ __launch_bounds__(PAIR_TILE) void included_kernel(const int4* tiles, 
     const int gi = group[i];
     const int m = min(PAIR_TILE, end - j0);
     bool hit_i = false;
-    for (int jj = 0; jj < m; jj++) {
+    for (int jj = quarter * TILE_COLS; jj < min(m, (quarter + 1) * TILE_COLS); jj++) {
         const int j = j0 + jj;
-        if (j <= i || tg[jj] == gi) continue;
+        // all six bounds are read before any is compared (no chain of
+        // dependent LDS reads): bitwise &, not &&
         const double* o = tb[jj];
-        if (mi[0] <= o[3] && o[0] <= mi[3] && mi[1] <= o[4] && o[1] <= mi[4] && mi[2] <= o[5] && o[2] <= mi[5]) {
+        const double o0 = o[0], o1 = o[1], o2 = o[2], o3 = o[3], o4 = o[4], o5 = o[5];
+        const bool hit = (j > i) & (tg[jj] != gi) & (mi[0] <= o3) & (o0 <= mi[3]) & (mi[1] <= o4) & (o1 <= mi[4]) &
+                         (mi[2] <= o5) & (o2 <= mi[5]);
+        if (hit) {
             hit_i = true;
             inc[j] = 1;
         }
@@ -335,26 +346,28 @@ __device__ __forceinline__ double hull_eps(const double* h) {
     return 1e-12 * m;
 }
 
-// Close body pairs (a < b) of one scene: one 64-lane block per
-// upper-triangular tile; hits appended with one atomic per wavefront.
-__global__ __launch_bounds__(PAIR_TILE) void body_pair_kernel(const int4* tiles, const int* inc, const int* group,
-                                                             const double* bhull, int* pairs, long long cap,
-                                                             unsigned long long* count) {
+// Close body pairs (a < b) of one scene: one block per upper-triangular
+// tile; hits appended with one atomic per wavefront.
+__global__ __launch_bounds__(TILE_THREADS) void body_pair_kernel(const int4* tiles, const int* inc,
+                                                                const int* group, const double* bhull, int* pairs,
+                                                                long long cap, unsigned long long* count) {
     __shared__ double th[PAIR_TILE][6];
     __shared__ int tg[PAIR_TILE], ti[PAIR_TILE];
     const int4 T = tiles[blockIdx.x];
     const int end = T.z;
-    const int a = T.x + threadIdx.x;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, quarter = threadIdx.x >> 6;
+    const int a = T.x + lane;
     const int b0 = T.y;
-    const int bb = b0 + threadIdx.x;
-    if (bb < end) {
-        ti[threadIdx.x] = inc[bb];
-        tg[threadIdx.x] = group[bb];
-        for (int k = 0; k < 6; k++) th[threadIdx.x][k] = bhull[6 * bb + k];
+    if (threadIdx.x < PAIR_TILE) {
+        const int bb = b0 + threadIdx.x;
+        if (bb < end) {
+            ti[threadIdx.x] = inc[bb];
+            tg[threadIdx.x] = group[bb];
+            for (int k = 0; k < 6; k++) th[threadIdx.x][k] = bhull[6 * bb + k];
+        }
     }
     __syncthreads();
-    double ha[6];
+    double ha[6] = { 0, 0, 0, 0, 0, 0 };
     int ga = 0;
     bool ia = false;
     double ea = 0;
@@ -365,15 +378,16 @@ __global__ __launch_bounds__(PAIR_TILE) void body_pair_kernel(const int4* tiles,
         if (ia) ea = hull_eps(ha);
     }
     const int m = min(PAIR_TILE, end - b0);
-    for (int j = 0; j < m; j++) {
+    for (int j = quarter * TILE_COLS; j < min(m, (quarter + 1) * TILE_COLS); j++) {
         const int b = b0 + j;
-        bool hit = false;
-        if (ia && b > a && ti[j] && tg[j] != ga) {
-            const double* hb = th[j];
-            const double eps = fmax(ea, hull_eps(hb));
-            hit = ha[0] <= hb[3] + eps && hb[0] <= ha[3] + eps && ha[1] <= hb[4] + eps && hb[1] <= ha[4] + eps
-                && ha[2] <= hb[5] + eps && hb[2] <= ha[5] + eps;
-        }
+        const double* th_j = th[j];
+        double hb[6];
+        for (int k = 0; k < 6; k++) hb[k] = th_j[k];
+        const double eps = fmax(ea, hull_eps(hb));
+        // bitwise &: the bounds are read together, not one after another
+        const bool hit = ia & (b > a) & (ti[j] != 0) & (tg[j] != ga) & (ha[0] <= hb[3] + eps) &
+                         (hb[0] <= ha[3] + eps) & (ha[1] <= hb[4] + eps) & (hb[1] <= ha[4] + eps) &
+                         (ha[2] <= hb[5] + eps) & (hb[2] <= ha[5] + eps);
         const unsigned long long mask = __ballot(hit);
         if (mask) {
             const int leader = __ffsll((long long)mask) - 1;
@@ -391,14 +405,16 @@ __global__ __launch_bounds__(PAIR_TILE) void body_pair_kernel(const int4* tiles,
     }
 }
 
+// Bitwise & in these tests: every operand is loaded up front instead of
+// each load waiting on the previous comparison.
 __device__ __forceinline__ bool aabb_overlap(const double* ca, const double* ha, const double* cb, const double* hb) {
-    return fabs(ca[0] - cb[0]) <= ha[0] + hb[0] && fabs(ca[1] - cb[1]) <= ha[1] + hb[1]
-        && fabs(ca[2] - cb[2]) <= ha[2] + hb[2];
+    return (fabs(ca[0] - cb[0]) <= ha[0] + hb[0]) & (fabs(ca[1] - cb[1]) <= ha[1] + hb[1]) &
+           (fabs(ca[2] - cb[2]) <= ha[2] + hb[2]);
 }
 
 __device__ __forceinline__ bool reaches(const double* c, const double* h, const double* hull, double eps) {
-    return c[0] + h[0] >= hull[0] - eps && c[0] - h[0] <= hull[3] + eps && c[1] + h[1] >= hull[1] - eps
-        && c[1] - h[1] <= hull[4] + eps && c[2] + h[2] >= hull[2] - eps && c[2] - h[2] <= hull[5] + eps;
+    return (c[0] + h[0] >= hull[0] - eps) & (c[0] - h[0] <= hull[3] + eps) & (c[1] + h[1] >= hull[1] - eps) &
+           (c[1] - h[1] <= hull[4] + eps) & (c[2] + h[2] >= hull[2] - eps) & (c[2] - h[2] <= hull[5] + eps);
 }
 
 // Filtered element list of one body held in LDS, or the body's whole range
@@ -553,7 +569,7 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& s
     hipLaunchKernelGGL(body_box_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.V, sc.body_vstart,
                        sc.is_static, sc.r_max, d_p0, p1, r, bp.box_raw.as<double>(), bp.box_inf.as<double>());
     BP_CHECK(hipMemsetAsync(bp.inc.p, 0, sizeof(int) * nb, st));
-    hipLaunchKernelGGL(included_kernel, dim3(ntiles), dim3(PAIR_TILE), 0, st, sel.tiles, sc.group,
+    hipLaunchKernelGGL(included_kernel, dim3(ntiles), dim3(TILE_THREADS), 0, st, sel.tiles, sc.group,
                        bp.box_inf.as<double>(), bp.inc.as<int>());
     hipLaunchKernelGGL(domain_kernel, dim3((unsigned)sel.n_scenes), dim3(BP_THREADS), 0, st, sel.scenes,
                        sc.scene_body, bp.inc.as<int>(), bp.box_raw.as<double>(), r, bp.dom.as<double>());
@@ -587,7 +603,7 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& s
         BP_CHECK(bp.keys_ee.ensure(8 * cap_ee));
         BP_CHECK(bp.keys_fv.ensure(8 * cap_fv));
         BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(body_pair_kernel, dim3(ntiles), dim3(PAIR_TILE), 0, st, sel.tiles, bp.inc.as<int>(),
+        hipLaunchKernelGGL(body_pair_kernel, dim3(ntiles), dim3(TILE_THREADS), 0, st, sel.tiles, bp.inc.as<int>(),
                            sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt + 2);
         const unsigned grid = (unsigned)std::min<long long>(cap_pairs, 2048);
         hipLaunchKernelGGL(element_pair_kernel, dim3(grid), dim3(BP_THREADS), 0, st, bp.pairs.as<int>(),
