@@ -11,7 +11,7 @@
 //      every vertex trajectory, with the forced split of t at 1/2 and the
 //      adaptive bisection of any t-piece whose enclosure leaves the domain
 //      (compute_vertices_intervals, :164-246), one workgroup per body with
-//      an LDS stack of t-pieces; for discrete detection the world vertex
+//      an LDS stack of t-pieces taken in batches; for discrete detection the world vertex
 //      inflated by r;
 //   5. element AABBs (vertex / edge / face) in the centre / half-extent form
 //      used by the overlap test;
@@ -174,18 +174,30 @@ __global__ __launch_bounds__(BP_THREADS) void domain_kernel(const int* scenes, c
     }
 }
 
-// Interval enclosure of every vertex trajectory of an included body.
+// Interval enclosure of every vertex trajectory of an included body. The
+// t-pieces still to examine are kept on an LDS stack and taken in batches
+// of up to 64: thread k computes the interval pose of piece k, then every
+// thread checks its vertices against the domain under each piece of the
+// batch. A piece that fits (or is deeper than level 40) adds its vertex
+// intervals; one that does not (or the forced root) is split in two. Each
+// piece's outcome depends on that piece alone, so the kept pieces, and the
+// enclosure (a min / max over them), are those of the reference's
+// depth-first recursion. A batch is a single piece whenever the stack is
+// within VI_ROOM of full: one piece adds at most one entry per level
+// below it, so the stack never overflows.
+constexpr int VI_STACK = 512, VI_BATCH = 64, VI_ROOM = 2 * VI_BATCH;
+
 __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* bodies, const int* body_scene,
                                                                     const double* V, const int* bvs, const int* inc,
                                                                     const double* p0, const double* p1,
                                                                     const double* dom_all, double r, double* vmin,
                                                                     double* vmax, double* bhull) {
-    __shared__ unsigned long long s_idx[64];
-    __shared__ int s_lev[64], s_force[64];
-    __shared__ int sp;
-    __shared__ unsigned long long c_idx;
-    __shared__ int c_lev, c_force;
-    __shared__ Iv Rs[3][3], ps[3];
+    __shared__ unsigned long long s_idx[VI_STACK];
+    __shared__ signed char s_lev[VI_STACK], s_force[VI_STACK];
+    __shared__ int sp, nbatch;
+    __shared__ unsigned long long c_idx[VI_BATCH];
+    __shared__ int c_lev[VI_BATCH], c_force[VI_BATCH], c_bad[VI_BATCH];
+    __shared__ Iv Rs[VI_BATCH][3][3], ps[VI_BATCH][3];
     __shared__ double red[6][BP_THREADS];
     const int b = bodies[blockIdx.x], tid = threadIdx.x;
     if (!inc[b]) return;
@@ -205,56 +217,78 @@ __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* 
         sp = 1;
     }
     __syncthreads();
-    for (;;) {
-        if (sp == 0) break;
-        __syncthreads();
-        if (tid == 0) {
-            sp--;
-            c_idx = s_idx[sp]; c_lev = s_lev[sp]; c_force = s_force[sp];
-        }
-        __syncthreads();
-        const unsigned long long idx = c_idx;
-        const int lev = c_lev, force = c_force;
-        const bool evaluate = force <= 0 || lev > 40;
-        bool keep = false;
-        if (evaluate) {
-            if (tid == 0) {
+    while (sp > 0) {
+        // take the top k pieces
+        const int top = sp;
+        const int k = (top + VI_BATCH <= VI_STACK - VI_ROOM) ? min(top, VI_BATCH) : 1;
+        if (tid < k) {
+            c_idx[tid] = s_idx[top - 1 - tid];
+            c_lev[tid] = s_lev[top - 1 - tid];
+            c_force[tid] = s_force[top - 1 - tid];
+            c_bad[tid] = 0;
+            const bool evaluate = c_force[tid] <= 0 || c_lev[tid] > 40;
+            if (evaluate) {
+                const unsigned long long idx = c_idx[tid];
+                const int lev = c_lev[tid];
                 const Iv t = Iv{ ldexp((double)idx, -lev), ldexp((double)(idx + 1), -lev) };
                 Iv p[3], R[3][3];
                 pose_iv(q0, q1, t, p, R);
                 for (int i = 0; i < 3; i++) {
-                    ps[i] = p[i];
-                    for (int j = 0; j < 3; j++) Rs[i][j] = R[i][j];
+                    ps[tid][i] = p[i];
+                    for (int j = 0; j < 3; j++) Rs[tid][i][j] = R[i][j];
                 }
             }
-            __syncthreads();
-            Iv R[3][3], p[3];
-            for (int i = 0; i < 3; i++) {
-                p[i] = ps[i];
-                for (int j = 0; j < 3; j++) R[i][j] = Rs[i][j];
-            }
-            bool bad = false;
-            for (int i = v0 + tid; i < v1; i += BP_THREADS) {
+        }
+        if (tid == 0) nbatch = k;
+        __syncthreads();
+        const int nb = nbatch;
+        // domain check of every evaluated piece
+        for (int i = v0 + tid; i < v1; i += BP_THREADS)
+            for (int c = 0; c < nb; c++) {
+                if (!(c_force[c] <= 0 || c_lev[c] > 40) || c_bad[c]) continue;
                 Iv w[3];
-                world_vertex_iv(R, p, V + 3 * i, w);
-                for (int k = 0; k < 3; k++) bad = bad || (w[k].lo - r < dmn[k]) || (w[k].hi + r > dmx[k]);
+                world_vertex_iv(Rs[c], ps[c], V + 3 * i, w);
+                bool bad = false;
+                for (int d = 0; d < 3; d++) bad = bad || (w[d].lo - r < dmn[d]) || (w[d].hi + r > dmx[d]);
+                if (bad) c_bad[c] = 1;
             }
-            const bool fits = !__syncthreads_or(bad);
-            keep = fits || lev > 40;
-            if (keep)
-                for (int i = v0 + tid; i < v1; i += BP_THREADS) {
-                    Iv w[3];
-                    world_vertex_iv(R, p, V + 3 * i, w);
-                    for (int k = 0; k < 3; k++) {
-                        vmin[3 * i + k] = fmin(vmin[3 * i + k], w[k].lo);
-                        vmax[3 * i + k] = fmax(vmax[3 * i + k], w[k].hi);
-                    }
+        __syncthreads();
+        // kept pieces add their vertex intervals
+        for (int i = v0 + tid; i < v1; i += BP_THREADS) {
+            double mn[3] = { vmin[3 * i], vmin[3 * i + 1], vmin[3 * i + 2] };
+            double mx[3] = { vmax[3 * i], vmax[3 * i + 1], vmax[3 * i + 2] };
+            bool any = false;
+            for (int c = 0; c < nb; c++) {
+                const bool evaluate = c_force[c] <= 0 || c_lev[c] > 40;
+                if (!evaluate || !(c_bad[c] == 0 || c_lev[c] > 40)) continue;
+                Iv w[3];
+                world_vertex_iv(Rs[c], ps[c], V + 3 * i, w);
+                for (int d = 0; d < 3; d++) {
+                    mn[d] = fmin(mn[d], w[d].lo);
+                    mx[d] = fmax(mx[d], w[d].hi);
+                }
+                any = true;
+            }
+            if (any)
+                for (int d = 0; d < 3; d++) {
+                    vmin[3 * i + d] = mn[d];
+                    vmax[3 * i + d] = mx[d];
                 }
         }
-        if (!keep && tid == 0 && sp + 2 <= 64) {
-            s_idx[sp] = 2 * idx + 1; s_lev[sp] = lev + 1; s_force[sp] = force - 1;
-            s_idx[sp + 1] = 2 * idx; s_lev[sp + 1] = lev + 1; s_force[sp + 1] = force - 1;
-            sp += 2;
+        // the others are split: children pushed in order, second half first
+        if (tid == 0) {
+            int s = top - nb;
+            for (int c = nb - 1; c >= 0; c--) {
+                const bool evaluate = c_force[c] <= 0 || c_lev[c] > 40;
+                const bool keep = evaluate && (c_bad[c] == 0 || c_lev[c] > 40);
+                if (keep) continue;
+                s_idx[s] = 2 * c_idx[c] + 1; s_lev[s] = (signed char)(c_lev[c] + 1);
+                s_force[s] = (signed char)(c_force[c] - 1);
+                s_idx[s + 1] = 2 * c_idx[c]; s_lev[s + 1] = (signed char)(c_lev[c] + 1);
+                s_force[s + 1] = (signed char)(c_force[c] - 1);
+                s += 2;
+            }
+            sp = s;
         }
         __syncthreads();
     }
