@@ -9,7 +9,8 @@
 //   compute_minimum_distance   src/opt/distance_barrier_constraint.cpp:308-323
 //
 // Pipeline (one HIP stream):
-//   body_frame_kernel      per body: R(r), dR/dr, d2R/dr2 (closed form)
+//   body_rotation_kernel   per body: R(r)
+//   body_frame_kernel      per body: dR/dr, d2R/dr2 (closed form), nine threads each
 //   world_vertex_kernel    per vertex: R x + p (round-to-nearest, as the host)
 //   static broad phase     (broadphase.hip) with inflation (dhat + dmin) / 2
 //   classify_kernel        per candidate: closest-feature type, activity
@@ -50,11 +51,38 @@ int bits_for(unsigned long long maxval) {
 }
 
 // --------------------------------------------------------------- frames
-__global__ __launch_bounds__(THREADS) void body_frame_kernel(int nb, const double* poses, double* R, BodyFrame* F) {
+// Rotations of all bodies (one thread each) and the frames: nine threads
+// per body, each computing the shared terms and the entries q, q + 9, ...
+// of the 81 (q = blockIdx.y, the same for a whole block, so each entry is
+// taken with constant indices and no branch diverges).
+__global__ __launch_bounds__(THREADS) void body_rotation_kernel(int nb, const double* poses, double* R) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) rotation_d(poses + 6 * b + 3, R + 9 * b);
+}
+
+template <int Q>
+__device__ __forceinline__ void frame_entries(const FrameTerms& T, double* out) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) out[Q + 9 * k] = frame_entry(T, Q + 9 * k);
+}
+
+__global__ __launch_bounds__(THREADS) void body_frame_kernel(int nb, const double* poses, BodyFrame* F) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    rotation_d(poses + 6 * b + 3, R + 9 * b);
-    if (F) body_frame(poses + 6 * b + 3, F[b]);
+    FrameTerms T;
+    frame_terms(poses + 6 * b + 3, T);
+    double* out = reinterpret_cast<double*>(F + b);  // dR[3][9] then d2R[6][9]
+    switch (blockIdx.y) {
+        case 0: frame_entries<0>(T, out); break;
+        case 1: frame_entries<1>(T, out); break;
+        case 2: frame_entries<2>(T, out); break;
+        case 3: frame_entries<3>(T, out); break;
+        case 4: frame_entries<4>(T, out); break;
+        case 5: frame_entries<5>(T, out); break;
+        case 6: frame_entries<6>(T, out); break;
+        case 7: frame_entries<7>(T, out); break;
+        default: frame_entries<8>(T, out); break;
+    }
 }
 
 __global__ __launch_bounds__(THREADS) void world_vertex_kernel(int nv, const double* V, const int* v_body,
@@ -827,8 +855,11 @@ hipError_t world_vertices(Assembly& as, const DevScene& sc, const SceneSel& sel,
     AS_CHECK(as.frames.ensure(sizeof(BodyFrame) * sc.nb));
     AS_CHECK(W.ensure(3 * sizeof(double) * std::max(sc.nv, 1)));
     if (sel.all) {
-        hipLaunchKernelGGL(body_frame_kernel, dim3(grid_for(sc.nb)), dim3(THREADS), 0, st, sc.nb, d_poses,
-                           as.R.as<double>(), frames ? as.frames.as<BodyFrame>() : nullptr);
+        hipLaunchKernelGGL(body_rotation_kernel, dim3(grid_for(sc.nb)), dim3(THREADS), 0, st, sc.nb, d_poses,
+                           as.R.as<double>());
+        if (frames)
+            hipLaunchKernelGGL(body_frame_kernel, dim3(grid_for(sc.nb), 9), dim3(THREADS), 0, st, sc.nb, d_poses,
+                               as.frames.as<BodyFrame>());
         if (sc.nv)
             hipLaunchKernelGGL(world_vertex_kernel, dim3(grid_for(sc.nv)), dim3(THREADS), 0, st, sc.nv, sc.V,
                                sc.v_body, d_poses, as.R.as<double>(), W.as<double>());

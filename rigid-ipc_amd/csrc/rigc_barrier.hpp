@@ -622,55 +622,85 @@ RIGC_HD int sym_index(int c, int d) {
     return c == 0 ? d : (c == 1 ? 2 + d : 5);
 }
 
-RIGC_HD void body_frame(const double* r, BodyFrame& F) {
+// The scalars and small tensors every entry of dR / d2R is built from.
+struct FrameTerms {
+    double r[3], a, b, rr, ga[3], gb[3], Ha[3][3], Hb[3][3];
+};
+
+RIGC_HD void frame_terms(const double* r, FrameTerms& T) {
+    for (int c = 0; c < 3; c++) T.r[c] = r[c];
     const double th = sqrt((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
     const double a = geom_sinc(th), ds = dsinc_over_x(th), dd = ddsinc_term(th);
     const double rh[3] = { r[0] / 2.0, r[1] / 2.0, r[2] / 2.0 };
     const double thh = sqrt((rh[0] * rh[0] + rh[1] * rh[1]) + rh[2] * rh[2]);
     const double h = geom_sinc(thh), dsh = dsinc_over_x(thh), ddh = ddsinc_term(thh);
-    double ga[3], gh[3], gb[3], Ha[3][3], Hb[3][3];
+    double gh[3];
     for (int c = 0; c < 3; c++) {
-        ga[c] = ds * r[c];
+        T.ga[c] = ds * r[c];
         gh[c] = dsh * rh[c] * 0.5;
     }
-    for (int c = 0; c < 3; c++) gb[c] = h * gh[c];
+    for (int c = 0; c < 3; c++) T.gb[c] = h * gh[c];
     for (int c = 0; c < 3; c++)
         for (int d = 0; d < 3; d++) {
             const double dl = (c == d) ? 1.0 : 0.0;
-            Ha[c][d] = dd * r[c] * r[d] + ds * dl;
+            T.Ha[c][d] = dd * r[c] * r[d] + ds * dl;
             const double Hh = (ddh * rh[c] * rh[d] + dsh * dl) * 0.25;
-            Hb[c][d] = gh[c] * gh[d] + h * Hh;
+            T.Hb[c][d] = gh[c] * gh[d] + h * Hh;
         }
-    const double b = (0.5 * h) * h;
+    T.a = a;
+    T.b = (0.5 * h) * h;
+    T.rr = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+}
+
+RIGC_HD double skew_entry(const double* r, int i, int j) {
     const double K[3][3] = { { 0.0, -r[2], r[1] }, { r[2], 0.0, -r[0] }, { -r[1], r[0], 0.0 } };
-    const double rr = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2];
+    return K[i][j];
+}
+
+// dR/dr_c, entry (i, j)
+RIGC_HD double frame_dR(const FrameTerms& T, int c, int i, int j) {
+    const double* r = T.r;
+    const double dl = (i == j) ? 1.0 : 0.0;
+    const double K2 = r[i] * r[j] - T.rr * dl;
+    const double Ec = v3_skew_unit(c, i, j);
+    // E_c K + K E_c = r e_c^T + e_c r^T - 2 r_c I
+    const double EK = (j == c ? r[i] : 0.0) + (i == c ? r[j] : 0.0) - 2.0 * r[c] * dl;
+    return T.ga[c] * skew_entry(r, i, j) + T.a * Ec + T.gb[c] * K2 + T.b * EK;
+}
+
+// d2R/dr_c dr_d, entry (i, j)
+RIGC_HD double frame_d2R(const FrameTerms& T, int c, int d, int i, int j) {
+    const double* r = T.r;
+    const double dcd = (c == d) ? 1.0 : 0.0;
+    const double dl = (i == j) ? 1.0 : 0.0;
+    const double K2 = r[i] * r[j] - T.rr * dl;
+    const double Ec = v3_skew_unit(c, i, j), Ed = v3_skew_unit(d, i, j);
+    const double EcK = (j == c ? r[i] : 0.0) + (i == c ? r[j] : 0.0) - 2.0 * r[c] * dl;
+    const double EdK = (j == d ? r[i] : 0.0) + (i == d ? r[j] : 0.0) - 2.0 * r[d] * dl;
+    // E_c E_d + E_d E_c = e_c e_d^T + e_d e_c^T - 2 delta_cd I
+    const double EE = (i == c && j == d ? 1.0 : 0.0) + (i == d && j == c ? 1.0 : 0.0) - 2.0 * dcd * dl;
+    return T.Ha[c][d] * skew_entry(r, i, j) + T.ga[c] * Ed + T.ga[d] * Ec + T.Hb[c][d] * K2 + T.gb[c] * EdK +
+           T.gb[d] * EcK + T.b * EE;
+}
+
+// Entry e of the frame: e < 27 is dR[e / 9][e % 9], else d2R[(e - 27) / 9][(e - 27) % 9]
+// (symmetric pair index s = sym_index(c, d)).
+RIGC_HD double frame_entry(const FrameTerms& T, int e) {
+    if (e < 27) return frame_dR(T, e / 9, (e % 9) / 3, e % 3);
+    const int s = (e - 27) / 9, k = (e - 27) % 9;
+    // s -> (c, d): 0 (0,0), 1 (0,1), 2 (0,2), 3 (1,1), 4 (1,2), 5 (2,2)
+    const int c = s < 3 ? 0 : (s < 5 ? 1 : 2), d = s < 3 ? s : (s < 5 ? s - 2 : 2);
+    return frame_d2R(T, c, d, k / 3, k % 3);
+}
+
+RIGC_HD void body_frame(const double* r, BodyFrame& F) {
+    FrameTerms T;
+    frame_terms(r, T);
     for (int c = 0; c < 3; c++)
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                const double dl = (i == j) ? 1.0 : 0.0;
-                const double K2 = r[i] * r[j] - rr * dl;
-                const double Ec = v3_skew_unit(c, i, j);
-                // E_c K + K E_c = r e_c^T + e_c r^T - 2 r_c I
-                const double EK = (j == c ? r[i] : 0.0) + (i == c ? r[j] : 0.0) - 2.0 * r[c] * dl;
-                F.dR[c][3 * i + j] = ga[c] * K[i][j] + a * Ec + gb[c] * K2 + b * EK;
-            }
+        for (int k = 0; k < 9; k++) F.dR[c][k] = frame_dR(T, c, k / 3, k % 3);
     for (int c = 0; c < 3; c++)
-        for (int d = c; d < 3; d++) {
-            const int s = sym_index(c, d);
-            const double dcd = (c == d) ? 1.0 : 0.0;
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) {
-                    const double dl = (i == j) ? 1.0 : 0.0;
-                    const double K2 = r[i] * r[j] - rr * dl;
-                    const double Ec = v3_skew_unit(c, i, j), Ed = v3_skew_unit(d, i, j);
-                    const double EcK = (j == c ? r[i] : 0.0) + (i == c ? r[j] : 0.0) - 2.0 * r[c] * dl;
-                    const double EdK = (j == d ? r[i] : 0.0) + (i == d ? r[j] : 0.0) - 2.0 * r[d] * dl;
-                    // E_c E_d + E_d E_c = e_c e_d^T + e_d e_c^T - 2 delta_cd I
-                    const double EE = (i == c && j == d ? 1.0 : 0.0) + (i == d && j == c ? 1.0 : 0.0) - 2.0 * dcd * dl;
-                    F.d2R[s][3 * i + j] = Ha[c][d] * K[i][j] + ga[c] * Ed + ga[d] * Ec + Hb[c][d] * K2
-                        + gb[c] * EdK + gb[d] * EcK + b * EE;
-                }
-        }
+        for (int d = c; d < 3; d++)
+            for (int k = 0; k < 9; k++) F.d2R[sym_index(c, d)][k] = frame_d2R(T, c, d, k / 3, k % 3);
 }
 
 }  // namespace rigc
