@@ -190,6 +190,9 @@ def bench_sim(args):
             contact_ms_per_step=st["contact_ms"] / args.steps, host_ms_per_step=st["host_ms"] / args.steps,
             contact_ms_by_call={k: round(st[k + "_ms"] / args.steps, 3)
                                 for k in ("ccd", "constraint", "barrier", "friction", "distance")},
+            ccd_device_ms_per_step={"broad_phase": round(st["ccd_broad_phase_ms"] / args.steps, 3),
+                                    "narrow_phase": round(st["ccd_narrow_phase_ms"] / args.steps, 3),
+                                    "ccd_kernel": round(st["ccd_kernel_ms"] / args.steps, 3)},
             ccd_candidate_pairs_per_s=(st["ccd_candidates"] / (ccd_dev_ms * 1e-3)) if ccd_dev_ms > 0 else None,
             ccd_candidate_pairs_per_s_wall=(st["ccd_candidates"] / (st["ccd_ms"] * 1e-3)) if st["ccd_ms"] > 0 else None,
             ccd_candidates=int(st["ccd_candidates"]), step_ms=[round(1e3 * t, 3) for t in per_step]),

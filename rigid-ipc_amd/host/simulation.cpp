@@ -153,10 +153,19 @@ struct rigs_sim {
     std::vector<double> poses_t0;      // start of the current step
     std::vector<double> cached_poses;  // poses of the backend's constraint set
     bool cache_valid = false;
-    // backend output staging
-    std::vector<double> hdiag, hoff;
-    std::vector<int32_t> hpairs;
-    int64_t hcap = 0;
+    // backend output staging (one per term: the friction term may run on a
+    // helper thread beside the barrier term)
+    struct Staging {
+        std::vector<double> hdiag, hoff;
+        std::vector<int32_t> hpairs;
+        int64_t hcap = 0;
+    };
+    Staging stg_barrier, stg_friction;
+    // The friction term's backend: with a context of the stepper's own, a
+    // second context (its own stream, its own copy of the lagged friction
+    // set) so that friction runs beside the barrier term; else the backend.
+    rigc_ctx* fric_ctx = nullptr;
+    rigs_backend fbe{};
     rigs_stats stats{};
     std::string err;
     int verbose = 0;  // RIGS_VERBOSE=1: one line per Newton iteration on stderr; 2: line-search trials too
@@ -188,41 +197,41 @@ struct rigs_sim {
         return RIGC_OK;
     }
 
-    // Calls a potential entry point, growing the off-diagonal capacity once.
+    // Calls a potential entry point, growing the off-diagonal capacity once;
+    // the time spent in the call is added to `spent`. Touches nothing shared
+    // but `S` (the friction term runs on a helper thread).
     template <typename F>
-    rigc_status potential(F&& call, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
-                          const char* what, double& kind_ms) {
+    rigc_status potential(F&& call, Staging& S, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
+                          double& spent) {
         if (g) g->assign(6 * nb, 0.0);
         rigc_block_hessian h{};
         for (int attempt = 0; attempt < 2; attempt++) {
             if (H) {
-                hdiag.resize(36 * (size_t)nb);
-                hpairs.resize(2 * (size_t)std::max<int64_t>(hcap, 1));
-                hoff.resize(36 * (size_t)std::max<int64_t>(hcap, 1));
-                h.diag = hdiag.data();
-                h.pairs = hpairs.data();
-                h.off = hoff.data();
-                h.capacity = hcap;
+                S.hdiag.resize(36 * (size_t)nb);
+                S.hpairs.resize(2 * (size_t)std::max<int64_t>(S.hcap, 1));
+                S.hoff.resize(36 * (size_t)std::max<int64_t>(S.hcap, 1));
+                h.diag = S.hdiag.data();
+                h.pairs = S.hpairs.data();
+                h.off = S.hoff.data();
+                h.capacity = S.hcap;
             }
             const double t0 = now_ms();
             rigc_status s = call(flags, &e, g ? g->data() : nullptr, H ? &h : nullptr);
-            const double dt = now_ms() - t0;
-            stats.contact_ms += dt;
-            kind_ms += dt;
-            if (s == RIGC_ERR_CAPACITY && H && h.count > hcap) {
-                hcap = h.count;
+            spent += now_ms() - t0;
+            if (s == RIGC_ERR_CAPACITY && H && h.count > S.hcap) {
+                S.hcap = h.count;
                 continue;
             }
-            if (s != RIGC_OK) return backend_fail(s, what);
+            if (s != RIGC_OK) return s;
             break;
         }
         if (H) {
             H->reset(nb);
-            for (int b = 0; b < nb; b++) std::copy(&hdiag[36 * b], &hdiag[36 * b] + 36, H->diag[b].begin());
+            for (int b = 0; b < nb; b++) std::copy(&S.hdiag[36 * b], &S.hdiag[36 * b] + 36, H->diag[b].begin());
             for (int64_t k = 0; k < h.count; k++) {
                 Block blk;
-                std::copy(&hoff[36 * k], &hoff[36 * k] + 36, blk.begin());
-                H->off[{ hpairs[2 * k], hpairs[2 * k + 1] }] = blk;
+                std::copy(&S.hoff[36 * k], &S.hoff[36 * k] + 36, blk.begin());
+                H->off[{ S.hpairs[2 * k], S.hpairs[2 * k + 1] }] = blk;
             }
         }
         return RIGC_OK;
@@ -276,37 +285,68 @@ struct rigs_sim {
         auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
             return be.compute_barrier_term(be.self, x, P.dhat, P.minimum_separation_distance, f, pe, pg, ph);
         };
+        double spent = 0;
         auto eval = [&](int32_t f, double& pe, std::vector<double>* pg, BlockHess* pH) {
-            return potential(call, f, pe, pg, pH, "compute_barrier_term", stats.barrier_ms);
+            return potential(call, stg_barrier, f, pe, pg, pH, spent);
         };
-        return cached_term(bcache, eval, x, flags, prefetch, e, g, H);
+        s = cached_term(bcache, eval, x, flags, prefetch, e, g, H);
+        stats.contact_ms += spent;
+        stats.barrier_ms += spent;
+        return s == RIGC_OK ? s : backend_fail(s, "compute_barrier_term");
+    }
+
+    // The friction term through fbe; touches only fcache, stg_friction, the
+    // outputs and `spent` / `msg`, so it may run on a helper thread.
+    rigc_status friction_eval(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
+                              bool prefetch, double& spent, std::string& msg) {
+        auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
+            return fbe.compute_friction_term(fbe.self, x, poses_t0.data(),
+                                             P.static_friction_speed_bound * P.timestep, f, pe, pg, ph);
+        };
+        auto eval = [&](int32_t f, double& pe, std::vector<double>* pg, BlockHess* pH) {
+            return potential(call, stg_friction, f, pe, pg, pH, spent);
+        };
+        rigc_status s = cached_term(fcache, eval, x, flags, prefetch, e, g, H);
+        if (s != RIGC_OK)
+            msg = std::string("compute_friction_term: ") + (fbe.last_error ? fbe.last_error(fbe.self) : "backend error");
+        return s;
     }
 
     rigc_status friction_term(const double* x, int32_t flags, double& e, std::vector<double>* g, BlockHess* H,
                               bool prefetch = false) {
-        auto call = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
-            return be.compute_friction_term(be.self, x, poses_t0.data(), P.static_friction_speed_bound * P.timestep,
-                                            f, pe, pg, ph);
-        };
-        auto eval = [&](int32_t f, double& pe, std::vector<double>* pg, BlockHess* pH) {
-            return potential(call, f, pe, pg, pH, "compute_friction_term", stats.friction_ms);
-        };
-        return cached_term(fcache, eval, x, flags, prefetch, e, g, H);
+        double spent = 0;
+        std::string msg;
+        rigc_status s = friction_eval(x, flags, e, g, H, prefetch, spent, msg);
+        stats.contact_ms += spent;
+        stats.friction_ms += spent;
+        if (s != RIGC_OK) err = msg;
+        return s;
     }
 
     rigc_status update_friction(const double* poses) {
         if (P.coefficient_friction <= 0) return RIGC_OK;
         fcache.valid = false;
-        rigc_status s = ensure_constraints(poses);
-        if (s != RIGC_OK) return s;
         int64_t n = 0;
+        rigc_status s = RIGC_OK;
         const double t0 = now_ms();
-        s = be.update_friction_constraints(be.self, poses, P.dhat, P.minimum_separation_distance, kappa,
-                                           P.coefficient_friction, 0, &n);
+        if (fric_ctx) {
+            // the friction context builds its own constraint set at poses
+            s = fbe.update_friction_constraints(fbe.self, poses, P.dhat, P.minimum_separation_distance, kappa,
+                                                P.coefficient_friction, RIGC_BUILD_CONSTRAINTS, &n);
+        } else {
+            s = ensure_constraints(poses);
+            if (s != RIGC_OK) return s;
+            s = be.update_friction_constraints(be.self, poses, P.dhat, P.minimum_separation_distance, kappa,
+                                               P.coefficient_friction, 0, &n);
+        }
         const double dt = now_ms() - t0;
         stats.contact_ms += dt;
         stats.friction_ms += dt;
-        if (s != RIGC_OK) return backend_fail(s, "update_friction_constraints");
+        if (s != RIGC_OK) {
+            err = std::string("update_friction_constraints: ") +
+                  (fbe.last_error ? fbe.last_error(fbe.self) : "backend error");
+            return s;
+        }
         return RIGC_OK;
     }
 
@@ -434,7 +474,7 @@ struct rigs_sim {
         BlockHess H;
     };
     EnergyCache ecache;
-    Helper helper;
+    Helper helper, fric_helper;
     bool overlap = true;
 
     rigc_status objective(const double* x, bool want_grad, bool want_hess, double& f, std::vector<double>& g,
@@ -468,9 +508,31 @@ struct rigs_sim {
         stats.host_ms += now_ms() - t0;
         const int32_t flags = (want_grad ? RIGC_GRADIENT : 0) | (want_hess ? RIGC_HESSIAN : 0);
         double B = 0, D = 0;
+        // with a friction context of its own, the trial point's friction
+        // term runs on a helper thread beside the constraint set and
+        // barrier term
+        const bool fric = P.coefficient_friction > 0;
+        const bool fric_beside = fric && prefetch && overlap && fric_ctx;
+        rigc_status fs = RIGC_OK;
+        double fspent = 0;
+        std::string fmsg;
+        if (fric_beside)
+            fric_helper.run([&] {
+                fs = friction_eval(x, flags, D, want_grad ? &gD : nullptr, want_hess ? &HD : nullptr, prefetch,
+                                   fspent, fmsg);
+            });
         rigc_status s = barrier_term(x, flags, B, want_grad ? &gB : nullptr, want_hess ? &HB : nullptr, prefetch);
-        if (s == RIGC_OK && P.coefficient_friction > 0)
+        if (fric_beside) {
+            fric_helper.wait();
+            stats.contact_ms += fspent;
+            stats.friction_ms += fspent;
+            if (s == RIGC_OK && fs != RIGC_OK) {
+                s = fs;
+                err = fmsg;
+            }
+        } else if (s == RIGC_OK && fric) {
             s = friction_term(x, flags, D, want_grad ? &gD : nullptr, want_hess ? &HD : nullptr, prefetch);
+        }
         if (pending) {
             const double tw = now_ms();
             helper.wait();
@@ -834,7 +896,13 @@ struct rigs_sim {
 namespace rigs {
 // The batch stepper runs every scene's stepper on its own thread already.
 void set_host_overlap(rigs_sim* sim, bool on) {
-    if (sim) sim->overlap = on;
+    if (!sim) return;
+    sim->overlap = on;
+    if (!on && sim->fric_ctx) {  // friction back on the main context
+        rigc_destroy(sim->fric_ctx);
+        sim->fric_ctx = nullptr;
+        sim->fbe = sim->be;
+    }
 }
 }  // namespace rigs
 
@@ -879,11 +947,23 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         sim->be = rigs_backend{ sim->own_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
                                 gb_fupdate, gb_friction, gb_error, gb_stats };
     }
+    sim->fbe = sim->be;
     rigc_status s = sim->be.set_scene(sim->be.self, scene);
     if (s != RIGC_OK) {
         if (sim->own_ctx) rigc_destroy(sim->own_ctx);
         delete sim;
         return s;
+    }
+    if (sim->own_ctx && params->coefficient_friction > 0) {
+        // a second context for the friction term (without one the friction
+        // term just runs through the main context)
+        if (rigc_create(&sim->fric_ctx, device) == RIGC_OK && rigc_set_scene(sim->fric_ctx, scene) == RIGC_OK) {
+            sim->fbe = rigs_backend{ sim->fric_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
+                                     gb_fupdate, gb_friction, gb_error, gb_stats };
+        } else if (sim->fric_ctx) {
+            rigc_destroy(sim->fric_ctx);
+            sim->fric_ctx = nullptr;
+        }
     }
     const int nb = scene->num_bodies;
     sim->nb = nb;
@@ -945,6 +1025,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
 
 rigc_status rigs_destroy(rigs_sim* sim) {
     if (!sim) return RIGC_OK;
+    if (sim->fric_ctx) rigc_destroy(sim->fric_ctx);
     if (sim->own_ctx) rigc_destroy(sim->own_ctx);
     delete sim;
     return RIGC_OK;
