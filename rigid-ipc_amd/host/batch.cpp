@@ -30,8 +30,10 @@
 
 namespace rigs {
 // host/simulation.cpp: whether a stepper computes the body energy on a
-// helper thread beside its device calls
+// helper thread beside its device calls, and the hooks its friction helper
+// calls around a call it makes beside the stepper's own
 void set_host_overlap(rigs_sim* sim, bool on);
+void set_fork_hooks(rigs_sim* sim, void (*join)(void*), void (*part)(void*), void* arg);
 }  // namespace rigs
 
 namespace {
@@ -230,7 +232,8 @@ rigc_status submit(Proxy* P, Request& r) {
     return r.status;
 }
 
-// A scene finished stepping (or failed): it no longer takes part in rounds.
+// A scene finished stepping (or failed), or one of its two callers is done
+// or waiting for the other: one caller fewer takes part in rounds.
 void leave(Batch& B) {
     std::unique_lock<std::mutex> lk(B.m);
     B.active--;
@@ -239,6 +242,16 @@ void leave(Batch& B) {
         B.cv.notify_all();
     }
 }
+
+// A stepper's friction helper is about to call beside the stepper's own
+// thread (or that thread resumes): one caller more, so that no round runs
+// without its request.
+void fork_join(void* arg) {
+    Batch& B = *static_cast<Batch*>(arg);
+    std::lock_guard<std::mutex> lk(B.m);
+    B.active++;
+}
+void fork_part(void* arg) { leave(*static_cast<Batch*>(arg)); }
 
 void Batch::execute() {
     const double t0 = now_ms();
@@ -448,8 +461,13 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
     for (int i = 0; i < num_scenes && s == RIGC_OK; i++) {
         s = rigs_create(&b->sims[i], mode == RIGS_BATCH_FUSED ? &b->tables[i] : nullptr, device, &scenes[i],
                         &bodies[i], &params[i]);
-        if (s != RIGC_OK) b->err = "rigs_create failed for scene " + std::to_string(i);
-        else rigs::set_host_overlap(b->sims[i], false);
+        if (s != RIGC_OK) {
+            b->err = "rigs_create failed for scene " + std::to_string(i);
+        } else {
+            rigs::set_host_overlap(b->sims[i], false);
+            // fused: a scene's friction term joins the round of its barrier term
+            if (mode == RIGS_BATCH_FUSED) rigs::set_fork_hooks(b->sims[i], fork_join, fork_part, &b->B);
+        }
     }
     if (s != RIGC_OK) {
         rigs_batch_destroy(b);

@@ -476,6 +476,13 @@ struct rigs_sim {
     EnergyCache ecache;
     Helper helper, fric_helper;
     bool overlap = true;
+    // A backend that serves several steppers' calls in rounds (the batch
+    // stepper) counts the threads that may call it: the friction helper
+    // joins (fork_join) before its call and parts (fork_part) after it, and
+    // the stepper's own thread parts while it waits for the helper.
+    void (*fork_join)(void*) = nullptr;
+    void (*fork_part)(void*) = nullptr;
+    void* fork_arg = nullptr;
 
     rigc_status objective(const double* x, bool want_grad, bool want_hess, double& f, std::vector<double>& g,
                           BlockHess& H, bool prefetch = false) {
@@ -512,18 +519,23 @@ struct rigs_sim {
         // term runs on a helper thread beside the constraint set and
         // barrier term
         const bool fric = P.coefficient_friction > 0;
-        const bool fric_beside = fric && prefetch && overlap && fric_ctx;
+        const bool fric_beside = fric && prefetch && ((overlap && fric_ctx) || fork_join);
         rigc_status fs = RIGC_OK;
         double fspent = 0;
         std::string fmsg;
-        if (fric_beside)
+        if (fric_beside) {
+            if (fork_join) fork_join(fork_arg);
             fric_helper.run([&] {
                 fs = friction_eval(x, flags, D, want_grad ? &gD : nullptr, want_hess ? &HD : nullptr, prefetch,
                                    fspent, fmsg);
+                if (fork_part) fork_part(fork_arg);
             });
+        }
         rigc_status s = barrier_term(x, flags, B, want_grad ? &gB : nullptr, want_hess ? &HB : nullptr, prefetch);
         if (fric_beside) {
+            if (fork_part) fork_part(fork_arg);
             fric_helper.wait();
+            if (fork_join) fork_join(fork_arg);
             stats.contact_ms += fspent;
             stats.friction_ms += fspent;
             if (s == RIGC_OK && fs != RIGC_OK) {
@@ -894,6 +906,13 @@ struct rigs_sim {
 };
 
 namespace rigs {
+void set_fork_hooks(rigs_sim* sim, void (*join)(void*), void (*part)(void*), void* arg) {
+    if (!sim) return;
+    sim->fork_join = join;
+    sim->fork_part = part;
+    sim->fork_arg = arg;
+}
+
 // The batch stepper runs every scene's stepper on its own thread already.
 void set_host_overlap(rigs_sim* sim, bool on) {
     if (!sim) return;
