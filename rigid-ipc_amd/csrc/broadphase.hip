@@ -58,10 +58,15 @@ __device__ __forceinline__ void block_minmax6(double* v, double (*red)[BP_THREAD
 __global__ __launch_bounds__(BP_THREADS) void body_box_kernel(const int* bodies, const double* V, const int* bvs,
                                                              const uint8_t* is_static, const double* r_max,
                                                              const double* p0, const double* p1, double infl,
-                                                             double* box_raw, double* box_inf) {
+                                                             double* box_raw, double* box_inf, int* inc,
+                                                             unsigned long long* counters) {
     __shared__ double R0[9], R1[9];
     __shared__ double red[6][BP_THREADS];
     const int b = bodies[blockIdx.x], tid = threadIdx.x;
+    // clears for the kernels that follow: the inclusion flag of each
+    // selected body and the pair counters
+    if (tid == 0) inc[b] = 0;
+    if (blockIdx.x == 0 && tid < 4) counters[tid] = 0;
     const double* q0 = p0 + 6 * b;
     const double* q1 = p1 + 6 * b;
     const bool same = q0[3] == q1[3] && q0[4] == q1[4] && q0[5] == q1[5];
@@ -601,8 +606,8 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& s
     const unsigned nbs = (unsigned)sel.n_bodies;
     const unsigned ntiles = (unsigned)sel.n_tiles;
     hipLaunchKernelGGL(body_box_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.V, sc.body_vstart,
-                       sc.is_static, sc.r_max, d_p0, p1, r, bp.box_raw.as<double>(), bp.box_inf.as<double>());
-    BP_CHECK(hipMemsetAsync(bp.inc.p, 0, sizeof(int) * nb, st));
+                       sc.is_static, sc.r_max, d_p0, p1, r, bp.box_raw.as<double>(), bp.box_inf.as<double>(),
+                       bp.inc.as<int>(), cnt);
     hipLaunchKernelGGL(included_kernel, dim3(ntiles), dim3(TILE_THREADS), 0, st, sel.tiles, sc.group,
                        bp.box_inf.as<double>(), bp.inc.as<int>());
     hipLaunchKernelGGL(domain_kernel, dim3((unsigned)sel.n_scenes), dim3(BP_THREADS), 0, st, sel.scenes,
@@ -636,7 +641,7 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& s
         BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * cap_pairs));
         BP_CHECK(bp.keys_ee.ensure(8 * cap_ee));
         BP_CHECK(bp.keys_fv.ensure(8 * cap_fv));
-        BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
+        if (attempt > 0) BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
         hipLaunchKernelGGL(body_pair_kernel, dim3(ntiles), dim3(TILE_THREADS), 0, st, sel.tiles, bp.inc.as<int>(),
                            sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt + 2);
         const unsigned grid = (unsigned)std::min<long long>(cap_pairs, 2048);

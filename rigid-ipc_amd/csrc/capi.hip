@@ -653,11 +653,10 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
             std::memcpy(&best_bits[k], &b, sizeof(double));
         }
         // the staging is rewritten only after the previous run's
-        // synchronisation (launch_ccd synchronises after every pass)
-        hipError_t e = hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), s);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(ctx->scene_best.p, best_bits, sizeof(unsigned long long) * ns, hipMemcpyHostToDevice,
-                               s);
+        // synchronisation (launch_ccd synchronises after every pass); the
+        // work and evaluation counters are cleared by launch_ccd
+        hipError_t e = hipMemcpyAsync(ctx->scene_best.p, best_bits, sizeof(unsigned long long) * ns,
+                                      hipMemcpyHostToDevice, s);
         if (e == hipSuccess && any_skip)
             e = hipMemcpyAsync(ctx->scene_skip.p, skip, (size_t)ns, hipMemcpyHostToDevice, s);
         a.scene_skip = any_skip ? ctx->scene_skip.as<unsigned char>() : nullptr;

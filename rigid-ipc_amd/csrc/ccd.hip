@@ -263,8 +263,15 @@ RIGC_DI int split_dim_at(const int lev[3], const double tol[3]) {
     return split_dim(w, tol, w[0] <= tol[0] && w[1] <= tol[1] && w[2] <= tol[2]);
 }
 
+// Per-query state, and the counters of the first pass (work counter,
+// evaluation total, export count), so that pass needs no clears of its own.
 __global__ void ccd_init_kernel(CcdArgs a) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *a.counter = 0;
+        *a.total_evals = 0;
+        *a.n_tasks_out = 0;
+    }
     if (i < a.n_ee + a.n_fv) {
         a.q_best[i] = 0x7ff0000000000000ULL;
         a.q_visits[i] = 0;
@@ -746,19 +753,21 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
     if ((e = w.h_count.ensure(sizeof(unsigned long long))) != hipSuccess) return e;
     unsigned long long* h_n = w.h_count.as<unsigned long long>();
     const unsigned eb = (unsigned)((nq + 255) / 256);
-    hipLaunchKernelGGL(ccd_init_kernel, dim3(eb), dim3(256), 0, stream, a);
     a.tasks_in = nullptr;
     a.n_tasks_in = 0;
     a.tasks_cap = cap;
     a.n_tasks_out = w.count.as<unsigned long long>();
+    hipLaunchKernelGGL(ccd_init_kernel, dim3(eb), dim3(256), 0, stream, a);
     const int budget0 = a.walk_budget;
     for (int pass = 0;; pass++) {
         // later passes export only much longer walks, and from the fourth
         // pass on not at all, so the number of passes stays small
         a.walk_budget = budget0 << (2 * std::min(pass, 8));
         a.tasks_out = (budget0 > 0 && pass < 3) ? w.tasks[pass & 1].as<CcdTask>() : nullptr;
-        if ((e = hipMemsetAsync(a.counter, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
-        if ((e = hipMemsetAsync(a.n_tasks_out, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
+        if (pass > 0) {  // the init kernel cleared them for the first pass
+            if ((e = hipMemsetAsync(a.counter, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
+            if ((e = hipMemsetAsync(a.n_tasks_out, 0, sizeof(unsigned long long), stream)) != hipSuccess) return e;
+        }
         if ((e = hipEventRecord(w.k0, stream)) != hipSuccess) return e;
         hipLaunchKernelGGL(ccd_kernel, dim3(grid), dim3(block), 0, stream, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -1,6 +1,6 @@
 // Exclusive prefix sum of 32-bit counts (three passes: tile scans in LDS,
-// one block over the tile totals, tile offsets added back). Used for
-// stable compaction of flagged entries.
+// one block over the tile totals, tile offsets added back; one launch when
+// the input fits one tile). Used for stable compaction of flagged entries.
 #include "rigc_internal.hpp"
 
 namespace rigc {
@@ -68,6 +68,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_sums_kernel(unsigned int* s
     if (threadIdx.x == 0) *total = carry;
 }
 
+// n <= SCAN_TILE: the whole scan and its total in one launch.
+__global__ __launch_bounds__(SCAN_THREADS) void scan_single_tile_kernel(const unsigned int* in, unsigned int* out,
+                                                                       long long n, unsigned int* total) {
+    const long long base = (long long)threadIdx.x * SCAN_ITEMS;
+    unsigned int v[SCAN_ITEMS];
+    unsigned int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    unsigned int pre;
+    const unsigned int t = block_exclusive_scan(s, &pre);
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = pre;
+        pre += v[k];
+    }
+    if (threadIdx.x == 0) *total = t;
+}
+
 __global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(unsigned int* out, long long n,
                                                                const unsigned int* tile_sums) {
     const long long base = (long long)blockIdx.x * SCAN_TILE;
@@ -87,6 +106,10 @@ size_t scan_workspace_entries(long long n) { return (size_t)((n + SCAN_TILE - 1)
 void exclusive_scan_u32(const unsigned int* in, unsigned int* out, long long n, unsigned int* work,
                         unsigned int* total, hipStream_t st) {
     const long long tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles <= 1) {
+        hipLaunchKernelGGL(scan_single_tile_kernel, dim3(1), dim3(SCAN_THREADS), 0, st, in, out, n, total);
+        return;
+    }
     if (n > 0)
         hipLaunchKernelGGL(scan_tiles_kernel, dim3((unsigned)tiles), dim3(SCAN_THREADS), 0, st, in, out, n, work);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(SCAN_THREADS), 0, st, work, tiles, total);
