@@ -110,9 +110,13 @@ __global__ __launch_bounds__(THREADS) void body_list_vertex_kernel(const int* bo
 
 // --------------------------------------------------------------- constraint set
 struct ClassifyArgs {
-    const int* ee;
-    const int* fv;
-    long long n_ee, n_fv;
+    // the broad phase's packed keys (e0 * ne + e1, f * nv + v) and their
+    // counts on the device; the grid covers both capacities
+    const unsigned long long* kee;
+    const unsigned long long* kfv;
+    const unsigned long long* n_pairs;  // [0] edge-edge, [1] face-vertex
+    long long cap_ee, cap_fv;
+    unsigned long long ne;
     const int* E;
     const int* F;
     const int* FE;
@@ -132,15 +136,20 @@ __device__ inline void load3(const double* W, int v, double* x) {
 // ipc::construct_constraint_set per candidate: the closest-feature pair
 // decides the constraint kind; a nearly parallel edge pair stays an
 // edge-edge constraint (mollified).
+// Classes are keyed by elements: VV (v0, v1), EV (e, v), EE and FV by
+// the candidate's own packed key, so each class sorts into the order of
+// the sorted candidate list.
 __global__ __launch_bounds__(THREADS) void classify_kernel(ClassifyArgs a) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n_ee + a.n_fv) return;
+    if (i >= a.cap_ee + a.cap_fv) return;
     double X[12];
     int cls;
     unsigned long long key;
     const unsigned long long NV = (unsigned long long)a.nv;
-    if (i < a.n_ee) {
-        const int ei = a.ee[2 * i], ej = a.ee[2 * i + 1];
+    if (i < a.cap_ee) {
+        if ((unsigned long long)i >= min(a.n_pairs[0], (unsigned long long)a.cap_ee)) return;
+        const unsigned long long kp = a.kee[i];
+        const int ei = (int)(kp / a.ne), ej = (int)(kp % a.ne);
         const int i0 = a.E[2 * ei], i1 = a.E[2 * ei + 1], j0 = a.E[2 * ej], j1 = a.E[2 * ej + 1];
         load3(a.W, i0, X);
         load3(a.W, i1, X + 3);
@@ -173,11 +182,13 @@ __global__ __launch_bounds__(THREADS) void classify_kernel(ClassifyArgs a) {
         case EE_AB1: ev(ei, j1); break;
         case EE_A0B: ev(ej, i0); break;
         case EE_A1B: ev(ej, i1); break;
-        default: cls = CON_EE; key = (unsigned long long)i; break;
+        default: cls = CON_EE; key = kp; break;
         }
     } else {
-        const long long j = i - a.n_ee;
-        const int fc = a.fv[2 * j], v = a.fv[2 * j + 1];
+        const long long j = i - a.cap_ee;
+        if ((unsigned long long)j >= min(a.n_pairs[1], (unsigned long long)a.cap_fv)) return;
+        const unsigned long long kp = a.kfv[j];
+        const int fc = (int)(kp / NV), v = (int)(kp % NV);
         const int f0 = a.F[3 * fc], f1 = a.F[3 * fc + 1], f2 = a.F[3 * fc + 2];
         load3(a.W, v, X);
         load3(a.W, f0, X + 3);
@@ -201,7 +212,7 @@ __global__ __launch_bounds__(THREADS) void classify_kernel(ClassifyArgs a) {
         case PT_E0: ev(a.FE[3 * fc], v); break;
         case PT_E1: ev(a.FE[3 * fc + 1], v); break;
         case PT_E2: ev(a.FE[3 * fc + 2], v); break;
-        default: cls = CON_FV; key = (unsigned long long)j; break;
+        default: cls = CON_FV; key = kp; break;
         }
     }
     const unsigned int slot = atomicAdd(&a.count[cls], 1u);
@@ -230,8 +241,7 @@ struct RecordArgs {
     long long n_keys;                // keys of this class
     long long n_out;                 // records to write
     ConRec* out;
-    const int* ee;
-    const int* fv;
+    unsigned long long ne;
     const int* E;
     const int* F;
     const int* v_body;
@@ -272,8 +282,8 @@ __global__ __launch_bounds__(THREADS) void record_kernel(RecordArgs a) {
         nvx = 3;
         break;
     case CON_EE: {
-        r.a = a.ee[2 * key];
-        r.b = a.ee[2 * key + 1];
+        r.a = (int)(key / a.ne);
+        r.b = (int)(key % a.ne);
         r.v[0] = a.E[2 * r.a];
         r.v[1] = a.E[2 * r.a + 1];
         r.v[2] = a.E[2 * r.b];
@@ -287,8 +297,8 @@ __global__ __launch_bounds__(THREADS) void record_kernel(RecordArgs a) {
         break;
     }
     default:
-        r.a = a.fv[2 * key];
-        r.b = a.fv[2 * key + 1];
+        r.a = (int)(key / NV);
+        r.b = (int)(key % NV);
         r.v[0] = r.b;
         r.v[1] = a.F[3 * r.a];
         r.v[2] = a.F[3 * r.a + 1];
@@ -967,41 +977,65 @@ hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc
         as.have_cons = true;
         return hipSuccess;
     }
-    AS_CHECK(run_broad_phase(bp, sc, sel, d_poses, d_poses, (dhat + dmin) / 2.0, false, st));
-    AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, false, st));
-    const long long nc = bp.n_ee + bp.n_fv;
-    if (nc == 0) {
+    if (sel.n_tiles == 0) {
         as.have_cons = true;
         return hipSuccess;
     }
-    for (int k = 0; k < 4; k++) AS_CHECK(as.ckeys[k].ensure(sizeof(unsigned long long) * nc));
+    // The broad phase is queued without a read-back: classification reads
+    // its packed keys and counts on the device, and one synchronisation
+    // returns both the candidate and the class counts (a pass whose
+    // candidates overflowed the capacities is run again at the counts).
+    AS_CHECK(queue_broad_phase(bp, sc, sel, d_poses, d_poses, (dhat + dmin) / 2.0, false, st));
+    AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, false, st));
     AS_CHECK(as.ccount.ensure(8 * sizeof(unsigned int)));
-    AS_CHECK(hipMemsetAsync(as.ccount.p, 0, 8 * sizeof(unsigned int), st));
-    ClassifyArgs ca;
-    ca.ee = bp.out_ee.as<int>();
-    ca.fv = bp.out_fv.as<int>();
-    ca.n_ee = bp.n_ee;
-    ca.n_fv = bp.n_fv;
-    ca.E = sc.E;
-    ca.F = sc.F;
-    ca.FE = sc.FE;
-    ca.W = as.W.as<double>();
-    ca.nv = sc.nv;
-    ca.offset_sqr = (dmin + dhat) * (dmin + dhat);
-    for (int k = 0; k < 4; k++) ca.keys[k] = as.ckeys[k].as<unsigned long long>();
-    ca.count = as.ccount.as<unsigned int>();
-    hipLaunchKernelGGL(classify_kernel, dim3(grid_for(nc)), dim3(THREADS), 0, st, ca);
-    AS_CHECK(hipGetLastError());
     AS_CHECK(as.h_mail.ensure(sizeof(unsigned int) * MAIL_SLOTS));
+    AS_CHECK(bp.h_counters.ensure(4 * sizeof(unsigned long long)));
     unsigned int* mail = as.h_mail.as<unsigned int>();
-    AS_CHECK(hipMemcpyAsync(mail + MAIL_CLASS, as.ccount.p, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-    AS_CHECK(hipStreamSynchronize(st));
+    const unsigned long long* got = bp.h_counters.as<unsigned long long>();
+    bool fits = false;
+    for (int attempt = 0; attempt < 3 && !fits; attempt++) {
+        if (attempt > 0) AS_CHECK(requeue_pairs(bp, sc, sel, st));
+        const long long cap = bp.cap_ee + bp.cap_fv;
+        for (int k = 0; k < 4; k++) AS_CHECK(as.ckeys[k].ensure(sizeof(unsigned long long) * cap));
+        AS_CHECK(hipMemsetAsync(as.ccount.p, 0, 8 * sizeof(unsigned int), st));
+        ClassifyArgs ca;
+        ca.kee = bp.keys_ee.as<unsigned long long>();
+        ca.kfv = bp.keys_fv.as<unsigned long long>();
+        ca.n_pairs = bp.counters.as<unsigned long long>();
+        ca.cap_ee = bp.cap_ee;
+        ca.cap_fv = bp.cap_fv;
+        ca.ne = (unsigned long long)sc.ne;
+        ca.E = sc.E;
+        ca.F = sc.F;
+        ca.FE = sc.FE;
+        ca.W = as.W.as<double>();
+        ca.nv = sc.nv;
+        ca.offset_sqr = (dmin + dhat) * (dmin + dhat);
+        for (int k = 0; k < 4; k++) ca.keys[k] = as.ckeys[k].as<unsigned long long>();
+        ca.count = as.ccount.as<unsigned int>();
+        hipLaunchKernelGGL(classify_kernel, dim3(grid_for(cap)), dim3(THREADS), 0, st, ca);
+        AS_CHECK(hipGetLastError());
+        AS_CHECK(hipMemcpyAsync(bp.h_counters.p, bp.counters.p, 3 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, st));
+        AS_CHECK(hipMemcpyAsync(mail + MAIL_CLASS, as.ccount.p, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                                st));
+        AS_CHECK(hipStreamSynchronize(st));
+        fits = broad_phase_fits(bp, got);
+    }
+    if (!fits) return hipErrorUnknown;
+    bp.n_body_pairs = (long long)got[2];
+    bp.n_ee = (long long)got[0];
+    bp.n_fv = (long long)got[1];
+    if (bp.n_ee + bp.n_fv == 0) {
+        as.have_cons = true;
+        return hipSuccess;
+    }
     unsigned int cnt[4];
     for (int k = 0; k < 4; k++) cnt[k] = mail[MAIL_CLASS + k];
 
     const unsigned long long key_max[4] = {
         (unsigned long long)sc.nv * (unsigned long long)sc.nv, (unsigned long long)sc.ne * (unsigned long long)sc.nv,
-        (unsigned long long)bp.n_ee, (unsigned long long)bp.n_fv };
+        (unsigned long long)sc.ne * (unsigned long long)sc.ne, (unsigned long long)sc.nf * (unsigned long long)sc.nv };
     long long nmax = 1;
     for (int k = 0; k < 4; k++) nmax = std::max<long long>(nmax, cnt[k]);
     AS_CHECK(as.ctmp.ensure(sizeof(unsigned long long) * nmax));
@@ -1050,8 +1084,7 @@ hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc
         ra.n_keys = cnt[k];
         ra.n_out = nout[k];
         ra.out = as.cons.as<ConRec>() + off;
-        ra.ee = bp.out_ee.as<int>();
-        ra.fv = bp.out_fv.as<int>();
+        ra.ne = (unsigned long long)sc.ne;
         ra.E = sc.E;
         ra.F = sc.F;
         ra.v_body = sc.v_body;

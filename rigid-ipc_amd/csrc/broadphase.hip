@@ -581,11 +581,12 @@ int bit_length(unsigned long long x) {
         if (e_ != hipSuccess) return e_; \
     } while (0)
 
-hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
-                           const double* d_p1, double r, bool continuous, hipStream_t st) {
-    bp.n_ee = bp.n_fv = bp.n_body_pairs = 0;
+namespace {
+
+// Per-body stages (boxes, inclusion, domain, vertex and element boxes).
+hipError_t queue_element_boxes(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                               const double* d_p1, double r, bool continuous, hipStream_t st) {
     const int nb = sc.nb;
-    if (nb <= 1 || sel.n_bodies <= 1 || sel.n_tiles == 0) return hipSuccess;
     BP_CHECK(bp.box_raw.ensure(6 * sizeof(double) * nb));
     BP_CHECK(bp.box_inf.ensure(6 * sizeof(double) * nb));
     BP_CHECK(bp.inc.ensure(sizeof(int) * nb));
@@ -624,47 +625,85 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& s
                        sc.body_estart, sc.body_fstart, sc.E, sc.F, bp.inc.as<int>(), bp.vmin.as<double>(),
                        bp.vmax.as<double>(), bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(),
                        bp.eh.as<double>(), bp.fc.as<double>(), bp.fh.as<double>());
-    BP_CHECK(hipGetLastError());
+    return hipGetLastError();
+}
 
-    // Body pairs, then element pairs of every body pair, with one read-back
-    // of the three counts ([0] edge-edge, [1] face-vertex, [2] body pairs).
-    // A pass whose body pairs overflow sees only part of them, so up to two
-    // more passes, each with capacities of at least the counts seen.
+// Body pairs, then element pairs of every body pair, at the capacities in
+// bp (the counters hold the true counts afterwards, also past a capacity).
+hipError_t queue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, bool clear, hipStream_t st) {
+    unsigned long long* cnt = bp.counters.as<unsigned long long>();
+    BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * bp.cap_pairs));
+    BP_CHECK(bp.keys_ee.ensure(8 * bp.cap_ee));
+    BP_CHECK(bp.keys_fv.ensure(8 * bp.cap_fv));
+    if (clear) BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(body_pair_kernel, dim3((unsigned)sel.n_tiles), dim3(TILE_THREADS), 0, st, sel.tiles,
+                       bp.inc.as<int>(), sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), bp.cap_pairs, cnt + 2);
+    const unsigned grid = (unsigned)std::min<long long>(bp.cap_pairs, 2048);
+    hipLaunchKernelGGL(element_pair_kernel, dim3(grid), dim3(BP_THREADS), 0, st, bp.pairs.as<int>(), sc.body_vstart,
+                       sc.body_estart, sc.body_fstart, sc.group, bp.bhull.as<double>(), bp.vc.as<double>(),
+                       bp.vh.as<double>(), bp.ec.as<double>(), bp.eh.as<double>(), bp.fc.as<double>(),
+                       bp.fh.as<double>(), (long long)sc.nv, (long long)sc.ne, bp.keys_ee.as<unsigned long long>(),
+                       bp.cap_ee, bp.keys_fv.as<unsigned long long>(), bp.cap_fv, cnt, cnt + 2, bp.cap_pairs);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+static void broad_phase_capacities(BroadPhase& bp, const SceneSel& sel) {
+    bp.cap_pairs = std::max<long long>(std::max<long long>(1024, 32LL * sel.n_bodies),
+                                       (long long)(bp.pairs.bytes / (2 * sizeof(int))));
+    bp.cap_ee = std::max<long long>(bp.keys_ee.bytes / 8, 1 << 16);
+    bp.cap_fv = std::max<long long>(bp.keys_fv.bytes / 8, 1 << 16);
+}
+
+bool broad_phase_fits(BroadPhase& bp, const unsigned long long* got) {
+    const bool fits = (long long)got[2] <= bp.cap_pairs && (long long)got[0] <= bp.cap_ee &&
+                      (long long)got[1] <= bp.cap_fv;
+    bp.cap_pairs = std::max<long long>(bp.cap_pairs, (long long)got[2]);
+    bp.cap_ee = std::max<long long>(bp.cap_ee, (long long)got[0]);
+    bp.cap_fv = std::max<long long>(bp.cap_fv, (long long)got[1]);
+    return fits;
+}
+
+hipError_t queue_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                             const double* d_p1, double r, bool continuous, hipStream_t st) {
+    BP_CHECK(queue_element_boxes(bp, sc, sel, d_p0, d_p1, r, continuous, st));
+    broad_phase_capacities(bp, sel);
+    return queue_pairs(bp, sc, sel, false, st);
+}
+
+hipError_t requeue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, hipStream_t st) {
+    return queue_pairs(bp, sc, sel, true, st);
+}
+
+hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                           const double* d_p1, double r, bool continuous, bool sorted, hipStream_t st) {
+    bp.n_ee = bp.n_fv = bp.n_body_pairs = 0;
+    if (sc.nb <= 1 || sel.n_bodies <= 1 || sel.n_tiles == 0) return hipSuccess;
+    // Pairs with one read-back of the three counts ([0] edge-edge, [1]
+    // face-vertex, [2] body pairs). A pass whose body pairs overflow sees
+    // only part of them, so up to two more passes, each with capacities of
+    // at least the counts seen.
     BP_CHECK(bp.h_counters.ensure(4 * sizeof(unsigned long long)));
     const unsigned long long* got = bp.h_counters.as<unsigned long long>();
-    long long cap_pairs = std::max<long long>(std::max<long long>(1024, 32LL * sel.n_bodies),
-                                              (long long)(bp.pairs.bytes / (2 * sizeof(int))));
-    long long cap_ee = std::max<long long>(bp.keys_ee.bytes / 8, 1 << 16);
-    long long cap_fv = std::max<long long>(bp.keys_fv.bytes / 8, 1 << 16);
+    BP_CHECK(queue_broad_phase(bp, sc, sel, d_p0, d_p1, r, continuous, st));
     bool fits = false;
     for (int attempt = 0; attempt < 3 && !fits; attempt++) {
-        BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * cap_pairs));
-        BP_CHECK(bp.keys_ee.ensure(8 * cap_ee));
-        BP_CHECK(bp.keys_fv.ensure(8 * cap_fv));
-        if (attempt > 0) BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(body_pair_kernel, dim3(ntiles), dim3(TILE_THREADS), 0, st, sel.tiles, bp.inc.as<int>(),
-                           sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), cap_pairs, cnt + 2);
-        const unsigned grid = (unsigned)std::min<long long>(cap_pairs, 2048);
-        hipLaunchKernelGGL(element_pair_kernel, dim3(grid), dim3(BP_THREADS), 0, st, bp.pairs.as<int>(),
-                           sc.body_vstart, sc.body_estart, sc.body_fstart, sc.group, bp.bhull.as<double>(),
-                           bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(), bp.eh.as<double>(),
-                           bp.fc.as<double>(), bp.fh.as<double>(), (long long)sc.nv, (long long)sc.ne,
-                           bp.keys_ee.as<unsigned long long>(), cap_ee, bp.keys_fv.as<unsigned long long>(), cap_fv,
-                           cnt, cnt + 2, cap_pairs);
-        BP_CHECK(hipGetLastError());
-        BP_CHECK(hipMemcpyAsync(bp.h_counters.p, cnt, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        if (attempt > 0) BP_CHECK(requeue_pairs(bp, sc, sel, st));
+        BP_CHECK(hipMemcpyAsync(bp.h_counters.p, bp.counters.p, 3 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, st));
         BP_CHECK(hipStreamSynchronize(st));
-        fits = (long long)got[2] <= cap_pairs && (long long)got[0] <= cap_ee && (long long)got[1] <= cap_fv;
-        cap_pairs = std::max<long long>(cap_pairs, (long long)got[2]);
-        cap_ee = std::max<long long>(cap_ee, (long long)got[0]);
-        cap_fv = std::max<long long>(cap_fv, (long long)got[1]);
+        fits = broad_phase_fits(bp, got);
     }
     if (!fits) return hipErrorUnknown;
     bp.n_body_pairs = (long long)got[2];
     bp.n_ee = (long long)got[0];
     bp.n_fv = (long long)got[1];
+    return finish_broad_phase(bp, sc, sorted, st);
+}
 
-    // sort and unpack
+// Unpacks (and, when `sorted`, first sorts) the element-pair keys into int32 pairs.
+hipError_t finish_broad_phase(BroadPhase& bp, const DevScene& sc, bool sorted, hipStream_t st) {
     const long long nmax = std::max(bp.n_ee, bp.n_fv);
     BP_CHECK(bp.tmp.ensure(8 * std::max<long long>(nmax, 1)));
     BP_CHECK(bp.hist.ensure(sizeof(unsigned int) * radix_sort_hist_entries(std::max<long long>(nmax, 1))));
@@ -672,15 +711,17 @@ hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& s
     BP_CHECK(bp.out_fv.ensure(2 * sizeof(int) * std::max<long long>(bp.n_fv, 1)));
     const unsigned long long me = (unsigned long long)sc.ne, mv = (unsigned long long)sc.nv;
     if (bp.n_ee) {
-        BP_CHECK(radix_sort_u64(bp.keys_ee.as<unsigned long long>(), bp.tmp.as<unsigned long long>(),
-                                bp.hist.as<unsigned int>(), bp.n_ee, bit_length(me * me), st));
+        if (sorted)
+            BP_CHECK(radix_sort_u64(bp.keys_ee.as<unsigned long long>(), bp.tmp.as<unsigned long long>(),
+                                    bp.hist.as<unsigned int>(), bp.n_ee, bit_length(me * me), st));
         hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)((bp.n_ee + 255) / 256)), dim3(256), 0, st,
                            bp.keys_ee.as<unsigned long long>(), bp.n_ee, me, bp.out_ee.as<int>());
     }
     if (bp.n_fv) {
-        BP_CHECK(radix_sort_u64(bp.keys_fv.as<unsigned long long>(), bp.tmp.as<unsigned long long>(),
-                                bp.hist.as<unsigned int>(), bp.n_fv, bit_length((unsigned long long)sc.nf * mv),
-                                st));
+        if (sorted)
+            BP_CHECK(radix_sort_u64(bp.keys_fv.as<unsigned long long>(), bp.tmp.as<unsigned long long>(),
+                                    bp.hist.as<unsigned int>(), bp.n_fv,
+                                    bit_length((unsigned long long)sc.nf * mv), st));
         hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)((bp.n_fv + 255) / 256)), dim3(256), 0, st,
                            bp.keys_fv.as<unsigned long long>(), bp.n_fv, mv, bp.out_fv.as<int>());
     }

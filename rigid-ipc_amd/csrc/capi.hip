@@ -595,8 +595,9 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
     const int ns = ctx->scene.n_scenes;
     hipStream_t s = ctx->stream;
     RIGC_HIP(hipEventRecord(ctx->ev0, s));
+    // the earliest TOI is a minimum over the candidates: their order is free
     RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, sel, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
-                             msd / 2.0, true, s));
+                             msd / 2.0, true, false, s));
     RIGC_HIP(hipEventRecord(ctx->ev1, s));
     const long long nq = ctx->bp.n_ee + ctx->bp.n_fv;
     RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
@@ -823,7 +824,7 @@ rigc_status rigc_broad_phase(rigc_ctx* ctx, const double* poses_t0, const double
     if (st != RIGC_OK) return st;
     RIGC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
-                             ctx->poses1.as<double>(), inflation_radius, poses_t1 != nullptr, ctx->stream));
+                             ctx->poses1.as<double>(), inflation_radius, poses_t1 != nullptr, true, ctx->stream));
     RIGC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     *n_ee = ctx->bp.n_ee;
     *n_fv = ctx->bp.n_fv;

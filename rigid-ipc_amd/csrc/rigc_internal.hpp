@@ -108,16 +108,30 @@ struct BroadPhase {
     DevBuf box_raw, box_inf, inc, dom, vmin, vmax, bhull;
     DevBuf vc, vh, ec, eh, fc, fh;
     DevBuf pairs, keys_ee, keys_fv, tmp, hist, counters;
-    DevBuf out_ee, out_fv;  // sorted (a, b) int32 pairs
+    DevBuf out_ee, out_fv;  // (a, b) int32 pairs
     HostBuf h_counters;     // element-pair counts read back
     long long n_ee = 0, n_fv = 0, n_body_pairs = 0;
+    long long cap_pairs = 0, cap_ee = 0, cap_fv = 0;  // capacities of the last queued pass
 };
 
 // Candidate detection over the selected scenes (continuous when p1 != p0
-// is meant: `continuous`), leaving sorted int32 pairs in bp.out_ee /
-// bp.out_fv.
+// is meant: `continuous`), leaving int32 pairs in bp.out_ee / bp.out_fv,
+// sorted (the reference's order) when `sorted`.
 hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
-                           const double* d_p1, double inflation_radius, bool continuous, hipStream_t stream);
+                           const double* d_p1, double inflation_radius, bool continuous, bool sorted,
+                           hipStream_t stream);
+// The same in parts, for callers that read the counts back with other
+// results: queue_broad_phase queues every stage (packed keys e0 * ne + e1
+// and f * nv + v in bp.keys_ee / keys_fv, counts [ee, fv, body pairs] in
+// bp.counters) without waiting; after the read-back, broad_phase_fits
+// checks the counts against the capacities (raising them to the counts);
+// requeue_pairs runs the pair stages again at the raised capacities;
+// finish_broad_phase unpacks (and sorts) the n_ee / n_fv keys.
+hipError_t queue_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                             const double* d_p1, double inflation_radius, bool continuous, hipStream_t stream);
+bool broad_phase_fits(BroadPhase& bp, const unsigned long long* counts);
+hipError_t requeue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, hipStream_t stream);
+hipError_t finish_broad_phase(BroadPhase& bp, const DevScene& sc, bool sorted, hipStream_t stream);
 
 hipError_t radix_sort_u64(unsigned long long* keys, unsigned long long* tmp, unsigned int* hist, long long n,
                           int bits, hipStream_t stream);
