@@ -45,8 +45,8 @@ FLOPS_PER_BOX = 1015.7
 # WRITE_SIZE runs (profiles/ccd_kernel_traffic_pmc.md): FETCH_SIZE scaled
 # by 2.0 and WRITE_SIZE by 1.0, the factors at which the counters read a
 # 256 MiB device copy as 256 MiB read and 256 MiB written.
-CCD_TRAFFIC = (1832.4 * 2.0 + 137.8) * 1024   # ballistic CCD sequence
-SIM_CCD_TRAFFIC = (664.7 * 2.0 + 78.3) * 1024  # time stepping, first 3 steps
+CCD_TRAFFIC = (812.0 * 2.0 + 2185.6) * 1024   # ballistic CCD sequence
+SIM_CCD_TRAFFIC = (223.9 * 2.0 + 2126.3) * 1024  # time stepping, first 3 steps
 
 
 def log(msg):
