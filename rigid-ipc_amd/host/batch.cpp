@@ -464,9 +464,13 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
         if (s != RIGC_OK) {
             b->err = "rigs_create failed for scene " + std::to_string(i);
         } else {
-            rigs::set_host_overlap(b->sims[i], false);
-            // fused: a scene's friction term joins the round of its barrier term
-            if (mode == RIGS_BATCH_FUSED) rigs::set_fork_hooks(b->sims[i], fork_join, fork_part, &b->B);
+            // fused: every scene's stepper already runs on its own thread,
+            // and a scene's friction term joins the round of its barrier
+            // term; sequential: each scene runs as the single-scene stepper
+            if (mode == RIGS_BATCH_FUSED) {
+                rigs::set_host_overlap(b->sims[i], false);
+                rigs::set_fork_hooks(b->sims[i], fork_join, fork_part, &b->B);
+            }
         }
     }
     if (s != RIGC_OK) {

@@ -36,6 +36,8 @@ using rigs::Block;
 
 namespace {
 
+constexpr int OVERLAP_MIN_BODIES = 200;
+
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -973,7 +975,10 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         delete sim;
         return s;
     }
-    if (sim->own_ctx && params->coefficient_friction > 0) {
+    // Helper threads pay a wake-up of some tens of microseconds per use:
+    // below OVERLAP_MIN_BODIES bodies the work they would hide is smaller.
+    sim->overlap = scene->num_bodies >= OVERLAP_MIN_BODIES;
+    if (sim->overlap && sim->own_ctx && params->coefficient_friction > 0) {
         // a second context for the friction term (without one the friction
         // term just runs through the main context)
         if (rigc_create(&sim->fric_ctx, device) == RIGC_OK && rigc_set_scene(sim->fric_ctx, scene) == RIGC_OK) {
