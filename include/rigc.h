@@ -60,7 +60,7 @@ typedef struct rigc_stats {
     double assembly_ms;        /* the assembly kernels of the last call */
     int64_t box_evaluations;   /* inclusion-function evaluations (CCD) */
     int64_t num_candidates;    /* candidates passed to the narrow phase */
-    int64_t num_items;         /* (element, cell) items sorted by the broad phase */
+    int64_t num_constraints;   /* records of the constraint set the last building call built */
     double ccd_kernel_ms;      /* the CCD traversal kernel launches alone (CCD) */
     int64_t ccd_launches;      /* CCD traversal kernel launches (passes) of the last call */
 } rigc_stats;

@@ -792,6 +792,7 @@ double total(const std::vector<double>& v) {
 rigc_status build_sel(rigc_ctx* ctx, const SelState& S, double dhat, double dmin) {
     RIGC_HIP(build_constraint_set(ctx->as, ctx->bp, ctx->scene, S.sel, ctx->poses0.as<double>(), dhat, dmin,
                                   ctx->num_cus, ctx->stream));
+    ctx->stats.num_constraints = ctx->as.n_con;
     return RIGC_OK;
 }
 

@@ -57,7 +57,7 @@ class Stats(ctypes.Structure):
     _fields_ = [
         ("broad_phase_ms", ctypes.c_double), ("narrow_phase_ms", ctypes.c_double),
         ("assembly_ms", ctypes.c_double), ("box_evaluations", ctypes.c_int64),
-        ("num_candidates", ctypes.c_int64), ("num_items", ctypes.c_int64),
+        ("num_candidates", ctypes.c_int64), ("num_constraints", ctypes.c_int64),
         ("ccd_kernel_ms", ctypes.c_double), ("ccd_launches", ctypes.c_int64),
     ]
 
