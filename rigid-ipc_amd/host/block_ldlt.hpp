@@ -42,9 +42,18 @@ public:
         std::vector<char> done(n, 0);
         std::set<std::pair<int, int>> queue;  // (degree, node)
         std::vector<int> deg(n);
+        // Bodies without off-diagonal blocks come first in minimum-degree
+        // order (degree 0, ascending) and need no elimination: factor them
+        // here, and queue only the coupled ones.
         for (int i = 0; i < n; i++) {
             deg[i] = (int)A[i].size();
-            queue.insert({ deg[i], i });
+            if (deg[i] == 0) {
+                done[i] = 1;
+                order_.push_back(i);
+                if (!ldlt6(D[i], cols_[i].L, cols_[i].d)) return false;
+            } else {
+                queue.insert({ deg[i], i });
+            }
         }
         while (!queue.empty()) {
             const int k = queue.begin()->second;

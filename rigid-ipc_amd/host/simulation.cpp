@@ -248,6 +248,7 @@ struct rigs_sim {
     // so the friction entry is dropped when that set changes.
     struct TermCache {
         bool valid = false;
+        bool derivs = false;  // g and H still held (they are handed out once, by move)
         std::vector<double> x;
         double e = 0;
         std::vector<double> g;
@@ -261,11 +262,16 @@ struct rigs_sim {
         const int32_t full = RIGC_GRADIENT | RIGC_HESSIAN;
         // an energy-only request is served by a full entry: the energy is
         // the same sum of the same per-constraint values
-        if (c.valid && (flags == 0 || flags == full) &&
+        // the Newton iteration that starts at the entry's point is the only
+        // user of its gradient and Hessian: they are moved out, not copied
+        if (c.valid && (flags == 0 || (flags == full && c.derivs)) &&
             std::memcmp(c.x.data(), x, sizeof(double) * 6 * (size_t)nb) == 0) {
             e = c.e;
-            if (g) *g = c.g;
-            if (H) *H = c.H;
+            if (flags == full) {
+                if (g) *g = std::move(c.g);
+                if (H) *H = std::move(c.H);
+                c.derivs = false;
+            }
             return RIGC_OK;
         }
         if (prefetch && flags == 0) {
@@ -274,6 +280,7 @@ struct rigs_sim {
             if (s != RIGC_OK) return s;
             c.x.assign(x, x + 6 * (size_t)nb);
             c.valid = true;
+            c.derivs = true;
             e = c.e;
             return RIGC_OK;
         }
@@ -470,6 +477,7 @@ struct rigs_sim {
     // energy-only call gives.
     struct EnergyCache {
         bool valid = false;
+        bool derivs = false;  // g and H still held (handed out once, by move)
         std::vector<double> x;
         double E = 0;
         std::vector<double> g;
@@ -494,11 +502,14 @@ struct rigs_sim {
         BlockHess HB, HD;
         double E = 0;
         bool pending = false;
-        if (ecache.valid && want_grad == want_hess &&
+        if (ecache.valid && want_grad == want_hess && (!want_grad || ecache.derivs) &&
             std::memcmp(ecache.x.data(), x, sizeof(double) * 6 * (size_t)nb) == 0) {
             E = ecache.E;
-            if (want_grad) g = ecache.g;
-            if (want_hess) H = ecache.H;
+            if (want_grad) {
+                g = std::move(ecache.g);
+                H = std::move(ecache.H);
+                ecache.derivs = false;
+            }
         } else if (prefetch && !want_grad && !want_hess) {
             ecache.valid = false;
             ecache.x.assign(x, x + 6 * (size_t)nb);
@@ -508,7 +519,7 @@ struct rigs_sim {
                 pending = true;
             } else {
                 task();
-                ecache.valid = true;
+                ecache.valid = ecache.derivs = true;
                 E = ecache.E;
             }
         } else {
@@ -551,7 +562,7 @@ struct rigs_sim {
             const double tw = now_ms();
             helper.wait();
             stats.host_ms += now_ms() - tw;
-            ecache.valid = true;
+            ecache.valid = ecache.derivs = true;
             E = ecache.E;
         }
         if (s != RIGC_OK) return s;
