@@ -23,12 +23,12 @@ public:
     // diag[i]: block (i,i); off: blocks (i,j) for i != j (each unordered
     // pair once; (j,i) is the transpose). Returns false on a zero or
     // non-finite pivot.
-    bool factorize(int n, const std::vector<Block>& diag, const std::vector<std::pair<int, int>>& pairs,
+    bool factorize(int n, std::vector<Block> diag, const std::vector<std::pair<int, int>>& pairs,
                    const std::vector<Block>& off) {
         n_ = n;
         order_.clear();
         cols_.assign(n, Col{});
-        std::vector<Block> D(diag);
+        std::vector<Block> D(std::move(diag));
         std::vector<std::map<int, Block>> A(n);  // A[i][j] = block (i,j), i != j
         for (size_t e = 0; e < pairs.size(); e++) {
             const int i = pairs[e].first, j = pairs[e].second;

@@ -158,7 +158,7 @@ struct rigs_sim {
     // backend output staging (one per term: the friction term may run on a
     // helper thread beside the barrier term)
     struct Staging {
-        std::vector<double> hdiag, hoff;
+        std::vector<double> hoff;
         std::vector<int32_t> hpairs;
         int64_t hcap = 0;
     };
@@ -207,12 +207,16 @@ struct rigs_sim {
                           double& spent) {
         if (g) g->assign(6 * nb, 0.0);
         rigc_block_hessian h{};
+        if (H) {
+            // the backend writes every diagonal block straight into H
+            H->diag.resize(nb);
+            H->off.clear();
+        }
         for (int attempt = 0; attempt < 2; attempt++) {
             if (H) {
-                S.hdiag.resize(36 * (size_t)nb);
                 S.hpairs.resize(2 * (size_t)std::max<int64_t>(S.hcap, 1));
                 S.hoff.resize(36 * (size_t)std::max<int64_t>(S.hcap, 1));
-                h.diag = S.hdiag.data();
+                h.diag = H->diag[0].data();
                 h.pairs = S.hpairs.data();
                 h.off = S.hoff.data();
                 h.capacity = S.hcap;
@@ -228,8 +232,6 @@ struct rigs_sim {
             break;
         }
         if (H) {
-            H->reset(nb);
-            for (int b = 0; b < nb; b++) std::copy(&S.hdiag[36 * b], &S.hdiag[36 * b] + 36, H->diag[b].begin());
             for (int64_t k = 0; k < h.count; k++) {
                 Block blk;
                 std::copy(&S.hoff[36 * k], &S.hoff[36 * k] + 36, blk.begin());
@@ -628,15 +630,22 @@ struct rigs_sim {
                 for (int k = 0; k < 6; k++) rhs[6 * dyn_index[b] + k] = -g[6 * b + k];
         dir.assign(6 * nb, 0.0);
         rigs::BlockLDLT ldlt;
-        bool ok = ldlt.factorize(n_free, diag, pairs, off);
+        bool ok = ldlt.factorize(n_free, std::move(diag), pairs, off);
         if (ok) {
             ldlt.solve(rhs.data(), d.data());
-            // residual must be finite (compute_direction :540-556)
+            // residual must be finite (compute_direction :540-556); the
+            // diagonal blocks are H's plus coeff on the diagonal
             double res = 0;
             std::vector<double> Hd(6 * (size_t)n_free, 0.0);
-            for (int i = 0; i < n_free; i++)
+            for (int b = 0; b < nb; b++) {
+                const int i = dyn_index[b];
+                if (i < 0) continue;
                 for (int r = 0; r < 6; r++)
-                    for (int c = 0; c < 6; c++) Hd[6 * i + r] += diag[i][6 * r + c] * d[6 * i + c];
+                    for (int c = 0; c < 6; c++) {
+                        const double hv = r == c ? H.diag[b][6 * r + c] + coeff : H.diag[b][6 * r + c];
+                        Hd[6 * i + r] += hv * d[6 * i + c];
+                    }
+            }
             for (size_t e = 0; e < pairs.size(); e++) {
                 const int i = pairs[e].first, j = pairs[e].second;
                 for (int r = 0; r < 6; r++)
