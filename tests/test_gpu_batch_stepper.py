@@ -40,3 +40,28 @@ def test_fused_batch_matches_sequential(ids, steps):
     assert newton > 0
     seq.close()
     fused.close()
+
+
+def test_single_scene_stepper_with_helpers_matches_fused():
+    # From 200 bodies up the single-scene stepper computes the body energy
+    # on a helper thread and the friction term on a second device context
+    # beside the barrier term; the fused batch runs the same scene without
+    # either. Both must give the same trajectory bit for bit.
+    sc = S.config3_scene(13)
+    assert len(sc.group_ids) >= 200
+    single = M.Simulation(sc)
+    fused = M.Batch([sc], mode=M.BATCH_FUSED)
+    for k in range(3):
+        single.step(1)
+        fused.step(1)
+        print("step", k + 1, flush=True)
+    p0, v0 = single.state()
+    p1, v1 = fused.state(0)
+    np.testing.assert_array_equal(p1, p0)
+    np.testing.assert_array_equal(v1, v0)
+    s0, s1 = single.stats(), fused.stats(0)
+    for key in ("newton_iterations", "line_search_iterations", "ccd_calls", "failed_solves"):
+        assert s1[key] == s0[key], key
+    assert s0["newton_iterations"] > 0
+    single.close()
+    fused.close()
