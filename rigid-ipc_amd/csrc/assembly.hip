@@ -12,10 +12,13 @@
 //   body_rotation_kernel   per body: R(r)
 //   body_frame_kernel      per body: dR/dr, d2R/dr2 (closed form), nine threads each
 //   world_vertex_kernel    per vertex: R x + p (round-to-nearest, as the host)
-//   static broad phase     (broadphase.hip) with inflation (dhat + dmin) / 2
-//   classify_kernel        per candidate: closest-feature type, activity
+//   static broad phase     (broadphase.hip) with inflation (dhat + dmin) / 2,
+//                          queued without a read-back
+//   classify_kernel        per packed candidate key (counts read on the
+//                          device): closest-feature type, activity
 //                          d < (dhat + dmin)^2, edge-edge mollifier test;
-//                          appends a sort key to the VV / EV / EE / FV list
+//                          appends a sort key to the VV / EV / EE / FV list;
+//                          one read-back returns candidate and class counts
 //   radix sort + run-length encoding: VV / EV duplicates -> multiplicity
 //   record_kernel          ConRec list, order VV, EV, EE, FV
 //   local_eg_kernel        thread per constraint: energy (+ 12-gradient)

@@ -21,7 +21,10 @@
 //      pair among them is tested, and hits are appended with one atomic per
 //      wavefront;
 //   8. a stable radix sort of the pair keys, giving the sorted, duplicate-
-//      free candidate lists of the reference.
+//      free candidate lists of the reference (rigc_broad_phase). The
+//      earliest-TOI path, whose answer is a minimum over the candidates,
+//      and the constraint build, which classifies the packed keys and
+//      sorts per class, skip it.
 // Every element pair of two included bodies with overlapping AABBs and
 // different groups is found exactly once (each pair belongs to exactly one
 // body pair), so the result equals the reference's hash-grid set, which is
