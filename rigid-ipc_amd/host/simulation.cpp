@@ -30,6 +30,7 @@
 
 #include "../csrc/rigc_barrier.hpp"
 #include "block_ldlt.hpp"
+#include "helper.hpp"
 #include "rigs.h"
 
 using rigs::Block;
@@ -79,54 +80,6 @@ struct BodyState {
     double target[3], Pm[9];
 };
 
-// One helper thread that runs a task while the caller makes backend calls.
-class Helper {
-  public:
-    ~Helper() {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            quit_ = true;
-        }
-        cv_.notify_all();
-        if (t_.joinable()) t_.join();
-    }
-    void run(std::function<void()> task) {
-        if (!t_.joinable()) t_ = std::thread([this] { loop(); });
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            task_ = std::move(task);
-            busy_ = true;
-        }
-        cv_.notify_all();
-    }
-    void wait() {
-        std::unique_lock<std::mutex> lk(m_);
-        cv_.wait(lk, [this] { return !busy_; });
-    }
-
-  private:
-    void loop() {
-        std::unique_lock<std::mutex> lk(m_);
-        for (;;) {
-            cv_.wait(lk, [this] { return busy_ || quit_; });
-            if (busy_) {
-                std::function<void()> task = std::move(task_);
-                lk.unlock();
-                task();
-                lk.lock();
-                busy_ = false;
-                cv_.notify_all();
-            } else if (quit_) {
-                return;
-            }
-        }
-    }
-    std::thread t_;
-    std::mutex m_;
-    std::condition_variable cv_;
-    std::function<void()> task_;
-    bool busy_ = false, quit_ = false;
-};
 
 // Symmetric block Hessian in body layout.
 struct BlockHess {
@@ -486,7 +439,7 @@ struct rigs_sim {
         BlockHess H;
     };
     EnergyCache ecache;
-    Helper helper, fric_helper;
+    rigs::Helper helper, fric_helper;
     bool overlap = true;
     // A backend that serves several steppers' calls in rounds (the batch
     // stepper) counts the threads that may call it: the friction helper
