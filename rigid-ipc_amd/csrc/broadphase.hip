@@ -653,8 +653,14 @@ hipError_t queue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, 
 }  // namespace
 
 static void broad_phase_capacities(BroadPhase& bp, const SceneSel& sel) {
-    bp.cap_pairs = std::max<long long>(std::max<long long>(1024, 32LL * sel.n_bodies),
-                                       (long long)(bp.pairs.bytes / (2 * sizeof(int))));
+    const long long held_pairs = (long long)(bp.pairs.bytes / (2 * sizeof(int)));
+    if (bp.min_cap > 0) {
+        bp.cap_pairs = std::max<long long>(bp.min_cap, held_pairs);
+        bp.cap_ee = std::max<long long>(bp.min_cap, (long long)(bp.keys_ee.bytes / 8));
+        bp.cap_fv = std::max<long long>(bp.min_cap, (long long)(bp.keys_fv.bytes / 8));
+        return;
+    }
+    bp.cap_pairs = std::max<long long>(std::max<long long>(1024, 32LL * sel.n_bodies), held_pairs);
     bp.cap_ee = std::max<long long>(bp.keys_ee.bytes / 8, 1 << 16);
     bp.cap_fv = std::max<long long>(bp.keys_fv.bytes / 8, 1 << 16);
 }

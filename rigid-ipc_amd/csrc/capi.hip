@@ -309,6 +309,10 @@ rigc_status rigc_create(rigc_ctx** out, int32_t device) {
     // RIGC_CCD_WALK_BUDGET: box evaluations after which a lane hands its
     // pending subtrees to the next narrow-phase pass (0 = never)
     if (const char* wb = std::getenv("RIGC_CCD_WALK_BUDGET")) ctx->walk_budget = std::atoi(wb);
+    // RIGC_BROAD_PHASE_MIN_CAPACITY: start the broad phase's candidate and
+    // body-pair capacities from this floor instead of the defaults (a small
+    // value drives the passes that regrow them; used by the tests)
+    if (const char* mc = std::getenv("RIGC_BROAD_PHASE_MIN_CAPACITY")) ctx->bp.min_cap = std::atoll(mc);
     *out = ctx;
     return RIGC_OK;
 }

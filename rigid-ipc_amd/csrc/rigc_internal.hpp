@@ -112,6 +112,7 @@ struct BroadPhase {
     HostBuf h_counters;     // element-pair counts read back
     long long n_ee = 0, n_fv = 0, n_body_pairs = 0;
     long long cap_pairs = 0, cap_ee = 0, cap_fv = 0;  // capacities of the last queued pass
+    long long min_cap = 0;  // > 0: start every capacity from this floor (exercises the overflow passes)
 };
 
 // Candidate detection over the selected scenes (continuous when p1 != p0

@@ -162,3 +162,43 @@ def query_record(sc, p0, p1, kind, a, b):
         A, B = vb[b], vb[f[0]]
         verts = [V[b], V[f[0]], V[f[1]], V[f[2]]]
     return np.concatenate(verts + [p0[A], p1[A], p0[B], p1[B]])
+
+
+def test_capacity_overflow_passes_give_the_same_results():
+    # Contexts whose broad phase starts from capacities of 4 candidates and
+    # 4 body pairs overflow on their first call and regrow in further
+    # passes (three code paths: the sorted broad phase, the earliest TOI,
+    # the constraint build that classifies the packed keys). Each result
+    # must equal a default context's.
+    import gpu_util
+    sc = cluster(7)
+    p0, p1 = moved(sc, 11, dpos=0.05, drot=0.2)
+    ref = gpu_util.context()
+    ref.set_scene(sc)
+    want_bp = ref.broad_phase(p0, p1, 0.0)
+    want_toi = ref.compute_earliest_toi(p0, p1)
+    ref.construct_constraint_set(p0, 0.05)
+    want_cons = ref.constraint_set()
+    want_e = ref.barrier_term(p0, 0.05)[0]
+    ref.close()
+    assert len(want_bp[0]) + len(want_bp[1]) > 100 and len(want_cons[0]) > 4
+    os.environ["RIGC_BROAD_PHASE_MIN_CAPACITY"] = "4"
+    try:
+        tiny = [gpu_util.context() for _ in range(3)]
+    finally:
+        del os.environ["RIGC_BROAD_PHASE_MIN_CAPACITY"]
+    for c in tiny:
+        c.set_scene(sc)
+    got_bp = tiny[0].broad_phase(p0, p1, 0.0)
+    np.testing.assert_array_equal(got_bp[0], want_bp[0])
+    np.testing.assert_array_equal(got_bp[1], want_bp[1])
+    assert tiny[1].compute_earliest_toi(p0, p1) == want_toi
+    tiny[2].construct_constraint_set(p0, 0.05)
+    got_cons = tiny[2].constraint_set()
+    np.testing.assert_array_equal(got_cons[0], want_cons[0])
+    np.testing.assert_array_equal(got_cons[1], want_cons[1])
+    assert tiny[2].barrier_term(p0, 0.05)[0] == want_e
+    print("candidates", len(want_bp[0]), len(want_bp[1]), "constraints", len(want_cons[0]), "toi", want_toi,
+          flush=True)
+    for c in tiny:
+        c.close()
