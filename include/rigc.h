@@ -180,6 +180,21 @@ rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double
 rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, double dhat, double dmin,
                                           int32_t flags, double* min_distance);
 
+/* Edge-triangle intersections at `poses`: every (edge, face) pair, the
+ * edge and the face on two close bodies of different groups, whose closed
+ * segment and closed triangle meet (a segment lying in the triangle's
+ * plane is not counted), decided with exact orientation predicates. *count
+ * receives the number of such pairs; pairs (optional, [capacity][2] =
+ * edge, face, sorted) receives them, RIGC_ERR_CAPACITY when more than
+ * capacity were found. The reference's answer is *count > 0. Replaces
+ * RigidBodyProblem::detect_intersections (3D branch,
+ * src/physics/rigid_body_problem.cpp:289-360, with
+ * detect_intersection_candidates_rigid_bvh, src/ccd/rigid/broad_phase.cpp:265-305,
+ * and ipc-toolkit's is_edge_intersecting_triangle). stats.broad_phase_ms
+ * receives its device time. */
+rigc_status rigc_detect_intersections(rigc_ctx* ctx, const double* poses, int32_t* pairs, int64_t capacity,
+                                      int64_t* count);
+
 /* Lagged friction set from the constraint set (rebuilt at `poses` first
  * when flags holds RIGC_BUILD_CONSTRAINTS): each constraint's closest
  * point, tangent basis and normal force -kappa b'(d) 2 sqrt(d) at `poses`

@@ -318,4 +318,21 @@ void oracle_vertex_derivatives(const double* pose, int vertex, double* w, double
     vertex_derivatives(scene(), pose, vertex, w, jac, hess);
 }
 
+// ---- intersection check
+int oracle_orient3d(const double* a, const double* b, const double* c, const double* d) {
+    return orient3d_sign(a, b, c, d);
+}
+int oracle_edge_triangle(const double* p, const double* q, const double* t0, const double* t1, const double* t2) {
+    return edge_triangle_intersect(p, q, t0, t1, t2) ? 1 : 0;
+}
+// Intersecting (edge, face) pairs at poses, sorted; pairs [cap][2]; returns the count.
+long oracle_detect_intersections(const double* poses, int* pairs, long cap) {
+    const std::vector<std::array<int, 2>> r = detect_intersections(scene(), poses);
+    for (long i = 0; i < (long)r.size() && i < cap; i++) {
+        pairs[2 * i] = r[i][0];
+        pairs[2 * i + 1] = r[i][1];
+    }
+    return (long)r.size();
+}
+
 }  // extern "C"

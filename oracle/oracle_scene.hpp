@@ -47,4 +47,11 @@ void broadphase_static(const Scene& s, const double* poses, double inflation_rad
 // [nv][7] included flag, AABB min, AABB max of the last broad-phase call.
 std::vector<double>& last_vertex_aabbs();
 
+// Exact orient3d sign and the edge-triangle rule; every intersecting
+// (edge, face) pair of the close body pairs, sorted
+// (RigidBodyProblem::detect_intersections, rigid_body_problem.cpp:289-360).
+int orient3d_sign(const double* a, const double* b, const double* c, const double* d);
+bool edge_triangle_intersect(const double* p, const double* q, const double* t0, const double* t1, const double* t2);
+std::vector<std::array<int, 2>> detect_intersections(const Scene& s, const double* poses);
+
 }  // namespace oracle

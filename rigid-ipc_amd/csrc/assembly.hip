@@ -958,6 +958,12 @@ hipError_t run_copies(Assembly& as, const std::vector<long long>& ent, const voi
 
 }  // namespace
 
+// World vertices at d_poses in as.W (kept while the pose tag is unchanged).
+hipError_t current_world_vertices(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                                  hipStream_t st) {
+    return world_vertices(as, sc, sel, d_poses, as.W, false, st);
+}
+
 void gather_rows(const int* rows, int n, int width, const double* src, double* dst, hipStream_t st) {
     if (n > 0)
         hipLaunchKernelGGL(gather_rows_kernel, dim3(grid_for((long long)n * width)), dim3(THREADS), 0, st, rows, n,

@@ -588,7 +588,8 @@ namespace {
 
 // Per-body stages (boxes, inclusion, domain, vertex and element boxes).
 hipError_t queue_element_boxes(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
-                               const double* d_p1, double r, bool continuous, hipStream_t st) {
+                               const double* d_p1, double r, bool continuous, hipStream_t st,
+                               bool element_boxes = true) {
     const int nb = sc.nb;
     BP_CHECK(bp.box_raw.ensure(6 * sizeof(double) * nb));
     BP_CHECK(bp.box_inf.ensure(6 * sizeof(double) * nb));
@@ -624,6 +625,7 @@ hipError_t queue_element_boxes(BroadPhase& bp, const DevScene& sc, const SceneSe
         hipLaunchKernelGGL(vertex_static_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.V,
                            sc.body_vstart, bp.inc.as<int>(), d_p0, r, bp.vmin.as<double>(), bp.vmax.as<double>(),
                            bp.bhull.as<double>());
+    if (!element_boxes) return hipGetLastError();
     hipLaunchKernelGGL(element_aabb_kernel, dim3(nbs), dim3(BP_THREADS), 0, st, sel.bodies, sc.body_vstart,
                        sc.body_estart, sc.body_fstart, sc.E, sc.F, bp.inc.as<int>(), bp.vmin.as<double>(),
                        bp.vmax.as<double>(), bp.vc.as<double>(), bp.vh.as<double>(), bp.ec.as<double>(),
@@ -631,16 +633,24 @@ hipError_t queue_element_boxes(BroadPhase& bp, const DevScene& sc, const SceneSe
     return hipGetLastError();
 }
 
+// Body pairs at the capacity in bp (count in counters[2]).
+hipError_t queue_body_pair_pass(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, bool clear,
+                                hipStream_t st) {
+    unsigned long long* cnt = bp.counters.as<unsigned long long>();
+    BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * bp.cap_pairs));
+    if (clear) BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(body_pair_kernel, dim3((unsigned)sel.n_tiles), dim3(TILE_THREADS), 0, st, sel.tiles,
+                       bp.inc.as<int>(), sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), bp.cap_pairs, cnt + 2);
+    return hipGetLastError();
+}
+
 // Body pairs, then element pairs of every body pair, at the capacities in
 // bp (the counters hold the true counts afterwards, also past a capacity).
 hipError_t queue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, bool clear, hipStream_t st) {
     unsigned long long* cnt = bp.counters.as<unsigned long long>();
-    BP_CHECK(bp.pairs.ensure(2 * sizeof(int) * bp.cap_pairs));
     BP_CHECK(bp.keys_ee.ensure(8 * bp.cap_ee));
     BP_CHECK(bp.keys_fv.ensure(8 * bp.cap_fv));
-    if (clear) BP_CHECK(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(body_pair_kernel, dim3((unsigned)sel.n_tiles), dim3(TILE_THREADS), 0, st, sel.tiles,
-                       bp.inc.as<int>(), sc.group, bp.bhull.as<double>(), bp.pairs.as<int>(), bp.cap_pairs, cnt + 2);
+    BP_CHECK(queue_body_pair_pass(bp, sc, sel, clear, st));
     const unsigned grid = (unsigned)std::min<long long>(bp.cap_pairs, 2048);
     hipLaunchKernelGGL(element_pair_kernel, dim3(grid), dim3(BP_THREADS), 0, st, bp.pairs.as<int>(), sc.body_vstart,
                        sc.body_estart, sc.body_fstart, sc.group, bp.bhull.as<double>(), bp.vc.as<double>(),
@@ -683,6 +693,17 @@ hipError_t queue_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel&
 
 hipError_t requeue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, hipStream_t st) {
     return queue_pairs(bp, sc, sel, true, st);
+}
+
+hipError_t queue_body_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_poses, double r,
+                            hipStream_t st) {
+    BP_CHECK(queue_element_boxes(bp, sc, sel, d_poses, d_poses, r, false, st, false));
+    broad_phase_capacities(bp, sel);
+    return queue_body_pair_pass(bp, sc, sel, false, st);
+}
+
+hipError_t requeue_body_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, hipStream_t st) {
+    return queue_body_pair_pass(bp, sc, sel, true, st);
 }
 
 hipError_t run_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,

@@ -56,6 +56,7 @@ struct rigc_ctx {
     BroadPhase bp;
     Assembly as;
     CcdWork ccdw;
+    Intersections ix;
     int walk_budget = 4096;
     std::vector<double> last_toi;  // per scene: answer of its previous earliest-TOI call
     hipEvent_t ev2 = nullptr;
@@ -340,6 +341,7 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
         b->release();
     ctx->as.release();
     ctx->ccdw.release();
+    ctx->ix.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RIGC_OK;
@@ -961,6 +963,32 @@ rigc_status rigc_compute_minimum_distance(rigc_ctx* ctx, const double* poses, do
     double m = HUGE_VAL;
     for (double x : m2) m = std::min(m, x);
     *min_distance = std::sqrt(m);
+    return RIGC_OK;
+}
+
+rigc_status rigc_detect_intersections(rigc_ctx* ctx, const double* poses, int32_t* pairs, int64_t capacity,
+                                      int64_t* count) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_detect_intersections: no scene");
+    if (!poses || !count || capacity < 0)
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_detect_intersections: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses, nullptr);
+    if (s != RIGC_OK) return s;
+    RIGC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    RIGC_HIP(rigc::detect_intersections(ctx->as, ctx->bp, ctx->ix, ctx->scene, ctx->sel_all.sel,
+                                        ctx->poses0.as<double>(), ctx->stream));
+    RIGC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    *count = ctx->ix.n;
+    const bool fits = !pairs || ctx->ix.n <= capacity;
+    if (pairs && fits && ctx->ix.n)
+        RIGC_HIP(hipMemcpyAsync(pairs, ctx->ix.out.p, 2 * sizeof(int32_t) * ctx->ix.n, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.broad_phase_ms = ms;
+    if (!fits) return ctx->fail(RIGC_ERR_CAPACITY, "rigc_detect_intersections: output capacity too small");
     return RIGC_OK;
 }
 

@@ -133,6 +133,30 @@ hipError_t queue_broad_phase(BroadPhase& bp, const DevScene& sc, const SceneSel&
 bool broad_phase_fits(BroadPhase& bp, const unsigned long long* counts);
 hipError_t requeue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, hipStream_t stream);
 hipError_t finish_broad_phase(BroadPhase& bp, const DevScene& sc, bool sorted, hipStream_t stream);
+// Close body pairs alone (discrete, inflation r) in bp.pairs, their count
+// in bp.counters[2] (past bp.cap_pairs when the pass overflowed: raise
+// bp.cap_pairs and requeue).
+hipError_t queue_body_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_poses, double r,
+                            hipStream_t stream);
+hipError_t requeue_body_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, hipStream_t stream);
+
+// Edge-triangle intersections between close bodies (working set).
+struct Intersections {
+    DevBuf keys, unsure, counts, tmp, hist, out;  // keys edge * nf + face; out (edge, face) int32
+    HostBuf h_counts;
+    long long n = 0, n_unsure = 0;                // intersecting pairs; pairs the filter left open
+    long long cap_keys = 0, cap_unsure = 0;
+    void release() {
+        for (DevBuf* b : { &keys, &unsure, &counts, &tmp, &hist, &out }) b->release();
+        h_counts.release();
+    }
+};
+struct Assembly;
+// Every (edge of one body, face of another) pair of the close body pairs
+// at d_poses (no inflation) whose closed segment and closed triangle meet,
+// sorted by (edge, face), into ix.out ([ix.n][2]).
+hipError_t detect_intersections(Assembly& as, BroadPhase& bp, Intersections& ix, const DevScene& sc,
+                                const SceneSel& sel, const double* d_poses, hipStream_t stream);
 
 hipError_t radix_sort_u64(unsigned long long* keys, unsigned long long* tmp, unsigned int* hist, long long n,
                           int bits, hipStream_t stream);
@@ -308,6 +332,9 @@ struct AssemblyOut {
 };
 
 // Constraint set of the selected scenes at poses, kept in as.cons.
+// World vertices at d_poses into as.W ([nv][3]).
+hipError_t current_world_vertices(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                                  hipStream_t stream);
 hipError_t build_constraint_set(Assembly& as, BroadPhase& bp, const DevScene& sc, const SceneSel& sel,
                                 const double* d_poses, double dhat, double dmin, int num_cus, hipStream_t stream);
 // Constraints per selected scene and class ([n_sel][4]) of the current set.

@@ -86,6 +86,7 @@ def lib():
     L.rigc_broad_phase.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_int32_p, ctypes.c_int64,
                                    _c_int64_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
     L.rigc_compute_earliest_toi.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p]
+    L.rigc_detect_intersections.argtypes = [vp, _c_double_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
     d, i32, i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
     bh = ctypes.POINTER(BlockHessian)
     L.rigc_construct_constraint_set.argtypes = [vp, _c_double_p, d, d, _c_int64_p]
@@ -113,7 +114,7 @@ def lib():
                  "rigc_broad_phase", "rigc_compute_earliest_toi", "rigc_construct_constraint_set",
                  "rigc_get_constraint_set", "rigc_compute_barrier_term", "rigc_compute_minimum_distance",
                  "rigc_update_friction_constraints", "rigc_get_friction_constraints",
-                 "rigc_compute_friction_term"):
+                 "rigc_compute_friction_term", "rigc_detect_intersections"):
         getattr(L, name).restype = ctypes.c_int32
     _lib = L
     return L
@@ -349,6 +350,31 @@ class Context:
             self._check(st)
             return ee[:ne.value].copy(), fv[:nf.value].copy()
         raise RigcError(RIGC_ERR_CAPACITY, "broad phase capacity retry failed")
+
+    def detect_intersections(self, poses):
+        """Edge-triangle intersections between close bodies at poses; returns
+        the sorted (edge, face) int32 pairs (n, 2). The reference's
+        RigidBodyProblem::detect_intersections is len(result) > 0."""
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        cap = getattr(self, "_cap_isect", 1024)
+        for _ in range(2):
+            out = np.empty((cap, 2), np.int32)
+            n = ctypes.c_int64()
+            st = self._L.rigc_detect_intersections(self._ctx, _p(p, _c_double_p), _p(out, _c_int32_p), cap,
+                                                   ctypes.byref(n))
+            if st == RIGC_ERR_CAPACITY:
+                cap = self._cap_isect = max(cap, n.value)
+                continue
+            self._check(st)
+            return out[:n.value].copy()
+        raise RigcError(RIGC_ERR_CAPACITY, "intersection capacity retry failed")
+
+    def count_intersections(self, poses):
+        """Number of intersecting (edge, face) pairs at poses (no list copied)."""
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        n = ctypes.c_int64()
+        self._check(self._L.rigc_detect_intersections(self._ctx, _p(p, _c_double_p), None, 0, ctypes.byref(n)))
+        return n.value
 
     def compute_earliest_toi(self, poses_t0, poses_t1, minimum_separation_distance=0.0):
         """Broad + narrow phase on the device; +inf when nothing collides."""

@@ -57,6 +57,12 @@ def lib():
             getattr(L, name).restype = ctypes.c_int
         L.oracle_set_eval_budget.argtypes = [ctypes.c_long]
         L.oracle_set_scene.argtypes = [ctypes.c_int] * 4 + [c_double_p] + [c_int32_p] * 7 + [ctypes.c_void_p]
+        L.oracle_orient3d.argtypes = [c_double_p] * 4
+        L.oracle_orient3d.restype = ctypes.c_int
+        L.oracle_edge_triangle.argtypes = [c_double_p] * 5
+        L.oracle_edge_triangle.restype = ctypes.c_int
+        L.oracle_detect_intersections.argtypes = [c_double_p, c_int32_p, ctypes.c_long]
+        L.oracle_detect_intersections.restype = ctypes.c_long
         L.oracle_broadphase.argtypes = [ctypes.c_int, c_double_p, c_double_p, ctypes.c_double, c_long_p, c_long_p]
         L.oracle_get_candidates.argtypes = [c_int32_p, c_int32_p]
         L.oracle_average_edge_length.restype = ctypes.c_double
@@ -129,6 +135,26 @@ def broadphase(poses_t0, poses_t1=None, inflation_radius=0.0):
     fv = np.zeros((nf.value, 2), np.int32)
     lib().oracle_get_candidates(i32ptr(ee), i32ptr(fv))
     return ee, fv
+
+
+def orient3d(a, b, c, d):
+    """Exact sign of det [a - d; b - d; c - d]."""
+    pts = [np.ascontiguousarray(x, np.float64) for x in (a, b, c, d)]
+    return lib().oracle_orient3d(*[dptr(x) for x in pts])
+
+
+def edge_triangle(p, q, t0, t1, t2):
+    pts = [np.ascontiguousarray(x, np.float64) for x in (p, q, t0, t1, t2)]
+    return bool(lib().oracle_edge_triangle(*[dptr(x) for x in pts]))
+
+
+def detect_intersections(poses):
+    """Sorted intersecting (edge, face) pairs at poses (current scene)."""
+    p = np.ascontiguousarray(poses, np.float64)
+    n = lib().oracle_detect_intersections(dptr(p), None, 0)
+    out = np.zeros((max(n, 1), 2), np.int32)
+    lib().oracle_detect_intersections(dptr(p), i32ptr(out), n)
+    return out[:n]
 
 
 def world_vertices(poses, nv):

@@ -38,5 +38,5 @@ def test_rigc_header_declares_the_path():
     names = declared(PAIRS[0][0], "rigc_")
     for n in ("rigc_broad_phase", "rigc_narrow_phase", "rigc_compute_earliest_toi", "rigc_construct_constraint_set",
               "rigc_compute_barrier_term", "rigc_compute_friction_term", "rigc_update_friction_constraints",
-              "rigc_compute_minimum_distance"):
+              "rigc_compute_minimum_distance", "rigc_detect_intersections"):
         assert n in names
