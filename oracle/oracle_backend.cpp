@@ -177,6 +177,11 @@ rigc_status ob_friction(void*, const double* poses, const double* poses_t0, doub
 
 const char* ob_error(void*) { return state().err.c_str(); }
 
+rigc_status ob_isect(void*, const double* poses, int64_t* n) {
+    *n = (int64_t)detect_intersections(scene(), poses).size();
+    return RIGC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -184,7 +189,7 @@ extern "C" {
 // Fills a rigs_backend table with the oracle's entry points.
 void oracle_backend(rigs_backend* out) {
     *out = rigs_backend{ nullptr, ob_set_scene, ob_toi, ob_build, ob_barrier, ob_mind, ob_fupdate, ob_friction,
-                         ob_error, ob_stats };
+                         ob_error, ob_stats, ob_isect };
 }
 
 }  // extern "C"

@@ -454,7 +454,7 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
                 P.scene = i;
                 P.nb = scenes[i].num_bodies;
                 b->tables[i] = rigs_backend{ &P, px_set_scene, px_toi, px_build, px_barrier, px_mind, px_fupdate,
-                                             px_friction, px_error, nullptr };
+                                             px_friction, px_error, nullptr, nullptr };
             }
         }
     }

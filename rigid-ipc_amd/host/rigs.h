@@ -48,6 +48,9 @@ typedef struct rigs_backend {
     const char* (*last_error)(void* self);
     /* Optional (may be NULL): counters of the last call (rigc_get_stats). */
     rigc_status (*get_stats)(void* self, rigc_stats* stats);
+    /* Optional (may be NULL): number of intersecting edge-face pairs at
+     * poses (rigc_detect_intersections without the list). */
+    rigc_status (*detect_intersections)(void* self, const double* poses, int64_t* count);
 } rigs_backend;
 
 /* Simulation settings; rigs_default_params fills the reference defaults
@@ -65,6 +68,9 @@ typedef struct rigs_params {
     int32_t max_iterations;
     double dhat_epsilon;
     double min_barrier_stiffness_scale;
+    int32_t do_intersection_check;     /* check the poses of every step for
+                                        * intersections (RigidBodyProblem,
+                                        * rigid_body_problem.cpp:261-267) */
 } rigs_params;
 
 typedef struct rigs_bodies {
@@ -107,6 +113,14 @@ typedef struct rigs_stats {
     double barrier_ms;           /* compute_barrier_term */
     double friction_ms;          /* compute_friction_term, update_friction_constraints */
     double distance_ms;          /* compute_minimum_distance */
+    /* Intersection checks (backend's detect_intersections): intersecting
+     * edge-face pairs of the initial state (checked at creation, as
+     * rigid_body_problem.cpp:100 does), of the last step's end poses when
+     * do_intersection_check is set, and the number of steps that ended
+     * with intersections; -1 where no check ran. */
+    int64_t initial_intersections;
+    int64_t intersections;
+    int64_t intersecting_steps;
 } rigs_stats;
 
 typedef struct rigs_sim rigs_sim;

@@ -68,6 +68,9 @@ rigc_status gb_friction(void* s, const double* p, const double* p0, double eh, i
 }
 const char* gb_error(void* s) { return rigc_last_error((rigc_ctx*)s); }
 rigc_status gb_stats(void* s, rigc_stats* st) { return rigc_get_stats((rigc_ctx*)s, st); }
+rigc_status gb_isect(void* s, const double* p, int64_t* n) {
+    return rigc_detect_intersections((rigc_ctx*)s, p, nullptr, 0, n);
+}
 
 struct BodyState {
     double mass = 0;
@@ -326,6 +329,18 @@ struct rigs_sim {
         stats.distance_ms += dt;
         if (s != RIGC_OK) return backend_fail(s, "compute_minimum_distance");
         out = std::isfinite(m) ? m : -1;
+        return RIGC_OK;
+    }
+
+    // RigidBodyProblem::detect_intersections through the backend (n = -1
+    // when the backend has no such entry point).
+    rigc_status intersections(const double* x, int64_t& n) {
+        n = -1;
+        if (!be.detect_intersections) return RIGC_OK;
+        const double t0 = now_ms();
+        rigc_status s = be.detect_intersections(be.self, x, &n);
+        stats.contact_ms += now_ms() - t0;
+        if (s != RIGC_OK) return backend_fail(s, "detect_intersections");
         return RIGC_OK;
     }
 
@@ -873,6 +888,13 @@ struct rigs_sim {
             B.velocity[4] = 0.5 * (W[2] - W[6]);
             B.velocity[5] = 0.5 * (W[3] - W[1]);
         }
+        if (P.do_intersection_check) {
+            int64_t n = -1;
+            s = intersections(x.data(), n);
+            if (s != RIGC_OK) return s;
+            stats.intersections = n;
+            if (n > 0) stats.intersecting_steps++;
+        }
         stats.steps++;
         stats.kappa = kappa;
         stats.total_ms += now_ms() - t_start;
@@ -918,6 +940,7 @@ void rigs_default_params(rigs_params* p) {
     p->max_iterations = 3000;
     p->dhat_epsilon = 1e-9;
     p->min_barrier_stiffness_scale = 1e11;
+    p->do_intersection_check = 0;
 }
 
 rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t device, const rigc_scene* scene,
@@ -939,7 +962,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
             return s;
         }
         sim->be = rigs_backend{ sim->own_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                gb_fupdate, gb_friction, gb_error, gb_stats };
+                                gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect };
     }
     sim->fbe = sim->be;
     rigc_status s = sim->be.set_scene(sim->be.self, scene);
@@ -956,7 +979,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         // term just runs through the main context)
         if (rigc_create(&sim->fric_ctx, device) == RIGC_OK && rigc_set_scene(sim->fric_ctx, scene) == RIGC_OK) {
             sim->fbe = rigs_backend{ sim->fric_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                     gb_fupdate, gb_friction, gb_error, gb_stats };
+                                     gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect };
         } else if (sim->fric_ctx) {
             rigc_destroy(sim->fric_ctx);
             sim->fric_ctx = nullptr;
@@ -1016,6 +1039,17 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
     for (int k = 0; k < 3; k++) d2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
     sim->bbox_diagonal = std::sqrt(d2);
     sim->poses_t0 = x0;
+    // the initial state is checked for intersections
+    // (RigidBodyProblem::init, rigid_body_problem.cpp:100)
+    int64_t n0 = -1;
+    s = sim->intersections(x0.data(), n0);
+    if (s != RIGC_OK) {
+        rigs_destroy(sim);
+        return s;
+    }
+    sim->stats.initial_intersections = n0;
+    sim->stats.intersections = -1;
+    sim->stats.contact_ms = 0;
     *out = sim;
     return RIGC_OK;
 }

@@ -229,6 +229,7 @@ def load(scene, meshes_dir=None, base_dir=None):
     sc.friction_iterations = int(fc["iterations"])
     sc.static_friction_speed_bound = float(fc["static_friction_speed_bound"])
     sc.newton = ipc
+    sc.do_intersection_check = bool(rbp.get("do_intersection_check", False))
     sc.name = str(scene.get("name", "scene"))
     return sc, args
 
@@ -256,6 +257,7 @@ def params_for(sc):
         W = np.concatenate(W)
         tol = tol / float(np.linalg.norm(W.max(0) - W.min(0)))
     p.velocity_conv_tol = tol
+    p.do_intersection_check = int(bool(sc.do_intersection_check))
     return p
 
 

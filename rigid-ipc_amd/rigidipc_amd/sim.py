@@ -22,7 +22,7 @@ class Backend(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in (
         "self", "set_scene", "compute_earliest_toi", "construct_constraint_set", "compute_barrier_term",
         "compute_minimum_distance", "update_friction_constraints", "compute_friction_term", "last_error",
-        "get_stats")]
+        "get_stats", "detect_intersections")]
 
 
 class Params(ctypes.Structure):
@@ -32,7 +32,7 @@ class Params(ctypes.Structure):
         ("friction_iterations", ctypes.c_int32), ("static_friction_speed_bound", ctypes.c_double),
         ("velocity_conv_tol", ctypes.c_double), ("line_search_lower_bound", ctypes.c_double),
         ("max_iterations", ctypes.c_int32), ("dhat_epsilon", ctypes.c_double),
-        ("min_barrier_stiffness_scale", ctypes.c_double),
+        ("min_barrier_stiffness_scale", ctypes.c_double), ("do_intersection_check", ctypes.c_int32),
     ]
 
 
@@ -53,7 +53,8 @@ class Stats(ctypes.Structure):
         ("ccd_narrow_phase_ms", ctypes.c_double), ("ccd_box_evaluations", ctypes.c_int64),
         ("ccd_kernel_ms", ctypes.c_double), ("ccd_launches", ctypes.c_int64),
         ("constraint_ms", ctypes.c_double), ("barrier_ms", ctypes.c_double), ("friction_ms", ctypes.c_double),
-        ("distance_ms", ctypes.c_double),
+        ("distance_ms", ctypes.c_double), ("initial_intersections", ctypes.c_int64),
+        ("intersections", ctypes.c_int64), ("intersecting_steps", ctypes.c_int64),
     ]
 
 

@@ -64,3 +64,25 @@ def test_config2_steps_without_penetration():
     assert st["failed_solves"] == 0
     assert min_vertex_height(sc, poses) > 0
     assert st["ccd_candidates"] > 0
+
+
+def test_intersection_checks_match_the_oracle_stepper():
+    """The stepper's intersection checks through the device backend: the
+    initial state's count and, with do_intersection_check, every step's."""
+    from test_host_stepper import intersecting_start_scene
+    counts = []
+    for backend in (None, oracle_backend()):
+        sim = M.Simulation(intersecting_start_scene(), backend=backend)
+        first = sim.stats()["initial_intersections"]
+        sim.close()
+        sc = S.config1(seed=0)
+        params = M.default_params(sc)
+        params.do_intersection_check = 1
+        sim = M.Simulation(sc, params=params, backend=backend)
+        sim.step(5)
+        st = sim.stats()
+        sim.close()
+        counts.append((first, st["initial_intersections"], st["intersections"], st["intersecting_steps"]))
+    print("intersection counts (device, oracle):", counts, flush=True)
+    assert counts[0] == counts[1]
+    assert counts[0][0] > 0 and counts[0][1:] == (0, 0, 0)

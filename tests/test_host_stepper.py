@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from stepper_util import free_body_scene, min_vertex_height, oracle_backend, resting_scene
+from rigidipc_amd import scenes as S
 from rigidipc_amd import sim as M
 
 
@@ -113,3 +114,43 @@ def test_friction_decelerates_sliding_body():
     assert np.abs(speeds[0.0] - 1.0).max() <= 1e-6
     decel = -np.diff(speeds[0.5][3:]).mean()
     assert decel == pytest.approx(0.5 * 9.81 * 0.01, rel=0.3)
+
+
+def intersecting_start_scene():
+    """A cube sunk 0.2 into the static plane: its side edges cross the
+    plane's triangles (an intersecting initial state)."""
+    Vc, Fc = S.cube(1.0)
+    body = S.Body(Vc, Fc, np.eye(3), np.array([4.0, 0.3, 1.0]), np.zeros(3), np.zeros(3))
+    sc = S.assemble([body, S._ground()])
+    sc.dt, sc.dhat, sc.mu, sc.gravity = 0.01, 1e-3, 0.0, np.array([0, -9.81, 0])
+    return sc
+
+
+def test_initial_state_is_checked_for_intersections():
+    import oracle_lib as O
+    sc = intersecting_start_scene()
+    O.set_scene(sc)
+    want = len(O.detect_intersections(sc.poses))
+    sim = M.Simulation(sc, backend=oracle_backend())
+    st = sim.stats()
+    sim.close()
+    assert want > 0 and st["initial_intersections"] == want
+    assert st["intersections"] == -1 and st["intersecting_steps"] == 0
+    sim = M.Simulation(resting_scene(), backend=oracle_backend())
+    assert sim.stats()["initial_intersections"] == 0
+    sim.close()
+
+
+def test_intersection_check_per_step_leaves_the_motion_alone():
+    runs = {}
+    for check in (0, 1):
+        sc = resting_scene(mu=0.5, v=(1.0, 0.0, 0.0))
+        params = M.default_params(sc)
+        params.do_intersection_check = check
+        sim = M.Simulation(sc, params=params, backend=oracle_backend())
+        sim.step(4)
+        runs[check] = (sim.state()[0], sim.stats())
+        sim.close()
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1]["intersections"] == -1
+    assert runs[1][1]["intersections"] == 0 and runs[1][1]["intersecting_steps"] == 0

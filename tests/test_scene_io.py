@@ -139,3 +139,15 @@ def test_sim_json_has_the_fields_the_reference_tools_read(tmp_path):
     seq = j["animation"]["state_sequence"]
     assert len(seq) == 4 and len(seq[0]["rigid_bodies"]) == 2
     assert j["stats"]["num_timesteps"] == 3 and len(j["stats"]["solver_iterations"]) == 3
+
+
+def test_intersection_check_flag_reaches_the_stepper(tmp_path):
+    base = os.path.join(FIX, "unit-tests-5-cubes.json")
+    scene = json.load(open(base))
+    sc, _ = IO.load(base)
+    assert IO.params_for(sc).do_intersection_check == 0
+    scene.setdefault("rigid_body_problem", {})["do_intersection_check"] = True
+    p = tmp_path / "checked.json"
+    p.write_text(json.dumps(scene))
+    sc, _ = IO.load(str(p), base_dir=os.path.dirname(base))
+    assert IO.params_for(sc).do_intersection_check == 1
