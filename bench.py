@@ -1,6 +1,6 @@
 """Benchmark of the MI355X contact hot path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sim|ccd|batch] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sim|ccd|batch|intersect] [--no-cpu]
 
 Workload "sim" (default): the 1,000-body mixed scene (ico2 / torus / cube,
 seeded, plus the ground plane; dt 0.01, dhat 1e-3, mu 0.2) stepped with
@@ -21,6 +21,11 @@ Workload "batch" runs that comparison alone for K steps (default all 10).
 Workload "ccd": one device CCD call (continuous broad phase + rigid
 interval narrow phase, rigc_compute_earliest_toi) per step along a
 ballistic trajectory of the same scene; value = CCD candidate pairs/s.
+
+Workload "intersect": the edge-triangle intersection check
+(rigc_detect_intersections) of the same scene with its dynamic bodies
+pulled towards their centroid (many interpenetrating pairs), K checks;
+value = checks/s, the CPU baseline one oracle check of the same poses.
 """
 import argparse
 import ctypes
@@ -272,6 +277,58 @@ def bench_ccd(args):
     return line
 
 
+# ------------------------------------------------------------------ intersection check
+def compressed_poses(sc, factor=0.85, seed=13):
+    """Dynamic bodies pulled towards their centroid by `factor` and turned by
+    N(0, 0.3) rad about each axis: many interpenetrating body pairs."""
+    rng = np.random.default_rng(seed)
+    p = sc.poses.copy()
+    dyn = ~sc.is_static.astype(bool)
+    c = p[dyn, :3].mean(axis=0)
+    p[dyn, :3] = c + (p[dyn, :3] - c) * factor
+    p[dyn, 3:] += rng.normal(size=(int(dyn.sum()), 3)) * 0.3
+    return p
+
+
+def bench_intersect(args):
+    import rigidipc_amd
+    from rigidipc_amd import scenes as S
+    sc = S.config2(seed=0)
+    p = compressed_poses(sc)
+    ctx = rigidipc_amd.Context(0)
+    ctx.set_scene(sc)
+    for _ in range(args.warmup):
+        ctx.detect_intersections(p)
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pairs = ctx.detect_intersections(p)
+        dev_ms += ctx.stats()["broad_phase_ms"]
+    wall = time.perf_counter() - t0
+    ctx.close()
+    line = dict(
+        metric=METRIC, value=args.steps / wall, unit="intersection checks/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f64", data="synthetic",
+        config=dict(workload="intersection check (rigc_detect_intersections, every intersecting edge-face pair, "
+                             "sorted) of the 1000-body mixed scene with the dynamic bodies pulled to 0.85 of their "
+                             "distance from the centroid and turned by N(0, 0.3) rad",
+                    bodies=int(len(sc.group_ids)), intersecting_pairs=int(len(pairs)),
+                    device_ms_per_check=dev_ms / args.steps),
+    )
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        O.set_scene(sc)
+        t = time.perf_counter()
+        n = len(O.detect_intersections(p))
+        dt = time.perf_counter() - t
+        line["cpu_baseline"] = dict(value=1.0 / dt, unit="intersection checks/s", cores=1, kind="port",
+                                    sample=f"one oracle check of the same poses (single thread, exact big-integer "
+                                           f"orientations): {dt:.2f} s, {n} intersecting pairs")
+    return line
+
+
 # ------------------------------------------------------------------ batch
 def batch_compare(n_scenes, steps, warm=True):
     """The batching workload: n_scenes independent scenes (body counts
@@ -346,7 +403,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "batch", "copy"])
+    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "batch", "copy", "intersect"])
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="steps of the batched-vs-plain comparison added to the sim line (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -370,6 +427,8 @@ def main():
         line = bench_batch(args)
     elif args.workload == "ccd":
         line = bench_ccd(args)
+    elif args.workload == "intersect":
+        line = bench_intersect(args)
     else:
         line = bench_sim(args)
         if args.batch_steps > 0:

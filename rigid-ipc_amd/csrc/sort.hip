@@ -31,25 +31,26 @@ __global__ __launch_bounds__(SORT_THREADS) void hist_kernel(const unsigned long 
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of m entries with one block of 1024 threads.
+// Exclusive scan of m entries with one block of 1024 threads: each thread
+// sums a contiguous chunk, the chunk sums are scanned with wavefront
+// shuffles plus one pass over the 16 wavefront totals, and each thread
+// rewrites its chunk from its offset.
 __global__ __launch_bounds__(1024) void scan_kernel(unsigned int* a, long long m) {
-    __shared__ unsigned int part[1024];
+    __shared__ unsigned int wsum[16];
     const long long chunk = (m + 1023) / 1024;
     const long long lo = (long long)threadIdx.x * chunk, hi = lo + chunk < m ? lo + chunk : m;
     unsigned int s = 0;
     for (long long i = lo; i < hi; i++) s += a[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int run = 0;
-        for (int i = 0; i < 1024; i++) {
-            const unsigned int v = part[i];
-            part[i] = run;
-            run += v;
-        }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned int x = s;  // inclusive prefix within the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
     }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    unsigned int run = part[threadIdx.x];
+    unsigned int run = x - s;
+    for (int w = 0; w < wave; w++) run += wsum[w];
     for (long long i = lo; i < hi; i++) {
         const unsigned int v = a[i];
         a[i] = run;

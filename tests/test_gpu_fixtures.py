@@ -4,7 +4,9 @@ path and over the oracle: the trajectories must agree as in
 test_gpu_stepper (same Newton and CCD counts, poses within 1e-6, velocities
 within 1e-4), covering stacks (5 cubes), a tet dropped into a static corner
 (dhat 1e-4), edge-edge and face-vertex first contacts, sliding with
-friction, and a concave body on a static corner."""
+friction, and a concave body on a static corner. The runs check every
+step's end poses for intersections (do_intersection_check): the counts of
+both backends agree, and no step ends intersecting."""
 import os
 
 import numpy as np
@@ -29,7 +31,9 @@ CASES = [
 
 def run(path, steps, backend):
     sc, _ = IO.load(path)
-    s = M.Simulation(sc, params=IO.params_for(sc), backend=backend)
+    params = IO.params_for(sc)
+    params.do_intersection_check = 1
+    s = M.Simulation(sc, params=params, backend=backend)
     s.step(steps)
     poses, vel = s.state()
     st = s.stats()
@@ -50,3 +54,5 @@ def test_fixture_device_matches_oracle(name, steps):
     assert sg["ccd_calls"] == so["ccd_calls"]
     assert np.abs(pg - po).max() <= 1e-6
     assert np.abs(vg - vo).max() <= 1e-4
+    for k in ("initial_intersections", "intersections", "intersecting_steps"):
+        assert sg[k] == so[k] == 0, (k, sg[k], so[k])
