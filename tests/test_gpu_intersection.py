@@ -93,6 +93,42 @@ def test_rotated_cluster_matches_oracle(ctx):
     assert n > 100
 
 
+@pytest.mark.parametrize("seed", [31, 32, 33, 34])
+def test_rotated_clusters_seed_sweep(ctx, seed):
+    rng = np.random.default_rng(seed)
+    meshes = {0: S.icosphere(1, 0.5), 1: S.torus(0.5, 0.2, 12, 6), 2: S.cube(0.9)}
+    sc = S.assemble(S.lattice_bodies(rng, 24, (4.0, 4.0, 4.0), 1.0, 0.0, 1.0, 2.0, meshes=meshes))
+    n = compare(ctx, sc, compressed(sc, 0.7 + 0.05 * (seed % 4), seed + 100))
+    print("seed", seed, "intersecting pairs", n, flush=True)
+
+
+def test_multi_scene_context_is_the_union_of_its_scenes(ctx):
+    """Scenes of one context never pair with each other: the list is each
+    scene's own list, shifted by the scene's edge and face offsets."""
+    rng = np.random.default_rng(41)
+    meshes = {0: S.icosphere(2, 0.5), 1: S.torus(0.5, 0.15), 2: S.cube(1.0)}
+    scs = [S.assemble(S.lattice_bodies(rng, 12, (4.0, 4.0, 4.0), 1.25, 0.0, 1.0, 2.0, meshes=meshes))
+           for _ in range(3)]
+    poses = [compressed(sc, 0.7, 50 + i) for i, sc in enumerate(scs)]
+    want, e0, f0 = [], 0, 0
+    for sc, p in zip(scs, poses):
+        O.set_scene(sc)
+        w = O.detect_intersections(p).astype(np.int64)
+        want.append(w + [e0, f0])
+        e0 += len(sc.edges)
+        f0 += len(sc.faces)
+    want = np.concatenate(want)
+    want = want[np.lexsort((want[:, 1], want[:, 0]))]
+    import gpu_util
+    multi = gpu_util.context()
+    multi.set_scenes(scs)
+    got = multi.detect_intersections(np.concatenate(poses))
+    multi.close()
+    print("multi-scene intersecting pairs", len(got), flush=True)
+    assert len(got) > 0
+    np.testing.assert_array_equal(got, want)
+
+
 def test_mixed_scene_compressed_matches_oracle(ctx):
     sc = S.config2(seed=3, n=200)
     n = compare(ctx, sc, compressed(sc, 0.85, 13))
