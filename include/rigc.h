@@ -237,6 +237,10 @@ rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* sce
 rigc_status rigc_batch_barrier_term(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* build_poses,
                                     const double* poses, double dhat, double dmin, int32_t flags, double* energy,
                                     double* grad, rigc_block_hessian* hess, int64_t* counts, double* min_distance);
+/* Per-scene rigc_detect_intersections at poses: counts[i] intersecting
+ * edge-face pairs of scene scene_ids[i]. */
+rigc_status rigc_batch_detect_intersections(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* poses,
+                                            int64_t* counts);
 /* Per-scene rigc_compute_minimum_distance over the set built at build_poses. */
 rigc_status rigc_batch_minimum_distance(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids,
                                         const double* build_poses, const double* poses, double dhat, double dmin,

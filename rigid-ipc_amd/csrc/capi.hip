@@ -192,7 +192,8 @@ rigc_status load_scenes(rigc_ctx* ctx, int n, const rigc_scene* scenes, const ch
     // of RigidBody's constructor (src/physics/rigid_body.cpp:124-142) and
     // RigidBody::edge_length (src/physics/rigid_body.hpp:151-156).
     std::vector<double> V, r_max, elen;
-    std::vector<int> E, F, FE, v_body, e_body, f_body, bvs{ 0 }, bes{ 0 }, bfs{ 0 }, group, body_scene, scene_body{ 0 };
+    std::vector<int> E, F, FE, v_body, e_body, f_body, bvs{ 0 }, bes{ 0 }, bfs{ 0 }, group, body_scene;
+    std::vector<int> scene_body{ 0 }, scene_edge{ 0 };
     std::vector<uint8_t> is_static;
     V.reserve(3 * (size_t)NV);
     int gmax = 0;
@@ -233,6 +234,7 @@ rigc_status load_scenes(rigc_ctx* ctx, int n, const rigc_scene* scenes, const ch
         }
         gmax += std::max(g0, 1);
         scene_body.push_back((int)group.size());
+        scene_edge.push_back((int)(E.size() / 2));
     }
 
     RIGC_HIP(hipSetDevice(ctx->device));
@@ -265,6 +267,7 @@ rigc_status load_scenes(rigc_ctx* ctx, int n, const rigc_scene* scenes, const ch
     RIGC_HIP(upload(ctx, &d.body_scene, body_scene.data(), body_scene.size()));
     RIGC_HIP(upload(ctx, &d.scene_body, scene_body.data(), scene_body.size()));
     d.h_scene_body = scene_body;
+    d.h_scene_edge = scene_edge;
     ctx->h_body_scene = body_scene;
     std::vector<int> all(n);
     for (int k = 0; k < n; k++) all[k] = k;
@@ -1141,6 +1144,37 @@ rigc_status rigc_batch_minimum_distance(rigc_ctx* ctx, int32_t n, const int32_t*
     RIGC_HIP(rigc::minimum_distance(ctx->as, ctx->scene, ctx->sel_req.sel, ctx->poses0.as<double>(), m2,
                                     ctx->stream));
     for (int i = 0; i < n; i++) min_distance[i] = std::sqrt(m2[i]);
+    return RIGC_OK;
+}
+
+rigc_status rigc_batch_detect_intersections(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids,
+                                            const double* poses, int64_t* counts) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!poses || !counts) return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_detect_intersections: bad arguments");
+    rigc_status s = select_scenes(ctx, n, scene_ids, "rigc_batch_detect_intersections");
+    if (s == RIGC_OK) s = upload_sel_poses(ctx, ctx->sel_req, poses, nullptr);
+    if (s != RIGC_OK) return s;
+    RIGC_HIP(rigc::detect_intersections(ctx->as, ctx->bp, ctx->ix, ctx->scene, ctx->sel_req.sel,
+                                        ctx->poses0.as<double>(), ctx->stream));
+    const long long m = ctx->ix.n;
+    std::vector<int32_t> pairs(2 * (size_t)m);
+    if (m)
+        RIGC_HIP(hipMemcpyAsync(pairs.data(), ctx->ix.out.p, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    RIGC_HIP(hipStreamSynchronize(ctx->stream));
+    // the pairs are sorted by edge, and each scene's edges are one range
+    const std::vector<int>& se = ctx->scene.h_scene_edge;
+    long long q = 0;
+    for (int i = 0; i < n; i++) {
+        const int lo = se[scene_ids[i]], hi = se[scene_ids[i] + 1];
+        while (q < m && pairs[2 * q] < lo) q++;
+        long long c = 0;
+        while (q < m && pairs[2 * q] < hi) {
+            c++;
+            q++;
+        }
+        counts[i] = c;
+    }
     return RIGC_OK;
 }
 

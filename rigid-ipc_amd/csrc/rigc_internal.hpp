@@ -37,6 +37,7 @@ struct DevScene {
     int* body_scene = nullptr;      // [nb]
     int* scene_body = nullptr;      // [n_scenes + 1] body ranges
     std::vector<int> h_scene_body;  // host copy of scene_body
+    std::vector<int> h_scene_edge;  // [n_scenes + 1] edge ranges (host)
 };
 
 // Scenes taking part in one call: their ids (ascending), their bodies, and

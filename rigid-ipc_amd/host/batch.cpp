@@ -42,7 +42,7 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-enum ReqKind { REQ_TOI = 0, REQ_BARRIER, REQ_MIND, REQ_FUPDATE, REQ_FRICTION };
+enum ReqKind { REQ_TOI = 0, REQ_BARRIER, REQ_MIND, REQ_FUPDATE, REQ_FRICTION, REQ_ISECT };
 
 struct Request {
     int kind = 0;
@@ -232,6 +232,28 @@ rigc_status submit(Proxy* P, Request& r) {
     return r.status;
 }
 
+// Intersection check: in a round like the other calls while the scenes
+// step; outside stepping (the check of the initial state when a scene's
+// stepper is created, before any round can run) a direct one-scene call.
+rigc_status px_isect(void* s, const double* poses, int64_t* n) {
+    Proxy* P = static_cast<Proxy*>(s);
+    Batch& B = *P->B;
+    {
+        std::lock_guard<std::mutex> lk(B.m);
+        if (B.active == 0) {
+            const int32_t id = P->scene;
+            const rigc_status st = rigc_batch_detect_intersections(B.ctx, 1, &id, poses, n);
+            if (st != RIGC_OK) P->err = rigc_last_error(B.ctx);
+            return st;
+        }
+    }
+    Request r;
+    r.kind = REQ_ISECT;
+    r.p0 = poses;
+    r.count = n;
+    return submit(P, r);
+}
+
 // A scene finished stepping (or failed), or one of its two callers is done
 // or waiting for the other: one caller fewer takes part in rounds.
 void leave(Batch& B) {
@@ -321,6 +343,12 @@ void Batch::run_group(int kind, std::vector<Request*>& g) {
         s = rigc_batch_minimum_distance(ctx, k, ids.data(), c0.data(), c1.data(), g[0]->a, g[0]->b, vals.data());
         if (s == RIGC_OK)
             for (int i = 0; i < k; i++) *g[i]->value = vals[i];
+    } else if (kind == REQ_ISECT) {
+        concat(c0, false);
+        cnt.assign(k, 0);
+        s = rigc_batch_detect_intersections(ctx, k, ids.data(), c0.data(), cnt.data());
+        if (s == RIGC_OK)
+            for (int i = 0; i < k; i++) *g[i]->count = cnt[i];
     } else if (kind == REQ_FUPDATE) {
         concat(c0, false);
         kap.resize(k);
@@ -454,7 +482,7 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
                 P.scene = i;
                 P.nb = scenes[i].num_bodies;
                 b->tables[i] = rigs_backend{ &P, px_set_scene, px_toi, px_build, px_barrier, px_mind, px_fupdate,
-                                             px_friction, px_error, nullptr, nullptr };
+                                             px_friction, px_error, nullptr, px_isect };
             }
         }
     }

@@ -102,12 +102,14 @@ def lib():
     L.rigc_batch_barrier_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, i32, _c_double_p,
                                           _c_double_p, bh, _c_int64_p, _c_double_p]
     L.rigc_batch_minimum_distance.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, _c_double_p]
+    L.rigc_batch_detect_intersections.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_int64_p]
     L.rigc_batch_update_friction_constraints.argtypes = [vp, i32, _c_int32_p, _c_double_p, d, d, _c_double_p,
                                                          _c_double_p, _c_int64_p]
     L.rigc_batch_friction_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, _c_double_p, i32,
                                            _c_double_p, _c_double_p, bh]
     for name in ("rigc_set_scenes", "rigc_get_scene_count", "rigc_batch_earliest_toi", "rigc_batch_barrier_term",
                  "rigc_batch_minimum_distance", "rigc_batch_update_friction_constraints",
+                 "rigc_batch_detect_intersections",
                  "rigc_batch_friction_term"):
         getattr(L, name).restype = ctypes.c_int32
     for name in ("rigc_create", "rigc_destroy", "rigc_get_stats", "rigc_set_scene", "rigc_narrow_phase",
@@ -284,6 +286,14 @@ class Context:
                                                     _p(p, _c_double_p), float(dhat), float(dmin), 0,
                                                     _p(e, _c_double_p), None, None, None, _p(md, _c_double_p)))
         return e, md
+
+    def batch_detect_intersections(self, ids, poses):
+        """Per-scene intersecting edge-face pair counts (rigc_batch_detect_intersections)."""
+        ids, p = self._ids_poses(ids, poses)
+        counts = np.zeros(len(ids), np.int64)
+        self._check(self._L.rigc_batch_detect_intersections(self._ctx, len(ids), _p(ids, _c_int32_p),
+                                                            _p(p, _c_double_p), _p(counts, _c_int64_p)))
+        return counts
 
     def batch_minimum_distance(self, ids, build_poses, poses, dhat, dmin=0.0):
         ids, pb, p = self._ids_poses(ids, build_poses, poses)

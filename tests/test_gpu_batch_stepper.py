@@ -65,3 +65,41 @@ def test_single_scene_stepper_with_helpers_matches_fused():
     assert s0["newton_iterations"] > 0
     single.close()
     fused.close()
+
+
+def test_fused_batch_intersection_checks_match_sequential():
+    """do_intersection_check in both batch modes: the fused proxies serve the
+    check in rounds (and directly for the initial state, before any round),
+    with the counts of the plain path; none of the steps ends intersecting."""
+    scenes = [S.config3_scene(i) for i in (1, 4, 6)]
+    params = []
+    for sc in scenes:
+        p = M.default_params(sc)
+        p.do_intersection_check = 1
+        params.append(p)
+    seq = M.Batch(scenes, mode=M.BATCH_SEQUENTIAL, params=params)
+    fused = M.Batch(scenes, mode=M.BATCH_FUSED, params=params)
+    seq.step(2)
+    fused.step(2)
+    for i in range(len(scenes)):
+        s0, s1 = seq.stats(i), fused.stats(i)
+        for key in ("initial_intersections", "intersections", "intersecting_steps"):
+            assert s1[key] == s0[key] == 0, (i, key, s0[key], s1[key])
+        np.testing.assert_array_equal(fused.state(i)[0], seq.state(i)[0])
+    seq.close()
+    fused.close()
+
+
+def test_fused_batch_counts_an_intersecting_initial_state():
+    import oracle_lib as O
+    from test_host_stepper import intersecting_start_scene
+    scenes = [intersecting_start_scene(), S.config3_scene(2), intersecting_start_scene()]
+    fused = M.Batch(scenes, mode=M.BATCH_FUSED)
+    got = [fused.stats(i)["initial_intersections"] for i in range(len(scenes))]
+    fused.close()
+    want = []
+    for sc in scenes:
+        O.set_scene(sc)
+        want.append(len(O.detect_intersections(sc.poses)))
+    print("initial intersections", got, want, flush=True)
+    assert got == want and want[0] > 0 and want[1] == 0

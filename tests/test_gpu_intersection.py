@@ -159,3 +159,23 @@ def test_overflow_passes_give_the_same_list():
     g = exact_scenes()["touching_grid"]
     np.testing.assert_array_equal(tiny.detect_intersections(g.poses), O.detect_intersections(g.poses))
     tiny.close()
+
+
+def test_batch_counts_match_single_scene_contexts():
+    import gpu_util
+    rng = np.random.default_rng(61)
+    meshes = {0: S.icosphere(2, 0.5), 1: S.torus(0.5, 0.15), 2: S.cube(1.0)}
+    scs = [S.assemble(S.lattice_bodies(rng, 10, (4.0, 4.0, 4.0), 1.25, 0.0, 1.0, 2.0, meshes=meshes))
+           for _ in range(4)]
+    poses = [compressed(sc, 0.7, 70 + i) for i, sc in enumerate(scs)]
+    multi = gpu_util.context()
+    multi.set_scenes(scs)
+    ids = [0, 2, 3]
+    got = multi.batch_detect_intersections(ids, [poses[i] for i in ids])
+    multi.close()
+    want = []
+    for i in ids:
+        O.set_scene(scs[i])
+        want.append(len(O.detect_intersections(poses[i])))
+    print("batch counts", list(got), want, flush=True)
+    assert list(got) == want and sum(want) > 0
