@@ -148,13 +148,16 @@ def edge_triangle(p, q, t0, t1, t2):
     return bool(lib().oracle_edge_triangle(*[dptr(x) for x in pts]))
 
 
-def detect_intersections(poses):
-    """Sorted intersecting (edge, face) pairs at poses (current scene)."""
+def detect_intersections(poses, capacity=1 << 18):
+    """Sorted intersecting (edge, face) pairs at poses (current scene); one
+    oracle run unless more than `capacity` pairs intersect."""
     p = np.ascontiguousarray(poses, np.float64)
-    n = lib().oracle_detect_intersections(dptr(p), None, 0)
-    out = np.zeros((max(n, 1), 2), np.int32)
-    lib().oracle_detect_intersections(dptr(p), i32ptr(out), n)
-    return out[:n]
+    out = np.zeros((capacity, 2), np.int32)
+    n = lib().oracle_detect_intersections(dptr(p), i32ptr(out), capacity)
+    if n > capacity:
+        out = np.zeros((n, 2), np.int32)
+        lib().oracle_detect_intersections(dptr(p), i32ptr(out), n)
+    return out[:n].copy()
 
 
 def world_vertices(poses, nv):
