@@ -22,6 +22,7 @@ import copy
 import json
 import math
 import os
+import sys
 import time
 
 import numpy as np
@@ -273,6 +274,10 @@ def run(scene, out_path=None, steps=None, meshes_dir=None, device=0, backend=Non
     args["max_iterations"], args["max_time"] = n, -1
     s = M.Simulation(sc, params=params_for(sc), device=device, backend=backend)
     states, timings, iters, contacts, mind = [], [], [], [], []
+    # the reference's log lines for intersections (rigid_body_problem.cpp:100,
+    # SimState.cpp:389-393); sim.json has no field for them
+    if s.stats()["initial_intersections"] > 0:
+        print("The initial state contains intersections!", file=sys.stderr)
 
     def state():
         poses, vel = s.state()
@@ -287,6 +292,9 @@ def run(scene, out_path=None, steps=None, meshes_dir=None, device=0, backend=Non
         s.step(1)
         timings.append(time.perf_counter() - t0)
         st = s.stats()
+        if st["intersecting_steps"] > prev["intersecting_steps"]:
+            print(f"sim_state action=simulation_step sim_it={len(timings)} status=has_intersections",
+                  file=sys.stderr)
         iters.append(int(st["newton_iterations"] - prev["newton_iterations"]))
         contacts.append(int(st["num_constraints"]))
         mind.append(float(st["min_distance"]) if st["min_distance"] >= 0 else None)

@@ -151,3 +151,16 @@ def test_intersection_check_flag_reaches_the_stepper(tmp_path):
     p.write_text(json.dumps(scene))
     sc, _ = IO.load(str(p), base_dir=os.path.dirname(base))
     assert IO.params_for(sc).do_intersection_check == 1
+
+
+def test_runner_reports_an_intersecting_initial_state(tmp_path, capsys):
+    from stepper_util import oracle_backend
+    base = os.path.join(FIX, "unit-tests-5-cubes.json")
+    scene = json.load(open(base))
+    scene["rigid_body_problem"]["rigid_bodies"][1]["position"] = [0.25, 1.5, 0.125]  # into the first cube
+    p = tmp_path / "overlap.json"
+    p.write_text(json.dumps(scene))
+    IO.run(str(p), steps=0, meshes_dir=FIX, backend=oracle_backend())
+    assert "The initial state contains intersections!" in capsys.readouterr().err
+    IO.run(base, steps=0, backend=oracle_backend())
+    assert "intersections" not in capsys.readouterr().err
