@@ -125,6 +125,30 @@ rigc_status rigc_broad_phase(rigc_ctx* ctx, const double* poses_t0, const double
 rigc_status rigc_compute_earliest_toi(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
                                       double minimum_separation_distance, double* earliest_toi);
 
+/* Narrow phase for TrajectoryType::PIECEWISE_LINEAR. Replaces
+ * edge_edge_ccd / face_vertex_ccd (src/ccd/ccd.cpp:271-274, 339-342) with
+ * compute_piecewise_linear_{edge_edge,face_vertex}_time_of_impact
+ * (src/ccd/piecewise_linear/time_of_impact.cpp:296-649): the rigid
+ * trajectory is cut into pieces until each piece's deviation from a straight
+ * line is bounded, and each piece goes through Tight-Inclusion's linear CCD
+ * with that bound added to the separation distance. Every query searches
+ * t in [0, earliest_toi_in] (the reference's earliest_toi argument).
+ * toi_out / hit_out (optional, length n_ee + n_fv, EE first): hit 1 =
+ * impact (toi_out = TOI), 0 = none (+inf). *earliest_toi = the minimum over
+ * impacting queries (+inf if none). */
+rigc_status rigc_narrow_phase_piecewise_linear(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                               const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                                               double earliest_toi_in, double minimum_separation_distance,
+                                               double* earliest_toi, double* toi_out, int8_t* hit_out);
+
+/* rigc_compute_earliest_toi for TrajectoryType::PIECEWISE_LINEAR: the same
+ * continuous hash-grid broad phase (inflation minimum_separation_distance / 2,
+ * src/ccd/ccd.cpp:75-81), then the piecewise-linear narrow phase over
+ * t in [0,1] (DistanceBarrierConstraint::compute_earliest_toi,
+ * src/opt/distance_barrier_constraint.cpp:130-149). */
+rigc_status rigc_compute_earliest_toi_piecewise_linear(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                                       double minimum_separation_distance, double* earliest_toi);
+
 /* ---------------------------------------------------------------- assembly
  * Derivative flags of the potential entry points. */
 #define RIGC_GRADIENT 1

@@ -286,6 +286,14 @@ static D2<N> constraint_distance_t(const Constraint& c, const double* X /*4x3*/)
 static D12 constraint_distance(const Constraint& c, const double* X) { return constraint_distance_t<12>(c, X); }
 static double constraint_distance_value(const Constraint& c, const double* X) { return constraint_distance_t<1>(c, X).v; }
 
+// Squared distance of an element pair (type EE or FV, vertices packed as in
+// a constraint record) with its distance type classified at X.
+double element_distance_sq(int type, const double* X) {
+    Constraint c;
+    c.type = type;
+    return constraint_distance_value(c, X);
+}
+
 // edge_edge_mollifier(ea0, ea1, eb0, eb1, eps_x) value.
 static double mollifier_value(const double* X, double eps_x) {
     double ea[3], eb[3], n[3];

@@ -62,6 +62,9 @@ double barrier_term(const Scene& s, const double* poses, const std::vector<Const
                     double* grad, std::vector<std::array<double, 144>>* local_hess);
 
 double minimum_distance(const Scene& s, const double* poses, const std::vector<Constraint>& C);
+// Squared distance of an EE (ea0, ea1, eb0, eb1) or FV (v, f0, f1, f2)
+// vertex quadruple X[4][3], distance type classified at X.
+double element_distance_sq(int type, const double* X);
 
 // One constraint's potential and its unprojected vertex-space derivatives
 // (X: 4 slots x 3; g: 12; H: 12x12 row-major).

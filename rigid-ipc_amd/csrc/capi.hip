@@ -38,6 +38,7 @@ struct rigc_ctx {
 
     // per-call buffers
     DevBuf poses0, poses1, stage0, stage1, out_rows, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
+    DevBuf pl_arena, pl_ids;  // piecewise-linear CCD lane slices and re-run query list
     DevBuf poses2;  // the friction term's start-of-step poses
     // Content tags of poses0 / poses1 / poses2 (0: unknown; equal non-zero
     // tags, equal contents) and the host copies of their last full uploads:
@@ -328,7 +329,8 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     free_scene(ctx);
     for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->poses2, &ctx->stage0, &ctx->stage1, &ctx->out_rows, &ctx->ee, &ctx->fv,
                        &ctx->toi, &ctx->hit, &ctx->counters, &ctx->qbest, &ctx->qvisits, &ctx->qflag,
-                       &ctx->scene_best, &ctx->scene_skip, &ctx->scene_par, &ctx->sel_all.buf, &ctx->sel_req.buf })
+                       &ctx->scene_best, &ctx->scene_skip, &ctx->scene_par, &ctx->sel_all.buf, &ctx->sel_req.buf,
+                       &ctx->pl_arena, &ctx->pl_ids })
         b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1240,3 +1242,164 @@ rigc_status rigc_batch_friction_term(rigc_ctx* ctx, int32_t n, const int32_t* sc
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ piecewise-linear CCD
+namespace {
+
+// Runs the piecewise-linear narrow phase over nq queries whose candidate
+// arrays are already on the device. Pass 1 gives every lane a small slice;
+// queries that outgrow it run again with larger slices on fewer lanes.
+// Leaves per-query results in ctx->toi / ctx->hit and returns them on the host.
+rigc_status plccd_run(rigc_ctx* ctx, const int* d_ee, long long n_ee, const int* d_fv, long long n_fv,
+                      double earliest_toi, double msd, std::vector<double>& tois, std::vector<signed char>& hits) {
+    const DevScene& d = ctx->scene;
+    const long long nq = n_ee + n_fv;
+    hipStream_t st = ctx->stream;
+    tois.assign((size_t)nq, HUGE_VAL);
+    hits.assign((size_t)nq, 0);
+    ctx->stats.box_evaluations = 0;
+    ctx->stats.num_candidates = nq;
+    ctx->stats.ccd_launches = 0;
+    ctx->stats.ccd_kernel_ms = 0;
+    if (nq == 0) return RIGC_OK;
+    RIGC_HIP(ctx->toi.ensure(sizeof(double) * (size_t)nq));
+    RIGC_HIP(ctx->hit.ensure((size_t)nq));
+    RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
+    RIGC_HIP(hipMemsetAsync(ctx->counters.p, 0, 4 * sizeof(unsigned long long), st));
+    // (tstack, heap) capacities and lane counts of the passes; the last
+    // slice holds any traversal (<= 1e6 refinements, two pushes each)
+    struct Pass { int tcap; long long hcap; long long lanes; };
+    const Pass passes[3] = { { 64, 256, (long long)ctx->num_cus * 256 }, { 2048, 1LL << 16, 256 },
+                             { 8192, 1LL << 21, 8 } };
+    PlccdArgs a{};
+    a.V = d.V; a.E = d.E; a.F = d.F;
+    a.v_body = d.v_body; a.e_body = d.e_body; a.f_body = d.f_body;
+    a.poses_t0 = ctx->poses0.as<double>();
+    a.poses_t1 = ctx->poses1.as<double>();
+    a.ee = d_ee; a.fv = d_fv; a.n_ee = n_ee;
+    a.earliest_toi = earliest_toi;
+    a.msd = msd;
+    a.toi_out = ctx->toi.as<double>();
+    a.hit_out = ctx->hit.as<signed char>();
+    a.counters = ctx->counters.as<unsigned long long>();
+    std::vector<int> redo;
+    float total_ms = 0;
+    for (int p = 0; p < 3; p++) {
+        const long long n_run = p == 0 ? nq : (long long)redo.size();
+        if (n_run == 0) break;
+        const long long lanes = std::min<long long>(passes[p].lanes, (n_run + 63) / 64 * 64);
+        const size_t slice = (size_t)passes[p].tcap + 7 * (size_t)passes[p].hcap;
+        RIGC_HIP(ctx->pl_arena.ensure(sizeof(double) * slice * (size_t)lanes));
+        a.arena = ctx->pl_arena.as<double>();
+        a.n_lanes = lanes;
+        a.tstack_cap = passes[p].tcap;
+        a.heap_cap = passes[p].hcap;
+        a.n_queries = n_run;
+        a.query_ids = nullptr;
+        if (p > 0) {
+            RIGC_HIP(ctx->pl_ids.ensure(sizeof(int) * redo.size()));
+            RIGC_HIP(hipMemcpyAsync(ctx->pl_ids.p, redo.data(), sizeof(int) * redo.size(), hipMemcpyHostToDevice, st));
+            a.query_ids = ctx->pl_ids.as<int>();
+        }
+        const int block = 256;
+        const int grid = (int)((lanes + block - 1) / block);
+        RIGC_HIP(hipEventRecord(ctx->ev0, st));
+        RIGC_HIP(launch_plccd(a, grid, st));
+        RIGC_HIP(hipEventRecord(ctx->ev1, st));
+        RIGC_HIP(hipMemcpyAsync(hits.data(), ctx->hit.p, (size_t)nq, hipMemcpyDeviceToHost, st));
+        RIGC_HIP(hipStreamSynchronize(st));
+        float ms = 0;
+        RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        total_ms += ms;
+        ctx->stats.ccd_launches++;
+        redo.clear();
+        for (long long q = 0; q < nq; q++)
+            if (hits[(size_t)q] == 2) redo.push_back((int)q);
+    }
+    if (!redo.empty())
+        return ctx->fail(RIGC_ERR_UNRESOLVED, "piecewise-linear CCD: a query outgrew the largest lane slice");
+    unsigned long long cnt[2] = { 0, 0 };
+    RIGC_HIP(hipMemcpyAsync(tois.data(), ctx->toi.p, sizeof(double) * (size_t)nq, hipMemcpyDeviceToHost, st));
+    RIGC_HIP(hipMemcpyAsync(cnt, ctx->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    RIGC_HIP(hipStreamSynchronize(st));
+    ctx->stats.box_evaluations = (int64_t)cnt[0];
+    ctx->stats.ccd_kernel_ms = total_ms;
+    ctx->stats.narrow_phase_ms = total_ms;
+    return RIGC_OK;
+}
+
+}  // namespace
+
+rigc_status rigc_narrow_phase_piecewise_linear(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                               const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                                               double earliest_toi_in, double minimum_separation_distance,
+                                               double* earliest_toi, double* toi_out, int8_t* hit_out) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_narrow_phase_piecewise_linear: no scene");
+    if (!poses_t0 || !poses_t1 || !earliest_toi || n_ee < 0 || n_fv < 0 || (n_ee && !ee) || (n_fv && !fv)
+        || !(earliest_toi_in > 0 && earliest_toi_in <= 1) || !(minimum_separation_distance >= 0)
+        || n_ee + n_fv > (int64_t)INT32_MAX)
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase_piecewise_linear: bad arguments");
+    const DevScene& d = ctx->scene;
+    for (int64_t i = 0; i < n_ee; i++)
+        if (ee[2 * i] < 0 || ee[2 * i] >= d.ne || ee[2 * i + 1] < 0 || ee[2 * i + 1] >= d.ne)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase_piecewise_linear: edge id out of range");
+    for (int64_t i = 0; i < n_fv; i++)
+        if (fv[2 * i] < 0 || fv[2 * i] >= d.nf || fv[2 * i + 1] < 0 || fv[2 * i + 1] >= d.nv)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT,
+                             "rigc_narrow_phase_piecewise_linear: face/vertex id out of range");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses_t0, poses_t1);
+    if (s != RIGC_OK) return s;
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(ctx->ee.ensure(2 * sizeof(int32_t) * std::max<size_t>((size_t)n_ee, 1)));
+    RIGC_HIP(ctx->fv.ensure(2 * sizeof(int32_t) * std::max<size_t>((size_t)n_fv, 1)));
+    if (n_ee) RIGC_HIP(hipMemcpyAsync(ctx->ee.p, ee, 2 * sizeof(int32_t) * n_ee, hipMemcpyHostToDevice, st));
+    if (n_fv) RIGC_HIP(hipMemcpyAsync(ctx->fv.p, fv, 2 * sizeof(int32_t) * n_fv, hipMemcpyHostToDevice, st));
+    std::vector<double> tois;
+    std::vector<signed char> hits;
+    s = plccd_run(ctx, ctx->ee.as<int>(), n_ee, ctx->fv.as<int>(), n_fv, earliest_toi_in, minimum_separation_distance,
+                  tois, hits);
+    if (s != RIGC_OK) return s;
+    double g = HUGE_VAL;
+    for (size_t q = 0; q < tois.size(); q++) {
+        if (hits[q] == 1) g = std::min(g, tois[q]);
+        if (toi_out) toi_out[q] = tois[q];
+        if (hit_out) hit_out[q] = (int8_t)hits[q];
+    }
+    *earliest_toi = g;
+    return RIGC_OK;
+}
+
+rigc_status rigc_compute_earliest_toi_piecewise_linear(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                                       double minimum_separation_distance, double* earliest_toi) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_compute_earliest_toi_piecewise_linear: no scene");
+    if (!poses_t0 || !poses_t1 || !earliest_toi || !(minimum_separation_distance >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_compute_earliest_toi_piecewise_linear: bad arguments");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses_t0, poses_t1);
+    if (s != RIGC_OK) return s;
+    hipStream_t st = ctx->stream;
+    RIGC_HIP(hipEventRecord(ctx->ev0, st));
+    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
+                             ctx->poses1.as<double>(), minimum_separation_distance / 2.0, true, false, st));
+    RIGC_HIP(hipEventRecord(ctx->ev1, st));
+    RIGC_HIP(hipStreamSynchronize(st));
+    float bp_ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&bp_ms, ctx->ev0, ctx->ev1));
+    const long long n_ee = ctx->bp.n_ee, n_fv = ctx->bp.n_fv;
+    if (n_ee + n_fv > (long long)INT32_MAX)
+        return ctx->fail(RIGC_ERR_CAPACITY, "rigc_compute_earliest_toi_piecewise_linear: too many candidates");
+    std::vector<double> tois;
+    std::vector<signed char> hits;
+    s = plccd_run(ctx, ctx->bp.out_ee.as<int>(), n_ee, ctx->bp.out_fv.as<int>(), n_fv, 1.0,
+                  minimum_separation_distance, tois, hits);
+    if (s != RIGC_OK) return s;
+    ctx->stats.broad_phase_ms = bp_ms;
+    double g = HUGE_VAL;
+    for (size_t q = 0; q < tois.size(); q++)
+        if (hits[q] == 1) g = std::min(g, tois[q]);
+    *earliest_toi = g;
+    return RIGC_OK;
+}

@@ -366,4 +366,33 @@ hipError_t minimum_distance(Assembly& as, const DevScene& sc, const SceneSel& se
 void gather_rows(const int* rows, int n, int width, const double* src, double* dst, hipStream_t stream);
 void scatter_rows(const int* rows, int n, int width, const double* src, double* dst, hipStream_t stream);
 
+// Piecewise-linear CCD (plccd.hip): one lane per query, EE queries first.
+// Each lane owns arena[lane * (tstack_cap + 7 * heap_cap)] doubles.
+// hit_out: 1 impact (toi_out = TOI), 0 none (+inf), 2 arena slice too small.
+struct PlccdArgs {
+    const double* V;
+    const int* E;
+    const int* F;
+    const int* v_body;
+    const int* e_body;
+    const int* f_body;
+    const double* poses_t0;
+    const double* poses_t1;
+    const int* ee;               // [n_ee][2] edge pairs
+    const int* fv;               // [n_fv][2] (face, vertex)
+    long long n_ee;
+    const int* query_ids;        // queries to run (nullptr: 0 .. n_queries-1)
+    long long n_queries;
+    double earliest_toi;
+    double msd;                  // minimum separation distance
+    double* toi_out;             // [n_ee + n_fv]
+    signed char* hit_out;        // [n_ee + n_fv]
+    double* arena;
+    long long n_lanes;           // lanes that run queries (one arena slice each)
+    int tstack_cap;
+    long long heap_cap;
+    unsigned long long* counters;  // [0] boxes visited, [1] trajectory pieces
+};
+hipError_t launch_plccd(const PlccdArgs& a, int grid, hipStream_t stream);
+
 }  // namespace rigc

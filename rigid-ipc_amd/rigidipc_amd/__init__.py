@@ -86,6 +86,11 @@ def lib():
     L.rigc_broad_phase.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_int32_p, ctypes.c_int64,
                                    _c_int64_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
     L.rigc_compute_earliest_toi.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p]
+    L.rigc_narrow_phase_piecewise_linear.argtypes = [vp, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_int64,
+                                                     _c_int32_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                                     _c_double_p, _c_double_p, _c_int8_p]
+    L.rigc_compute_earliest_toi_piecewise_linear.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double,
+                                                             _c_double_p]
     L.rigc_detect_intersections.argtypes = [vp, _c_double_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
     d, i32, i64 = ctypes.c_double, ctypes.c_int32, ctypes.c_int64
     bh = ctypes.POINTER(BlockHessian)
@@ -116,7 +121,8 @@ def lib():
                  "rigc_broad_phase", "rigc_compute_earliest_toi", "rigc_construct_constraint_set",
                  "rigc_get_constraint_set", "rigc_compute_barrier_term", "rigc_compute_minimum_distance",
                  "rigc_update_friction_constraints", "rigc_get_friction_constraints",
-                 "rigc_compute_friction_term", "rigc_detect_intersections"):
+                 "rigc_compute_friction_term", "rigc_detect_intersections",
+                 "rigc_narrow_phase_piecewise_linear", "rigc_compute_earliest_toi_piecewise_linear"):
         getattr(L, name).restype = ctypes.c_int32
     _lib = L
     return L
@@ -393,6 +399,34 @@ class Context:
         toi = ctypes.c_double()
         self._check(self._L.rigc_compute_earliest_toi(self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p),
                                                       float(minimum_separation_distance), ctypes.byref(toi)))
+        return toi.value
+
+    def narrow_phase_piecewise_linear(self, poses_t0, poses_t1, ee, fv, earliest_toi=1.0,
+                                      minimum_separation_distance=0.0):
+        """Piecewise-linear CCD over candidates (TrajectoryType::PIECEWISE_LINEAR);
+        returns (earliest_toi, toi[n], hit[n]), EE queries first."""
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        p1 = np.ascontiguousarray(poses_t1, np.float64).reshape(-1, 6)
+        ee = np.ascontiguousarray(np.asarray(ee, np.int32).reshape(-1, 2))
+        fv = np.ascontiguousarray(np.asarray(fv, np.int32).reshape(-1, 2))
+        n = len(ee) + len(fv)
+        toi = np.empty(max(n, 1))
+        hit = np.empty(max(n, 1), np.int8)
+        best = ctypes.c_double()
+        self._check(self._L.rigc_narrow_phase_piecewise_linear(
+            self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p), _p(ee, _c_int32_p), len(ee), _p(fv, _c_int32_p),
+            len(fv), float(earliest_toi), float(minimum_separation_distance), ctypes.byref(best),
+            _p(toi, _c_double_p), _p(hit, _c_int8_p)))
+        return best.value, toi[:n], hit[:n]
+
+    def compute_earliest_toi_piecewise_linear(self, poses_t0, poses_t1, minimum_separation_distance=0.0):
+        """Broad phase + piecewise-linear narrow phase; +inf when nothing collides."""
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        p1 = np.ascontiguousarray(poses_t1, np.float64).reshape(-1, 6)
+        toi = ctypes.c_double()
+        self._check(self._L.rigc_compute_earliest_toi_piecewise_linear(
+            self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p), float(minimum_separation_distance),
+            ctypes.byref(toi)))
         return toi.value
 
     # ------------------------------------------------------------ assembly

@@ -55,6 +55,15 @@ def lib():
         for name in ("oracle_ee_toi", "oracle_fv_toi"):
             getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p]
             getattr(L, name).restype = ctypes.c_int
+        for name in ("oracle_pl_ee_toi", "oracle_pl_fv_toi"):
+            getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p,
+                                         c_long_p]
+            getattr(L, name).restype = ctypes.c_int
+        for name in ("oracle_ti_edge_edge", "oracle_ti_vertex_face"):
+            getattr(L, name).argtypes = [c_double_p, ctypes.c_double, c_double_p, c_double_p, c_long_p]
+            getattr(L, name).restype = ctypes.c_int
+        L.oracle_element_distance_sq.argtypes = [ctypes.c_int, c_double_p]
+        L.oracle_element_distance_sq.restype = ctypes.c_double
         L.oracle_set_eval_budget.argtypes = [ctypes.c_long]
         L.oracle_set_scene.argtypes = [ctypes.c_int] * 4 + [c_double_p] + [c_int32_p] * 7 + [ctypes.c_void_p]
         L.oracle_orient3d.argtypes = [c_double_p] * 4
@@ -101,6 +110,43 @@ def fv_toi(q, earliest_toi=1.0, toi_tol=1e-4):
     n = ctypes.c_long(0)
     hit = lib().oracle_fv_toi(dptr(q), earliest_toi, toi_tol, ctypes.byref(toi), ctypes.byref(n))
     return (hit if hit < 0 else bool(hit)), toi.value, n.value
+
+
+def pl_ee_toi(q, earliest_toi=1.0, msd=0.0):
+    """Piecewise-linear TOI of a packed EE query: (hit, toi, pieces, boxes)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    toi = ctypes.c_double()
+    npc, nbx = ctypes.c_long(0), ctypes.c_long(0)
+    hit = lib().oracle_pl_ee_toi(dptr(q), float(earliest_toi), float(msd), ctypes.byref(toi), ctypes.byref(npc),
+                                 ctypes.byref(nbx))
+    return bool(hit), toi.value, npc.value, nbx.value
+
+
+def pl_fv_toi(q, earliest_toi=1.0, msd=0.0):
+    """Piecewise-linear TOI of a packed FV query: (hit, toi, pieces, boxes)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    toi = ctypes.c_double()
+    npc, nbx = ctypes.c_long(0), ctypes.c_long(0)
+    hit = lib().oracle_pl_fv_toi(dptr(q), float(earliest_toi), float(msd), ctypes.byref(toi), ctypes.byref(npc),
+                                 ctypes.byref(nbx))
+    return bool(hit), toi.value, npc.value, nbx.value
+
+
+def ti_linear(V, vf, ms=0.0):
+    """Tight-Inclusion linear CCD of 8 vertices (start then end):
+    (hit, toi, output_tolerance, boxes)."""
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(8, 3)
+    toi, tol = ctypes.c_double(), ctypes.c_double()
+    nbx = ctypes.c_long(0)
+    fn = lib().oracle_ti_vertex_face if vf else lib().oracle_ti_edge_edge
+    hit = fn(dptr(V), float(ms), ctypes.byref(toi), ctypes.byref(tol), ctypes.byref(nbx))
+    return bool(hit), toi.value, tol.value, nbx.value
+
+
+def element_distance_sq(kind, X):
+    """Squared EE (kind 0: ea0 ea1 eb0 eb1) or FV (kind 1: v f0 f1 f2) distance."""
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(4, 3)
+    return lib().oracle_element_distance_sq(2 if kind == 0 else 3, dptr(X))
 
 
 def set_eval_budget(n):
