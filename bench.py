@@ -277,6 +277,84 @@ def bench_ccd(args):
     return line
 
 
+# ------------------------------------------------------------------ piecewise-linear CCD
+# FP64 operations of one box evaluation of the linear-trajectory function:
+# 8 corners x 3 axes x 19 (four lerps of the vertices in t, the two
+# interpolations in u, v, one difference), plus the 3 box widths.
+PL_FLOPS_PER_BOX = 8 * 3 * 19 + 3
+
+
+def cpu_baseline_plccd(sc, steps, budget_s):
+    """The oracle's piecewise-linear CCD, one thread, over the candidates of
+    the same sequence (oracle broad phase), until the budget runs out."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from test_gpu_broadphase import query_record
+    O.set_scene(sc)
+    t_q = 0.0
+    nq = done = 0
+    for p0, p1 in steps:
+        ee, fv = O.broadphase(p0, p1, 0.0)
+        work = [(O.pl_ee_toi if k == 0 else O.pl_fv_toi, query_record(sc, p0, p1, k, a, b))
+                for k, pairs in ((0, ee), (1, fv)) for a, b in pairs]
+        t0 = time.perf_counter()
+        for fn, q in work:
+            fn(q)
+        nq += len(work)
+        t_q += time.perf_counter() - t0
+        done += 1
+        if t_q > budget_s:
+            break
+    return dict(value=nq / t_q, unit="CCD candidate pairs/s", cores=1, kind="port",
+                sample=f"piecewise-linear narrow phase of the oracle over all {nq} candidates of the first {done} "
+                       f"calls of the same sequence, one thread (query packing and broad phase not timed)")
+
+
+def bench_plccd(args):
+    import rigidipc_amd
+    from rigidipc_amd import scenes as S
+    sc = S.config2(seed=0)
+    seq = ballistic_steps(sc, args.warmup + args.steps)
+    ctx = rigidipc_amd.Context(0)
+    ctx.set_scene(sc)
+    for p0, p1 in seq[:args.warmup]:
+        ctx.compute_earliest_toi_piecewise_linear(p0, p1)
+    tot_cand = tot_evals = launches = 0
+    bp_ms = k_ms = 0.0
+    t0 = time.perf_counter()
+    for p0, p1 in seq[args.warmup:]:
+        ctx.compute_earliest_toi_piecewise_linear(p0, p1)
+        st = ctx.stats()
+        tot_cand += st["num_candidates"]
+        tot_evals += st["box_evaluations"]
+        bp_ms += st["broad_phase_ms"]
+        k_ms += st["ccd_kernel_ms"]
+        launches += st["ccd_launches"]
+    wall = time.perf_counter() - t0
+    ctx.close()
+    roof = None
+    if k_ms and launches:
+        avg_s = k_ms * 1e-3 / launches
+        achieved = tot_evals / launches * PL_FLOPS_PER_BOX / avg_s / 1e12
+        roof = dict(bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / FP64_PEAK_TFLOPS, traffic=None, kernel="plccd_kernel",
+                    flops_per_box=PL_FLOPS_PER_BOX, box_evaluations=int(tot_evals), launches=int(launches),
+                    kernel_ms_total=k_ms, kernel_us_avg=1e3 * k_ms / launches)
+    line = dict(
+        metric=METRIC, value=tot_cand / wall, unit="CCD candidate pairs/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f64", data="synthetic",
+        config=dict(workload="config2 ballistic CCD sequence, piecewise-linear trajectories: 1000 mixed bodies + "
+                             "plane, one rigc_compute_earliest_toi_piecewise_linear per step",
+                    bodies=int(len(sc.group_ids)), candidates_per_step=tot_cand / args.steps,
+                    broad_phase_ms_per_step=bp_ms / args.steps, narrow_phase_ms_per_step=k_ms / args.steps),
+        roofline=roof,
+    )
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_plccd(sc, seq[args.warmup:], args.cpu_budget)
+    return line
+
+
 # ------------------------------------------------------------------ intersection check
 def compressed_poses(sc, factor=0.85, seed=13):
     """Dynamic bodies pulled towards their centroid by `factor` and turned by
@@ -403,7 +481,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "batch", "copy", "intersect"])
+    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "plccd", "batch", "copy", "intersect"])
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="steps of the batched-vs-plain comparison added to the sim line (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -427,6 +505,8 @@ def main():
         line = bench_batch(args)
     elif args.workload == "ccd":
         line = bench_ccd(args)
+    elif args.workload == "plccd":
+        line = bench_plccd(args)
     elif args.workload == "intersect":
         line = bench_intersect(args)
     else:
