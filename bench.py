@@ -337,7 +337,7 @@ def bench_plccd(args):
         avg_s = k_ms * 1e-3 / launches
         achieved = tot_evals / launches * PL_FLOPS_PER_BOX / avg_s / 1e12
         roof = dict(bound="fp64", achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / FP64_PEAK_TFLOPS, traffic=None, kernel="plccd_kernel",
+                    frac=achieved / FP64_PEAK_TFLOPS, traffic=None, kernel="plccd_wave_kernel",
                     flops_per_box=PL_FLOPS_PER_BOX, box_evaluations=int(tot_evals), launches=int(launches),
                     kernel_ms_total=k_ms, kernel_us_avg=1e3 * k_ms / launches)
     line = dict(

@@ -1269,7 +1269,9 @@ rigc_status plccd_run(rigc_ctx* ctx, const int* d_ee, long long n_ee, const int*
     // (tstack, heap) capacities and lane counts of the passes; the last
     // slice holds any traversal (<= 1e6 refinements, two pushes each)
     struct Pass { int tcap; long long hcap; long long lanes; };
-    const Pass passes[3] = { { 64, 256, (long long)ctx->num_cus * 256 }, { 2048, 1LL << 16, 256 },
+    // pass 0: one query per wavefront in LDS (stack 64, heap 512), up to
+    // 16 wavefronts per CU resident
+    const Pass passes[3] = { { 64, 512, (long long)ctx->num_cus * 16 }, { 2048, 1LL << 16, 256 },
                              { 8192, 1LL << 21, 8 } };
     PlccdArgs a{};
     a.V = d.V; a.E = d.E; a.F = d.F;
@@ -1287,9 +1289,12 @@ rigc_status plccd_run(rigc_ctx* ctx, const int* d_ee, long long n_ee, const int*
     for (int p = 0; p < 3; p++) {
         const long long n_run = p == 0 ? nq : (long long)redo.size();
         if (n_run == 0) break;
-        const long long lanes = std::min<long long>(passes[p].lanes, (n_run + 63) / 64 * 64);
-        const size_t slice = (size_t)passes[p].tcap + 7 * (size_t)passes[p].hcap;
-        RIGC_HIP(ctx->pl_arena.ensure(sizeof(double) * slice * (size_t)lanes));
+        const long long lanes = p == 0 ? std::min<long long>(passes[p].lanes, n_run)
+                                       : std::min<long long>(passes[p].lanes, (n_run + 63) / 64 * 64);
+        if (p > 0) {
+            const size_t slice = (size_t)passes[p].tcap + 7 * (size_t)passes[p].hcap;
+            RIGC_HIP(ctx->pl_arena.ensure(sizeof(double) * slice * (size_t)lanes));
+        }
         a.arena = ctx->pl_arena.as<double>();
         a.n_lanes = lanes;
         a.tstack_cap = passes[p].tcap;
@@ -1301,10 +1306,9 @@ rigc_status plccd_run(rigc_ctx* ctx, const int* d_ee, long long n_ee, const int*
             RIGC_HIP(hipMemcpyAsync(ctx->pl_ids.p, redo.data(), sizeof(int) * redo.size(), hipMemcpyHostToDevice, st));
             a.query_ids = ctx->pl_ids.as<int>();
         }
-        const int block = 256;
-        const int grid = (int)((lanes + block - 1) / block);
         RIGC_HIP(hipEventRecord(ctx->ev0, st));
-        RIGC_HIP(launch_plccd(a, grid, st));
+        if (p == 0) RIGC_HIP(launch_plccd_wave(a, (int)lanes, st));
+        else RIGC_HIP(launch_plccd(a, (int)((lanes + 255) / 256), st));
         RIGC_HIP(hipEventRecord(ctx->ev1, st));
         RIGC_HIP(hipMemcpyAsync(hits.data(), ctx->hit.p, (size_t)nq, hipMemcpyDeviceToHost, st));
         RIGC_HIP(hipStreamSynchronize(st));

@@ -16,10 +16,11 @@
 // traversal over (t, u, v). The traversal's priority queue (level first,
 // then the lower end of t) is a binary heap with the sift-up / sift-down
 // steps of libstdc++'s std::priority_queue, so boxes with equal keys are
-// served in the same order as in the reference's build. Each lane's piece
-// stack and heap live in its own slice of an arena in HBM; a query whose
-// heap or stack would outgrow its slice stops with status 2 and is run
-// again in a later pass with larger slices (fewer lanes). A heap of 2^21
+// served in the same order as in the reference's build. The first pass
+// runs one query per wavefront with its piece stack and heap in LDS; later
+// passes run one query per lane with both in the lane's slice of an arena in
+// HBM. A query whose heap or stack would outgrow its space stops with
+// status 2 and is run again in the next pass with more space. A heap of 2^21
 // entries holds any traversal: at most 1e6 refinements push at most two
 // boxes each.
 #include "rigc_internal.hpp"
@@ -374,6 +375,44 @@ RIGC_DI int pl_query(Lane& L, bool vf, const double (*loc)[3], const double* pA0
     return result;
 }
 
+// Query i of the list: gathers its vertices and bodies, runs it, writes its
+// result (writer lanes only) and keeps the lane's counters.
+RIGC_DI void run_query(const PlccdArgs& a, Lane& L, long long i, bool writer) {
+    const long long q = a.query_ids ? (long long)a.query_ids[i] : i;
+    const bool vf = q >= a.n_ee;
+    double loc[4][3];
+    int bodyA, bodyB;
+    if (!vf) {
+        const int ea = a.ee[2 * q], eb = a.ee[2 * q + 1];
+        const int vid[4] = { a.E[2 * ea], a.E[2 * ea + 1], a.E[2 * eb], a.E[2 * eb + 1] };
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++) loc[j][k] = a.V[3 * vid[j] + k];
+        bodyA = a.e_body[ea];
+        bodyB = a.e_body[eb];
+    } else {
+        const long long r = q - a.n_ee;
+        const int f = a.fv[2 * r], v = a.fv[2 * r + 1];
+        const int vid[4] = { v, a.F[3 * f], a.F[3 * f + 1], a.F[3 * f + 2] };
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++) loc[j][k] = a.V[3 * vid[j] + k];
+        bodyA = a.v_body[v];
+        bodyB = a.f_body[f];
+    }
+    double toi;
+    const unsigned long long boxes0 = L.boxes, pieces0 = L.pieces;
+    const int st = pl_query(L, vf, loc, a.poses_t0 + 6 * bodyA, a.poses_t1 + 6 * bodyA, a.poses_t0 + 6 * bodyB,
+                            a.poses_t1 + 6 * bodyB, a.earliest_toi, a.msd, toi);
+    if (st == 2) {  // counted when the query is run again with a larger slice
+        L.boxes = boxes0;
+        L.pieces = pieces0;
+    }
+    if (writer) {
+        a.toi_out[q] = st == 1 ? toi : HUGE_VAL;
+        a.hit_out[q] = (signed char)st;
+    }
+}
+
+// One query per lane, piece stack and heap in the lane's arena slice (HBM).
 __global__ __launch_bounds__(256) void plccd_kernel(PlccdArgs a) {
     const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long lanes = a.n_lanes;  // the arena holds n_lanes slices
@@ -385,40 +424,31 @@ __global__ __launch_bounds__(256) void plccd_kernel(PlccdArgs a) {
     L.hcap = a.heap_cap;
     L.boxes = 0;
     L.pieces = 0;
-    for (long long i = lane; i < a.n_queries; i += lanes) {
-        const long long q = a.query_ids ? (long long)a.query_ids[i] : i;
-        const bool vf = q >= a.n_ee;
-        double loc[4][3];
-        int bodyA, bodyB;
-        if (!vf) {
-            const int ea = a.ee[2 * q], eb = a.ee[2 * q + 1];
-            const int vid[4] = { a.E[2 * ea], a.E[2 * ea + 1], a.E[2 * eb], a.E[2 * eb + 1] };
-            for (int j = 0; j < 4; j++)
-                for (int k = 0; k < 3; k++) loc[j][k] = a.V[3 * vid[j] + k];
-            bodyA = a.e_body[ea];
-            bodyB = a.e_body[eb];
-        } else {
-            const long long r = q - a.n_ee;
-            const int f = a.fv[2 * r], v = a.fv[2 * r + 1];
-            const int vid[4] = { v, a.F[3 * f], a.F[3 * f + 1], a.F[3 * f + 2] };
-            for (int j = 0; j < 4; j++)
-                for (int k = 0; k < 3; k++) loc[j][k] = a.V[3 * vid[j] + k];
-            bodyA = a.v_body[v];
-            bodyB = a.f_body[f];
-        }
-        double toi;
-        const unsigned long long boxes0 = L.boxes, pieces0 = L.pieces;
-        const int st = pl_query(L, vf, loc, a.poses_t0 + 6 * bodyA, a.poses_t1 + 6 * bodyA, a.poses_t0 + 6 * bodyB,
-                                a.poses_t1 + 6 * bodyB, a.earliest_toi, a.msd, toi);
-        if (st == 2) {  // counted when the query is run again with a larger slice
-            L.boxes = boxes0;
-            L.pieces = pieces0;
-        }
-        a.toi_out[q] = st == 1 ? toi : HUGE_VAL;
-        a.hit_out[q] = (signed char)st;
-    }
+    for (long long i = lane; i < a.n_queries; i += lanes) run_query(a, L, i, true);
     if (L.boxes) atomicAdd(a.counters + 0, L.boxes);
     if (L.pieces) atomicAdd(a.counters + 1, L.pieces);
+}
+
+// One query per wavefront, piece stack and heap in LDS: every lane runs the
+// same query on the same LDS data (uniform control flow, identical values),
+// lane 0 writes the result. A heap operation then waits on LDS instead of
+// HBM; queries that outgrow the LDS heap go to the arena passes.
+constexpr int kWaveStack = 64;
+constexpr int kWaveHeap = 512;
+__global__ __launch_bounds__(64) void plccd_wave_kernel(PlccdArgs a) {
+    __shared__ double s_tstack[kWaveStack];
+    __shared__ double s_heap[kItem * kWaveHeap];
+    Lane L;
+    L.tcap = kWaveStack;
+    L.tstack = s_tstack;
+    L.heap = s_heap;
+    L.hcap = kWaveHeap;
+    L.boxes = 0;
+    L.pieces = 0;
+    const bool writer = threadIdx.x == 0;
+    for (long long i = blockIdx.x; i < a.n_queries; i += gridDim.x) run_query(a, L, i, writer);
+    if (writer && L.boxes) atomicAdd(a.counters + 0, L.boxes);
+    if (writer && L.pieces) atomicAdd(a.counters + 1, L.pieces);
 }
 
 }  // namespace
@@ -426,6 +456,12 @@ __global__ __launch_bounds__(256) void plccd_kernel(PlccdArgs a) {
 hipError_t launch_plccd(const PlccdArgs& a, int grid, hipStream_t st) {
     if (a.n_queries <= 0) return hipSuccess;
     hipLaunchKernelGGL(plccd_kernel, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_plccd_wave(const PlccdArgs& a, int grid, hipStream_t st) {
+    if (a.n_queries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(plccd_wave_kernel, dim3(grid), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

@@ -394,5 +394,7 @@ struct PlccdArgs {
     unsigned long long* counters;  // [0] boxes visited, [1] trajectory pieces
 };
 hipError_t launch_plccd(const PlccdArgs& a, int grid, hipStream_t stream);
+// One query per 64-lane block, stack (64) and heap (512 entries) in LDS; n_lanes unused.
+hipError_t launch_plccd_wave(const PlccdArgs& a, int grid, hipStream_t stream);
 
 }  // namespace rigc
