@@ -19,6 +19,8 @@
 using namespace oracle;
 
 extern "C" double oracle_earliest_toi(const double* p0, const double* p1, double msd, long* n_candidates);
+extern "C" double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double msd, int trajectory,
+                                                 long* n_candidates);
 
 namespace {
 
@@ -55,6 +57,12 @@ rigc_status ob_set_scene(void*, const rigc_scene* sc) {
 rigc_status ob_toi(void*, const double* p0, const double* p1, double msd, double* toi) {
     long n = 0;
     *toi = oracle_earliest_toi(p0, p1, msd, &n);
+    state().last_candidates = n;
+    return RIGC_OK;
+}
+rigc_status ob_toi_pl(void*, const double* p0, const double* p1, double msd, double* toi) {
+    long n = 0;
+    *toi = oracle_earliest_toi_trajectory(p0, p1, msd, 1, &n);
     state().last_candidates = n;
     return RIGC_OK;
 }
@@ -189,7 +197,7 @@ extern "C" {
 // Fills a rigs_backend table with the oracle's entry points.
 void oracle_backend(rigs_backend* out) {
     *out = rigs_backend{ nullptr, ob_set_scene, ob_toi, ob_build, ob_barrier, ob_mind, ob_fupdate, ob_friction,
-                         ob_error, ob_stats, ob_isect };
+                         ob_error, ob_stats, ob_isect, ob_toi_pl };
 }
 
 }  // extern "C"

@@ -190,10 +190,14 @@ void oracle_last_vertex_aabbs(double* out) {
     const std::vector<double>& v = last_vertex_aabbs();
     std::copy(v.begin(), v.end(), out);
 }
-// DistanceBarrierConstraint::compute_earliest_toi for RIGID trajectories
+// DistanceBarrierConstraint::compute_earliest_toi
 // (src/opt/distance_barrier_constraint.cpp:130-265) with every query on
-// t in [0,1] and the minimum over queries, parallel over candidates.
-double oracle_earliest_toi(const double* p0, const double* p1, double msd, long* n_candidates) {
+// t in [0,1] and the minimum over queries, parallel over candidates; the
+// narrow phase per trajectory type as edge_edge_ccd / face_vertex_ccd
+// dispatch it (src/ccd/ccd.cpp:271-279, 339-347): 0 RIGID, 1 PIECEWISE_LINEAR
+// (with the caller's minimum separation).
+double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double msd, int trajectory,
+                                      long* n_candidates) {
     const Scene& s = scene();
     Candidates c;
     broadphase_ccd(s, p0, p1, msd / 2.0, c);
@@ -216,7 +220,7 @@ double oracle_earliest_toi(const double* p0, const double* p1, double msd, long*
                 q.poseA_t0[k] = p0[6 * A + k]; q.poseA_t1[k] = p1[6 * A + k];
                 q.poseB_t0[k] = p0[6 * B + k]; q.poseB_t1[k] = p1[6 * B + k];
             }
-            hit = edge_edge_toi(q, toi);
+            hit = trajectory == 1 ? pl_edge_edge_toi(q, toi, 1.0, msd, nullptr, nullptr) : edge_edge_toi(q, toi);
         } else {
             const int f = c.fv[i - ne][0], v = c.fv[i - ne][1];
             const int A = s.v_body[v], B = s.f_body[f];
@@ -230,11 +234,14 @@ double oracle_earliest_toi(const double* p0, const double* p1, double msd, long*
                 q.poseA_t0[k] = p0[6 * A + k]; q.poseA_t1[k] = p1[6 * A + k];
                 q.poseB_t0[k] = p0[6 * B + k]; q.poseB_t1[k] = p1[6 * B + k];
             }
-            hit = face_vertex_toi(q, toi);
+            hit = trajectory == 1 ? pl_face_vertex_toi(q, toi, 1.0, msd, nullptr, nullptr) : face_vertex_toi(q, toi);
         }
         if (hit && toi < best) best = toi;
     }
     return best;
+}
+double oracle_earliest_toi(const double* p0, const double* p1, double msd, long* n_candidates) {
+    return oracle_earliest_toi_trajectory(p0, p1, msd, 0, n_candidates);
 }
 
 double oracle_average_edge_length() { return scene().average_edge_length; }

@@ -55,6 +55,11 @@ bool edge_edge_toi(const EEQuery& q, double& toi, double earliest_toi = 1.0,
                    double toi_tol = 1e-4, long* n_evals = nullptr);
 bool face_vertex_toi(const FVQuery& q, double& toi, double earliest_toi = 1.0,
                      double toi_tol = 1e-4, long* n_evals = nullptr);
+// Piecewise-linear trajectories (oracle_plccd.cpp).
+bool pl_edge_edge_toi(const EEQuery& q, double& toi, double earliest_toi, double msd, long* n_pieces,
+                      long* n_boxes);
+bool pl_face_vertex_toi(const FVQuery& q, double& toi, double earliest_toi, double msd, long* n_pieces,
+                        long* n_boxes);
 
 // Inclusion functions (rigid_trajectory_aabb.cpp:70-102), exposed for tests.
 Vec3I edge_edge_aabb(const EEQuery& q, const Interval& t, const Interval& alpha, const Interval& beta);

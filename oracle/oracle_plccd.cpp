@@ -327,11 +327,10 @@ bool pl_toi(bool vf, const double* const* LA, int nA, const double* const* LB, i
         double pa_ti1[6], pb_ti1[6];
         interpolate_pose(pA0, pA1, ti1, pa_ti1);
         interpolate_pose(pB0, pB1, ti1, pb_ti1);
-        // EE measures the distance at the piece's start, FV at its end
-        // (time_of_impact.cpp:335-339 and :525-528)
+        // both element types measure the distance at the piece's start
+        // (time_of_impact.cpp:335-339 and :529-533)
         double Xd[4][3];
-        if (vf) world_all(pa_ti1, pb_ti1, Xd);
-        else world_all(pa_ti0, pb_ti0, Xd);
+        world_all(pa_ti0, pb_ti0, Xd);
         const double distance_ti0 = std::sqrt(element_distance_sq(type, &Xd[0][0]));
         if (distance_ti0 < kDecreasingDistanceFactor * distance_t0 && ti0 >= kDecreasingDistanceMinTime) {
             toi = ti0;

@@ -39,6 +39,7 @@ struct rigc_ctx {
     // per-call buffers
     DevBuf poses0, poses1, stage0, stage1, out_rows, ee, fv, toi, hit, counters, qbest, qvisits, qflag;
     DevBuf pl_arena, pl_ids;  // piecewise-linear CCD lane slices and re-run query list
+    bool pl_small_slices = false;
     DevBuf poses2;  // the friction term's start-of-step poses
     // Content tags of poses0 / poses1 / poses2 (0: unknown; equal non-zero
     // tags, equal contents) and the host copies of their last full uploads:
@@ -318,6 +319,10 @@ rigc_status rigc_create(rigc_ctx** out, int32_t device) {
     // body-pair capacities from this floor instead of the defaults (a small
     // value drives the passes that regrow them; used by the tests)
     if (const char* mc = std::getenv("RIGC_BROAD_PHASE_MIN_CAPACITY")) ctx->bp.min_cap = std::atoll(mc);
+    // RIGC_PLCCD_SMALL_SLICES: the piecewise-linear CCD skips its LDS pass
+    // and starts with tiny HBM lane slices, so that most queries outgrow the
+    // first two passes (used by the tests of the re-run passes)
+    if (const char* ss = std::getenv("RIGC_PLCCD_SMALL_SLICES")) ctx->pl_small_slices = std::atoi(ss) != 0;
     *out = ctx;
     return RIGC_OK;
 }
@@ -1269,10 +1274,13 @@ rigc_status plccd_run(rigc_ctx* ctx, const int* d_ee, long long n_ee, const int*
     // (tstack, heap) capacities and lane counts of the passes; the last
     // slice holds any traversal (<= 1e6 refinements, two pushes each)
     struct Pass { int tcap; long long hcap; long long lanes; };
-    // pass 0: one query per wavefront in LDS (stack 64, heap 512), up to
-    // 16 wavefronts per CU resident
-    const Pass passes[3] = { { 64, 512, (long long)ctx->num_cus * 16 }, { 2048, 1LL << 16, 256 },
+    // pass 0: one query per wavefront (64-lane block) in LDS (stack 64,
+    // heap 512), 16 blocks per CU requested
+    const Pass normal[3] = { { 64, 512, (long long)ctx->num_cus * 16 }, { 2048, 1LL << 16, 256 },
                              { 8192, 1LL << 21, 8 } };
+    // test knob: every pass in HBM slices, the first two tiny
+    const Pass small[3] = { { 2, 8, (long long)ctx->num_cus * 16 }, { 4, 64, 256 }, { 8192, 1LL << 21, 8 } };
+    const Pass* passes = ctx->pl_small_slices ? small : normal;
     PlccdArgs a{};
     a.V = d.V; a.E = d.E; a.F = d.F;
     a.v_body = d.v_body; a.e_body = d.e_body; a.f_body = d.f_body;
@@ -1289,9 +1297,10 @@ rigc_status plccd_run(rigc_ctx* ctx, const int* d_ee, long long n_ee, const int*
     for (int p = 0; p < 3; p++) {
         const long long n_run = p == 0 ? nq : (long long)redo.size();
         if (n_run == 0) break;
-        const long long lanes = p == 0 ? std::min<long long>(passes[p].lanes, n_run)
+        const bool in_lds = p == 0 && !ctx->pl_small_slices;
+        const long long lanes = in_lds ? std::min<long long>(passes[p].lanes, n_run)
                                        : std::min<long long>(passes[p].lanes, (n_run + 63) / 64 * 64);
-        if (p > 0) {
+        if (!in_lds) {
             const size_t slice = (size_t)passes[p].tcap + 7 * (size_t)passes[p].hcap;
             RIGC_HIP(ctx->pl_arena.ensure(sizeof(double) * slice * (size_t)lanes));
         }
@@ -1307,7 +1316,7 @@ rigc_status plccd_run(rigc_ctx* ctx, const int* d_ee, long long n_ee, const int*
             a.query_ids = ctx->pl_ids.as<int>();
         }
         RIGC_HIP(hipEventRecord(ctx->ev0, st));
-        if (p == 0) RIGC_HIP(launch_plccd_wave(a, (int)lanes, st));
+        if (in_lds) RIGC_HIP(launch_plccd_wave(a, (int)lanes, st));
         else RIGC_HIP(launch_plccd(a, (int)((lanes + 255) / 256), st));
         RIGC_HIP(hipEventRecord(ctx->ev1, st));
         RIGC_HIP(hipMemcpyAsync(hits.data(), ctx->hit.p, (size_t)nq, hipMemcpyDeviceToHost, st));

@@ -325,9 +325,8 @@ RIGC_DI int pl_query(Lane& L, bool vf, const double (*loc)[3], const double* pA0
             pa1[i] = (pA1[i] - pA0[i]) * ti1 + pA0[i];
             pb1[i] = (pB1[i] - pB0[i]) * ti1 + pB0[i];
         }
-        // EE measures the distance at the piece's start, FV at its end
-        if (vf) world_all(pa1, pb1, X);
-        else world_all(pa0, pb0, X);
+        // both element types measure the distance at the piece's start
+        world_all(pa0, pb0, X);
         const double distance_ti0 = element_distance(vf, X);
         if (distance_ti0 < kDecreasingDistanceFactor * distance_t0 && ti0 >= kDecreasingDistanceMinTime) {
             toi = ti0;

@@ -458,6 +458,10 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
         || (mode != RIGS_BATCH_FUSED && mode != RIGS_BATCH_SEQUENTIAL))
         return RIGC_ERR_INVALID_ARGUMENT;
     *out = nullptr;
+    if (mode == RIGS_BATCH_FUSED)
+        for (int i = 0; i < num_scenes; i++)
+            if (params[i].trajectory_type != RIGS_TRAJECTORY_RIGID)
+                return RIGC_ERR_INVALID_ARGUMENT;  // the fused rounds serve the rigid CCD only
     rigs_batch* b = new rigs_batch();
     b->mode = mode;
     b->n = num_scenes;

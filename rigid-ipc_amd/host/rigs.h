@@ -51,7 +51,16 @@ typedef struct rigs_backend {
     /* Optional (may be NULL): number of intersecting edge-face pairs at
      * poses (rigc_detect_intersections without the list). */
     rigc_status (*detect_intersections)(void* self, const double* poses, int64_t* count);
+    /* Optional (may be NULL; then piecewise-linear scenes are refused):
+     * compute_earliest_toi over piecewise-linear trajectories
+     * (rigc_compute_earliest_toi_piecewise_linear). */
+    rigc_status (*compute_earliest_toi_piecewise_linear)(void* self, const double* poses_t0, const double* poses_t1,
+                                                         double minimum_separation_distance, double* earliest_toi);
 } rigs_backend;
+
+/* Trajectory types of the CCD (TrajectoryType, src/ccd/ccd.hpp:30-38; the
+ * linearised and Redon types are not built). */
+enum { RIGS_TRAJECTORY_RIGID = 0, RIGS_TRAJECTORY_PIECEWISE_LINEAR = 1 };
 
 /* Simulation settings; rigs_default_params fills the reference defaults
  * (src/SimState.cpp:130-208, src/constants.hpp). */
@@ -71,6 +80,8 @@ typedef struct rigs_params {
     int32_t do_intersection_check;     /* check the poses of every step for
                                         * intersections (RigidBodyProblem,
                                         * rigid_body_problem.cpp:261-267) */
+    int32_t trajectory_type;           /* RIGS_TRAJECTORY_*: the line search's
+                                        * CCD (distance_barrier_constraint.trajectory_type) */
 } rigs_params;
 
 typedef struct rigs_bodies {

@@ -71,6 +71,9 @@ rigc_status gb_stats(void* s, rigc_stats* st) { return rigc_get_stats((rigc_ctx*
 rigc_status gb_isect(void* s, const double* p, int64_t* n) {
     return rigc_detect_intersections((rigc_ctx*)s, p, nullptr, 0, n);
 }
+rigc_status gb_toi_pl(void* s, const double* p0, const double* p1, double msd, double* toi) {
+    return rigc_compute_earliest_toi_piecewise_linear((rigc_ctx*)s, p0, p1, msd, toi);
+}
 
 struct BodyState {
     double mass = 0;
@@ -346,7 +349,10 @@ struct rigs_sim {
 
     rigc_status earliest_toi(const double* x0, const double* x1, double& toi) {
         const double t0 = now_ms();
-        rigc_status s = be.compute_earliest_toi(be.self, x0, x1, P.minimum_separation_distance, &toi);
+        // compute_earliest_toi dispatches on the trajectory type (ccd.cpp:271-279)
+        auto fn = P.trajectory_type == RIGS_TRAJECTORY_PIECEWISE_LINEAR ? be.compute_earliest_toi_piecewise_linear
+                                                                         : be.compute_earliest_toi;
+        rigc_status s = fn(be.self, x0, x1, P.minimum_separation_distance, &toi);
         const double dt = now_ms() - t0;
         stats.contact_ms += dt;
         stats.ccd_ms += dt;
@@ -941,12 +947,18 @@ void rigs_default_params(rigs_params* p) {
     p->dhat_epsilon = 1e-9;
     p->min_barrier_stiffness_scale = 1e11;
     p->do_intersection_check = 0;
+    p->trajectory_type = RIGS_TRAJECTORY_RIGID;
 }
 
 rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t device, const rigc_scene* scene,
                         const rigs_bodies* bodies, const rigs_params* params) {
     if (!out || !scene || !bodies || !params || bodies->num_bodies != scene->num_bodies || !bodies->poses
-        || !bodies->velocities || !bodies->mass || !bodies->inertia || !(params->timestep > 0) || !(params->dhat > 0))
+        || !bodies->velocities || !bodies->mass || !bodies->inertia || !(params->timestep > 0) || !(params->dhat > 0)
+        || (params->trajectory_type != RIGS_TRAJECTORY_RIGID
+            && params->trajectory_type != RIGS_TRAJECTORY_PIECEWISE_LINEAR))
+        return RIGC_ERR_INVALID_ARGUMENT;
+    if (params->trajectory_type == RIGS_TRAJECTORY_PIECEWISE_LINEAR && backend
+        && !backend->compute_earliest_toi_piecewise_linear)
         return RIGC_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     rigs_sim* sim = new rigs_sim();
@@ -962,7 +974,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
             return s;
         }
         sim->be = rigs_backend{ sim->own_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect };
+                                gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl };
     }
     sim->fbe = sim->be;
     rigc_status s = sim->be.set_scene(sim->be.self, scene);
@@ -979,7 +991,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         // term just runs through the main context)
         if (rigc_create(&sim->fric_ctx, device) == RIGC_OK && rigc_set_scene(sim->fric_ctx, scene) == RIGC_OK) {
             sim->fbe = rigs_backend{ sim->fric_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                     gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect };
+                                     gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl };
         } else if (sim->fric_ctx) {
             rigc_destroy(sim->fric_ctx);
             sim->fric_ctx = nullptr;

@@ -10,10 +10,11 @@ all fixed is static, as in RigidBody's constructor,
 src/physics/rigid_body.cpp:78-83), meshes from OBJ files or inline arrays,
 scale / dimensions, xyz Euler rotations and angular velocities in degrees,
 densities and group ids (with the reference's id assignment), and the
-solver settings the stepper uses. The contact path is this build's
-(hash-grid broad phase, rigid-trajectory CCD); a scene that asks for
-another detection method or trajectory type runs with these, and the
-written args say so. The coefficient of restitution is read and, as in the
+solver settings the stepper uses. The CCD runs with the scene's
+trajectory_type, "rigid" or "piecewise_linear" (the reference's default,
+src/SimState.cpp:179); "linear" and "redon" raise NotImplementedError. The
+broad phase is always the hash grid (the BVH gives the same constraint
+sets), and the written args say so. The coefficient of restitution is read and, as in the
 reference's distance-barrier problem, not used. Anything else (2D,
 kinematic bodies, partially fixed dofs, forces and torques, split
 components) raises NotImplementedError.
@@ -28,6 +29,10 @@ import time
 import numpy as np
 
 from . import scenes as S
+
+# trajectory_type names (src/ccd/ccd.hpp:30-38 via NLOHMANN_JSON_SERIALIZE_ENUM)
+# -> rigs_params.trajectory_type
+TRAJECTORY_TYPES = {"rigid": 0, "piecewise_linear": 1}
 
 # SimState::init defaults (src/SimState.cpp:126-205) for the keys used here,
 # with the constants filled in there (src/constants.hpp).
@@ -231,6 +236,10 @@ def load(scene, meshes_dir=None, base_dir=None):
     sc.static_friction_speed_bound = float(fc["static_friction_speed_bound"])
     sc.newton = ipc
     sc.do_intersection_check = bool(rbp.get("do_intersection_check", False))
+    traj = str(dbc.get("trajectory_type", "piecewise_linear"))
+    if traj not in TRAJECTORY_TYPES:
+        raise NotImplementedError(f"trajectory_type {traj!r}: only 'rigid' and 'piecewise_linear'")
+    sc.trajectory_type = traj
     sc.name = str(scene.get("name", "scene"))
     return sc, args
 
@@ -259,6 +268,7 @@ def params_for(sc):
         tol = tol / float(np.linalg.norm(W.max(0) - W.min(0)))
     p.velocity_conv_tol = tol
     p.do_intersection_check = int(bool(sc.do_intersection_check))
+    p.trajectory_type = TRAJECTORY_TYPES[getattr(sc, "trajectory_type", "rigid")]
     return p
 
 
@@ -270,7 +280,6 @@ def run(scene, out_path=None, steps=None, meshes_dir=None, device=0, backend=Non
     n = sc.steps if steps is None else int(steps)
     args = copy.deepcopy(args)
     args["distance_barrier_constraint"]["detection_method"] = "hash_grid"
-    args["distance_barrier_constraint"]["trajectory_type"] = "rigid"
     args["max_iterations"], args["max_time"] = n, -1
     s = M.Simulation(sc, params=params_for(sc), device=device, backend=backend)
     states, timings, iters, contacts, mind = [], [], [], [], []

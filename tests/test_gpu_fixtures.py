@@ -4,9 +4,10 @@ path and over the oracle: the trajectories must agree as in
 test_gpu_stepper (same Newton and CCD counts, poses within 1e-6, velocities
 within 1e-4), covering stacks (5 cubes), a tet dropped into a static corner
 (dhat 1e-4), edge-edge and face-vertex first contacts, sliding with
-friction, and a concave body on a static corner. The runs check every
-step's end poses for intersections (do_intersection_check): the counts of
-both backends agree, and no step ends intersecting."""
+friction, and a concave body on a static corner, each with the scene's
+trajectory type (piecewise-linear CCD) and with the rigid CCD. The runs
+check every step's end poses for intersections (do_intersection_check):
+the counts of both backends agree, and no step ends intersecting."""
 import os
 
 import numpy as np
@@ -29,8 +30,10 @@ CASES = [
 ]
 
 
-def run(path, steps, backend):
+def run(path, steps, backend, trajectory):
     sc, _ = IO.load(path)
+    if trajectory is not None:
+        sc.trajectory_type = trajectory
     params = IO.params_for(sc)
     params.do_intersection_check = 1
     s = M.Simulation(sc, params=params, backend=backend)
@@ -41,12 +44,15 @@ def run(path, steps, backend):
     return poses, vel, st
 
 
+@pytest.mark.parametrize("trajectory", [None, "rigid"])
 @pytest.mark.parametrize("name,steps", CASES)
-def test_fixture_device_matches_oracle(name, steps):
+def test_fixture_device_matches_oracle(name, steps, trajectory):
+    """trajectory None: the scene's own (these fixtures leave it at the
+    reference's default, piecewise_linear); "rigid": the rigid CCD."""
     path = os.path.join(FIX, name)
-    print(f"fixture {name}: {steps} steps", flush=True)
-    pg, vg, sg = run(path, steps, None)
-    po, vo, so = run(path, steps, oracle_backend())
+    print(f"fixture {name}: {steps} steps, trajectory {trajectory or 'scene'}", flush=True)
+    pg, vg, sg = run(path, steps, None, trajectory)
+    po, vo, so = run(path, steps, oracle_backend(), trajectory)
     print(f"newton {sg['newton_iterations']} / {so['newton_iterations']}, "
           f"max pose diff {np.abs(pg - po).max():.3e}", flush=True)
     assert sg["failed_solves"] == so["failed_solves"]

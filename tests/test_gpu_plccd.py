@@ -89,21 +89,64 @@ def test_partial_time_domain_bit_exact(ctx):
     check(ctx, kinds, qs, earliest=0.4)
 
 
-def test_earliest_toi_matches_oracle(ctx):
-    from test_gpu_broadphase import cluster, moved, query_record
+@pytest.fixture(scope="module")
+def small_ctx():
+    """A context whose piecewise-linear CCD starts in tiny HBM lane slices
+    (stack 2 / heap 8, then 4 / 64), so queries take the re-run passes."""
+    import gpu_util
+    os.environ["RIGC_PLCCD_SMALL_SLICES"] = "1"
+    try:
+        c = gpu_util.context()
+    finally:
+        del os.environ["RIGC_PLCCD_SMALL_SLICES"]
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("seed,rot,msd,earliest", [(2, 1.0, 0.0, 1.0), (3, 2.5, 1e-4, 1.0), (5, 1.0, 0.0, 0.4)])
+def test_rerun_passes_bit_exact(small_ctx, seed, rot, msd, earliest):
+    """Queries that outgrow their lane slice stop, have their box counts
+    taken back, and run again from a re-run list with larger slices; the
+    results and box totals must still be the oracle's."""
+    kinds, qs = scenes.random_queries(150, seed, rot_scale=rot)
+    check(small_ctx, kinds, qs, earliest=earliest, msd=msd)
+    assert small_ctx.stats()['ccd_launches'] >= 2
+
+
+def test_rerun_passes_through_broad_phase(small_ctx):
+    from test_gpu_broadphase import cluster, moved
     sc = cluster(4, n=27)
     p0, p1 = moved(sc, 34, dpos=0.05, drot=0.1)
+    small_ctx.set_scene(sc)
+    got = small_ctx.compute_earliest_toi_piecewise_linear(p0, p1, 0.0)
+    assert small_ctx.stats()['ccd_launches'] >= 2
+    assert got == _oracle_earliest(sc, p0, p1, 0.0)[0]
+
+
+def _oracle_earliest(sc, p0, p1, msd):
+    """The oracle's minimum TOI over the oracle's candidates (broad phase
+    inflated by msd / 2, as compute_earliest_toi does)."""
+    from test_gpu_broadphase import query_record
     O.set_scene(sc)
-    ee, fv = O.broadphase(p0, p1, 0.0)
+    ee, fv = O.broadphase(p0, p1, msd / 2)
     best = math.inf
     for k, pairs in ((0, ee), (1, fv)):
         for a, b in pairs:
-            hit, toi, _, _ = (O.pl_ee_toi if k == 0 else O.pl_fv_toi)(query_record(sc, p0, p1, k, a, b))
+            hit, toi, _, _ = (O.pl_ee_toi if k == 0 else O.pl_fv_toi)(query_record(sc, p0, p1, k, a, b), 1.0, msd)
             if hit:
                 best = min(best, toi)
+    return best, len(ee) + len(fv)
+
+
+@pytest.mark.parametrize("msd", [0.0, 1e-4])
+def test_earliest_toi_matches_oracle(ctx, msd):
+    from test_gpu_broadphase import cluster, moved
+    sc = cluster(4, n=27)
+    p0, p1 = moved(sc, 34, dpos=0.05, drot=0.1)
+    best, n_cand = _oracle_earliest(sc, p0, p1, msd)
     ctx.set_scene(sc)
-    got = ctx.compute_earliest_toi_piecewise_linear(p0, p1, 0.0)
+    got = ctx.compute_earliest_toi_piecewise_linear(p0, p1, msd)
     st = ctx.stats()
-    print(f"candidates {len(ee) + len(fv)} earliest toi {best} kernel {st['ccd_kernel_ms']:.3f} ms", flush=True)
-    assert st['num_candidates'] == len(ee) + len(fv)
+    print(f"msd {msd} candidates {n_cand} earliest toi {best} kernel {st['ccd_kernel_ms']:.3f} ms", flush=True)
+    assert st['num_candidates'] == n_cand
     assert got == best

@@ -51,11 +51,11 @@ def test_translation_closed_forms_piecewise(kind):
     a piece's deviation from a straight line does not see that a pure
     translation has none (the position interval appears on both sides), so
     it cuts the step into pieces, and it reports the start of a piece as an
-    impact once the distance at that piece's start (EE) or end (FV) has
-    fallen below a fifth of the initial one (time_of_impact.cpp:341-347,
-    :525-535). So: every expected impact is reported, never later than the
-    exact time, and a reported impact without contact is followed by a
-    distance below a fifth of the initial one."""
+    impact once the distance at that piece's start has fallen below a fifth
+    of the initial one (time_of_impact.cpp:341-347, :529-541). So: every
+    expected impact is reported, never later than the exact time, and a
+    reported impact without contact is followed by a distance below a fifth
+    of the initial one."""
     cases = C.ee_translation_cases() if kind == 0 else C.fv_translation_cases()
     fn = O.pl_ee_toi if kind == 0 else O.pl_fv_toi
     for q, want_hit, want in cases:
@@ -63,11 +63,46 @@ def test_translation_closed_forms_piecewise(kind):
         if want_hit:
             assert hit and toi <= want, (q[12:24], toi, want)
         elif hit:
-            # the distance is measured at a piece's start (EE) or end (FV)
+            # the distance is measured at a piece's start
             d0 = math.sqrt(O.element_distance_sq(kind, C.world(q, kind, [0.0])[0]))
             X = C.world(q, kind, np.linspace(toi, 1.0, 257))
             dmin = min(math.sqrt(O.element_distance_sq(kind, x)) for x in X)
             assert dmin < 0.2 * d0, (q[12:24], toi, d0, dmin)
+
+
+def test_face_vertex_exit_measures_the_piece_start():
+    """A vertex approaches a triangle that spins in its own plane and stops
+    short of it at a fraction (1-19 %) of the initial distance, so the
+    elements never touch. The spin axis passes through the vertex's foot on
+    the triangle. The decreasing-distance exit reports the start t0
+    of the current piece once the distance AT t0 is below a fifth of the
+    initial one (time_of_impact.cpp:529-541: the face-vertex distance is
+    taken at pose_ti0, as for edges). The distance along these trajectories
+    is exact (the vertex translates along the normal of a plane that stays
+    put), so every reported impact must lie where the true distance is
+    already below a fifth of the initial one. Measuring at the piece's end
+    instead reports the start of the piece in which the threshold is
+    crossed, where the distance is still above it: that fails every case
+    here."""
+    rng = np.random.default_rng(0)
+    hits = 0
+    for _ in range(200):
+        x, y, h0 = rng.uniform(-0.3, 0.3), rng.uniform(0.1, 0.4), rng.uniform(0.5, 2.0)
+        frac = rng.uniform(0.01, 0.19)
+        zero = [0.0] * 6
+        # the triangle's body origin sits under the vertex, inside the
+        # triangle, so the spin keeps the vertex's foot inside it
+        verts = [[0, 0, h0], [-1 - x, -y, 0], [1 - x, -y, 0], [-x, 1 - y, 0]]
+        pa1 = [0, 0, -(1 - frac) * h0, 0, 0, 0]
+        pb1 = [0, 0, 0, 0, 0, rng.uniform(-0.5, 0.5)]
+        q = np.array(sum(verts, []) + zero + pa1 + zero + pb1, np.float64)
+        hit, toi, _, _ = O.pl_fv_toi(q)
+        # exact distance: the vertex height above the plane z = 0
+        d0, d_toi = h0, h0 * (1 - (1 - frac) * toi) if hit else None
+        if hit:
+            hits += 1
+            assert 0 < toi <= 1 and d_toi < 0.2 * d0, (x, y, h0, frac, toi, d_toi / d0)
+    assert hits >= 100
 
 
 @pytest.mark.parametrize("seed,rot", [(1, 0.3), (2, 1.0), (3, 2.5)])

@@ -135,7 +135,9 @@ def test_sim_json_has_the_fields_the_reference_tools_read(tmp_path):
         x = a
         for p in path:
             x = x[p]
-    assert a["distance_barrier_constraint"]["trajectory_type"] == "rigid"
+    # the fixture leaves trajectory_type at the reference's default, and the
+    # run uses it
+    assert a["distance_barrier_constraint"]["trajectory_type"] == "piecewise_linear"
     seq = j["animation"]["state_sequence"]
     assert len(seq) == 4 and len(seq[0]["rigid_bodies"]) == 2
     assert j["stats"]["num_timesteps"] == 3 and len(j["stats"]["solver_iterations"]) == 3
@@ -164,3 +166,44 @@ def test_runner_reports_an_intersecting_initial_state(tmp_path, capsys):
     assert "The initial state contains intersections!" in capsys.readouterr().err
     IO.run(base, steps=0, backend=oracle_backend())
     assert "intersections" not in capsys.readouterr().err
+
+
+def test_trajectory_type_reaches_the_stepper(tmp_path):
+    base = os.path.join(FIX, "unit-tests-edge-edge.json")
+    sc, args = IO.load(base)
+    assert sc.trajectory_type == "piecewise_linear"  # SimState.cpp:179 default
+    assert IO.params_for(sc).trajectory_type == 1
+    scene = json.load(open(base))
+    for name, want in (("rigid", 0), ("piecewise_linear", 1)):
+        scene.setdefault("distance_barrier_constraint", {})["trajectory_type"] = name
+        p = tmp_path / f"{name}.json"
+        p.write_text(json.dumps(scene))
+        sc, _ = IO.load(str(p), base_dir=os.path.dirname(base))
+        assert IO.params_for(sc).trajectory_type == want
+    for name in ("linear", "redon"):
+        scene["distance_barrier_constraint"]["trajectory_type"] = name
+        with pytest.raises(NotImplementedError, match="trajectory_type"):
+            IO.load(scene, base_dir=os.path.dirname(base))
+
+
+def test_piecewise_linear_steps_through_the_oracle_stepper():
+    """Both trajectory types step the edge-edge fixture over the oracle
+    backend. The piecewise-linear CCD is conservative with respect to the
+    rigid trajectory, so its line search never takes a longer step than the
+    rigid CCD allows: both runs end without intersections, and both reach
+    contact (constraints active)."""
+    from stepper_util import oracle_backend
+    from rigidipc_amd import sim as M
+    base = os.path.join(FIX, "unit-tests-edge-edge.json")
+    out = {}
+    for name in ("rigid", "piecewise_linear"):
+        sc, _ = IO.load(base)
+        sc.trajectory_type = name
+        p = IO.params_for(sc)
+        p.do_intersection_check = 1
+        s = M.Simulation(sc, params=p, backend=oracle_backend())
+        s.step(40)
+        out[name] = s.stats()
+        s.close()
+    for st in out.values():
+        assert st["failed_solves"] == 0 and st["intersecting_steps"] == 0 and st["ccd_calls"] > 0
