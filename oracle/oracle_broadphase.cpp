@@ -264,7 +264,7 @@ void export_vertex_aabbs(const std::vector<AABB>& va, const std::vector<char>& v
 // compute_vertices_intervals recursion (rigid_body_hash_grid.cpp:188-246)
 // for one body; out receives the hull of the leaf enclosures.
 void vertex_intervals(const Scene& s, int b, const double* p0, const double* p1, const double dmn[3],
-                      const double dmx[3], double r, const Interval& t, int force, int depth,
+                      const double dmx[3], const Interval& t, int force, int depth,
                       std::vector<Interval>& out) {
     const int n = s.bv[b + 1] - s.bv[b];
     if (force <= 0 || depth > 40) {
@@ -276,7 +276,13 @@ void vertex_intervals(const Scene& s, int b, const double* p0, const double* p1,
             Vec3I w = world_vertex(R, pose.p, &s.V[3 * (s.bv[b] + i)]);
             for (int k = 0; k < 3; k++) {
                 out[3 * i + k] = w[k];
-                if (w[k].lo - r < dmn[k] || w[k].hi + r > dmx[k]) fits = false;
+                // the reference tests w.lo - r < min || w.hi + r > max
+                // (rigid_body_hash_grid.cpp:203-206). The domain is not
+                // grown by r (:103-104 leave it as it was), so for r > 0 a
+                // vertex on the domain's edge never fits and the reference
+                // splits t without end. The test leaves r out: identical
+                // for r = 0, and it ends for r > 0.
+                if (w[k].lo < dmn[k] || w[k].hi > dmx[k]) fits = false;
             }
         }
         if (fits || depth > 40) return;
@@ -284,8 +290,8 @@ void vertex_intervals(const Scene& s, int b, const double* p0, const double* p1,
     force--;
     const double m = median(t);
     std::vector<Interval> a, c;
-    vertex_intervals(s, b, p0, p1, dmn, dmx, r, Interval::raw(t.lo, m), force, depth + 1, a);
-    vertex_intervals(s, b, p0, p1, dmn, dmx, r, Interval::raw(m, t.hi), force, depth + 1, c);
+    vertex_intervals(s, b, p0, p1, dmn, dmx, Interval::raw(t.lo, m), force, depth + 1, a);
+    vertex_intervals(s, b, p0, p1, dmn, dmx, Interval::raw(m, t.hi), force, depth + 1, c);
     out.resize(3 * (size_t)n);
     for (size_t i = 0; i < out.size(); i++) out[i] = hull(a[i], c[i]);
 }
@@ -328,7 +334,7 @@ void broadphase_ccd(const Scene& s, const double* p0, const double* p1, double r
     for (int b = 0; b < s.nb; b++) {
         if (!inc[b]) continue;
         std::vector<Interval> iv;
-        vertex_intervals(s, b, p0 + 6 * b, p1 + 6 * b, dmn, dmx, r, Interval(0, 1), 1, 0, iv);
+        vertex_intervals(s, b, p0 + 6 * b, p1 + 6 * b, dmn, dmx, Interval(0, 1), 1, 0, iv);
         for (int i = 0; i < s.bv[b + 1] - s.bv[b]; i++) {
             AABB& a = va[s.bv[b] + i];
             for (int k = 0; k < 3; k++) {

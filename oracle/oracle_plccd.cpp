@@ -265,6 +265,14 @@ constexpr double kTrajectoryDistanceFactor = 0.5;     // :40
 constexpr double kDecreasingDistanceFactor = 0.2;     // :43
 constexpr double kDecreasingDistanceMinTime = 1e-6;   // :44
 constexpr long kMaxNumSubdivisions = 1000;            // :35
+// Not in the reference: a linear CCD that reports an impact at the very
+// start of the first piece makes the reference halve that piece again
+// (time_of_impact.cpp:428-438, "impossible because distance_t0 > MS_DIST").
+// A pair that rests just outside the separation distance (closer than the
+// linear CCD's error filter) gets that answer for every piece, so the
+// reference halves without end. After this many such halvings an impact at
+// t = 0 is reported, the conservative answer.
+constexpr int kMaxZeroToiSplits = 64;
 
 // Pose<double>::interpolate (src/physics/pose.tpp:141-148).
 void interpolate_pose(const double* p0, const double* p1, double t, double* out) {
@@ -321,6 +329,7 @@ bool pl_toi(bool vf, const double* const* LA, int nA, const double* const* LB, i
     std::stack<double> ts;
     ts.push(earliest_toi);
     long num_subdivisions = 1;
+    int zero_splits = 0;
     while (!ts.empty()) {
         const double ti1 = ts.top();
         if (n_pieces) ++*n_pieces;
@@ -361,6 +370,7 @@ bool pl_toi(bool vf, const double* const* LA, int nA, const double* const* LB, i
         if (is_impacting) {
             toi = (ti1 - ti0) * ltoi + ti0;
             if (toi == 0) {
+                if (++zero_splits > kMaxZeroToiSplits) break;  // impact at t = 0
                 ts.push((ti1 + ti0) / 2);
                 num_subdivisions++;
                 continue;

@@ -190,7 +190,8 @@ __global__ __launch_bounds__(BP_THREADS) void domain_kernel(const int* scenes, c
 // intervals; one that does not (or the forced root) is split in two. Each
 // piece's outcome depends on that piece alone, so the kept pieces, and the
 // enclosure (a min / max over them), are those of the reference's
-// depth-first recursion. A batch is a single piece whenever the stack is
+// depth-first recursion (the fit test without the inflation radius, see
+// below). A batch is a single piece whenever the stack is
 // within VI_ROOM of full: one piece adds at most one entry per level
 // below it, so the stack never overflows.
 constexpr int VI_STACK = 512, VI_BATCH = 64, VI_ROOM = 2 * VI_BATCH;
@@ -257,7 +258,11 @@ __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* 
                 Iv w[3];
                 world_vertex_iv(Rs[c], ps[c], V + 3 * i, w);
                 bool bad = false;
-                for (int d = 0; d < 3; d++) bad = bad || (w[d].lo - r < dmn[d]) || (w[d].hi + r > dmx[d]);
+                // the reference also subtracts / adds the inflation radius r
+                // here; with r > 0 a vertex on the domain's edge then never
+                // fits and the split never ends, so the test leaves r out
+                // (identical for r = 0, the CCD's radius without separation)
+                for (int d = 0; d < 3; d++) bad = bad || (w[d].lo < dmn[d]) || (w[d].hi > dmx[d]);
                 if (bad) c_bad[c] = 1;
             }
         __syncthreads();

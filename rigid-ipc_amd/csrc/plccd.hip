@@ -38,6 +38,12 @@ constexpr double kTrajectoryDistanceFactor = 0.5;
 constexpr double kDecreasingDistanceFactor = 0.2;
 constexpr double kDecreasingDistanceMinTime = 1e-6;
 constexpr long long kMaxNumSubdivisions = 1000;
+// A linear CCD impact at the very start of the first piece halves that piece
+// again (time_of_impact.cpp:428-438). A pair resting just outside the
+// separation distance gets that answer for every piece, and the reference
+// halves without end; after this many halvings an impact at t = 0 is
+// reported (the conservative answer; not in the reference).
+constexpr int kMaxZeroToiSplits = 64;
 constexpr int kItem = 7;  // t_lo t_hi u_lo u_hi v_lo v_hi level
 
 struct Lane {
@@ -316,6 +322,7 @@ RIGC_DI int pl_query(Lane& L, bool vf, const double (*loc)[3], const double* pA0
     int ntop = 0;
     L.tstack[ntop++] = earliest_toi;
     long long num_subdivisions = 1;
+    int zero_splits = 0;
     int result = 0;
     while (ntop > 0) {
         const double ti1 = L.tstack[ntop - 1];
@@ -356,6 +363,7 @@ RIGC_DI int pl_query(Lane& L, bool vf, const double (*loc)[3], const double* pA0
         if (result == 1) {
             toi = (ti1 - ti0) * ltoi + ti0;
             if (toi == 0) {
+                if (++zero_splits > kMaxZeroToiSplits) return 1;  // impact at t = 0
                 if (ntop == L.tcap) return 2;
                 L.tstack[ntop++] = (ti1 + ti0) / 2;
                 num_subdivisions++;

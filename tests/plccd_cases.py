@@ -116,3 +116,21 @@ def first_crossing(q, kind, n=4000):
         if _inside(kind, world(q, kind, [hi])):
             return hi
     return math.inf
+
+
+def resting_cases(msd=1e-3):
+    """Stationary pairs at the separation distance's edge: a vertex above a
+    triangle's interior and two crossing edges, msd * f apart. For f just
+    above 1 the gap beyond msd is below the linear CCD's error filter, so
+    it reports an impact at the piece's start for every piece; the
+    reference then halves the first piece without end. Returns
+    (kind, query, f)."""
+    out = []
+    zero = [0.0] * 6
+    for f in (1.5, 1.000000000001, 1.0000000000001):
+        h = f * msd
+        fv = [[0.2, 0.2, h], [-1, 0, 0], [1, 0, 0], [0, 1, 0]]
+        ee = [[-1, 0.1, h], [1, 0.1, h], [0.2, -1, 0], [0.2, 1, 0]]
+        for kind, verts in ((1, fv), (0, ee)):
+            out.append((kind, np.array(sum(verts, []) + zero * 4, np.float64), f))
+    return out

@@ -66,6 +66,9 @@ def test_config1_continuous_and_discrete(ctx, seed):
     sc = S.config1(seed=seed)
     p0, p1 = moved(sc, seed + 10, dpos=0.1, drot=0.3)
     compare(ctx, sc, p0, p1, 0.0)
+    # inflated continuous phase (the CCD's msd / 2); the plane's corners
+    # bound the domain
+    compare(ctx, sc, p0, p1, 5e-5)
     compare(ctx, sc, p1, None, 5e-4)
     compare(ctx, sc, p0, None, 0.3)
 
@@ -95,6 +98,7 @@ def test_dense_cluster(ctx, seed):
     sc = cluster(seed)
     p0, p1 = moved(sc, seed + 20, dpos=0.1, drot=0.3)
     compare(ctx, sc, p0, p1, 0.0)
+    compare(ctx, sc, p0, p1, 5e-5)
     compare(ctx, sc, p1, None, 0.05)
 
 
@@ -106,12 +110,15 @@ def test_single_body_scene_has_no_candidates(ctx):
     assert ctx.compute_earliest_toi(sc.poses, sc.poses) == math.inf
 
 
-@pytest.mark.parametrize("seed", [0, 1, 2])
-def test_earliest_toi_matches_oracle(ctx, seed):
+@pytest.mark.parametrize("seed,msd", [(0, 0.0), (1, 0.0), (2, 0.0), (0, 1e-4)])
+def test_earliest_toi_matches_oracle(ctx, seed, msd):
+    """msd > 0 widens the rigid CCD's candidates (broad phase inflated by
+    msd / 2, distance_barrier_constraint.cpp:139-142); the rigid TOI itself
+    does not take a separation."""
     sc = cluster(seed, n=27)
     p0, p1 = moved(sc, seed + 30, dpos=0.05, drot=0.1)
     O.set_scene(sc)
-    ee, fv = O.broadphase(p0, p1, 0.0)
+    ee, fv = O.broadphase(p0, p1, msd / 2)
     best = math.inf
     O.set_eval_budget(0)
     for k, pairs in ((0, ee), (1, fv)):
@@ -121,8 +128,8 @@ def test_earliest_toi_matches_oracle(ctx, seed):
             if hit:
                 best = min(best, toi)
     ctx.set_scene(sc)
-    got = ctx.compute_earliest_toi(p0, p1, 0.0)
-    print(f"candidates {len(ee) + len(fv)} earliest toi {best}", flush=True)
+    got = ctx.compute_earliest_toi(p0, p1, msd)
+    print(f"msd {msd} candidates {len(ee) + len(fv)} earliest toi {best}", flush=True)
     assert got == best
 
 

@@ -84,6 +84,16 @@ def test_random_queries_bit_exact(ctx, seed, rot, msd):
     check(ctx, kinds, qs, msd=msd)
 
 
+def test_resting_pairs_at_the_separation_edge_bit_exact(ctx):
+    """Pairs resting just outside the separation distance, where the
+    reference halves the first piece without end: the device stops after
+    the same 64 halvings as the oracle, with an impact at t = 0."""
+    cases = C.resting_cases()
+    kinds = np.array([k for k, _, _ in cases], np.int8)
+    qs = np.array([q for _, q, _ in cases])
+    check(ctx, kinds, qs, msd=1e-3)
+
+
 def test_partial_time_domain_bit_exact(ctx):
     kinds, qs = scenes.random_queries(150, 5, rot_scale=1.0)
     check(ctx, kinds, qs, earliest=0.4)

@@ -146,8 +146,14 @@ def test_large_fixture_golden_is_current():
     np.testing.assert_array_equal(fv, sc.fv_ccd)
 
 
+@pytest.mark.parametrize("r", [0.0, 5e-5])
 @pytest.mark.parametrize("which", ["config1", "large"])
-def test_continuous_candidates_and_enclosures(which):
+def test_continuous_candidates_and_enclosures(which, r):
+    """r > 0 (the CCD's msd / 2): the reference's t-split test demands a
+    margin of r inside a domain that is not grown by r, which a vertex on
+    the domain's edge (the static plane's corners here) never meets; the
+    oracle leaves r out of that test, so the call returns, with the r = 0
+    enclosures grown by r and a superset of the r = 0 candidates."""
     if which == "config1":
         sc = S.config1(seed=2)
         rng = np.random.default_rng(6)
@@ -159,8 +165,16 @@ def test_continuous_candidates_and_enclosures(which):
         sc = large_fixture()
         p0, p1 = sc.poses_t0, sc.poses_t1
     O.set_scene(sc)
-    ee, fv = O.broadphase(p0, p1, 0.0)
+    if r > 0:
+        ee0, fv0 = O.broadphase(p0, p1, 0.0)
+        inc0, vmn0, vmx0 = O.last_vertex_aabbs(len(sc.vertices))
+    ee, fv = O.broadphase(p0, p1, r)
     inc, vmn, vmx = O.last_vertex_aabbs(len(sc.vertices))
+    if r > 0:
+        np.testing.assert_array_equal(inc, inc0)
+        np.testing.assert_array_equal(vmn[inc], vmn0[inc] - r)
+        np.testing.assert_array_equal(vmx[inc], vmx0[inc] + r)
+        assert set(map(tuple, ee0)) <= set(map(tuple, ee)) and set(map(tuple, fv0)) <= set(map(tuple, fv))
     nb = len(sc.group_ids)
     inc_body = np.array([inc[sc.body_vertex_start[b]] if sc.body_vertex_start[b + 1] > sc.body_vertex_start[b] else False
                          for b in range(nb)])

@@ -105,6 +105,20 @@ def test_face_vertex_exit_measures_the_piece_start():
     assert hits >= 100
 
 
+def test_resting_pair_at_the_separation_edge_terminates():
+    """A pair resting msd * (1 + 1e-12) apart: the linear CCD reports an
+    impact at the start of every piece, which the reference answers by
+    halving the first piece again, without end. The oracle stops after 64
+    halvings (65 pieces) with an impact at t = 0, the conservative answer.
+    A pair 1.5 msd apart has no impact and needs one piece."""
+    for kind, q, f in C.resting_cases():
+        hit, toi, pieces, boxes = (O.pl_ee_toi if kind == 0 else O.pl_fv_toi)(q, 1.0, 1e-3)
+        if f == 1.5:
+            assert not hit and pieces == 1, (kind, f)
+        else:
+            assert hit and toi == 0.0 and pieces == 65, (kind, f, hit, toi, pieces)
+
+
 @pytest.mark.parametrize("seed,rot", [(1, 0.3), (2, 1.0), (3, 2.5)])
 def test_conservative_against_exact_crossings(seed, rot):
     kinds, qs = scenes.random_queries(150, seed, rot_scale=rot)
