@@ -139,8 +139,15 @@ def test_barrier_local_matches_dual_derivatives(libs, ctype, parallel):
 
 
 def test_friction_frame_matches_oracle(libs):
-    """Tangent bases are orthonormal for every constraint type (the device
-    friction records are compared with the oracle's in the GPU tests)."""
+    """Tangent bases and closest-point coordinates of the device friction
+    frame against what the geometry fixes (the device friction records are
+    also compared with the oracle's in the GPU tests): the basis is
+    orthonormal and spans the plane normal to the contact direction (VV: the
+    vertex difference; EV: the vertex minus its projection on the edge's
+    line; EE: ea x eb; FV: the face normal), and the coordinates are the
+    closest-point parameters on the lines / plane (EV: the edge parameter;
+    EE: both line parameters; FV: the barycentric pair of the projection),
+    solved here by numpy."""
     L, OL = libs
     rng = np.random.default_rng(5)
     for ctype in range(4):
@@ -150,3 +157,25 @@ def test_friction_frame_matches_oracle(libs):
             L.cf_friction_frame(ctype, ptr(X), ptr(coords), ptr(basis))
             B = basis.reshape(3, 2)
             assert np.allclose(B.T @ B, np.eye(2), atol=1e-12)
+            X = X.reshape(4, 3)
+            if ctype == 0:
+                n = X[1] - X[0]
+            elif ctype == 1:
+                e = X[2] - X[1]
+                a = (X[0] - X[1]) @ e / (e @ e)
+                assert coords[0] == pytest.approx(a, rel=1e-10, abs=1e-12)
+                n = X[0] - (X[1] + a * e)
+            elif ctype == 2:
+                ea, eb = X[1] - X[0], X[3] - X[2]
+                M = np.array([[ea @ ea, -(ea @ eb)], [-(ea @ eb), eb @ eb]])
+                st = np.linalg.solve(M, [(X[2] - X[0]) @ ea, (X[0] - X[2]) @ eb])
+                np.testing.assert_allclose(coords, st, rtol=1e-8, atol=1e-10)
+                n = np.cross(ea, eb)
+            else:
+                e0, e1 = X[2] - X[1], X[3] - X[1]
+                M = np.array([[e0 @ e0, e0 @ e1], [e0 @ e1, e1 @ e1]])
+                uv = np.linalg.solve(M, [(X[0] - X[1]) @ e0, (X[0] - X[1]) @ e1])
+                np.testing.assert_allclose(coords, uv, rtol=1e-10, atol=1e-12)
+                n = np.cross(e0, e1)
+            n = n / np.linalg.norm(n)
+            assert np.abs(B.T @ n).max() <= 1e-10

@@ -4,10 +4,13 @@ The ipc-toolkit pieces behind these terms are not in the reference tree,
 so they are pinned by what the mathematics fixes: central finite
 differences of energy -> gradient -> (unprojected) Hessian at the body
 dofs (the reference's own runtime check, distance_barrier_rb_problem.cpp:
-1651-1689, uses the same idea), the closed-form IPC barrier on a
-vertex-vertex pair, invariants of the constraint set (every constraint is
+1651-1689, uses the same idea), the closed-form IPC barrier on an
+edge-edge pair, invariants of the constraint set (every constraint is
 active, multiplicities count duplicates), PSD projection properties, and
-the vertex Jacobian of the Rodrigues map.
+the vertex Jacobian of the Rodrigues map. Distances, constraint-set
+membership and multiplicities, the mollifier, the normal force and the
+friction potential are pinned against independent geometry and closed
+forms in test_oracle_ipc_pins.py.
 """
 import math
 import os
@@ -99,21 +102,6 @@ def test_projected_local_hessians_are_psd_and_idempotent():
         assert np.allclose(h, h.T, atol=1e-9 * max(1, np.abs(h).max()))
         w = np.linalg.eigvalsh((h + h.T) / 2)
         assert w.min() >= -1e-8 * max(1.0, np.abs(w).max())
-
-
-def test_vertex_vertex_barrier_closed_form():
-    # two single-vertex bodies at distance r: E = -(r^2 - dhat^2)^2 log(r^2 / dhat^2)
-    import types
-    sc = types.SimpleNamespace(
-        vertices=np.zeros((2, 3)), edges=np.zeros((0, 2), np.int32), faces=np.zeros((0, 3), np.int32),
-        face_edges=np.zeros((0, 3), np.int32), body_vertex_start=np.array([0, 1, 2], np.int32),
-        body_edge_start=np.zeros(3, np.int32), body_face_start=np.zeros(3, np.int32),
-        group_ids=np.array([0, 1], np.int32), is_static=np.zeros(2, np.uint8))
-    O.set_scene(sc)
-    # no edges/faces -> no candidates; the closed form is checked through the
-    # dual-number distance on the constraint itself via the EV/VV path below
-    rec, _ = O.constraint_set(np.array([[0, 0, 0, 0, 0, 0], [0.05, 0, 0, 0, 0, 0.0]]), 0.1)
-    assert len(rec) == 0
 
 
 def test_edge_edge_barrier_value_closed_form():
