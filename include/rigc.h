@@ -154,6 +154,10 @@ rigc_status rigc_compute_earliest_toi_piecewise_linear(rigc_ctx* ctx, const doub
 #define RIGC_GRADIENT 1
 #define RIGC_HESSIAN 2            /* implies RIGC_GRADIENT */
 #define RIGC_BUILD_CONSTRAINTS 4  /* rebuild the constraint set at `poses` first */
+#define RIGC_UNPROJECTED 8        /* Hessian blocks without project_to_psd: the exact
+                                   * Hessian that an implicit-function adjoint needs
+                                   * (the reference always projects,
+                                   * distance_barrier_rb_problem.cpp:1241) */
 
 /* Block-sparse symmetric Hessian in the 6-dof body layout. diag holds
  * every body's (i,i) block; pairs / off hold the coupled body pairs
@@ -240,6 +244,20 @@ rigc_status rigc_get_friction_constraints(rigc_ctx* ctx, int64_t capacity, doubl
 rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const double* poses_t0,
                                        double epsv_times_h, int32_t flags, double* energy, double* grad,
                                        rigc_block_hessian* hess);
+
+/* Adjoint of the friction gradient with respect to the start-of-step poses:
+ * out = (d grad_x D(x; x0) / d x0)^T lambda, with x = poses, x0 = poses_t0
+ * and D the friction potential of rigc_compute_friction_term over the
+ * current lagged friction set, which must have been built at poses_t0 with
+ * barrier_stiffness, dhat and dmin. D depends on x0 through the lagged
+ * displacement V(x) - V(x0) and through each normal force
+ * -kappa b'(d(x0)) 2 sqrt(d(x0)); the frozen tangent bases and closest-point
+ * coordinates are held fixed. lambda, out: [num_bodies][6]. No counterpart
+ * in the reference, which has no gradients (python/src/bindings.cpp:30-193):
+ * it serves the implicit-function adjoint of a time step. */
+rigc_status rigc_friction_adjoint(rigc_ctx* ctx, const double* poses, const double* poses_t0, double epsv_times_h,
+                                  double dhat, double dmin, double barrier_stiffness, const double* lambda,
+                                  double* out);
 
 /* ---- Batched scenes (rigc_set_scenes). One call serves the n scenes
  * scene_ids[0..n) (strictly ascending) with one launch sequence. Pose

@@ -129,7 +129,10 @@ rigc_status ob_barrier(void*, const double* poses, double dhat, double dmin, int
     const int nb = scene().nb;
     std::vector<double> grad(6 * (size_t)nb, 0.0);
     std::vector<std::array<double, 144>> lh;
+    const bool proj = psd_projection_enabled();
+    psd_projection_enabled() = (flags & RIGC_UNPROJECTED) == 0;
     *e = barrier_term(scene(), poses, B.cons, dhat, dmin, want_grad ? grad.data() : nullptr, want_hess ? &lh : nullptr);
+    psd_projection_enabled() = proj;
     if (want_grad) std::memcpy(g, grad.data(), sizeof(double) * grad.size());
     if (want_hess) {
         std::vector<const Constraint*> C;
@@ -170,8 +173,11 @@ rigc_status ob_friction(void*, const double* poses, const double* poses_t0, doub
     const int nb = scene().nb;
     std::vector<double> grad(6 * (size_t)nb, 0.0);
     std::vector<std::array<double, 144>> lh;
+    const bool proj = psd_projection_enabled();
+    psd_projection_enabled() = (flags & RIGC_UNPROJECTED) == 0;
     *e = friction_term(scene(), poses, poses_t0, B.fric, eh, want_grad ? grad.data() : nullptr,
                        want_hess ? &lh : nullptr);
+    psd_projection_enabled() = proj;
     if (want_grad) std::memcpy(g, grad.data(), sizeof(double) * grad.size());
     if (want_hess) {
         std::vector<const Constraint*> C;
@@ -180,6 +186,17 @@ rigc_status ob_friction(void*, const double* poses, const double* poses_t0, doub
         if (s != RIGC_OK) B.err = "off-diagonal Hessian capacity too small";
         return s;
     }
+    return RIGC_OK;
+}
+
+rigc_status ob_fadjoint(void*, const double* poses, const double* poses_t0, double eh, double dhat, double dmin,
+                        double kappa, const double* lambda, double* out) {
+    OracleBackend& B = state();
+    if (!B.have_fric) {
+        B.err = "no friction set";
+        return RIGC_ERR_NO_SCENE;
+    }
+    friction_adjoint(scene(), poses, poses_t0, B.fric, eh, dhat, dmin, kappa, lambda, out);
     return RIGC_OK;
 }
 
@@ -197,7 +214,7 @@ extern "C" {
 // Fills a rigs_backend table with the oracle's entry points.
 void oracle_backend(rigs_backend* out) {
     *out = rigs_backend{ nullptr, ob_set_scene, ob_toi, ob_build, ob_barrier, ob_mind, ob_fupdate, ob_friction,
-                         ob_error, ob_stats, ob_isect, ob_toi_pl };
+                         ob_error, ob_stats, ob_isect, ob_toi_pl, ob_fadjoint };
 }
 
 }  // extern "C"

@@ -238,11 +238,19 @@ double barrier_hessian(double d, double dhat) {
 // dual number over the constraint's (up to 4) vertices (N = 12), or its
 // value only (N = 1, no variables seeded).
 template <int N>
+static D2<N> constraint_distance_p(const Constraint& c, const double* X /*4x3*/, const V3<N>* P);
+template <int N>
 static D2<N> constraint_distance_t(const Constraint& c, const double* X /*4x3*/) {
     V3<N> P[4];
     for (int i = 0; i < 4; i++)
         for (int k = 0; k < 3; k++)
             P[i].c[k] = (N == 12) ? D2<N>::var(X[3 * i + k], (3 * i + k) % N) : D2<N>(X[3 * i + k]);
+    return constraint_distance_p<N>(c, X, P);
+}
+// The same with the vertices given as dual numbers P (types classified at
+// their values X).
+template <int N>
+static D2<N> constraint_distance_p(const Constraint& c, const double* X /*4x3*/, const V3<N>* P) {
     const double* x0 = X;
     const double* x1 = X + 3;
     const double* x2 = X + 6;
@@ -803,6 +811,92 @@ void construct_friction_constraint_set(const Scene& s, const double* poses, cons
         f.normal_force = -kappa * (barrier_gradient(d - dm2, dh) * (2 * std::sqrt(d)));
         f.mu = mu;
         out.push_back(f);
+    }
+}
+
+// Adjoint of the friction gradient with respect to the start-of-step poses
+// (the device's rigc_friction_adjoint), by second-order dual numbers over
+// the 12 coordinates of a record's vertices at x (variables 0-11) and at x0
+// (12-23): out += J(x0)^T (d2 D / dW0 dW) J(x) lambda, record by record. D
+// depends on x0 through the displacement and through the normal force
+// -kappa b'(d(x0)) 2 sqrt(d(x0)); the tangent basis and weights are held
+// fixed. Only the mixed block of the dual Hessian is used, so the normal
+// force's own second derivative is not seeded.
+void friction_adjoint(const Scene& s, const double* poses, const double* poses_t0,
+                      const std::vector<FrictionConstraint>& FC, double eh, double dhat, double dmin, double kappa,
+                      const double* lambda, double* out) {
+    using D24 = D2<24>;
+    using V24 = V3<24>;
+    const std::vector<double> W1 = all_world_vertices(s, poses), W0 = all_world_vertices(s, poses_t0);
+    for (int i = 0; i < 6 * s.nb; i++) out[i] = 0;
+    const double dh = 2 * dmin * dhat + dhat * dhat, dm2 = dmin * dmin;
+    for (const FrictionConstraint& f : FC) {
+        const Constraint& c = f.c;
+        double X1[12], X0[12];
+        gather(W1, c, X1);
+        gather(W0, c, X0);
+        V24 P1[4], P0[4];
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 3; k++) {
+                P1[i].c[k] = i < c.nv ? D24::var(X1[3 * i + k], 3 * i + k) : D24(0.0);
+                P0[i].c[k] = i < c.nv ? D24::var(X0[3 * i + k], 12 + 3 * i + k) : D24(0.0);
+            }
+        double w[4];
+        switch (c.type) {
+        case VV: w[0] = 1; w[1] = -1; w[2] = w[3] = 0; break;
+        case EV: w[0] = 1; w[1] = -(1 - f.coords[0]); w[2] = -f.coords[0]; w[3] = 0; break;
+        case EE: w[0] = 1 - f.coords[0]; w[1] = f.coords[0]; w[2] = -(1 - f.coords[1]); w[3] = -f.coords[1]; break;
+        default: w[0] = 1; w[1] = -(1 - f.coords[0] - f.coords[1]); w[2] = -f.coords[0]; w[3] = -f.coords[1]; break;
+        }
+        D24 ua[2] = { D24(0.0), D24(0.0) };
+        for (int a = 0; a < 2; a++)
+            for (int i = 0; i < c.nv; i++)
+                for (int k = 0; k < 3; k++) ua[a] = ua[a] + (w[i] * f.basis[k][a]) * (P1[i].c[k] - P0[i].c[k]);
+        const D24 s2 = ua[0] * ua[0] + ua[1] * ua[1];
+        D24 F0;
+        if (s2.v > 0) {
+            const double sq = std::sqrt(s2.v);
+            const D24 nu = chain(s2, sq, 0.5 / sq, -0.25 / (s2.v * sq));
+            const double x = nu.v;
+            const double f0 = x >= eh ? x : x * x * (-x / 3.0 / eh + 1.0) / eh + eh / 3.0;
+            const double f1 = x >= eh ? 1.0 : -x * x / (eh * eh) + 2.0 * x / eh;
+            const double f1d = x >= eh ? 0.0 : -2.0 * x / (eh * eh) + 2.0 / eh;
+            F0 = chain(nu, f0, f1, f1d);
+        } else {
+            // no lagged displacement: f0 = s / eh + eh / 3 - s^(3/2) / (3 eh^2)
+            // in s = |T^T u|^2, whose last term has no first or second
+            // derivative at s = 0 (the gradient vanishes, the Hessian
+            // 2 T T^T / eh does not)
+            F0 = chain(s2, eh / 3.0, 1.0 / eh, 0.0);
+        }
+        const D24 d0 = constraint_distance_p<24>(c, X0, P0);
+        const double sd = std::sqrt(d0.v);
+        const double b1 = barrier_gradient(d0.v - dm2, dh), b2 = barrier_hessian(d0.v - dm2, dh);
+        const D24 lamN = chain(d0, -kappa * (b1 * (2 * sd)), -kappa * (b2 * (2 * sd) + b1 / sd), 0.0);
+        const D24 Dk = ((double)c.multiplicity * f.mu) * (lamN * F0);
+        // y = J(x) lambda on the record's vertex slots
+        double y[12] = { 0 }, J1[4][18], J0[4][18], Hv[27], wv[3];
+        for (int i = 0; i < c.nv; i++) {
+            const int b = s.v_body[c.v[i]];
+            vertex_derivatives(s, poses + 6 * b, c.v[i], wv, J1[i], Hv);
+            vertex_derivatives(s, poses_t0 + 6 * b, c.v[i], wv, J0[i], Hv);
+            for (int k = 0; k < 3; k++)
+                for (int d = 0; d < 6; d++) y[3 * i + k] += J1[i][6 * k + d] * lambda[6 * b + d];
+        }
+        for (int i = 0; i < c.nv; i++) {
+            const int b = s.v_body[c.v[i]];
+            double z[3];
+            for (int k = 0; k < 3; k++) {
+                double sum = 0;
+                for (int r = 0; r < 12; r++) sum += Dk.h[12 + 3 * i + k][r] * y[r];
+                z[k] = sum;
+            }
+            for (int d = 0; d < 6; d++) {
+                double sum = 0;
+                for (int k = 0; k < 3; k++) sum += J0[i][6 * k + d] * z[k];
+                out[6 * b + d] += sum;
+            }
+        }
     }
 }
 

@@ -79,6 +79,12 @@ double friction_term(const Scene& s, const double* poses, const double* poses_t0
                      const std::vector<FrictionConstraint>& FC, double epsv_times_h, double* grad,
                      std::vector<std::array<double, 144>>* local_hess);
 
+// (d grad_x D(x; x0) / d x0)^T lambda over the friction set (lambda, out:
+// [nb][6]); the set must have been built at poses_t0 with kappa.
+void friction_adjoint(const Scene& s, const double* poses, const double* poses_t0,
+                      const std::vector<FrictionConstraint>& FC, double epsv_times_h, double dhat, double dmin,
+                      double kappa, const double* lambda, double* out);
+
 // Vertex position and derivatives w.r.t. its body's 6 dofs: w (3),
 // jac (3x6 row-major), hess (3 x 3x3 rotational block, [j][k][l]).
 void vertex_derivatives(const Scene& s, const double* pose, int vertex, double w[3], double jac[18],

@@ -295,6 +295,10 @@ long oracle_friction_update(const double* poses, double dhat, double dmin, doubl
     construct_friction_constraint_set(scene(), poses, g_constraints, dhat, dmin, kappa, mu, g_friction);
     return (long)g_friction.size();
 }
+void oracle_friction_adjoint(const double* poses, const double* poses_t0, double epsv_times_h, double dhat,
+                             double dmin, double kappa, const double* lambda, double* out) {
+    friction_adjoint(scene(), poses, poses_t0, g_friction, epsv_times_h, dhat, dmin, kappa, lambda, out);
+}
 // per friction constraint: normal force, coords (2), basis (3x2 row-major)
 void oracle_get_friction(double* rec) {
     for (size_t i = 0; i < g_friction.size(); i++) {

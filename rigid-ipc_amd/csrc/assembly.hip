@@ -334,6 +334,7 @@ struct LocalArgs {
     const double* epsv_h_scene;  // friction: per-scene epsv_h (nullptr: epsv_h)
     const int* body_scene;
     int want_grad;
+    int project;       // project_to_psd the 12 x 12 blocks (0: exact Hessian)
     double* energy;    // [n]
     double* lgrad;     // [n][12]
     double* lhess;     // [n][144]
@@ -536,7 +537,7 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
         }
         __syncthreads();
         // project_to_psd: Jacobi in parallel order; unchanged when PSD
-        for (int sweep = 0; sweep < 100; sweep++) {
+        for (int sweep = 0; a.project && sweep < 100; sweep++) {
             double off = 0.0, tot = 0.0;
             for (int e = lane; e < 144; e += 64) {
                 const double v = sA[e];
@@ -597,6 +598,7 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
         }
         bool psd = true;
         for (int i = 0; i < 12; i++) psd = psd && sA[13 * i] >= 0.0;
+        psd = psd || !a.project;
         for (int e = lane; e < 144; e += 64) {
             double out;
             if (psd) {
@@ -827,6 +829,91 @@ __global__ __launch_bounds__(THREADS) void friction_record_kernel(const ConRec* 
     o.normal_force = -kappa * (ipc_barrier_gradient(f.d - dm2, dh) * (2 * sqrt(f.d)));
     o.mu = mu;
     out[c] = o;
+}
+
+// Friction adjoint, one thread per lagged friction record: with
+// y = J(x) lambda (the record's vertex-space image of lambda at the
+// evaluation poses), the record's share of (d grad_x D / d x0)^T lambda in
+// the vertex space of x0 is
+//   -H_v y                       (D depends on x0 through U = V(x) - V(x0);
+//                                 H_v = m mu lambda_N T M T^T, the friction
+//                                 Hessian in U)
+//   + (g1 . y) m mu dlambda_N/dd grad_V d(x0)
+//                                (through lambda_N = -kappa b'(d) 2 sqrt(d);
+//                                 g1 = f1(|T^T U|) T T^T U / |T^T U|)
+// and J(x0)^T maps it to the two bodies' dofs at x0. The frozen tangent
+// basis T and closest-point weights are held fixed.
+__global__ __launch_bounds__(THREADS) void friction_adjoint_kernel(const FricRec* fric, long long n, const double* V,
+                                                                  const int* v_body, const double* W,
+                                                                  const double* W0, const BodyFrame* F1,
+                                                                  const BodyFrame* F0, double epsv_h, double dhat,
+                                                                  double dmin, double kappa, const double* lambda,
+                                                                  double* out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const FricRec f = fric[c];
+    const ConRec& r = f.c;
+    const int nv = con_nv(r.type);
+    double U[12], X0[12];
+    for (int s = 0; s < 4; s++)
+        for (int k = 0; k < 3; k++) {
+            U[3 * s + k] = s < nv ? W[3 * r.v[s] + k] - W0[3 * r.v[s] + k] : 0.0;
+            X0[3 * s + k] = s < nv ? W0[3 * r.v[s] + k] : 0.0;
+        }
+    double basis[3][2];
+    for (int k = 0; k < 3; k++) {
+        basis[k][0] = f.basis[2 * k];
+        basis[k][1] = f.basis[2 * k + 1];
+    }
+    FrictionLocal L;
+    friction_local_setup(L, r.type, nv, f.coords, basis, f.normal_force, f.mu, r.mult, U, epsv_h);
+    double y[12];
+    for (int e = 0; e < 12; e++) y[e] = 0.0;
+    for (int s = 0; s < nv; s++) {
+        const int v = r.v[s];
+        const int b = v_body[v];
+        const double* lam = lambda + 6 * b;
+        const double* x = V + 3 * v;
+        for (int k = 0; k < 3; k++) {
+            double sum = lam[k];
+            for (int d = 0; d < 3; d++) sum += jac_rot(F1[b], x, d, k) * lam[3 + d];
+            y[3 * s + k] = sum;
+        }
+    }
+    double ty[2] = { 0.0, 0.0 };
+    for (int e = 0; e < 12; e++) {
+        ty[0] += L.T[e][0] * y[e];
+        ty[1] += L.T[e][1] * y[e];
+    }
+    const double mt0 = L.M[0][0] * ty[0] + L.M[0][1] * ty[1];
+    const double mt1 = L.M[1][0] * ty[0] + L.M[1][1] * ty[1];
+    double z[12];
+    for (int e = 0; e < 12; e++) z[e] = -L.scale * (L.T[e][0] * mt0 + L.T[e][1] * mt1);
+    DistForm fm;
+    constraint_form(r.type, X0, fm);
+    const double dh = 2 * dmin * dhat + dhat * dhat, dm2 = dmin * dmin;
+    const double sd = sqrt(fm.d);
+    const double b1 = ipc_barrier_gradient(fm.d - dm2, dh), b2 = ipc_barrier_hessian(fm.d - dm2, dh);
+    const double dlam = sd > 0.0 ? -kappa * (b2 * (2 * sd) + b1 / sd) : 0.0;
+    const double g1y = L.f1x * (L.u[0] * ty[0] + L.u[1] * ty[1]);
+    const double coeff = g1y * ((double)r.mult * f.mu * dlam);
+    for (int s = 0; s < nv; s++)
+        for (int k = 0; k < 3; k++) z[3 * s + k] += coeff * form_grad_vertex(fm, s, k);
+    double o[12];
+    for (int e = 0; e < 12; e++) o[e] = 0.0;
+    for (int s = 0; s < nv; s++) {
+        const int side = slot_side(r.type, s);
+        const int v = r.v[s];
+        const int b = v_body[v];
+        const double* x = V + 3 * v;
+        for (int k = 0; k < 3; k++) o[6 * side + k] += z[3 * s + k];
+        for (int d = 0; d < 3; d++) {
+            double sum = 0.0;
+            for (int k = 0; k < 3; k++) sum += jac_rot(F0[b], x, d, k) * z[3 * s + k];
+            o[6 * side + 3 + d] += sum;
+        }
+    }
+    for (int e = 0; e < 12; e++) out[12 * c + e] = o[e];
 }
 
 // Minimum squared distance over the set (non-negative doubles order as
@@ -1210,7 +1297,7 @@ hipError_t minimum_distance(Assembly& as, const DevScene& sc, const SceneSel& se
 hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& sel, bool friction,
                               const double* d_poses, const double* d_poses_t0, double dhat, double dmin,
                               double epsv_h, const double* d_epsv_h_scene, bool want_grad, bool want_hess,
-                              int num_cus, AssemblyOut& out, hipStream_t st) {
+                              int num_cus, AssemblyOut& out, hipStream_t st, bool project) {
     out = AssemblyOut{};
     out.energy.assign(sel.n_scenes, 0.0);
     out.hess = want_hess;
@@ -1271,6 +1358,7 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& 
     la.epsv_h_scene = d_epsv_h_scene;
     la.body_scene = sc.body_scene;
     la.want_grad = want_grad ? 1 : 0;
+    la.project = project ? 1 : 0;
     la.energy = as.energy.as<double>();
     la.lgrad = as.lgrad.as<double>();
     la.lhess = want_hess ? as.lhess.as<double>() : nullptr;
@@ -1343,6 +1431,36 @@ hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& 
     AS_CHECK(hipMemcpyAsync(as.h_esum.p, as.esum.p, sizeof(double) * sel.n_scenes, hipMemcpyDeviceToHost, st));
     out.pending = true;
     return hipGetLastError();
+}
+
+hipError_t friction_adjoint(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                            const double* d_poses_t0, double epsv_h, double dhat, double dmin, double kappa,
+                            const double* d_lambda, double* out, hipStream_t st) {
+    const int nb = sc.nb;
+    for (int i = 0; i < 6 * nb; i++) out[i] = 0.0;
+    const long long n = as.n_fric;
+    if (n == 0) return hipSuccess;
+    AS_CHECK(world_vertices(as, sc, sel, d_poses_t0, as.W0, false, st));
+    AS_CHECK(world_vertices(as, sc, sel, d_poses, as.W, true, st));
+    AS_CHECK(as.frames0.ensure(sizeof(BodyFrame) * nb));
+    hipLaunchKernelGGL(body_frame_kernel, dim3(grid_for(nb), 9), dim3(THREADS), 0, st, nb, d_poses_t0,
+                       as.frames0.as<BodyFrame>());
+    AS_CHECK(as.fadj.ensure(12 * sizeof(double) * n));
+    hipLaunchKernelGGL(friction_adjoint_kernel, dim3(grid_for(n)), dim3(THREADS), 0, st, as.fric.as<FricRec>(), n,
+                       sc.V, sc.v_body, as.W.as<double>(), as.W0.as<double>(), as.frames.as<BodyFrame>(),
+                       as.frames0.as<BodyFrame>(), epsv_h, dhat, dmin, kappa, d_lambda, as.fadj.as<double>());
+    AS_CHECK(hipGetLastError());
+    std::vector<double> loc(12 * (size_t)n);
+    std::vector<FricRec> recs((size_t)n);
+    AS_CHECK(hipMemcpyAsync(loc.data(), as.fadj.p, 12 * sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    AS_CHECK(hipMemcpyAsync(recs.data(), as.fric.p, sizeof(FricRec) * n, hipMemcpyDeviceToHost, st));
+    AS_CHECK(hipStreamSynchronize(st));
+    for (long long c = 0; c < n; c++)
+        for (int side = 0; side < 2; side++) {
+            const int b = recs[(size_t)c].c.body[side];
+            for (int k = 0; k < 6; k++) out[6 * b + k] += loc[12 * (size_t)c + 6 * side + k];
+        }
+    return hipSuccess;
 }
 
 void finish_potential(Assembly& as, AssemblyOut& out) {

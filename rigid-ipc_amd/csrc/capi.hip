@@ -448,7 +448,7 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     a.q_flag = ctx->qflag.as<int>();
 
     const int block = 256;
-    const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
+    const int grid = ctx->num_cus * CCD_BLOCKS_PER_CU;  // persistent, all blocks resident
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
     a.walk_budget = ctx->walk_budget;
     if (nq) RIGC_HIP(launch_ccd(a, grid, block, st, ctx->ccdw));
@@ -633,7 +633,7 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
     a.q_best = ctx->qbest.as<unsigned long long>();
     a.q_visits = ctx->qvisits.as<unsigned int>();
     a.q_flag = ctx->qflag.as<int>();
-    const int grid = ctx->num_cus * 2;  // persistent: two 256-lane blocks per CU
+    const int grid = ctx->num_cus * CCD_BLOCKS_PER_CU;  // persistent, all blocks resident
     a.walk_budget = ctx->walk_budget;
 
     const double inf = HUGE_VAL;
@@ -942,7 +942,7 @@ rigc_status rigc_compute_barrier_term(rigc_ctx* ctx, const double* poses, double
     AssemblyOut out;
     RIGC_HIP(assemble_potential(ctx->as, ctx->scene, ctx->sel_all.sel, false, ctx->poses0.as<double>(), nullptr,
                                 dhat, dmin, 0.0, nullptr, (flags & RIGC_GRADIENT) != 0,
-                                (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
+                                (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st, (flags & RIGC_UNPROJECTED) == 0));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
     s = copy_potential(ctx, ctx->sel_all, out, flags, grad, hess);
     if (!out.pending) *energy = total(out.energy);
@@ -1067,7 +1067,8 @@ rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const
     AssemblyOut out;
     RIGC_HIP(assemble_potential(ctx->as, ctx->scene, ctx->sel_all.sel, true, ctx->poses0.as<double>(),
                                 ctx->poses2.as<double>(), 0.0, 0.0, epsv_times_h, nullptr,
-                                (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st));
+                                (flags & RIGC_GRADIENT) != 0, (flags & RIGC_HESSIAN) != 0, ctx->num_cus, out, st,
+                                (flags & RIGC_UNPROJECTED) == 0));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
     s = copy_potential(ctx, ctx->sel_all, out, flags, grad, hess);
     if (!out.pending) *energy = total(out.energy);
@@ -1075,6 +1076,31 @@ rigc_status rigc_compute_friction_term(rigc_ctx* ctx, const double* poses, const
     RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->stats.assembly_ms = ms;
     return s;
+}
+
+rigc_status rigc_friction_adjoint(rigc_ctx* ctx, const double* poses, const double* poses_t0, double epsv_times_h,
+                                  double dhat, double dmin, double barrier_stiffness, const double* lambda,
+                                  double* out) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_friction_adjoint: no scene");
+    if (!poses || !poses_t0 || !lambda || !out || !(epsv_times_h > 0) || !(dhat > 0) || !(dmin >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_friction_adjoint: bad arguments");
+    if (ctx->scene.n_scenes != 1)
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_friction_adjoint: single-scene contexts only");
+    if (!ctx->as.have_fric)
+        return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_friction_adjoint: no friction set (update it first)");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    rigc_status s = upload_poses(ctx, poses, nullptr);
+    if (s == RIGC_OK) s = put_poses(ctx, 2, poses_t0);
+    if (s != RIGC_OK) return s;
+    hipStream_t st = ctx->stream;
+    const int nb = ctx->scene.nb;
+    RIGC_HIP(ctx->out_rows.ensure(6 * sizeof(double) * (size_t)std::max(nb, 1)));
+    RIGC_HIP(hipMemcpyAsync(ctx->out_rows.p, lambda, 6 * sizeof(double) * (size_t)nb, hipMemcpyHostToDevice, st));
+    RIGC_HIP(friction_adjoint(ctx->as, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
+                              ctx->poses2.as<double>(), epsv_times_h, dhat, dmin, barrier_stiffness,
+                              ctx->out_rows.as<double>(), out, st));
+    return RIGC_OK;
 }
 
 // ------------------------------------------------------------ batched scenes

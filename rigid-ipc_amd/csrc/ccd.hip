@@ -289,7 +289,7 @@ __global__ void ccd_finish_kernel(CcdArgs a) {
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void ccd_kernel(CcdArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void ccd_kernel(CcdArgs a) {
     const long long nq = a.n_ee + a.n_fv;
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ULL << lane) - 1ULL;

@@ -242,6 +242,10 @@ struct CcdReadback {
     const void* src;
     size_t bytes;
 };
+// Resident 256-lane blocks of the persistent CCD kernel per CU: the kernel
+// runs one wavefront per SIMD so that its register demand (above 256 per
+// lane) lives in VGPRs and AGPRs instead of scratch.
+constexpr int CCD_BLOCKS_PER_CU = 1;
 hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& work,
                       const CcdReadback* readback = nullptr, int n_readback = 0);
 
@@ -282,6 +286,7 @@ struct ClassRanges {
 // Assembly working set (device buffers reused across calls).
 struct Assembly {
     DevBuf R, frames, W, W0;            // body rotations / frames, world vertices
+    DevBuf frames0, fadj;               // friction adjoint: frames at the step start, per-record output
     DevBuf ckeys[4], cstart[2], ccount, ctmp, chist, cflags, cpos, cwork;
     DevBuf cons;                        // ConRec [n_con], classes VV, EV, EE, FV in turn
     long long n_con = 0, counts[4] = { 0, 0, 0, 0 };
@@ -349,8 +354,14 @@ hipError_t constraint_counts(Assembly& as, const DevScene& sc, const SceneSel& s
 hipError_t assemble_potential(Assembly& as, const DevScene& sc, const SceneSel& sel, bool friction,
                               const double* d_poses, const double* d_poses_t0, double dhat, double dmin,
                               double epsv_h, const double* d_epsv_h_scene, bool want_grad, bool want_hess,
-                              int num_cus, AssemblyOut& out, hipStream_t stream);
+                              int num_cus, AssemblyOut& out, hipStream_t stream, bool project = true);
 void finish_potential(Assembly& as, AssemblyOut& out);
+// Friction adjoint over the whole friction set (single-scene contexts):
+// per record, the 12 body dofs of (d grad_x D / d x0)^T lambda; summed into
+// out [nb][6] on the host in record order (deterministic).
+hipError_t friction_adjoint(Assembly& as, const DevScene& sc, const SceneSel& sel, const double* d_poses,
+                            const double* d_poses_t0, double epsv_h, double dhat, double dmin, double kappa,
+                            const double* d_lambda, double* out, hipStream_t stream);
 // Friction sets of the selected scenes from the current constraint set
 // (which must be the selected scenes' set at d_poses); kappa / mu per scene
 // (device arrays indexed by scene id).

@@ -56,6 +56,11 @@ typedef struct rigs_backend {
      * (rigc_compute_earliest_toi_piecewise_linear). */
     rigc_status (*compute_earliest_toi_piecewise_linear)(void* self, const double* poses_t0, const double* poses_t1,
                                                          double minimum_separation_distance, double* earliest_toi);
+    /* Optional (may be NULL; then rollouts cannot be differentiated):
+     * rigc_friction_adjoint. */
+    rigc_status (*friction_adjoint)(void* self, const double* poses, const double* poses_t0, double epsv_times_h,
+                                    double dhat, double dmin, double barrier_stiffness, const double* lambda,
+                                    double* out);
 } rigs_backend;
 
 /* Trajectory types of the CCD (TrajectoryType, src/ccd/ccd.hpp:30-38; the
@@ -82,6 +87,13 @@ typedef struct rigs_params {
                                         * rigid_body_problem.cpp:261-267) */
     int32_t trajectory_type;           /* RIGS_TRAJECTORY_*: the line search's
                                         * CCD (distance_barrier_constraint.trajectory_type) */
+    int32_t newton_polish;             /* full Newton steps taken after the solve has
+                                        * converged (0 in the reference): each is
+                                        * taken when the CCD allows the whole step,
+                                        * with the barrier stiffness held, so that the
+                                        * step ends at the minimiser to rounding and
+                                        * a rollout is a smooth function of its inputs
+                                        * (differentiable rollouts) */
 } rigs_params;
 
 typedef struct rigs_bodies {
@@ -146,6 +158,29 @@ const char* rigs_last_error(const rigs_sim* sim);
 /* Advance num_steps time steps (DistanceBarrierRBProblem::simulation_step). */
 rigc_status rigs_step(rigs_sim* sim, int32_t num_steps);
 rigc_status rigs_get_state(const rigs_sim* sim, double* poses, double* velocities);
+/* Back to poses / velocities (layout of rigs_bodies) with the solver state
+ * of a fresh rigs_create (barrier stiffness, caches); clears the record. */
+rigc_status rigs_set_state(rigs_sim* sim, const double* poses, const double* velocities);
+
+/* ---- Differentiable rollouts (no counterpart in the reference, which has
+ * no gradients: python/src/bindings.cpp:30-193).
+ * rigs_rollout advances num_steps steps as rigs_step does and keeps a
+ * checkpoint of each step's start state (poses, velocities, Qdot) and of
+ * the barrier stiffnesses its solve used.
+ * rigs_rollout_backward differentiates a loss L of the rollout's end poses
+ * and (optionally, may be NULL) end velocities: given dL_dposes [nb][6]
+ * and dL_dvelocities [nb][6], it writes dL/d(initial velocities) [nb][6]
+ * (linear, then angular in the body frame, as rigs_bodies holds them) and,
+ * when dL_dposes0 is not NULL, dL/d(initial poses). Each step is rebuilt
+ * from its checkpoint (constraint set at its end poses, lagged friction set
+ * at its start poses) and differentiated through its converged Newton
+ * solve by the implicit function theorem with the unprojected Hessian.
+ * The barrier stiffness is held at the values the forward solve used.
+ * Needs converged steps, one lagged friction set per step, and a backend
+ * with friction_adjoint when mu > 0; the simulation's end state is kept. */
+rigc_status rigs_rollout(rigs_sim* sim, int32_t num_steps);
+rigc_status rigs_rollout_backward(rigs_sim* sim, const double* dL_dposes, const double* dL_dvelocities,
+                                  double* dL_dvelocities0, double* dL_dposes0);
 rigc_status rigs_get_stats(const rigs_sim* sim, rigs_stats* stats);
 
 /* ---- Batches of independent scenes (no counterpart in the reference,

@@ -71,6 +71,10 @@ rigc_status gb_stats(void* s, rigc_stats* st) { return rigc_get_stats((rigc_ctx*
 rigc_status gb_isect(void* s, const double* p, int64_t* n) {
     return rigc_detect_intersections((rigc_ctx*)s, p, nullptr, 0, n);
 }
+rigc_status gb_fadjoint(void* s, const double* p, const double* p0, double eh, double dhat, double dmin, double kappa,
+                        const double* lambda, double* out) {
+    return rigc_friction_adjoint((rigc_ctx*)s, p, p0, eh, dhat, dmin, kappa, lambda, out);
+}
 rigc_status gb_toi_pl(void* s, const double* p0, const double* p1, double msd, double* toi) {
     return rigc_compute_earliest_toi_piecewise_linear((rigc_ctx*)s, p0, p1, msd, toi);
 }
@@ -747,6 +751,27 @@ struct rigs_sim {
         return RIGC_OK;
     }
 
+    // P.newton_polish full Newton steps from a converged x (dir: the Newton
+    // direction at x), each taken only when the CCD allows the whole step;
+    // the barrier stiffness is held.
+    rigc_status polish(std::vector<double>& x, std::vector<double>& dir, std::vector<double>& g, BlockHess& H) {
+        for (int k = 0; k < P.newton_polish; k++) {
+            std::vector<double> x1(x);
+            for (size_t i = 0; i < x.size(); i++) x1[i] += dir[i];
+            double toi = 0;
+            rigc_status s = earliest_toi(x.data(), x1.data(), toi);
+            if (s != RIGC_OK) return s;
+            if (!(toi >= 1.0)) break;
+            x.swap(x1);
+            stats.newton_iterations++;
+            double fx = 0;
+            s = objective(x.data(), true, true, fx, g, H);
+            if (s != RIGC_OK) return s;
+            if (!solve_direction(g, H, 0.0, dir)) break;
+        }
+        return RIGC_OK;
+    }
+
     // IPCSolver::solve = init_solve + NewtonSolver::solve
     rigc_status newton_solve(std::vector<double>& x, bool& success) {
         rigc_status s = init_solve(x);
@@ -781,6 +806,8 @@ struct rigs_sim {
             stats.host_ms += now_ms() - t0;
             if (it > 0 && converged(x, dir)) {
                 success = true;
+                s = polish(x, dir, g, H);
+                if (s != RIGC_OK) return s;
                 break;
             }
             double step = 1;
@@ -815,6 +842,71 @@ struct rigs_sim {
         return RIGC_OK;
     }
 
+    // ------------------------------------------------------------ state
+    // Body b at pose / velocity (linear in world coordinates, angular in
+    // the body frame); Qdot = Q [omega]_x (rigid_body.cpp:159-160).
+    void set_body_state(int b, const double* pose, const double* velocity) {
+        BodyState& B = bodies[b];
+        std::memcpy(B.pose, pose, sizeof(B.pose));
+        std::memcpy(B.pose_prev, B.pose, sizeof(B.pose));
+        std::memcpy(B.velocity, velocity, sizeof(B.velocity));
+        qdot_from_omega(B.pose + 3, B.velocity + 3, B.Qdot);
+        if (!B.dynamic) {
+            for (int k = 0; k < 6; k++) B.velocity[k] = 0;
+            for (int e = 0; e < 9; e++) B.Qdot[e] = 0;
+        }
+    }
+    static void qdot_from_omega(const double* r, const double* w, double* Qdot) {
+        double Q[9];
+        rigc::rotation_d(r, Q);
+        const double K[9] = { 0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0 };
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double v = 0;
+                for (int k = 0; k < 3; k++) v += Q[3 * i + k] * K[3 * k + j];
+                Qdot[3 * i + j] = v;
+            }
+    }
+    // init_bbox_diagonal over the current world vertices; poses_t0 = poses
+    void init_bbox() {
+        std::vector<double> x0(6 * (size_t)nb), W;
+        for (int b = 0; b < nb; b++) std::memcpy(&x0[6 * b], bodies[b].pose, sizeof(double) * 6);
+        world_vertices(x0.data(), W);
+        double lo[3] = { HUGE_VAL, HUGE_VAL, HUGE_VAL }, hi[3] = { -HUGE_VAL, -HUGE_VAL, -HUGE_VAL };
+        for (int i = 0; i < nv; i++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], W[3 * i + k]);
+                hi[k] = std::max(hi[k], W[3 * i + k]);
+            }
+        double d2 = 0;
+        for (int k = 0; k < 3; k++) d2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
+        bbox_diagonal = std::sqrt(d2);
+        poses_t0 = x0;
+    }
+    // Back to the state rigs_create leaves (barrier stiffness, caches).
+    void reset_solver_state() {
+        kappa = 1;
+        max_kappa = 0;
+        prev_min_distance = -1;
+        cache_valid = false;
+        bcache.valid = fcache.valid = false;
+        ecache.valid = false;
+    }
+
+    // ------------------------------------------------------------ differentiable rollouts
+    // A step's start state and the barrier stiffnesses its solve used: the
+    // lagged friction set's (built at the step start) and the final one.
+    struct Checkpoint {
+        std::vector<double> poses, velocities, Qdot;  // [nb][6], [nb][6], [nb][9]
+        double kappa_friction = 0, kappa = 0;
+        bool differentiable = true;  // converged, one lagged friction set
+    };
+    bool recording = false;
+    std::vector<Checkpoint> tape;
+    std::vector<double> tape_end;  // poses after the last recorded step
+
+    rigc_status rollout_backward(const double* dLdx, const double* dLdv, double* dLdv0, double* dLdx0);
+
     // ------------------------------------------------------------ step
     rigc_status step() {
         const double t_start = now_ms();
@@ -824,6 +916,18 @@ struct rigs_sim {
         for (int b = 0; b < nb; b++) std::memcpy(&poses_t0[6 * b], bodies[b].pose, sizeof(double) * 6);
         bcache.valid = fcache.valid = false;  // the friction term depends on poses_t0
         ecache.valid = false;                 // and the body energy on the step's targets
+        if (recording) {
+            Checkpoint cp;
+            cp.poses = poses_t0;
+            cp.velocities.resize(6 * (size_t)nb);
+            cp.Qdot.resize(9 * (size_t)nb);
+            for (int b = 0; b < nb; b++) {
+                std::memcpy(&cp.velocities[6 * b], bodies[b].velocity, sizeof(double) * 6);
+                std::memcpy(&cp.Qdot[9 * b], bodies[b].Qdot, sizeof(double) * 9);
+            }
+            cp.kappa_friction = kappa;  // update_friction below uses it
+            tape.push_back(std::move(cp));
+        }
         step_targets();
         // update_constraints: barrier gradient at t0 (kept by the reference for
         // its integrators) and the lagged friction set at t0
@@ -843,6 +947,12 @@ struct rigs_sim {
         for (;;) {
             rigc_status s = newton_solve(x, success);
             if (s != RIGC_OK) return s;
+            if (recording) {
+                Checkpoint& cp = tape.back();
+                if (i == 0) cp.kappa = kappa;
+                else cp.differentiable = false;  // a second lagged friction set
+                if (!success) cp.differentiable = false;
+            }
             if (!success) {
                 stats.failed_solves++;
                 break;
@@ -894,6 +1004,7 @@ struct rigs_sim {
             B.velocity[4] = 0.5 * (W[2] - W[6]);
             B.velocity[5] = 0.5 * (W[3] - W[1]);
         }
+        if (recording) tape_end = x;
         if (P.do_intersection_check) {
             int64_t n = -1;
             s = intersections(x.data(), n);
@@ -907,6 +1018,255 @@ struct rigs_sim {
         return RIGC_OK;
     }
 };
+
+// Implicit-function adjoint of a recorded rollout. Step n maps its start
+// state s_n = (x_n, v_n, Qdot_n) to x_{n+1} = argmin_x F(x; s_n) with
+//   F = E(x; p_n + h (v_n + h g), Q(r_n) + h Qdot_n) / m + kappa / m B(x)
+//       + D(x; x_n) / m
+// (compute_objective, distance_barrier_rb_problem.cpp:633-703), then sets
+// v_{n+1} = (p_{n+1} - p_n) / h and Qdot_{n+1} = (Q(r_{n+1}) - Q(r_n)) / h
+// (take_step, :488-626 for IMPLICIT_EULER). At a converged step
+// grad_x F(x_{n+1}; s_n) = 0 on the free dofs, so with H = hess_x F there
+// (unprojected) and lambda = H^-1 dL/dx_{n+1}:
+//   dL/ds_n = -lambda^T d(grad_x F)/ds_n.
+// The body energy's dependence on s_n is through its inertial targets; the
+// friction potential's through x_n (rigc_friction_adjoint); the barrier
+// potential does not depend on s_n. The barrier stiffnesses are taken as
+// the forward solve left them.
+rigc_status rigs_sim::rollout_backward(const double* dLdx, const double* dLdv, double* dLdv0, double* dLdx0) {
+    if (tape.empty()) return fail(RIGC_ERR_INVALID_ARGUMENT, "rollout_backward: no recorded rollout");
+    for (size_t n = 0; n < tape.size(); n++)
+        if (!tape[n].differentiable)
+            return fail(RIGC_ERR_UNRESOLVED, "rollout_backward: step " + std::to_string(n) +
+                                                 " did not converge or took several lagged friction sets");
+    const bool fric = P.coefficient_friction > 0;
+    if (fric && !be.friction_adjoint)
+        return fail(RIGC_ERR_INVALID_ARGUMENT, "rollout_backward: the backend has no friction adjoint");
+    const double h = P.timestep, im = 1.0 / average_mass;
+    const double eh = P.static_friction_speed_bound * P.timestep;
+    const int T = (int)tape.size();
+    // the forward end state is restored afterwards
+    const std::vector<BodyState> keep_bodies = bodies;
+    const std::vector<double> keep_t0 = poses_t0;
+    const double keep_k = kappa, keep_mk = max_kappa, keep_pm = prev_min_distance;
+
+    std::vector<double> ax(dLdx, dLdx + 6 * (size_t)nb), av(3 * (size_t)nb, 0.0), aQ(9 * (size_t)nb, 0.0);
+    if (dLdv) {
+        // v_T (linear) and omega_T = vee of the skew part of Q_T^T Qdot_T
+        const std::vector<double>& x1 = tape_end;
+        const std::vector<double>& x0 = tape[T - 1].poses;
+        for (int b = 0; b < nb; b++) {
+            if (!bodies[b].dynamic) continue;
+            for (int k = 0; k < 3; k++) av[3 * b + k] = dLdv[6 * b + k];
+            double Q1[9], Q0[9], Qd[9];
+            rigc::rotation_d(&x1[6 * b + 3], Q1);
+            rigc::rotation_d(&x0[6 * b + 3], Q0);
+            for (int e = 0; e < 9; e++) Qd[e] = (Q1[e] - Q0[e]) / h;
+            // omega = 0.5 (W21 - W12, W02 - W20, W10 - W01), W = Q^T Qdot
+            const double* aw = dLdv + 6 * b + 3;
+            double dW[9] = { 0 };
+            dW[7] += 0.5 * aw[0]; dW[5] -= 0.5 * aw[0];
+            dW[2] += 0.5 * aw[1]; dW[6] -= 0.5 * aw[1];
+            dW[3] += 0.5 * aw[2]; dW[1] -= 0.5 * aw[2];
+            // dL/dQdot = Q dW; dL/dQ = Qdot dW^T
+            double dQ[9] = { 0 };
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++)
+                    for (int k = 0; k < 3; k++) {
+                        aQ[9 * b + 3 * i + j] += Q1[3 * i + k] * dW[3 * k + j];
+                        dQ[3 * i + j] += Qd[3 * i + k] * dW[3 * j + k];
+                    }
+            rigc::BodyFrame F;
+            rigc::body_frame(&x1[6 * b + 3], F);
+            for (int c = 0; c < 3; c++)
+                for (int e = 0; e < 9; e++) ax[6 * b + 3 + c] += F.dR[c][e] * dQ[e];
+        }
+    }
+    rigc_status s = RIGC_OK;
+    std::vector<double> lambda(6 * (size_t)nb), fadj(6 * (size_t)nb);
+    for (int n = T - 1; n >= 0 && s == RIGC_OK; n--) {
+        const Checkpoint& cp = tape[n];
+        const std::vector<double>& x1 = n == T - 1 ? tape_end : tape[n + 1].poses;
+        const std::vector<double>& x0 = cp.poses;
+        // the step's velocity and Qdot updates
+        std::vector<double> g(ax), b0(6 * (size_t)nb, 0.0);
+        for (int b = 0; b < nb; b++) {
+            if (!bodies[b].dynamic) continue;
+            for (int k = 0; k < 3; k++) {
+                g[6 * b + k] += av[3 * b + k] / h;
+                b0[6 * b + k] -= av[3 * b + k] / h;
+            }
+            rigc::BodyFrame F1, F0;
+            rigc::body_frame(&x1[6 * b + 3], F1);
+            rigc::body_frame(&x0[6 * b + 3], F0);
+            for (int c = 0; c < 3; c++) {
+                double s1 = 0, s0 = 0;
+                for (int e = 0; e < 9; e++) {
+                    s1 += F1.dR[c][e] * aQ[9 * b + e];
+                    s0 += F0.dR[c][e] * aQ[9 * b + e];
+                }
+                g[6 * b + 3 + c] += s1 / h;
+                b0[6 * b + 3 + c] -= s0 / h;
+            }
+        }
+        // the step's objective at x_{n+1}, rebuilt from the checkpoint
+        for (int b = 0; b < nb; b++) {
+            BodyState& B = bodies[b];
+            std::memcpy(B.pose, &x0[6 * b], sizeof(double) * 6);
+            std::memcpy(B.velocity, &cp.velocities[6 * b], sizeof(double) * 6);
+            std::memcpy(B.Qdot, &cp.Qdot[9 * b], sizeof(double) * 9);
+        }
+        step_targets();
+        poses_t0 = x0;
+        reset_solver_state();
+        if (fric) {
+            int64_t nf = 0;
+            const double t0 = now_ms();
+            s = be.update_friction_constraints(be.self, x0.data(), P.dhat, P.minimum_separation_distance,
+                                               cp.kappa_friction, P.coefficient_friction, RIGC_BUILD_CONSTRAINTS, &nf);
+            stats.contact_ms += now_ms() - t0;
+            if (s != RIGC_OK) {
+                backend_fail(s, "update_friction_constraints");
+                break;
+            }
+        }
+        kappa = cp.kappa;
+        s = ensure_constraints(x1.data());
+        if (s != RIGC_OK) break;
+        std::vector<double> gE, gB, gD;
+        BlockHess H, HB, HD;
+        energy_term(x1.data(), &gE, &H);
+        const int32_t flags = RIGC_HESSIAN | RIGC_UNPROJECTED;
+        double B = 0, D = 0, spent = 0;
+        auto bcall = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
+            return be.compute_barrier_term(be.self, x1.data(), P.dhat, P.minimum_separation_distance, f, pe, pg, ph);
+        };
+        s = potential(bcall, stg_barrier, flags, B, &gB, &HB, spent);
+        if (s != RIGC_OK) {
+            backend_fail(s, "compute_barrier_term");
+            break;
+        }
+        if (fric) {
+            auto fcall = [&](int32_t f, double* pe, double* pg, rigc_block_hessian* ph) {
+                return be.compute_friction_term(be.self, x1.data(), x0.data(), eh, f, pe, pg, ph);
+            };
+            s = potential(fcall, stg_friction, flags, D, &gD, &HD, spent);
+            if (s != RIGC_OK) {
+                backend_fail(s, "compute_friction_term");
+                break;
+            }
+        }
+        stats.contact_ms += spent;
+        const double km = kappa * im;
+        for (int b = 0; b < nb; b++)
+            for (int e = 0; e < 36; e++) {
+                double v = H.diag[b][e] * im + km * HB.diag[b][e];
+                if (fric) v += HD.diag[b][e] * im;
+                H.diag[b][e] = v;
+            }
+        for (auto& kv : HB.off) {
+            Block& o = H.off[kv.first];
+            for (int e = 0; e < 36; e++) o[e] += km * kv.second[e];
+        }
+        if (fric)
+            for (auto& kv : HD.off) {
+                Block& o = H.off[kv.first];
+                for (int e = 0; e < 36; e++) o[e] += im * kv.second[e];
+            }
+        // lambda = H^-1 dL/dx_{n+1} on the free dofs (solve_direction solves
+        // H d = -rhs)
+        std::vector<double> neg(6 * (size_t)nb);
+        for (int i = 0; i < 6 * nb; i++) neg[i] = -g[i];
+        if (!solve_direction(neg, H, 0.0, lambda)) {
+            s = fail(RIGC_ERR_UNRESOLVED, "rollout_backward: the Hessian of step " + std::to_string(n) +
+                                              " could not be factorized");
+            break;
+        }
+        // -lambda^T d(grad F)/ds_n
+        std::vector<double> nax(b0), nav(3 * (size_t)nb, 0.0), naQ(9 * (size_t)nb, 0.0);
+        for (int b = 0; b < nb; b++) {
+            const BodyState& Bs = bodies[b];
+            if (!Bs.dynamic) continue;
+            const double* lb = &lambda[6 * b];
+            for (int k = 0; k < 3; k++) {
+                nax[6 * b + k] += Bs.mass * im * lb[k];
+                nav[3 * b + k] += h * Bs.mass * im * lb[k];
+            }
+            rigc::BodyFrame F1, F0;
+            rigc::body_frame(&x1[6 * b + 3], F1);
+            rigc::body_frame(&x0[6 * b + 3], F0);
+            double dP[9];
+            for (int e = 0; e < 9; e++) {
+                double sum = 0;
+                for (int c = 0; c < 3; c++) sum += lb[3 + c] * F1.dR[c][e];
+                dP[e] = sum * Bs.J[e % 3] * im;
+            }
+            for (int e = 0; e < 9; e++) naQ[9 * b + e] += h * dP[e];
+            for (int c = 0; c < 3; c++) {
+                double sum = 0;
+                for (int e = 0; e < 9; e++) sum += F0.dR[c][e] * dP[e];
+                nax[6 * b + 3 + c] += sum;
+            }
+        }
+        if (fric) {
+            const double t0 = now_ms();
+            s = be.friction_adjoint(be.self, x1.data(), x0.data(), eh, P.dhat, P.minimum_separation_distance,
+                                    cp.kappa_friction, lambda.data(), fadj.data());
+            stats.contact_ms += now_ms() - t0;
+            if (s != RIGC_OK) {
+                backend_fail(s, "friction_adjoint");
+                break;
+            }
+            for (int i = 0; i < 6 * nb; i++) nax[i] -= im * fadj[i];
+        }
+        ax.swap(nax);
+        av.swap(nav);
+        aQ.swap(naQ);
+    }
+    if (s == RIGC_OK) {
+        const std::vector<double>& x0 = tape[0].poses;
+        for (int b = 0; b < nb; b++) {
+            double* o = dLdv0 + 6 * b;
+            for (int k = 0; k < 6; k++) o[k] = 0;
+            if (dLdx0)
+                for (int k = 0; k < 6; k++) dLdx0[6 * b + k] = bodies[b].dynamic ? ax[6 * b + k] : 0.0;
+            if (!bodies[b].dynamic) continue;
+            for (int k = 0; k < 3; k++) o[k] = av[3 * b + k];
+            // Qdot_0 = Q_0 [omega_0]_x
+            double Q0[9];
+            rigc::rotation_d(&x0[6 * b + 3], Q0);
+            for (int c = 0; c < 3; c++) {
+                double sum = 0;
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        double qk = 0;  // (Q_0 [e_c]_x)_ij
+                        for (int k = 0; k < 3; k++) qk += Q0[3 * i + k] * rigc::v3_skew_unit(c, k, j);
+                        sum += aQ[9 * b + 3 * i + j] * qk;
+                    }
+                o[3 + c] = sum;
+            }
+            if (dLdx0) {
+                const double* w = &tape[0].velocities[6 * b + 3];
+                const double K[9] = { 0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0 };
+                double dQ[9] = { 0 };  // dL/dQ_0 = aQ K^T
+                for (int i = 0; i < 3; i++)
+                    for (int k = 0; k < 3; k++)
+                        for (int j = 0; j < 3; j++) dQ[3 * i + k] += aQ[9 * b + 3 * i + j] * K[3 * k + j];
+                rigc::BodyFrame F0;
+                rigc::body_frame(&x0[6 * b + 3], F0);
+                for (int c = 0; c < 3; c++)
+                    for (int e = 0; e < 9; e++) dLdx0[6 * b + 3 + c] += F0.dR[c][e] * dQ[e];
+            }
+        }
+    }
+    bodies = keep_bodies;
+    poses_t0 = keep_t0;
+    reset_solver_state();
+    kappa = keep_k;
+    max_kappa = keep_mk;
+    prev_min_distance = keep_pm;
+    return s;
+}
 
 namespace rigs {
 void set_fork_hooks(rigs_sim* sim, void (*join)(void*), void (*part)(void*), void* arg) {
@@ -948,6 +1308,7 @@ void rigs_default_params(rigs_params* p) {
     p->min_barrier_stiffness_scale = 1e11;
     p->do_intersection_check = 0;
     p->trajectory_type = RIGS_TRAJECTORY_RIGID;
+    p->newton_polish = 0;
 }
 
 rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t device, const rigc_scene* scene,
@@ -974,7 +1335,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
             return s;
         }
         sim->be = rigs_backend{ sim->own_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl };
+                                gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl, gb_fadjoint };
     }
     sim->fbe = sim->be;
     rigc_status s = sim->be.set_scene(sim->be.self, scene);
@@ -991,7 +1352,7 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         // term just runs through the main context)
         if (rigc_create(&sim->fric_ctx, device) == RIGC_OK && rigc_set_scene(sim->fric_ctx, scene) == RIGC_OK) {
             sim->fbe = rigs_backend{ sim->fric_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                     gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl };
+                                     gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl, gb_fadjoint };
         } else if (sim->fric_ctx) {
             rigc_destroy(sim->fric_ctx);
             sim->fric_ctx = nullptr;
@@ -1014,43 +1375,16 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         B.J[0] = 0.5 * (-B.inertia[0] + B.inertia[1] + B.inertia[2]);
         B.J[1] = 0.5 * (B.inertia[0] - B.inertia[1] + B.inertia[2]);
         B.J[2] = 0.5 * (B.inertia[0] + B.inertia[1] - B.inertia[2]);
-        std::memcpy(B.pose, bodies->poses + 6 * b, sizeof(B.pose));
-        std::memcpy(B.pose_prev, B.pose, sizeof(B.pose));
-        std::memcpy(B.velocity, bodies->velocities + 6 * b, sizeof(B.velocity));
-        // Qdot = Q [omega]_x (rigid_body.cpp:159-160)
-        double Q[9];
-        rigc::rotation_d(B.pose + 3, Q);
-        const double* w = B.velocity + 3;
-        const double K[9] = { 0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0 };
-        for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 3; c++) {
-                double v = 0;
-                for (int k = 0; k < 3; k++) v += Q[3 * r + k] * K[3 * k + c];
-                B.Qdot[3 * r + c] = B.dynamic ? v : 0.0;
-            }
+        sim->set_body_state(b, bodies->poses + 6 * b, bodies->velocities + 6 * b);
         if (B.dynamic) {
             sim->dyn_index[b] = sim->n_free++;
             mass_sum += 3 * B.mass + B.inertia[0] + B.inertia[1] + B.inertia[2];
             n_free_dof += 6;
-        } else {
-            for (int k = 0; k < 6; k++) B.velocity[k] = 0;
         }
     }
     sim->average_mass = n_free_dof ? mass_sum / n_free_dof : 1.0;
-    // init_bbox_diagonal over the initial world vertices
-    std::vector<double> x0(6 * (size_t)nb), W;
-    for (int b = 0; b < nb; b++) std::memcpy(&x0[6 * b], sim->bodies[b].pose, sizeof(double) * 6);
-    sim->world_vertices(x0.data(), W);
-    double lo[3] = { HUGE_VAL, HUGE_VAL, HUGE_VAL }, hi[3] = { -HUGE_VAL, -HUGE_VAL, -HUGE_VAL };
-    for (int i = 0; i < sim->nv; i++)
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], W[3 * i + k]);
-            hi[k] = std::max(hi[k], W[3 * i + k]);
-        }
-    double d2 = 0;
-    for (int k = 0; k < 3; k++) d2 += (hi[k] - lo[k]) * (hi[k] - lo[k]);
-    sim->bbox_diagonal = std::sqrt(d2);
-    sim->poses_t0 = x0;
+    sim->init_bbox();
+    std::vector<double> x0(sim->poses_t0);
     // the initial state is checked for intersections
     // (RigidBodyProblem::init, rigid_body_problem.cpp:100)
     int64_t n0 = -1;
@@ -1083,6 +1417,33 @@ rigc_status rigs_step(rigs_sim* sim, int32_t num_steps) {
         if (s != RIGC_OK) return s;
     }
     return RIGC_OK;
+}
+
+rigc_status rigs_set_state(rigs_sim* sim, const double* poses, const double* velocities) {
+    if (!sim || !poses || !velocities) return RIGC_ERR_INVALID_ARGUMENT;
+    for (int b = 0; b < sim->nb; b++) sim->set_body_state(b, poses + 6 * b, velocities + 6 * b);
+    sim->init_bbox();
+    sim->reset_solver_state();
+    sim->tape.clear();
+    sim->tape_end.clear();
+    return RIGC_OK;
+}
+
+rigc_status rigs_rollout(rigs_sim* sim, int32_t num_steps) {
+    if (!sim || num_steps < 0) return RIGC_ERR_INVALID_ARGUMENT;
+    sim->tape.clear();
+    sim->tape_end.clear();
+    sim->recording = true;
+    rigc_status s = RIGC_OK;
+    for (int i = 0; i < num_steps && s == RIGC_OK; i++) s = sim->step();
+    sim->recording = false;
+    return s;
+}
+
+rigc_status rigs_rollout_backward(rigs_sim* sim, const double* dL_dposes, const double* dL_dvelocities,
+                                  double* dL_dvelocities0, double* dL_dposes0) {
+    if (!sim || !dL_dposes || !dL_dvelocities0) return RIGC_ERR_INVALID_ARGUMENT;
+    return sim->rollout_backward(dL_dposes, dL_dvelocities, dL_dvelocities0, dL_dposes0);
 }
 
 rigc_status rigs_get_state(const rigs_sim* sim, double* poses, double* velocities) {

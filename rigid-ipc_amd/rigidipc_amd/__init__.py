@@ -46,6 +46,7 @@ class Scene(ctypes.Structure):
 RIGC_GRADIENT = 1
 RIGC_HESSIAN = 2
 RIGC_BUILD_CONSTRAINTS = 4
+RIGC_UNPROJECTED = 8
 
 
 class BlockHessian(ctypes.Structure):
@@ -101,6 +102,7 @@ def lib():
     L.rigc_update_friction_constraints.argtypes = [vp, _c_double_p, d, d, d, d, i32, _c_int64_p]
     L.rigc_get_friction_constraints.argtypes = [vp, i64, _c_double_p, _c_int64_p]
     L.rigc_compute_friction_term.argtypes = [vp, _c_double_p, _c_double_p, d, i32, _c_double_p, _c_double_p, bh]
+    L.rigc_friction_adjoint.argtypes = [vp, _c_double_p, _c_double_p, d, d, d, d, _c_double_p, _c_double_p]
     L.rigc_set_scenes.argtypes = [vp, i32, ctypes.POINTER(Scene)]
     L.rigc_get_scene_count.argtypes = [vp, _c_int32_p]
     L.rigc_batch_earliest_toi.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, _c_double_p]
@@ -121,7 +123,7 @@ def lib():
                  "rigc_broad_phase", "rigc_compute_earliest_toi", "rigc_construct_constraint_set",
                  "rigc_get_constraint_set", "rigc_compute_barrier_term", "rigc_compute_minimum_distance",
                  "rigc_update_friction_constraints", "rigc_get_friction_constraints",
-                 "rigc_compute_friction_term", "rigc_detect_intersections",
+                 "rigc_compute_friction_term", "rigc_friction_adjoint", "rigc_detect_intersections",
                  "rigc_narrow_phase_piecewise_linear", "rigc_compute_earliest_toi_piecewise_linear"):
         getattr(L, name).restype = ctypes.c_int32
     _lib = L
@@ -477,12 +479,13 @@ class Context:
             self._check(fn(self._ctx, *args, int(flags), ctypes.byref(energy), _p(grad, _c_double_p), None))
         return energy.value, grad, hess
 
-    def barrier_term(self, poses, dhat, dmin=0.0, grad=False, hess=False, rebuild=False):
+    def barrier_term(self, poses, dhat, dmin=0.0, grad=False, hess=False, rebuild=False, unprojected=False):
         """Barrier energy over the current constraint set (rebuilt at poses
         if rebuild); grad (6 nb) and hess (diag (nb,6,6), pairs (m,2),
-        off (m,6,6)) when asked."""
+        off (m,6,6)) when asked, without project_to_psd when unprojected."""
         p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
         flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0) | (RIGC_BUILD_CONSTRAINTS if rebuild else 0)
+        flags |= RIGC_UNPROJECTED if unprojected else 0
         self._keep = [p]
         return self._potential(self._L.rigc_compute_barrier_term, (_p(p, _c_double_p), float(dhat), float(dmin)), flags)
 
@@ -513,12 +516,24 @@ class Context:
         self._check(self._L.rigc_get_friction_constraints(self._ctx, n.value, _p(rec, _c_double_p), ctypes.byref(n)))
         return rec[:n.value]
 
-    def friction_term(self, poses, poses_t0, epsv_times_h, grad=False, hess=False):
+    def friction_term(self, poses, poses_t0, epsv_times_h, grad=False, hess=False, unprojected=False):
         p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
         p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
-        flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0)
+        flags = (RIGC_GRADIENT if grad else 0) | (RIGC_HESSIAN if hess else 0) | (RIGC_UNPROJECTED if unprojected else 0)
         return self._potential(self._L.rigc_compute_friction_term,
                                (_p(p, _c_double_p), _p(p0, _c_double_p), float(epsv_times_h)), flags)
+
+    def friction_adjoint(self, poses, poses_t0, epsv_times_h, dhat, dmin, barrier_stiffness, lam):
+        """(d grad_x D(poses; poses_t0) / d poses_t0)^T lam over the current
+        lagged friction set (rigc_friction_adjoint)."""
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 6)
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        lm = np.ascontiguousarray(lam, np.float64).reshape(-1, 6)
+        out = np.zeros_like(p)
+        self._check(self._L.rigc_friction_adjoint(self._ctx, _p(p, _c_double_p), _p(p0, _c_double_p),
+                                                  float(epsv_times_h), float(dhat), float(dmin),
+                                                  float(barrier_stiffness), _p(lm, _c_double_p), _p(out, _c_double_p)))
+        return out.reshape(-1)
 
 
 def dense_hessian(hess, nb):

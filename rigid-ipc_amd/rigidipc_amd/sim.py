@@ -22,7 +22,7 @@ class Backend(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in (
         "self", "set_scene", "compute_earliest_toi", "construct_constraint_set", "compute_barrier_term",
         "compute_minimum_distance", "update_friction_constraints", "compute_friction_term", "last_error",
-        "get_stats", "detect_intersections", "compute_earliest_toi_piecewise_linear")]
+        "get_stats", "detect_intersections", "compute_earliest_toi_piecewise_linear", "friction_adjoint")]
 
 
 class Params(ctypes.Structure):
@@ -33,7 +33,7 @@ class Params(ctypes.Structure):
         ("velocity_conv_tol", ctypes.c_double), ("line_search_lower_bound", ctypes.c_double),
         ("max_iterations", ctypes.c_int32), ("dhat_epsilon", ctypes.c_double),
         ("min_barrier_stiffness_scale", ctypes.c_double), ("do_intersection_check", ctypes.c_int32),
-        ("trajectory_type", ctypes.c_int32),
+        ("trajectory_type", ctypes.c_int32), ("newton_polish", ctypes.c_int32),
     ]
 
 
@@ -91,6 +91,9 @@ def sim_lib():
     L.rigs_last_error.restype = ctypes.c_char_p
     L.rigs_step.argtypes = [vp, ctypes.c_int32]
     L.rigs_get_state.argtypes = [vp, _c_double_p, _c_double_p]
+    L.rigs_set_state.argtypes = [vp, _c_double_p, _c_double_p]
+    L.rigs_rollout.argtypes = [vp, ctypes.c_int32]
+    L.rigs_rollout_backward.argtypes = [vp, _c_double_p, _c_double_p, _c_double_p, _c_double_p]
     L.rigs_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.rigs_block_ldlt_solve.argtypes = [ctypes.c_int32, _c_double_p, ctypes.c_int64, _c_int32_p, _c_double_p,
                                         _c_double_p, _c_double_p]
@@ -103,7 +106,8 @@ def sim_lib():
     L.rigs_batch_get_state.argtypes = [vp, ctypes.c_int32, _c_double_p, _c_double_p]
     L.rigs_batch_get_stats.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Stats)]
     L.rigs_batch_get_batch_stats.argtypes = [vp, ctypes.POINTER(BatchStats)]
-    for name in ("rigs_create", "rigs_destroy", "rigs_step", "rigs_get_state", "rigs_get_stats",
+    for name in ("rigs_create", "rigs_destroy", "rigs_step", "rigs_get_state", "rigs_get_stats", "rigs_set_state",
+                 "rigs_rollout", "rigs_rollout_backward",
                  "rigs_block_ldlt_solve", "rigs_batch_create", "rigs_batch_destroy", "rigs_batch_step",
                  "rigs_batch_get_state", "rigs_batch_get_stats", "rigs_batch_get_batch_stats"):
         getattr(L, name).restype = ctypes.c_int32
@@ -284,6 +288,36 @@ class Simulation:
         vel = np.zeros((self.num_bodies, 6))
         self._L.rigs_get_state(self._sim, _p(poses, _c_double_p), _p(vel, _c_double_p))
         return poses, vel
+
+    def set_state(self, poses, velocities):
+        """Restart from poses / velocities with a fresh solver state."""
+        p = np.ascontiguousarray(poses, np.float64).reshape(self.num_bodies, 6)
+        v = np.ascontiguousarray(velocities, np.float64).reshape(self.num_bodies, 6)
+        st = self._L.rigs_set_state(self._sim, _p(p, _c_double_p), _p(v, _c_double_p))
+        if st != RIGC_OK:
+            raise RigcError(st, "rigs_set_state failed")
+
+    def rollout(self, n):
+        """Advance n steps keeping the checkpoints rollout_backward needs."""
+        st = self._L.rigs_rollout(self._sim, int(n))
+        if st != RIGC_OK:
+            raise RigcError(st, self._L.rigs_last_error(self._sim).decode())
+
+    def rollout_backward(self, dL_dposes, dL_dvelocities=None, want_poses0=False):
+        """dL/d(initial velocities) [nb, 6] of the last rollout (and dL/d(initial
+        poses) when want_poses0) from dL/d(end poses) and, optionally,
+        dL/d(end velocities)."""
+        gx = np.ascontiguousarray(dL_dposes, np.float64).reshape(self.num_bodies, 6)
+        gv = None if dL_dvelocities is None else np.ascontiguousarray(dL_dvelocities, np.float64).reshape(
+            self.num_bodies, 6)
+        out = np.zeros((self.num_bodies, 6))
+        out_x = np.zeros((self.num_bodies, 6)) if want_poses0 else None
+        st = self._L.rigs_rollout_backward(self._sim, _p(gx, _c_double_p),
+                                           None if gv is None else _p(gv, _c_double_p), _p(out, _c_double_p),
+                                           None if out_x is None else _p(out_x, _c_double_p))
+        if st != RIGC_OK:
+            raise RigcError(st, self._L.rigs_last_error(self._sim).decode())
+        return (out, out_x) if want_poses0 else out
 
     def stats(self):
         s = Stats()

@@ -244,6 +244,7 @@ def _declare_barrier():
     L.oracle_friction.argtypes = [c_double_p, c_double_p, ctypes.c_double, c_double_p, c_double_p]
     L.oracle_friction.restype = ctypes.c_double
     L.oracle_vertex_derivatives.argtypes = [c_double_p, ctypes.c_int, c_double_p, c_double_p, c_double_p]
+    L.oracle_friction_adjoint.argtypes = [c_double_p, c_double_p] + [ctypes.c_double] * 4 + [c_double_p, c_double_p]
     L._barrier_declared = True
     return L
 
@@ -315,3 +316,15 @@ def assemble_hessian(rec, local_hess, nb):
             for j in range(2):
                 Hd[6 * bodies[i]:6 * bodies[i] + 6, 6 * bodies[j]:6 * bodies[j] + 6] += h[6 * i:6 * i + 6, 6 * j:6 * j + 6]
     return Hd
+
+
+def friction_adjoint(poses, poses_t0, epsv_times_h, dhat, dmin, kappa, lam):
+    """(d grad_x D(poses; poses_t0) / d poses_t0)^T lam over the current
+    friction set (dual numbers)."""
+    L = _declare_barrier()
+    p = np.ascontiguousarray(poses, np.float64)
+    out = np.zeros(p.size)
+    L.oracle_friction_adjoint(dptr(p), dptr(np.ascontiguousarray(poses_t0, np.float64)), float(epsv_times_h),
+                              float(dhat), float(dmin), float(kappa), dptr(np.ascontiguousarray(lam, np.float64)),
+                              dptr(out))
+    return out
