@@ -1,6 +1,6 @@
 """Benchmark of the MI355X contact hot path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sim|ccd|batch|intersect] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sim|ccd|plccd|batch|intersect|grad] [--no-cpu]
 
 Workload "sim" (default): the 1,000-body mixed scene (ico2 / torus / cube,
 seeded, plus the ground plane; dt 0.01, dhat 1e-3, mu 0.2) stepped with
@@ -17,6 +17,12 @@ The sim line also carries "batched": the batching workload (256
 independent scenes of 8-512 bodies) advanced --batch-steps steps with the
 fused batch and with the plain path scene after scene (same trajectories).
 Workload "batch" runs that comparison alone for K steps (default all 10).
+
+Workload "grad" (also "gradient" in the sim line): the differentiation
+scene (64 spheres dropped on the plane, 20 steps, mu 0.5), gradient of the
+end-centre loss with respect to all 384 initial velocities by the stepper's
+implicit-function adjoint; checked against central differences on a sample
+of parameters and against the same adjoint over the CPU oracle.
 
 Workload "ccd": one device CCD call (continuous broad phase + rigid
 interval narrow phase, rigc_compute_earliest_toi) per step along a
@@ -407,6 +413,68 @@ def bench_intersect(args):
     return line
 
 
+# ------------------------------------------------------------------ gradient
+def grad_compare(fd_params=6, cpu=True):
+    """The differentiation workload: 64 ico1 spheres (r 0.3) dropped from a
+    jittered 4^3 lattice onto the plane, 20 steps at dt 0.01, mu 0.5; loss =
+    sum over spheres of |end centre - target|^2; gradient with respect to
+    all 384 initial linear and angular velocities by the stepper's
+    implicit-function adjoint (rigs_rollout_backward, device contact path).
+    Reports forward (checkpointed rollout) and backward times, the gradient's
+    agreement with central differences (h 1e-6) on fd_params parameters, and
+    the same adjoint over the CPU oracle (time and agreement)."""
+    from rigidipc_amd import diff as Dm
+    from rigidipc_amd import scenes as S
+    from rigidipc_amd import sim as M
+    sc = S.config4(0)
+    sim = M.Simulation(sc, params=Dm.params_for_gradients(sc))
+
+    def loss(v, s=sim):
+        s.set_state(sc.poses, v)
+        s.rollout(sc.steps)
+        x, _ = s.state()
+        d = x[:64, :3] - sc.targets
+        g = np.zeros_like(x)
+        g[:64, :3] = 2 * d
+        return float((d * d).sum()), g
+
+    loss(sc.velocities)  # warm-up
+    t0 = time.perf_counter()
+    L, gx = loss(sc.velocities)
+    t_fwd = time.perf_counter() - t0
+    st = sim.stats()
+    t0 = time.perf_counter()
+    g = sim.rollout_backward(gx)
+    t_bwd = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    picks = [(int(b), int(k)) for b, k in zip(rng.integers(0, 64, fd_params), rng.integers(0, 6, fd_params))]
+    fd = []
+    for b, k in picks:
+        vp, vm = sc.velocities.copy(), sc.velocities.copy()
+        vp[b, k] += 1e-6
+        vm[b, k] -= 1e-6
+        fd.append((loss(vp)[0] - loss(vm)[0]) / 2e-6)
+    ad = np.array([g[b, k] for b, k in picks])
+    fd = np.array(fd)
+    out = dict(workload="config4: 64 ico1 + plane, 20 steps, mu 0.5, 384 parameters (initial v, omega)",
+               loss=L, forward_ms=1e3 * t_fwd, backward_ms=1e3 * t_bwd, gradient_ms=1e3 * (t_fwd + t_bwd),
+               newton_iterations=int(st["newton_iterations"]), contacts_at_end=int(st["num_constraints"]),
+               fd_params=picks, fd_max_rel_err=float(np.abs(ad - fd).max() / np.abs(fd).max()))
+    sim.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from stepper_util import oracle_backend
+        orc = M.Simulation(sc, params=Dm.params_for_gradients(sc), backend=oracle_backend())
+        t0 = time.perf_counter()
+        Lo, gxo = loss(sc.velocities, orc)
+        go = orc.rollout_backward(gxo)
+        t_cpu = time.perf_counter() - t0
+        orc.close()
+        out["cpu_oracle"] = dict(gradient_ms=1e3 * t_cpu, cores=cpu_threads(),
+                                 max_rel_diff=float(np.abs(g - go).max() / np.abs(go).max()))
+    return out
+
+
 # ------------------------------------------------------------------ batch
 def batch_compare(n_scenes, steps, warm=True):
     """The batching workload: n_scenes independent scenes (body counts
@@ -454,6 +522,18 @@ def bench_batch(args):
                             **r))
 
 
+def bench_grad(args):
+    r = grad_compare(cpu=not args.no_cpu)
+    line = dict(metric=METRIC, value=1e3 / r["gradient_ms"], unit="gradients/s (384-parameter rollout adjoint)",
+                n_gpus=1, steps=20, warmup=1, ms_per_step=r["gradient_ms"] / 20, higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype="f64", data="synthetic", config=r)
+    if "cpu_oracle" in r:
+        line["cpu_baseline"] = dict(value=1e3 / r["cpu_oracle"]["gradient_ms"], unit="gradients/s",
+                                    cores=r["cpu_oracle"]["cores"], kind="port",
+                                    sample="one forward rollout and adjoint over the CPU oracle")
+    return line
+
+
 def bench_copy(args):
     """Calibration for the PMC traffic counters: device-to-device copies of a
     known size (reads and writes 256 MiB each per launch)."""
@@ -481,7 +561,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "plccd", "batch", "copy", "intersect"])
+    ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "plccd", "batch", "copy", "intersect", "grad"])
     ap.add_argument("--batch-steps", type=int, default=3,
                     help="steps of the batched-vs-plain comparison added to the sim line (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -509,12 +589,15 @@ def main():
         line = bench_plccd(args)
     elif args.workload == "intersect":
         line = bench_intersect(args)
+    elif args.workload == "grad":
+        line = bench_grad(args)
     else:
         line = bench_sim(args)
         if args.batch_steps > 0:
             line["batched"] = batch_compare(256, args.batch_steps)
             line["batched"]["note"] = (f"first {args.batch_steps} of the batching workload's 10 steps; "
                                        "bench.py --workload batch runs all 10")
+        line["gradient"] = grad_compare(cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -327,3 +327,65 @@ def config4(seed=0):
     sc.targets = np.array([b.pose[:3] for b in bodies[:-1]]) + rng.normal(0, 0.1, (64, 3))
     sc.name = "config4: 64 ico1 + plane"
     return sc
+
+
+def _wall(dims, centre, group):
+    V, F = cube(1.0)
+    return Body(V * np.asarray(dims, float), F, np.eye(3), np.asarray(centre, float), np.zeros(3), np.zeros(3),
+                static=True, group=group)
+
+
+def config5(seed=0, n_spheres=9000, n_tori=1000, box=10.0):
+    """Dense pile: 9,000 subdivision-3 icospheres (642 vertices, r 0.25) and
+    1,000 24x12 tori (R 0.5, r 0.15) in uniform random orientations, stacked
+    in a box x box column inside a box x box box with walls, above the
+    plane, falling at v = (0, -2, 0) + N(0, 0.1) m/s per component; dt 0.005,
+    dhat 1e-4, mu 0.5, 20 steps. Spheres sit in 0.52 m cells, tori in
+    1.35 m cells (a torus's extent is at most 1.3 m in any orientation),
+    jittered within them; layers of tori and layers of spheres alternate up
+    the column in proportion to their numbers (with the default sizes 21
+    torus layers and 25 sphere layers, about 42 m of the 60 m column). The
+    plane and the four walls (0.5 m thick, 62 m tall) are static and share
+    one group. 12.10M triangles with the defaults."""
+    rng = np.random.default_rng(seed)
+    Vs, Fs = icosphere(3, 0.25)
+    Vt, Ft = torus(0.5, 0.15)
+    half = box / 2.0
+    layers = []
+    for kind, n, cell in (("s", n_spheres, 0.52), ("t", n_tori, 1.35)):
+        per_side = int(box // cell)
+        per_layer = per_side * per_side
+        n_layers = -(-n // per_layer)
+        for k in range(n_layers):
+            count = min(per_layer, n - k * per_layer)
+            layers.append(((k + 0.5) / n_layers, kind, cell, per_side, count))
+    layers.sort(key=lambda t: (t[0], t[1]))
+    bodies = []
+    y = 0.05
+    for _, kind, cell, per_side, count in layers:
+        V, F = (Vs, Fs) if kind == "s" else (Vt, Ft)
+        slots = rng.permutation(per_side * per_side)[:count]
+        off = (box - per_side * cell) / 2.0
+        for slot in sorted(slots):
+            i, j = divmod(int(slot), per_side)
+            R = random_rotation(rng)
+            W = V @ R.T
+            lo, hi = W.min(0), W.max(0)
+            slack = np.maximum(cell - 0.01 - (hi - lo), 0.0) / 2.0
+            centre = np.array([-half + off + (i + 0.5) * cell, y + cell / 2.0, -half + off + (j + 0.5) * cell])
+            p = centre + rng.uniform(-1, 1, 3) * slack - (lo + hi) / 2.0
+            v = np.array([0.0, -2.0, 0.0]) + rng.normal(0.0, 0.1, 3)
+            bodies.append(Body(V, F, R, p, v, np.zeros(3)))
+        y += cell + 0.01
+    statics = len(bodies)
+    ground = _ground()
+    ground.group = statics
+    t = 0.5
+    h = 62.0
+    walls = [_wall((t, h, box + 2 * t), (s * (half + t / 2), h / 2, 0.0), statics) for s in (-1, 1)]
+    walls += [_wall((box + 2 * t, h, t), (0.0, h / 2, s * (half + t / 2)), statics) for s in (-1, 1)]
+    sc = assemble(bodies + [ground] + walls)
+    sc.dt, sc.dhat, sc.mu, sc.gravity, sc.steps = 0.005, 1e-4, 0.5, np.array([0, -9.81, 0]), 20
+    sc.column_height = y
+    sc.name = f"config5: {n_spheres} ico3 + {n_tori} tori in a walled {box:g} x {box:g} box"
+    return sc
