@@ -27,6 +27,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from gpu_util import PKG  # noqa: F401  (puts the package on sys.path)
 from rigidipc_amd import scenes as S
 from rigidipc_amd import sim as M
 
