@@ -1,5 +1,9 @@
 // ORACLE — test infrastructure only (see oracle_interval.hpp).
 // extern "C" entry points that the Python tests load with ctypes.
+#include <atomic>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstring>
 
 #include "oracle_ccd.hpp"
@@ -191,11 +195,42 @@ void oracle_last_vertex_aabbs(double* out) {
     std::copy(v.begin(), v.end(), out);
 }
 // DistanceBarrierConstraint::compute_earliest_toi
-// (src/opt/distance_barrier_constraint.cpp:130-265) with every query on
-// t in [0,1] and the minimum over queries, parallel over candidates; the
-// narrow phase per trajectory type as edge_edge_ccd / face_vertex_ccd
-// dispatch it (src/ccd/ccd.cpp:271-279, 339-347): 0 RIGID, 1 PIECEWISE_LINEAR
-// (with the caller's minimum separation).
+// (src/opt/distance_barrier_constraint.cpp:130-265), parallel over
+// candidates as the reference's tbb::parallel_for; the narrow phase per
+// trajectory type as edge_edge_ccd / face_vertex_ccd dispatch it
+// (src/ccd/ccd.cpp:271-279, 339-347): 0 RIGID, 1 PIECEWISE_LINEAR (with the
+// caller's minimum separation).
+//
+// As in the reference, the queries share the running earliest TOI
+// (:185-244), but deterministically: every rigid query searches t in [0,1]
+// and drops the boxes whose t.lo is at or above the running minimum
+// (interval_root_finder_nd), so the call returns the minimum over queries
+// of their [0,1] results, bit for bit, in any schedule (the reference
+// instead shrinks each query's t-domain to a racily read earliest_toi,
+// which makes its trees depend on the schedule).
+// oracle_set_ccd_pruning(0) turns the sharing off (each query runs its
+// whole tree); oracle_last_box_evaluations() gives the inclusion-function
+// evaluations of the last call, summed over its queries.
+namespace {
+bool g_ccd_pruning = true;
+long g_last_box_evaluations = 0;
+}
+void oracle_set_ccd_pruning(int on) { g_ccd_pruning = on != 0; }
+long oracle_last_box_evaluations() { return g_last_box_evaluations; }
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int oracle_max_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double msd, int trajectory,
                                       long* n_candidates) {
     const Scene& s = scene();
@@ -203,10 +238,13 @@ double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double
     broadphase_ccd(s, p0, p1, msd / 2.0, c);
     const long ne = (long)c.ee.size(), nf = (long)c.fv.size();
     *n_candidates = ne + nf;
-    double best = INFINITY;
-#pragma omp parallel for schedule(dynamic, 16) reduction(min : best)
+    std::atomic<double> shared(INFINITY);
+    const std::atomic<double>* prune = g_ccd_pruning ? &shared : nullptr;
+    long boxes = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : boxes)
     for (long i = 0; i < ne + nf; i++) {
         double toi = INFINITY;
+        long evals = 0;
         bool hit;
         if (i < ne) {
             const int ea = c.ee[i][0], eb = c.ee[i][1];
@@ -220,7 +258,8 @@ double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double
                 q.poseA_t0[k] = p0[6 * A + k]; q.poseA_t1[k] = p1[6 * A + k];
                 q.poseB_t0[k] = p0[6 * B + k]; q.poseB_t1[k] = p1[6 * B + k];
             }
-            hit = trajectory == 1 ? pl_edge_edge_toi(q, toi, 1.0, msd, nullptr, nullptr) : edge_edge_toi(q, toi);
+            hit = trajectory == 1 ? pl_edge_edge_toi(q, toi, 1.0, msd, nullptr, nullptr)
+                                  : edge_edge_toi(q, toi, 1.0, 1e-4, &evals, prune);
         } else {
             const int f = c.fv[i - ne][0], v = c.fv[i - ne][1];
             const int A = s.v_body[v], B = s.f_body[f];
@@ -234,11 +273,18 @@ double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double
                 q.poseA_t0[k] = p0[6 * A + k]; q.poseA_t1[k] = p1[6 * A + k];
                 q.poseB_t0[k] = p0[6 * B + k]; q.poseB_t1[k] = p1[6 * B + k];
             }
-            hit = trajectory == 1 ? pl_face_vertex_toi(q, toi, 1.0, msd, nullptr, nullptr) : face_vertex_toi(q, toi);
+            hit = trajectory == 1 ? pl_face_vertex_toi(q, toi, 1.0, msd, nullptr, nullptr)
+                                  : face_vertex_toi(q, toi, 1.0, 1e-4, &evals, prune);
         }
-        if (hit && toi < best) best = toi;
+        boxes += evals;
+        if (hit) {
+            double cur = shared.load(std::memory_order_relaxed);
+            while (toi < cur && !shared.compare_exchange_weak(cur, toi, std::memory_order_relaxed)) {
+            }
+        }
     }
-    return best;
+    g_last_box_evaluations = boxes;
+    return shared.load();
 }
 double oracle_earliest_toi(const double* p0, const double* p1, double msd, long* n_candidates) {
     return oracle_earliest_toi_trajectory(p0, p1, msd, 0, n_candidates);

@@ -138,7 +138,8 @@ Vec3I face_vertex_aabb(const FVQuery& q, const Interval& t, const Interval& u, c
 }
 
 // time_of_impact.cpp:138-217
-bool edge_edge_toi(const EEQuery& q, double& toi, double earliest_toi, double toi_tol, long* n_evals) {
+bool edge_edge_toi(const EEQuery& q, double& toi, double earliest_toi, double toi_tol, long* n_evals,
+                   const std::atomic<double>* shared_min) {
     double tol[3] = { toi_tol, 1e-4 / edge_length(q.a0, q.a1), 1e-4 / edge_length(q.b0, q.b1) };
     Interval x0[3] = { Interval(0, earliest_toi), Interval(0, 1), Interval(0, 1) };
     Interval x[3];
@@ -148,14 +149,15 @@ bool edge_edge_toi(const EEQuery& q, double& toi, double earliest_toi, double to
             for (int i = 0; i < 3; i++) y[i] = d[i];
         },
         [](const Interval*) { return true; }, [](const Interval*) { return true; }, x0, tol, x,
-        n_evals);
+        n_evals, shared_min);
     toi = hit ? x[0].lo : INFINITY;
     return hit;
 }
 
 // time_of_impact.cpp:222-310. The face's u runs along face edge 0 (f0,f1)
 // and v along face edge 1 (f1,f2), whose lengths set the tolerances.
-bool face_vertex_toi(const FVQuery& q, double& toi, double earliest_toi, double toi_tol, long* n_evals) {
+bool face_vertex_toi(const FVQuery& q, double& toi, double earliest_toi, double toi_tol, long* n_evals,
+                     const std::atomic<double>* shared_min) {
     double tol[3] = { toi_tol, 1e-4 / edge_length(q.f0, q.f1), 1e-4 / edge_length(q.f1, q.f2) };
     Interval x0[3] = { Interval(0, earliest_toi), Interval(0, 1), Interval(0, 1) };
     Interval x[3];
@@ -165,7 +167,7 @@ bool face_vertex_toi(const FVQuery& q, double& toi, double earliest_toi, double 
             for (int i = 0; i < 3; i++) y[i] = d[i];
         },
         [](const Interval* p) { return overlap(p[1] + p[2], Interval(0, 1)); },
-        [](const Interval*) { return true; }, x0, tol, x, n_evals);
+        [](const Interval*) { return true; }, x0, tol, x, n_evals, shared_min);
     toi = hit ? x[0].lo : INFINITY;
     return hit;
 }

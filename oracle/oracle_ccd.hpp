@@ -16,6 +16,8 @@
 // left to right ((a0 + a1) + a2). No FMA contraction.
 #pragma once
 
+#include <atomic>
+
 #include "oracle_interval.hpp"
 
 #include <vector>
@@ -50,11 +52,15 @@ struct FVQuery {
 
 // compute_edge_edge_time_of_impact / compute_face_vertex_time_of_impact
 // with the t-domain [0, earliest_toi]. Returns is_impacting; toi = +inf when
-// not impacting. n_evals (optional) counts inclusion-function evaluations.
+// not impacting. n_evals (optional) counts inclusion-function evaluations;
+// shared_min (optional) is the running minimum of an earliest-TOI call
+// (interval_root_finder_nd).
 bool edge_edge_toi(const EEQuery& q, double& toi, double earliest_toi = 1.0,
-                   double toi_tol = 1e-4, long* n_evals = nullptr);
+                   double toi_tol = 1e-4, long* n_evals = nullptr,
+                   const std::atomic<double>* shared_min = nullptr);
 bool face_vertex_toi(const FVQuery& q, double& toi, double earliest_toi = 1.0,
-                     double toi_tol = 1e-4, long* n_evals = nullptr);
+                     double toi_tol = 1e-4, long* n_evals = nullptr,
+                     const std::atomic<double>* shared_min = nullptr);
 // Piecewise-linear trajectories (oracle_plccd.cpp).
 bool pl_edge_edge_toi(const EEQuery& q, double& toi, double earliest_toi, double msd, long* n_pieces,
                       long* n_boxes);

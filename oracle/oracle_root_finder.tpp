@@ -2,6 +2,7 @@
 // interval_root_finder, src/interval/interval_root_finder.cpp:97-203.
 #pragma once
 
+#include <atomic>
 #include <stack>
 #include <vector>
 
@@ -16,10 +17,18 @@ inline bool& budget_exceeded() { static thread_local bool e = false; return e; }
 
 // n-dimensional version. f maps n intervals to n (or 3) intervals via out;
 // is_domain_valid filters boxes; constraint_predicate accepts roots.
+//
+// shared_min (optional): the running minimum of the TOIs that other queries
+// of the same earliest-TOI call have found. Boxes whose t.lo is at or above
+// it are dropped, as the query's own earliest root drops them. A child's
+// t.lo is never below its parent's, so the dropped subtrees hold no root
+// below the bound: the minimum over the call's queries is the same, bit for
+// bit, whatever the order in which they run (only queries whose own root
+// lies at or above the bound report no root).
 template <int N, typename F, typename V, typename P>
 bool interval_root_finder_nd(F f, V is_domain_valid, P constraint_predicate,
                              const Interval (&x0)[N], double (&tol)[N], Interval (&x_out)[N],
-                             long* n_evals = nullptr) {
+                             long* n_evals = nullptr, const std::atomic<double>* shared_min = nullptr) {
     struct Box { Interval x[N]; };
     Box earliest_root;
     for (int i = 0; i < N; i++) earliest_root.x[i] = Interval(INFINITY);
@@ -53,6 +62,7 @@ bool interval_root_finder_nd(F f, V is_domain_valid, P constraint_predicate,
         xs.pop();
 
         if (x.x[0].lo >= earliest_root.x[0].lo) continue;
+        if (shared_min && x.x[0].lo >= shared_min->load(std::memory_order_relaxed)) continue;
         if (!is_domain_valid(x.x)) continue;
 
         Interval y[N];

@@ -11,8 +11,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+# The -O3 build of the same sources (the CPU baseline's timing build).
+ORACLE_O3_SO = os.path.join(ORACLE_DIR, "_build", "liboracle_o3.so")
 
 _lib = None
+_lib_o3 = None
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_long_p = ctypes.POINTER(ctypes.c_long)
@@ -36,52 +39,88 @@ def lib():
     """Load the oracle, building it first if it is not there."""
     global _lib
     if _lib is None:
-        if not os.path.exists(ORACLE_SO):
-            subprocess.check_call(["make", "-C", ORACLE_DIR, "-j8"])
-        _lib = ctypes.CDLL(ORACLE_SO)
-        L = _lib
-        for name in ("oracle_interval_cos", "oracle_interval_sin", "oracle_interval_sinc"):
-            getattr(L, name).argtypes = [ctypes.c_double, ctypes.c_double, c_double_p]
-        for name in ("oracle_sinc", "oracle_cos_rn", "oracle_sin_rn"):
-            getattr(L, name).argtypes = [ctypes.c_double]
-            getattr(L, name).restype = ctypes.c_double
-        L.oracle_interval_binop.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4 + [c_double_p]
-        L.oracle_rotation_interval.argtypes = [c_double_p, c_double_p]
-        L.oracle_rotation_double.argtypes = [c_double_p, c_double_p]
-        L.oracle_root_quadratic.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_double, c_double_p]
-        L.oracle_root_quadratic.restype = ctypes.c_int
-        L.oracle_root_trig.argtypes = [ctypes.c_double] * 3 + [c_double_p]
-        L.oracle_root_trig.restype = ctypes.c_int
-        for name in ("oracle_ee_toi", "oracle_fv_toi"):
-            getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p]
-            getattr(L, name).restype = ctypes.c_int
-        for name in ("oracle_pl_ee_toi", "oracle_pl_fv_toi"):
-            getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p,
-                                         c_long_p]
-            getattr(L, name).restype = ctypes.c_int
-        for name in ("oracle_ti_edge_edge", "oracle_ti_vertex_face"):
-            getattr(L, name).argtypes = [c_double_p, ctypes.c_double, c_double_p, c_double_p, c_long_p]
-            getattr(L, name).restype = ctypes.c_int
-        L.oracle_element_distance_sq.argtypes = [ctypes.c_int, c_double_p]
-        L.oracle_element_distance_sq.restype = ctypes.c_double
-        L.oracle_set_eval_budget.argtypes = [ctypes.c_long]
-        L.oracle_set_scene.argtypes = [ctypes.c_int] * 4 + [c_double_p] + [c_int32_p] * 7 + [ctypes.c_void_p]
-        L.oracle_orient3d.argtypes = [c_double_p] * 4
-        L.oracle_orient3d.restype = ctypes.c_int
-        L.oracle_edge_triangle.argtypes = [c_double_p] * 5
-        L.oracle_edge_triangle.restype = ctypes.c_int
-        L.oracle_detect_intersections.argtypes = [c_double_p, c_int32_p, ctypes.c_long]
-        L.oracle_detect_intersections.restype = ctypes.c_long
-        L.oracle_broadphase.argtypes = [ctypes.c_int, c_double_p, c_double_p, ctypes.c_double, c_long_p, c_long_p]
-        L.oracle_get_candidates.argtypes = [c_int32_p, c_int32_p]
-        L.oracle_average_edge_length.restype = ctypes.c_double
-        L.oracle_world_vertices.argtypes = [c_double_p, c_double_p]
-        L.oracle_last_vertex_aabbs.argtypes = [c_double_p]
-        L.oracle_earliest_toi.argtypes = [c_double_p, c_double_p, ctypes.c_double, c_long_p]
-        L.oracle_earliest_toi.restype = ctypes.c_double
-        for name in ("oracle_ee_aabb", "oracle_fv_aabb"):
-            getattr(L, name).argtypes = [c_double_p, c_double_p, c_double_p]
+        _lib = _load(ORACLE_SO)
     return _lib
+
+
+def lib_o3():
+    """The -O3 timing build: a second, separate instance of the oracle (its
+    own scene and backend state)."""
+    global _lib_o3
+    if _lib_o3 is None:
+        _lib_o3 = _load(ORACLE_O3_SO)
+    return _lib_o3
+
+
+def _load(path):
+    if not os.path.exists(path):
+        subprocess.check_call(["make", "-C", ORACLE_DIR, "-j8"])
+    L = ctypes.CDLL(path)
+    for name in ("oracle_interval_cos", "oracle_interval_sin", "oracle_interval_sinc"):
+        getattr(L, name).argtypes = [ctypes.c_double, ctypes.c_double, c_double_p]
+    for name in ("oracle_sinc", "oracle_cos_rn", "oracle_sin_rn"):
+        getattr(L, name).argtypes = [ctypes.c_double]
+        getattr(L, name).restype = ctypes.c_double
+    L.oracle_interval_binop.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4 + [c_double_p]
+    L.oracle_rotation_interval.argtypes = [c_double_p, c_double_p]
+    L.oracle_rotation_double.argtypes = [c_double_p, c_double_p]
+    L.oracle_root_quadratic.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_double, c_double_p]
+    L.oracle_root_quadratic.restype = ctypes.c_int
+    L.oracle_root_trig.argtypes = [ctypes.c_double] * 3 + [c_double_p]
+    L.oracle_root_trig.restype = ctypes.c_int
+    for name in ("oracle_ee_toi", "oracle_fv_toi"):
+        getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p]
+        getattr(L, name).restype = ctypes.c_int
+    for name in ("oracle_pl_ee_toi", "oracle_pl_fv_toi"):
+        getattr(L, name).argtypes = [c_double_p, ctypes.c_double, ctypes.c_double, c_double_p, c_long_p,
+                                     c_long_p]
+        getattr(L, name).restype = ctypes.c_int
+    for name in ("oracle_ti_edge_edge", "oracle_ti_vertex_face"):
+        getattr(L, name).argtypes = [c_double_p, ctypes.c_double, c_double_p, c_double_p, c_long_p]
+        getattr(L, name).restype = ctypes.c_int
+    L.oracle_element_distance_sq.argtypes = [ctypes.c_int, c_double_p]
+    L.oracle_element_distance_sq.restype = ctypes.c_double
+    L.oracle_set_eval_budget.argtypes = [ctypes.c_long]
+    L.oracle_set_scene.argtypes = [ctypes.c_int] * 4 + [c_double_p] + [c_int32_p] * 7 + [ctypes.c_void_p]
+    L.oracle_orient3d.argtypes = [c_double_p] * 4
+    L.oracle_orient3d.restype = ctypes.c_int
+    L.oracle_edge_triangle.argtypes = [c_double_p] * 5
+    L.oracle_edge_triangle.restype = ctypes.c_int
+    L.oracle_detect_intersections.argtypes = [c_double_p, c_int32_p, ctypes.c_long]
+    L.oracle_detect_intersections.restype = ctypes.c_long
+    L.oracle_broadphase.argtypes = [ctypes.c_int, c_double_p, c_double_p, ctypes.c_double, c_long_p, c_long_p]
+    L.oracle_get_candidates.argtypes = [c_int32_p, c_int32_p]
+    L.oracle_average_edge_length.restype = ctypes.c_double
+    L.oracle_world_vertices.argtypes = [c_double_p, c_double_p]
+    L.oracle_last_vertex_aabbs.argtypes = [c_double_p]
+    L.oracle_earliest_toi.argtypes = [c_double_p, c_double_p, ctypes.c_double, c_long_p]
+    L.oracle_earliest_toi.restype = ctypes.c_double
+    for name in ("oracle_ee_aabb", "oracle_fv_aabb"):
+        getattr(L, name).argtypes = [c_double_p, c_double_p, c_double_p]
+    L.oracle_set_ccd_pruning.argtypes = [ctypes.c_int]
+    L.oracle_last_box_evaluations.restype = ctypes.c_long
+    L.oracle_set_num_threads.argtypes = [ctypes.c_int]
+    L.oracle_max_threads.restype = ctypes.c_int
+    return L
+
+
+def set_ccd_pruning(on, L=None):
+    """Earliest-TOI queries share the running minimum (on, the default) or
+    each runs its whole tree (off)."""
+    (L or lib()).oracle_set_ccd_pruning(int(bool(on)))
+
+
+def last_box_evaluations(L=None):
+    """Inclusion-function evaluations of the last earliest-TOI call."""
+    return (L or lib()).oracle_last_box_evaluations()
+
+
+def set_num_threads(n, L=None):
+    (L or lib()).oracle_set_num_threads(int(n))
+
+
+def max_threads(L=None):
+    return (L or lib()).oracle_max_threads()
 
 
 def interval_fn(name, lo, hi):

@@ -10,8 +10,10 @@ from rigidipc_amd import scenes as S
 from rigidipc_amd import sim as M
 
 
-def oracle_backend():
-    L = O.lib()
+def oracle_backend(L=None):
+    """The oracle's contact-backend table (of the library L, by default the
+    oracle's parity build)."""
+    L = L or O.lib()
     L.oracle_backend.argtypes = [ctypes.POINTER(M.Backend)]
     L.oracle_backend.restype = None
     be = M.Backend()
