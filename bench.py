@@ -9,9 +9,10 @@ over the device contact path (librigc.so: CCD in every line search,
 constraint set, barrier and friction assembly, minimum distance). W
 warm-up steps run on a separate simulation of the same scene; the K timed
 steps start from the initial state. value = K / wall time of the K steps.
-The CPU baseline runs the same stepper over the oracle (the CPU
-restatement, OpenMP over the host cores) from the same initial state for a
-bounded sample of steps.
+The CPU baseline replays a uniform sample of the run's contact calls (over
+all K steps) on the -O3 oracle, the CPU restatement, at all host cores
+and at one (see cpu_estimate); "config1" runs the 10-body scene's 100
+steps outright on the device and on the oracle.
 
 The sim line also carries "batched": the batching workload (256
 independent scenes of 8-512 bodies) advanced --batch-steps steps with the
@@ -58,6 +59,25 @@ FLOPS_PER_BOX = 1015.7
 # 256 MiB device copy as 256 MiB read and 256 MiB written.
 CCD_TRAFFIC = (812.0 * 2.0 + 2185.6) * 1024   # ballistic CCD sequence
 SIM_CCD_TRAFFIC = (223.9 * 2.0 + 2126.3) * 1024  # time stepping, first 3 steps
+HBM_PEAK_GBS = 8000.0
+# Algorithmic FP64 work of one barrier constraint's Hessian block: the
+# vertex-space value / gradient / Hessian with the rotation Jacobians
+# (about 2,000, counted from the kernel's code), J^T H J over 12 dofs
+# (2 x 12^3 x 2 = 6,912) and a 12 x 12 symmetric eigendecomposition with
+# vectors at the textbook 9 n^3 = 15,552 (tridiagonal QR; the device's
+# parallel Jacobi does more).
+ASSEMBLY_FLOP_PER_CONSTRAINT = 2000 + 6912 + 9 * 12 ** 3
+
+
+def broad_phase_bytes(sc, calls, candidates):
+    """Algorithmic HBM bytes of `calls` continuous broad phases of sc that
+    emitted `candidates` pairs in all: per call 96 B of poses per body,
+    the mesh (24 B a vertex, 8 an edge, 12 a face), each element's box
+    written and read once (96 B) and keyed into the grid (24 B); 8 B per
+    candidate pair written."""
+    nb, nv, ne, nf = len(sc.group_ids), len(sc.vertices), len(sc.edges), len(sc.faces)
+    per_call = 96 * nb + 24 * nv + 8 * ne + 12 * nf + (96 + 24) * (nv + ne + nf)
+    return calls * per_call + 8 * candidates
 
 
 def log(msg):
@@ -99,70 +119,420 @@ def cpu_threads():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
-# ------------------------------------------------------------------ sim
-def cpu_baseline_child(budget_s):
-    """Child process: the same stepper over the oracle backend, from the
-    initial state. A polling thread prints the Newton iterations completed
-    so far every half second (the stepper runs in C without the GIL)."""
-    import threading
+def host_cpu():
+    """nproc, the usable cores, and lscpu's model name and clocks."""
+    import subprocess
+    info = {"nproc": None, "affinity": len(os.sched_getaffinity(0)), "model": None, "mhz_max": None}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "CPU max MHz":
+                info["mhz_max"] = float(v)
+            elif k.strip() == "Socket(s)":
+                info["sockets"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+# ------------------------------------------------------- sampled CPU replay
+# The CPU baseline of a time-stepping run: the device run is repeated with
+# a contact backend that forwards every call to the device and keeps a
+# uniform sample of each entry point's calls (with the constraint set and
+# lagged friction set in force at the call); the sampled calls are then
+# replayed on the -O3 build of the oracle (the CPU restatement, OpenMP
+# over the host cores), each timed alone, at all cores and, on a subset of
+# the same calls, at one core. CPU time of the run = sum over entry points
+# of (calls x mean CPU time per call) + the stepper's own host time.
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp, _d, _i32 = ctypes.c_void_p, ctypes.c_double, ctypes.c_int32
+BACKEND_PROTO = {
+    "compute_earliest_toi": ctypes.CFUNCTYPE(ctypes.c_int, _vp, _dp, _dp, _d, _dp),
+    "construct_constraint_set": ctypes.CFUNCTYPE(ctypes.c_int, _vp, _dp, _d, _d, _i64p),
+    "compute_barrier_term": ctypes.CFUNCTYPE(ctypes.c_int, _vp, _dp, _d, _d, _i32, _dp, _dp, _vp),
+    "compute_minimum_distance": ctypes.CFUNCTYPE(ctypes.c_int, _vp, _dp, _d, _d, _i32, _dp),
+    "update_friction_constraints": ctypes.CFUNCTYPE(ctypes.c_int, _vp, _dp, _d, _d, _d, _d, _i32, _i64p),
+    "compute_friction_term": ctypes.CFUNCTYPE(ctypes.c_int, _vp, _dp, _dp, _d, _i32, _dp, _dp, _vp),
+}
+RIGC_BUILD_CONSTRAINTS, RIGC_HESSIAN, RIGC_ERR_CAPACITY = 4, 2, 4
+
+
+def device_backend(ctx):
+    """The device library's entry points as a contact-backend table over
+    the rigidipc_amd.Context ctx (the table rigs_create builds itself)."""
+    from rigidipc_amd import sim as M
+    L = ctx._L
+    be = M.Backend()
+    be.self = ctx._ctx.value
+    for member, fn in (("set_scene", "rigc_set_scene"), ("compute_earliest_toi", "rigc_compute_earliest_toi"),
+                       ("construct_constraint_set", "rigc_construct_constraint_set"),
+                       ("compute_barrier_term", "rigc_compute_barrier_term"),
+                       ("compute_minimum_distance", "rigc_compute_minimum_distance"),
+                       ("update_friction_constraints", "rigc_update_friction_constraints"),
+                       ("compute_friction_term", "rigc_compute_friction_term"), ("last_error", "rigc_last_error"),
+                       ("get_stats", "rigc_get_stats")):
+        setattr(be, member, ctypes.cast(getattr(L, fn), ctypes.c_void_p).value)
+    return be
+
+
+class CallRecorder:
+    """A contact backend that forwards every call to `inner` and keeps, per
+    entry point, a uniform sample (reservoir of k, fixed seed) of its calls
+    with what a replay needs."""
+
+    def __init__(self, inner, num_bodies, k=64, seed=0):
+        import random
+        from rigidipc_amd import sim as M
+        self.n = 6 * num_bodies
+        self.k = k
+        self.rng = random.Random(seed)
+        self.count = {name: 0 for name in BACKEND_PROTO}
+        self.sample = {name: [] for name in BACKEND_PROTO}
+        self.hessian_calls = {"calls": 0, "constraints": 0, "assembly_ms": 0.0}
+        self.build = None  # (poses, dhat, dmin) of the constraint set in force
+        self.fric = None   # (build, poses, dhat, dmin, kappa, mu, flags) of the lagged friction set
+        self.table = M.Backend()
+        ctypes.memmove(ctypes.byref(self.table), ctypes.byref(inner), ctypes.sizeof(M.Backend))
+        self.table.detect_intersections = None
+        self.table.compute_earliest_toi_piecewise_linear = None
+        self.table.friction_adjoint = None
+        from rigidipc_amd import Stats
+        self._stats_t = Stats
+        self._stats = (ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.POINTER(Stats))(inner.get_stats)
+                       if inner.get_stats else None)
+        self._callbacks = []
+        for name, proto in BACKEND_PROTO.items():
+            fwd = proto(getattr(inner, name))
+            cb = proto(getattr(self, "_" + name)(fwd))
+            self._callbacks.append(cb)
+            setattr(self.table, name, ctypes.cast(cb, ctypes.c_void_p).value)
+
+    def _p(self, ptr):
+        return np.ctypeslib.as_array(ptr, shape=(self.n,)).copy()
+
+    def _keep(self, name, make):
+        self.count[name] += 1
+        s = self.sample[name]
+        if len(s) < self.k:
+            s.append(make())
+        else:
+            j = self.rng.randrange(self.count[name])
+            if j < self.k:
+                s[j] = make()
+
+    def _compute_earliest_toi(self, fwd):
+        def cb(h, p0, p1, msd, out):
+            kept = []
+
+            def make():
+                kept.append([self._p(p0), self._p(p1), msd, None])
+                return kept[0]
+            self._keep("compute_earliest_toi", make)
+            st = fwd(h, p0, p1, msd, out)
+            if kept and self._stats is not None:
+                # the device's inclusion-function evaluations of this call
+                s = self._stats_t()
+                if self._stats(h, ctypes.byref(s)) == 0:
+                    kept[0][3] = int(s.box_evaluations)
+            return st
+        return cb
+
+    def _construct_constraint_set(self, fwd):
+        def cb(h, p, dhat, dmin, counts):
+            self.build = (self._p(p), dhat, dmin)
+            self._keep("construct_constraint_set", lambda: self.build)
+            return fwd(h, p, dhat, dmin, counts)
+        return cb
+
+    def _with_build(self, name, p, dhat, dmin, flags, rest):
+        before = None if flags & RIGC_BUILD_CONSTRAINTS else self.build
+        item = [None]
+
+        def make():
+            item[0] = (before, self._p(p), dhat, dmin, flags) + rest
+            return item[0]
+        self._keep(name, make)
+        if flags & RIGC_BUILD_CONSTRAINTS:
+            self.build = (self._p(p), dhat, dmin)
+
+    def _compute_barrier_term(self, fwd):
+        def cb(h, p, dhat, dmin, flags, e, g, hess):
+            self._with_build("compute_barrier_term", p, dhat, dmin, flags, ())
+            st = fwd(h, p, dhat, dmin, flags, e, g, hess)
+            if flags & RIGC_HESSIAN and self._stats is not None:
+                s = self._stats_t()
+                if self._stats(h, ctypes.byref(s)) == 0:
+                    self.hessian_calls["calls"] += 1
+                    self.hessian_calls["constraints"] += int(s.num_constraints)
+                    self.hessian_calls["assembly_ms"] += float(s.assembly_ms)
+            return st
+        return cb
+
+    def _compute_minimum_distance(self, fwd):
+        def cb(h, p, dhat, dmin, flags, out):
+            self._with_build("compute_minimum_distance", p, dhat, dmin, flags, ())
+            return fwd(h, p, dhat, dmin, flags, out)
+        return cb
+
+    def _update_friction_constraints(self, fwd):
+        def cb(h, p, dhat, dmin, kappa, mu, flags, count):
+            before = None if flags & RIGC_BUILD_CONSTRAINTS else self.build
+            self.fric = (before, self._p(p), dhat, dmin, kappa, mu, flags)
+            if flags & RIGC_BUILD_CONSTRAINTS:
+                self.build = (self._p(p), dhat, dmin)
+            self._keep("update_friction_constraints", lambda: self.fric)
+            return fwd(h, p, dhat, dmin, kappa, mu, flags, count)
+        return cb
+
+    def _compute_friction_term(self, fwd):
+        def cb(h, p, p0, eh, flags, e, g, hess):
+            self._keep("compute_friction_term", lambda: (self.fric, self._p(p), self._p(p0), eh, flags))
+            return fwd(h, p, p0, eh, flags, e, g, hess)
+        return cb
+
+
+class Replayer:
+    """Replays recorded calls on a contact-backend table (the oracle's),
+    timing each call alone; its set-up (the constraint set and lagged
+    friction set in force at the call) is not timed."""
+
+    def __init__(self, table, num_bodies):
+        from rigidipc_amd import BlockHessian
+        self.t = table
+        self.nb = num_bodies
+        self.fn = {name: proto(getattr(table, name)) for name, proto in BACKEND_PROTO.items()}
+        self.grad = np.zeros(6 * num_bodies)
+        self.diag = np.zeros(36 * num_bodies)
+        self.cap = 64 * num_bodies
+        self._alloc()
+        self.BlockHessian = BlockHessian
+
+    def _alloc(self):
+        self.pairs = np.zeros(2 * self.cap, np.int32)
+        self.off = np.zeros(36 * self.cap)
+
+    def _hess(self):
+        from rigidipc_amd import _p, _c_double_p, _c_int32_p
+        return self.BlockHessian(_p(self.diag, _c_double_p), _p(self.pairs, _c_int32_p), _p(self.off, _c_double_p),
+                                 self.cap, 0)
+
+    def _check(self, st, what):
+        if st != 0:
+            raise RuntimeError(f"oracle {what} failed ({st})")
+
+    def _build(self, b):
+        counts = np.zeros(4, np.int64)
+        p, dhat, dmin = b
+        self._check(self.fn["construct_constraint_set"](self.t.self, p.ctypes.data_as(_dp), dhat, dmin,
+                                                        counts.ctypes.data_as(_i64p)), "constraint set")
+
+    def _fupdate(self, f):
+        before, p, dhat, dmin, kappa, mu, flags = f
+        if before is not None:
+            self._build(before)
+        n = ctypes.c_int64()
+        self._check(self.fn["update_friction_constraints"](self.t.self, p.ctypes.data_as(_dp), dhat, dmin, kappa, mu,
+                                                           flags, ctypes.byref(n)), "friction update")
+
+    def run(self, name, item):
+        """Seconds of one replayed call."""
+        e = ctypes.c_double()
+        g = self.grad.ctypes.data_as(_dp)
+        for _ in range(3):
+            if name == "compute_earliest_toi":
+                p0, p1, msd = item[:3]
+                out = ctypes.c_double()
+                t0 = time.perf_counter()
+                st = self.fn[name](self.t.self, p0.ctypes.data_as(_dp), p1.ctypes.data_as(_dp), msd, ctypes.byref(out))
+                self.last_toi = out.value
+            elif name == "construct_constraint_set":
+                t0 = time.perf_counter()
+                self._build(item)
+                st = 0
+            elif name == "update_friction_constraints":
+                if item[0] is not None:
+                    self._build(item[0])
+                t0 = time.perf_counter()
+                self._fupdate((None,) + item[1:])
+                st = 0
+            elif name in ("compute_barrier_term", "compute_minimum_distance"):
+                before, p, dhat, dmin, flags = item
+                if before is not None:
+                    self._build(before)
+                pp = p.ctypes.data_as(_dp)
+                if name == "compute_minimum_distance":
+                    t0 = time.perf_counter()
+                    st = self.fn[name](self.t.self, pp, dhat, dmin, flags, ctypes.byref(e))
+                else:
+                    h = self._hess()
+                    t0 = time.perf_counter()
+                    st = self.fn[name](self.t.self, pp, dhat, dmin, flags, ctypes.byref(e), g,
+                                       ctypes.byref(h) if flags & RIGC_HESSIAN else None)
+            else:
+                fric, p, p0, eh, flags = item
+                self._fupdate(fric)
+                h = self._hess()
+                t0 = time.perf_counter()
+                st = self.fn[name](self.t.self, p.ctypes.data_as(_dp), p0.ctypes.data_as(_dp), eh, flags,
+                                   ctypes.byref(e), g, ctypes.byref(h) if flags & RIGC_HESSIAN else None)
+            dt = time.perf_counter() - t0
+            if st == RIGC_ERR_CAPACITY and name in ("compute_barrier_term", "compute_friction_term"):
+                self.cap = 2 * max(self.cap, int(h.count))
+                self._alloc()
+                continue
+            self._check(st, name)
+            return dt
+        raise RuntimeError(f"oracle {name}: Hessian capacity")
+
+
+def record_device_run(sc, steps, params=None, k=64):
+    """The device run repeated through a CallRecorder: (recorder, stepper
+    stats)."""
+    import rigidipc_amd as R
+    from rigidipc_amd import sim as M
+    ctx = R.Context()
+    rec = CallRecorder(device_backend(ctx), len(sc.group_ids), k=k)
+    sim = M.Simulation(sc, params=params, backend=ctypes.pointer(rec.table))
+    for k in range(steps):
+        sim.step(1)
+        if steps <= 20 or (k + 1) % 10 == 0:
+            log(f"recording run: step {k + 1}/{steps}")
+    st = sim.stats()
+    sim.close()
+    ctx.close()
+    return rec, st
+
+
+def replay_cpu(sc, rec, budget_s, params=None):
+    """Replays rec's samples on the -O3 oracle. All cores: per entry point,
+    samples in order until its share of budget_s is spent (at least 2);
+    one core: the first of those same calls until half that share is spent
+    (at least 1), giving per entry point the one-core / all-core ratio on
+    identical calls. Returns per entry point (calls, mean all-core seconds,
+    samples timed, one-core ratio, samples timed at one core)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    from rigidipc_amd import sim as M
+    L3 = O.lib_o3()
+    L3.oracle_backend.argtypes = [ctypes.POINTER(M.Backend)]
+    L3.oracle_backend.restype = None
+    table = M.Backend()
+    L3.oracle_backend(ctypes.byref(table))
+    holder = M.Simulation(sc, params=params, backend=ctypes.pointer(table))  # sets the oracle's scene
+    rp = Replayer(table, len(sc.group_ids))
+    names = [n for n in BACKEND_PROTO if rec.count[n] > 0]
+    share = budget_s / max(len(names), 1)
+    all_threads = O.max_threads(L3)
+    out = {}
+    boxes = {"oracle_tree": 0, "device_executed": 0, "calls": 0}
+    for name in names:
+        times = []
+        t_spent = 0.0
+        for item in rec.sample[name]:
+            if len(times) >= 2 and t_spent >= share:
+                break
+            times.append(rp.run(name, item))
+            t_spent += times[-1]
+            if name == "compute_earliest_toi" and item[3] is not None:
+                boxes["oracle_tree"] += O.last_box_evaluations(L3)
+                boxes["device_executed"] += item[3]
+                boxes["calls"] += 1
+        one = []
+        O.set_num_threads(1, L3)
+        try:
+            t_spent = 0.0
+            for item in rec.sample[name][:len(times)]:
+                if one and t_spent >= share / 2:
+                    break
+                one.append(rp.run(name, item))
+                t_spent += one[-1]
+        finally:
+            O.set_num_threads(all_threads, L3)
+        out[name] = dict(calls=rec.count[name], mean_s=float(np.mean(times)), timed=len(times),
+                         one_core_ratio=float(sum(one) / max(sum(times[:len(one)]), 1e-12)), timed_one_core=len(one))
+        log(f"cpu replay {name}: {rec.count[name]} calls, {len(times)} timed, mean {1e3 * out[name]['mean_s']:.2f} ms "
+            f"({all_threads} threads), x{out[name]['one_core_ratio']:.2f} at one core over {len(one)}")
+    holder.close()
+    out["_boxes"] = boxes
+    return out, all_threads
+
+
+def cpu_estimate(sc, steps, budget_s, params=None, k=64, recorded=None):
+    """Sampled-replay CPU baseline of `steps` steps of sc from its initial
+    state: dict with all-core and one-core time steps/s and the details.
+    recorded: a (recorder, stats) pair from record_device_run of the same
+    run, else the run is recorded here."""
+    rec, st = recorded if recorded is not None else record_device_run(sc, steps, params, k)
+    per, threads = replay_cpu(sc, rec, budget_s, params)
+    boxes = per.pop("_boxes")
+    host_s = (st["total_ms"] - st["contact_ms"]) * 1e-3
+    contact_all = sum(v["calls"] * v["mean_s"] for v in per.values())
+    contact_one = sum(v["calls"] * v["mean_s"] * v["one_core_ratio"] for v in per.values())
+    return dict(
+        all_cores=steps / (contact_all + host_s), one_core=steps / (contact_one + host_s), threads=threads,
+        contact_s_all_cores=contact_all, contact_s_one_core=contact_one, host_s=host_s,
+        calls={n: v["calls"] for n, v in per.items()},
+        cpu_ms_per_call={n: round(1e3 * v["mean_s"], 3) for n, v in per.items()},
+        samples_timed={n: v["timed"] for n, v in per.items()},
+        one_core_ratio={n: round(v["one_core_ratio"], 2) for n, v in per.items()},
+        samples_timed_one_core={n: v["timed_one_core"] for n, v in per.items()},
+        newton_iterations=int(st["newton_iterations"]),
+        ccd_box_evaluations=dict(boxes, note="over the CCD calls timed: inclusion-function evaluations of the "
+                                             "oracle's trees (queries sharing the running minimum) and executed "
+                                             "by the device kernel (speculative evaluations included)"))
+
+
+def config1_outright(steps=100, no_cpu=False):
+    """Config 1 (10 ico2 spheres + plane, 100 steps) run outright on the
+    device and, all cores, on the -O3 oracle; plus the sampled estimate of
+    the same run, checked against the outright CPU time."""
     from rigidipc_amd import scenes as S
     from rigidipc_amd import sim as M
-    sc = S.config2(seed=0)
-    L = O.lib()
-    L.oracle_backend.argtypes = [ctypes.POINTER(M.Backend)]
-    L.oracle_backend.restype = None
-    be = M.Backend()
-    L.oracle_backend(ctypes.byref(be))
-    sim = M.Simulation(sc, backend=ctypes.pointer(be))
-    print("READY", flush=True)
+    sc = S.config1(seed=0)
+    sim = M.Simulation(sc)
     t0 = time.perf_counter()
-
-    def poll():
-        while True:
-            st = sim.stats()
-            print(f"ITER {int(st['newton_iterations'])} {int(st['steps'])} {time.perf_counter() - t0:.6f}",
-                  flush=True)
-            time.sleep(0.5)
-
-    threading.Thread(target=poll, daemon=True).start()
-    while time.perf_counter() - t0 < budget_s + 60:
-        sim.step(1)
-
-
-def cpu_baseline_sim(budget_s):
-    """Runs cpu_baseline_child in a subprocess (it never touches the GPU) and
-    stops it at the deadline. Returns (Newton iterations, steps, seconds) of
-    its last progress report before the deadline."""
-    import selectors
-    import subprocess
-    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--cpu-child", "--cpu-budget", str(budget_s)]
-    proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
-    sel = selectors.DefaultSelector()
-    sel.register(proc.stdout, selectors.EVENT_READ)
-    if sel.select(timeout=300.0):  # scene set-up in the child
-        proc.stdout.readline()
+    sim.step(steps)
+    gpu_s = time.perf_counter() - t0
+    x_gpu, _ = sim.state()
+    st = sim.stats()
+    sim.close()
+    out = dict(workload="config1: 10 ico2 spheres + static plane, dt 0.01, dhat 1e-3, mu 0, 100 steps",
+               steps=steps, gpu_steps_per_s=steps / gpu_s, gpu_newton_iterations=int(st["newton_iterations"]))
+    log(f"config1: device {steps} steps in {gpu_s:.2f} s")
+    if no_cpu:
+        return out
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    L3 = O.lib_o3()
+    L3.oracle_backend.argtypes = [ctypes.POINTER(M.Backend)]
+    L3.oracle_backend.restype = None
+    table = M.Backend()
+    L3.oracle_backend(ctypes.byref(table))
+    cpu = M.Simulation(sc, backend=ctypes.pointer(table))
     t0 = time.perf_counter()
-    iters, steps, elapsed = 0, 0, 0.0
-    last_log = 0.0
-    while time.perf_counter() - t0 < budget_s:
-        if not sel.select(timeout=1.0):
-            continue
-        line = proc.stdout.readline()
-        if not line:
-            break
-        if line.startswith("ITER"):
-            _, it, st, t = line.split()
-            if float(t) <= budget_s:
-                iters, steps, elapsed = int(it), int(st), float(t)
-            if elapsed - last_log >= 5.0:
-                last_log = elapsed
-                log(f"cpu baseline: {iters} Newton iterations, {steps} steps, {elapsed:.1f} s")
-    if proc.poll() is None:
-        proc.kill()
-    proc.wait()
-    return iters, steps, elapsed
+    for k in range(steps):
+        cpu.step(1)
+        if (k + 1) % 20 == 0:
+            log(f"config1 cpu: {k + 1} steps, {time.perf_counter() - t0:.1f} s")
+    cpu_s = time.perf_counter() - t0
+    x_cpu, _ = cpu.state()
+    cst = cpu.stats()
+    cpu.close()
+    est = cpu_estimate(sc, steps, 10.0)
+    out.update(cpu_steps_per_s_all_cores=steps / cpu_s, cpu_threads=O.max_threads(L3),
+               cpu_newton_iterations=int(cst["newton_iterations"]),
+               max_end_pose_difference=float(np.abs(x_gpu - x_cpu).max()),
+               gpu_vs_cpu=(steps / gpu_s) / (steps / cpu_s),
+               sampled_estimate=dict(all_cores=est["all_cores"], one_core=est["one_core"],
+                                     cpu_ms_per_call=est["cpu_ms_per_call"], calls=est["calls"]),
+               gpu_vs_cpu_one_core=(steps / gpu_s) / est["one_core"])
+    log(f"config1: cpu {steps} steps in {cpu_s:.1f} s (sampled estimate {steps / est['all_cores']:.1f} s)")
+    return out
 
 
 def bench_sim(args):
@@ -209,20 +579,40 @@ def bench_sim(args):
             ccd_candidates=int(st["ccd_candidates"]), step_ms=[round(1e3 * t, 3) for t in per_step]),
         roofline=ccd_roofline(st["ccd_box_evaluations"], st["ccd_kernel_ms"], st["ccd_launches"], SIM_CCD_TRAFFIC),
     )
+    bp_bytes = broad_phase_bytes(sc, st["ccd_calls"], st["ccd_candidates"])
+    bp_s = st["ccd_broad_phase_ms"] * 1e-3
+    line["broad_phase"] = dict(
+        bound="hbm", achieved=bp_bytes / bp_s * 1e-9 if bp_s > 0 else None, peak=HBM_PEAK_GBS, unit="GB/s",
+        frac=bp_bytes / bp_s * 1e-9 / HBM_PEAK_GBS if bp_s > 0 else None, bytes_per_call=bp_bytes / st["ccd_calls"],
+        calls=int(st["ccd_calls"]), device_ms=st["ccd_broad_phase_ms"],
+        note="continuous broad phase of the CCD calls: algorithmic bytes (poses, mesh, element boxes written and "
+             "read, grid keys, candidate pairs) over the device time between its events")
+    log("recording run (contact calls sampled for the CPU baseline)")
+    recorded = record_device_run(sc, args.steps)
+    hc = recorded[0].hessian_calls
+    flop = hc["constraints"] * ASSEMBLY_FLOP_PER_CONSTRAINT
+    line["assembly"] = dict(
+        bound="fp64", achieved=flop / (hc["assembly_ms"] * 1e-3) * 1e-12 if hc["assembly_ms"] > 0 else None,
+        peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+        frac=flop / (hc["assembly_ms"] * 1e-3) * 1e-12 / FP64_PEAK_TFLOPS if hc["assembly_ms"] > 0 else None,
+        flop_per_constraint=ASSEMBLY_FLOP_PER_CONSTRAINT, calls=hc["calls"], constraints=hc["constraints"],
+        device_ms=hc["assembly_ms"],
+        note="barrier terms with Hessian of the same run: constraints x algorithmic FLOP per constraint over the "
+             "device time of the assembly (constraint terms, PSD projection, block reduction)")
     if not args.no_cpu:
-        iters, steps, t_cpu = cpu_baseline_sim(args.cpu_budget)
-        gpu_iters = int(st["newton_iterations"])
-        # Both paths take the same Newton iterations (same algorithm, contact
-        # terms at parity), so the CPU's iteration rate over the sample is
-        # scaled by the timed run's iterations per step.
-        value = (iters / t_cpu) * (args.steps / gpu_iters) if iters and t_cpu > 0 and gpu_iters else None
+        est = cpu_estimate(sc, args.steps, args.cpu_budget, recorded=recorded)
+        hc = host_cpu()
         line["cpu_baseline"] = dict(
-            value=value, unit="time steps/s", cores=cpu_threads(), kind="port",
-            sample=f"same scene and initial state, same host stepper over the oracle backend (OpenMP, "
-                   f"{cpu_threads()} threads), run for {t_cpu:.1f} s: {iters} Newton iterations, {steps} steps "
-                   f"completed ({iters / max(t_cpu, 1e-9):.2f} iterations/s); scaled by the timed run's "
-                   f"{gpu_iters} Newton iterations over {args.steps} steps (device: "
-                   f"{gpu_iters / wall:.1f} iterations/s)")
+            value=est["all_cores"], unit="time steps/s", cores=est["threads"], kind="port",
+            value_one_core=est["one_core"], host=hc,
+            sample=f"the same {args.steps} steps from the same initial state, repeated on the device through a "
+                   f"recording backend; a uniform sample of each contact entry point's calls over all the steps "
+                   f"replayed on the -O3 oracle (OpenMP, {est['threads']} threads; at one thread on a subset of "
+                   f"the same calls), each timed alone; CPU time = sum of calls x mean time per call + the "
+                   f"stepper's host time ({est['host_s']:.2f} s)",
+            detail=est)
+        line["gpu_vs_cpu"] = dict(all_cores=line["value"] / est["all_cores"], one_core=line["value"] / est["one_core"])
+        line["config1"] = config1_outright()
     return line
 
 
@@ -566,11 +956,7 @@ def main():
                     help="steps of the batched-vs-plain comparison added to the sim line (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
-    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.cpu_child:
-        cpu_baseline_child(args.cpu_budget)
-        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not part of the build)", file=sys.stderr)
@@ -593,6 +979,8 @@ def main():
         line = bench_grad(args)
     else:
         line = bench_sim(args)
+        if args.no_cpu:
+            line["config1"] = config1_outright(no_cpu=True)
         if args.batch_steps > 0:
             line["batched"] = batch_compare(256, args.batch_steps)
             line["batched"]["note"] = (f"first {args.batch_steps} of the batching workload's 10 steps; "
