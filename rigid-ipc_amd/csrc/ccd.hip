@@ -20,9 +20,11 @@
 // per level, and whether the second half of that level is taken); popping
 // the next box is a find-last-zero plus popcounts. The t-only part of the
 // inclusion function (interval poses, interval Rodrigues matrices, world
-// vertices) is cached per lane and reused while only alpha/beta split.
-// Idle lanes first take new queries from a global counter; once that pool
-// is empty, each busy lane with an untaken second half on its path hands
+// vertices) is computed by the wavefront together for the distinct
+// (query, t-interval) pairs of a step, and cached per lane while only
+// alpha/beta split. Idle lanes first take new queries from a global
+// counter; once that pool is empty, each busy lane with an untaken second
+// half on its path hands
 // the shallowest one (its largest pending subtree) to an idle lane of the
 // same wavefront through cross-lane shuffles, so a deep query ends up
 // bisected by up to 64 lanes. Per-query minima live in global memory and
@@ -92,92 +94,157 @@ struct Query {
 // t-dependent part of the inclusion function.
 // EE: A0 = ea0(t), DA = ea1(t) - ea0(t), B0 = eb0(t), DB = eb1(t) - eb0(t)
 // FV: A0 = v(t),   DA = f1(t) - f0(t),   B0 = f0(t),  DB = f2(t) - f0(t)
-// Written as loops over the body, the sinc argument (the norm of r and of
-// r / 2), the sinc piece (the monotone piece's end points and the piece
-// beyond its maximum) and the vertices, so that each interval routine is
-// inlined once: the kernel stays within the instruction cache. The
-// operations and their order per value are those of pose_iv,
-// rotation_iv (sinc_iv) and world_vertex_iv.
-RIGC_DI void t_part(const CcdArgs& a, const Query& q, Iv t, Iv A0[3], Iv DA[3], Iv B0[3], Iv DB[3]) {
-#pragma unroll 1
-    for (int side = 0; side < 2; side++) {
-        const int body = side ? q.bB : q.bA;
-        const double* p0 = a.poses_t0 + 6 * body;
-        const double* p1 = a.poses_t1 + 6 * body;
-        Iv r[3], xa, xh;
-        pose_rot_iv(p0, p1, t, r);
-        rotation_norms(r, xa, xh);
-        double lo_a = 0.0, hi_a = 0.0, lo_h = 0.0, hi_h = 0.0;
-        Iv g_a = iv(0.0), g_h = iv(0.0);
-#pragma unroll 1
-        for (int k = 0; k < 6; k++) {
-            const bool half = k >= 3;
-            const int piece = k % 3;
-            const Iv x = half ? xh : xa;
-            const bool active = piece < 2 ? x.lo <= RIGC_SINC_MB : sinc_has_gt(x);
-            if (!active) continue;
-            Iv arg, val;
-            bool taylor = false;
-            if (piece < 2) {
+//
+// It is computed by the wavefront together. The boxes a wavefront
+// evaluates in one step share few distinct (query, t-interval) pairs (the
+// speculative subtree of a box splits t on a few of its levels only), and
+// every value of the t part is a pure function of that pair. The pairs
+// needing a new t part are numbered (up to COOP_ITEMS at a time) and the
+// work is spread over the lanes in stages, each a short instruction stream
+// run once for all pairs: the interval rotation vector and its two norms per
+// (pair, body); the monotone-piece ends of sinc per (pair, body, angle,
+// piece) and the piece beyond its maximum where a norm reaches it; one
+// rotation row per (pair, body, row); the position and one world vertex per
+// (pair, vertex). Each lane then forms its pair's differences. The operations and their order per
+// value are those of pose_iv, rotation_iv (sinc_iv) and world_vertex_iv,
+// so the result is the same, bit for bit, as evaluating each box alone.
+constexpr int COOP_ITEMS = 32;
+struct CoopWave {
+    int kind[COOP_ITEMS], body[COOP_ITEMS][2], g[COOP_ITEMS][4];
+    double t_lo[COOP_ITEMS], t_hi[COOP_ITEMS];
+    Iv r[COOP_ITEMS][2][3];       // interval rotation vector per body
+    Iv x[COOP_ITEMS][2][2];       // its norm and the norm of r / 2
+    double se[COOP_ITEMS][2][2][2];  // sinc ends [body][angle][piece]: upper at x.lo, lower at min(x.hi, mb)
+    Iv gp[COOP_ITEMS][2][2];      // sin(x) / x beyond mb [body][angle]
+    Iv R[COOP_ITEMS][2][9];       // rotation matrices, row-major
+    Iv w[COOP_ITEMS][4][3];       // world vertices, in the order of Query::g
+};
+
+// Value of lane `lane` (wavefront-uniform) in every lane.
+RIGC_DI long long bcast_lane(long long x, int lane) {
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)x, lane);
+    const int hi = __builtin_amdgcn_readlane((int)((unsigned long long)x >> 32), lane);
+    return (long long)(((unsigned long long)(unsigned int)hi << 32) | (unsigned int)lo);
+}
+
+// Orders the wavefront's LDS writes before its later LDS reads (lanes of one
+// wavefront communicating through LDS; no other wavefront takes part).
+RIGC_DI void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wavefront-uniform call: lanes with `need` set get the t part of their
+// (query key_q, t) in A0..DB; q is the lane's query record.
+RIGC_DI void coop_t_part(const CcdArgs& a, CoopWave& L, bool need, long long key_q, const Query& q, Iv t,
+                         Iv A0[3], Iv DA[3], Iv B0[3], Iv DB[3]) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(need);
+    while (pending) {
+        // number up to COOP_ITEMS distinct pairs; the first lane of each
+        // records its pair
+        int item = -1, n = 0;
+        while (pending && n < COOP_ITEMS) {
+            const int leader = __ffsll((long long)pending) - 1;  // wavefront-uniform
+            const long long lq = bcast_lane(key_q, leader);
+            const double ll = __longlong_as_double(bcast_lane(__double_as_longlong(t.lo), leader));
+            const double lh = __longlong_as_double(bcast_lane(__double_as_longlong(t.hi), leader));
+            const bool same = ((pending >> lane) & 1ULL) && key_q == lq && t.lo == ll && t.hi == lh;
+            if (same) item = n;
+            if (lane == leader) {
+                L.kind[n] = q.kind;
+                L.body[n][0] = q.bA;
+                L.body[n][1] = q.bB;
+                for (int k = 0; k < 4; k++) L.g[n][k] = q.g[k];
+                L.t_lo[n] = t.lo;
+                L.t_hi[n] = t.hi;
+            }
+            pending &= ~__ballot(same);
+            n++;
+        }
+        wave_lds_sync();
+        // rotation vector and norms per (pair, body)
+        for (int k = lane; k < 2 * n; k += 64) {
+            const int u = k >> 1, side = k & 1;
+            const int body = L.body[u][side];
+            const Iv tt = Iv{ L.t_lo[u], L.t_hi[u] };
+            Iv r[3], xa, xh;
+            pose_rot_iv(a.poses_t0 + 6 * body, a.poses_t1 + 6 * body, tt, r);
+            rotation_norms(r, xa, xh);
+            for (int j = 0; j < 3; j++) L.r[u][side][j] = r[j];
+            L.x[u][side][0] = xa;
+            L.x[u][side][1] = xh;
+        }
+        wave_lds_sync();
+        // monotone-piece ends of sinc per (pair, body, angle, piece)
+        bool any_gt = false;
+        for (int k = lane; k < 8 * n; k += 64) {
+            const int u = k >> 3, side = (k >> 2) & 1, ang = (k >> 1) & 1, piece = k & 1;
+            const Iv x = L.x[u][side][ang];
+            if (x.lo <= RIGC_SINC_MB) {
                 const double d = piece == 0 ? x.lo : (x.hi > RIGC_SINC_MB ? RIGC_SINC_MB : x.hi);
-                arg = iv(d);
-                taylor = !(fabs(d) >= 1.220703125e-04);
-            } else {
-                arg = (x.lo <= RIGC_SINC_MB) ? Iv{ RIGC_SINC_MB, x.hi } : x;
+                L.se[u][side][ang][piece] = sinc_point_end(d, piece == 0);
             }
-            // sinc_point(d) for a point argument (Taylor below 2^-13),
-            // sin(x) / x for the piece beyond mb
-            if (taylor) {
-                const Iv x2 = arg * arg;
-                val = 1.0 + (x2 / 6.0) * ((x2 / 20.0) - 1.0);
-            } else {
-                val = div_nonzero(sin_iv(arg), arg);
-            }
-            if (k == 0) lo_a = val.hi;
-            else if (k == 1) hi_a = val.lo;
-            else if (k == 2) g_a = val;
-            else if (k == 3) lo_h = val.hi;
-            else if (k == 4) hi_h = val.lo;
-            else g_h = val;
+            any_gt = any_gt || (piece == 0 && sinc_has_gt(x));
         }
-        Iv R[3][3], p[3];
-        rotation_from_sincs(r, sinc_combine(xa, lo_a, hi_a, g_a), sinc_combine(xh, lo_h, hi_h, g_h), R);
-        pose_pos_iv(p0, p1, t, p);
-        // EE: this body's edge (2 vertices); FV: the vertex (side 0) or the
-        // face (side 1, 3 vertices)
-        const int nvx = q.kind == 0 ? 2 : (side ? 3 : 1);
-        const int v0 = q.kind == 0 ? (side ? q.g[2] : q.g[0]) : (side ? q.g[1] : q.g[0]);
-        const int v1 = q.kind == 0 ? (side ? q.g[3] : q.g[1]) : q.g[2];
-        Iv o1[3], o2[3], o3[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) o1[j] = o2[j] = o3[j] = iv(0.0);
-#pragma unroll 1
-        for (int m = 0; m < nvx; m++) {
-            const int vid = m == 0 ? v0 : (m == 1 ? v1 : q.g[3]);
-            Iv w[3];
-            world_vertex_iv(R, p, a.V + 3 * vid, w);
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                if (m == 0) o1[j] = w[j];
-                else if (m == 1) o2[j] = w[j];
-                else o3[j] = w[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            if (side == 0) {
-                A0[j] = o1[j];
-                if (q.kind == 0) DA[j] = o2[j] - o1[j];
-            } else {
-                B0[j] = o1[j];
-                if (q.kind == 0) {
-                    DB[j] = o2[j] - o1[j];
-                } else {
-                    DA[j] = o2[j] - o1[j];
-                    DB[j] = o3[j] - o1[j];
+        // the piece beyond mb, where a norm reaches it
+        if (__ballot(any_gt)) {
+            for (int k = lane; k < 4 * n; k += 64) {
+                const int u = k >> 2, side = (k >> 1) & 1, ang = k & 1;
+                const Iv x = L.x[u][side][ang];
+                if (sinc_has_gt(x)) {
+                    const Iv arg = (x.lo <= RIGC_SINC_MB) ? Iv{ RIGC_SINC_MB, x.hi } : x;
+                    L.gp[u][side][ang] = div_nonzero(sin_iv(arg), arg);
                 }
             }
         }
+        wave_lds_sync();
+        // one rotation row per (pair, body, row)
+        for (int k = lane; k < 6 * n; k += 64) {
+            const int u = k / 6, side = (k / 3) & 1, i = k % 3;
+            const Iv xa = L.x[u][side][0], xh = L.x[u][side][1];
+            const Iv sa = sinc_combine(xa, L.se[u][side][0][0], L.se[u][side][0][1], L.gp[u][side][0]);
+            const Iv sh = sinc_combine(xh, L.se[u][side][1][0], L.se[u][side][1][1], L.gp[u][side][1]);
+            const Iv c2 = rotation_c2(sh);
+            Iv r[3];
+            for (int j = 0; j < 3; j++) r[j] = L.r[u][side][j];
+            for (int j = 0; j < 3; j++) L.R[u][side][3 * i + j] = rotation_entry(r, sa, c2, i, j);
+        }
+        wave_lds_sync();
+        // position and one world vertex per (pair, vertex)
+        for (int k = lane; k < 4 * n; k += 64) {
+            const int u = k >> 2, m = k & 3;
+            const int side = L.kind[u] == 0 ? (m >= 2) : (m >= 1);
+            const int body = L.body[u][side];
+            const Iv tt = Iv{ L.t_lo[u], L.t_hi[u] };
+            const double* v = a.V + 3 * L.g[u][m];
+            for (int j = 0; j < 3; j++) {
+                const Iv pj = pose_pos_component(a.poses_t0 + 6 * body, a.poses_t1 + 6 * body, tt, j);
+                L.w[u][m][j] = world_coordinate(L.R[u][side][3 * j], L.R[u][side][3 * j + 1], L.R[u][side][3 * j + 2],
+                                                pj, v);
+            }
+        }
+        wave_lds_sync();
+        if (item >= 0) {
+            const int u = item;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const Iv w0 = L.w[u][0][j], w1 = L.w[u][1][j], w2 = L.w[u][2][j], w3 = L.w[u][3][j];
+                if (L.kind[u] == 0) {
+                    A0[j] = w0;
+                    DA[j] = w1 - w0;
+                    B0[j] = w2;
+                    DB[j] = w3 - w2;
+                } else {
+                    A0[j] = w0;
+                    B0[j] = w1;
+                    DA[j] = w2 - w1;
+                    DB[j] = w3 - w1;
+                }
+            }
+        }
+        wave_lds_sync();  // the next batch overwrites the pairs
     }
 }
 
@@ -293,6 +360,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     const long long nq = a.n_ee + a.n_fv;
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
+    __shared__ CoopWave coop[4];  // one per wavefront of the 256-thread block
 
     long long qi = -1;  // query of the subtree this lane walks, -1 = idle
     int phase = 0;      // 1 = start-box check, 2 = bisection
@@ -596,14 +664,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         }
 
         const bool evaluating = own_eval || spec;
+        const bool need_t = evaluating && (spec || phase == 1 || !cache_ok || c_idx != e_idx0 || c_lev != e_lev0);
+        coop_t_part(a, coop[threadIdx.x >> 6], need_t, spec ? s_q : qi, q, t, A0, DA, B0, DB);
+        if (need_t) {
+            cache_ok = !spec && phase == 2;
+            c_idx = e_idx0;
+            c_lev = e_lev0;
+        }
         bool z = false;
         if (evaluating) {
-            if (spec || phase == 1 || !cache_ok || c_idx != e_idx0 || c_lev != e_lev0) {
-                t_part(a, q, t, A0, DA, B0, DB);
-                cache_ok = !spec && phase == 2;
-                c_idx = e_idx0;
-                c_lev = e_lev0;
-            }
             z = contains_zero(q.kind, A0, DA, B0, DB, u, v);
             evals++;
         }

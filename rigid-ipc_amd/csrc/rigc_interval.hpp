@@ -238,12 +238,76 @@ RIGC_DI Iv cos_iv(Iv x) {
 // Boost sin(x) = cos(x - [pi/2]) with the subtraction in round-to-nearest.
 RIGC_DI Iv sin_iv(Iv x) { return cos_iv(x - Iv{ RIGC_PI_LO / 2, RIGC_PI_HI / 2 }); }
 
+// One end of cos_iv(x) (the upper one if `upper`), with the same operations
+// as cos_iv for that end: the reduction is shared, and only the cosine the
+// end needs is evaluated.
+RIGC_DI double cos_iv_end(Iv x, bool upper) {
+    const Iv pi2 = Iv{ RIGC_PI_LO * 2, RIGC_PI_HI * 2 };
+    const Iv pi = Iv{ RIGC_PI_LO, RIGC_PI_HI };
+    bool negate = false;
+    bool downward = false;
+    // r = [-1, 1] (kind 0), [cos_down(u), cos_up(l)] (kind 1) or
+    // [-1, cos_up(m)] (kind 2)
+    int kind = 0;
+    double l = 0.0, u = 0.0;
+#pragma unroll 1
+    for (int level = 0; level < 4; level++) {
+        const double yb = (x.lo < 0.0) ? pi2.lo : pi2.hi;
+        const double q = downward ? step_dn(div_rd(x.lo, yb)) : step_dn(x.lo / yb);
+        const double n = floor(q);
+        downward = true;
+        const Iv tmp = sub_rd(x, mul_rd(pi2, n));
+        const double w = step_up(sub_rd(tmp.hi, tmp.lo));
+        if (w >= pi2.lo) break;
+        if (tmp.lo >= RIGC_PI_HI) {
+            x = sub_rd(tmp, pi);
+            negate = !negate;
+            continue;
+        }
+        l = tmp.lo;
+        u = tmp.hi;
+        if (u <= RIGC_PI_LO) {
+            kind = 1;
+        } else if (u <= pi2.lo) {
+            kind = 2;
+            l = fmin(step_dn(sub_rd(pi2.lo, u)), l);
+        }
+        break;
+    }
+    // the end of r that gives the requested end after the negation
+    const bool r_hi = upper != negate;
+    double e;
+    if (kind == 0 || (kind == 2 && !r_hi)) {
+        e = r_hi ? 1.0 : -1.0;
+    } else {
+        const double c = cos_rn(r_hi ? l : u);
+        e = r_hi ? clamp1(next_up(c + 1e-15)) : clamp1(next_dn(c - 1e-15));
+    }
+    return negate ? -e : e;
+}
+
 // _sinc_interval_taylor (src/utils/sinc.cpp:36-47)
 RIGC_DI Iv sinc_point(double xd) {
     const Iv x = iv(xd);
     if (fabs(xd) >= 1.220703125e-04) return div_nonzero(sin_iv(x), x);
     const Iv x2 = x * x;
     return 1.0 + (x2 / 6.0) * ((x2 / 20.0) - 1.0);
+}
+
+// One end of sinc_point(xd) for xd >= 0, the same bits as sinc_point's.
+// Off the Taylor branch xd > 0, so the quotient by the point [xd, xd] is
+// monotone: its ends are those of sin over xd, and each needs one end of
+// the sine.
+RIGC_DI double sinc_point_end(double xd, bool upper) {
+    if (!(fabs(xd) >= 1.220703125e-04)) {
+        const Iv x = iv(xd);
+        const Iv x2 = x * x;
+        const Iv y = 1.0 + (x2 / 6.0) * ((x2 / 20.0) - 1.0);
+        return upper ? y.hi : y.lo;
+    }
+    const Iv a = iv(xd) - Iv{ RIGC_PI_LO / 2, RIGC_PI_HI / 2 };
+    const double s = cos_iv_end(a, upper);
+    return upper ? step_up(s / xd) : step_dn(s / xd);
 }
 
 // sinc(Interval) (src/utils/sinc.cpp:49-101) for x.lo >= 0 (x is always a
@@ -253,10 +317,10 @@ RIGC_DI Iv sinc_point(double xd) {
 // combination.
 #define RIGC_SINC_MB 4.4934094579
 RIGC_DI bool sinc_has_gt(Iv x) { return !(x.lo <= RIGC_SINC_MB) || x.hi > RIGC_SINC_MB; }
-RIGC_DI double sinc_part_lo(Iv x) { return x.lo <= RIGC_SINC_MB ? sinc_point(x.lo).hi : 0.0; }
+RIGC_DI double sinc_part_lo(Iv x) { return x.lo <= RIGC_SINC_MB ? sinc_point_end(x.lo, true) : 0.0; }
 RIGC_DI double sinc_part_hi(Iv x, Iv& g) {
     double v = 0.0;
-    if (x.lo <= RIGC_SINC_MB) v = sinc_point(x.hi > RIGC_SINC_MB ? RIGC_SINC_MB : x.hi).lo;
+    if (x.lo <= RIGC_SINC_MB) v = sinc_point_end(x.hi > RIGC_SINC_MB ? RIGC_SINC_MB : x.hi, false);
     g = iv(0.0);
     if (sinc_has_gt(x)) {
         const Iv xg = (x.lo <= RIGC_SINC_MB) ? Iv{ RIGC_SINC_MB, x.hi } : x;
