@@ -406,21 +406,22 @@ __global__ __launch_bounds__(THREADS) void local_eg_kernel(LocalArgs a) {
     for (int r = 0; r < 12; r++) a.lgrad[12 * c + r] = lg[r];
 }
 
-// Round-robin pairing of 12 indices: 11 rounds of 6 disjoint pairs that
-// together cover each of the 66 pairs once.
-__constant__ signed char JACOBI_PAIRS[11][6][2] = {
-    { { 0, 11 }, { 1, 10 }, { 2, 9 }, { 3, 8 }, { 4, 7 }, { 5, 6 } },
-    { { 0, 10 }, { 11, 9 }, { 1, 8 }, { 2, 7 }, { 3, 6 }, { 4, 5 } },
-    { { 0, 9 }, { 10, 8 }, { 11, 7 }, { 1, 6 }, { 2, 5 }, { 3, 4 } },
-    { { 0, 8 }, { 9, 7 }, { 10, 6 }, { 11, 5 }, { 1, 4 }, { 2, 3 } },
-    { { 0, 7 }, { 8, 6 }, { 9, 5 }, { 10, 4 }, { 11, 3 }, { 1, 2 } },
-    { { 0, 6 }, { 7, 5 }, { 8, 4 }, { 9, 3 }, { 10, 2 }, { 11, 1 } },
-    { { 0, 5 }, { 6, 4 }, { 7, 3 }, { 8, 2 }, { 9, 1 }, { 10, 11 } },
-    { { 0, 4 }, { 5, 3 }, { 6, 2 }, { 7, 1 }, { 8, 11 }, { 9, 10 } },
-    { { 0, 3 }, { 4, 2 }, { 5, 1 }, { 6, 11 }, { 7, 10 }, { 8, 9 } },
-    { { 0, 2 }, { 3, 1 }, { 4, 11 }, { 5, 10 }, { 6, 9 }, { 7, 8 } },
-    { { 0, 1 }, { 2, 11 }, { 3, 10 }, { 4, 9 }, { 5, 8 }, { 6, 7 } },
-};
+// Pair k (0..5) of round r (0..10) of the parallel Jacobi ordering: the
+// circle method with index 0 fixed and 1..11 turning one place per round,
+// so the 11 rounds of 6 disjoint pairs cover the 66 pairs of 12 indices
+// once. Round 0 is (0,11) (1,10) (2,9) (3,8) (4,7) (5,6).
+__device__ inline void jacobi_pair(int r, int k, int& p, int& q) {
+    if (k == 0) {
+        p = 0;
+        q = 1 + (10 - r);
+    } else {
+        int a = k - 1 - r, b = 10 - k - r;
+        if (a < 0) a += 11;
+        if (b < 0) b += 11;
+        p = 1 + a;
+        q = 1 + b;
+    }
+}
 
 __device__ inline double wave_sum_uniform(double x) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -451,6 +452,7 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
     __shared__ double sg[12];
     __shared__ double sG[4][9];  // slot Jacobian (rotation columns): [k][c]
     __shared__ double sT[4][6];  // slot rotational Hessian term, (c,d) packed
+    __shared__ double sCS[6][2];  // the round's rotations (c, s)
     const int lane = threadIdx.x;
     for (long long c = blockIdx.x; c < a.n; c += gridDim.x) {
         ConRec rec;
@@ -537,6 +539,7 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
         }
         __syncthreads();
         // project_to_psd: Jacobi in parallel order; unchanged when PSD
+        double prev_off = INFINITY;
         for (int sweep = 0; a.project && sweep < 100; sweep++) {
             double off = 0.0, tot = 0.0;
             for (int e = lane; e < 144; e += 64) {
@@ -547,37 +550,48 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
             off = wave_sum_uniform(off);
             tot = wave_sum_uniform(tot);
             if (off <= 1e-30 * tot || off == 0) break;
-            // Round r rotates the six pairs of JACOBI_PAIRS[r] at once.
-            // Lanes 0..5 compute the rotations, which the other lanes take
-            // by shuffles; lane l < 36 owns the 2 x 2 block of A at
-            // (pair l / 6, pair l % 6), and the 72 (row, pair) entries of V
-            // go to lanes l and l + 64. One barrier separates a round's
-            // reads from its writes, one follows the writes.
+            // Rank-deficient blocks settle at a rounding floor just above
+            // that bound (their near-zero rows keep rounding noise): stop
+            // once a sweep no longer cuts the off-diagonal mass by 4 and it
+            // is already below 1e-24 of the total.
+            if (off > 0.25 * prev_off && off <= 1e-24 * tot) break;
+            prev_off = off;
+            // Round r rotates its six pairs (jacobi_pair) at once. Lanes
+            // 0..5 compute the rotations, which the other lanes read from
+            // LDS; lane l < 36 owns the 2 x 2 block of A at (pair l / 6,
+            // pair l % 6), and the 72 (row, pair) entries of V go to lanes l
+            // and l + 64. One barrier publishes the rotations and separates
+            // the round's reads from its writes, one follows the writes.
             for (int round = 0; round < 11; round++) {
-                const signed char(*P)[2] = JACOBI_PAIRS[round];
-                double rc = 1.0, rs = 0.0;
-                if (lane < 6) jacobi_rotation(sA, P[lane][0], P[lane][1], rc, rs);
                 const int ia = lane < 36 ? lane / 6 : 0, ib = lane % 6;
-                const int pa = P[ia][0], qa = P[ia][1], pb = P[ib][0], qb = P[ib][1];
+                const int k1 = (lane + 64) % 6;
+                int pa, qa, pb, qb, vp1, vq1;
+                jacobi_pair(round, ia, pa, qa);
+                jacobi_pair(round, ib, pb, qb);
+                jacobi_pair(round, k1, vp1, vq1);
+                if (lane < 6) {
+                    double rc, rs;
+                    jacobi_rotation(sA, pb, qb, rc, rs);
+                    sCS[lane][0] = rc;
+                    sCS[lane][1] = rs;
+                }
                 double x00 = 0.0, x01 = 0.0, x10 = 0.0, x11 = 0.0;
                 if (lane < 36) {
                     x00 = sA[12 * pa + pb]; x01 = sA[12 * pa + qb];
                     x10 = sA[12 * qa + pb]; x11 = sA[12 * qa + qb];
                 }
                 // V entries: row t / 6 with pair t % 6, t = lane, lane + 64
-                const int k1 = (lane + 64) % 6;
                 const int vr0 = lane / 6, vr1 = (lane + 64) / 6;
-                const int vp0 = P[ib][0], vq0 = P[ib][1], vp1 = P[k1][0], vq1 = P[k1][1];
-                const double u0 = sV[12 * vr0 + vp0], w0 = sV[12 * vr0 + vq0];
+                const double u0 = sV[12 * vr0 + pb], w0 = sV[12 * vr0 + qb];
                 double u1 = 0.0, w1 = 0.0;
                 if (lane + 64 < 72) {
                     u1 = sV[12 * vr1 + vp1];
                     w1 = sV[12 * vr1 + vq1];
                 }
-                const double ca = __shfl(rc, ia), sa = __shfl(rs, ia);
-                const double cb = __shfl(rc, ib), sb = __shfl(rs, ib);
-                const double c1 = __shfl(rc, k1), s1 = __shfl(rs, k1);
                 __syncthreads();
+                const double ca = sCS[ia][0], sa = sCS[ia][1];
+                const double cb = sCS[ib][0], sb = sCS[ib][1];
+                const double c1 = sCS[k1][0], s1 = sCS[k1][1];
                 if (lane < 36) {
                     // columns (pair b), then rows (pair a)
                     const double y00 = cb * x00 - sb * x01, y01 = sb * x00 + cb * x01;
@@ -587,8 +601,8 @@ __global__ __launch_bounds__(64) void local_hess_kernel(LocalArgs a) {
                     sA[12 * pa + qb] = ca * y01 - sa * y11;
                     sA[12 * qa + qb] = sa * y01 + ca * y11;
                 }
-                sV[12 * vr0 + vp0] = cb * u0 - sb * w0;
-                sV[12 * vr0 + vq0] = sb * u0 + cb * w0;
+                sV[12 * vr0 + pb] = cb * u0 - sb * w0;
+                sV[12 * vr0 + qb] = sb * u0 + cb * w0;
                 if (lane + 64 < 72) {
                     sV[12 * vr1 + vp1] = c1 * u1 - s1 * w1;
                     sV[12 * vr1 + vq1] = s1 * u1 + c1 * w1;
@@ -733,35 +747,42 @@ __device__ long long lower_bound_u64(const unsigned long long* k, long long n, u
 }
 
 // Block per body: lanes 0..35 the diagonal 6x6 block, lanes 36..41 the
-// gradient, summed in sorted (constraint) order.
+// gradient, summed in sorted (constraint) order. The terms are fetched
+// BODY_BATCH at a time (independent loads in flight; a body on the plane
+// has hundreds of constraints) and added in that same order.
+constexpr int BODY_BATCH = 8;
 __global__ __launch_bounds__(64) void body_reduce_kernel(const int* bodies, const unsigned long long* keys,
                                                         long long nk, const double* lgrad, const double* lhess,
                                                         int want_hess, double* grad, double* diag) {
     const int b = bodies[blockIdx.x], lane = threadIdx.x;
     const long long lo = lower_bound_u64(keys, nk, (unsigned long long)b << 32);
     const long long hi = lower_bound_u64(keys, nk, (unsigned long long)(b + 1) << 32);
-    if (lane < 36) {
-        if (!want_hess) return;
-        const int p = lane / 6, q = lane % 6;
-        double s = 0.0;
-        for (long long i = lo; i < hi; i++) {
-            const unsigned long long t = keys[i] & 0xffffffffULL;
-            const long long c = (long long)(t >> 1);
-            const int side = (int)(t & 1ULL);
-            s += lhess[144 * c + 12 * (6 * side + p) + 6 * side + q];
+    if (lane >= 42 || (lane < 36 && !want_hess)) return;
+    // this lane's entry: offset within a constraint's 12 x 12 block (or 12
+    // gradient entries) for side 0; side 1 adds 78 (or 6)
+    const bool hess = lane < 36;
+    const double* src = hess ? lhess : lgrad;
+    const long long stride = hess ? 144 : 12;
+    const int off0 = hess ? 12 * (lane / 6) + lane % 6 : lane - 36;
+    const int side_step = hess ? 78 : 6;
+    double s = 0.0;
+    long long i = lo;
+    for (; i + BODY_BATCH <= hi; i += BODY_BATCH) {
+        double v[BODY_BATCH];
+#pragma unroll
+        for (int k = 0; k < BODY_BATCH; k++) {
+            const unsigned long long t = keys[i + k] & 0xffffffffULL;
+            v[k] = src[stride * (long long)(t >> 1) + off0 + side_step * (int)(t & 1ULL)];
         }
-        diag[36 * b + lane] = s;
-    } else if (lane < 42) {
-        const int p = lane - 36;
-        double s = 0.0;
-        for (long long i = lo; i < hi; i++) {
-            const unsigned long long t = keys[i] & 0xffffffffULL;
-            const long long c = (long long)(t >> 1);
-            const int side = (int)(t & 1ULL);
-            s += lgrad[12 * c + 6 * side + p];
-        }
-        grad[6 * b + p] = s;
+#pragma unroll
+        for (int k = 0; k < BODY_BATCH; k++) s += v[k];
     }
+    for (; i < hi; i++) {
+        const unsigned long long t = keys[i] & 0xffffffffULL;
+        s += src[stride * (long long)(t >> 1) + off0 + side_step * (int)(t & 1ULL)];
+    }
+    if (hess) diag[36 * b + lane] = s;
+    else grad[6 * b + lane - 36] = s;
 }
 
 __global__ __launch_bounds__(THREADS) void off_heads_kernel(const unsigned long long* keys, long long n, int idx_bits,
