@@ -104,6 +104,23 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
                               double toi_tolerance, int64_t eval_budget, int32_t per_query,
                               double* earliest_toi, double* toi_out, int8_t* hit_out);
 
+/* rigc_narrow_phase that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it). Same arguments
+ * and results as rigc_narrow_phase with eval_budget = 0. Every per-call
+ * buffer (poses, candidate lists, per-query state, results) lives in one
+ * static device array of RIGC_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a call needs about 96 bytes per body of the scene plus 41
+ * per query (each buffer rounded up to 256 bytes) and returns
+ * RIGC_ERR_CAPACITY, allocating nothing, when that does not fit. The narrow
+ * phase runs in one pass, which gives the same results. Calls that overlap
+ * on one GPU (two contexts or threads at once) share the array and race. */
+#define RIGC_NOALLOC_SCRATCH_BYTES (32u << 20)
+rigc_status rigc_narrow_phase_noalloc(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                      const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                                      double toi_tolerance, int32_t per_query, double* earliest_toi,
+                                      double* toi_out, int8_t* hit_out);
+
 /* Broad phase. Replaces detect_collision_candidates_rigid with
  * DetectionMethod::HASH_GRID, continuous (poses_t1 != NULL;
  * src/ccd/rigid/broad_phase.cpp:127-203, RigidBodyHashGrid::{resize,addBodies}

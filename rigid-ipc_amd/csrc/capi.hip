@@ -62,6 +62,7 @@ struct rigc_ctx {
     int walk_budget = 4096;
     std::vector<double> last_toi;  // per scene: answer of its previous earliest-TOI call
     hipEvent_t ev2 = nullptr;
+    unsigned char* scratch = nullptr;  // this device's no-allocation array (rigc_narrow_phase_noalloc)
 
     rigc_status fail(rigc_status s, const std::string& msg) {
         err = msg;
@@ -307,10 +308,13 @@ rigc_status rigc_create(rigc_ctx** out, int32_t device) {
     if (e == hipSuccess) e = hipEventRecord(ctx->ev_stage, ctx->stream);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    void* scratch = nullptr;
+    if (e == hipSuccess) e = noalloc_scratch(device, &scratch);  // device current
     if (e != hipSuccess) {
         delete ctx;
         return RIGC_ERR_DEVICE;
     }
+    ctx->scratch = (unsigned char*)scratch;
     ctx->num_cus = prop.multiProcessorCount;
     // RIGC_CCD_WALK_BUDGET: box evaluations after which a lane hands its
     // pending subtrees to the next narrow-phase pass (0 = never)
@@ -495,6 +499,83 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     if (hit_out) std::memcpy(hit_out, hits.data(), nq);
     if (unresolved)
         return ctx->fail(RIGC_ERR_UNRESOLVED, "rigc_narrow_phase: a query exceeded its evaluation budget");
+    return RIGC_OK;
+}
+
+rigc_status rigc_narrow_phase_noalloc(rigc_ctx* ctx, const double* poses_t0, const double* poses_t1,
+                                      const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                                      double toi_tolerance, int32_t per_query, double* earliest_toi,
+                                      double* toi_out, int8_t* hit_out) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_narrow_phase_noalloc: no scene");
+    if (!poses_t0 || !poses_t1 || !earliest_toi || n_ee < 0 || n_fv < 0 || (n_ee && !ee) || (n_fv && !fv)
+        || !(toi_tolerance > 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase_noalloc: bad arguments");
+    const DevScene& d = ctx->scene;
+    for (int64_t i = 0; i < n_ee; i++)
+        if (ee[2 * i] < 0 || ee[2 * i] >= d.ne || ee[2 * i + 1] < 0 || ee[2 * i + 1] >= d.ne)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase_noalloc: edge id out of range");
+    for (int64_t i = 0; i < n_fv; i++)
+        if (fv[2 * i] < 0 || fv[2 * i] >= d.nf || fv[2 * i + 1] < 0 || fv[2 * i + 1] >= d.nv)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase_noalloc: face/vertex id out of range");
+    const NoallocLayout L = noalloc_layout(d.nb, n_ee, n_fv, d.n_scenes);
+    if (L.total > NOALLOC_SCRATCH_BYTES)
+        return ctx->fail(RIGC_ERR_CAPACITY, "rigc_narrow_phase_noalloc: the scene's bodies and the queries need more "
+                                            "than RIGC_NOALLOC_SCRATCH_BYTES of scratch");
+    RIGC_HIP(hipSetDevice(ctx->device));
+    unsigned char* base = ctx->scratch;
+    hipStream_t st = ctx->stream;
+    const size_t nq = (size_t)(n_ee + n_fv);
+    const size_t pose_bytes = 48 * (size_t)d.nb;
+    RIGC_HIP(hipMemcpyAsync(base + L.poses0, poses_t0, pose_bytes, hipMemcpyHostToDevice, st));
+    RIGC_HIP(hipMemcpyAsync(base + L.poses1, poses_t1, pose_bytes, hipMemcpyHostToDevice, st));
+    if (n_ee) RIGC_HIP(hipMemcpyAsync(base + L.ee, ee, 8 * (size_t)n_ee, hipMemcpyHostToDevice, st));
+    if (n_fv) RIGC_HIP(hipMemcpyAsync(base + L.fv, fv, 8 * (size_t)n_fv, hipMemcpyHostToDevice, st));
+
+    CcdArgs a{};
+    a.V = d.V; a.E = d.E; a.F = d.F; a.FE = d.FE;
+    a.v_body = d.v_body; a.e_body = d.e_body; a.f_body = d.f_body; a.edge_len = d.edge_len;
+    a.poses_t0 = (const double*)(base + L.poses0);
+    a.poses_t1 = (const double*)(base + L.poses1);
+    a.ee = (const int*)(base + L.ee);
+    a.fv = (const int*)(base + L.fv);
+    a.n_ee = n_ee;
+    a.n_fv = n_fv;
+    a.toi_tol = toi_tolerance;
+    a.eval_budget = 0;
+    a.use_global_best = per_query ? 0 : 1;
+    a.global_best = (unsigned long long*)(base + L.scene_best);
+    a.body_scene = d.body_scene;
+    a.scene_skip = nullptr;
+    unsigned long long* counters = (unsigned long long*)(base + L.counters);
+    a.counter = counters + 1;
+    a.total_evals = counters + 2;
+    a.n_tasks_out = counters + 3;
+    a.toi_out = (double*)(base + L.toi);
+    a.hit_out = (signed char*)(base + L.hit);
+    a.q_best = (unsigned long long*)(base + L.qbest);
+    a.q_visits = (unsigned int*)(base + L.qvisits);
+    a.q_flag = (int*)(base + L.qflag);
+
+    RIGC_HIP(hipEventRecord(ctx->ev0, st));
+    RIGC_HIP(noalloc_narrow_phase(a, ctx->num_cus * CCD_BLOCKS_PER_CU, base, L, d.n_scenes, per_query ? 1 : 0, st));
+    RIGC_HIP(hipEventRecord(ctx->ev1, st));
+    unsigned long long res[2] = { 0, 0 }, evals = 0;
+    RIGC_HIP(hipMemcpyAsync(res, base + L.result, sizeof(res), hipMemcpyDeviceToHost, st));
+    if (nq) RIGC_HIP(hipMemcpyAsync(&evals, counters + 2, sizeof(evals), hipMemcpyDeviceToHost, st));
+    if (nq && toi_out) RIGC_HIP(hipMemcpyAsync(toi_out, base + L.toi, 8 * nq, hipMemcpyDeviceToHost, st));
+    if (nq && hit_out) RIGC_HIP(hipMemcpyAsync(hit_out, base + L.hit, nq, hipMemcpyDeviceToHost, st));
+    RIGC_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    RIGC_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->stats.narrow_phase_ms = ms;
+    ctx->stats.box_evaluations = (int64_t)evals;
+    ctx->stats.num_candidates = (int64_t)nq;
+    ctx->stats.ccd_kernel_ms = ms;
+    ctx->stats.ccd_launches = nq ? 1 : 0;
+    std::memcpy(earliest_toi, &res[0], sizeof(double));
+    if (res[1])
+        return ctx->fail(RIGC_ERR_UNRESOLVED, "rigc_narrow_phase_noalloc: a query exceeded the search depth");
     return RIGC_OK;
 }
 

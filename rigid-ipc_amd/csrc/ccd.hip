@@ -806,6 +806,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 
 }  // namespace
 
+hipError_t launch_ccd_single(CcdArgs a, int grid, int block, hipStream_t stream) {
+    const long long nq = a.n_ee + a.n_fv;
+    if (nq <= 0) return hipSuccess;
+    const unsigned eb = (unsigned)((nq + 255) / 256);
+    a.tasks_in = nullptr;
+    a.n_tasks_in = 0;
+    a.tasks_out = nullptr;
+    a.tasks_cap = 0;
+    a.walk_budget = 0;
+    hipLaunchKernelGGL(ccd_init_kernel, dim3(eb), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(ccd_kernel, dim3(grid), dim3(block), 0, stream, a);
+    if (a.toi_out || a.hit_out) hipLaunchKernelGGL(ccd_finish_kernel, dim3(eb), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& w, const CcdReadback* rb,
                       int n_rb) {
     const long long nq = a.n_ee + a.n_fv;

@@ -248,6 +248,27 @@ struct CcdReadback {
 constexpr int CCD_BLOCKS_PER_CU = 1;
 hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& work,
                       const CcdReadback* readback = nullptr, int n_readback = 0);
+// The narrow phase in one pass (no export of long walks, so no task
+// buffers): a.n_tasks_out must point to one device counter, a.tasks_out is
+// ignored. Same results as launch_ccd.
+hipError_t launch_ccd_single(CcdArgs a, int grid, int block, hipStream_t stream);
+
+// ---- the narrow phase without allocation (noalloc.hip)
+// Size of the static device array that holds every per-call buffer.
+constexpr size_t NOALLOC_SCRATCH_BYTES = RIGC_NOALLOC_SCRATCH_BYTES;
+// Byte offsets of one call's buffers in that array (256-aligned).
+struct NoallocLayout {
+    size_t poses0, poses1, ee, fv, toi, hit, counters, qbest, qvisits, qflag, scene_best, result, total;
+};
+// Device address of the array on `device` (the current device), looked up
+// once per device.
+hipError_t noalloc_scratch(int device, void** ptr);
+NoallocLayout noalloc_layout(long long nb, long long n_ee, long long n_fv, int n_scenes);
+// Sets the per-scene minima, runs the narrow phase over a (whose buffers
+// point into base) and writes the result words (least TOI bits, unresolved
+// flag) at base + L.result.
+hipError_t noalloc_narrow_phase(CcdArgs a, int grid, unsigned char* base, const NoallocLayout& L, int n_scenes,
+                                int per_query, hipStream_t stream);
 
 // Exclusive prefix sum of n counts; work holds scan_workspace_entries(n)
 // entries and *total (device) receives the sum.

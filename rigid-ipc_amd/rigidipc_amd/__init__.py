@@ -84,6 +84,9 @@ def lib():
     L.rigc_narrow_phase.argtypes = [vp, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_int64,
                                     _c_int32_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int64,
                                     ctypes.c_int32, _c_double_p, _c_double_p, _c_int8_p]
+    L.rigc_narrow_phase_noalloc.argtypes = [vp, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_int64, _c_int32_p,
+                                            ctypes.c_int64, ctypes.c_double, ctypes.c_int32, _c_double_p,
+                                            _c_double_p, _c_int8_p]
     L.rigc_broad_phase.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_int32_p, ctypes.c_int64,
                                    _c_int64_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
     L.rigc_compute_earliest_toi.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p]
@@ -346,6 +349,24 @@ class Context:
                                        ctypes.byref(best), _p(toi, _c_double_p), _p(hit, _c_int8_p))
         if not (allow_unresolved and st == RIGC_ERR_UNRESOLVED):
             self._check(st)
+        return best.value, toi[:n], hit[:n]
+
+    def narrow_phase_noalloc(self, poses_t0, poses_t1, ee, fv, toi_tolerance=1e-4, per_query=False):
+        """rigc_narrow_phase_noalloc: rigid CCD over candidates with every
+        per-call buffer in the library's static device array; returns
+        (earliest_toi, toi[n], hit[n])."""
+        p0 = np.ascontiguousarray(poses_t0, np.float64).reshape(-1, 6)
+        p1 = np.ascontiguousarray(poses_t1, np.float64).reshape(-1, 6)
+        ee = np.ascontiguousarray(np.asarray(ee, np.int32).reshape(-1, 2))
+        fv = np.ascontiguousarray(np.asarray(fv, np.int32).reshape(-1, 2))
+        n = len(ee) + len(fv)
+        toi = np.empty(max(n, 1))
+        hit = np.empty(max(n, 1), np.int8)
+        best = ctypes.c_double()
+        self._check(self._L.rigc_narrow_phase_noalloc(self._ctx, _p(p0, _c_double_p), _p(p1, _c_double_p),
+                                                      _p(ee, _c_int32_p), len(ee), _p(fv, _c_int32_p), len(fv),
+                                                      float(toi_tolerance), int(bool(per_query)), ctypes.byref(best),
+                                                      _p(toi, _c_double_p), _p(hit, _c_int8_p)))
         return best.value, toi[:n], hit[:n]
 
     def broad_phase(self, poses_t0, poses_t1=None, inflation_radius=0.0):
