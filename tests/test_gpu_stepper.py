@@ -38,6 +38,9 @@ CASES = {
     "config1": (lambda: S.config1(seed=0), 10, None),
     "sliding_cube": (lambda: resting_scene(mu=0.5, v=(1.0, 0.0, 0.0)), 6, 1e-7),
     "drop": (drop_scene, 8, None),
+    # the differentiation scene's forward rollout: 64 spheres dropped on
+    # the plane, mu 0.5, its 20 steps
+    "config4": (lambda: S.config4(0), 20, None),
 }
 
 
