@@ -883,22 +883,35 @@ def batch_compare(n_scenes, steps, warm=True):
         w.step(1)
         w.close()
     out = dict(scenes=n_scenes, steps=steps, bodies=int(sum(len(sc.group_ids) for sc in scenes)))
+    iters = None
     for name, mode in (("fused", M.BATCH_FUSED), ("plain", M.BATCH_SEQUENTIAL)):
         b = M.Batch(scenes, mode=mode)
         t0 = time.perf_counter()
-        for k in range(steps):
-            t = time.perf_counter()
-            b.step(1)
-            log(f"batch {name} step {k + 1}/{steps}: {time.perf_counter() - t:.2f} s")
+        if mode == M.BATCH_FUSED:
+            # one call: every scene runs its `steps` steps on its own thread,
+            # so a scene that is slow in one step does not hold the others
+            # at the step boundary
+            b.step(steps)
+            log(f"batch {name}: {steps} steps of every scene in {time.perf_counter() - t0:.2f} s")
+        else:
+            for k in range(steps):
+                t = time.perf_counter()
+                b.step(1)
+                log(f"batch {name} step {k + 1}/{steps}: {time.perf_counter() - t:.2f} s")
         wall = time.perf_counter() - t0
-        newton = sum(b.stats(i)["newton_iterations"] for i in range(n_scenes))
-        out[name] = dict(scene_steps_per_s=n_scenes * steps / wall, wall_s=wall, newton_iterations=int(newton))
+        it = np.array([b.stats(i)["newton_iterations"] for i in range(n_scenes)], dtype=np.int64)
+        out[name] = dict(scene_steps_per_s=n_scenes * steps / wall, wall_s=wall, newton_iterations=int(it.sum()))
         if mode == M.BATCH_FUSED:
             bs = b.batch_stats()
             out[name].update(rounds=int(bs["rounds"]), fused_calls=int(bs["fused_calls"]),
                              requests=int(bs["requests"]))
+            iters = it
         b.close()
     out["fused_over_plain"] = out["fused"]["scene_steps_per_s"] / out["plain"]["scene_steps_per_s"]
+    # Newton iterations per scene over all the steps: the fused run needs at
+    # least max(iters) rounds, the plain run does sum(iters) iterations
+    out["newton_iterations_per_scene"] = dict(max=int(iters.max()), median=float(np.median(iters)),
+                                              sum_over_max=float(iters.sum() / max(1, iters.max())))
     return out
 
 
