@@ -890,7 +890,9 @@ def batch_compare(n_scenes, steps, warm=True):
         if mode == M.BATCH_FUSED:
             # one call: every scene runs its `steps` steps on its own thread,
             # so a scene that is slow in one step does not hold the others
-            # at the step boundary
+            # at the step boundary (RIGS_BATCH_LOG: progress lines as scenes
+            # finish)
+            os.environ.setdefault("RIGS_BATCH_LOG", "1")
             b.step(steps)
             log(f"batch {name}: {steps} steps of every scene in {time.perf_counter() - t0:.2f} s")
         else:
@@ -904,7 +906,12 @@ def batch_compare(n_scenes, steps, warm=True):
         if mode == M.BATCH_FUSED:
             bs = b.batch_stats()
             out[name].update(rounds=int(bs["rounds"]), fused_calls=int(bs["fused_calls"]),
-                             requests=int(bs["requests"]))
+                             requests=int(bs["requests"]), executors=int(bs["executors"]),
+                             device_ms_all_executors=bs["device_ms"],
+                             autotuned=dict(ccd_walk_budget=int(bs["ccd_walk_budget"]),
+                                            ccd_spec_levels=int(bs["ccd_spec_levels"]),
+                                            ccd_window_factor=bs["ccd_window_factor"],
+                                            sample_device_ms=bs["autotune_ms"]))
             iters = it
         b.close()
     out["fused_over_plain"] = out["fused"]["scene_steps_per_s"] / out["plain"]["scene_steps_per_s"]

@@ -286,6 +286,39 @@ rigc_status rigc_friction_adjoint(rigc_ctx* ctx, const double* poses, const doub
 /* Per-scene rigc_compute_earliest_toi: toi[i] for scene scene_ids[i]. */
 rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* poses_t0,
                                     const double* poses_t1, double minimum_separation_distance, double* toi);
+/* Tunables of the rigid CCD narrow phase. No counterpart in the reference
+ * (its CCD is sequential per query); none of them changes a result, only
+ * the schedule of the device's search. */
+typedef struct rigc_tuning {
+    int32_t ccd_walk_budget;   /* boxes a lane walks before its open subtrees go
+                                * to the next pass (0: never); default 4096 */
+    int32_t ccd_spec_levels;   /* levels below a box that idle lanes evaluate
+                                * ahead of its owner (0..5); default 5 */
+    double ccd_window_factor;  /* first pass of an earliest-TOI call bounded by
+                                * this times the scene's previous answer (0: no
+                                * window); default 4 */
+} rigc_tuning;
+rigc_status rigc_get_tuning(const rigc_ctx* ctx, rigc_tuning* tuning);
+rigc_status rigc_set_tuning(rigc_ctx* ctx, const rigc_tuning* tuning);
+/* Autotuner: coordinate descent over the tunings (speculative levels, walk
+ * budget, window factor, three values each) on num_samples earliest-TOI
+ * calls (poses_t0 / poses_t1: [num_samples][num_bodies][6], num_bodies of
+ * the whole context; each call as rigc_compute_earliest_toi, in order, so a
+ * line-search-like sequence exercises the window), median of three timed
+ * runs per tuning; a knob changes only for a value at least 5% faster.
+ * Checks that every tuning returns the same TOIs bit for bit
+ * (RIGC_ERR_DEVICE otherwise) and sets the choice on the context.
+ * best / best_ms (optional): the tuning chosen and its median device time
+ * over the sample. */
+rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const double* poses_t0,
+                                       const double* poses_t1, double minimum_separation_distance,
+                                       rigc_tuning* best, double* best_ms);
+
+/* Box evaluations of the rigid CCD per scene in the last earliest-TOI call
+ * of a multi-scene context (0 for a scene the call did not serve). No
+ * counterpart in the reference: it tells a batch caller which scenes'
+ * interval searches made a fused call's work. */
+rigc_status rigc_batch_ccd_evaluations(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, int64_t* evals);
 /* Per-scene rigc_compute_barrier_term with the constraint set built at
  * build_poses (DistanceBarrierConstraint::construct_constraint_set) and the
  * potential evaluated at poses; energy[i] per scene; counts (optional,

@@ -48,6 +48,8 @@ struct rigc_ctx {
     unsigned long long pose_tag[3] = { 0, 0, 0 }, tag_counter = 0;
     std::vector<double> h_last[3];
     DevBuf scene_best, scene_skip, scene_par;
+    DevBuf scene_evals;                       // box evaluations per scene of the last earliest-TOI call (multi-scene contexts)
+    std::vector<unsigned long long> h_scene_evals;
     // page-locked staging: poses (uploads), per-scene parameters, the
     // earliest-TOI bounds and results, and potential outputs; a staging
     // buffer is rewritten only after ev_stage, recorded behind the last
@@ -59,7 +61,10 @@ struct rigc_ctx {
     Assembly as;
     CcdWork ccdw;
     Intersections ix;
+    // rigid CCD tunables (rigc_tuning; results do not depend on them)
     int walk_budget = 4096;
+    int spec_levels = 5;
+    double window_factor = 4.0;
     std::vector<double> last_toi;  // per scene: answer of its previous earliest-TOI call
     hipEvent_t ev2 = nullptr;
     unsigned char* scratch = nullptr;  // this device's no-allocation array (rigc_narrow_phase_noalloc)
@@ -338,7 +343,7 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     free_scene(ctx);
     for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->poses2, &ctx->stage0, &ctx->stage1, &ctx->out_rows, &ctx->ee, &ctx->fv,
                        &ctx->toi, &ctx->hit, &ctx->counters, &ctx->qbest, &ctx->qvisits, &ctx->qflag,
-                       &ctx->scene_best, &ctx->scene_skip, &ctx->scene_par, &ctx->sel_all.buf, &ctx->sel_req.buf,
+                       &ctx->scene_best, &ctx->scene_skip, &ctx->scene_par, &ctx->scene_evals, &ctx->sel_all.buf, &ctx->sel_req.buf,
                        &ctx->pl_arena, &ctx->pl_ids })
         b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -455,6 +460,7 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     const int grid = ctx->num_cus * CCD_BLOCKS_PER_CU;  // persistent, all blocks resident
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
     a.walk_budget = ctx->walk_budget;
+    a.spec_levels = ctx->spec_levels;
     if (nq) RIGC_HIP(launch_ccd(a, grid, block, st, ctx->ccdw));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
 
@@ -541,6 +547,7 @@ rigc_status rigc_narrow_phase_noalloc(rigc_ctx* ctx, const double* poses_t0, con
     a.fv = (const int*)(base + L.fv);
     a.n_ee = n_ee;
     a.n_fv = n_fv;
+    a.spec_levels = ctx->spec_levels;
     a.toi_tol = toi_tolerance;
     a.eval_budget = 0;
     a.use_global_best = per_query ? 0 : 1;
@@ -675,14 +682,15 @@ CcdArgs ccd_args(rigc_ctx* ctx) {
     a.body_scene = d.body_scene;
     a.counter = ctx->counters.as<unsigned long long>() + 1;
     a.total_evals = ctx->counters.as<unsigned long long>() + 2;
+    a.spec_levels = ctx->spec_levels;
     return a;
 }
 
 // Earliest TOI of each selected scene (poses already uploaded): broad phase
 // over the selection, then the narrow phase with a running minimum per
 // scene. Windowed search: a first pass starts each scene's minimum at a
-// bound T_s (four times that scene's previous answer; successive
-// line-search calls return similar times). It prunes every box with
+// bound T_s (window_factor, 4 unless tuned, times that scene's previous
+// answer; successive line-search calls return similar times). It prunes every box with
 // t.lower >= T_s, so if it accepts a box of scene s the minimum below T_s
 // is s's answer; the scenes where it accepts none are searched again in
 // full, with the queries of every other scene skipped. Either way each
@@ -716,14 +724,22 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
     a.q_flag = ctx->qflag.as<int>();
     const int grid = ctx->num_cus * CCD_BLOCKS_PER_CU;  // persistent, all blocks resident
     a.walk_budget = ctx->walk_budget;
+    // a multi-scene context counts each scene's box evaluations (what a
+    // batch scheduler uses to keep deep searches out of shared calls)
+    const bool count_scenes = ns > 1;
+    if (count_scenes) {
+        RIGC_HIP(ctx->scene_evals.ensure(sizeof(unsigned long long) * ns));
+        RIGC_HIP(hipMemsetAsync(ctx->scene_evals.p, 0, sizeof(unsigned long long) * ns, s));
+        a.scene_evals = ctx->scene_evals.as<unsigned long long>();
+    }
 
     const double inf = HUGE_VAL;
     std::vector<double> window(sel.n_scenes, inf);
     bool any_window = false;
     for (int i = 0; i < sel.n_scenes; i++) {
         const double lt = ctx->last_toi[sel.h_scenes[i]];
-        if (lt < 0.25) {
-            window[i] = std::max(4.0 * lt, std::ldexp(1.0, -20));
+        if (ctx->window_factor > 0 && lt * ctx->window_factor < 1.0) {
+            window[i] = std::max(ctx->window_factor * lt, std::ldexp(1.0, -20));
             any_window = true;
         }
     }
@@ -793,6 +809,11 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
         (void)any_window;
     }
     RIGC_HIP(hipEventRecord(ctx->ev2, s));
+    if (count_scenes) {
+        ctx->h_scene_evals.assign(ns, 0ULL);
+        RIGC_HIP(hipMemcpyAsync(ctx->h_scene_evals.data(), ctx->scene_evals.p, sizeof(unsigned long long) * ns,
+                                hipMemcpyDeviceToHost, s));
+    }
     RIGC_HIP(hipStreamSynchronize(s));
     float bms = 0, nms = 0;
     RIGC_HIP(hipEventElapsedTime(&bms, ctx->ev0, ctx->ev1));
@@ -1198,6 +1219,118 @@ rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* sce
     s = earliest_toi_sel(ctx, ctx->sel_req, minimum_separation_distance, out);
     if (s != RIGC_OK) return s;
     std::memcpy(toi, out.data(), sizeof(double) * n);
+    return RIGC_OK;
+}
+
+rigc_status rigc_get_tuning(const rigc_ctx* ctx, rigc_tuning* t) {
+    if (!ctx || !t) return RIGC_ERR_INVALID_ARGUMENT;
+    t->ccd_walk_budget = ctx->walk_budget;
+    t->ccd_spec_levels = ctx->spec_levels;
+    t->ccd_window_factor = ctx->window_factor;
+    return RIGC_OK;
+}
+
+rigc_status rigc_set_tuning(rigc_ctx* ctx, const rigc_tuning* t) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!t || t->ccd_walk_budget < 0 || t->ccd_spec_levels < 0 || t->ccd_spec_levels > 5 ||
+        !(t->ccd_window_factor >= 0) || !std::isfinite(t->ccd_window_factor))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_tuning: bad tuning");
+    ctx->walk_budget = t->ccd_walk_budget;
+    ctx->spec_levels = t->ccd_spec_levels;
+    ctx->window_factor = t->ccd_window_factor;
+    return RIGC_OK;
+}
+
+// Coordinate descent over the CCD tunables on the caller's sample calls:
+// for each knob in turn (speculative levels, walk budget, window factor),
+// every value of its grid runs the whole sample (from the same per-scene
+// window state) three times with the other knobs at their current choice,
+// and the median device time (broad + narrow phase) decides; a value
+// replaces the current one only when it is at least 5% faster (smaller
+// differences are the timing's noise). Every run must return the first
+// run's TOIs bit for bit. The choice is set on the context.
+rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const double* poses_t0,
+                                       const double* poses_t1, double minimum_separation_distance, rigc_tuning* best,
+                                       double* best_ms) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_autotune_earliest_toi: no scene");
+    if (num_samples < 1 || !poses_t0 || !poses_t1 || !(minimum_separation_distance >= 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_autotune_earliest_toi: bad arguments");
+    const size_t stride = 6 * (size_t)ctx->scene.nb;
+    const std::vector<double> window0 = ctx->last_toi;
+    const rigc_tuning start{ ctx->walk_budget, ctx->spec_levels, ctx->window_factor };
+    std::vector<double> ref, got(num_samples);
+    auto restore = [&](const rigc_tuning& t) {
+        ctx->walk_budget = t.ccd_walk_budget;
+        ctx->spec_levels = t.ccd_spec_levels;
+        ctx->window_factor = t.ccd_window_factor;
+        ctx->last_toi = window0;
+    };
+    // median device time of the sample under t (3 runs); RIGC_OK or an error
+    auto time_of = [&](const rigc_tuning& t, double& med) -> rigc_status {
+        double ms[3];
+        for (int rep = 0; rep < 3; rep++) {
+            restore(t);
+            ms[rep] = 0;
+            for (int i = 0; i < num_samples; i++) {
+                const rigc_status st = rigc_compute_earliest_toi(ctx, poses_t0 + stride * i, poses_t1 + stride * i,
+                                                                 minimum_separation_distance, &got[i]);
+                if (st != RIGC_OK) return st;
+                ms[rep] += ctx->stats.broad_phase_ms + ctx->stats.narrow_phase_ms;
+            }
+            if (ref.empty()) ref = got;
+            if (std::memcmp(ref.data(), got.data(), sizeof(double) * num_samples) != 0)
+                return ctx->fail(RIGC_ERR_DEVICE, "rigc_autotune_earliest_toi: a tuning changed a result");
+        }
+        std::sort(ms, ms + 3);
+        med = ms[1];
+        return RIGC_OK;
+    };
+    rigc_tuning cur = start;
+    double cur_ms = 0;
+    rigc_status st = time_of(cur, cur_ms);
+    for (int knob = 0; knob < 3 && st == RIGC_OK; knob++) {
+        const double grid[3][3] = { { 3, 4, 5 }, { 1024, 4096, 16384 }, { 0.0, 4.0, 16.0 } };
+        rigc_tuning pick = cur;
+        double pick_ms = cur_ms;
+        for (int g = 0; g < 3 && st == RIGC_OK; g++) {
+            rigc_tuning t = cur;
+            if (knob == 0) t.ccd_spec_levels = (int32_t)grid[0][g];
+            else if (knob == 1) t.ccd_walk_budget = (int32_t)grid[1][g];
+            else t.ccd_window_factor = grid[2][g];
+            if (t.ccd_spec_levels == cur.ccd_spec_levels && t.ccd_walk_budget == cur.ccd_walk_budget &&
+                t.ccd_window_factor == cur.ccd_window_factor)
+                continue;
+            double ms = 0;
+            st = time_of(t, ms);
+            if (st == RIGC_OK && ms < 0.95 * pick_ms) {
+                pick = t;
+                pick_ms = ms;
+            }
+        }
+        cur = pick;
+        cur_ms = pick_ms;
+    }
+    if (st != RIGC_OK) {
+        restore(start);
+        return st;
+    }
+    restore(cur);
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return RIGC_OK;
+}
+
+rigc_status rigc_batch_ccd_evaluations(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, int64_t* evals) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (n < 0 || (n > 0 && (!scene_ids || !evals)))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_ccd_evaluations: bad arguments");
+    for (int i = 0; i < n; i++) {
+        const int k = scene_ids[i];
+        if (k < 0 || k >= ctx->scene.n_scenes)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_ccd_evaluations: scene id out of range");
+        evals[i] = k < (int)ctx->h_scene_evals.size() ? (int64_t)ctx->h_scene_evals[k] : 0;
+    }
     return RIGC_OK;
 }
 

@@ -48,7 +48,7 @@ namespace {
 
 constexpr int MAX_DEPTH = 192;
 constexpr int THR_REFRESH = 8;
-constexpr int SPEC_LEVELS = 5;  // speculative levels below an owner's box (62 boxes)
+constexpr int SPEC_LEVELS = 5;  // most speculative levels below an owner's box (62 boxes: one 64-bit mask)
 
 struct Bits192 {
     unsigned long long w[3];
@@ -367,6 +367,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     Query q;
     double tol[3];
     long long evals = 0;
+    int ev_sc = -1;              // scene of the evaluations counted in ev_n
+    unsigned long long ev_n = 0;
     int walk = 0;       // evaluations since this lane took its subtree
     double thr = 0.0;   // cached pruning bound of query thr_q
     long long thr_q = -1;
@@ -498,7 +500,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         }
         // C. one box per lane of its own query. Lanes left without work
         // evaluate, for lanes evaluating a bisection box, the descendants of
-        // that box down to SPEC_LEVELS levels (each box of the subtree is a
+        // that box down to spec_levels levels (each box of the subtree is a
         // fixed function of its root: the split dimension depends only on
         // the box's widths); the owner then walks those levels from the
         // recorded results without waiting for further evaluations. Which
@@ -621,7 +623,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         const int n_own = __popcll(owners), n_idle = __popcll(idle);
         const int g = n_own ? n_idle / n_own : 0;
         int levels = 0;
-        while (levels < SPEC_LEVELS && (2 << (levels + 1)) - 2 <= g) levels++;
+        const int max_levels = min(a.spec_levels, SPEC_LEVELS);
+        while (levels < max_levels && (2 << (levels + 1)) - 2 <= g) levels++;
         const int per = levels ? (2 << levels) - 2 : 0;
         int src = lane, node = 0;
         if (per && qi < 0) {
@@ -675,6 +678,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         if (evaluating) {
             z = contains_zero(q.kind, A0, DA, B0, DB, u, v);
             evals++;
+            if (a.scene_evals) {
+                // per-scene counts: accumulated in the lane, added to the
+                // scene's counter when the lane moves to another scene
+                if (q.sc != ev_sc) {
+                    if (ev_n) atomicAdd(a.scene_evals + ev_sc, ev_n);
+                    ev_sc = q.sc;
+                    ev_n = 0;
+                }
+                ev_n++;
+            }
         }
         if (spec) cache_ok = false;
         const unsigned long long zmask = __ballot(spec && z);
@@ -802,6 +815,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         if (done) qi = -1;
     }
     if (evals) atomicAdd(a.total_evals, (unsigned long long)evals);
+    if (ev_n) atomicAdd(a.scene_evals + ev_sc, ev_n);
 }
 
 }  // namespace

@@ -52,6 +52,7 @@ struct Lane {
     double* heap;
     long long hcap;
     unsigned long long boxes, pieces;
+    bool wave;  // the whole wavefront runs this query (uniform control flow)
 };
 
 // Tight-Inclusion's queue order: comp(a, b) = a is served after b.
@@ -143,6 +144,40 @@ RIGC_DI bool origin_in_box(const double (*V)[3], bool vf, const double* b, const
     return true;
 }
 
+// origin_in_box with the wavefront's lanes sharing the 24 function values:
+// lane l takes value j = l mod 24 (axis j / 8, corner j mod 8), so every
+// aligned group of 8 lanes holds one axis's corners; a butterfly over each
+// group gives that axis's minimum and maximum (fmin / fmax are exact and
+// order-free), and lanes 0, 8 and 16 hand them to all. Same answer as the
+// loop above. Every lane of the wavefront must call it with the same box.
+RIGC_DI bool origin_in_box_wave(const double (*V)[3], bool vf, const double* b, const double err[3], double ms,
+                                bool& box_in, double& true_tol) {
+    const int j = (int)(threadIdx.x & 63) % 24;
+    const int kk = j >> 3, c = j & 7;
+    const double f = ti_function(V, vf, kk, (c & 4) ? b[1] : b[0], (c & 2) ? b[3] : b[2], (c & 1) ? b[5] : b[4]);
+    double mn = f, mx = f;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        mn = fmin(mn, __shfl_xor(mn, o));
+        mx = fmax(mx, __shfl_xor(mx, o));
+    }
+    double amn[3], amx[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        amn[k] = __shfl(mn, 8 * k);
+        amx[k] = __shfl(mx, 8 * k);
+    }
+    box_in = true;
+    true_tol = 0;
+    for (int k = 0; k < 3; k++) {
+        const double eps_ms = err[k] + ms;
+        if (amn[k] > eps_ms || amx[k] < -eps_ms) return false;
+        if (!(amn[k] >= -eps_ms && amx[k] <= eps_ms)) box_in = false;
+        true_tol = fmax(true_tol, amx[k] - amn[k]);
+    }
+    return true;
+}
+
 RIGC_DI double max_linf_4(const double (*p)[3], int a0, int a1, int a2, int a3, int b0, int b1, int b2, int b3) {
     const int A[4] = { a0, a1, a2, a3 }, B[4] = { b0, b1, b2, b3 };
     double m = 0;
@@ -211,7 +246,9 @@ RIGC_DI int ti_ccd(Lane& L, const double (*V)[3], bool vf, double ms, double& to
         L.boxes++;
         bool box_in;
         double true_tol;
-        if (!origin_in_box(V, vf, cur, err, ms, box_in, true_tol)) continue;
+        const bool in = L.wave ? origin_in_box_wave(V, vf, cur, err, ms, box_in, true_tol)
+                               : origin_in_box(V, vf, cur, err, ms, box_in, true_tol);
+        if (!in) continue;
         const double w0 = cur[1] - cur[0], w1 = cur[3] - cur[2], w2 = cur[5] - cur[4];
         const bool tol_condition = true_tol <= co_tol;
         const bool condition1 = w0 <= tol[0] && w1 <= tol[1] && w2 <= tol[2];
@@ -431,6 +468,7 @@ __global__ __launch_bounds__(256) void plccd_kernel(PlccdArgs a) {
     L.hcap = a.heap_cap;
     L.boxes = 0;
     L.pieces = 0;
+    L.wave = false;
     for (long long i = lane; i < a.n_queries; i += lanes) run_query(a, L, i, true);
     if (L.boxes) atomicAdd(a.counters + 0, L.boxes);
     if (L.pieces) atomicAdd(a.counters + 1, L.pieces);
@@ -438,7 +476,8 @@ __global__ __launch_bounds__(256) void plccd_kernel(PlccdArgs a) {
 
 // One query per wavefront, piece stack and heap in LDS: every lane runs the
 // same query on the same LDS data (uniform control flow, identical values),
-// lane 0 writes the result. A heap operation then waits on LDS instead of
+// the lanes share each box's 24 function values (origin_in_box_wave), lane
+// 0 writes the result. A heap operation then waits on LDS instead of
 // HBM; queries that outgrow the LDS heap go to the arena passes.
 constexpr int kWaveStack = 64;
 constexpr int kWaveHeap = 512;
@@ -452,6 +491,7 @@ __global__ __launch_bounds__(64) void plccd_wave_kernel(PlccdArgs a) {
     L.hcap = kWaveHeap;
     L.boxes = 0;
     L.pieces = 0;
+    L.wave = true;
     const bool writer = threadIdx.x == 0;
     for (long long i = blockIdx.x; i < a.n_queries; i += gridDim.x) run_query(a, L, i, writer);
     if (writer && L.boxes) atomicAdd(a.counters + 0, L.boxes);

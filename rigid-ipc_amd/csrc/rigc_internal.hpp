@@ -212,6 +212,8 @@ struct CcdArgs {
     unsigned long long* n_tasks_out;
     long long tasks_cap;
     int walk_budget;             // evaluations after which a lane exports its open subtrees
+    unsigned long long* scene_evals;  // optional [n_scenes]: box evaluations per scene
+    int spec_levels;             // speculative levels below an owner's box (0..5; 0 = none)
 };
 
 // Task buffers of the multi-pass narrow phase (grow-only).

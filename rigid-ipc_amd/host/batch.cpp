@@ -2,24 +2,36 @@
 //
 // Fused mode: every scene runs the host stepper (simulation.cpp, unchanged)
 // on a thread of its own, over a proxy contact backend. A proxy call parks
-// its thread with a request; once every scene still stepping has parked, the
-// last one to arrive executes all pending requests as fused device calls on
-// one multi-scene context (one rigc_batch_* call per kind of request and
-// argument set: earliest TOI, barrier, minimum distance, friction update,
-// friction potential), hands each scene its part of the results and wakes
-// the others. Each scene's Newton loop therefore proceeds independently
-// (different iteration counts, line-search lengths and step ends), while
-// the device sees one launch sequence per round for all scenes that are
-// waiting on it. Per-scene results are bit-identical to a single-scene
-// context, so every scene follows exactly the trajectory it follows alone.
+// its thread with a request. Executor threads, each with its own
+// multi-scene device context (and so its own HIP stream), take the waiting
+// requests and execute them as fused device calls (one rigc_batch_* call
+// per kind of request and argument set: earliest TOI, barrier, minimum
+// distance, friction update, friction potential), hand each scene its part
+// of the results and wake the scenes. An executor takes the requests once
+// every scene still stepping is parked (waiting, or served by another
+// executor), or once the oldest waiting request has waited a gather window.
+// So a fused call that runs long (a scene's deep bisection) holds only the
+// executor serving it: the requests that come meanwhile go to another
+// executor, on another stream, and their kernels run beside it on the idle
+// part of the device. The friction kinds go to one executor of their own, whose context
+// holds the lagged friction sets; every other kind of request carries all
+// its inputs and may go to any general executor. Each scene's Newton loop
+// proceeds independently (different iteration counts, line-search lengths
+// and step ends). Per-scene results are bit-identical to a single-scene
+// context, whatever the grouping and whichever context serves a call, so
+// every scene follows exactly the trajectory it follows alone.
 //
 // Sequential mode: the plain path, one scene after another, each with its
 // own single-scene context (the comparison point for the fused mode).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -62,7 +74,10 @@ struct Request {
     rigc_status status = RIGC_OK;
     std::string error;
     bool done = false;
+    double t_submit = 0;         // when it was queued (ms)
 };
+
+bool is_friction_kind(int kind) { return kind == REQ_FUPDATE || kind == REQ_FRICTION; }
 
 struct Batch;
 
@@ -196,45 +211,65 @@ rigc_status px_friction(void* s, const double* poses, const double* poses_t0, do
 const char* px_error(void* s) { return static_cast<Proxy*>(s)->err.c_str(); }
 
 // ------------------------------------------------------------ combiner
-struct Batch {
+// One executor: a device context holding every scene, the thread that makes
+// its fused calls, and that thread's staging.
+struct Exec {
+    Batch* B = nullptr;
     rigc_ctx* ctx = nullptr;
-    int n = 0;
-    std::vector<int> nb;
-    std::vector<Proxy> proxies;
-    std::mutex m;
-    std::condition_variable cv;
-    std::vector<Request*> pending;
-    int active = 0;
-    rigs_batch_stats st{};
-    // fused-call staging
+    bool friction = false;  // serves the friction kinds
+    std::thread th;
     std::vector<int32_t> ids;
     std::vector<double> c0, c1, vals, kap, mus, grad, diag, off;
     std::vector<int32_t> pairs;
     std::vector<int64_t> cnt;
     int64_t cap = 0;
+    rigs_batch_stats st{};  // this executor's share of the batch statistics
 
-    void execute();
+    void loop();
+    void execute(std::vector<Request*>& reqs);
     void run_group(int kind, std::vector<Request*>& g);
+};
+
+struct Batch {
+    int n = 0;
+    std::vector<int> nb;
+    std::vector<Proxy> proxies;
+    std::vector<std::unique_ptr<Exec>> execs;  // general executors, then the friction executor (if any)
+    bool has_friction_exec = false;
+    std::mutex m;
+    std::condition_variable cv;       // callers: a request is done
+    std::condition_variable cv_exec;  // executors: a request came, a caller left, or stop
+    std::vector<Request*> pending;
+    int in_flight = 0;  // requests taken by an executor and not yet done
+    int active = 0;     // callers taking part (stepping scenes and their helpers)
+    bool stop = false;
+    double gather_ms = 0.2;
+    rigc_tuning tuning{};  // what the autotuner chose (zeros when it did not run)
+    double tune_ms = 0;    // its device time over the sample
+
+    bool serves(const Exec& e, const Request* r) const {
+        return has_friction_exec ? (is_friction_kind(r->kind) == e.friction) : true;
+    }
+    void start();
+    void shutdown();
 };
 
 rigc_status submit(Proxy* P, Request& r) {
     Batch& B = *P->B;
     r.scene = P->scene;
     std::unique_lock<std::mutex> lk(B.m);
+    r.t_submit = now_ms();
     B.pending.push_back(&r);
-    if ((int)B.pending.size() == B.active) {
-        B.execute();
-        B.cv.notify_all();
-    } else {
-        B.cv.wait(lk, [&] { return r.done; });
-    }
+    B.cv_exec.notify_all();
+    B.cv.wait(lk, [&] { return r.done; });
     if (r.status != RIGC_OK) P->err = r.error;
     return r.status;
 }
 
 // Intersection check: in a round like the other calls while the scenes
 // step; outside stepping (the check of the initial state when a scene's
-// stepper is created, before any round can run) a direct one-scene call.
+// stepper is created, before any round can run) a direct one-scene call on
+// the first executor's context, which no executor uses meanwhile.
 rigc_status px_isect(void* s, const double* poses, int64_t* n) {
     Proxy* P = static_cast<Proxy*>(s);
     Batch& B = *P->B;
@@ -242,8 +277,9 @@ rigc_status px_isect(void* s, const double* poses, int64_t* n) {
         std::lock_guard<std::mutex> lk(B.m);
         if (B.active == 0) {
             const int32_t id = P->scene;
-            const rigc_status st = rigc_batch_detect_intersections(B.ctx, 1, &id, poses, n);
-            if (st != RIGC_OK) P->err = rigc_last_error(B.ctx);
+            rigc_ctx* ctx = B.execs[0]->ctx;
+            const rigc_status st = rigc_batch_detect_intersections(ctx, 1, &id, poses, n);
+            if (st != RIGC_OK) P->err = rigc_last_error(ctx);
             return st;
         }
     }
@@ -257,17 +293,14 @@ rigc_status px_isect(void* s, const double* poses, int64_t* n) {
 // A scene finished stepping (or failed), or one of its two callers is done
 // or waiting for the other: one caller fewer takes part in rounds.
 void leave(Batch& B) {
-    std::unique_lock<std::mutex> lk(B.m);
+    std::lock_guard<std::mutex> lk(B.m);
     B.active--;
-    if (B.active > 0 && (int)B.pending.size() == B.active) {
-        B.execute();
-        B.cv.notify_all();
-    }
+    B.cv_exec.notify_all();
 }
 
 // A stepper's friction helper is about to call beside the stepper's own
-// thread (or that thread resumes): one caller more, so that no round runs
-// without its request.
+// thread (or that thread resumes): one caller more, so that a round does
+// not start without its request unless the gather window has passed.
 void fork_join(void* arg) {
     Batch& B = *static_cast<Batch*>(arg);
     std::lock_guard<std::mutex> lk(B.m);
@@ -275,11 +308,50 @@ void fork_join(void* arg) {
 }
 void fork_part(void* arg) { leave(*static_cast<Batch*>(arg)); }
 
-void Batch::execute() {
-    const double t0 = now_ms();
-    std::vector<Request*> reqs;
-    reqs.swap(pending);
-    // group by kind and by the scalar arguments every scene of a fused call shares
+// Executor thread: take this executor's kind of waiting requests once every
+// caller is parked or the oldest has waited the gather window, execute them
+// outside the lock, and wake each group's callers as soon as its call
+// returns.
+void Exec::loop() {
+    Batch& b = *B;
+    std::unique_lock<std::mutex> lk(b.m);
+    for (;;) {
+        double oldest = -1;
+        for (const Request* r : b.pending)
+            if (b.serves(*this, r) && (oldest < 0 || r->t_submit < oldest)) oldest = r->t_submit;
+        if (oldest < 0) {
+            if (b.stop) return;
+            b.cv_exec.wait(lk);
+            continue;
+        }
+        const bool all_parked = (int)b.pending.size() + b.in_flight >= b.active;
+        if (!all_parked && !b.stop && b.gather_ms != 0) {
+            if (b.gather_ms < 0) {
+                b.cv_exec.wait(lk);
+                continue;
+            }
+            const double wait_ms = oldest + b.gather_ms - now_ms();
+            if (wait_ms > 0) {
+                b.cv_exec.wait_for(lk, std::chrono::duration<double, std::milli>(wait_ms));
+                continue;  // look again: another executor may have taken them
+            }
+        }
+        std::vector<Request*> reqs, keep;
+        for (Request* r : b.pending) (b.serves(*this, r) ? reqs : keep).push_back(r);
+        b.pending.swap(keep);
+        b.in_flight += (int)reqs.size();
+        lk.unlock();
+        const double t0 = now_ms();
+        execute(reqs);
+        st.device_ms += now_ms() - t0;
+        st.rounds++;
+        lk.lock();
+    }
+}
+
+// Fused calls for reqs, one per kind and shared scalar arguments; each
+// group's callers are woken when its call returns.
+void Exec::execute(std::vector<Request*>& reqs) {
     std::map<std::tuple<int, int32_t, double, double>, std::vector<Request*>> groups;
     for (Request* r : reqs) {
         const double k1 = r->kind == REQ_TOI ? r->a : (r->kind == REQ_FRICTION ? 0.0 : r->a);
@@ -292,10 +364,31 @@ void Batch::execute() {
         run_group(std::get<0>(kv.first), g);
         st.fused_calls++;
         st.requests += (int64_t)g.size();
+        std::lock_guard<std::mutex> lk(B->m);
+        B->in_flight -= (int)g.size();
+        for (Request* r : g) r->done = true;
+        B->cv.notify_all();
+        B->cv_exec.notify_all();
     }
-    for (Request* r : reqs) r->done = true;
-    st.rounds++;
-    st.device_ms += now_ms() - t0;
+}
+
+void Batch::start() {
+    for (auto& e : execs) e->th = std::thread([p = e.get()] { p->loop(); });
+}
+
+void Batch::shutdown() {
+    {
+        std::lock_guard<std::mutex> lk(m);
+        stop = true;
+    }
+    cv_exec.notify_all();
+    for (auto& e : execs)
+        if (e->th.joinable()) e->th.join();
+    for (auto& e : execs)
+        if (e->ctx) {
+            rigc_destroy(e->ctx);
+            e->ctx = nullptr;
+        }
 }
 
 void fail_all(std::vector<Request*>& g, rigc_status s, const char* msg) {
@@ -305,20 +398,20 @@ void fail_all(std::vector<Request*>& g, rigc_status s, const char* msg) {
     }
 }
 
-void Batch::run_group(int kind, std::vector<Request*>& g) {
+void Exec::run_group(int kind, std::vector<Request*>& g) {
     const int k = (int)g.size();
     ids.resize(k);
     std::vector<int> start(k + 1, 0);
     for (int i = 0; i < k; i++) {
         ids[i] = g[i]->scene;
-        start[i + 1] = start[i] + nb[g[i]->scene];
+        start[i + 1] = start[i] + B->nb[g[i]->scene];
     }
     const int tot = start[k];
     auto concat = [&](std::vector<double>& dst, bool second) {
         dst.resize(6 * (size_t)tot);
         for (int i = 0; i < k; i++) {
             const double* src = second ? g[i]->p1 : g[i]->p0;
-            std::memcpy(&dst[6 * (size_t)start[i]], src, sizeof(double) * 6 * nb[g[i]->scene]);
+            std::memcpy(&dst[6 * (size_t)start[i]], src, sizeof(double) * 6 * B->nb[g[i]->scene]);
         }
     };
     vals.assign(k, 0.0);
@@ -407,14 +500,14 @@ void Batch::run_group(int kind, std::vector<Request*>& g) {
                 Request* r = g[i];
                 *r->value = vals[i];
                 if (kind == REQ_BARRIER) {
-                    Proxy& P = proxies[r->scene];
+                    Proxy& P = B->proxies[r->scene];
                     for (int c = 0; c < 4; c++) P.counts[c] = cnt[4 * i + c];
                     if (r->mind) *r->mind = md[i];
                 }
                 if (want_grad && r->grad)
-                    std::memcpy(r->grad, &grad[6 * (size_t)start[i]], sizeof(double) * 6 * nb[r->scene]);
+                    std::memcpy(r->grad, &grad[6 * (size_t)start[i]], sizeof(double) * 6 * B->nb[r->scene]);
                 if (want_hess && r->hess) {
-                    std::memcpy(r->hess->diag, &diag[36 * (size_t)start[i]], sizeof(double) * 36 * nb[r->scene]);
+                    std::memcpy(r->hess->diag, &diag[36 * (size_t)start[i]], sizeof(double) * 36 * B->nb[r->scene]);
                     int64_t q1 = q;
                     while (q1 < h.count && pairs[2 * q1] < start[i + 1]) q1++;
                     const int64_t m = q1 - q;
@@ -471,11 +564,25 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
     if (mode == RIGS_BATCH_FUSED) {
         Batch& B = b->B;
         B.n = num_scenes;
-        s = rigc_create(&B.ctx, device);
-        if (s == RIGC_OK) s = rigc_set_scenes(B.ctx, num_scenes, scenes);
-        if (s != RIGC_OK) {
-            b->err = B.ctx ? rigc_last_error(B.ctx) : "rigc_create failed";
-        } else {
+        // general executors (RIGS_BATCH_EXECUTORS, default 3), plus one for
+        // the friction kinds when a scene has friction
+        int n_general = 3;
+        if (const char* e = std::getenv("RIGS_BATCH_EXECUTORS")) n_general = std::max(1, std::min(8, std::atoi(e)));
+        if (const char* g = std::getenv("RIGS_BATCH_GATHER_US")) B.gather_ms = std::atof(g) * 1e-3;
+        bool any_friction = false;
+        for (int i = 0; i < num_scenes; i++) any_friction = any_friction || params[i].coefficient_friction > 0;
+        B.has_friction_exec = any_friction;
+        const int n_exec = n_general + (any_friction ? 1 : 0);
+        for (int k = 0; k < n_exec && s == RIGC_OK; k++) {
+            B.execs.emplace_back(new Exec());
+            Exec& e = *B.execs.back();
+            e.B = &B;
+            e.friction = any_friction && k == n_general;
+            s = rigc_create(&e.ctx, device);
+            if (s == RIGC_OK) s = rigc_set_scenes(e.ctx, num_scenes, scenes);
+            if (s != RIGC_OK) b->err = e.ctx ? rigc_last_error(e.ctx) : "rigc_create failed";
+        }
+        if (s == RIGC_OK) {
             B.proxies.resize(num_scenes);
             B.nb.resize(num_scenes);
             b->tables.resize(num_scenes);
@@ -488,6 +595,41 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
                 b->tables[i] = rigs_backend{ &P, px_set_scene, px_toi, px_build, px_barrier, px_mind, px_fupdate,
                                              px_friction, px_error, nullptr, px_isect };
             }
+            // autotuner (RIGS_BATCH_AUTOTUNE, default on): the CCD tunables
+            // timed on a line-search-like sequence of earliest-TOI calls over
+            // all scenes (each body's start pose, moved at its initial
+            // velocity over four time steps, then over half and a quarter of
+            // that), then set on every executor's context
+            const char* at = std::getenv("RIGS_BATCH_AUTOTUNE");
+            if (!at || std::atoi(at) != 0) {
+                size_t tot = 0;
+                for (int i = 0; i < num_scenes; i++) tot += 6 * (size_t)scenes[i].num_bodies;
+                const int n_samples = 3;
+                std::vector<double> p0(n_samples * tot), p1(n_samples * tot);
+                for (int rep = 0; rep < n_samples; rep++) {
+                    size_t o = rep * tot;
+                    for (int i = 0; i < num_scenes; i++) {
+                        const double h = std::ldexp(4.0, -rep) * params[i].timestep;
+                        for (int j = 0; j < 6 * scenes[i].num_bodies; j++) {
+                            p0[o + j] = bodies[i].poses[j];
+                            p1[o + j] = bodies[i].poses[j] + h * bodies[i].velocities[j];
+                        }
+                        o += 6 * (size_t)scenes[i].num_bodies;
+                    }
+                }
+                rigc_tuning best{};
+                s = rigc_autotune_earliest_toi(B.execs[0]->ctx, n_samples, p0.data(), p1.data(), 0.0, &best, &B.tune_ms);
+                for (size_t k = 1; k < B.execs.size() && s == RIGC_OK; k++) s = rigc_set_tuning(B.execs[k]->ctx, &best);
+                if (s != RIGC_OK) b->err = std::string("autotuner: ") + rigc_last_error(B.execs[0]->ctx);
+                B.tuning = best;
+                const char* lg = std::getenv("RIGS_BATCH_LOG");
+                if (s == RIGC_OK && lg && std::atoi(lg) != 0) {
+                    std::fprintf(stderr, "batch: autotuned CCD walk budget %d, speculative levels %d, window %g (%.2f ms)\n",
+                                 best.ccd_walk_budget, best.ccd_spec_levels, best.ccd_window_factor, B.tune_ms);
+                    std::fflush(stderr);
+                }
+            }
+            if (s == RIGC_OK) B.start();
         }
     }
     for (int i = 0; i < num_scenes && s == RIGC_OK; i++) {
@@ -517,7 +659,7 @@ rigc_status rigs_batch_destroy(rigs_batch* b) {
     if (!b) return RIGC_OK;
     for (rigs_sim* s : b->sims)
         if (s) rigs_destroy(s);
-    if (b->B.ctx) rigc_destroy(b->B.ctx);
+    b->B.shutdown();
     delete b;
     return RIGC_OK;
 }
@@ -534,14 +676,35 @@ rigc_status rigs_batch_step(rigs_batch* b, int32_t num_steps) {
         }
     } else {
         Batch& B = b->B;
-        B.active = b->n;
+        {
+            std::lock_guard<std::mutex> lk(B.m);
+            B.active = b->n;
+        }
         std::vector<std::thread> threads;
         threads.reserve(b->n);
+        const char* lg = std::getenv("RIGS_BATCH_LOG");
+        const bool log = lg && std::atoi(lg) != 0;
+        const double t0 = now_ms();
+        int finished = 0;
+        std::mutex log_m;
         for (int i = 0; i < b->n; i++)
-            threads.emplace_back([b, i, num_steps, &res] {
+            threads.emplace_back([b, i, num_steps, &res, log, t0, &finished, &log_m] {
                 res[i] = rigs_step(b->sims[i], num_steps);
                 if (res[i] != RIGC_OK) b->errors[i] = rigs_last_error(b->sims[i]);
                 leave(b->B);
+                if (log) {
+                    // progress of a long fused run: a line each time another
+                    // sixteenth of the scenes has finished, and for the last
+                    std::lock_guard<std::mutex> lk(log_m);
+                    ++finished;
+                    if (finished % std::max(1, b->n / 16) == 0 || finished == b->n) {
+                        rigs_stats s{};
+                        rigs_get_stats(b->sims[i], &s);
+                        std::fprintf(stderr, "batch: %d/%d scenes done after %.2f s (last: scene %d, %lld Newton iterations)\n",
+                                     finished, b->n, 1e-3 * (now_ms() - t0), i, (long long)s.newton_iterations);
+                        std::fflush(stderr);
+                    }
+                }
             });
         for (auto& t : threads) t.join();
     }
@@ -565,7 +728,30 @@ rigc_status rigs_batch_get_stats(const rigs_batch* b, int32_t scene, rigs_stats*
 
 rigc_status rigs_batch_get_batch_stats(const rigs_batch* b, rigs_batch_stats* stats) {
     if (!b || !stats) return RIGC_ERR_INVALID_ARGUMENT;
-    *stats = b->B.st;
+    rigs_batch_stats t{};
+    for (const auto& e : b->B.execs) {
+        t.rounds += e->st.rounds;
+        t.fused_calls += e->st.fused_calls;
+        t.requests += e->st.requests;
+        t.device_ms += e->st.device_ms;
+        t.ccd_calls += e->st.ccd_calls;
+        t.ccd_candidates += e->st.ccd_candidates;
+        t.ccd_kernel_ms += e->st.ccd_kernel_ms;
+        t.ccd_launches += e->st.ccd_launches;
+        t.ccd_box_evaluations += e->st.ccd_box_evaluations;
+    }
+    t.executors = (int32_t)b->B.execs.size();
+    t.tuning = b->B.tuning;
+    t.autotune_ms = b->B.tune_ms;
+    *stats = t;
+    return RIGC_OK;
+}
+
+rigc_status rigs_batch_set_gather(rigs_batch* b, double gather_us) {
+    if (!b || std::isnan(gather_us)) return RIGC_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(b->B.m);
+    b->B.gather_ms = gather_us < 0 ? -1.0 : gather_us * 1e-3;
+    b->B.cv_exec.notify_all();
     return RIGC_OK;
 }
 

@@ -194,15 +194,19 @@ rigc_status rigs_get_stats(const rigs_sim* sim, rigs_stats* stats);
 enum { RIGS_BATCH_SEQUENTIAL = 0, RIGS_BATCH_FUSED = 1 };
 
 typedef struct rigs_batch_stats {
-    int64_t rounds;          /* combiner rounds (every stepping scene parked) */
+    int64_t rounds;          /* executor rounds (requests taken together) */
     int64_t fused_calls;     /* device calls made for those rounds */
     int64_t requests;        /* scene requests they served */
-    double device_ms;        /* wall time of the rounds */
+    double device_ms;        /* wall time of the rounds, summed over executors */
     int64_t ccd_calls;       /* fused earliest-TOI calls */
     int64_t ccd_candidates;
     double ccd_kernel_ms;
     int64_t ccd_launches;
     int64_t ccd_box_evaluations;
+    int32_t executors;       /* device contexts (streams) serving the rounds */
+    int32_t pad;
+    rigc_tuning tuning;      /* CCD tunables the autotuner chose (zeros: not run) */
+    double autotune_ms;      /* their device time over the autotuner's sample */
 } rigs_batch_stats;
 
 typedef struct rigs_batch rigs_batch;
@@ -214,6 +218,12 @@ rigc_status rigs_batch_destroy(rigs_batch* batch);
 const char* rigs_batch_last_error(const rigs_batch* batch);
 /* Every scene advances num_steps time steps. */
 rigc_status rigs_batch_step(rigs_batch* batch, int32_t num_steps);
+/* Fused mode's dispatch rule. A round runs once every scene still stepping
+ * has a request waiting, or once the oldest waiting request has waited
+ * gather_us microseconds (gather_us < 0: only the first rule). The default
+ * is the environment's RIGS_BATCH_GATHER_US, else 200. Results do not
+ * depend on it, only the grouping of requests into rounds. */
+rigc_status rigs_batch_set_gather(rigs_batch* batch, double gather_us);
 rigc_status rigs_batch_get_state(const rigs_batch* batch, int32_t scene, double* poses, double* velocities);
 rigc_status rigs_batch_get_stats(const rigs_batch* batch, int32_t scene, rigs_stats* stats);
 rigc_status rigs_batch_get_batch_stats(const rigs_batch* batch, rigs_batch_stats* stats);

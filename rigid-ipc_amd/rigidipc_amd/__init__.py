@@ -54,6 +54,12 @@ class BlockHessian(ctypes.Structure):
                 ("capacity", ctypes.c_int64), ("count", ctypes.c_int64)]
 
 
+class Tuning(ctypes.Structure):
+    """rigc_tuning: schedule knobs of the rigid CCD (no effect on results)."""
+    _fields_ = [("ccd_walk_budget", ctypes.c_int32), ("ccd_spec_levels", ctypes.c_int32),
+                ("ccd_window_factor", ctypes.c_double)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [
         ("broad_phase_ms", ctypes.c_double), ("narrow_phase_ms", ctypes.c_double),
@@ -90,6 +96,12 @@ def lib():
     L.rigc_broad_phase.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_int32_p, ctypes.c_int64,
                                    _c_int64_p, _c_int32_p, ctypes.c_int64, _c_int64_p]
     L.rigc_compute_earliest_toi.argtypes = [vp, _c_double_p, _c_double_p, ctypes.c_double, _c_double_p]
+    L.rigc_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.rigc_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.rigc_autotune_earliest_toi.argtypes = [vp, ctypes.c_int32, _c_double_p, _c_double_p, ctypes.c_double,
+                                             ctypes.POINTER(Tuning), _c_double_p]
+    for name in ("rigc_get_tuning", "rigc_set_tuning", "rigc_autotune_earliest_toi"):
+        getattr(L, name).restype = ctypes.c_int32
     L.rigc_narrow_phase_piecewise_linear.argtypes = [vp, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_int64,
                                                      _c_int32_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                                      _c_double_p, _c_double_p, _c_int8_p]
@@ -109,6 +121,7 @@ def lib():
     L.rigc_set_scenes.argtypes = [vp, i32, ctypes.POINTER(Scene)]
     L.rigc_get_scene_count.argtypes = [vp, _c_int32_p]
     L.rigc_batch_earliest_toi.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, _c_double_p]
+    L.rigc_batch_ccd_evaluations.argtypes = [vp, i32, _c_int32_p, ctypes.POINTER(ctypes.c_int64)]
     L.rigc_batch_barrier_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, i32, _c_double_p,
                                           _c_double_p, bh, _c_int64_p, _c_double_p]
     L.rigc_batch_minimum_distance.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, _c_double_p]
@@ -117,7 +130,8 @@ def lib():
                                                          _c_double_p, _c_int64_p]
     L.rigc_batch_friction_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, _c_double_p, i32,
                                            _c_double_p, _c_double_p, bh]
-    for name in ("rigc_set_scenes", "rigc_get_scene_count", "rigc_batch_earliest_toi", "rigc_batch_barrier_term",
+    for name in ("rigc_set_scenes", "rigc_get_scene_count", "rigc_batch_earliest_toi", "rigc_batch_ccd_evaluations",
+                 "rigc_batch_barrier_term",
                  "rigc_batch_minimum_distance", "rigc_batch_update_friction_constraints",
                  "rigc_batch_detect_intersections",
                  "rigc_batch_friction_term"):
@@ -225,6 +239,15 @@ class Context:
                                                     _p(p1, _c_double_p), float(minimum_separation_distance),
                                                     _p(toi, _c_double_p)))
         return toi
+
+    def batch_ccd_evaluations(self, ids):
+        """Box evaluations per scene of the last earliest-TOI call of this
+        multi-scene context (0 for scenes it did not serve)."""
+        ids = np.ascontiguousarray(np.asarray(ids, np.int32))
+        out = np.zeros(len(ids), np.int64)
+        self._check(self._L.rigc_batch_ccd_evaluations(self._ctx, len(ids), _p(ids, _c_int32_p),
+                                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return out
 
     def _batch_potential(self, fn, ids, args, flags):
         n = len(ids)
@@ -414,6 +437,31 @@ class Context:
         n = ctypes.c_int64()
         self._check(self._L.rigc_detect_intersections(self._ctx, _p(p, _c_double_p), None, 0, ctypes.byref(n)))
         return n.value
+
+    def tuning(self):
+        """The CCD tunables in force (rigc_get_tuning) as a dict."""
+        t = Tuning()
+        self._check(self._L.rigc_get_tuning(self._ctx, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        """rigc_set_tuning with the given fields changed."""
+        cur = self.tuning()
+        cur.update(kw)
+        t = Tuning(**cur)
+        self._check(self._L.rigc_set_tuning(self._ctx, ctypes.byref(t)))
+
+    def autotune_earliest_toi(self, samples, minimum_separation_distance=0.0):
+        """rigc_autotune_earliest_toi over samples = [(poses_t0, poses_t1), ...]
+        (poses of the whole context); returns (tuning dict, device ms)."""
+        p0 = np.ascontiguousarray(np.stack([np.asarray(a, np.float64).reshape(-1, 6) for a, _ in samples]))
+        p1 = np.ascontiguousarray(np.stack([np.asarray(b, np.float64).reshape(-1, 6) for _, b in samples]))
+        t = Tuning()
+        ms = ctypes.c_double()
+        self._check(self._L.rigc_autotune_earliest_toi(self._ctx, len(samples), _p(p0, _c_double_p),
+                                                       _p(p1, _c_double_p), float(minimum_separation_distance),
+                                                       ctypes.byref(t), ctypes.byref(ms)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}, ms.value
 
     def compute_earliest_toi(self, poses_t0, poses_t1, minimum_separation_distance=0.0):
         """Broad + narrow phase on the device; +inf when nothing collides."""

@@ -64,7 +64,9 @@ class BatchStats(ctypes.Structure):
         ("rounds", ctypes.c_int64), ("fused_calls", ctypes.c_int64), ("requests", ctypes.c_int64),
         ("device_ms", ctypes.c_double), ("ccd_calls", ctypes.c_int64), ("ccd_candidates", ctypes.c_int64),
         ("ccd_kernel_ms", ctypes.c_double), ("ccd_launches", ctypes.c_int64),
-        ("ccd_box_evaluations", ctypes.c_int64),
+        ("ccd_box_evaluations", ctypes.c_int64), ("executors", ctypes.c_int32),
+        ("pad", ctypes.c_int32), ("ccd_walk_budget", ctypes.c_int32), ("ccd_spec_levels", ctypes.c_int32),
+        ("ccd_window_factor", ctypes.c_double), ("autotune_ms", ctypes.c_double),
     ]
 
 
@@ -106,10 +108,12 @@ def sim_lib():
     L.rigs_batch_get_state.argtypes = [vp, ctypes.c_int32, _c_double_p, _c_double_p]
     L.rigs_batch_get_stats.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Stats)]
     L.rigs_batch_get_batch_stats.argtypes = [vp, ctypes.POINTER(BatchStats)]
+    L.rigs_batch_set_gather.argtypes = [vp, ctypes.c_double]
     for name in ("rigs_create", "rigs_destroy", "rigs_step", "rigs_get_state", "rigs_get_stats", "rigs_set_state",
                  "rigs_rollout", "rigs_rollout_backward",
                  "rigs_block_ldlt_solve", "rigs_batch_create", "rigs_batch_destroy", "rigs_batch_step",
-                 "rigs_batch_get_state", "rigs_batch_get_stats", "rigs_batch_get_batch_stats"):
+                 "rigs_batch_get_state", "rigs_batch_get_stats", "rigs_batch_get_batch_stats",
+                 "rigs_batch_set_gather"):
         getattr(L, name).restype = ctypes.c_int32
     _sim = L
     return L
@@ -210,6 +214,13 @@ class Batch:
         st = self._L.rigs_batch_step(self._b, int(n))
         if st != RIGC_OK:
             raise RigcError(st, self._L.rigs_batch_last_error(self._b).decode())
+
+    def set_gather(self, gather_us):
+        """Fused mode's gather window in microseconds (< 0: a round waits
+        until every stepping scene has a request waiting)."""
+        st = self._L.rigs_batch_set_gather(self._b, float(gather_us))
+        if st != RIGC_OK:
+            raise RigcError(st, "rigs_batch_set_gather failed")
 
     def state(self, i):
         poses = np.zeros((self.num_bodies[i], 6))

@@ -76,6 +76,28 @@ def test_earliest_toi_matches_single_scene_contexts(setup):
     assert finite > 0
 
 
+def test_per_scene_ccd_evaluations_add_up(setup):
+    """rigc_batch_ccd_evaluations: the per-scene box evaluations of the last
+    earliest-TOI call sum to the call's total, and a scene the call did not
+    serve reads 0."""
+    scenes, batch, plain = setup
+    subset = (0, 2, 3)
+    p0s, p1s = [], []
+    for k in subset:
+        p0, p1 = moved(scenes[k], 40 + k)
+        p0s.append(p0)
+        p1s.append(p1)
+    batch.batch_earliest_toi(subset, p0s, p1s)
+    total = batch.stats()["box_evaluations"]
+    per = batch.batch_ccd_evaluations(range(len(scenes)))
+    print(per, total, flush=True)
+    assert total > 0
+    assert int(per.sum()) == int(total)
+    for k in range(len(scenes)):
+        if k not in subset:
+            assert per[k] == 0
+
+
 def test_single_scene_entry_points_on_a_batch_context(setup):
     scenes, batch, plain = setup
     p0 = np.concatenate([moved(sc, 60 + k)[0] for k, sc in enumerate(scenes)])
