@@ -911,6 +911,7 @@ def batch_compare(n_scenes, steps, warm=True):
                              autotuned=dict(ccd_walk_budget=int(bs["ccd_walk_budget"]),
                                             ccd_spec_levels=int(bs["ccd_spec_levels"]),
                                             ccd_window_factor=bs["ccd_window_factor"],
+                                            ccd_wide_queries=int(bs["ccd_wide_queries"]),
                                             sample_device_ms=bs["autotune_ms"]))
             iters = it
         b.close()

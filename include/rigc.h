@@ -297,11 +297,14 @@ typedef struct rigc_tuning {
     double ccd_window_factor;  /* first pass of an earliest-TOI call bounded by
                                 * this times the scene's previous answer (0: no
                                 * window); default 4 */
+    int64_t ccd_wide_queries;  /* calls with at least this many queries launch
+                                * two blocks per CU (throughput), fewer one
+                                * (latency); default 4096 */
 } rigc_tuning;
 rigc_status rigc_get_tuning(const rigc_ctx* ctx, rigc_tuning* tuning);
 rigc_status rigc_set_tuning(rigc_ctx* ctx, const rigc_tuning* tuning);
 /* Autotuner: coordinate descent over the tunings (speculative levels, walk
- * budget, window factor, three values each) on num_samples earliest-TOI
+ * budget, window factor, wide-launch query count; three values each) on num_samples earliest-TOI
  * calls (poses_t0 / poses_t1: [num_samples][num_bodies][6], num_bodies of
  * the whole context; each call as rigc_compute_earliest_toi, in order, so a
  * line-search-like sequence exercises the window), median of three timed
@@ -314,6 +317,12 @@ rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const
                                        const double* poses_t1, double minimum_separation_distance,
                                        rigc_tuning* best, double* best_ms);
 
+/* The previous earliest TOI of each listed scene (>= 0, +inf for none): the
+ * bound of the first pass of its next earliest-TOI call on this context.
+ * A context keeps it by itself from its own calls; a caller that spreads a
+ * scene's calls over several contexts hands it over. Results do not depend
+ * on it. No counterpart in the reference (its search has no first pass). */
+rigc_status rigc_batch_set_previous_toi(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* toi);
 /* Box evaluations of the rigid CCD per scene in the last earliest-TOI call
  * of a multi-scene context (0 for a scene the call did not serve). No
  * counterpart in the reference: it tells a batch caller which scenes'

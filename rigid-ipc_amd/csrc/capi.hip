@@ -65,6 +65,7 @@ struct rigc_ctx {
     int walk_budget = 4096;
     int spec_levels = 5;
     double window_factor = 4.0;
+    long long wide_queries = 4096;
     std::vector<double> last_toi;  // per scene: answer of its previous earliest-TOI call
     hipEvent_t ev2 = nullptr;
     unsigned char* scratch = nullptr;  // this device's no-allocation array (rigc_narrow_phase_noalloc)
@@ -457,7 +458,7 @@ rigc_status rigc_narrow_phase(rigc_ctx* ctx, const double* poses_t0, const doubl
     a.q_flag = ctx->qflag.as<int>();
 
     const int block = 256;
-    const int grid = ctx->num_cus * CCD_BLOCKS_PER_CU;  // persistent, all blocks resident
+    const int grid = ccd_grid(ctx->num_cus, (long long)(n_ee + n_fv), ctx->wide_queries);  // persistent
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
     a.walk_budget = ctx->walk_budget;
     a.spec_levels = ctx->spec_levels;
@@ -565,7 +566,8 @@ rigc_status rigc_narrow_phase_noalloc(rigc_ctx* ctx, const double* poses_t0, con
     a.q_flag = (int*)(base + L.qflag);
 
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
-    RIGC_HIP(noalloc_narrow_phase(a, ctx->num_cus * CCD_BLOCKS_PER_CU, base, L, d.n_scenes, per_query ? 1 : 0, st));
+    RIGC_HIP(noalloc_narrow_phase(a, ccd_grid(ctx->num_cus, (long long)nq, ctx->wide_queries), base, L, d.n_scenes,
+                                  per_query ? 1 : 0, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
     unsigned long long res[2] = { 0, 0 }, evals = 0;
     RIGC_HIP(hipMemcpyAsync(res, base + L.result, sizeof(res), hipMemcpyDeviceToHost, st));
@@ -722,7 +724,7 @@ rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::
     a.q_best = ctx->qbest.as<unsigned long long>();
     a.q_visits = ctx->qvisits.as<unsigned int>();
     a.q_flag = ctx->qflag.as<int>();
-    const int grid = ctx->num_cus * CCD_BLOCKS_PER_CU;  // persistent, all blocks resident
+    const int grid = ccd_grid(ctx->num_cus, nq, ctx->wide_queries);  // persistent
     a.walk_budget = ctx->walk_budget;
     // a multi-scene context counts each scene's box evaluations (what a
     // batch scheduler uses to keep deep searches out of shared calls)
@@ -1227,22 +1229,25 @@ rigc_status rigc_get_tuning(const rigc_ctx* ctx, rigc_tuning* t) {
     t->ccd_walk_budget = ctx->walk_budget;
     t->ccd_spec_levels = ctx->spec_levels;
     t->ccd_window_factor = ctx->window_factor;
+    t->ccd_wide_queries = ctx->wide_queries;
     return RIGC_OK;
 }
 
 rigc_status rigc_set_tuning(rigc_ctx* ctx, const rigc_tuning* t) {
     if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
     if (!t || t->ccd_walk_budget < 0 || t->ccd_spec_levels < 0 || t->ccd_spec_levels > 5 ||
-        !(t->ccd_window_factor >= 0) || !std::isfinite(t->ccd_window_factor))
+        !(t->ccd_window_factor >= 0) || !std::isfinite(t->ccd_window_factor) || t->ccd_wide_queries < 0)
         return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_tuning: bad tuning");
     ctx->walk_budget = t->ccd_walk_budget;
     ctx->spec_levels = t->ccd_spec_levels;
     ctx->window_factor = t->ccd_window_factor;
+    ctx->wide_queries = t->ccd_wide_queries;
     return RIGC_OK;
 }
 
 // Coordinate descent over the CCD tunables on the caller's sample calls:
-// for each knob in turn (speculative levels, walk budget, window factor),
+// for each knob in turn (speculative levels, walk budget, window factor,
+// the query count from which a launch takes two blocks per CU),
 // every value of its grid runs the whole sample (from the same per-scene
 // window state) three times with the other knobs at their current choice,
 // and the median device time (broad + narrow phase) decides; a value
@@ -1258,12 +1263,13 @@ rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const
         return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_autotune_earliest_toi: bad arguments");
     const size_t stride = 6 * (size_t)ctx->scene.nb;
     const std::vector<double> window0 = ctx->last_toi;
-    const rigc_tuning start{ ctx->walk_budget, ctx->spec_levels, ctx->window_factor };
+    const rigc_tuning start{ ctx->walk_budget, ctx->spec_levels, ctx->window_factor, ctx->wide_queries };
     std::vector<double> ref, got(num_samples);
     auto restore = [&](const rigc_tuning& t) {
         ctx->walk_budget = t.ccd_walk_budget;
         ctx->spec_levels = t.ccd_spec_levels;
         ctx->window_factor = t.ccd_window_factor;
+        ctx->wide_queries = t.ccd_wide_queries;
         ctx->last_toi = window0;
     };
     // median device time of the sample under t (3 runs); RIGC_OK or an error
@@ -1289,17 +1295,18 @@ rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const
     rigc_tuning cur = start;
     double cur_ms = 0;
     rigc_status st = time_of(cur, cur_ms);
-    for (int knob = 0; knob < 3 && st == RIGC_OK; knob++) {
-        const double grid[3][3] = { { 3, 4, 5 }, { 1024, 4096, 16384 }, { 0.0, 4.0, 16.0 } };
+    for (int knob = 0; knob < 4 && st == RIGC_OK; knob++) {
+        const double grid[4][3] = { { 3, 4, 5 }, { 1024, 4096, 16384 }, { 0.0, 4.0, 16.0 }, { 0, 4096, 65536 } };
         rigc_tuning pick = cur;
         double pick_ms = cur_ms;
         for (int g = 0; g < 3 && st == RIGC_OK; g++) {
             rigc_tuning t = cur;
             if (knob == 0) t.ccd_spec_levels = (int32_t)grid[0][g];
             else if (knob == 1) t.ccd_walk_budget = (int32_t)grid[1][g];
-            else t.ccd_window_factor = grid[2][g];
+            else if (knob == 2) t.ccd_window_factor = grid[2][g];
+            else t.ccd_wide_queries = (int64_t)grid[3][g];
             if (t.ccd_spec_levels == cur.ccd_spec_levels && t.ccd_walk_budget == cur.ccd_walk_budget &&
-                t.ccd_window_factor == cur.ccd_window_factor)
+                t.ccd_window_factor == cur.ccd_window_factor && t.ccd_wide_queries == cur.ccd_wide_queries)
                 continue;
             double ms = 0;
             st = time_of(t, ms);
@@ -1318,6 +1325,20 @@ rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const
     restore(cur);
     if (best) *best = cur;
     if (best_ms) *best_ms = cur_ms;
+    return RIGC_OK;
+}
+
+rigc_status rigc_batch_set_previous_toi(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* toi) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_batch_set_previous_toi: no scene");
+    if (n < 0 || (n > 0 && (!scene_ids || !toi)))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_set_previous_toi: bad arguments");
+    for (int i = 0; i < n; i++) {
+        const int k = scene_ids[i];
+        if (k < 0 || k >= ctx->scene.n_scenes || !(toi[i] >= 0))
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_batch_set_previous_toi: bad scene id or TOI");
+    }
+    for (int i = 0; i < n; i++) ctx->last_toi[scene_ids[i]] = toi[i];
     return RIGC_OK;
 }
 

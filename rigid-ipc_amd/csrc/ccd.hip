@@ -50,17 +50,22 @@ constexpr int MAX_DEPTH = 192;
 constexpr int THR_REFRESH = 8;
 constexpr int SPEC_LEVELS = 5;  // most speculative levels below an owner's box (62 boxes: one 64-bit mask)
 
+// Three named words rather than an array: the masks are read and written at
+// bit positions known only at run time, and a dynamically indexed array
+// would live in scratch memory instead of registers.
 struct Bits192 {
-    unsigned long long w[3];
+    unsigned long long w0, w1, w2;
 };
+RIGC_DI unsigned long long word(const Bits192& b, int wi) { return wi == 0 ? b.w0 : (wi == 1 ? b.w1 : b.w2); }
+RIGC_DI void clear_bits(Bits192& b) { b.w0 = b.w1 = b.w2 = 0ULL; }
 RIGC_DI void bset(Bits192& b, int k, bool v) {
     const unsigned long long m = 1ULL << (k & 63);
-    if (k < 64) b.w[0] = v ? (b.w[0] | m) : (b.w[0] & ~m);
-    else if (k < 128) b.w[1] = v ? (b.w[1] | m) : (b.w[1] & ~m);
-    else b.w[2] = v ? (b.w[2] | m) : (b.w[2] & ~m);
+    if (k < 64) b.w0 = v ? (b.w0 | m) : (b.w0 & ~m);
+    else if (k < 128) b.w1 = v ? (b.w1 | m) : (b.w1 & ~m);
+    else b.w2 = v ? (b.w2 | m) : (b.w2 & ~m);
 }
 RIGC_DI bool bget(const Bits192& b, int k) {
-    const unsigned long long w = k < 64 ? b.w[0] : (k < 128 ? b.w[1] : b.w[2]);
+    const unsigned long long w = k < 64 ? b.w0 : (k < 128 ? b.w1 : b.w2);
     return (w >> (k & 63)) & 1ULL;
 }
 // mask of bits [lo, hi) within word wi
@@ -72,19 +77,34 @@ RIGC_DI unsigned long long range_mask(int wi, int lo, int hi) {
     return top & ~bot;
 }
 RIGC_DI int popcount_range(const Bits192& b, int lo, int hi) {
-    return __popcll(b.w[0] & range_mask(0, lo, hi)) + __popcll(b.w[1] & range_mask(1, lo, hi))
-        + __popcll(b.w[2] & range_mask(2, lo, hi));
+    return __popcll(b.w0 & range_mask(0, lo, hi)) + __popcll(b.w1 & range_mask(1, lo, hi))
+        + __popcll(b.w2 & range_mask(2, lo, hi));
 }
 // Largest level L < depth whose bit is clear, or -1.
 RIGC_DI int last_zero_below(const Bits192& b, int depth) {
+#pragma unroll
     for (int wi = 2; wi >= 0; wi--) {
-        const unsigned long long m = ~b.w[wi] & range_mask(wi, 0, depth);
+        const unsigned long long m = ~word(b, wi) & range_mask(wi, 0, depth);
         if (m) return 64 * wi + 63 - __clzll(m);
     }
     return -1;
 }
 
 RIGC_DI double dyad(unsigned long long i, int l) { return ldexp((double)i, -l); }
+
+// a[k] += v and a[k] = 2 a[k] + bit for a dimension k known only at run
+// time, written as selects so that the three-element arrays stay in
+// registers (a dynamic index would put them in scratch memory).
+template <typename T>
+RIGC_DI void add_at(T (&a)[3], int k, T v) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) a[i] = (i == k) ? a[i] + v : a[i];
+}
+template <typename T>
+RIGC_DI void descend_at(T (&a)[3], int k, T bit) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) a[i] = (i == k) ? (a[i] << 1) + bit : a[i];
+}
 
 struct Query {
     int kind, bA, bB, sc;
@@ -108,17 +128,31 @@ struct Query {
 // (pair, vertex). Each lane then forms its pair's differences. The operations and their order per
 // value are those of pose_iv, rotation_iv (sinc_iv) and world_vertex_iv,
 // so the result is the same, bit for bit, as evaluating each box alone.
+// The stages' scratch (rotation vectors, norms, sinc pieces) is dead once
+// the rotation rows are formed, so the world vertices of the last stage
+// share its space: 19.8 KB per wavefront, which lets two 256-thread blocks
+// share a CU's LDS (two wavefronts per SIMD).
 constexpr int COOP_ITEMS = 32;
 struct CoopWave {
     int kind[COOP_ITEMS], body[COOP_ITEMS][2], g[COOP_ITEMS][4];
     double t_lo[COOP_ITEMS], t_hi[COOP_ITEMS];
-    Iv r[COOP_ITEMS][2][3];       // interval rotation vector per body
-    Iv x[COOP_ITEMS][2][2];       // its norm and the norm of r / 2
-    double se[COOP_ITEMS][2][2][2];  // sinc ends [body][angle][piece]: upper at x.lo, lower at min(x.hi, mb)
-    Iv gp[COOP_ITEMS][2][2];      // sin(x) / x beyond mb [body][angle]
     Iv R[COOP_ITEMS][2][9];       // rotation matrices, row-major
-    Iv w[COOP_ITEMS][4][3];       // world vertices, in the order of Query::g
+    union {
+        struct {
+            Iv r[COOP_ITEMS][2][3];       // interval rotation vector per body
+            Iv x[COOP_ITEMS][2][2];       // its norm and the norm of r / 2
+            double se[COOP_ITEMS][2][2][2];  // sinc ends [body][angle][piece]: upper at x.lo, lower at min(x.hi, mb)
+            Iv gp[COOP_ITEMS][2][2];      // sin(x) / x beyond mb [body][angle]
+        } s;
+        Iv w[COOP_ITEMS][4][3];   // world vertices, in the order of Query::g (written after R)
+    } u;
 };
+
+template <int I>
+RIGC_DI void rotation_row(const Iv r[3], Iv sa, Iv c2, Iv* row) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) row[j] = rotation_entry(r, sa, c2, I, j);
+}
 
 // Value of lane `lane` (wavefront-uniform) in every lane.
 RIGC_DI long long bcast_lane(long long x, int lane) {
@@ -172,19 +206,19 @@ RIGC_DI void coop_t_part(const CcdArgs& a, CoopWave& L, bool need, long long key
             Iv r[3], xa, xh;
             pose_rot_iv(a.poses_t0 + 6 * body, a.poses_t1 + 6 * body, tt, r);
             rotation_norms(r, xa, xh);
-            for (int j = 0; j < 3; j++) L.r[u][side][j] = r[j];
-            L.x[u][side][0] = xa;
-            L.x[u][side][1] = xh;
+            for (int j = 0; j < 3; j++) L.u.s.r[u][side][j] = r[j];
+            L.u.s.x[u][side][0] = xa;
+            L.u.s.x[u][side][1] = xh;
         }
         wave_lds_sync();
         // monotone-piece ends of sinc per (pair, body, angle, piece)
         bool any_gt = false;
         for (int k = lane; k < 8 * n; k += 64) {
             const int u = k >> 3, side = (k >> 2) & 1, ang = (k >> 1) & 1, piece = k & 1;
-            const Iv x = L.x[u][side][ang];
+            const Iv x = L.u.s.x[u][side][ang];
             if (x.lo <= RIGC_SINC_MB) {
                 const double d = piece == 0 ? x.lo : (x.hi > RIGC_SINC_MB ? RIGC_SINC_MB : x.hi);
-                L.se[u][side][ang][piece] = sinc_point_end(d, piece == 0);
+                L.u.s.se[u][side][ang][piece] = sinc_point_end(d, piece == 0);
             }
             any_gt = any_gt || (piece == 0 && sinc_has_gt(x));
         }
@@ -192,10 +226,10 @@ RIGC_DI void coop_t_part(const CcdArgs& a, CoopWave& L, bool need, long long key
         if (__ballot(any_gt)) {
             for (int k = lane; k < 4 * n; k += 64) {
                 const int u = k >> 2, side = (k >> 1) & 1, ang = k & 1;
-                const Iv x = L.x[u][side][ang];
+                const Iv x = L.u.s.x[u][side][ang];
                 if (sinc_has_gt(x)) {
                     const Iv arg = (x.lo <= RIGC_SINC_MB) ? Iv{ RIGC_SINC_MB, x.hi } : x;
-                    L.gp[u][side][ang] = div_nonzero(sin_iv(arg), arg);
+                    L.u.s.gp[u][side][ang] = div_nonzero(sin_iv(arg), arg);
                 }
             }
         }
@@ -203,13 +237,17 @@ RIGC_DI void coop_t_part(const CcdArgs& a, CoopWave& L, bool need, long long key
         // one rotation row per (pair, body, row)
         for (int k = lane; k < 6 * n; k += 64) {
             const int u = k / 6, side = (k / 3) & 1, i = k % 3;
-            const Iv xa = L.x[u][side][0], xh = L.x[u][side][1];
-            const Iv sa = sinc_combine(xa, L.se[u][side][0][0], L.se[u][side][0][1], L.gp[u][side][0]);
-            const Iv sh = sinc_combine(xh, L.se[u][side][1][0], L.se[u][side][1][1], L.gp[u][side][1]);
+            const Iv xa = L.u.s.x[u][side][0], xh = L.u.s.x[u][side][1];
+            const Iv sa = sinc_combine(xa, L.u.s.se[u][side][0][0], L.u.s.se[u][side][0][1], L.u.s.gp[u][side][0]);
+            const Iv sh = sinc_combine(xh, L.u.s.se[u][side][1][0], L.u.s.se[u][side][1][1], L.u.s.gp[u][side][1]);
             const Iv c2 = rotation_c2(sh);
             Iv r[3];
-            for (int j = 0; j < 3; j++) r[j] = L.r[u][side][j];
-            for (int j = 0; j < 3; j++) L.R[u][side][3 * i + j] = rotation_entry(r, sa, c2, i, j);
+#pragma unroll
+            for (int j = 0; j < 3; j++) r[j] = L.u.s.r[u][side][j];
+            // the row as a compile-time index (the entries index r by it)
+            if (i == 0) rotation_row<0>(r, sa, c2, &L.R[u][side][0]);
+            else if (i == 1) rotation_row<1>(r, sa, c2, &L.R[u][side][3]);
+            else rotation_row<2>(r, sa, c2, &L.R[u][side][6]);
         }
         wave_lds_sync();
         // position and one world vertex per (pair, vertex)
@@ -221,7 +259,7 @@ RIGC_DI void coop_t_part(const CcdArgs& a, CoopWave& L, bool need, long long key
             const double* v = a.V + 3 * L.g[u][m];
             for (int j = 0; j < 3; j++) {
                 const Iv pj = pose_pos_component(a.poses_t0 + 6 * body, a.poses_t1 + 6 * body, tt, j);
-                L.w[u][m][j] = world_coordinate(L.R[u][side][3 * j], L.R[u][side][3 * j + 1], L.R[u][side][3 * j + 2],
+                L.u.w[u][m][j] = world_coordinate(L.R[u][side][3 * j], L.R[u][side][3 * j + 1], L.R[u][side][3 * j + 2],
                                                 pj, v);
             }
         }
@@ -230,7 +268,7 @@ RIGC_DI void coop_t_part(const CcdArgs& a, CoopWave& L, bool need, long long key
             const int u = item;
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const Iv w0 = L.w[u][0][j], w1 = L.w[u][1][j], w2 = L.w[u][2][j], w3 = L.w[u][3][j];
+                const Iv w0 = L.u.w[u][0][j], w1 = L.u.w[u][1][j], w2 = L.u.w[u][2][j], w3 = L.u.w[u][3][j];
                 if (L.kind[u] == 0) {
                     A0[j] = w0;
                     DA[j] = w1 - w0;
@@ -319,9 +357,14 @@ RIGC_DI void box_widths(const int lev[3], double w[3]) {
 }
 RIGC_DI int split_dim(const double w[3], const double tol[3], bool all_sat) {
     int sp = -1;
+    double ws = 0.0, ts = 1.0;  // w[sp], tol[sp]
 #pragma unroll
     for (int i = 0; i < 3; i++)
-        if ((all_sat || w[i] > tol[i]) && (sp == -1 || w[i] * tol[sp] > w[sp] * tol[i])) sp = i;
+        if ((all_sat || w[i] > tol[i]) && (sp == -1 || w[i] * ts > ws * tol[i])) {
+            sp = i;
+            ws = w[i];
+            ts = tol[i];
+        }
     return sp;
 }
 RIGC_DI int split_dim_at(const int lev[3], const double tol[3]) {
@@ -356,7 +399,7 @@ __global__ void ccd_finish_kernel(CcdArgs a) {
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void ccd_kernel(CcdArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void ccd_kernel(CcdArgs a) {
     const long long nq = a.n_ee + a.n_fv;
     const int lane = threadIdx.x & 63;
     const unsigned long long lt_mask = (1ULL << lane) - 1ULL;
@@ -421,7 +464,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                             idx[0] = tk.idx[0]; idx[1] = tk.idx[1]; idx[2] = tk.idx[2];
                             lev[0] = tk.lev & 255; lev[1] = (tk.lev >> 8) & 255; lev[2] = (tk.lev >> 16) & 255;
                             depth = 0;
-                            s.w[0] = s.w[1] = s.w[2] = 0;
+                            clear_bits(s);
                             mt = s;
                             ma = s;
                             phase = 2;
@@ -442,8 +485,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             if (im == ~0ULL && !pool_open) break;  // no lane of this wavefront has work left
             if (im) {
                 const int first_open = (qi >= 0 && phase == 2) ? [&] {
+#pragma unroll
                     for (int wi = 0; wi < 3; wi++) {
-                        const unsigned long long m = ~s.w[wi] & range_mask(wi, 0, depth);
+                        const unsigned long long m = ~word(s, wi) & range_mask(wi, 0, depth);
                         if (m) return 64 * wi + __ffsll((long long)m) - 1;
                     }
                     return -1;
@@ -466,7 +510,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     d_idx[2] = idx[2] >> cb;
                     const int l0 = lev[0] - ct, l1 = lev[1] - ca, l2 = lev[2] - cb;
                     const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
-                    d_idx[d] += 1;
+                    add_at(d_idx, d, 1ULL);
                     bset(s, L, true);
                     d_q = qi;
                     d_tol0 = tol[0];
@@ -489,7 +533,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     idx[0] = r0; idx[1] = r1; idx[2] = r2;
                     lev[0] = r_lev & 255; lev[1] = (r_lev >> 8) & 255; lev[2] = (r_lev >> 16) & 255;
                     depth = 0;
-                    s.w[0] = s.w[1] = s.w[2] = 0;
+                    clear_bits(s);
                     mt = s;
                     ma = s;
                     phase = 2;
@@ -523,7 +567,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             idx[1] >>= ca; lev[1] -= ca;
             idx[2] >>= cb; lev[2] -= cb;
             const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
-            idx[d] += 1;
+            add_at(idx, d, 1ULL);
             bset(s, L, true);
             depth = L + 1;
         };
@@ -572,8 +616,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                         bset(s, depth, false);
                         bset(mt, depth, sp == 0);
                         bset(ma, depth, sp == 1);
-                        idx[sp] <<= 1;
-                        lev[sp] += 1;
+                        descend_at(idx, sp, 0ULL);
+                        add_at(lev, sp, 1);
                         depth++;
                     }
                 }
@@ -654,8 +698,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             const unsigned m = (unsigned)node + 1u;
             for (int j = 30 - __clz(m); j >= 0; j--) {
                 const int sp = split_dim_at(bl, tol);
-                bi[sp] = (bi[sp] << 1) + (unsigned long long)((m >> j) & 1u);
-                bl[sp] += 1;
+                descend_at(bi, sp, (unsigned long long)((m >> j) & 1u));
+                add_at(bl, sp, 1);
             }
             t = Iv{ dyad(bi[0], bl[0]), dyad(bi[0] + 1, bl[0]) };
             u = Iv{ dyad(bi[1], bl[1]), dyad(bi[1] + 1, bl[1]) };
@@ -698,7 +742,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 if (z) tol[0] /= 1e2;
                 phase = 2;
                 depth = 0;
-                s.w[0] = s.w[1] = s.w[2] = 0;
+                clear_bits(s);
                 mt = s;
                 ma = s;
                 idx[0] = idx[1] = idx[2] = 0;
@@ -775,7 +819,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         // D. export the pending subtrees of a long walk to the next pass
         if (qi >= 0 && !done && phase == 2 && !pool_open && a.tasks_out && walk >= a.walk_budget) {
             int n_open = 0;
-            for (int wi = 0; wi < 3; wi++) n_open += __popcll(~s.w[wi] & range_mask(wi, 0, depth));
+#pragma unroll
+            for (int wi = 0; wi < 3; wi++) n_open += __popcll(~word(s, wi) & range_mask(wi, 0, depth));
             const unsigned long long base = atomicAdd(a.n_tasks_out, (unsigned long long)(n_open + 1));
             const unsigned long long cap = (unsigned long long)a.tasks_cap;
             int k = 0;
@@ -792,7 +837,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                 tk.idx[1] = idx[1] >> ca;
                 tk.idx[2] = idx[2] >> cb;
                 const int d = bget(mt, L) ? 0 : (bget(ma, L) ? 1 : 2);
-                tk.idx[d] += 1;
+                add_at(tk.idx, d, 1ULL);
                 tk.lev = (lev[0] - ct) | ((lev[1] - ca) << 8) | ((lev[2] - cb) << 16);
                 tk.pad = 0;
                 a.tasks_out[slot] = tk;

@@ -247,7 +247,14 @@ struct CcdReadback {
 // Resident 256-lane blocks of the persistent CCD kernel per CU: the kernel
 // runs one wavefront per SIMD so that its register demand (above 256 per
 // lane) lives in VGPRs and AGPRs instead of scratch.
-constexpr int CCD_BLOCKS_PER_CU = 1;
+// The CCD kernel fits two 256-thread blocks per CU (two wavefronts per SIMD).
+// A call with few queries is bound by its longest bisection, whose
+// wavefront runs faster alone on its SIMD, so it launches one block per CU;
+// a call with many queries is bound by throughput and launches two.
+constexpr int CCD_BLOCKS_PER_CU = 2;
+inline int ccd_grid(int num_cus, long long n_queries, long long wide_queries) {
+    return num_cus * (n_queries >= wide_queries ? CCD_BLOCKS_PER_CU : 1);
+}
 hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWork& work,
                       const CcdReadback* readback = nullptr, int n_readback = 0);
 // The narrow phase in one pass (no export of long walks, so no task

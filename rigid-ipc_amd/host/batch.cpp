@@ -244,6 +244,7 @@ struct Batch {
     int active = 0;     // callers taking part (stepping scenes and their helpers)
     bool stop = false;
     double gather_ms = 0.2;
+    std::vector<double> last_toi;  // each scene's previous earliest TOI (any executor)
     rigc_tuning tuning{};  // what the autotuner chose (zeros when it did not run)
     double tune_ms = 0;    // its device time over the sample
 
@@ -419,7 +420,15 @@ void Exec::run_group(int kind, std::vector<Request*>& g) {
     if (kind == REQ_TOI) {
         concat(c0, false);
         concat(c1, true);
-        s = rigc_batch_earliest_toi(ctx, k, ids.data(), c0.data(), c1.data(), g[0]->a, vals.data());
+        // each scene's previous TOI, from whichever executor served it
+        kap.resize(k);
+        {
+            std::lock_guard<std::mutex> lk(B->m);
+            for (int i = 0; i < k; i++) kap[i] = B->last_toi[ids[i]];
+        }
+        s = rigc_batch_set_previous_toi(ctx, k, ids.data(), kap.data());
+        if (s == RIGC_OK)
+            s = rigc_batch_earliest_toi(ctx, k, ids.data(), c0.data(), c1.data(), g[0]->a, vals.data());
         if (s == RIGC_OK) {
             rigc_stats cs{};
             rigc_get_stats(ctx, &cs);
@@ -429,6 +438,8 @@ void Exec::run_group(int kind, std::vector<Request*>& g) {
             st.ccd_box_evaluations += cs.box_evaluations;
             st.ccd_calls++;
             for (int i = 0; i < k; i++) *g[i]->value = vals[i];
+            std::lock_guard<std::mutex> lk(B->m);
+            for (int i = 0; i < k; i++) B->last_toi[ids[i]] = vals[i];
         }
     } else if (kind == REQ_MIND) {
         concat(c0, false);
@@ -572,6 +583,7 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
         bool any_friction = false;
         for (int i = 0; i < num_scenes; i++) any_friction = any_friction || params[i].coefficient_friction > 0;
         B.has_friction_exec = any_friction;
+        B.last_toi.assign(num_scenes, HUGE_VAL);
         const int n_exec = n_general + (any_friction ? 1 : 0);
         for (int k = 0; k < n_exec && s == RIGC_OK; k++) {
             B.execs.emplace_back(new Exec());

@@ -57,7 +57,7 @@ class BlockHessian(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """rigc_tuning: schedule knobs of the rigid CCD (no effect on results)."""
     _fields_ = [("ccd_walk_budget", ctypes.c_int32), ("ccd_spec_levels", ctypes.c_int32),
-                ("ccd_window_factor", ctypes.c_double)]
+                ("ccd_window_factor", ctypes.c_double), ("ccd_wide_queries", ctypes.c_int64)]
 
 
 class Stats(ctypes.Structure):
@@ -122,6 +122,7 @@ def lib():
     L.rigc_get_scene_count.argtypes = [vp, _c_int32_p]
     L.rigc_batch_earliest_toi.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, _c_double_p]
     L.rigc_batch_ccd_evaluations.argtypes = [vp, i32, _c_int32_p, ctypes.POINTER(ctypes.c_int64)]
+    L.rigc_batch_set_previous_toi.argtypes = [vp, i32, _c_int32_p, _c_double_p]
     L.rigc_batch_barrier_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, i32, _c_double_p,
                                           _c_double_p, bh, _c_int64_p, _c_double_p]
     L.rigc_batch_minimum_distance.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, d, d, _c_double_p]
@@ -131,6 +132,7 @@ def lib():
     L.rigc_batch_friction_term.argtypes = [vp, i32, _c_int32_p, _c_double_p, _c_double_p, _c_double_p, i32,
                                            _c_double_p, _c_double_p, bh]
     for name in ("rigc_set_scenes", "rigc_get_scene_count", "rigc_batch_earliest_toi", "rigc_batch_ccd_evaluations",
+                 "rigc_batch_set_previous_toi",
                  "rigc_batch_barrier_term",
                  "rigc_batch_minimum_distance", "rigc_batch_update_friction_constraints",
                  "rigc_batch_detect_intersections",
@@ -239,6 +241,14 @@ class Context:
                                                     _p(p1, _c_double_p), float(minimum_separation_distance),
                                                     _p(toi, _c_double_p)))
         return toi
+
+    def batch_set_previous_toi(self, ids, toi):
+        """rigc_batch_set_previous_toi: each scene's previous earliest TOI
+        (the bound of the first pass of its next call on this context)."""
+        ids = np.ascontiguousarray(np.asarray(ids, np.int32))
+        toi = np.ascontiguousarray(np.asarray(toi, np.float64))
+        self._check(self._L.rigc_batch_set_previous_toi(self._ctx, len(ids), _p(ids, _c_int32_p),
+                                                        _p(toi, _c_double_p)))
 
     def batch_ccd_evaluations(self, ids):
         """Box evaluations per scene of the last earliest-TOI call of this

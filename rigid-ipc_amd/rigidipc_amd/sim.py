@@ -66,7 +66,7 @@ class BatchStats(ctypes.Structure):
         ("ccd_kernel_ms", ctypes.c_double), ("ccd_launches", ctypes.c_int64),
         ("ccd_box_evaluations", ctypes.c_int64), ("executors", ctypes.c_int32),
         ("pad", ctypes.c_int32), ("ccd_walk_budget", ctypes.c_int32), ("ccd_spec_levels", ctypes.c_int32),
-        ("ccd_window_factor", ctypes.c_double), ("autotune_ms", ctypes.c_double),
+        ("ccd_window_factor", ctypes.c_double), ("ccd_wide_queries", ctypes.c_int64), ("autotune_ms", ctypes.c_double),
     ]
 
 

@@ -76,6 +76,24 @@ def test_earliest_toi_matches_single_scene_contexts(setup):
     assert finite > 0
 
 
+def test_previous_toi_handover_keeps_results(setup):
+    """rigc_batch_set_previous_toi only bounds the first pass of the next
+    call: whatever previous TOI a scene is handed (tiny, large, none), the
+    earliest TOI is the single-scene context's."""
+    scenes, batch, plain = setup
+    subset = (0, 2, 3)
+    p0s, p1s, want = [], [], []
+    for k in subset:
+        p0, p1 = moved(scenes[k], 40 + k)
+        p0s.append(p0)
+        p1s.append(p1)
+        want.append(plain[k].compute_earliest_toi(p0, p1))
+    for prev in (1e-9, 1e-3, 0.2, float("inf")):
+        batch.batch_set_previous_toi(subset, [prev] * len(subset))
+        assert list(batch.batch_earliest_toi(subset, p0s, p1s)) == want, prev
+    assert any(np.isfinite(want))
+
+
 def test_per_scene_ccd_evaluations_add_up(setup):
     """rigc_batch_ccd_evaluations: the per-scene box evaluations of the last
     earliest-TOI call sum to the call's total, and a scene the call did not

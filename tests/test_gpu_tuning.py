@@ -42,11 +42,12 @@ def test_set_and_get_round_trip_and_bad_values(ctx):
     import rigidipc_amd as R
     ctx.set_scene(cluster(3, n=27))
     t0 = ctx.tuning()
-    assert t0 == {"ccd_walk_budget": 4096, "ccd_spec_levels": 5, "ccd_window_factor": 4.0}
-    ctx.set_tuning(ccd_walk_budget=256, ccd_spec_levels=2, ccd_window_factor=0.0)
-    assert ctx.tuning() == {"ccd_walk_budget": 256, "ccd_spec_levels": 2, "ccd_window_factor": 0.0}
+    assert t0 == {"ccd_walk_budget": 4096, "ccd_spec_levels": 5, "ccd_window_factor": 4.0, "ccd_wide_queries": 4096}
+    ctx.set_tuning(ccd_walk_budget=256, ccd_spec_levels=2, ccd_window_factor=0.0, ccd_wide_queries=0)
+    assert ctx.tuning() == {"ccd_walk_budget": 256, "ccd_spec_levels": 2, "ccd_window_factor": 0.0,
+                            "ccd_wide_queries": 0}
     for bad in ({"ccd_spec_levels": 6}, {"ccd_spec_levels": -1}, {"ccd_walk_budget": -5},
-                {"ccd_window_factor": -1.0}, {"ccd_window_factor": float("inf")}):
+                {"ccd_window_factor": -1.0}, {"ccd_window_factor": float("inf")}, {"ccd_wide_queries": -1}):
         with pytest.raises(R.RigcError):
             ctx.set_tuning(**bad)
     ctx.set_tuning(**t0)
@@ -65,9 +66,10 @@ def test_every_tuning_gives_the_oracle_toi(ctx, seed):
     for wb in (0, 64, 4096):
         for sl in (0, 3, 5):
             for wf in (0.0, 4.0):
-                ctx.set_tuning(ccd_walk_budget=wb, ccd_spec_levels=sl, ccd_window_factor=wf)
-                got = [ctx.compute_earliest_toi(p0, p1) for p0, p1 in seq]
-                assert got == want, (wb, sl, wf)
+                for wq in (0, 1 << 40):
+                    ctx.set_tuning(ccd_walk_budget=wb, ccd_spec_levels=sl, ccd_window_factor=wf, ccd_wide_queries=wq)
+                    got = [ctx.compute_earliest_toi(p0, p1) for p0, p1 in seq]
+                    assert got == want, (wb, sl, wf, wq)
     ctx.set_tuning(**base)
 
 
@@ -82,5 +84,6 @@ def test_autotuner_sets_a_grid_tuning_and_keeps_results(ctx):
     assert best["ccd_walk_budget"] in (1024, 4096, 16384)
     assert best["ccd_spec_levels"] in (3, 4, 5)
     assert best["ccd_window_factor"] in (0.0, 4.0, 16.0)
+    assert best["ccd_wide_queries"] in (0, 4096, 65536)
     assert ctx.tuning() == best
     assert [ctx.compute_earliest_toi(p0, p1) for p0, p1 in seq] == want
