@@ -175,7 +175,7 @@ def device_backend(ctx):
                        ("compute_minimum_distance", "rigc_compute_minimum_distance"),
                        ("update_friction_constraints", "rigc_update_friction_constraints"),
                        ("compute_friction_term", "rigc_compute_friction_term"), ("last_error", "rigc_last_error"),
-                       ("get_stats", "rigc_get_stats")):
+                       ("get_stats", "rigc_get_stats"), ("set_detection_method", "rigc_set_detection_method")):
         setattr(be, member, ctypes.cast(getattr(L, fn), ctypes.c_void_p).value)
     return be
 

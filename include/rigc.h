@@ -286,6 +286,21 @@ rigc_status rigc_friction_adjoint(rigc_ctx* ctx, const double* poses, const doub
 /* Per-scene rigc_compute_earliest_toi: toi[i] for scene scene_ids[i]. */
 rigc_status rigc_batch_earliest_toi(rigc_ctx* ctx, int32_t n, const int32_t* scene_ids, const double* poses_t0,
                                     const double* poses_t1, double minimum_separation_distance, double* toi);
+/* DetectionMethod (src/ccd/ccd.hpp:20-24) of rigc_broad_phase and of the
+ * earliest-TOI calls' candidates: RIGC_DETECTION_HASH_GRID (default) or
+ * RIGC_DETECTION_BVH (the reference's default detection_method,
+ * src/SimState.cpp:178; detect_collision_candidates_rigid_bvh,
+ * src/ccd/rigid/broad_phase.cpp:92-121, 205-230). The BVH path needs every
+ * vertex on an edge and every edge on a face, and at most 64 KiB of
+ * per-body staging (48 B per vertex + 8 B per face of the largest body);
+ * RIGC_ERR_INVALID_ARGUMENT otherwise. The constraint set is built from
+ * the hash grid's candidates either way: every pair within the activation
+ * distance is a candidate of both, so the set is the same. A scene set
+ * afterwards keeps the method (set it again to re-check the scene). */
+#define RIGC_DETECTION_HASH_GRID 0
+#define RIGC_DETECTION_BVH 1
+rigc_status rigc_set_detection_method(rigc_ctx* ctx, int32_t method);
+
 /* Tunables of the rigid CCD narrow phase. No counterpart in the reference
  * (its CCD is sequential per query); none of them changes a result, only
  * the schedule of the device's search. */

@@ -22,6 +22,8 @@ extern "C" double oracle_earliest_toi(const double* p0, const double* p1, double
 extern "C" double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double msd, int trajectory,
                                                  long* n_candidates);
 
+extern "C" void oracle_set_detection_method(int m);
+
 namespace {
 
 struct OracleBackend {
@@ -38,6 +40,7 @@ OracleBackend& state() {
 }
 
 rigc_status ob_set_scene(void*, const rigc_scene* sc) {
+    oracle_set_detection_method(0);  // a new scene starts with the hash grid
     Scene& s = scene();
     s.nb = sc->num_bodies; s.nv = sc->num_vertices; s.ne = sc->num_edges; s.nf = sc->num_faces;
     s.V.assign(sc->vertices, sc->vertices + 3 * (size_t)s.nv);
@@ -211,10 +214,15 @@ rigc_status ob_isect(void*, const double* poses, int64_t* n) {
 
 extern "C" {
 
-// Fills a rigs_backend table with the oracle's entry points.
+// Fills a rigs_backend table with the oracle's entry points. The detection
+// method is the oracle's (process-wide) setting.
 void oracle_backend(rigs_backend* out) {
     *out = rigs_backend{ nullptr, ob_set_scene, ob_toi, ob_build, ob_barrier, ob_mind, ob_fupdate, ob_friction,
-                         ob_error, ob_stats, ob_isect, ob_toi_pl, ob_fadjoint };
+                         ob_error, ob_stats, ob_isect, ob_toi_pl, ob_fadjoint,
+                         [](void*, int32_t m) -> rigc_status {
+                             oracle_set_detection_method(m);
+                             return RIGC_OK;
+                         } };
 }
 
 }  // extern "C"

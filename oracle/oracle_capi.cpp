@@ -158,9 +158,15 @@ void oracle_flops_reset() {
 }
 
 // ---- scene + broad phase
+// DetectionMethod of the broad phase and of the earliest TOI's candidates:
+// 0 HASH_GRID, 1 BVH (src/ccd/ccd.hpp:20-24). Loading a scene resets it to
+// the hash grid; a caller that wants the BVH sets it after the scene.
+static int g_detection = 0;
+
 void oracle_set_scene(int nb, int nv, int ne, int nf, const double* V, const int* E, const int* F,
                       const int* FE, const int* bv, const int* be, const int* bf, const int* group,
                       const unsigned char* is_static) {
+    g_detection = 0;
     Scene& s = scene();
     s.nb = nb; s.nv = nv; s.ne = ne; s.nf = nf;
     s.V.assign(V, V + 3 * (size_t)nv);
@@ -176,11 +182,14 @@ void oracle_set_scene(int nb, int nv, int ne, int nf, const double* V, const int
 }
 
 static Candidates g_candidates;
+void oracle_set_detection_method(int m) { g_detection = m; }
+int oracle_bvh_supported() { return bvh_supported(scene()) ? 1 : 0; }
 
 // continuous != 0: detect_collision_candidates_rigid(poses_t0, poses_t1);
 // otherwise the discrete version at poses_t0.
 void oracle_broadphase(int continuous, const double* p0, const double* p1, double r, long* n_ee, long* n_fv) {
-    if (continuous) broadphase_ccd(scene(), p0, p1, r, g_candidates);
+    if (g_detection == 1) broadphase_bvh(scene(), p0, p1, r, continuous != 0, g_candidates);
+    else if (continuous) broadphase_ccd(scene(), p0, p1, r, g_candidates);
     else broadphase_static(scene(), p0, r, g_candidates);
     *n_ee = (long)g_candidates.ee.size();
     *n_fv = (long)g_candidates.fv.size();
@@ -235,7 +244,8 @@ double oracle_earliest_toi_trajectory(const double* p0, const double* p1, double
                                       long* n_candidates) {
     const Scene& s = scene();
     Candidates c;
-    broadphase_ccd(s, p0, p1, msd / 2.0, c);
+    if (g_detection == 1) broadphase_bvh(s, p0, p1, msd / 2.0, true, c);
+    else broadphase_ccd(s, p0, p1, msd / 2.0, c);
     const long ne = (long)c.ee.size(), nf = (long)c.fv.size();
     *n_candidates = ne + nf;
     std::atomic<double> shared(INFINITY);

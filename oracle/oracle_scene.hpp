@@ -44,6 +44,16 @@ void broadphase_ccd(const Scene& s, const double* p0, const double* p1, double i
                     Candidates& out);
 void broadphase_static(const Scene& s, const double* poses, double inflation_radius, Candidates& out);
 
+// detect_collision_candidates_rigid with BVH (src/ccd/rigid/broad_phase.cpp:92-121,
+// 205-230; oracle_bvh.cpp): continuous over poses p0 -> p1, or discrete at
+// p0. EE candidates keep the reference's orientation (an edge of the body
+// with fewer faces first); both lists sorted. bvh_supported: every vertex
+// lies on an edge and every edge on a face (no codimensional elements).
+void broadphase_bvh(const Scene& s, const double* p0, const double* p1, double inflation_radius, bool continuous,
+                    Candidates& out);
+std::vector<std::array<int, 2>> close_bodies_bvh(const Scene& s, const double* p0, const double* p1, double r);
+bool bvh_supported(const Scene& s);
+
 // [nv][7] included flag, AABB min, AABB max of the last broad-phase call.
 std::vector<double>& last_vertex_aabbs();
 

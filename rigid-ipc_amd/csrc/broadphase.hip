@@ -667,6 +667,19 @@ hipError_t queue_pairs(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, 
 
 }  // namespace
 
+hipError_t queue_body_boxes(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                            const double* d_p1, double r, hipStream_t st) {
+    const int nb = sc.nb;
+    BP_CHECK(bp.box_raw.ensure(6 * sizeof(double) * nb));
+    BP_CHECK(bp.box_inf.ensure(6 * sizeof(double) * nb));
+    BP_CHECK(bp.inc.ensure(sizeof(int) * nb));
+    BP_CHECK(bp.counters.ensure(4 * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(body_box_kernel, dim3((unsigned)sel.n_bodies), dim3(BP_THREADS), 0, st, sel.bodies, sc.V,
+                       sc.body_vstart, sc.is_static, sc.r_max, d_p0, d_p1, r, bp.box_raw.as<double>(),
+                       bp.box_inf.as<double>(), bp.inc.as<int>(), bp.counters.as<unsigned long long>());
+    return hipGetLastError();
+}
+
 static void broad_phase_capacities(BroadPhase& bp, const SceneSel& sel) {
     const long long held_pairs = (long long)(bp.pairs.bytes / (2 * sizeof(int)));
     if (bp.min_cap > 0) {

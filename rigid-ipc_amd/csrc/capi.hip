@@ -66,6 +66,7 @@ struct rigc_ctx {
     int spec_levels = 5;
     double window_factor = 4.0;
     long long wide_queries = 4096;
+    int detection = RIGC_DETECTION_HASH_GRID;  // broad phase of rigc_broad_phase and the earliest TOI
     std::vector<double> last_toi;  // per scene: answer of its previous earliest-TOI call
     hipEvent_t ev2 = nullptr;
     unsigned char* scratch = nullptr;  // this device's no-allocation array (rigc_narrow_phase_noalloc)
@@ -192,6 +193,20 @@ rigc_status load_scenes(rigc_ctx* ctx, int n, const rigc_scene* scenes, const ch
         for (int i = 0; i < 3 * nf; i++)
             if (s->faces[i] < 0 || s->faces[i] >= nv || s->face_edges[i] < 0 || s->face_edges[i] >= ne)
                 return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": face index out of range");
+        // every element refers to its own body's vertices (and a face to its
+        // body's edges): the per-body kernels index body-local arrays
+        for (int b = 0; b < nb; b++) {
+            const int v0 = s->body_vertex_start[b], v1 = s->body_vertex_start[b + 1];
+            const int e0 = s->body_edge_start[b], e1 = s->body_edge_start[b + 1];
+            if (v1 < v0 || e1 < e0 || s->body_face_start[b + 1] < s->body_face_start[b])
+                return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": body ranges not ascending");
+            for (int i = 2 * e0; i < 2 * e1; i++)
+                if (s->edges[i] < v0 || s->edges[i] >= v1)
+                    return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": an edge refers to another body's vertex");
+            for (int i = 3 * s->body_face_start[b]; i < 3 * s->body_face_start[b + 1]; i++)
+                if (s->faces[i] < v0 || s->faces[i] >= v1 || s->face_edges[i] < e0 || s->face_edges[i] >= e1)
+                    return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": a face refers to another body's element");
+        }
         if ((long long)NV + nv > INT32_MAX / 2 || (long long)NE + ne > INT32_MAX / 2 || (long long)NF + nf > INT32_MAX / 2)
             return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, w + ": scenes too large");
         NB += nb; NV += nv; NE += ne; NF += nf;
@@ -275,6 +290,29 @@ rigc_status load_scenes(rigc_ctx* ctx, int n, const rigc_scene* scenes, const ch
     RIGC_HIP(upload(ctx, &d.r_max, r_max.data(), r_max.size()));
     RIGC_HIP(upload(ctx, &d.body_scene, body_scene.data(), body_scene.size()));
     RIGC_HIP(upload(ctx, &d.scene_body, scene_body.data(), scene_body.size()));
+    {
+        // MeshSelector ownership for the BVH broad phase
+        std::vector<int> v2f(NV, INT32_MAX), e2f(NE, INT32_MAX), on_edge(NV, 0);
+        for (int f = 0; f < NF; f++)
+            for (int j = 0; j < 3; j++) {
+                v2f[F[3 * f + j]] = std::min(v2f[F[3 * f + j]], f);
+                e2f[FE[3 * f + j]] = std::min(e2f[FE[3 * f + j]], f);
+            }
+        for (int e = 0; e < 2 * NE; e++) on_edge[E[e]] = 1;
+        bool ok = true;
+        for (int v = 0; v < NV && ok; v++) ok = on_edge[v] != 0;
+        for (int e = 0; e < NE && ok; e++) ok = e2f[e] != INT32_MAX;
+        d.bvh_ok = ok;
+        d.max_body_vertices = 0;
+        d.max_body_faces = 0;
+        for (int b = 0; b < NB; b++) {
+            d.max_body_vertices = std::max(d.max_body_vertices, bvs[b + 1] - bvs[b]);
+            d.max_body_faces = std::max(d.max_body_faces, bfs[b + 1] - bfs[b]);
+        }
+        RIGC_HIP(upload(ctx, &d.v2f, v2f.data(), v2f.size()));
+        RIGC_HIP(upload(ctx, &d.e2f, e2f.data(), e2f.size()));
+        RIGC_HIP(hipStreamSynchronize(ctx->stream));  // the host arrays end with this block
+    }
     d.h_scene_body = scene_body;
     d.h_scene_edge = scene_edge;
     ctx->h_body_scene = body_scene;
@@ -673,6 +711,17 @@ rigc_status select_scenes(rigc_ctx* ctx, int32_t n, const int32_t* ids, const ch
     return RIGC_OK;
 }
 
+// The BVH broad phase is chosen for the context: the scene in force must
+// support it (a scene set after the choice is checked here).
+rigc_status check_detection(rigc_ctx* ctx, const char* who) {
+    if (ctx->detection != RIGC_DETECTION_BVH) return RIGC_OK;
+    if (!ctx->scene.bvh_ok || bvh_lds_bytes(ctx->scene) > 65536)
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, std::string(who) + ": the scene does not support the BVH broad "
+                                                                       "phase (codimensional elements or a body too "
+                                                                       "large)");
+    return RIGC_OK;
+}
+
 CcdArgs ccd_args(rigc_ctx* ctx) {
     const DevScene& d = ctx->scene;
     CcdArgs a{};
@@ -698,13 +747,19 @@ CcdArgs ccd_args(rigc_ctx* ctx) {
 // full, with the queries of every other scene skipped. Either way each
 // result is the exact minimum.
 rigc_status earliest_toi_sel(rigc_ctx* ctx, const SelState& S, double msd, std::vector<double>& out) {
+    const rigc_status cs = check_detection(ctx, "earliest TOI");
+    if (cs != RIGC_OK) return cs;
     const SceneSel& sel = S.sel;
     const int ns = ctx->scene.n_scenes;
     hipStream_t s = ctx->stream;
     RIGC_HIP(hipEventRecord(ctx->ev0, s));
     // the earliest TOI is a minimum over the candidates: their order is free
-    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, sel, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
-                             msd / 2.0, true, false, s));
+    if (ctx->detection == RIGC_DETECTION_BVH)
+        RIGC_HIP(run_broad_phase_bvh(ctx->bp, ctx->scene, sel, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
+                                     msd / 2.0, true, false, ctx->num_cus, s));
+    else
+        RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, sel, ctx->poses0.as<double>(), ctx->poses1.as<double>(),
+                                 msd / 2.0, true, false, s));
     RIGC_HIP(hipEventRecord(ctx->ev1, s));
     const long long nq = ctx->bp.n_ee + ctx->bp.n_fv;
     RIGC_HIP(ctx->counters.ensure(4 * sizeof(unsigned long long)));
@@ -941,11 +996,18 @@ rigc_status rigc_broad_phase(rigc_ctx* ctx, const double* poses_t0, const double
         || (ee_capacity && !ee) || (fv_capacity && !fv))
         return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_broad_phase: bad arguments");
     RIGC_HIP(hipSetDevice(ctx->device));
-    rigc_status st = upload_poses(ctx, poses_t0, poses_t1);
+    rigc_status st = check_detection(ctx, "rigc_broad_phase");
+    if (st != RIGC_OK) return st;
+    st = upload_poses(ctx, poses_t0, poses_t1);
     if (st != RIGC_OK) return st;
     RIGC_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
-                             ctx->poses1.as<double>(), inflation_radius, poses_t1 != nullptr, true, ctx->stream));
+    if (ctx->detection == RIGC_DETECTION_BVH)
+        RIGC_HIP(run_broad_phase_bvh(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
+                                     ctx->poses1.as<double>(), inflation_radius, poses_t1 != nullptr, true,
+                                     ctx->num_cus, ctx->stream));
+    else
+        RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
+                                 ctx->poses1.as<double>(), inflation_radius, poses_t1 != nullptr, true, ctx->stream));
     RIGC_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     *n_ee = ctx->bp.n_ee;
     *n_fv = ctx->bp.n_fv;
@@ -1242,6 +1304,24 @@ rigc_status rigc_set_tuning(rigc_ctx* ctx, const rigc_tuning* t) {
     ctx->spec_levels = t->ccd_spec_levels;
     ctx->window_factor = t->ccd_window_factor;
     ctx->wide_queries = t->ccd_wide_queries;
+    return RIGC_OK;
+}
+
+rigc_status rigc_set_detection_method(rigc_ctx* ctx, int32_t method) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (method != RIGC_DETECTION_HASH_GRID && method != RIGC_DETECTION_BVH)
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_detection_method: unknown method");
+    if (method == RIGC_DETECTION_BVH) {
+        if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_set_detection_method: no scene");
+        if (!ctx->scene.bvh_ok)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT,
+                             "rigc_set_detection_method: the BVH broad phase needs every vertex on an edge and every "
+                             "edge on a face");
+        if (bvh_lds_bytes(ctx->scene) > 65536)
+            return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_set_detection_method: a body is too large for the BVH "
+                                                        "broad phase's LDS staging");
+    }
+    ctx->detection = method;
     return RIGC_OK;
 }
 
@@ -1655,9 +1735,15 @@ rigc_status rigc_compute_earliest_toi_piecewise_linear(rigc_ctx* ctx, const doub
     rigc_status s = upload_poses(ctx, poses_t0, poses_t1);
     if (s != RIGC_OK) return s;
     hipStream_t st = ctx->stream;
+    if ((s = check_detection(ctx, "rigc_compute_earliest_toi_piecewise_linear")) != RIGC_OK) return s;
     RIGC_HIP(hipEventRecord(ctx->ev0, st));
-    RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
-                             ctx->poses1.as<double>(), minimum_separation_distance / 2.0, true, false, st));
+    if (ctx->detection == RIGC_DETECTION_BVH)
+        RIGC_HIP(run_broad_phase_bvh(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
+                                     ctx->poses1.as<double>(), minimum_separation_distance / 2.0, true, false,
+                                     ctx->num_cus, st));
+    else
+        RIGC_HIP(run_broad_phase(ctx->bp, ctx->scene, ctx->sel_all.sel, ctx->poses0.as<double>(),
+                                 ctx->poses1.as<double>(), minimum_separation_distance / 2.0, true, false, st));
     RIGC_HIP(hipEventRecord(ctx->ev1, st));
     RIGC_HIP(hipStreamSynchronize(st));
     float bp_ms = 0;

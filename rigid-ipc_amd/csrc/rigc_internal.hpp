@@ -38,6 +38,13 @@ struct DevScene {
     int* scene_body = nullptr;      // [n_scenes + 1] body ranges
     std::vector<int> h_scene_body;  // host copy of scene_body
     std::vector<int> h_scene_edge;  // [n_scenes + 1] edge ranges (host)
+    // BVH broad phase (MeshSelector, src/utils/mesh_selector.cpp:47-120):
+    // the lowest face index holding each vertex / edge; bvh_ok: every
+    // vertex on an edge and every edge on a face (no codimensional elements)
+    int* v2f = nullptr;             // [nv]
+    int* e2f = nullptr;             // [ne]
+    bool bvh_ok = false;
+    int max_body_vertices = 0, max_body_faces = 0;
 };
 
 // Scenes taking part in one call: their ids (ascending), their bodies, and
@@ -115,6 +122,26 @@ struct BroadPhase {
     long long cap_pairs = 0, cap_ee = 0, cap_fv = 0;  // capacities of the last queued pass
     long long min_cap = 0;  // > 0: start every capacity from this floor (exercises the overflow passes)
 };
+
+// Body boxes of the selected bodies (RigidBody::compute_bounding_box over
+// p0 -> p1, raw and grown by r, in bp.box_raw / bp.box_inf) with the pair
+// counters cleared.
+hipError_t queue_body_boxes(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                            const double* d_p1, double r, hipStream_t stream);
+// LDS bytes the BVH broad phase stages per workgroup for this scene (A's
+// vertex boxes and two face lists); at most 64 KiB is supported.
+inline size_t bvh_lds_bytes(const DevScene& sc) {
+    return sizeof(double) * 6 * (size_t)sc.max_body_vertices + 2 * sizeof(int) * (size_t)sc.max_body_faces;
+}
+// DetectionMethod::BVH candidates (bvh.hip): close body pairs from the
+// inflated body boxes, then per pair the owner-face rules of
+// detect_body_pair_collision_candidates_from_aabbs with the smaller body's
+// vertices carried into the other's frame with interval poses (over t in
+// [0, 1] when continuous). EE keys keep the reference's (edge of the body
+// with fewer faces, other edge) orientation. Output as run_broad_phase.
+hipError_t run_broad_phase_bvh(BroadPhase& bp, const DevScene& sc, const SceneSel& sel, const double* d_p0,
+                               const double* d_p1, double inflation_radius, bool continuous, bool sorted,
+                               int num_cus, hipStream_t stream);
 
 // Candidate detection over the selected scenes (continuous when p1 != p0
 // is meant: `continuous`), leaving int32 pairs in bp.out_ee / bp.out_fv,

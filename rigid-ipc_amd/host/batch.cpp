@@ -210,6 +210,10 @@ rigc_status px_friction(void* s, const double* poses, const double* poses_t0, do
 
 const char* px_error(void* s) { return static_cast<Proxy*>(s)->err.c_str(); }
 
+// The executors' contexts were given the batch's detection method at
+// creation; a scene asks for the same one.
+rigc_status px_detection(void* s, int32_t method);
+
 // ------------------------------------------------------------ combiner
 // One executor: a device context holding every scene, the thread that makes
 // its fused calls, and that thread's staging.
@@ -245,6 +249,7 @@ struct Batch {
     bool stop = false;
     double gather_ms = 0.2;
     std::vector<double> last_toi;  // each scene's previous earliest TOI (any executor)
+    int detection = RIGC_DETECTION_HASH_GRID;  // broad phase of every scene's CCD calls
     rigc_tuning tuning{};  // what the autotuner chose (zeros when it did not run)
     double tune_ms = 0;    // its device time over the sample
 
@@ -254,6 +259,13 @@ struct Batch {
     void start();
     void shutdown();
 };
+
+rigc_status px_detection(void* s, int32_t method) {
+    Proxy* P = static_cast<Proxy*>(s);
+    if (method == P->B->detection) return RIGC_OK;
+    P->err = "the batch's scenes must share one detection method";
+    return RIGC_ERR_INVALID_ARGUMENT;
+}
 
 rigc_status submit(Proxy* P, Request& r) {
     Batch& B = *P->B;
@@ -583,6 +595,12 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
         bool any_friction = false;
         for (int i = 0; i < num_scenes; i++) any_friction = any_friction || params[i].coefficient_friction > 0;
         B.has_friction_exec = any_friction;
+        B.detection = params[0].detection_method;
+        for (int i = 1; i < num_scenes; i++)
+            if (params[i].detection_method != B.detection) {
+                delete b;
+                return RIGC_ERR_INVALID_ARGUMENT;  // one detection method per batch (the contexts hold it)
+            }
         B.last_toi.assign(num_scenes, HUGE_VAL);
         const int n_exec = n_general + (any_friction ? 1 : 0);
         for (int k = 0; k < n_exec && s == RIGC_OK; k++) {
@@ -592,6 +610,7 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
             e.friction = any_friction && k == n_general;
             s = rigc_create(&e.ctx, device);
             if (s == RIGC_OK) s = rigc_set_scenes(e.ctx, num_scenes, scenes);
+            if (s == RIGC_OK && B.detection != RIGC_DETECTION_HASH_GRID) s = rigc_set_detection_method(e.ctx, B.detection);
             if (s != RIGC_OK) b->err = e.ctx ? rigc_last_error(e.ctx) : "rigc_create failed";
         }
         if (s == RIGC_OK) {
@@ -605,7 +624,8 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
                 P.scene = i;
                 P.nb = scenes[i].num_bodies;
                 b->tables[i] = rigs_backend{ &P, px_set_scene, px_toi, px_build, px_barrier, px_mind, px_fupdate,
-                                             px_friction, px_error, nullptr, px_isect };
+                                             px_friction, px_error, nullptr, px_isect, nullptr, nullptr,
+                                             px_detection };
             }
             // autotuner (RIGS_BATCH_AUTOTUNE, default on): the CCD tunables
             // timed on a line-search-like sequence of earliest-TOI calls over

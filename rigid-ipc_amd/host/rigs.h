@@ -61,6 +61,9 @@ typedef struct rigs_backend {
     rigc_status (*friction_adjoint)(void* self, const double* poses, const double* poses_t0, double epsv_times_h,
                                     double dhat, double dmin, double barrier_stiffness, const double* lambda,
                                     double* out);
+    /* Optional (may be NULL; then only RIGC_DETECTION_HASH_GRID): the
+     * broad phase of compute_earliest_toi (rigc_set_detection_method). */
+    rigc_status (*set_detection_method)(void* self, int32_t method);
 } rigs_backend;
 
 /* Trajectory types of the CCD (TrajectoryType, src/ccd/ccd.hpp:30-38; the
@@ -94,6 +97,10 @@ typedef struct rigs_params {
                                         * step ends at the minimiser to rounding and
                                         * a rollout is a smooth function of its inputs
                                         * (differentiable rollouts) */
+    int32_t detection_method;          /* RIGC_DETECTION_*: the broad phase of the
+                                        * line search's CCD (distance_barrier_constraint.
+                                        * detection_method; the reference's default is
+                                        * BVH, rigs_default_params sets HASH_GRID) */
 } rigs_params;
 
 typedef struct rigs_bodies {

@@ -75,6 +75,7 @@ rigc_status gb_fadjoint(void* s, const double* p, const double* p0, double eh, d
                         const double* lambda, double* out) {
     return rigc_friction_adjoint((rigc_ctx*)s, p, p0, eh, dhat, dmin, kappa, lambda, out);
 }
+rigc_status gb_detection(void* s, int32_t method) { return rigc_set_detection_method((rigc_ctx*)s, method); }
 rigc_status gb_toi_pl(void* s, const double* p0, const double* p1, double msd, double* toi) {
     return rigc_compute_earliest_toi_piecewise_linear((rigc_ctx*)s, p0, p1, msd, toi);
 }
@@ -1309,6 +1310,7 @@ void rigs_default_params(rigs_params* p) {
     p->do_intersection_check = 0;
     p->trajectory_type = RIGS_TRAJECTORY_RIGID;
     p->newton_polish = 0;
+    p->detection_method = RIGC_DETECTION_HASH_GRID;
 }
 
 rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t device, const rigc_scene* scene,
@@ -1320,6 +1322,9 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         return RIGC_ERR_INVALID_ARGUMENT;
     if (params->trajectory_type == RIGS_TRAJECTORY_PIECEWISE_LINEAR && backend
         && !backend->compute_earliest_toi_piecewise_linear)
+        return RIGC_ERR_INVALID_ARGUMENT;
+    if (params->detection_method != RIGC_DETECTION_HASH_GRID
+        && (params->detection_method != RIGC_DETECTION_BVH || (backend && !backend->set_detection_method)))
         return RIGC_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     rigs_sim* sim = new rigs_sim();
@@ -1335,10 +1340,13 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
             return s;
         }
         sim->be = rigs_backend{ sim->own_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl, gb_fadjoint };
+                                gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl, gb_fadjoint,
+                                gb_detection };
     }
     sim->fbe = sim->be;
     rigc_status s = sim->be.set_scene(sim->be.self, scene);
+    if (s == RIGC_OK && params->detection_method != RIGC_DETECTION_HASH_GRID)
+        s = sim->be.set_detection_method(sim->be.self, params->detection_method);
     if (s != RIGC_OK) {
         if (sim->own_ctx) rigc_destroy(sim->own_ctx);
         delete sim;
@@ -1352,7 +1360,8 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
         // term just runs through the main context)
         if (rigc_create(&sim->fric_ctx, device) == RIGC_OK && rigc_set_scene(sim->fric_ctx, scene) == RIGC_OK) {
             sim->fbe = rigs_backend{ sim->fric_ctx, gb_set_scene, gb_toi, gb_build, gb_barrier, gb_mind,
-                                     gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl, gb_fadjoint };
+                                     gb_fupdate, gb_friction, gb_error, gb_stats, gb_isect, gb_toi_pl, gb_fadjoint,
+                                gb_detection };
         } else if (sim->fric_ctx) {
             rigc_destroy(sim->fric_ctx);
             sim->fric_ctx = nullptr;

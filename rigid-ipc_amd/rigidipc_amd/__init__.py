@@ -100,7 +100,8 @@ def lib():
     L.rigc_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.rigc_autotune_earliest_toi.argtypes = [vp, ctypes.c_int32, _c_double_p, _c_double_p, ctypes.c_double,
                                              ctypes.POINTER(Tuning), _c_double_p]
-    for name in ("rigc_get_tuning", "rigc_set_tuning", "rigc_autotune_earliest_toi"):
+    L.rigc_set_detection_method.argtypes = [vp, ctypes.c_int32]
+    for name in ("rigc_get_tuning", "rigc_set_tuning", "rigc_autotune_earliest_toi", "rigc_set_detection_method"):
         getattr(L, name).restype = ctypes.c_int32
     L.rigc_narrow_phase_piecewise_linear.argtypes = [vp, _c_double_p, _c_double_p, _c_int32_p, ctypes.c_int64,
                                                      _c_int32_p, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
@@ -447,6 +448,11 @@ class Context:
         n = ctypes.c_int64()
         self._check(self._L.rigc_detect_intersections(self._ctx, _p(p, _c_double_p), None, 0, ctypes.byref(n)))
         return n.value
+
+    def set_detection_method(self, method):
+        """rigc_set_detection_method: "hash_grid" or "bvh" (the broad phase of
+        broad_phase() and of the earliest-TOI calls)."""
+        self._check(self._L.rigc_set_detection_method(self._ctx, {"hash_grid": 0, "bvh": 1}[method]))
 
     def tuning(self):
         """The CCD tunables in force (rigc_get_tuning) as a dict."""

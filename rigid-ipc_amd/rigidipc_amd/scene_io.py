@@ -13,8 +13,9 @@ densities and group ids (with the reference's id assignment), and the
 solver settings the stepper uses. The CCD runs with the scene's
 trajectory_type, "rigid" or "piecewise_linear" (the reference's default,
 src/SimState.cpp:179); "linear" and "redon" raise NotImplementedError. The
-broad phase is always the hash grid (the BVH gives the same constraint
-sets), and the written args say so. The coefficient of restitution is read and, as in the
+broad phase of the CCD is the scene's detection_method, "bvh" (the
+reference's default, src/SimState.cpp:178) or "hash_grid"; the constraint
+set comes out the same from either. The coefficient of restitution is read and, as in the
 reference's distance-barrier problem, not used. Anything else (2D,
 kinematic bodies, partially fixed dofs, forces and torques, split
 components) raises NotImplementedError.
@@ -30,6 +31,9 @@ import numpy as np
 
 from . import scenes as S
 
+# detection_method names (src/ccd/ccd.hpp:20-28 via NLOHMANN_JSON_SERIALIZE_ENUM)
+# -> rigs_params.detection_method (RIGC_DETECTION_*)
+DETECTION_METHODS = {"hash_grid": 0, "bvh": 1}
 # trajectory_type names (src/ccd/ccd.hpp:30-38 via NLOHMANN_JSON_SERIALIZE_ENUM)
 # -> rigs_params.trajectory_type
 TRAJECTORY_TYPES = {"rigid": 0, "piecewise_linear": 1}
@@ -240,6 +244,10 @@ def load(scene, meshes_dir=None, base_dir=None):
     if traj not in TRAJECTORY_TYPES:
         raise NotImplementedError(f"trajectory_type {traj!r}: only 'rigid' and 'piecewise_linear'")
     sc.trajectory_type = traj
+    det = str(dbc.get("detection_method", "bvh"))
+    if det not in DETECTION_METHODS:
+        raise NotImplementedError(f"detection_method {det!r}: only 'hash_grid' and 'bvh'")
+    sc.detection_method = det
     sc.name = str(scene.get("name", "scene"))
     return sc, args
 
@@ -269,6 +277,7 @@ def params_for(sc):
     p.velocity_conv_tol = tol
     p.do_intersection_check = int(bool(sc.do_intersection_check))
     p.trajectory_type = TRAJECTORY_TYPES[getattr(sc, "trajectory_type", "rigid")]
+    p.detection_method = DETECTION_METHODS[getattr(sc, "detection_method", "hash_grid")]
     return p
 
 
@@ -279,7 +288,6 @@ def run(scene, out_path=None, steps=None, meshes_dir=None, device=0, backend=Non
     sc, args = load(scene, meshes_dir=meshes_dir)
     n = sc.steps if steps is None else int(steps)
     args = copy.deepcopy(args)
-    args["distance_barrier_constraint"]["detection_method"] = "hash_grid"
     args["max_iterations"], args["max_time"] = n, -1
     s = M.Simulation(sc, params=params_for(sc), device=device, backend=backend)
     states, timings, iters, contacts, mind = [], [], [], [], []

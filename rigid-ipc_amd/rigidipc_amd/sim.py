@@ -22,7 +22,8 @@ class Backend(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in (
         "self", "set_scene", "compute_earliest_toi", "construct_constraint_set", "compute_barrier_term",
         "compute_minimum_distance", "update_friction_constraints", "compute_friction_term", "last_error",
-        "get_stats", "detect_intersections", "compute_earliest_toi_piecewise_linear", "friction_adjoint")]
+        "get_stats", "detect_intersections", "compute_earliest_toi_piecewise_linear", "friction_adjoint",
+        "set_detection_method")]
 
 
 class Params(ctypes.Structure):
@@ -34,6 +35,7 @@ class Params(ctypes.Structure):
         ("max_iterations", ctypes.c_int32), ("dhat_epsilon", ctypes.c_double),
         ("min_barrier_stiffness_scale", ctypes.c_double), ("do_intersection_check", ctypes.c_int32),
         ("trajectory_type", ctypes.c_int32), ("newton_polish", ctypes.c_int32),
+        ("detection_method", ctypes.c_int32),
     ]
 
 

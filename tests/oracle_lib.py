@@ -101,6 +101,8 @@ def _load(path):
     L.oracle_last_box_evaluations.restype = ctypes.c_long
     L.oracle_set_num_threads.argtypes = [ctypes.c_int]
     L.oracle_max_threads.restype = ctypes.c_int
+    L.oracle_set_detection_method.argtypes = [ctypes.c_int]
+    L.oracle_bvh_supported.restype = ctypes.c_int
     return L
 
 
@@ -220,6 +222,16 @@ def broadphase(poses_t0, poses_t1=None, inflation_radius=0.0):
     fv = np.zeros((nf.value, 2), np.int32)
     lib().oracle_get_candidates(i32ptr(ee), i32ptr(fv))
     return ee, fv
+
+
+def set_detection_method(method):
+    """The oracle's broad phase and earliest-TOI candidates: "hash_grid" or
+    "bvh" (DetectionMethod, src/ccd/ccd.hpp:20-24)."""
+    lib().oracle_set_detection_method({"hash_grid": 0, "bvh": 1}[method])
+
+
+def bvh_supported():
+    return bool(lib().oracle_bvh_supported())
 
 
 def orient3d(a, b, c, d):
