@@ -57,8 +57,8 @@ FLOPS_PER_BOX = 1015.7
 # WRITE_SIZE runs (profiles/ccd_kernel_traffic_pmc.md): FETCH_SIZE scaled
 # by 2.0 and WRITE_SIZE by 1.0, the factors at which the counters read a
 # 256 MiB device copy as 256 MiB read and 256 MiB written.
-CCD_TRAFFIC = (812.0 * 2.0 + 2185.6) * 1024   # ballistic CCD sequence
-SIM_CCD_TRAFFIC = (223.9 * 2.0 + 2126.3) * 1024  # time stepping, first 3 steps
+CCD_TRAFFIC = (795.8 * 2.0 + 98.3) * 1024   # ballistic CCD sequence
+SIM_CCD_TRAFFIC = (473.1 * 2.0 + 46.2) * 1024  # time stepping, first 3 steps
 HBM_PEAK_GBS = 8000.0
 # Algorithmic FP64 work of one barrier constraint's Hessian block: the
 # vertex-space value / gradient / Hessian with the rotation Jacobians
@@ -612,7 +612,8 @@ def bench_sim(args):
                    f"stepper's host time ({est['host_s']:.2f} s)",
             detail=est)
         line["gpu_vs_cpu"] = dict(all_cores=line["value"] / est["all_cores"], one_core=line["value"] / est["one_core"])
-        line["config1"] = config1_outright()
+        if not args.sim_only:
+            line["config1"] = config1_outright()
     return line
 
 
@@ -973,9 +974,12 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="sim", choices=["sim", "ccd", "plccd", "batch", "copy", "intersect", "grad"])
-    ap.add_argument("--batch-steps", type=int, default=3,
-                    help="steps of the batched-vs-plain comparison added to the sim line (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="steps of the batched-vs-plain comparison added to the sim line (0: skip; the "
+                         "batching workload is 10 steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sim-only", action="store_true",
+                    help="the stepping workload alone: no config1, batched or gradient entries (for profiles)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -1000,13 +1004,14 @@ def main():
         line = bench_grad(args)
     else:
         line = bench_sim(args)
-        if args.no_cpu:
+        if args.no_cpu and not args.sim_only:
             line["config1"] = config1_outright(no_cpu=True)
-        if args.batch_steps > 0:
+        if args.batch_steps > 0 and not args.sim_only:
             line["batched"] = batch_compare(256, args.batch_steps)
-            line["batched"]["note"] = (f"first {args.batch_steps} of the batching workload's 10 steps; "
-                                       "bench.py --workload batch runs all 10")
-        line["gradient"] = grad_compare(cpu=not args.no_cpu)
+            line["batched"]["note"] = (f"{args.batch_steps} of the batching workload's 10 steps, every scene "
+                                       "from its initial state, fused then plain")
+        if not args.sim_only:
+            line["gradient"] = grad_compare(cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

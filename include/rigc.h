@@ -318,18 +318,25 @@ typedef struct rigc_tuning {
 } rigc_tuning;
 rigc_status rigc_get_tuning(const rigc_ctx* ctx, rigc_tuning* tuning);
 rigc_status rigc_set_tuning(rigc_ctx* ctx, const rigc_tuning* tuning);
-/* Autotuner: coordinate descent over the tunings (speculative levels, walk
- * budget, window factor, wide-launch query count; three values each) on num_samples earliest-TOI
- * calls (poses_t0 / poses_t1: [num_samples][num_bodies][6], num_bodies of
- * the whole context; each call as rigc_compute_earliest_toi, in order, so a
- * line-search-like sequence exercises the window), median of three timed
- * runs per tuning; a knob changes only for a value at least 5% faster.
- * Checks that every tuning returns the same TOIs bit for bit
- * (RIGC_ERR_DEVICE otherwise) and sets the choice on the context.
- * best / best_ms (optional): the tuning chosen and its median device time
- * over the sample. */
+/* Autotuner: coordinate descent over the tunings selected by knobs (an OR
+ * of RIGC_TUNE_*; 0 means all: speculative levels, walk budget, window
+ * factor, wide-launch query count; three values each) on num_samples
+ * earliest-TOI calls (poses_t0 / poses_t1: [num_samples][num_bodies][6],
+ * num_bodies of the whole context; each call as rigc_compute_earliest_toi,
+ * in order, so a line-search-like sequence exercises the window), median of
+ * three timed runs per tuning; a knob changes only for a value at least 5%
+ * faster. Knobs left out keep the context's setting. The window factor acts
+ * through each scene's previous TOI, so a sample tunes it only when it
+ * follows the workload's own call sequence. Checks that every tuning
+ * returns the same TOIs bit for bit (RIGC_ERR_DEVICE otherwise) and sets
+ * the choice on the context. best / best_ms (optional): the tuning chosen
+ * and its median device time over the sample. */
+#define RIGC_TUNE_SPEC_LEVELS 1
+#define RIGC_TUNE_WALK_BUDGET 2
+#define RIGC_TUNE_WINDOW_FACTOR 4
+#define RIGC_TUNE_WIDE_QUERIES 8
 rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const double* poses_t0,
-                                       const double* poses_t1, double minimum_separation_distance,
+                                       const double* poses_t1, double minimum_separation_distance, int32_t knobs,
                                        rigc_tuning* best, double* best_ms);
 
 /* The previous earliest TOI of each listed scene (>= 0, +inf for none): the

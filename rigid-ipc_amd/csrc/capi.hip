@@ -1335,12 +1335,13 @@ rigc_status rigc_set_detection_method(rigc_ctx* ctx, int32_t method) {
 // differences are the timing's noise). Every run must return the first
 // run's TOIs bit for bit. The choice is set on the context.
 rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const double* poses_t0,
-                                       const double* poses_t1, double minimum_separation_distance, rigc_tuning* best,
-                                       double* best_ms) {
+                                       const double* poses_t1, double minimum_separation_distance, int32_t knobs,
+                                       rigc_tuning* best, double* best_ms) {
     if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
     if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_autotune_earliest_toi: no scene");
-    if (num_samples < 1 || !poses_t0 || !poses_t1 || !(minimum_separation_distance >= 0))
+    if (num_samples < 1 || !poses_t0 || !poses_t1 || !(minimum_separation_distance >= 0) || knobs < 0 || knobs > 15)
         return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_autotune_earliest_toi: bad arguments");
+    if (knobs == 0) knobs = 15;
     const size_t stride = 6 * (size_t)ctx->scene.nb;
     const std::vector<double> window0 = ctx->last_toi;
     const rigc_tuning start{ ctx->walk_budget, ctx->spec_levels, ctx->window_factor, ctx->wide_queries };
@@ -1376,6 +1377,7 @@ rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const
     double cur_ms = 0;
     rigc_status st = time_of(cur, cur_ms);
     for (int knob = 0; knob < 4 && st == RIGC_OK; knob++) {
+        if (!(knobs & (1 << knob))) continue;  // RIGC_TUNE_* bit of this knob
         const double grid[4][3] = { { 3, 4, 5 }, { 1024, 4096, 16384 }, { 0.0, 4.0, 16.0 }, { 0, 4096, 65536 } };
         rigc_tuning pick = cur;
         double pick_ms = cur_ms;

@@ -627,11 +627,14 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
                                              px_friction, px_error, nullptr, px_isect, nullptr, nullptr,
                                              px_detection };
             }
-            // autotuner (RIGS_BATCH_AUTOTUNE, default on): the CCD tunables
-            // timed on a line-search-like sequence of earliest-TOI calls over
-            // all scenes (each body's start pose, moved at its initial
-            // velocity over four time steps, then over half and a quarter of
-            // that), then set on every executor's context
+            // autotuner (RIGS_BATCH_AUTOTUNE, default on): the CCD's launch
+            // width, walk budget and speculative levels timed on a
+            // line-search-like sequence of earliest-TOI calls over all scenes
+            // (each body's start pose, moved at its initial velocity over
+            // four time steps, then over half and a quarter of that), then
+            // set on every executor's context. The window factor keeps its
+            // default: it acts through each scene's previous TOI, which this
+            // sample at the start poses does not reproduce.
             const char* at = std::getenv("RIGS_BATCH_AUTOTUNE");
             if (!at || std::atoi(at) != 0) {
                 size_t tot = 0;
@@ -650,14 +653,19 @@ rigc_status rigs_batch_create(rigs_batch** out, int32_t device, int32_t num_scen
                     }
                 }
                 rigc_tuning best{};
-                s = rigc_autotune_earliest_toi(B.execs[0]->ctx, n_samples, p0.data(), p1.data(), 0.0, &best, &B.tune_ms);
+                s = rigc_autotune_earliest_toi(B.execs[0]->ctx, n_samples, p0.data(), p1.data(), 0.0,
+                                               RIGC_TUNE_SPEC_LEVELS | RIGC_TUNE_WALK_BUDGET | RIGC_TUNE_WIDE_QUERIES,
+                                               &best, &B.tune_ms);
                 for (size_t k = 1; k < B.execs.size() && s == RIGC_OK; k++) s = rigc_set_tuning(B.execs[k]->ctx, &best);
                 if (s != RIGC_OK) b->err = std::string("autotuner: ") + rigc_last_error(B.execs[0]->ctx);
                 B.tuning = best;
                 const char* lg = std::getenv("RIGS_BATCH_LOG");
                 if (s == RIGC_OK && lg && std::atoi(lg) != 0) {
-                    std::fprintf(stderr, "batch: autotuned CCD walk budget %d, speculative levels %d, window %g (%.2f ms)\n",
-                                 best.ccd_walk_budget, best.ccd_spec_levels, best.ccd_window_factor, B.tune_ms);
+                    std::fprintf(stderr,
+                                 "batch: autotuned CCD walk budget %d, speculative levels %d, wide launches from %lld "
+                                 "queries (%.2f ms)\n",
+                                 best.ccd_walk_budget, best.ccd_spec_levels, (long long)best.ccd_wide_queries,
+                                 B.tune_ms);
                     std::fflush(stderr);
                 }
             }

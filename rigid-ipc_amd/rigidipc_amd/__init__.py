@@ -99,7 +99,7 @@ def lib():
     L.rigc_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.rigc_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.rigc_autotune_earliest_toi.argtypes = [vp, ctypes.c_int32, _c_double_p, _c_double_p, ctypes.c_double,
-                                             ctypes.POINTER(Tuning), _c_double_p]
+                                             ctypes.c_int32, ctypes.POINTER(Tuning), _c_double_p]
     L.rigc_set_detection_method.argtypes = [vp, ctypes.c_int32]
     for name in ("rigc_get_tuning", "rigc_set_tuning", "rigc_autotune_earliest_toi", "rigc_set_detection_method"):
         getattr(L, name).restype = ctypes.c_int32
@@ -467,16 +467,19 @@ class Context:
         t = Tuning(**cur)
         self._check(self._L.rigc_set_tuning(self._ctx, ctypes.byref(t)))
 
-    def autotune_earliest_toi(self, samples, minimum_separation_distance=0.0):
+    def autotune_earliest_toi(self, samples, minimum_separation_distance=0.0, knobs=None):
         """rigc_autotune_earliest_toi over samples = [(poses_t0, poses_t1), ...]
-        (poses of the whole context); returns (tuning dict, device ms)."""
+        (poses of the whole context); knobs: names of the rigc_tuning fields
+        to tune (default all); returns (tuning dict, device ms)."""
+        bits = {"ccd_spec_levels": 1, "ccd_walk_budget": 2, "ccd_window_factor": 4, "ccd_wide_queries": 8}
+        mask = 0 if knobs is None else sum(bits[k] for k in set(knobs))
         p0 = np.ascontiguousarray(np.stack([np.asarray(a, np.float64).reshape(-1, 6) for a, _ in samples]))
         p1 = np.ascontiguousarray(np.stack([np.asarray(b, np.float64).reshape(-1, 6) for _, b in samples]))
         t = Tuning()
         ms = ctypes.c_double()
         self._check(self._L.rigc_autotune_earliest_toi(self._ctx, len(samples), _p(p0, _c_double_p),
                                                        _p(p1, _c_double_p), float(minimum_separation_distance),
-                                                       ctypes.byref(t), ctypes.byref(ms)))
+                                                       mask, ctypes.byref(t), ctypes.byref(ms)))
         return {k: getattr(t, k) for k, _ in Tuning._fields_}, ms.value
 
     def compute_earliest_toi(self, poses_t0, poses_t1, minimum_separation_distance=0.0):

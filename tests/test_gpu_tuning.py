@@ -87,3 +87,24 @@ def test_autotuner_sets_a_grid_tuning_and_keeps_results(ctx):
     assert best["ccd_wide_queries"] in (0, 4096, 65536)
     assert ctx.tuning() == best
     assert [ctx.compute_earliest_toi(p0, p1) for p0, p1 in seq] == want
+
+
+def test_autotuner_leaves_knobs_outside_its_mask(ctx):
+    """knobs restricts the search: the other fields keep the context's
+    setting, and the results stay the same."""
+    sc = cluster(8, n=60)
+    ctx.set_scene(sc)
+    seq = sequence(sc, 3, 310)
+    want = [ctx.compute_earliest_toi(p0, p1) for p0, p1 in seq]
+    base = ctx.tuning()
+    ctx.set_tuning(ccd_spec_levels=4, ccd_window_factor=16.0)
+    try:
+        best, ms = ctx.autotune_earliest_toi(seq, knobs=["ccd_walk_budget", "ccd_wide_queries"])
+        print("autotuned (walk budget, launch width)", best, ms, flush=True)
+        assert best["ccd_spec_levels"] == 4 and best["ccd_window_factor"] == 16.0
+        assert best["ccd_walk_budget"] in (1024, 4096, 16384)
+        assert best["ccd_wide_queries"] in (0, 4096, 65536)
+        assert ctx.tuning() == best
+        assert [ctx.compute_earliest_toi(p0, p1) for p0, p1 in seq] == want
+    finally:
+        ctx.set_tuning(**base)
