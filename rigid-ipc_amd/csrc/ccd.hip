@@ -145,6 +145,14 @@ struct CoopWave {
             Iv gp[COOP_ITEMS][2][2];      // sin(x) / x beyond mb [body][angle]
         } s;
         Iv w[COOP_ITEMS][4][3];   // world vertices, in the order of Query::g (written after R)
+        // after the t part: what each helper found for its speculative node,
+        // at slot = its rank among the idle lanes (owner o reads node k at
+        // o * per + k - 1); info bit 0 zero inclusion, 1 accepted if
+        // reached, 2 outside the face-vertex domain, bits 3-4 split dimension
+        struct {
+            double tlo[64];
+            unsigned int info[64];
+        } n;
     } u;
 };
 
@@ -553,14 +561,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         bool own_eval = false;
         Iv t, u, v;
         // Pops to the next pending second half; done when none is left.
-        int pop_level = -1;  // level of the last pop (-1: none since reset)
         auto pop = [&]() {
             const int L = last_zero_below(s, depth);
             if (L < 0) {
                 done = true;
                 return;
             }
-            pop_level = L;
             const int ct = popcount_range(mt, L + 1, depth), ca = popcount_range(ma, L + 1, depth);
             const int cb = depth - 1 - L - ct - ca;
             idx[0] >>= ct; lev[0] -= ct;
@@ -592,6 +598,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 pop();
             }
         };
+        // An accepted box with t.lower tlo: the query's minimum (and the
+        // scene's running minimum) and this lane's pruning bound.
+        auto accept = [&](double tlo) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(tlo);
+            atomicMin(a.q_best + qi, bits);
+            thr = fmin(thr, tlo);
+            if (a.use_global_best) atomicMin(a.global_best + q.sc, bits);
+        };
         // Consumes the zero-inclusion result z of the current box.
         auto process = [&](bool z) {
             bool leaf = !z;
@@ -601,11 +615,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const bool all_sat = w[0] <= tol[0] && w[1] <= tol[1] && w[2] <= tol[2];
                 const bool all_zero = w[0] <= 1e-10 && w[1] <= 1e-10 && w[2] <= 1e-10;
                 if ((idx[0] > 0 || all_zero) && all_sat) {
-                    const double tlo = dyad(idx[0], lev[0]);
-                    const unsigned long long bits = (unsigned long long)__double_as_longlong(tlo);
-                    atomicMin(a.q_best + qi, bits);
-                    thr = fmin(thr, tlo);
-                    if (a.use_global_best) atomicMin(a.global_best + q.sc, bits);
+                    accept(dyad(idx[0], lev[0]));
                     leaf = true;
                 } else {
                     const int sp = split_dim(w, tol, all_sat);
@@ -670,13 +680,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const int max_levels = min(a.spec_levels, SPEC_LEVELS);
         while (levels < max_levels && (2 << (levels + 1)) - 2 <= g) levels++;
         const int per = levels ? (2 << levels) - 2 : 0;
-        int src = lane, node = 0;
+        int src = lane, node = 0, slot = 0;
         if (per && qi < 0) {
             const int h = __popcll(idle & lt_mask);
             const int o = h / per;
             if (o < n_own) {
                 src = nth_set_bit(owners, o);
                 node = h % per + 1;
+                slot = h;
             }
         }
         const long long s_q = __shfl(qi, src);
@@ -684,6 +695,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         const unsigned long long s_i0 = __shfl(idx[0], src), s_i1 = __shfl(idx[1], src), s_i2 = __shfl(idx[2], src);
         const int s_lev = __shfl(lev[0] | (lev[1] << 8) | (lev[2] << 16), src);
         bool spec = false;
+        unsigned int spec_info = 0;  // the node's record without its zero-inclusion bit
         unsigned long long e_idx0 = idx[0];
         int e_lev0 = lev[0];
         if (node > 0) {
@@ -708,6 +720,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             e_lev0 = bl[0];
             spec = true;
             cache_ok = false;
+            // what process / skip_pruned would decide at this box, for the owner
+            double w[3];
+            box_widths(bl, w);
+            const bool all_sat = w[0] <= tol[0] && w[1] <= tol[1] && w[2] <= tol[2];
+            const bool all_zero = w[0] <= 1e-10 && w[1] <= 1e-10 && w[2] <= 1e-10;
+            const bool acc = (bi[0] > 0 || all_zero) && all_sat;
+            const bool outside = q.kind == 1 && !overlap(u + v, Iv{ 0.0, 1.0 });
+            spec_info = (acc ? 2u : 0u) | (outside ? 4u : 0u) | ((unsigned int)split_dim(w, tol, all_sat) << 3);
         }
 
         const bool evaluating = own_eval || spec;
@@ -733,8 +753,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 ev_n++;
             }
         }
-        if (spec) cache_ok = false;
-        const unsigned long long zmask = __ballot(spec && z);
+        CoopWave& cw = coop[threadIdx.x >> 6];
+        if (spec) {
+            cache_ok = false;
+            cw.u.n.tlo[slot] = t.lo;
+            cw.u.n.info[slot] = spec_info | (z ? 1u : 0u);
+        }
+        wave_lds_sync();
 
         if (own_eval) {
             walk++;
@@ -750,69 +775,89 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 cache_ok = false;
                 continue;
             }
-            // results of this owner's helpers (idle lanes of ranks
-            // [o * per, o * per + per)), bit k = subtree node k
-            unsigned long long known = 0;
-            int k_levels = 0;
-            if (per && ((owners >> lane) & 1ULL)) {
-                const int o = __popcll(owners & lt_mask);
-                unsigned long long hm = idle;
-                int r = 0;
-                while (hm) {
-                    const int l = __ffsll((long long)hm) - 1;
-                    hm &= hm - 1ULL;
-                    if (r >= o * per && r < o * per + per) known |= ((zmask >> l) & 1ULL) << (r - o * per + 1);
-                    r++;
-                }
-                k_levels = levels;
-            }
-            // The current box's heap index kc and depth dc below the subtree
-            // root (the box just evaluated), tracked through descents (child
-            // 2 kc + 1) and pops (the second half at the popped level: the
-            // path's node there, plus one).
-            const int root_d = depth;
-            int kc = 0, dc = 0;
-            auto track = [&](int d_before) {
-                if (pop_level >= 0) {
-                    const int dn = pop_level + 1 - root_d;
-                    if (dn < 1) {
-                        dc = -1;
+            const int k_levels = (per && ((owners >> lane) & 1ULL)) ? levels : 0;
+            if (k_levels == 0) {
+                process(z);
+            } else {
+                // The owner walks the known levels in heap order (root 0,
+                // children of node j at 2j + 1 and 2j + 2) from its helpers'
+                // records, deciding as process / pop / skip_pruned do on the
+                // path masks, and writes the path masks once at the end. The
+                // boxes reached and accepted are those of the box-by-box walk.
+                const int base = __popcll(owners & lt_mask) * per - 1;  // record of node k: base + k
+                const int root_d = depth;
+                double w[3];
+                box_widths(lev, w);
+                const bool all_sat = w[0] <= tol[0] && w[1] <= tol[1] && w[2] <= tol[2];
+                const bool all_zero = w[0] <= 1e-10 && w[1] <= 1e-10 && w[2] <= 1e-10;
+                const int sp0 = split_dim(w, tol, all_sat);
+                bool zc = z, acc = (idx[0] > 0 || all_zero) && all_sat;
+                double tl = dyad(idx[0], lev[0]);
+                int kc = 0, dc = 0;
+                int ex = 0;  // 0: descended below the known levels, 1: left the subtree, 2: too deep
+                for (;;) {
+                    if (zc && !acc) {
+                        if (root_d + dc >= MAX_DEPTH) {
+                            atomicMax(a.q_flag + qi, 2);
+                            done = true;
+                            ex = 2;
+                            break;
+                        }
+                        kc = 2 * kc + 1;
+                        dc++;
+                        if (dc > k_levels) break;
                     } else {
-                        kc = ((kc + 1) >> (dc - dn)) - 1 + 1;
-                        dc = dn;
+                        if (zc) accept(tl);
+                        // next pending second half: up over the levels whose
+                        // second half is taken, then across
+                        const unsigned int m = (unsigned int)kc + 1u;
+                        const int up = __builtin_ctz(~m);
+                        if (up >= dc) {
+                            ex = 1;
+                            break;
+                        }
+                        kc = (int)(((m >> up) | 1u) - 1u);
+                        dc -= up;
                     }
-                } else if (depth == d_before + 1) {
-                    kc = 2 * kc + 1;
-                    dc++;
+                    const unsigned int inf = cw.u.n.info[base + kc];
+                    const double tk = cw.u.n.tlo[base + kc];
+                    if (tk >= thr || (inf & 4u)) {  // pruned or outside the domain: a leaf
+                        zc = false;
+                        continue;
+                    }
+                    walk++;
+                    zc = (inf & 1u) != 0u;
+                    acc = (inf & 2u) != 0u;
+                    tl = tk;
                 }
-            };
-            bool zc = z;
-            for (;;) {
-                int d0 = depth;
-                pop_level = -1;
-                process(zc);
-                track(d0);
-                if (done || !(dc >= 1 && dc <= k_levels)) break;
-                d0 = depth;
-                pop_level = -1;
-                skip_pruned();
-                if (done) break;
-                track(d0);
-                if (!(dc >= 1 && dc <= k_levels)) break;
-                walk++;
-                zc = ((known >> kc) & 1ULL) != 0ULL;
-            }
-            // The walk left the subtree below its last level: every pending
-            // second half inside the subtree whose box is known not to hold
-            // zero is a leaf of the search, so it is marked visited (what
-            // visiting it would conclude).
-            if (!done && k_levels > 0 && dc > k_levels) {
-                for (int L = depth - 1; L >= root_d; L--) {
-                    if (bget(s, L)) continue;
-                    const int dn = L + 1 - root_d;
-                    if (dn > k_levels) continue;
-                    const int ks = ((kc + 1) >> (dc - dn)) - 1 + 1;
-                    if (((known >> ks) & 1ULL) == 0ULL) bset(s, L, true);
+                if (ex == 1) {
+                    pop();
+                } else if (ex == 0) {
+                    // the path from the root to node kc (depth dc): split
+                    // dimensions from the records; a pending second half
+                    // whose box is known to be a leaf (no zero, pruned or
+                    // outside the domain) is marked visited, which is what
+                    // visiting it would conclude
+                    const unsigned int path = (unsigned int)kc + 1u;  // a leading 1, then dc child bits
+                    for (int i = 1; i <= dc; i++) {
+                        const unsigned int above = path >> (dc - i + 1);  // node at depth i - 1, plus 1
+                        const int sp = i == 1 ? sp0 : (int)((cw.u.n.info[base + (int)above - 1] >> 3) & 3u);
+                        const unsigned int here = path >> (dc - i);
+                        const bool second = (here & 1u) != 0u;
+                        bool closed = second;
+                        if (!second && i <= k_levels) {
+                            const int sib = (int)here;  // (here | 1) - 1
+                            const unsigned int si = cw.u.n.info[base + sib];
+                            closed = !(si & 1u) || (si & 4u) || cw.u.n.tlo[base + sib] >= thr;
+                        }
+                        const int P = root_d + i - 1;
+                        bset(s, P, closed);
+                        bset(mt, P, sp == 0);
+                        bset(ma, P, sp == 1);
+                        descend_at(idx, sp, second ? 1ULL : 0ULL);
+                        add_at(lev, sp, 1);
+                    }
+                    depth = root_d + dc;
                 }
             }
         }
