@@ -109,6 +109,22 @@ typedef struct rigs_bodies {
     const double* velocities;
     const double* mass;
     const double* inertia;
+    /* Optional (NULL: none): external force and torque [num_bodies][6],
+     * constant, in world coordinates (RigidBody::force; compute_body_energy,
+     * distance_barrier_rb_problem.cpp:847-944). */
+    const double* force;
+    /* Optional (NULL: none): kinematic bodies (RigidBodyType::KINEMATIC).
+     * kinematic[b] = 1 marks body b (not static in the scene): it has no body
+     * energy, and an augmented Lagrangian (distance_barrier_rb_problem.cpp:
+     * 190-235, 339-435, 946-1109) moves it each step to its next scripted
+     * pose (kinematic_poses[kinematic_pose_start[b] ..], consumed one per
+     * step; degrees of freedom as `poses`) or, without one, by its velocity
+     * times the timestep. Once kinematic_max_time[b] (NULL or < 0: never)
+     * runs out it becomes static (step_kinematic_bodies, :237-251). */
+    const int32_t* kinematic;
+    const double* kinematic_max_time;
+    const int32_t* kinematic_pose_start; /* [num_bodies + 1], NULL: no scripted poses */
+    const double* kinematic_poses;       /* [kinematic_pose_start[num_bodies]][6] */
 } rigs_bodies;
 
 typedef struct rigs_stats {

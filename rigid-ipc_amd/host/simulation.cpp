@@ -15,12 +15,14 @@
 // initial_barrier_stiffness / update_barrier_stiffness (not in the
 // reference tree).
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -83,12 +85,20 @@ rigc_status gb_toi_pl(void* s, const double* p0, const double* p1, double msd, d
 struct BodyState {
     double mass = 0;
     double inertia[3] = { 0, 0, 0 };
-    double J[3] = { 0, 0, 0 };  // compute_J: 1/2 (-I0 + I1 + I2), ...
+    double J[3] = { 0, 0, 0 };      // compute_J: 1/2 (-I0 + I1 + I2), ...
+    double Jsqrt[3] = { 0, 0, 0 };  // compute_Jsqrt: sqrt(max(J, 0))
     bool dynamic = false;
     double pose[6], pose_prev[6], velocity[6];
     double Qdot[9];
-    // inertial targets of the current step: position and rotation matrix
-    double target[3], Pm[9];
+    double force[6] = { 0, 0, 0, 0, 0, 0 };  // external force and torque (world)
+    // inertial targets of the current step: position and rotation matrix,
+    // and h^2 Q_t0^T [torque]_x
+    double target[3], Pm[9], Tau[9];
+    // RigidBodyType::KINEMATIC: scripted poses (front first) and the time
+    // after which the body becomes static
+    bool kinematic = false;
+    double kin_max_time = HUGE_VAL;
+    std::deque<std::array<double, 6>> kin_poses;
 };
 
 
@@ -114,6 +124,15 @@ struct rigs_sim {
     std::vector<BodyState> bodies;
     std::vector<int> dyn_index;        // body -> free block index (-1 fixed)
     int n_free = 0;
+    // augmented Lagrangian of the kinematic bodies (distance_barrier_rb_problem.cpp:
+    // 190-235 init, 339-435 update, 946-1109 potential); per body, only the
+    // kinematic bodies' entries are used
+    int n_kin = 0;
+    std::vector<double> x_pred;           // [nb][6] the step's target poses
+    std::vector<unsigned char> dof_sat;   // [nb][6] is_dof_satisfied: out of the free set
+    double al_kq = 1e3, al_kQ = 1e3;      // linear and angular penalties
+    std::vector<double> al_lam, al_Lam;   // [nb][3] and [nb][9] multipliers
+    std::vector<BodyState> bodies_init;   // kinematic state restored by rigs_set_state
     double bbox_diagonal = 0, average_mass = 0;
     double kappa = 1, max_kappa = 0, prev_min_distance = -1;
     std::vector<double> poses_t0;      // start of the current step
@@ -382,13 +401,27 @@ struct rigs_sim {
     //   1/2 m q.q - m q.(q_t0 + h (qdot_t0 + h g))
     //   + 1/2 tr(Q J Q^T) - tr(Q J (Q_t0 + h Qdot_t0)^T)
     // Inertial targets of the step starting at the bodies' current state.
+    // With an external force f and torque tau (compute_body_energy,
+    // :859-904): the target adds h^2 f / m, and the energy h^2 tr(Q Tau)
+    // with Tau = Q_t0^T [tau]_x.
     void step_targets() {
         const double h = P.timestep;
         for (auto& B : bodies) {
-            for (int k = 0; k < 3; k++) B.target[k] = B.pose[k] + h * (B.velocity[k] + h * P.gravity[k]);
+            for (int k = 0; k < 3; k++) {
+                const double acc = B.mass > 0 ? P.gravity[k] + B.force[k] / B.mass : P.gravity[k];
+                B.target[k] = B.pose[k] + h * (B.velocity[k] + h * acc);
+            }
             double Q0[9];
             rigc::rotation_d(B.pose + 3, Q0);
             for (int e = 0; e < 9; e++) B.Pm[e] = Q0[e] + h * B.Qdot[e];
+            const double* t = B.force + 3;
+            const double K[9] = { 0, -t[2], t[1], t[2], 0, -t[0], -t[1], t[0], 0 };
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double v = 0;
+                    for (int k = 0; k < 3; k++) v += Q0[3 * k + i] * K[3 * k + j];
+                    B.Tau[3 * i + j] = h * h * v;
+                }
         }
     }
 
@@ -417,13 +450,23 @@ struct rigs_sim {
                     tQJP += Q[3 * i + j] * B.J[j] * Pm[3 * i + j];
                 }
             E += 0.5 * tQJQ - tQJP;
+            const bool torque = B.force[3] != 0 || B.force[4] != 0 || B.force[5] != 0;
+            if (torque) {
+                double tQT = 0;  // tr(Q Tau)
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) tQT += Q[3 * i + j] * B.Tau[3 * j + i];
+                E += tQT;
+            }
             if (!g && !H) continue;
             rigc::BodyFrame F;
             rigc::body_frame(q + 3, F);
-            // dE/dQ_ij = J_j (Q_ij - P_ij)
+            // dE/dQ_ij = J_j (Q_ij - P_ij) (+ Tau_ji)
             double G[9];
             for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) G[3 * i + j] = B.J[j] * (Q[3 * i + j] - Pm[3 * i + j]);
+                for (int j = 0; j < 3; j++) {
+                    G[3 * i + j] = B.J[j] * (Q[3 * i + j] - Pm[3 * i + j]);
+                    if (torque) G[3 * i + j] += B.Tau[3 * j + i];
+                }
             if (g) {
                 for (int k = 0; k < 3; k++) (*g)[6 * b + k] = B.mass * q[k] - B.mass * target[k];
                 for (int c = 0; c < 3; c++) {
@@ -449,8 +492,183 @@ struct rigs_sim {
         return E;
     }
 
+    // ------------------------------------------------------------ augmented Lagrangian
+    bool dof_free(int b, int k) const { return dyn_index[b] >= 0 && !dof_sat[6 * b + k]; }
+
+    // init_augmented_lagrangian (:190-235), at the start of a step: kinematic
+    // bodies whose time ran out become static; the targets are the next
+    // scripted pose, else the pose moved by the velocity over a timestep.
+    void init_al() {
+        al_kq = al_kQ = 1e3;
+        std::fill(al_lam.begin(), al_lam.end(), 0.0);
+        std::fill(al_Lam.begin(), al_Lam.end(), 0.0);
+        for (auto& B : bodies)
+            if (B.kinematic && B.kin_max_time < 0) convert_to_static(B);
+        const double h = P.timestep;
+        for (int b = 0; b < nb; b++) {
+            const BodyState& B = bodies[b];
+            for (int k = 0; k < 6; k++)
+                x_pred[6 * b + k] = B.kin_poses.empty() ? B.pose[k] + h * B.velocity[k] : B.kin_poses.front()[k];
+            // is_dof_satisfied: the static bodies (free blocks of bodies that
+            // became static stay, with every dof out of the free set)
+            const bool fixed = !B.dynamic && !B.kinematic;
+            for (int k = 0; k < 6; k++) dof_sat[6 * b + k] = fixed ? 1 : 0;
+        }
+    }
+    // RigidBody::convert_to_static: fixed, at rest
+    static void convert_to_static(BodyState& B) {
+        B.kinematic = false;
+        B.dynamic = false;
+        for (int k = 0; k < 6; k++) B.velocity[k] = 0, B.force[k] = 0;
+        for (int e = 0; e < 9; e++) B.Qdot[e] = 0;
+    }
+    // step_kinematic_bodies (:237-251), after a step
+    void step_kinematic() {
+        for (auto& B : bodies) {
+            if (!B.kinematic) continue;
+            if (B.kin_max_time < 0) {
+                convert_to_static(B);
+            } else {
+                B.kin_max_time -= P.timestep;
+                if (!B.kin_poses.empty()) B.kin_poses.pop_front();
+            }
+        }
+    }
+    // compute_{linear,angular}_augment_lagrangian_progress (:253-337)
+    void al_progress(const double* x, double& eta_q, double& eta_Q) const {
+        double aq = 0, bq = 0, aQ = 0, bQ = 0;
+        for (int b = 0; b < nb; b++) {
+            if (!bodies[b].kinematic) continue;
+            const double* xp = &x_pred[6 * b];
+            for (int k = 0; k < 3; k++) {
+                aq += (xp[k] - x[6 * b + k]) * (xp[k] - x[6 * b + k]);
+                bq += (xp[k] - poses_t0[6 * b + k]) * (xp[k] - poses_t0[6 * b + k]);
+            }
+            double Qp[9], Q[9], Q0[9];
+            rigc::rotation_d(xp + 3, Qp);
+            rigc::rotation_d(x + 6 * b + 3, Q);
+            rigc::rotation_d(&poses_t0[6 * b + 3], Q0);
+            for (int e = 0; e < 9; e++) {
+                aQ += (Q[e] - Qp[e]) * (Q[e] - Qp[e]);
+                bQ += (Q0[e] - Qp[e]) * (Q0[e] - Qp[e]);
+            }
+        }
+        eta_q = (aq == 0 && bq == 0) ? 1.0 : 1 - std::sqrt(aq / (bq == 0 ? 1 : bq));
+        eta_Q = (aQ == 0 && bQ == 0) ? 1.0 : 1 - std::sqrt(aQ / (bQ == 0 ? 1 : bQ));
+    }
+    // are_equality_constraints_satisfied (:425-433)
+    bool al_satisfied(const double* x) const {
+        if (!n_kin) return true;
+        double eq, eQ;
+        al_progress(x, eq, eQ);
+        return eq >= 0.999 && eQ >= 0.999;
+    }
+    // update_augmented_lagrangian (:339-423)
+    void al_update(const double* x) {
+        double eq, eQ;
+        al_progress(x, eq, eQ);
+        if (eq >= 0.999) {
+            for (int b = 0; b < nb; b++)
+                if (bodies[b].kinematic)
+                    for (int k = 0; k < 3; k++) dof_sat[6 * b + k] = 1;
+        } else if (eq < 0.99 && al_kq < 1e8) {
+            al_kq *= 2;
+        } else {
+            for (int b = 0; b < nb; b++) {
+                if (!bodies[b].kinematic) continue;
+                const double sm = std::sqrt(bodies[b].mass);
+                for (int k = 0; k < 3; k++) al_lam[3 * b + k] -= al_kq * sm * (x[6 * b + k] - x_pred[6 * b + k]);
+            }
+        }
+        if (eQ >= 0.999) {
+            for (int b = 0; b < nb; b++)
+                if (bodies[b].kinematic)
+                    for (int k = 3; k < 6; k++) dof_sat[6 * b + k] = 1;
+        } else if (eQ < 0.99 && al_kQ < 1e8) {
+            al_kQ *= 2;
+        } else {
+            for (int b = 0; b < nb; b++) {
+                const BodyState& B = bodies[b];
+                if (!B.kinematic) continue;
+                double Qp[9], Q[9];
+                rigc::rotation_d(&x_pred[6 * b + 3], Qp);
+                rigc::rotation_d(x + 6 * b + 3, Q);
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++)
+                        al_Lam[9 * b + 3 * i + j] -= al_kQ * (Q[3 * i + j] - Qp[3 * i + j]) * B.Jsqrt[j];
+            }
+        }
+        if (verbose)
+            std::fprintf(stderr, "rigs: augmented Lagrangian kappa_q=%g kappa_Q=%g eta_q=%g eta_Q=%g\n", al_kq, al_kQ,
+                         eq, eQ);
+    }
+    // compute_augmented_lagrangian (:946-1109): for each kinematic body
+    //   kq / 2 m |q - q_pred|^2 - sqrt(m) lam . (q - q_pred)
+    //   + kQ / 2 tr((Q - Q_pred) J (Q - Q_pred)^T) - tr(Lam^T (Q - Q_pred) Jsqrt)
+    // zero when every kinematic dof is satisfied. Gradient and Hessian are
+    // added into g and H (which hold other terms already).
+    double al_term(const double* x, std::vector<double>* g, BlockHess* H) const {
+        bool all_sat = true;
+        for (int b = 0; b < nb && all_sat; b++)
+            if (bodies[b].kinematic)
+                for (int k = 0; k < 6; k++) all_sat = all_sat && dof_sat[6 * b + k];
+        if (all_sat) return 0.0;
+        double A = 0;
+        for (int b = 0; b < nb; b++) {
+            const BodyState& B = bodies[b];
+            if (!B.kinematic) continue;
+            const double* q = x + 6 * b;
+            const double* qp = &x_pred[6 * b];
+            const double m = B.mass, sm = std::sqrt(m);
+            double dd = 0, ld = 0;
+            for (int k = 0; k < 3; k++) {
+                dd += (q[k] - qp[k]) * (q[k] - qp[k]);
+                ld += al_lam[3 * b + k] * (q[k] - qp[k]);
+            }
+            A += al_kq / 2 * m * dd - sm * ld;
+            if (g)
+                for (int k = 0; k < 3; k++) (*g)[6 * b + k] += al_kq * m * (q[k] - qp[k]) - sm * al_lam[3 * b + k];
+            if (H)
+                for (int k = 0; k < 3; k++) H->diag[b][7 * k] += al_kq * m;
+            double Q[9], Qp[9];
+            rigc::rotation_d(q + 3, Q);
+            rigc::rotation_d(qp + 3, Qp);
+            double tDJD = 0, tLDJ = 0;
+            double G[9];  // d/dQ_ij = kQ J_j D_ij - Lam_ij Jsqrt_j
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    const double D = Q[3 * i + j] - Qp[3 * i + j], L = al_Lam[9 * b + 3 * i + j];
+                    tDJD += D * B.J[j] * D;
+                    tLDJ += L * D * B.Jsqrt[j];
+                    G[3 * i + j] = al_kQ * B.J[j] * D - L * B.Jsqrt[j];
+                }
+            A += al_kQ / 2 * tDJD - tLDJ;
+            if (!g && !H) continue;
+            rigc::BodyFrame F;
+            rigc::body_frame(q + 3, F);
+            if (g)
+                for (int c = 0; c < 3; c++) {
+                    double sum = 0;
+                    for (int e = 0; e < 9; e++) sum += F.dR[c][e] * G[e];
+                    (*g)[6 * b + 3 + c] += sum;
+                }
+            if (H)
+                for (int c = 0; c < 3; c++)
+                    for (int d = 0; d < 3; d++) {
+                        const double* d2 = F.d2R[rigc::sym_index(c, d)];
+                        double sum = 0;
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++)
+                                sum += d2[3 * i + j] * G[3 * i + j] +
+                                       al_kQ * F.dR[c][3 * i + j] * B.J[j] * F.dR[d][3 * i + j];
+                        H->diag[b][6 * (3 + c) + 3 + d] += sum;
+                    }
+        }
+        return A;
+    }
+
     // ------------------------------------------------------------ objective
-    // f = E / m + kappa / m B + D / m (compute_objective :633-703)
+    // f = E / m + AL / m + kappa / m B + D / m (compute_objective :633-703)
     // The body energy at a line search's first trial point, with gradient
     // and Hessian: computed on the helper thread (when `overlap`) while the
     // device terms are evaluated, and taken by the next Newton iteration
@@ -548,6 +766,8 @@ struct rigs_sim {
         }
         if (s != RIGC_OK) return s;
         const double t1 = now_ms();
+        // the augmented Lagrangian (kinematic bodies) joins the body energy
+        if (n_kin) E += al_term(x, want_grad ? &g : nullptr, want_hess ? &H : nullptr);
         const double im = 1.0 / average_mass, km = kappa / average_mass;
         f = E * im + km * B + D * im;
         last_terms[0] = E * im;
@@ -588,27 +808,51 @@ struct rigs_sim {
     }
 
     // H_free d = -g_free (compute_direction with SimplicialLDLT)
+    // A dof out of the free set (is_dof_satisfied: a kinematic body's dofs
+    // once its augmented Lagrangian has converged) keeps its row and column
+    // out of the system: both are zeroed and its diagonal is 1 with a zero
+    // right-hand side, so its direction is 0 and the other rows are those
+    // of the reduced system.
     bool solve_direction(const std::vector<double>& g, const BlockHess& H, double coeff, std::vector<double>& dir) {
         std::vector<Block> diag(n_free);
         std::vector<std::pair<int, int>> pairs;
         std::vector<Block> off;
+        bool masked = false;
         for (int b = 0; b < nb; b++)
             if (dyn_index[b] >= 0) {
-                diag[dyn_index[b]] = H.diag[b];
-                for (int k = 0; k < 6; k++) diag[dyn_index[b]][7 * k] += coeff;
+                Block& D = diag[dyn_index[b]];
+                D = H.diag[b];
+                for (int k = 0; k < 6; k++) D[7 * k] += coeff;
+                for (int k = 0; k < 6; k++)
+                    if (dof_sat[6 * b + k]) {
+                        masked = true;
+                        for (int e = 0; e < 6; e++) D[6 * k + e] = D[6 * e + k] = 0.0;
+                        D[7 * k] = 1.0;
+                    }
             }
         for (const auto& kv : H.off) {
             const int i = dyn_index[kv.first.first], j = dyn_index[kv.first.second];
             if (i < 0 || j < 0) continue;
             pairs.push_back({ i, j });
             off.push_back(kv.second);
+            if (masked) {
+                Block& O = off.back();
+                for (int k = 0; k < 6; k++) {
+                    if (dof_sat[6 * kv.first.first + k])
+                        for (int e = 0; e < 6; e++) O[6 * k + e] = 0.0;
+                    if (dof_sat[6 * kv.first.second + k])
+                        for (int e = 0; e < 6; e++) O[6 * e + k] = 0.0;
+                }
+            }
         }
         std::vector<double> rhs(6 * (size_t)n_free), d(6 * (size_t)n_free);
         for (int b = 0; b < nb; b++)
             if (dyn_index[b] >= 0)
-                for (int k = 0; k < 6; k++) rhs[6 * dyn_index[b] + k] = -g[6 * b + k];
+                for (int k = 0; k < 6; k++) rhs[6 * dyn_index[b] + k] = dof_sat[6 * b + k] ? 0.0 : -g[6 * b + k];
         dir.assign(6 * nb, 0.0);
         rigs::BlockLDLT ldlt;
+        std::vector<Block> diag_copy;
+        if (masked) diag_copy = diag;
         bool ok = ldlt.factorize(n_free, std::move(diag), pairs, off);
         if (ok) {
             ldlt.solve(rhs.data(), d.data());
@@ -621,7 +865,8 @@ struct rigs_sim {
                 if (i < 0) continue;
                 for (int r = 0; r < 6; r++)
                     for (int c = 0; c < 6; c++) {
-                        const double hv = r == c ? H.diag[b][6 * r + c] + coeff : H.diag[b][6 * r + c];
+                        const double hv = masked ? diag_copy[i][6 * r + c]
+                                                 : (r == c ? H.diag[b][6 * r + c] + coeff : H.diag[b][6 * r + c]);
                         Hd[6 * i + r] += hv * d[6 * i + c];
                     }
             }
@@ -642,8 +887,8 @@ struct rigs_sim {
                     for (int k = 0; k < 6; k++) dir[6 * b + k] = d[6 * dyn_index[b] + k];
         } else {
             for (int b = 0; b < nb; b++)
-                if (dyn_index[b] >= 0)
-                    for (int k = 0; k < 6; k++) dir[6 * b + k] = -g[6 * b + k];
+                for (int k = 0; k < 6; k++)
+                    if (dof_free(b, k)) dir[6 * b + k] = -g[6 * b + k];
         }
         return ok;
     }
@@ -652,7 +897,8 @@ struct rigs_sim {
         double s = 0;
         for (int i = 0; i < nb; i++)
             if (dyn_index[i] >= 0)
-                for (int k = 0; k < 6; k++) s += a[6 * i + k] * b[6 * i + k];
+                for (int k = 0; k < 6; k++)
+                    if (!dof_sat[6 * i + k]) s += a[6 * i + k] * b[6 * i + k];
         return s;
     }
 
@@ -805,7 +1051,10 @@ struct rigs_sim {
                 }
             }
             stats.host_ms += now_ms() - t0;
-            if (it > 0 && converged(x, dir)) {
+            // converged (newton_solver.cpp:95-157): the step criterion and the
+            // equality constraints of the kinematic bodies
+            const bool energy_converged = it > 0 && converged(x, dir);
+            if (energy_converged && al_satisfied(x.data())) {
                 success = true;
                 s = polish(x, dir, g, H);
                 if (s != RIGC_OK) return s;
@@ -824,19 +1073,21 @@ struct rigs_sim {
             if (!found) {
                 gdir.assign(6 * nb, 0.0);
                 for (int b = 0; b < nb; b++)
-                    if (dyn_index[b] >= 0)
-                        for (int k = 0; k < 6; k++) gdir[6 * b + k] = -g[6 * b + k];
+                    for (int k = 0; k < 6; k++)
+                        if (dof_free(b, k)) gdir[6 * b + k] = -g[6 * b + k];
                 dir = gdir;
                 step = 1;
                 s = line_search(x, dir, fx, g, step, found);
                 if (s != RIGC_OK) return s;
                 if (!found) {
-                    if (it == 0 && converged(x, dir)) success = true;
+                    if (it == 0 && converged(x, dir) && al_satisfied(x.data())) success = true;
                     break;
                 }
             }
             for (int i = 0; i < 6 * nb; i++) x[i] += step * dir[i];
             stats.newton_iterations++;
+            // NewtonSolver::post_step_update (:606-613)
+            if (energy_converged && !al_satisfied(x.data())) al_update(x.data());
             s = post_step_update(x);
             if (s != RIGC_OK) return s;
         }
@@ -852,7 +1103,7 @@ struct rigs_sim {
         std::memcpy(B.pose_prev, B.pose, sizeof(B.pose));
         std::memcpy(B.velocity, velocity, sizeof(B.velocity));
         qdot_from_omega(B.pose + 3, B.velocity + 3, B.Qdot);
-        if (!B.dynamic) {
+        if (!B.dynamic && !B.kinematic) {
             for (int k = 0; k < 6; k++) B.velocity[k] = 0;
             for (int e = 0; e < 9; e++) B.Qdot[e] = 0;
         }
@@ -927,6 +1178,7 @@ struct rigs_sim {
                 std::memcpy(&cp.Qdot[9 * b], bodies[b].Qdot, sizeof(double) * 9);
             }
             cp.kappa_friction = kappa;  // update_friction below uses it
+            cp.differentiable = n_kin == 0;  // the adjoint has no augmented Lagrangian
             tape.push_back(std::move(cp));
         }
         step_targets();
@@ -940,6 +1192,7 @@ struct rigs_sim {
             s = update_friction(poses_t0.data());
             if (s != RIGC_OK) return s;
         }
+        init_al();
         // solve_constraints with lagged friction
         std::vector<double> x(poses_t0);
         const double eps_d = 1e-2 * bbox_diagonal;
@@ -1006,6 +1259,7 @@ struct rigs_sim {
             B.velocity[5] = 0.5 * (W[3] - W[1]);
         }
         if (recording) tape_end = x;
+        step_kinematic();
         if (P.do_intersection_check) {
             int64_t n = -1;
             s = intersections(x.data(), n);
@@ -1039,7 +1293,8 @@ rigc_status rigs_sim::rollout_backward(const double* dLdx, const double* dLdv, d
     for (size_t n = 0; n < tape.size(); n++)
         if (!tape[n].differentiable)
             return fail(RIGC_ERR_UNRESOLVED, "rollout_backward: step " + std::to_string(n) +
-                                                 " did not converge or took several lagged friction sets");
+                                                 " did not converge, took several lagged friction sets or "
+                                                 "moved kinematic bodies");
     const bool fric = P.coefficient_friction > 0;
     if (fric && !be.friction_adjoint)
         return fail(RIGC_ERR_INVALID_ARGUMENT, "rollout_backward: the backend has no friction adjoint");
@@ -1376,21 +1631,53 @@ rigc_status rigs_create(rigs_sim** out, const rigs_backend* backend, int32_t dev
     sim->dyn_index.assign(nb, -1);
     double mass_sum = 0;
     int n_free_dof = 0;
+    const int32_t* ks = bodies->kinematic_pose_start;
     for (int b = 0; b < nb; b++) {
         BodyState& B = sim->bodies[b];
-        B.dynamic = !scene->is_static[b];
+        const bool kin = bodies->kinematic && bodies->kinematic[b] != 0;
+        if (kin && scene->is_static[b]) {
+            rigs_destroy(sim);
+            return RIGC_ERR_INVALID_ARGUMENT;  // a kinematic body is not static in the scene
+        }
+        B.kinematic = kin;
+        B.dynamic = !scene->is_static[b] && !kin;
         B.mass = bodies->mass[b];
         for (int k = 0; k < 3; k++) B.inertia[k] = bodies->inertia[3 * b + k];
         B.J[0] = 0.5 * (-B.inertia[0] + B.inertia[1] + B.inertia[2]);
         B.J[1] = 0.5 * (B.inertia[0] - B.inertia[1] + B.inertia[2]);
         B.J[2] = 0.5 * (B.inertia[0] + B.inertia[1] - B.inertia[2]);
+        for (int k = 0; k < 3; k++) B.Jsqrt[k] = std::sqrt(std::max(B.J[k], 0.0));
+        if (bodies->force && !scene->is_static[b])
+            for (int k = 0; k < 6; k++) B.force[k] = bodies->force[6 * b + k];
+        if (kin) {
+            const double mt = bodies->kinematic_max_time ? bodies->kinematic_max_time[b] : -1.0;
+            B.kin_max_time = mt < 0 ? HUGE_VAL : mt;
+            if (ks && bodies->kinematic_poses) {
+                if (ks[b] < 0 || ks[b + 1] < ks[b]) {
+                    rigs_destroy(sim);
+                    return RIGC_ERR_INVALID_ARGUMENT;
+                }
+                for (int i = ks[b]; i < ks[b + 1]; i++) {
+                    std::array<double, 6> pz;
+                    for (int k = 0; k < 6; k++) pz[k] = bodies->kinematic_poses[6 * (size_t)i + k];
+                    B.kin_poses.push_back(pz);
+                }
+            }
+            sim->n_kin++;
+        }
         sim->set_body_state(b, bodies->poses + 6 * b, bodies->velocities + 6 * b);
-        if (B.dynamic) {
+        if (!scene->is_static[b]) {
+            // the free set: dynamic and kinematic bodies (is_rb_dof_fixed)
             sim->dyn_index[b] = sim->n_free++;
             mass_sum += 3 * B.mass + B.inertia[0] + B.inertia[1] + B.inertia[2];
             n_free_dof += 6;
         }
     }
+    sim->bodies_init = sim->bodies;
+    sim->x_pred.assign(6 * (size_t)nb, 0.0);
+    sim->dof_sat.assign(6 * (size_t)nb, 0);
+    sim->al_lam.assign(3 * (size_t)nb, 0.0);
+    sim->al_Lam.assign(9 * (size_t)nb, 0.0);
     sim->average_mass = n_free_dof ? mass_sum / n_free_dof : 1.0;
     sim->init_bbox();
     std::vector<double> x0(sim->poses_t0);
@@ -1430,7 +1717,17 @@ rigc_status rigs_step(rigs_sim* sim, int32_t num_steps) {
 
 rigc_status rigs_set_state(rigs_sim* sim, const double* poses, const double* velocities) {
     if (!sim || !poses || !velocities) return RIGC_ERR_INVALID_ARGUMENT;
-    for (int b = 0; b < sim->nb; b++) sim->set_body_state(b, poses + 6 * b, velocities + 6 * b);
+    for (int b = 0; b < sim->nb; b++) {
+        // kinematic scripts and external forces as rigs_create left them
+        BodyState& B = sim->bodies[b];
+        const BodyState& I = sim->bodies_init[b];
+        B.dynamic = I.dynamic;
+        B.kinematic = I.kinematic;
+        B.kin_max_time = I.kin_max_time;
+        B.kin_poses = I.kin_poses;
+        std::memcpy(B.force, I.force, sizeof(B.force));
+        sim->set_body_state(b, poses + 6 * b, velocities + 6 * b);
+    }
     sim->init_bbox();
     sim->reset_solver_state();
     sim->tape.clear();

@@ -5,9 +5,11 @@ sim.json written by SimState::save_simulation (src/SimState.cpp:420-457),
 so that the reference's fixtures run through this build's stepper and the
 reference's tools read its results.
 
-Supported: 3D scenes of dynamic and static bodies (a body whose six dofs are
-all fixed is static, as in RigidBody's constructor,
-src/physics/rigid_body.cpp:78-83), meshes from OBJ files or inline arrays,
+Supported: 3D scenes of dynamic, static and kinematic bodies (a body whose
+six dofs are all fixed is static, as in RigidBody's constructor,
+src/physics/rigid_body.cpp:78-83; kinematic bodies with their scripted
+poses and kinematic_max_time), constant external forces and torques,
+meshes from OBJ files or inline arrays,
 scale / dimensions, xyz Euler rotations and angular velocities in degrees,
 densities and group ids (with the reference's id assignment), and the
 solver settings the stepper uses. The CCD runs with the scene's
@@ -17,8 +19,7 @@ broad phase of the CCD is the scene's detection_method, "bvh" (the
 reference's default, src/SimState.cpp:178) or "hash_grid"; the constraint
 set comes out the same from either. The coefficient of restitution is read and, as in the
 reference's distance-barrier problem, not used. Anything else (2D,
-kinematic bodies, partially fixed dofs, forces and torques, split
-components) raises NotImplementedError.
+partially fixed dofs, split components) raises NotImplementedError.
 """
 import copy
 import json
@@ -196,10 +197,12 @@ def load(scene, meshes_dir=None, base_dir=None):
             raise NotImplementedError("2D bodies")
         if b["split_components"]:
             raise NotImplementedError("split_components")
-        if b["kinematic_poses"] or b["type"] == "kinematic":
-            raise NotImplementedError("kinematic bodies")
-        if any(float(x) != 0.0 for x in list(b["force"]) + list(b["torque"])):
-            raise NotImplementedError("external forces and torques")
+        kinematic = b["type"] == "kinematic"
+        # scripted poses: degrees of freedom as given (position, rotation
+        # vector), read_rb_scene.cpp:233-246
+        kposes = [list(np.array(kp.get("position", [0, 0, 0]), float)[:3])
+                  + list(np.array(kp.get("rotation", [0, 0, 0]), float)[:3]) for kp in b["kinematic_poses"]]
+        force = list(np.array(b["force"], float)[:3]) + list(np.array(b["torque"], float)[:3])
         fixed = b["is_dof_fixed"]
         fixed = [bool(fixed)] * 6 if isinstance(fixed, bool) else [bool(x) for x in fixed[:6]]
         static = b["type"] == "static" or all(fixed)
@@ -218,8 +221,11 @@ def load(scene, meshes_dir=None, base_dir=None):
         w = np.array(b["angular_velocity"], float)[:3] * math.pi / 180.0
         v = np.array(b["linear_velocity"], float)[:3]
         p = np.array(b["position"], float)[:3]
-        bodies.append(S.Body(V, F.astype(np.int64), np.eye(3), p, v, w, density=float(b["density"]), static=static))
-        types.append("static" if static else "dynamic")
+        bodies.append(S.Body(V, F.astype(np.int64), np.eye(3), p, v, w, density=float(b["density"]), static=static,
+                             kinematic=kinematic and not static,
+                             kinematic_max_time=float(b["kinematic_max_time"]), kinematic_poses=kposes,
+                             force=None if static or not any(x != 0.0 for x in force) else force))
+        types.append("static" if static else ("kinematic" if kinematic else "dynamic"))
         groups.append(int(b["group_id"]))
     for body, g in zip(bodies, assign_group_ids(types, groups)):
         body.group = g

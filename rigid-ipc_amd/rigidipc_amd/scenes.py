@@ -117,6 +117,40 @@ def mass_properties(V, F):
     return vol, com, inertia
 
 
+def open_mass_properties(V, F):
+    """Mass, centre of mass and inertia tensor (unit density) of an open
+    triangle mesh with fewer than 4 faces, as src/physics/mass.cpp:11-45
+    gives them: lumped vertex masses from the mixed Voronoi areas (Meyer et
+    al.; igl::massmatrix VORONOI: an obtuse corner takes half its triangle's
+    area, the other two a quarter each), then point masses about the centre."""
+    V = np.asarray(V, np.float64)
+    m = np.zeros(len(V))
+    for f in np.asarray(F):
+        P = V[f]
+        area = 0.5 * np.linalg.norm(np.cross(P[1] - P[0], P[2] - P[0]))
+        ang = []
+        for c in range(3):
+            u, w = P[(c + 1) % 3] - P[c], P[(c + 2) % 3] - P[c]
+            ang.append(np.arctan2(np.linalg.norm(np.cross(u, w)), np.dot(u, w)))
+        if max(ang) > np.pi / 2:
+            ob = int(np.argmax(ang))
+            for c in range(3):
+                m[f[c]] += area / 2 if c == ob else area / 4
+        else:
+            for c in range(3):
+                j, k = (c + 1) % 3, (c + 2) % 3
+                ejc = np.dot(P[j] - P[c], P[j] - P[c])
+                ekc = np.dot(P[k] - P[c], P[k] - P[c])
+                m[f[c]] += (ejc / np.tan(ang[k]) + ekc / np.tan(ang[j])) / 8
+    mass = m.sum()
+    com = (m[:, None] * V).sum(0) / mass if mass > 0 else np.zeros(3)
+    r = V - com
+    inertia = np.zeros((3, 3))
+    for mi, v in zip(m, r):
+        inertia += mi * (np.dot(v, v) * np.eye(3) - np.outer(v, v))
+    return mass, com, inertia
+
+
 def rotvec_to_matrix(r):
     th = np.linalg.norm(r)
     if th == 0:
@@ -150,14 +184,24 @@ def random_rotation(rng):
 
 # ------------------------------------------------------------------- scene
 class Body:
-    def __init__(self, V, F, R, p, v, w, density=1000.0, static=False, group=None):
+    def __init__(self, V, F, R, p, v, w, density=1000.0, static=False, group=None, kinematic=False,
+                 kinematic_max_time=-1.0, kinematic_poses=(), force=None):
         """V, F: mesh in its own frame; world pose = rotation R, translation p;
-        v, w: world linear / angular velocity (rad/s)."""
+        v, w: world linear / angular velocity (rad/s). A kinematic body
+        follows kinematic_poses (degrees of freedom as `pose`, one per step)
+        or its velocity, until kinematic_max_time (< 0: never); force: its
+        external force and torque (world), or None."""
         self.static = static
         self.group = group
+        self.kinematic = bool(kinematic) and not static
+        self.kinematic_max_time = float(kinematic_max_time)
+        self.kinematic_poses = [np.asarray(x, np.float64).reshape(6) for x in kinematic_poses]
+        self.force = None if force is None or static else np.asarray(force, np.float64).reshape(6)
         E = edges_of(F)
         if not static:
-            vol, com, I = _cached("mass", (np.asarray(V, np.float64), np.asarray(F)), mass_properties)
+            props = mass_properties if len(F) >= 4 else open_mass_properties
+            vol, com, I = _cached("mass" if len(F) >= 4 else "open_mass", (np.asarray(V, np.float64), np.asarray(F)),
+                                  props)
             V = V - com
             p = p + R @ com
             evals, evecs = np.linalg.eigh(I)
@@ -213,8 +257,27 @@ def assemble(bodies):
         is_static=np.array([b.static for b in bodies], np.uint8),
         poses=np.array([b.pose for b in bodies]), velocities=np.array([b.velocity for b in bodies]),
         mass=np.array([b.mass for b in bodies]), inertia=np.array([b.inertia for b in bodies]),
-        bodies=bodies,
+        bodies=bodies, **_kinematic_arrays(bodies),
     )
+
+
+def _kinematic_arrays(bodies):
+    """External forces and kinematic scripts of bodies, as the stepper takes
+    them (none when no body has them)."""
+    out = {}
+    if any(getattr(b, "force", None) is not None for b in bodies):
+        out["force"] = np.array([b.force if getattr(b, "force", None) is not None else np.zeros(6)
+                                 for b in bodies])
+    if any(getattr(b, "kinematic", False) for b in bodies):
+        start, poses = [0], []
+        for b in bodies:
+            poses.extend(b.kinematic_poses if b.kinematic else [])
+            start.append(len(poses))
+        out["kinematic"] = np.array([int(b.kinematic) for b in bodies], np.int32)
+        out["kinematic_max_time"] = np.array([b.kinematic_max_time for b in bodies])
+        out["kinematic_pose_start"] = np.array(start, np.int32)
+        out["kinematic_poses"] = np.array(poses).reshape(-1, 6)
+    return out
 
 
 def _ground():

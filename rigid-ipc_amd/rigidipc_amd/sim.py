@@ -41,7 +41,28 @@ class Params(ctypes.Structure):
 
 class Bodies(ctypes.Structure):
     _fields_ = [("num_bodies", ctypes.c_int32), ("poses", _c_double_p), ("velocities", _c_double_p),
-                ("mass", _c_double_p), ("inertia", _c_double_p)]
+                ("mass", _c_double_p), ("inertia", _c_double_p), ("force", _c_double_p),
+                ("kinematic", _c_int32_p), ("kinematic_max_time", _c_double_p),
+                ("kinematic_pose_start", _c_int32_p), ("kinematic_poses", _c_double_p)]
+
+
+def _body_extras(scene, keep):
+    """Optional rigs_bodies arrays of a scene (external forces, kinematic
+    bodies): keeps them alive in `keep` and returns the Bodies field values."""
+    out = {}
+    if getattr(scene, "force", None) is not None:
+        keep["force"] = np.ascontiguousarray(scene.force, np.float64).reshape(-1, 6)
+        out["force"] = _p(keep["force"], _c_double_p)
+    if getattr(scene, "kinematic", None) is not None:
+        keep["kinematic"] = np.ascontiguousarray(scene.kinematic, np.int32)
+        keep["kinematic_max_time"] = np.ascontiguousarray(scene.kinematic_max_time, np.float64)
+        keep["kinematic_pose_start"] = np.ascontiguousarray(scene.kinematic_pose_start, np.int32)
+        keep["kinematic_poses"] = np.ascontiguousarray(scene.kinematic_poses, np.float64).reshape(-1, 6)
+        out["kinematic"] = _p(keep["kinematic"], _c_int32_p)
+        out["kinematic_max_time"] = _p(keep["kinematic_max_time"], _c_double_p)
+        out["kinematic_pose_start"] = _p(keep["kinematic_pose_start"], _c_int32_p)
+        out["kinematic_poses"] = _p(keep["kinematic_poses"], _c_double_p)
+    return out
 
 
 class Stats(ctypes.Structure):
@@ -177,6 +198,8 @@ def _scene_inputs(scene):
         setattr(s, name, _p(a, t))
     b = Bodies(s.num_bodies, _p(k["poses"], _c_double_p), _p(k["velocities"], _c_double_p),
                _p(k["mass"], _c_double_p), _p(k["inertia"], _c_double_p))
+    for name, v in _body_extras(scene, k).items():
+        setattr(b, name, v)
     return k, s, b
 
 
@@ -272,6 +295,8 @@ class Simulation:
             setattr(s, name, _p(a, t))
         b = Bodies(s.num_bodies, _p(k["poses"], _c_double_p), _p(k["velocities"], _c_double_p),
                    _p(k["mass"], _c_double_p), _p(k["inertia"], _c_double_p))
+        for name, v in _body_extras(scene, k).items():
+            setattr(b, name, v)
         self.params = params if params is not None else default_params(scene)
         self.num_bodies = s.num_bodies
         self._sim = ctypes.c_void_p()

@@ -18,6 +18,8 @@ SCENES = [
     "fixtures/3D/unit-tests/face-vertex.json",
     "fixtures/3D/friction/sliding.json",
     "fixtures/3D/examples/concave.json",
+    "fixtures/3D/examples/kinematic.json",
+    "fixtures/3D/examples/kinematically-scripted.json",
 ]
 
 os.makedirs(os.path.join(OUT, "meshes"), exist_ok=True)

@@ -96,10 +96,27 @@ def test_rotation_and_angular_velocity_in_degrees():
     np.testing.assert_allclose(sc.poses[0, :3], [1, 2, 3], atol=1e-12)
 
 
+def test_kinematic_bodies_and_forces_are_read():
+    """type "kinematic" with its scripted poses (position, rotation vector as
+    given) and max time, and force / torque (read_rb_scene.cpp:85-104,
+    225-246); the stepper's handling is tested in test_kinematic.py."""
+    Vc, Fc = S.cube(1.0)
+    kin = {"vertices": Vc.tolist(), "faces": Fc.tolist(), "type": "kinematic", "kinematic_max_time": 0.5,
+           "kinematic_poses": [{"position": [0, 1, 0]}, {"position": [0, 2, 0], "rotation": [0.1, 0, 0]}]}
+    dyn = {"vertices": Vc.tolist(), "faces": Fc.tolist(), "position": [3, 0, 0], "force": [0, 1, 0],
+           "torque": [0, 0, 2]}
+    sc, _ = IO.load({"rigid_body_problem": {"rigid_bodies": [kin, dyn]}})
+    assert sc.kinematic.tolist() == [1, 0]
+    assert sc.kinematic_max_time.tolist() == [0.5, -1.0]
+    assert sc.kinematic_pose_start.tolist() == [0, 2, 2]
+    np.testing.assert_array_equal(sc.kinematic_poses, [[0, 1, 0, 0, 0, 0], [0, 2, 0, 0.1, 0, 0]])
+    np.testing.assert_array_equal(sc.force, [[0] * 6, [0, 1, 0, 0, 0, 2]])
+    # kinematic bodies join the static bodies' group (read_rb_scene.cpp:292-336)
+    assert sc.group_ids[0] != sc.group_ids[1]
+
+
 @pytest.mark.parametrize("body,msg", [
-    ({"type": "kinematic"}, "kinematic"),
     ({"is_dof_fixed": [True, False, False, False, False, False]}, "partially fixed"),
-    ({"force": [0, 1, 0]}, "forces"),
     ({"split_components": True}, "split_components"),
 ])
 def test_unsupported_features_raise(body, msg):
