@@ -218,7 +218,10 @@ def load(scene, meshes_dir=None, base_dir=None):
         else:
             scale = np.array(b["scale"], float)[:3]
         V = (V * scale) @ euler_xyz_degrees(b["rotation"]).T
-        w = np.array(b["angular_velocity"], float)[:3] * math.pi / 180.0
+        # angular velocity in the mesh's own frame, as the reference takes it
+        # (rigid_body.cpp:132-140: pose rotation R_i R0, omega = R0^T omega_0),
+        # so its world value is R_i omega_0
+        w = euler_xyz_degrees(b["rotation"]) @ (np.array(b["angular_velocity"], float)[:3] * math.pi / 180.0)
         v = np.array(b["linear_velocity"], float)[:3]
         p = np.array(b["position"], float)[:3]
         bodies.append(S.Body(V, F.astype(np.int64), np.eye(3), p, v, w, density=float(b["density"]), static=static,

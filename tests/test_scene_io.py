@@ -96,6 +96,19 @@ def test_rotation_and_angular_velocity_in_degrees():
     np.testing.assert_allclose(sc.poses[0, :3], [1, 2, 3], atol=1e-12)
 
 
+def test_angular_velocity_is_in_the_mesh_frame():
+    """The reference stores omega = R0^T omega_0 with the pose rotation
+    R_i R0 (rigid_body.cpp:132-140): the JSON angular velocity is in the
+    unrotated mesh's frame, so a body rotated 90 degrees about x spinning
+    about its own z turns about world -y."""
+    Vc, Fc = S.cube(1.0)
+    scene = {"rigid_body_problem": {"rigid_bodies": [
+        {"vertices": Vc.tolist(), "faces": Fc.tolist(), "rotation": [90, 0, 0], "angular_velocity": [0, 0, 180]}]}}
+    sc, _ = IO.load(scene)
+    Rw = S.rotvec_to_matrix(sc.bodies[0].pose[3:])
+    np.testing.assert_allclose(Rw @ sc.velocities[0, 3:], [0, -np.pi, 0], atol=1e-12)
+
+
 def test_kinematic_bodies_and_forces_are_read():
     """type "kinematic" with its scripted poses (position, rotation vector as
     given) and max time, and force / torque (read_rb_scene.cpp:85-104,
