@@ -327,6 +327,20 @@ RIGC_DI int nth_set_bit(unsigned long long m, int i) {
     return pos;
 }
 
+// Lane src's query record and alpha / beta tolerances, in every lane (a
+// receiver of a donated subtree or a helper takes them from the lane that
+// holds the same query instead of loading them again).
+RIGC_DI void shfl_query(const Query& q, const double tol[3], int src, Query& out, double& tol1, double& tol2) {
+    out.kind = __shfl(q.kind, src);
+    out.bA = __shfl(q.bA, src);
+    out.bB = __shfl(q.bB, src);
+    out.sc = __shfl(q.sc, src);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out.g[k] = __shfl(q.g[k], src);
+    tol1 = __shfl(tol[1], src);
+    tol2 = __shfl(tol[2], src);
+}
+
 RIGC_DI void load_query(const CcdArgs& a, long long qi, Query& q, double tol[3]) {
     if (qi < a.n_ee) {
         const int ea = a.ee[2 * qi], eb = a.ee[2 * qi + 1];
@@ -533,10 +547,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const unsigned long long r1 = __shfl(d_idx[1], src);
                 const unsigned long long r2 = __shfl(d_idx[2], src);
                 const int r_lev = __shfl(d_lev, src);
+                // the donor holds the query's record
+                Query r_rec;
+                double r_tol1 = 0, r_tol2 = 0;
+                if (k > 0) shfl_query(q, tol, src, r_rec, r_tol1, r_tol2);
                 if (qi < 0 && irank < k) {
                     qi = r_q;
                     spec_q = -1;
-                    load_query(a, qi, q, tol);
+                    q = r_rec;
+                    tol[1] = r_tol1;
+                    tol[2] = r_tol2;
                     tol[0] = r_tol0;
                     idx[0] = r0; idx[1] = r1; idx[2] = r2;
                     lev[0] = r_lev & 255; lev[1] = (r_lev >> 8) & 255; lev[2] = (r_lev >> 16) & 255;
@@ -698,10 +718,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         unsigned int spec_info = 0;  // the node's record without its zero-inclusion bit
         unsigned long long e_idx0 = idx[0];
         int e_lev0 = lev[0];
+        // a helper that changes query takes the record from its owner
+        const bool reload = node > 0 && s_q != spec_q;
+        Query h_rec;
+        double h_tol1 = 0, h_tol2 = 0;
+        if (__ballot(reload)) shfl_query(q, tol, src, h_rec, h_tol1, h_tol2);
         if (node > 0) {
             // this lane is idle: its query registers are free until it takes work
-            if (s_q != spec_q) {
-                load_query(a, s_q, q, tol);
+            if (reload) {
+                q = h_rec;
+                tol[1] = h_tol1;
+                tol[2] = h_tol2;
                 spec_q = s_q;
             }
             tol[0] = s_tol0;
