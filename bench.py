@@ -57,8 +57,8 @@ FLOPS_PER_BOX = 1015.7
 # WRITE_SIZE runs (profiles/ccd_kernel_traffic_pmc.md): FETCH_SIZE scaled
 # by 2.0 and WRITE_SIZE by 1.0, the factors at which the counters read a
 # 256 MiB device copy as 256 MiB read and 256 MiB written.
-CCD_TRAFFIC = (817.4 * 2.0 + 98.4) * 1024   # ballistic CCD sequence
-SIM_CCD_TRAFFIC = (483.3 * 2.0 + 46.3) * 1024  # time stepping, first 3 steps
+CCD_TRAFFIC = (787.5 * 2.0 + 98.3) * 1024   # ballistic CCD sequence
+SIM_CCD_TRAFFIC = (465.0 * 2.0 + 46.3) * 1024  # time stepping, first 3 steps
 HBM_PEAK_GBS = 8000.0
 # Algorithmic FP64 work of one barrier constraint's Hessian block: the
 # vertex-space value / gradient / Hessian with the rotation Jacobians
