@@ -195,6 +195,11 @@ __global__ __launch_bounds__(BP_THREADS) void domain_kernel(const int* scenes, c
 // within VI_ROOM of full: one piece adds at most one entry per level
 // below it, so the stack never overflows.
 constexpr int VI_STACK = 512, VI_BATCH = 64, VI_ROOM = 2 * VI_BATCH;
+// Pieces of levels 1..VI_TABLE_LEV have their interval poses evaluated
+// together, one per thread, before the splitting starts (a pose is a
+// function of its t-interval only): a round that reaches them reads them
+// instead of waiting for its own evaluation.
+constexpr int VI_TABLE_LEV = 4, VI_TABLE = (2 << VI_TABLE_LEV) - 2;
 
 __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* bodies, const int* body_scene,
                                                                     const double* V, const int* bvs, const int* inc,
@@ -207,6 +212,7 @@ __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* 
     __shared__ unsigned long long c_idx[VI_BATCH];
     __shared__ int c_lev[VI_BATCH], c_force[VI_BATCH], c_bad[VI_BATCH];
     __shared__ Iv Rs[VI_BATCH][3][3], ps[VI_BATCH][3];
+    __shared__ Iv Rt[VI_TABLE][3][3], pt[VI_TABLE][3];  // levels 1..VI_TABLE_LEV, piece (l, i) at 2^l - 2 + i
     __shared__ double red[6][BP_THREADS];
     const int b = bodies[blockIdx.x], tid = threadIdx.x;
     if (!inc[b]) return;
@@ -225,6 +231,17 @@ __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* 
         s_idx[0] = 0; s_lev[0] = 0; s_force[0] = 1;
         sp = 1;
     }
+    if (tid < VI_TABLE) {
+        const int lev = 31 - __clz(tid + 2);
+        const unsigned long long idx = (unsigned long long)(tid + 2 - (1 << lev));
+        const Iv t = Iv{ ldexp((double)idx, -lev), ldexp((double)(idx + 1), -lev) };
+        Iv p[3], R[3][3];
+        pose_iv(q0, q1, t, p, R);
+        for (int i = 0; i < 3; i++) {
+            pt[tid][i] = p[i];
+            for (int j = 0; j < 3; j++) Rt[tid][i][j] = R[i][j];
+        }
+    }
     __syncthreads();
     while (sp > 0) {
         // take the top k pieces
@@ -239,12 +256,20 @@ __global__ __launch_bounds__(BP_THREADS) void vertex_interval_kernel(const int* 
             if (evaluate) {
                 const unsigned long long idx = c_idx[tid];
                 const int lev = c_lev[tid];
-                const Iv t = Iv{ ldexp((double)idx, -lev), ldexp((double)(idx + 1), -lev) };
-                Iv p[3], R[3][3];
-                pose_iv(q0, q1, t, p, R);
-                for (int i = 0; i < 3; i++) {
-                    ps[tid][i] = p[i];
-                    for (int j = 0; j < 3; j++) Rs[tid][i][j] = R[i][j];
+                if (lev >= 1 && lev <= VI_TABLE_LEV) {
+                    const int e = (1 << lev) - 2 + (int)idx;
+                    for (int i = 0; i < 3; i++) {
+                        ps[tid][i] = pt[e][i];
+                        for (int j = 0; j < 3; j++) Rs[tid][i][j] = Rt[e][i][j];
+                    }
+                } else {
+                    const Iv t = Iv{ ldexp((double)idx, -lev), ldexp((double)(idx + 1), -lev) };
+                    Iv p[3], R[3][3];
+                    pose_iv(q0, q1, t, p, R);
+                    for (int i = 0; i < 3; i++) {
+                        ps[tid][i] = p[i];
+                        for (int j = 0; j < 3; j++) Rs[tid][i][j] = R[i][j];
+                    }
                 }
             }
         }
