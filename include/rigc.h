@@ -121,6 +121,25 @@ rigc_status rigc_narrow_phase_noalloc(rigc_ctx* ctx, const double* poses_t0, con
                                       double toi_tolerance, int32_t per_query, double* earliest_toi,
                                       double* toi_out, int8_t* hit_out);
 
+/* rigc_narrow_phase's earliest TOI with device-resident inputs and output,
+ * queued on the caller's stream as kernels only (no host copy, no
+ * synchronisation, no memset), so that a caller can capture it in a HIP
+ * graph and replay it on new contents of the same buffers. No counterpart
+ * in the reference (its CCD runs on the host); for callers that batch
+ * device work in graphs. poses_t0 / poses_t1 [num_bodies][6], ee [n_ee][2],
+ * fv [n_fv][2] and earliest_toi (one double, +inf when nothing collides)
+ * are device pointers (the element ids are not range-checked: they stay
+ * on the device); the search is the one-pass narrow phase of
+ * rigc_narrow_phase with eval_budget 0 (same result). The per-call state
+ * lives in a buffer of the context used by no other call: a call outside
+ * capture grows it (waiting for the stream first), a call under capture
+ * that would need more returns RIGC_ERR_CAPACITY, and a graph captured
+ * before a call that grew it must be captured again. Calls that overlap on
+ * one context share the state and race. */
+rigc_status rigc_narrow_phase_enqueue(rigc_ctx* ctx, void* stream, const double* poses_t0, const double* poses_t1,
+                                      const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                                      double toi_tolerance, double* earliest_toi);
+
 /* Broad phase. Replaces detect_collision_candidates_rigid with
  * DetectionMethod::HASH_GRID, continuous (poses_t1 != NULL;
  * src/ccd/rigid/broad_phase.cpp:127-203, RigidBodyHashGrid::{resize,addBodies}

@@ -48,6 +48,7 @@ struct rigc_ctx {
     unsigned long long pose_tag[3] = { 0, 0, 0 }, tag_counter = 0;
     std::vector<double> h_last[3];
     DevBuf scene_best, scene_skip, scene_par;
+    DevBuf enq_state;  // rigc_narrow_phase_enqueue's per-call state, used by no other call
     DevBuf scene_evals;                       // box evaluations per scene of the last earliest-TOI call (multi-scene contexts)
     std::vector<unsigned long long> h_scene_evals;
     // page-locked staging: poses (uploads), per-scene parameters, the
@@ -383,7 +384,7 @@ rigc_status rigc_destroy(rigc_ctx* ctx) {
     for (DevBuf* b : { &ctx->poses0, &ctx->poses1, &ctx->poses2, &ctx->stage0, &ctx->stage1, &ctx->out_rows, &ctx->ee, &ctx->fv,
                        &ctx->toi, &ctx->hit, &ctx->counters, &ctx->qbest, &ctx->qvisits, &ctx->qflag,
                        &ctx->scene_best, &ctx->scene_skip, &ctx->scene_par, &ctx->scene_evals, &ctx->sel_all.buf, &ctx->sel_req.buf,
-                       &ctx->pl_arena, &ctx->pl_ids })
+                       &ctx->pl_arena, &ctx->pl_ids, &ctx->enq_state })
         b->release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -1407,6 +1408,60 @@ rigc_status rigc_autotune_earliest_toi(rigc_ctx* ctx, int32_t num_samples, const
     restore(cur);
     if (best) *best = cur;
     if (best_ms) *best_ms = cur_ms;
+    return RIGC_OK;
+}
+
+rigc_status rigc_narrow_phase_enqueue(rigc_ctx* ctx, void* stream, const double* poses_t0, const double* poses_t1,
+                                      const int32_t* ee, int64_t n_ee, const int32_t* fv, int64_t n_fv,
+                                      double toi_tolerance, double* earliest_toi) {
+    if (!ctx) return RIGC_ERR_INVALID_ARGUMENT;
+    if (!ctx->has_scene) return ctx->fail(RIGC_ERR_NO_SCENE, "rigc_narrow_phase_enqueue: no scene");
+    if (!poses_t0 || !poses_t1 || !earliest_toi || n_ee < 0 || n_fv < 0 || (n_ee && !ee) || (n_fv && !fv)
+        || !(toi_tolerance > 0))
+        return ctx->fail(RIGC_ERR_INVALID_ARGUMENT, "rigc_narrow_phase_enqueue: bad arguments");
+    const DevScene& d = ctx->scene;
+    const size_t nq = (size_t)(n_ee + n_fv), nq1 = std::max<size_t>(nq, 1);
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    // counters [4], scene minima [n_scenes], per-query best / visits / flags
+    const size_t o_sb = up(4 * sizeof(unsigned long long)), o_qb = o_sb + up(8 * (size_t)d.n_scenes);
+    const size_t o_qv = o_qb + up(8 * nq1), o_qf = o_qv + up(4 * nq1), total = o_qf + up(4 * nq1);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    RIGC_HIP(hipSetDevice(ctx->device));
+    if (ctx->enq_state.bytes < total) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        RIGC_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return ctx->fail(RIGC_ERR_CAPACITY, "rigc_narrow_phase_enqueue: more queries than a call outside "
+                                                "capture has sized the state for");
+        RIGC_HIP(hipStreamSynchronize(st));  // work already queued may still use the old state
+        RIGC_HIP(ctx->enq_state.ensure(total));
+    }
+    unsigned char* base = ctx->enq_state.as<unsigned char>();
+    CcdArgs a = ccd_args(ctx);
+    a.poses_t0 = poses_t0;
+    a.poses_t1 = poses_t1;
+    a.ee = ee;
+    a.fv = fv;
+    a.n_ee = n_ee;
+    a.n_fv = n_fv;
+    a.toi_tol = toi_tolerance;
+    a.eval_budget = 0;
+    a.use_global_best = 1;
+    unsigned long long* cnt = reinterpret_cast<unsigned long long*>(base);
+    a.counter = cnt + 1;
+    a.total_evals = cnt + 2;
+    a.n_tasks_out = cnt + 3;
+    a.global_best = reinterpret_cast<unsigned long long*>(base + o_sb);
+    a.q_best = reinterpret_cast<unsigned long long*>(base + o_qb);
+    a.q_visits = reinterpret_cast<unsigned int*>(base + o_qv);
+    a.q_flag = reinterpret_cast<int*>(base + o_qf);
+    a.toi_out = nullptr;
+    a.hit_out = nullptr;
+    a.scene_skip = nullptr;
+    a.scene_evals = nullptr;
+    a.spec_levels = ctx->spec_levels;
+    const int grid = ccd_grid(ctx->num_cus, (long long)nq, ctx->wide_queries);
+    RIGC_HIP(enqueue_narrow_phase(a, grid, a.global_best, d.n_scenes, earliest_toi, st));
     return RIGC_OK;
 }
 

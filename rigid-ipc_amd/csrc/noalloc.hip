@@ -68,7 +68,29 @@ __global__ __launch_bounds__(256) void noalloc_result_kernel(const double* toi, 
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// The minimum over the scenes' running minima (bit patterns of t >= 0,
+// +inf when none) into out[0].
+__global__ void enqueue_result_kernel(const unsigned long long* sb, int n_scenes, double* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long m = 0x7ff0000000000000ULL;
+    for (int s = 0; s < n_scenes; s++) m = sb[s] < m ? sb[s] : m;
+    out[0] = __longlong_as_double((long long)m);
+}
+
 }  // namespace
+
+// rigc_narrow_phase_enqueue: kernels only (fill, the one-pass narrow phase,
+// the result), so that the sequence can be captured in a graph.
+hipError_t enqueue_narrow_phase(CcdArgs a, int grid, unsigned long long* scene_best, int n_scenes, double* out,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)((n_scenes + 255) / 256)), dim3(256), 0, stream, scene_best,
+                       (long long)n_scenes, 0x7ff0000000000000ULL);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (a.n_ee + a.n_fv > 0 && (e = launch_ccd_single(a, grid, 256, stream)) != hipSuccess) return e;
+    hipLaunchKernelGGL(enqueue_result_kernel, dim3(1), dim3(64), 0, stream, scene_best, n_scenes, out);
+    return hipGetLastError();
+}
 
 hipError_t noalloc_scratch(int device, void** ptr) {
     if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;

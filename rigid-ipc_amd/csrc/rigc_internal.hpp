@@ -288,6 +288,9 @@ hipError_t launch_ccd(CcdArgs a, int grid, int block, hipStream_t stream, CcdWor
 // buffers): a.n_tasks_out must point to one device counter, a.tasks_out is
 // ignored. Same results as launch_ccd.
 hipError_t launch_ccd_single(CcdArgs a, int grid, int block, hipStream_t stream);
+// The narrow phase of rigc_narrow_phase_enqueue on `stream`, kernels only.
+hipError_t enqueue_narrow_phase(CcdArgs a, int grid, unsigned long long* scene_best, int n_scenes, double* out,
+                                hipStream_t stream);
 
 // ---- the narrow phase without allocation (noalloc.hip)
 // Size of the static device array that holds every per-call buffer.

@@ -366,6 +366,18 @@ class Context:
         return self._batch_potential(self._L.rigc_batch_friction_term, ids,
                                      (_p(p, _c_double_p), _p(p0, _c_double_p), _p(eh, _c_double_p)), flags)
 
+    def narrow_phase_enqueue(self, stream, poses_t0, poses_t1, ee, n_ee, fv, n_fv, out, toi_tolerance=1e-4):
+        """rigc_narrow_phase_enqueue on `stream` (a hipStream_t handle, e.g.
+        torch.cuda.Stream().cuda_stream); the other pointer arguments are
+        device addresses (e.g. tensor.data_ptr())."""
+        L = self._L
+        L.rigc_narrow_phase_enqueue.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                                ctypes.c_double, ctypes.c_void_p]
+        L.rigc_narrow_phase_enqueue.restype = ctypes.c_int32
+        self._check(L.rigc_narrow_phase_enqueue(self._ctx, stream, poses_t0, poses_t1, ee, int(n_ee), fv, int(n_fv),
+                                                float(toi_tolerance), out))
+
     def narrow_phase(self, poses_t0, poses_t1, ee, fv, toi_tolerance=1e-4, eval_budget=0,
                      per_query=False, allow_unresolved=False):
         """Rigid CCD over candidates; returns (earliest_toi, toi[n], hit[n])."""
